@@ -1,0 +1,184 @@
+/*
+ * riv_gpu.h -- C ABI of the MI355X (gfx950) backend for the RIV (.riv) video codec.
+ *
+ * The encoder/decoder objects mirror the reference Rust API one-for-one; each
+ * declaration names the reference item it replaces (paths relative to the
+ * reference tree, under reitero_video/). Plain pointers and sizes only; no C++
+ * exceptions cross this boundary; every call returns a riv_status.
+ *
+ * Threading: one handle per thread. Each handle owns one HIP stream on its device
+ * and keeps its reference frame, motion vectors and scratch in that device's HBM.
+ */
+#ifndef RIV_GPU_H
+#define RIV_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. 1..5 map EncodeError (reitero_encode/src/error.rs:4-19),
+ * 11..14 map DecodeError (reitero_decode/src/error.rs:5-20). */
+typedef enum {
+    RIV_OK = 0,
+    RIV_E_IO = 1,                 /* EncodeError::Io / DecodeError::Io */
+    RIV_E_INVALID_CONFIG = 2,     /* EncodeError::InvalidConfig */
+    RIV_E_ENCODING_FAILED = 3,    /* EncodeError::EncodingFailed */
+    RIV_E_UNSUPPORTED_CODEC = 4,  /* EncodeError::UnsupportedCodec */
+    RIV_E_FRAME = 5,              /* EncodeError::FrameError */
+    RIV_E_INVALID_HEADER = 11,    /* DecodeError::InvalidHeader */
+    RIV_E_DECODING_FAILED = 12,   /* DecodeError::DecodingFailed (also corrupt entropy data, where the reference panics) */
+    RIV_E_INVALID_FRAME = 13,     /* DecodeError::InvalidFrame */
+    RIV_E_END_OF_STREAM = 14,     /* DecodeError::EndOfStream */
+    RIV_E_GPU = 20,               /* HIP runtime / kernel failure (no reference counterpart) */
+    RIV_E_UNSUPPORTED = 21,       /* valid reference config outside this backend's limits (e.g. search_range > 64) */
+    RIV_E_ARG = 22                /* NULL handle / bad argument */
+} riv_status;
+
+/* KeyframeInterval (reitero_encode/src/config.rs:5-19) */
+typedef enum { RIV_KEYFRAME_FIXED = 0, RIV_KEYFRAME_ALL_INTRA = 1, RIV_KEYFRAME_AUTOMATIC = 2 } riv_keyframe_mode;
+
+/* EncoderConfig (reitero_encode/src/config.rs:24-53) + device selection */
+typedef struct {
+    uint32_t display_width, display_height;
+    uint32_t storage_width, storage_height;   /* round_up(display, 16) (config.rs:57-58) */
+    uint32_t fps;
+    int32_t keyframe_mode;                    /* riv_keyframe_mode */
+    uint64_t keyframe_interval;               /* N for RIV_KEYFRAME_FIXED */
+    uint8_t intra_quality, inter_quality;
+    uint8_t search_range;
+    uint8_t mv_deflate_level;                 /* vestigial in the reference, validated only */
+    uint8_t skip_threshold;
+    int64_t me_zero_mv_threshold, me_predictor_threshold;
+    double rdo_lambda_mult;
+    float inter_dead_zone;
+    int32_t device_id;                        /* HIP device ordinal */
+} riv_encoder_config;
+
+/* EncoderConfig::new(w, h, fps) defaults (config.rs:56-77) */
+void riv_encoder_config_default(riv_encoder_config* cfg, uint32_t display_width, uint32_t display_height, uint32_t fps);
+/* EncoderConfig::validate (config.rs:129-164) */
+int riv_encoder_config_validate(const riv_encoder_config* cfg);
+
+/* VideoWriter trait (reitero_encode/src/writer.rs:5-18). Callbacks return 0 on success. */
+typedef struct {
+    int (*write)(void* user, const uint8_t* data, size_t len);
+    uint64_t (*position)(void* user);
+    int (*seek)(void* user, uint64_t pos);
+    int (*flush)(void* user);
+} riv_writer_vtbl;
+
+/* VideoReader trait (reitero_decode/src/reader.rs:4-13): read returns bytes read, 0 at end. */
+typedef struct {
+    size_t (*read)(void* user, uint8_t* buf, size_t cap);
+    uint64_t (*position)(void* user);
+    int (*seek)(void* user, uint64_t pos);
+} riv_reader_vtbl;
+
+/* VideoHeader (reitero_video_common/src/format.rs:34-88) */
+typedef struct {
+    uint32_t display_width, display_height, storage_width, storage_height, fps;
+    uint64_t frame_count;
+} riv_video_header;
+
+/* EncodeFrameStats (reitero_encode/src/encoder.rs:58-124), the fields this backend fills. */
+typedef struct {
+    uint64_t frame_index, timestamp_ms;
+    int32_t frame_type;                       /* 1 = intra, 2 = inter (FrameType, format.rs:12-17) */
+    uint32_t total_bytes, mv_bytes, residual_bytes;
+    uint32_t storage_width, storage_height;
+    uint32_t blocks_total, blocks_skipped;
+    uint32_t mv_mode_counts[7];               /* Zero, Nearest, Near, TopRight, TopLeft, New, Temporal */
+    uint32_t mv_new_blocks;
+    int32_t mv_bias_dx, mv_bias_dy;
+    double rdo_lambda;
+    uint32_t rdo_blocks_evaluated, rdo_blocks_selected_skip, rdo_forced_fractional, rdo_forced_threshold, rdo_forced_disabled;
+    double gpu_ms;                            /* device time of this frame's kernels (no reference counterpart) */
+    double host_entropy_ms;                   /* host rANS + packing time (no reference counterpart) */
+} riv_frame_stats;
+
+typedef struct riv_encoder riv_encoder;
+typedef struct riv_decoder riv_decoder;
+
+/* Encoder::new (reitero_encode/src/encoder.rs:162-188): validates, writes the 36-byte header. */
+int riv_encoder_create(const riv_encoder_config* cfg, const riv_writer_vtbl* writer, void* user, riv_encoder** out);
+/* Encoder::new with VecWriter (writer.rs:21-77): output kept in memory, see riv_encoder_output. */
+int riv_encoder_create_mem(const riv_encoder_config* cfg, riv_encoder** out);
+/* Encoder::encode_frame / encode_frame_with_stats (encoder.rs:229-256). rgb: host RGB24,
+ * width*height*3 bytes. stats may be NULL. */
+int riv_encode_frame(riv_encoder* enc, const uint8_t* rgb, size_t len, uint32_t width, uint32_t height,
+                     uint64_t timestamp_ms, riv_frame_stats* stats);
+/* Same, with the RGB24 frame already resident in device memory of the encoder's device. */
+int riv_encode_frame_device(riv_encoder* enc, const void* d_rgb, size_t len, uint32_t width, uint32_t height,
+                            uint64_t timestamp_ms, riv_frame_stats* stats);
+/* Encoder::finish (encoder.rs:950-961): patches frame_count into the header, flushes. */
+int riv_encoder_finish(riv_encoder* enc);
+/* Encoder::frame_count / header (encoder.rs:969-981) */
+uint64_t riv_encoder_frame_count(const riv_encoder* enc);
+int riv_encoder_header(const riv_encoder* enc, riv_video_header* out);
+/* VecWriter::into_inner (writer.rs:33): bytes written so far (mem writer only). */
+int riv_encoder_output(const riv_encoder* enc, const uint8_t** data, size_t* len);
+void riv_encoder_destroy(riv_encoder* enc);
+
+/* Per-block record of the last encoded P-frame: search MV (dx,dy,sx,sy), reconstructed MV
+ * (dx,dy,sx,sy), mode, new_base, raw NEW delta (dx,dy) -- the values of
+ * motion/scalar.rs:546-636 and encoder.rs:447-733. rec12: nblocks*12 int8, sad: nblocks. */
+int riv_encoder_debug_blocks(riv_encoder* enc, int8_t* rec12, int64_t* sad, uint8_t* rdo_skip, uint8_t* opt_skip);
+/* Last frame's planes: which = 0 current (after RGB->YUV), 1 motion-compensated prediction,
+ * 2 reconstruction after deblock (the next reference). */
+int riv_encoder_debug_yuv(riv_encoder* enc, int which, uint8_t* y, uint8_t* u, uint8_t* v);
+/* Last frame's quantised coefficients, nblocks*384 int16 (Y 256 | U 64 | V 64, row-major), zero for skipped blocks. */
+int riv_encoder_debug_coeffs(riv_encoder* enc, int16_t* coeffs);
+/* Last P-frame's adaptive quant steps (residual.rs:116-167), nblocks floats. */
+int riv_encoder_debug_aq(riv_encoder* enc, float* steps);
+
+/* Decoder::new (reitero_decode/src/decoder.rs:112-171): reads and validates the header. */
+int riv_decoder_create(const riv_reader_vtbl* reader, void* user, int32_t device_id, riv_decoder** out);
+int riv_decoder_create_mem(const uint8_t* data, size_t len, int32_t device_id, riv_decoder** out);
+
+/* DecodedFrame (decoder.rs:36-49) metadata; the RGB24 pixels go to the caller's buffer. */
+typedef struct {
+    uint32_t width, height;
+    uint64_t timestamp_ms, frame_index;
+    int32_t frame_type;
+} riv_decoded_info;
+
+/* Decoder::decode_frame (decoder.rs:174-192): RGB24 cropped to display size. */
+int riv_decode_frame(riv_decoder* dec, uint8_t* rgb_out, size_t cap, riv_decoded_info* info);
+/* Decoder::decode_frame_null (decoder.rs:463-470) */
+int riv_decoder_decode_frame_null(riv_decoder* dec);
+/* Decoder::has_more_frames / header / set_skip_residuals (decoder.rs:472-499) */
+int riv_decoder_has_more_frames(const riv_decoder* dec);
+int riv_decoder_header(const riv_decoder* dec, riv_video_header* out);
+int riv_decoder_set_skip_residuals(riv_decoder* dec, int skip);
+/* Last decoded frame's storage-size YUV planes (before RGB conversion). */
+int riv_decoder_debug_yuv(riv_decoder* dec, uint8_t* y, uint8_t* u, uint8_t* v);
+void riv_decoder_destroy(riv_decoder* dec);
+
+/* ---- kernel-level seams (reference internal plug points), host buffers in/out ---- */
+/* build_predicted (reitero_video_common/src/fast_motion.rs:103-108); mv4 = nblocks*(dx,dy,sx,sy) */
+int riv_gpu_build_predicted(int32_t device_id, const uint8_t* y, const uint8_t* u, const uint8_t* v, uint32_t w, uint32_t h,
+                            const int8_t* mv4, uint8_t* out_y, uint8_t* out_u, uint8_t* out_v);
+/* deblock_yuv420 (reitero_video_common/src/deblock.rs:189-202); mask may be NULL (all edges) */
+int riv_gpu_deblock_yuv420(int32_t device_id, uint8_t* y, uint8_t* u, uint8_t* v, uint32_t w, uint32_t h, int32_t level,
+                           int32_t intra, const uint8_t* mask);
+/* encode_plane_8x8_aq + decode_plane_8x8_aq (reitero_dct/src/dct8.rs:80-135, fast_dct8.rs:69-110) on one plane */
+int riv_gpu_dct8_aq_plane(int32_t device_id, const int16_t* plane, uint32_t w, uint32_t h, const float* qs, float dead_zone,
+                          int16_t* coeffs, int16_t* recon);
+/* encode_plane_16x16_aq + decode_plane_16x16_aq (reitero_dct/src/dct16.rs:274-412) on one plane */
+int riv_gpu_dct16_aq_plane(int32_t device_id, const int16_t* plane, uint32_t w, uint32_t h, const float* qs, float dead_zone,
+                           int16_t* coeffs, int16_t* recon);
+/* compute_aq_quant_steps (reitero_residual/src/residual.rs:116-167) on a luma plane */
+int riv_gpu_aq_steps(int32_t device_id, const uint8_t* y, uint32_t w, uint32_t h, float base_step, float* out);
+/* glibc-identical logf / exp2f on the device (used by AQ), n values */
+int riv_gpu_logf_exp2f(int32_t device_id, const float* in, size_t n, int32_t which, float* out);
+
+/* Last error message of the calling thread (static storage). */
+const char* riv_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RIV_GPU_H */

@@ -1,0 +1,267 @@
+// Decoder driver of the C ABI: Decoder::decode_frame of reitero_decode
+// (reitero_decode/src/decoder.rs). Entropy decoding and MV reconstruction run on the host;
+// motion compensation, dequantisation, IDCT, reconstruction, deblock and YUV -> RGB on the GPU.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <vector>
+#include "host_common.hpp"
+#include "riv_kernels.hpp"
+#include "entropy.hpp"
+
+using namespace rivhost;
+
+struct riv_decoder {
+    riv_reader_vtbl rv{};
+    void* ruser = nullptr;
+    bool mem = false;
+    std::vector<uint8_t> membuf;
+    size_t mempos = 0;
+    riv_video_header hdr{};
+    uint64_t current = 0;
+    bool skip_residuals = false;
+    int dev = 0;
+    int W = 0, H = 0, bw = 0, bh = 0, nb = 0;
+    hipStream_t st = nullptr;
+    DevBuf ref[2], pred, coeffs, mvs, skip, mask, var, lnv, steps, rgb, yt, ct;
+    HostBuf h_coeffs, h_rgb;
+    int ref_idx = 0;
+    bool have_ref = false, have_prev_mvs = false;
+    std::vector<Mv4> prev_mvs;
+    rivh::MvCoder mvc;
+    uint8_t last_yt_q = 0;
+
+    // read_exact (decoder.rs:502-511): a short read is EndOfStream
+    bool read_exact(uint8_t* dst, size_t n) {
+        if (mem) {
+            if (mempos + n > membuf.size()) return false;
+            std::memcpy(dst, membuf.data() + mempos, n);
+            mempos += n;
+            return true;
+        }
+        while (n) {
+            size_t got = rv.read(ruser, dst, n);
+            if (got == 0) return false;
+            dst += got; n -= got;
+        }
+        return true;
+    }
+};
+
+extern "C" {
+
+static int decoder_init(riv_decoder* d) {
+    uint8_t h[36];
+    if (!d->read_exact(h, 36)) { set_error("truncated header"); return RIV_E_END_OF_STREAM; }
+    if (!(h[0] == 'R' && h[1] == 'I' && h[2] == 'V' && h[3] == 0)) { set_error("Invalid magic bytes"); return RIV_E_INVALID_HEADER; }
+    if (get_u32(h + 4) != 5) { set_error("Unsupported version"); return RIV_E_INVALID_HEADER; }
+    d->hdr.display_width = get_u32(h + 8); d->hdr.display_height = get_u32(h + 12);
+    d->hdr.storage_width = get_u32(h + 16); d->hdr.storage_height = get_u32(h + 20);
+    d->hdr.fps = get_u32(h + 24); d->hdr.frame_count = get_u64(h + 28);
+    if (d->hdr.storage_width % 16 || d->hdr.storage_height % 16 || d->hdr.storage_width == 0 || d->hdr.storage_height == 0 ||
+        d->hdr.display_width > d->hdr.storage_width || d->hdr.display_height > d->hdr.storage_height) {
+        set_error("storage dims must be non-zero multiples of 16 covering the display");
+        return RIV_E_INVALID_HEADER;
+    }
+    int r = ensure_device(d->dev);
+    if (r) return r;
+    d->W = (int)d->hdr.storage_width; d->H = (int)d->hdr.storage_height;
+    d->bw = d->W / 16; d->bh = d->H / 16; d->nb = d->bw * d->bh;
+    const size_t ysz = (size_t)d->W * d->H, fsz = ysz + ysz / 2;
+    RIV_HIP(hipStreamCreateWithFlags(&d->st, hipStreamNonBlocking));
+    RIV_HIP(d->ref[0].alloc(fsz)); RIV_HIP(d->ref[1].alloc(fsz)); RIV_HIP(d->pred.alloc(fsz));
+    RIV_HIP(d->coeffs.alloc(768 * (size_t)d->nb)); RIV_HIP(d->mvs.alloc(4 * (size_t)d->nb));
+    RIV_HIP(d->skip.alloc(d->nb)); RIV_HIP(d->mask.alloc(d->nb));
+    RIV_HIP(d->var.alloc(8 * (size_t)d->nb)); RIV_HIP(d->lnv.alloc(4 * (size_t)d->nb)); RIV_HIP(d->steps.alloc(4 * (size_t)d->nb));
+    size_t rgbsz = (size_t)d->hdr.display_width * d->hdr.display_height * 3;
+    RIV_HIP(d->rgb.alloc(rgbsz)); RIV_HIP(d->h_rgb.alloc(rgbsz));
+    RIV_HIP(d->h_coeffs.alloc(768 * (size_t)d->nb));
+    RIV_HIP(d->yt.alloc(512)); RIV_HIP(d->ct.alloc(128));
+    return RIV_OK;
+}
+
+int riv_decoder_create(const riv_reader_vtbl* reader, void* user, int32_t device_id, riv_decoder** out) {
+    if (!reader || !reader->read || !out) return RIV_E_ARG;
+    riv_decoder* d = new riv_decoder();
+    d->rv = *reader; d->ruser = user; d->dev = device_id;
+    int r = decoder_init(d);
+    if (r) { delete d; return r; }
+    *out = d;
+    return RIV_OK;
+}
+int riv_decoder_create_mem(const uint8_t* data, size_t len, int32_t device_id, riv_decoder** out) {
+    if ((!data && len) || !out) return RIV_E_ARG;
+    riv_decoder* d = new riv_decoder();
+    d->mem = true; d->membuf.assign(data, data + len); d->dev = device_id;
+    int r = decoder_init(d);
+    if (r) { delete d; return r; }
+    *out = d;
+    return RIV_OK;
+}
+
+// decode_next (decoder.rs:195-459)
+static int decode_next(riv_decoder* d, uint8_t* rgb_out, riv_decoded_info* info) {
+    uint8_t head[9];
+    if (!d->read_exact(head, 9)) { set_error("end of stream"); return RIV_E_END_OF_STREAM; }
+    const uint64_t ts = get_u64(head);
+    const uint8_t ft = head[8];
+    if (ft != 1 && ft != 2) { set_error("Bad frame_type"); return RIV_E_INVALID_FRAME; }
+    RIV_HIP(hipSetDevice(d->dev));
+    const int W = d->W, H = d->H, nb = d->nb, bw = d->bw, bh = d->bh;
+    const size_t ysz = (size_t)W * H, csz = ysz / 4;
+    uint8_t* nY = d->ref[1 - d->ref_idx].as<uint8_t>(); uint8_t* nU = nY + ysz; uint8_t* nV = nU + csz;
+    uint8_t* rY = d->ref[d->ref_idx].as<uint8_t>(); uint8_t* rU = rY + ysz; uint8_t* rV = rU + csz;
+    uint8_t* pY = d->pred.as<uint8_t>(); uint8_t* pU = pY + ysz; uint8_t* pV = pU + csz;
+    hipStream_t st = d->st;
+    int16_t* hco = d->h_coeffs.as<int16_t>();
+    // host staging for this frame's uploads; kept alive until the stream is synchronised
+    std::vector<Mv4> mv;
+    std::vector<uint8_t> skip, coded, mask;
+    uint16_t yt[256], ct[64];
+    try {
+        if (ft == 1) {
+            uint8_t qb[5];
+            if (!d->read_exact(qb, 5)) return RIV_E_END_OF_STREAM;
+            const uint8_t q = qb[0];
+            std::vector<uint8_t> payload(get_u32(qb + 1));
+            if (!d->read_exact(payload.data(), payload.size())) return RIV_E_END_OF_STREAM;
+            // ResidualDecoder::decode_intra (reitero_residual/src/residual.rs:815-887)
+            std::fill(hco, hco + (size_t)nb * 384, 0);
+            coded.assign(nb, 1);
+            rivh::decode_residuals(payload.data(), payload.size(), coded.data(), nb, true, hco);
+            const float qs = quant_step(q);
+            intra_tables(qs, yt, ct);
+            RIV_HIP(hipMemcpyAsync(d->yt.p, yt, sizeof(yt), hipMemcpyHostToDevice, st));
+            RIV_HIP(hipMemcpyAsync(d->ct.p, ct, sizeof(ct), hipMemcpyHostToDevice, st));
+            RIV_HIP(hipMemcpyAsync(d->coeffs.p, hco, 768 * (size_t)nb, hipMemcpyHostToDevice, st));
+            rivk::intra_recon(st, W, H, d->yt.as<uint16_t>(), d->ct.as<uint16_t>(), d->coeffs.as<int16_t>(), nY, nU, nV);
+            if (!d->skip_residuals) rivk::deblock(st, nY, nU, nV, W, H, deblock_level(qs), true, nullptr);
+            d->have_prev_mvs = false;
+            d->mvc.reset();
+        } else {
+            uint8_t hb[8];
+            if (!d->read_exact(hb, 8)) return RIV_E_END_OF_STREAM;
+            const uint8_t q = hb[0];
+            const int bias_dx = (int8_t)hb[1], bias_dy = (int8_t)hb[2];
+            std::vector<uint8_t> mvp(get_u32(hb + 4));
+            if (!d->read_exact(mvp.data(), mvp.size())) return RIV_E_END_OF_STREAM;
+            uint8_t rs[4];
+            if (!d->read_exact(rs, 4)) return RIV_E_END_OF_STREAM;
+            std::vector<uint8_t> res(get_u32(rs));
+            if (!d->read_exact(res.data(), res.size())) return RIV_E_END_OF_STREAM;
+            if (!d->have_ref) { set_error("Inter frame before first intra frame"); return RIV_E_INVALID_FRAME; }
+            std::vector<rivh::MvBlock> blocks = d->mvc.decode(mvp.data(), mvp.size(), bw, bh);
+            // MV reconstruction (decoder.rs:316-382): base from the predictors of the growing list
+            mv.assign(nb, Mv4{});
+            skip.assign(nb, 0); coded.assign(nb, 0); mask.assign(nb, 0);
+            const Mv4* prev = d->have_prev_mvs ? d->prev_mvs.data() : nullptr;
+            for (int i = 0; i < nb; i++) {
+                const rivh::MvBlock& b = blocks[i];
+                Neighbours n = neighbours(mv.data(), i, prev, bw, bh, i % bw, i / bw);
+                Mv4 base{};
+                switch (b.mode) {
+                    case riv::M_ZERO: break;
+                    case riv::M_NEAREST: base = n.nearest; break;
+                    case riv::M_NEAR: base = n.nearp; break;
+                    case riv::M_TOPRIGHT: base = n.has_tr ? n.tr : n.nearest; break;
+                    case riv::M_TOPLEFT: base = n.has_tl ? n.tl : n.nearest; break;
+                    case riv::M_TEMPORAL: base = n.temporal; break;
+                    default:
+                        base = b.new_base == 1 ? n.nearp : b.new_base == 2 ? (n.has_tr ? n.tr : n.nearest)
+                               : b.new_base == 3 ? (n.has_tl ? n.tl : n.nearest) : b.new_base == 4 ? n.temporal : n.nearest;
+                }
+                int ddx = 0, ddy = 0;
+                if (b.mode == riv::M_NEW) {
+                    ddx = std::max(-128, std::min(127, (int)b.dx + bias_dx));
+                    ddy = std::max(-128, std::min(127, (int)b.dy + bias_dy));
+                }
+                Mv4 m;
+                m.v[0] = (int8_t)std::max(-128, std::min(127, (int)base.v[0] + ddx));
+                m.v[1] = (int8_t)std::max(-128, std::min(127, (int)base.v[1] + ddy));
+                m.v[2] = b.mode == riv::M_NEW ? b.sx : base.v[2];
+                m.v[3] = b.mode == riv::M_NEW ? b.sy : base.v[3];
+                mv[i] = m;
+                skip[i] = b.skip;
+                coded[i] = !b.skip;
+                mask[i] = !b.skip || m.v[0] || m.v[1] || m.v[2] || m.v[3];   // decoder.rs:412-423
+            }
+            RIV_HIP(hipMemcpyAsync(d->mvs.p, mv.data(), 4 * (size_t)nb, hipMemcpyHostToDevice, st));
+            rivk::mc(st, rY, rU, rV, W, H, d->mvs.as<int8_t>(), 4, pY, pU, pV, d->var.as<uint32_t>());
+            if (d->skip_residuals) {
+                RIV_HIP(hipMemcpyAsync(nY, pY, ysz + 2 * csz, hipMemcpyDeviceToDevice, st));
+            } else {
+                std::fill(hco, hco + (size_t)nb * 384, 0);
+                rivh::decode_residuals(res.data(), res.size(), coded.data(), nb, false, hco);
+                RIV_HIP(hipMemcpyAsync(d->coeffs.p, hco, 768 * (size_t)nb, hipMemcpyHostToDevice, st));
+                RIV_HIP(hipMemcpyAsync(d->skip.p, skip.data(), nb, hipMemcpyHostToDevice, st));
+                RIV_HIP(hipMemcpyAsync(d->mask.p, mask.data(), nb, hipMemcpyHostToDevice, st));
+                rivk::aq(st, d->var.as<uint32_t>(), nb, quant_step(q), d->lnv.as<float>(), d->steps.as<float>());
+                rivk::inter_recon(st, pY, pU, pV, W, H, d->skip.as<uint8_t>(), d->steps.as<float>(), d->coeffs.as<int16_t>(), nY, nU, nV);
+                rivk::deblock(st, nY, nU, nV, W, H, deblock_level(quant_step(q)), false, d->mask.as<uint8_t>());
+            }
+            d->prev_mvs = mv;
+            d->have_prev_mvs = true;
+        }
+    } catch (const rivh::CorruptStream& ex) {
+        set_error(std::string("corrupt entropy data: ") + ex.what());
+        return RIV_E_DECODING_FAILED;
+    }
+    d->ref_idx = 1 - d->ref_idx;
+    d->have_ref = true;
+    const int dw = (int)d->hdr.display_width, dh = (int)d->hdr.display_height;
+    if (rgb_out) {
+        rivk::yuv420_to_rgb(st, nY, nU, nV, W, dw, dh, d->rgb.as<uint8_t>());
+        RIV_HIP(hipMemcpyAsync(rgb_out, d->rgb.p, (size_t)dw * dh * 3, hipMemcpyDeviceToHost, st));
+    }
+    RIV_HIP(hipGetLastError());
+    RIV_HIP(hipStreamSynchronize(st));
+    if (info) { info->width = (uint32_t)dw; info->height = (uint32_t)dh; info->timestamp_ms = ts; info->frame_index = d->current; info->frame_type = ft; }
+    return RIV_OK;
+}
+
+int riv_decode_frame(riv_decoder* d, uint8_t* rgb_out, size_t cap, riv_decoded_info* info) {
+    if (!d) return RIV_E_ARG;
+    if (d->current >= d->hdr.frame_count) { set_error("end of stream"); return RIV_E_END_OF_STREAM; }   // decoder.rs:175-177
+    if (rgb_out && cap < (size_t)d->hdr.display_width * d->hdr.display_height * 3) return RIV_E_ARG;
+    int r = decode_next(d, rgb_out, info);
+    if (r) return r;
+    d->current++;
+    return RIV_OK;
+}
+int riv_decoder_decode_frame_null(riv_decoder* d) {
+    if (!d) return RIV_E_ARG;
+    if (d->current >= d->hdr.frame_count) return RIV_E_END_OF_STREAM;
+    std::vector<uint8_t> sink((size_t)d->hdr.display_width * d->hdr.display_height * 3);
+    int r = decode_next(d, sink.data(), nullptr);
+    if (r) return r;
+    d->current++;
+    return RIV_OK;
+}
+int riv_decoder_has_more_frames(const riv_decoder* d) { return d && d->current < d->hdr.frame_count; }
+int riv_decoder_header(const riv_decoder* d, riv_video_header* out) {
+    if (!d || !out) return RIV_E_ARG;
+    *out = d->hdr;
+    return RIV_OK;
+}
+int riv_decoder_set_skip_residuals(riv_decoder* d, int skip) {
+    if (!d) return RIV_E_ARG;
+    d->skip_residuals = skip != 0;
+    return RIV_OK;
+}
+int riv_decoder_debug_yuv(riv_decoder* d, uint8_t* y, uint8_t* u, uint8_t* v) {
+    if (!d || !d->have_ref) return RIV_E_ARG;
+    RIV_HIP(hipSetDevice(d->dev));
+    const size_t ysz = (size_t)d->W * d->H, csz = ysz / 4;
+    const uint8_t* src = d->ref[d->ref_idx].as<uint8_t>();
+    RIV_HIP(hipMemcpy(y, src, ysz, hipMemcpyDeviceToHost));
+    RIV_HIP(hipMemcpy(u, src + ysz, csz, hipMemcpyDeviceToHost));
+    RIV_HIP(hipMemcpy(v, src + ysz + csz, csz, hipMemcpyDeviceToHost));
+    return RIV_OK;
+}
+void riv_decoder_destroy(riv_decoder* d) {
+    if (!d) return;
+    (void)hipSetDevice(d->dev);
+    if (d->st) { (void)hipStreamSynchronize(d->st); (void)hipStreamDestroy(d->st); }
+    delete d;
+}
+
+}  // extern "C"

@@ -1,0 +1,420 @@
+// Encoder driver of the C ABI: Encoder / EncoderConfig / encode_frame of reitero_encode
+// (reitero_encode/src/encoder.rs, config.rs), with the per-frame analysis on the GPU and the
+// rANS entropy coding on the host.
+#include <hip/hip_runtime.h>
+#include <chrono>
+#include <cmath>
+#include <algorithm>
+#include <vector>
+#include "host_common.hpp"
+#include "riv_kernels.hpp"
+#include "entropy.hpp"
+
+using namespace rivhost;
+using riv::BlockRec;
+
+struct riv_encoder {
+    riv_encoder_config cfg{};
+    riv_writer_vtbl wv{};
+    void* wuser = nullptr;
+    bool mem = false;
+    std::vector<uint8_t> membuf;
+    uint64_t mempos = 0;
+    uint64_t header_pos = 0;
+    riv_video_header hdr{};
+    uint64_t frame_count = 0;
+
+    int W = 0, H = 0, bw = 0, bh = 0, nb = 0, R = 4;
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevBuf rgb, cur, pred, ref[2], rec[2], var, lnv, steps, coeffs, optskip, mask, progress, bits, rdo, err, yt, ct;
+    HostBuf h_rec, h_optskip, h_coeffs;
+    int ref_idx = 0, rec_idx = 0;
+    bool have_ref = false, have_prev_mvs = false, last_inter = false;
+    unsigned long long epoch = 0;
+    // frame constants
+    double lambda = 0;
+    int c_zero = 0, c_pred = 0, rdo_max = -1;
+    float inter_qs = 0, intra_qs = 0;
+    rivh::MvCoder mvc;
+
+    int write(const uint8_t* d, size_t n) {
+        if (mem) {
+            if (mempos + n > membuf.size()) membuf.resize(mempos + n);
+            std::memcpy(membuf.data() + mempos, d, n);
+            mempos += n;
+            return RIV_OK;
+        }
+        return wv.write(wuser, d, n) == 0 ? RIV_OK : RIV_E_IO;
+    }
+    uint64_t position() const { return mem ? mempos : wv.position(wuser); }
+    int seek(uint64_t pos) {
+        if (mem) { if (pos > membuf.size()) membuf.resize(pos); mempos = pos; return RIV_OK; }
+        return wv.seek(wuser, pos) == 0 ? RIV_OK : RIV_E_IO;
+    }
+};
+
+extern "C" {
+
+void riv_encoder_config_default(riv_encoder_config* c, uint32_t w, uint32_t h, uint32_t fps) {
+    std::memset(c, 0, sizeof(*c));
+    c->display_width = w; c->display_height = h;
+    c->storage_width = (w + 15) / 16 * 16; c->storage_height = (h + 15) / 16 * 16;
+    c->fps = fps;
+    c->keyframe_mode = RIV_KEYFRAME_FIXED; c->keyframe_interval = 30;
+    c->intra_quality = 90; c->inter_quality = 35;
+    c->search_range = 12; c->mv_deflate_level = 6; c->skip_threshold = 3;
+    c->me_zero_mv_threshold = 0; c->me_predictor_threshold = 0;
+    c->rdo_lambda_mult = 0.49; c->inter_dead_zone = 0.75f;
+    c->device_id = 0;
+}
+
+int riv_encoder_config_validate(const riv_encoder_config* c) {
+    if (!c) return RIV_E_ARG;
+    if (c->display_width == 0 || c->display_height == 0) { set_error("Width and height must be greater than 0"); return RIV_E_INVALID_CONFIG; }
+    if (c->storage_width < c->display_width || c->storage_height < c->display_height) { set_error("Storage dimensions must be >= display dimensions"); return RIV_E_INVALID_CONFIG; }
+    if (c->fps == 0) { set_error("FPS must be greater than 0"); return RIV_E_INVALID_CONFIG; }
+    if (c->search_range > 127) { set_error("search_range must be <= 127 for current MV encoding"); return RIV_E_INVALID_CONFIG; }
+    if (c->mv_deflate_level > 10) { set_error("mv_deflate_level must be in 0..=10"); return RIV_E_INVALID_CONFIG; }
+    if (c->keyframe_mode == RIV_KEYFRAME_FIXED && c->keyframe_interval == 0) { set_error("Keyframe interval must be greater than 0"); return RIV_E_INVALID_CONFIG; }
+    return RIV_OK;
+}
+
+static int encoder_init(riv_encoder* e) {
+    const riv_encoder_config& c = e->cfg;
+    if (c.storage_width % 16 || c.storage_height % 16) { set_error("storage dims must be multiples of 16"); return RIV_E_INVALID_CONFIG; }
+    if (c.search_range > 32) { set_error("this backend supports search_range <= 32"); return RIV_E_UNSUPPORTED; }
+    int r = ensure_device(c.device_id);
+    if (r) return r;
+    e->W = (int)c.storage_width; e->H = (int)c.storage_height;
+    e->bw = e->W / 16; e->bh = e->H / 16; e->nb = e->bw * e->bh;
+    // rows per wavefront workgroup: as many as fit the LDS budget, at most 4
+    e->R = 4;
+    while (e->R > 1 && rivk::walk_lds_bytes(c.search_range, e->bw, e->R) > 60000) e->R--;
+    const size_t ysz = (size_t)e->W * e->H, csz = ysz / 4, fsz = ysz + 2 * csz;
+    RIV_HIP(hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking));
+    RIV_HIP(hipEventCreate(&e->ev0)); RIV_HIP(hipEventCreate(&e->ev1));
+    RIV_HIP(e->rgb.alloc((size_t)c.display_width * c.display_height * 3));
+    RIV_HIP(e->cur.alloc(fsz)); RIV_HIP(e->pred.alloc(fsz));
+    RIV_HIP(e->ref[0].alloc(fsz)); RIV_HIP(e->ref[1].alloc(fsz));
+    RIV_HIP(e->rec[0].alloc(sizeof(BlockRec) * e->nb)); RIV_HIP(e->rec[1].alloc(sizeof(BlockRec) * e->nb));
+    RIV_HIP(e->var.alloc(8 * (size_t)e->nb)); RIV_HIP(e->lnv.alloc(4 * (size_t)e->nb)); RIV_HIP(e->steps.alloc(4 * (size_t)e->nb));
+    RIV_HIP(e->coeffs.alloc(768 * (size_t)e->nb)); RIV_HIP(e->optskip.alloc(e->nb)); RIV_HIP(e->mask.alloc(e->nb));
+    RIV_HIP(e->progress.alloc(8 * (size_t)e->bh)); RIV_HIP(hipMemset(e->progress.p, 0, 8 * (size_t)e->bh));
+    RIV_HIP(e->err.alloc(4)); RIV_HIP(hipMemset(e->err.p, 0, 4));
+    RIV_HIP(e->h_rec.alloc(sizeof(BlockRec) * e->nb)); RIV_HIP(e->h_optskip.alloc(e->nb)); RIV_HIP(e->h_coeffs.alloc(768 * (size_t)e->nb));
+    // MV-cost bit table (motion/scalar.rs:495-525) and lambda-derived costs
+    rivhost::RdoModel rdo(c.inter_quality, c.rdo_lambda_mult, c.skip_threshold);
+    e->lambda = rdo.lambda;
+    double bitsv[256];
+    for (int a = 0; a < 256; a++) bitsv[a] = a == 0 ? 1.0 : 2.0 * std::log2((double)a) + 2.0;
+    RIV_HIP(e->bits.alloc(sizeof(bitsv)));
+    RIV_HIP(hipMemcpy(e->bits.p, bitsv, sizeof(bitsv), hipMemcpyHostToDevice));
+    const double lam2 = e->lambda * 2.0;
+    e->c_zero = (int)std::min(1e9, std::trunc(lam2 * 0.5));
+    e->c_pred = (int)std::min(1e9, std::trunc(lam2 * 1.5));
+    // RDO skip table over every reachable SAD (a 16x16 SAD is at most 65280)
+    e->rdo_max = c.skip_threshold == 0 ? -1 : (int)std::min<int64_t>(65280, 768 * (int64_t)c.skip_threshold);
+    std::vector<uint32_t> rb((65281 + 31) / 32, 0);
+    for (int s = 0; s <= e->rdo_max; s++) if (rdo.skip(s, true)) rb[s >> 5] |= 1u << (s & 31);
+    RIV_HIP(e->rdo.alloc(rb.size() * 4));
+    RIV_HIP(hipMemcpy(e->rdo.p, rb.data(), rb.size() * 4, hipMemcpyHostToDevice));
+    e->inter_qs = quant_step(c.inter_quality);
+    e->intra_qs = quant_step(c.intra_quality);
+    uint16_t yt[256], ct[64];
+    intra_tables(e->intra_qs, yt, ct);
+    RIV_HIP(e->yt.alloc(sizeof(yt))); RIV_HIP(e->ct.alloc(sizeof(ct)));
+    RIV_HIP(hipMemcpy(e->yt.p, yt, sizeof(yt), hipMemcpyHostToDevice));
+    RIV_HIP(hipMemcpy(e->ct.p, ct, sizeof(ct), hipMemcpyHostToDevice));
+    return RIV_OK;
+}
+
+static int write_header(riv_encoder* e) {
+    std::vector<uint8_t> h;
+    put_header(h, e->hdr);
+    return e->write(h.data(), h.size());
+}
+
+static int create_common(const riv_encoder_config* cfg, riv_encoder* e) {
+    int r = riv_encoder_config_validate(cfg);
+    if (r) return r;
+    e->cfg = *cfg;
+    r = encoder_init(e);
+    if (r) return r;
+    e->hdr.display_width = cfg->display_width; e->hdr.display_height = cfg->display_height;
+    e->hdr.storage_width = cfg->storage_width; e->hdr.storage_height = cfg->storage_height;
+    e->hdr.fps = cfg->fps; e->hdr.frame_count = 0;
+    e->header_pos = e->position();
+    return write_header(e);   // Encoder::new writes the header immediately (encoder.rs:181-185)
+}
+
+int riv_encoder_create(const riv_encoder_config* cfg, const riv_writer_vtbl* writer, void* user, riv_encoder** out) {
+    if (!cfg || !writer || !out || !writer->write || !writer->position || !writer->seek || !writer->flush) return RIV_E_ARG;
+    riv_encoder* e = new riv_encoder();
+    e->wv = *writer; e->wuser = user;
+    int r = create_common(cfg, e);
+    if (r) { delete e; return r; }
+    *out = e;
+    return RIV_OK;
+}
+
+int riv_encoder_create_mem(const riv_encoder_config* cfg, riv_encoder** out) {
+    if (!cfg || !out) return RIV_E_ARG;
+    riv_encoder* e = new riv_encoder();
+    e->mem = true;
+    int r = create_common(cfg, e);
+    if (r) { delete e; return r; }
+    *out = e;
+    return RIV_OK;
+}
+
+// compute_delta_bias_for_new (encoder.rs:984-1048) from a histogram of the raw NEW deltas:
+// per axis, the bias in [-24, 24] with the most zero deltas, ties to the smallest L1 sum.
+static void compute_bias(const BlockRec* rec, int nb, int& bx_out, int& by_out) {
+    uint32_t hx[256] = {0}, hy[256] = {0};
+    uint32_t n = 0;
+    for (int i = 0; i < nb; i++)
+        if (rec[i].mode == riv::M_NEW) { hx[(uint8_t)rec[i].raw_dx]++; hy[(uint8_t)rec[i].raw_dy]++; n++; }
+    bx_out = 0; by_out = 0;
+    if (!n) return;
+    for (int axis = 0; axis < 2; axis++) {
+        const uint32_t* h = axis ? hy : hx;
+        int64_t best_zero = -1, best_cost = INT64_MAX; int best = 0;
+        for (int b = -24; b <= 24; b++) {
+            int64_t zero = 0, cost = 0;
+            for (int v = -128; v < 128; v++) {
+                uint32_t cnt = h[(uint8_t)(int8_t)v];
+                if (!cnt) continue;
+                int adj = std::max(-128, std::min(127, v - b));
+                if (adj == 0) zero += cnt;
+                cost += (int64_t)cnt * std::abs(adj);
+            }
+            if (zero > best_zero || (zero == best_zero && cost < best_cost)) { best_zero = zero; best_cost = cost; best = b; }
+        }
+        (axis ? by_out : bx_out) = best;
+    }
+}
+
+static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t len, uint32_t w, uint32_t h, uint64_t ts, riv_frame_stats* stats) {
+    const riv_encoder_config& c = e->cfg;
+    if (w != c.display_width || h != c.display_height) { set_error("Frame dimensions do not match config"); return RIV_E_FRAME; }
+    if (len != (size_t)w * h * 3) { set_error("Display RGB24 size mismatch"); return RIV_E_FRAME; }
+    RIV_HIP(hipSetDevice(c.device_id));
+    // keyframe decision (encoder.rs:265-291)
+    bool intra;
+    if (c.keyframe_mode == RIV_KEYFRAME_ALL_INTRA) intra = true;
+    else if (c.keyframe_mode == RIV_KEYFRAME_AUTOMATIC) intra = e->frame_count % 30 == 0;
+    else intra = e->frame_count % c.keyframe_interval == 0;
+    if (!e->have_ref) intra = true;
+
+    const int W = e->W, H = e->H, nb = e->nb;
+    const size_t ysz = (size_t)W * H, csz = ysz / 4;
+    uint8_t* cY = e->cur.as<uint8_t>(); uint8_t* cU = cY + ysz; uint8_t* cV = cU + csz;
+    uint8_t* rY = e->ref[e->ref_idx].as<uint8_t>(); uint8_t* rU = rY + ysz; uint8_t* rV = rU + csz;
+    uint8_t* nY = e->ref[1 - e->ref_idx].as<uint8_t>(); uint8_t* nU = nY + ysz; uint8_t* nV = nU + csz;
+    uint8_t* pY = e->pred.as<uint8_t>(); uint8_t* pU = pY + ysz; uint8_t* pV = pU + csz;
+    hipStream_t st = e->st;
+
+    RIV_HIP(hipEventRecord(e->ev0, st));
+    const uint8_t* d_rgb = (const uint8_t*)rgb;
+    if (!on_device) {
+        RIV_HIP(hipMemcpyAsync(e->rgb.p, rgb, len, hipMemcpyHostToDevice, st));
+        d_rgb = e->rgb.as<uint8_t>();
+    }
+    rivk::rgb_to_yuv420(st, d_rgb, (int)w, (int)h, W, H, cY, cU, cV);
+
+    BlockRec* rec = e->rec[e->rec_idx].as<BlockRec>();
+    if (intra) {
+        rivk::intra(st, cY, cU, cV, W, H, e->yt.as<uint16_t>(), e->ct.as<uint16_t>(), nY, nU, nV, e->coeffs.as<int16_t>());
+        rivk::deblock(st, nY, nU, nV, W, H, deblock_level(e->intra_qs), true, nullptr);
+        RIV_HIP(hipMemcpyAsync(e->h_coeffs.p, e->coeffs.p, 768 * (size_t)nb, hipMemcpyDeviceToHost, st));
+    } else {
+        riv::WalkParams p{};
+        p.cur = cY; p.ref = rY; p.W = W; p.H = H; p.bw = e->bw; p.bh = e->bh; p.sr = c.search_range;
+        p.prev_mv = e->have_prev_mvs ? (const int8_t*)e->rec[1 - e->rec_idx].p + 4 : nullptr;
+        p.prev_stride = sizeof(BlockRec);
+        p.rec = rec; p.progress = e->progress.as<unsigned long long>();
+        e->epoch++;
+        p.base = e->epoch * 65536ull;
+        p.bits = e->bits.as<double>(); p.lam2 = e->lambda * 2.0; p.c_zero = e->c_zero; p.c_pred = e->c_pred;
+        p.lam_zero = e->lambda == 0.0;
+        p.rdo_bits = e->rdo.as<uint32_t>(); p.rdo_max = e->rdo_max;
+        p.zthr = c.me_zero_mv_threshold; p.pthr = c.me_predictor_threshold;
+        p.R = e->R; p.err = e->err.as<int>();
+        RIV_HIP(rivk::me_walk(st, p));
+        rivk::mc(st, rY, rU, rV, W, H, (const int8_t*)rec, sizeof(BlockRec), pY, pU, pV, e->var.as<uint32_t>());
+        rivk::aq(st, e->var.as<uint32_t>(), nb, e->inter_qs, e->lnv.as<float>(), e->steps.as<float>());
+        rivk::inter_residual(st, cY, cU, cV, pY, pU, pV, W, H, rec, e->steps.as<float>(), c.inter_dead_zone, nY, nU, nV,
+                             e->coeffs.as<int16_t>(), e->optskip.as<uint8_t>());
+        rivk::filter_mask(st, rec, e->optskip.as<uint8_t>(), nb, e->mask.as<uint8_t>());
+        rivk::deblock(st, nY, nU, nV, W, H, deblock_level(e->inter_qs), false, e->mask.as<uint8_t>());
+        RIV_HIP(hipMemcpyAsync(e->h_rec.p, rec, sizeof(BlockRec) * nb, hipMemcpyDeviceToHost, st));
+        RIV_HIP(hipMemcpyAsync(e->h_optskip.p, e->optskip.p, nb, hipMemcpyDeviceToHost, st));
+        RIV_HIP(hipMemcpyAsync(e->h_coeffs.p, e->coeffs.p, 768 * (size_t)nb, hipMemcpyDeviceToHost, st));
+    }
+    RIV_HIP(hipGetLastError());
+    RIV_HIP(hipEventRecord(e->ev1, st));
+    RIV_HIP(hipStreamSynchronize(st));
+    if (!intra) {
+        int werr = 0;
+        RIV_HIP(hipMemcpy(&werr, e->err.p, 4, hipMemcpyDeviceToHost));
+        if (werr) { set_error("motion-search wavefront timed out"); return RIV_E_GPU; }
+    }
+    float gpu_ms = 0;
+    RIV_HIP(hipEventElapsedTime(&gpu_ms, e->ev0, e->ev1));
+
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<uint8_t> bytes;
+    riv_frame_stats s{};
+    s.frame_index = e->frame_count; s.timestamp_ms = ts;
+    s.storage_width = (uint32_t)W; s.storage_height = (uint32_t)H; s.blocks_total = (uint32_t)nb;
+    const int16_t* hco = e->h_coeffs.as<int16_t>();
+    try {
+        if (intra) {
+            std::vector<uint8_t> coded(nb, 1);
+            std::vector<uint8_t> res = rivh::encode_residuals(hco, coded.data(), nb, true);
+            put_u64(bytes, ts); bytes.push_back(1); bytes.push_back(c.intra_quality);   // PackedFrame intra (format.rs:171-181)
+            put_u32(bytes, (uint32_t)res.size());
+            bytes.insert(bytes.end(), res.begin(), res.end());
+            e->have_prev_mvs = false;
+            e->mvc.reset();
+            s.frame_type = 1; s.residual_bytes = (uint32_t)res.size();
+            e->last_inter = false;
+        } else {
+            const BlockRec* hr = e->h_rec.as<BlockRec>();
+            const uint8_t* os = e->h_optskip.as<uint8_t>();
+            int bdx, bdy;
+            compute_bias(hr, nb, bdx, bdy);
+            std::vector<rivh::MvBlock> blocks(nb);
+            std::vector<uint8_t> coded(nb);
+            for (int i = 0; i < nb; i++) {
+                rivh::MvBlock& b = blocks[i];
+                b.mode = hr[i].mode; b.new_base = hr[i].new_base;
+                b.dx = b.dy = 0; b.sx = b.sy = 0;
+                if (b.mode == riv::M_NEW) {
+                    b.dx = (int8_t)std::max(-128, std::min(127, (int)hr[i].raw_dx - bdx));
+                    b.dy = (int8_t)std::max(-128, std::min(127, (int)hr[i].raw_dy - bdy));
+                    b.sx = hr[i].s[2]; b.sy = hr[i].s[3];
+                    s.mv_new_blocks++;
+                }
+                b.skip = os[i] != 0;
+                coded[i] = !os[i];
+                s.blocks_skipped += b.skip;
+                s.mv_mode_counts[b.mode]++;
+                s.rdo_blocks_selected_skip += hr[i].rdo_skip;
+                if (hr[i].s[2] || hr[i].s[3]) s.rdo_forced_fractional++;
+            }
+            std::vector<uint8_t> mvb = e->mvc.encode(blocks, e->bw, e->bh);
+            std::vector<uint8_t> res = rivh::encode_residuals(hco, coded.data(), nb, false);
+            put_u64(bytes, ts); bytes.push_back(2); bytes.push_back(c.inter_quality);   // PackedFrame inter (format.rs:183-205)
+            bytes.push_back((uint8_t)(int8_t)bdx); bytes.push_back((uint8_t)(int8_t)bdy); bytes.push_back(0);
+            put_u32(bytes, (uint32_t)mvb.size()); bytes.insert(bytes.end(), mvb.begin(), mvb.end());
+            put_u32(bytes, (uint32_t)res.size()); bytes.insert(bytes.end(), res.begin(), res.end());
+            e->have_prev_mvs = true;
+            s.frame_type = 2; s.mv_bytes = (uint32_t)mvb.size(); s.residual_bytes = (uint32_t)res.size();
+            s.mv_bias_dx = bdx; s.mv_bias_dy = bdy;
+            s.rdo_lambda = e->lambda; s.rdo_blocks_evaluated = (uint32_t)nb;
+            if (c.skip_threshold == 0) s.rdo_forced_disabled = (uint32_t)nb;
+            e->rec_idx = 1 - e->rec_idx;
+            e->last_inter = true;
+        }
+    } catch (const std::exception& ex) {
+        set_error(std::string("entropy coding failed: ") + ex.what());
+        return RIV_E_ENCODING_FAILED;
+    }
+    e->ref_idx = 1 - e->ref_idx;
+    e->have_ref = true;
+    int r = e->write(bytes.data(), bytes.size());
+    if (r) { set_error("writer failed"); return r; }
+    e->frame_count++;
+    e->hdr.frame_count = e->frame_count;
+    s.total_bytes = (uint32_t)bytes.size();
+    s.gpu_ms = gpu_ms;
+    s.host_entropy_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) *stats = s;
+    return RIV_OK;
+}
+
+int riv_encode_frame(riv_encoder* e, const uint8_t* rgb, size_t len, uint32_t w, uint32_t h, uint64_t ts, riv_frame_stats* stats) {
+    if (!e || !rgb) return RIV_E_ARG;
+    return encode_impl(e, rgb, false, len, w, h, ts, stats);
+}
+int riv_encode_frame_device(riv_encoder* e, const void* d_rgb, size_t len, uint32_t w, uint32_t h, uint64_t ts, riv_frame_stats* stats) {
+    if (!e || !d_rgb) return RIV_E_ARG;
+    return encode_impl(e, d_rgb, true, len, w, h, ts, stats);
+}
+
+int riv_encoder_finish(riv_encoder* e) {   // encoder.rs:204-221, :950-961
+    if (!e) return RIV_E_ARG;
+    uint64_t end = e->position();
+    int r = e->seek(e->header_pos);
+    if (!r) r = write_header(e);
+    if (!r) r = e->seek(end);
+    if (!r && !e->mem && e->wv.flush(e->wuser) != 0) r = RIV_E_IO;
+    return r;
+}
+uint64_t riv_encoder_frame_count(const riv_encoder* e) { return e ? e->frame_count : 0; }
+int riv_encoder_header(const riv_encoder* e, riv_video_header* out) {
+    if (!e || !out) return RIV_E_ARG;
+    *out = e->hdr;
+    return RIV_OK;
+}
+int riv_encoder_output(const riv_encoder* e, const uint8_t** data, size_t* len) {
+    if (!e || !e->mem || !data || !len) return RIV_E_ARG;
+    *data = e->membuf.data(); *len = e->membuf.size();
+    return RIV_OK;
+}
+void riv_encoder_destroy(riv_encoder* e) {
+    if (!e) return;
+    (void)hipSetDevice(e->cfg.device_id);
+    if (e->st) (void)hipStreamSynchronize(e->st);
+    if (e->ev0) (void)hipEventDestroy(e->ev0);
+    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    if (e->st) (void)hipStreamDestroy(e->st);
+    delete e;
+}
+
+int riv_encoder_debug_blocks(riv_encoder* e, int8_t* rec12, int64_t* sad, uint8_t* rdo_skip, uint8_t* opt_skip) {
+    if (!e || !e->last_inter) return RIV_E_ARG;
+    const BlockRec* hr = e->h_rec.as<BlockRec>();
+    for (int i = 0; i < e->nb; i++) {
+        int8_t* r = rec12 + 12 * i;
+        std::memcpy(r, hr[i].s, 4); std::memcpy(r + 4, hr[i].r, 4);
+        r[8] = (int8_t)hr[i].mode; r[9] = (int8_t)hr[i].new_base; r[10] = hr[i].raw_dx; r[11] = hr[i].raw_dy;
+        if (sad) sad[i] = hr[i].sad;
+        if (rdo_skip) rdo_skip[i] = hr[i].rdo_skip;
+        if (opt_skip) opt_skip[i] = e->h_optskip.as<uint8_t>()[i];
+    }
+    return RIV_OK;
+}
+int riv_encoder_debug_yuv(riv_encoder* e, int which, uint8_t* y, uint8_t* u, uint8_t* v) {
+    if (!e || which < 0 || which > 2) return RIV_E_ARG;
+    RIV_HIP(hipSetDevice(e->cfg.device_id));
+    const size_t ysz = (size_t)e->W * e->H, csz = ysz / 4;
+    const uint8_t* src = which == 0 ? e->cur.as<uint8_t>() : which == 1 ? e->pred.as<uint8_t>() : e->ref[e->ref_idx].as<uint8_t>();
+    RIV_HIP(hipMemcpy(y, src, ysz, hipMemcpyDeviceToHost));
+    RIV_HIP(hipMemcpy(u, src + ysz, csz, hipMemcpyDeviceToHost));
+    RIV_HIP(hipMemcpy(v, src + ysz + csz, csz, hipMemcpyDeviceToHost));
+    return RIV_OK;
+}
+int riv_encoder_debug_coeffs(riv_encoder* e, int16_t* out) {
+    if (!e) return RIV_E_ARG;
+    const int16_t* hco = e->h_coeffs.as<int16_t>();
+    const uint8_t* os = e->h_optskip.as<uint8_t>();
+    for (int b = 0; b < e->nb; b++) {
+        bool zero = e->last_inter && os[b];
+        if (zero) std::memset(out + (size_t)b * 384, 0, 768);
+        else std::memcpy(out + (size_t)b * 384, hco + (size_t)b * 384, 768);
+    }
+    return RIV_OK;
+}
+int riv_encoder_debug_aq(riv_encoder* e, float* steps) {
+    if (!e || !e->last_inter) return RIV_E_ARG;
+    RIV_HIP(hipSetDevice(e->cfg.device_id));
+    RIV_HIP(hipMemcpy(steps, e->steps.p, 4 * (size_t)e->nb, hipMemcpyDeviceToHost));
+    return RIV_OK;
+}
+
+const char* riv_last_error(void) { return rivhost::last_error(); }
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+// Host-side entropy coding of the RIV bitstream: adaptive binary rANS core, residual
+// coefficient coder and motion-vector coder. These stay on the CPU (inherently serial per
+// frame); the GPU produces their inputs.
+#pragma once
+#include <cstdint>
+#include <cstddef>
+#include <vector>
+#include <array>
+#include <stdexcept>
+
+namespace rivh {
+
+struct CorruptStream : std::runtime_error { using std::runtime_error::runtime_error; };
+
+// Adaptive binary context: two saturating 8-bit counters packed in 16 bits
+// (reitero_video_common/src/rans.rs:25-85).
+struct Ctx {
+    uint16_t c = 0x0101;
+    uint8_t p0() const;      // P(bit == 0) * 256, in [1, 255]
+    void update(bool bit);
+};
+
+// Symbol-batched rANS encoder with two interleaved 32-bit states and u16 renormalisation
+// (rans.rs:88-298): symbols are buffered, then coded in reverse in blocks of 16386.
+class RansWriter {
+public:
+    explicit RansWriter(std::vector<uint8_t>& out);
+    void bit(bool b, Ctx& c) { uint8_t p = c.p0(); c.update(b); push(b, p); }
+    void bypass(bool b) { push(b, 128); }
+    void flush();
+private:
+    void push(bool b, uint8_t p) {
+        if (n_ == kBatch) drain();
+        sym_[n_] = p;
+        bits_[n_] = b;
+        n_++;
+    }
+    void drain();
+    static constexpr int kBatch = 16386;
+    std::vector<uint8_t>& out_;
+    std::vector<uint8_t> sym_;   // probability of each buffered symbol, in coding order
+    std::vector<uint8_t> bits_;
+    int n_ = 0;
+};
+
+class RansReader {
+public:
+    RansReader(const uint8_t* data, size_t len);
+    bool bit(Ctx& c);
+    bool bypass();
+private:
+    uint32_t state();
+    uint16_t word();
+    void step();
+    const uint8_t* d_; size_t len_, pos_ = 0;
+    uint32_t a_ = 0, b_ = 0;
+    int count_ = 0;
+};
+
+// Residual coefficient coder (reitero_residual/src/rans.rs:1-460). One instance per frame
+// (contexts are fresh every frame). `dc_pred` is the intra DC-prediction mode.
+struct ResidualCtx {
+    Ctx sig_y[256], sig_c[64];
+    Ctx mag_y[4][13], mag_c[3][13];
+    Ctx eob_y[9], eob_c[7];
+    int16_t prev_dc[3] = {0, 0, 0};
+};
+// Codes the coefficients of the MBs with coded[b] != 0, in raster order. coeffs: nb*384.
+std::vector<uint8_t> encode_residuals(const int16_t* coeffs, const uint8_t* coded, int nb, bool dc_pred);
+// Inverse; fills coeffs (nb*384, caller-zeroed) of the MBs with coded[b] != 0.
+void decode_residuals(const uint8_t* data, size_t len, const uint8_t* coded, int nb, bool dc_pred, int16_t* coeffs);
+
+// Motion-vector coder state (reitero_residual/src/mv_rans.rs, mv_rans/encoder.rs, decoder.rs):
+// contexts and the NEW-delta candidate history persist across P-frames, reset at I-frames.
+struct MvBlock {
+    uint8_t mode, new_base;
+    int8_t dx, dy;           // NEW delta after the frame bias
+    int8_t sx, sy;           // sub-pel of a NEW MV: 0, +1 (-1 is rejected by the coder)
+    bool skip;
+};
+class MvCoder {
+public:
+    void reset();
+    std::vector<uint8_t> encode(const std::vector<MvBlock>& blocks, int bw, int bh);
+    std::vector<MvBlock> decode(const uint8_t* data, size_t len, int bw, int bh);
+private:
+    void refresh_history(const std::vector<MvBlock>& blocks, bool from_decoder);
+    int lookup(int8_t dx, int8_t dy, int8_t sx, int8_t sy) const;
+    Ctx skip_;
+    Ctx mode_[6][4];         // is_new, nonzero, is_near, is_temporal, is_top_right, is_top_left
+    Ctx base_[4][4];         // nearest, near, top_right, top_left
+    Ctx cls_[2][4], sign_[2], mag_[2][10];
+    Ctx frac_[2];
+    Ctx hit_[3], idx_[4];
+    // candidate history: key = (dx, dy, sub-pel order x, sub-pel order y)
+    std::vector<std::array<int, 4>> cand_;
+    std::vector<std::pair<std::array<int, 4>, uint32_t>> counts_;
+    uint32_t total_ = 0;
+};
+
+int mv_class(int magnitude);   // mv_class_from_magnitude (mv_rans.rs:81-87)
+
+}  // namespace rivh

@@ -1,0 +1,1053 @@
+// gfx950 kernels of the RIV backend: colour conversion, motion search + mode decision
+// wavefront, motion compensation, adaptive quantisation, residual transforms, deblock.
+// Each kernel states the reference function it reproduces bit-exactly (paths relative
+// to the reference tree, under reitero_video/).
+#include "riv_dev.hpp"
+#include "riv_kernels.hpp"
+
+using namespace rivdev;
+using riv::BlockRec;
+using riv::WalkParams;
+
+namespace rivdev {
+__constant__ int32_t c_B16[16][16];
+__constant__ int32_t c_B8[8][8];
+}
+
+// ============================================================================================
+// Colour conversion
+// ============================================================================================
+// pad_rgb24 (reitero_encode/src/encoder.rs:1049-1096: replicate the last column, then the last
+// row) fused with the project's BT.709 limited-range RGB24 -> YUV420 (14-bit fixed point, 2x2
+// box chroma). One thread per chroma sample; writes the 2x2 luma quad and one U, one V.
+__global__ void k_rgb_to_yuv420(const uint8_t* __restrict__ rgb, int dw, int dh, int sw, int sh,
+                                uint8_t* __restrict__ Y, uint8_t* __restrict__ U, uint8_t* __restrict__ V) {
+    int cw = sw >> 1;
+    int cx = blockIdx.x * blockDim.x + threadIdx.x, cy = blockIdx.y;
+    if (cx >= cw) return;
+    int R = 0, G = 0, B = 0;
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        int y = min(2 * cy + j, dh - 1);
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            int x = min(2 * cx + i, dw - 1);
+            const uint8_t* p = rgb + ((size_t)y * dw + x) * 3;
+            int r = p[0], g = p[1], b = p[2];
+            int yy = (2991 * r + 10064 * g + 1016 * b + (16 << 14) + (1 << 13)) >> 14;
+            Y[(size_t)(2 * cy + j) * sw + 2 * cx + i] = (uint8_t)dclamp(yy, 0, 255);
+            R += r; G += g; B += b;
+        }
+    }
+    int u = (-1649 * R - 5547 * G + 7196 * B + (128 << 16) + (1 << 15)) >> 16;
+    int v = (7196 * R - 6536 * G - 660 * B + (128 << 16) + (1 << 15)) >> 16;
+    U[(size_t)cy * cw + cx] = (uint8_t)dclamp(u, 0, 255);
+    V[(size_t)cy * cw + cx] = (uint8_t)dclamp(v, 0, 255);
+}
+
+// YUV420 -> RGB24 (BT.709 limited, nearest chroma) fused with crop_rgb24
+// (reitero_decode/src/decoder.rs:18-32): only the display area is produced.
+__global__ void k_yuv420_to_rgb(const uint8_t* __restrict__ Y, const uint8_t* __restrict__ U, const uint8_t* __restrict__ V,
+                                int sw, int dw, int dh, uint8_t* __restrict__ rgb) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= dw || y >= dh) return;
+    int cw = sw >> 1;
+    int yy = Y[(size_t)y * sw + x] - 16, uu = U[(size_t)(y >> 1) * cw + (x >> 1)] - 128, vv = V[(size_t)(y >> 1) * cw + (x >> 1)] - 128;
+    int r = (19077 * yy + 29372 * vv + (1 << 13)) >> 14;
+    int g = (19077 * yy - 3494 * uu - 8731 * vv + (1 << 13)) >> 14;
+    int b = (19077 * yy + 34610 * uu + (1 << 13)) >> 14;
+    uint8_t* p = rgb + ((size_t)y * dw + x) * 3;
+    p[0] = (uint8_t)dclamp(r, 0, 255); p[1] = (uint8_t)dclamp(g, 0, 255); p[2] = (uint8_t)dclamp(b, 0, 255);
+}
+
+// ============================================================================================
+// Motion search + MV mode decision wavefront
+// ============================================================================================
+// One workgroup per band of R MB rows, one wave64 per row. Block (bx, by) depends on the
+// left block and on (bx-1..bx+1, by-1) through the ME seeds / cost predictors
+// (motion/scalar.rs:56-156) and the mode-decision predictors (mv_predictor.rs:59-272), so
+// wave w processes block bx = s - 2w at step s; rows of one band synchronise with s_barrier,
+// the first row of a band waits on the previous band's last row through a release/acquire
+// progress counter in HBM (bounded spin: a timeout sets p.err and every wave exits).
+//
+// Each wave stages its block's replicate-padded reference window
+// ref[clamp(y0-sr-1 .. y0+16+sr)][clamp(x0-sr-1 .. x0+16+sr)] and its 16x16 current block in
+// LDS, one block ahead (double buffer), and evaluates every SAD of the hex/diamond/half-pel
+// walk on the fly: lane l owns current row l>>2, columns 4*(l&3)..+3, computes one packed
+// v_sad_u8 per candidate, and the 64 partial sums are reduced with DPP + readlane (two
+// candidates packed per 32-bit word; a 16x16 SAD is <= 65280 so the halves never carry).
+namespace {
+
+struct Cand { int n; int dx[6]; int dy[6]; };
+
+__device__ __forceinline__ int median3(int a, int b, int c) {   // motion/scalar.rs:24-43
+    if (a > b) { if (b > c) return b; else if (a > c) return c; else return a; }
+    else { if (a > c) return a; else if (b > c) return c; else return b; }
+}
+
+struct CostCtx {
+    int nx, ny, rx, ry, mx, my;
+    const double* bits;
+    double lam2;
+    int c_zero, c_pred, lam_zero;
+    // estimate_mv_cost (motion/scalar.rs:453-544); the min over the applicable terms
+    __device__ __forceinline__ int cost(int dx, int dy) const {
+        if (lam_zero) return 0;
+        long long best = 0x7fffffffffffffffll;
+        if ((dx == nx && dy == ny) || (dx == rx && dy == ry) || (dx == mx && dy == my)) best = c_pred;
+        if (dx == 0 && dy == 0) best = min(best, (long long)c_zero);
+        {
+            double t = __dadd_rn(__dadd_rn(6.0, bits[abs(dx - nx)]), bits[abs(dy - ny)]);
+            best = min(best, (long long)__dmul_rn(lam2, t));
+        }
+        {
+            double t = __dadd_rn(__dadd_rn(6.0, bits[abs(dx - rx)]), bits[abs(dy - ry)]);
+            best = min(best, (long long)__dmul_rn(lam2, t));
+        }
+        return (int)min(best, (long long)0x3fffffff);
+    }
+};
+
+// Integer SAD with the reference's per-4x4 edge handling (satd_scalar.rs:4-99: each 4x4 sub-
+// block's reference origin is clamped to [0,W-4] x [0,H-4] as a unit). Evaluates c.n <= 6
+// candidates; returns the SADs in out[].
+__device__ __forceinline__ void int_sads(const uint32_t* win32, int WSP, uint32_t cur4, int lane, int x0, int y0, int W, int H,
+                                         int sr, const Cand& c, int* out) {
+    const int g = lane & 3, j = lane >> 2;
+    const int sxo = x0 + 4 * g, syo = y0 + 4 * (j >> 2), jr = j & 3;
+    const int wx0 = x0 - sr - 1, wy0 = y0 - sr - 1;
+#pragma unroll
+    for (int k = 0; k < 6; k += 2) {
+        if (k < c.n) {
+            int ox = dclamp(sxo + c.dx[k], 0, W - 4) - wx0, oy = dclamp(syo + c.dy[k], 0, H - 4) - wy0 + jr;
+            uint32_t pk = sad4(cur4, lds_read4(win32, oy * WSP + ox), 0u);
+            if (k + 1 < c.n) {
+                int ox1 = dclamp(sxo + c.dx[k + 1], 0, W - 4) - wx0, oy1 = dclamp(syo + c.dy[k + 1], 0, H - 4) - wy0 + jr;
+                pk |= sad4(cur4, lds_read4(win32, oy1 * WSP + ox1), 0u) << 16;
+            }
+            uint32_t t = wave_sum(pk);
+            out[k] = (int)(t & 0xffff);
+            if (k + 1 < c.n) out[k + 1] = (int)(t >> 16);
+        }
+    }
+}
+
+// Half-pel reference samples for the lane's 4 pixels at half-pel offset (hx, hy), floor
+// interpolation (satd_scalar.rs:113-166). The per-pixel clamp of the reference's sampler
+// equals plain interpolation on the replicate-padded window (averaging replicated samples
+// returns the edge sample), so no clamping is needed here.
+__device__ __forceinline__ uint32_t hp_ref4(const uint32_t* win32, int WSP, int lane, int sr, int hx, int hy) {
+    const int g = lane & 3, j = lane >> 2;
+    int xs = 4 * g + (hx >> 1) + sr + 1, ys = j + (hy >> 1) + sr + 1;
+    int ax = hx & 1, ay = hy & 1;
+    uint32_t a = lds_read4(win32, ys * WSP + xs);
+    if (!ax && !ay) return a;
+    if (ax && !ay) return avg2f(a, lds_read4(win32, ys * WSP + xs + 1));
+    if (!ax && ay) return avg2f(a, lds_read4(win32, (ys + 1) * WSP + xs));
+    return avg4f(a, lds_read4(win32, ys * WSP + xs + 1), lds_read4(win32, (ys + 1) * WSP + xs), lds_read4(win32, (ys + 1) * WSP + xs + 1));
+}
+
+__device__ __forceinline__ bool q4eq(const int8_t* a, const int8_t* b) { return a[0] == b[0] && a[1] == b[1] && a[2] == b[2] && a[3] == b[3]; }
+__device__ __forceinline__ void q4cpy(int8_t* d, const int8_t* s) { d[0] = s[0]; d[1] = s[1]; d[2] = s[2]; d[3] = s[3]; }
+__device__ __forceinline__ int mv_class(int mag) {   // mv_class_from_magnitude (mv_rans.rs:81-87)
+    if (mag == 0) return 0;
+    return min(32 - __clz(mag), 10);
+}
+
+template <int NPF>
+__device__ __forceinline__ void load_window(const WalkParams& p, int bx, int by, int lane, int WS, uint32_t* pf, uint32_t& pcur) {
+    const int x0 = bx * 16, y0 = by * 16, wx0 = x0 - p.sr - 1, wy0 = y0 - p.sr - 1;
+    const int tot = WS * WS;
+#pragma unroll
+    for (int k = 0; k < NPF; k++) {
+        int e = lane + 64 * k;
+        if (e < tot) {
+            int wy = e / WS, wx = e - wy * WS;
+            int yy = dclamp(wy0 + wy, 0, p.H - 1), xx = dclamp(wx0 + wx, 0, p.W - 1);
+            pf[k] = p.ref[(size_t)yy * p.W + xx];
+        }
+    }
+    pcur = *(const uint32_t*)(p.cur + (size_t)(y0 + (lane >> 2)) * p.W + x0 + 4 * (lane & 3));
+}
+template <int NPF>
+__device__ __forceinline__ void store_window(uint8_t* win, uint8_t* curb, int lane, int WS, int WSP, const uint32_t* pf, uint32_t pcur) {
+    const int tot = WS * WS;
+#pragma unroll
+    for (int k = 0; k < NPF; k++) {
+        int e = lane + 64 * k;
+        if (e < tot) { int wy = e / WS, wx = e - wy * WS; win[wy * WSP + wx] = (uint8_t)pf[k]; }
+    }
+    ((uint32_t*)curb)[lane] = pcur;
+}
+
+}  // namespace
+
+template <int NPF>
+__global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int R = p.R, r0 = blockIdx.x * R, row = r0 + w, bw = p.bw, sr = p.sr;
+    const int WS = 16 + 2 * sr + 2, WSP = (WS + 4 + 3) & ~3, WINB = (WS * WSP + 15) & ~15;
+    uint8_t* wbase = lds + w * (2 * WINB + 512);
+    uint8_t* winb0 = wbase;
+    uint8_t* curb0 = wbase + 2 * WINB;
+    int8_t* rowbuf = (int8_t*)(lds + R * (2 * WINB + 512));
+    double* bits = (double*)((uint8_t*)rowbuf + ((R * bw * 8 + 15) & ~15));
+    int* abortf = (int*)(bits + 256);
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) bits[i] = p.bits[i];
+    if (threadIdx.x == 0) *abortf = 0;
+    const bool row_valid = row < p.bh;
+    uint32_t pf[NPF];
+    uint32_t pcur = 0;
+    if (row_valid) {
+        load_window<NPF>(p, 0, row, lane, WS, pf, pcur);
+        store_window<NPF>(winb0, curb0, lane, WS, WSP, pf, pcur);
+    }
+    __syncthreads();
+
+    int8_t lS[4] = {0, 0, 0, 0}, lR[4] = {0, 0, 0, 0};   // left neighbour (search, reconstructed)
+    const int nsteps = bw + 2 * (R - 1);
+    for (int s = 0; s < nsteps; s++) {
+        const int bx = s - 2 * w;
+        if (row_valid && bx >= 0 && bx < bw) {
+            const int idx = row * bw + bx, x0 = bx * 16, y0 = row * 16;
+            const bool has_l = bx > 0, has_t = row > 0, has_tr = has_t && bx + 1 < bw, has_tl = has_t && bx > 0;
+            // ---- top neighbours (search + reconstructed MVs) ----
+            int8_t tS[3][4] = {{0}}, tR[3][4] = {{0}};   // [0] = top-left, [1] = top, [2] = top-right
+            bool timed_out = false;
+            if (has_t) {
+                if (w == 0) {
+                    unsigned long long need = p.base + (unsigned long long)min(bx + 2, bw);
+                    unsigned spins = 0;
+                    while (__hip_atomic_load(&p.progress[row - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+                        if (++spins > (1u << 22)) { timed_out = true; break; }
+                        __builtin_amdgcn_s_sleep(2);
+                    }
+                    if (!timed_out) {
+#pragma unroll
+                        for (int k = 0; k < 3; k++) {
+                            int xx = bx - 1 + k;
+                            if (xx >= 0 && xx < bw) {
+                                const int8_t* q = (const int8_t*)&p.rec[(row - 1) * bw + xx];
+                                uint64_t v = *(const uint64_t*)q;
+#pragma unroll
+                                for (int c = 0; c < 4; c++) { tS[k][c] = (int8_t)(v >> (8 * c)); tR[k][c] = (int8_t)(v >> (32 + 8 * c)); }
+                            }
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 3; k++) {
+                        int xx = bx - 1 + k;
+                        if (xx >= 0 && xx < bw) {
+                            const int8_t* q = rowbuf + ((w - 1) * bw + xx) * 8;
+                            q4cpy(tS[k], q); q4cpy(tR[k], q + 4);
+                        }
+                    }
+                }
+            }
+            if (timed_out) {
+                if (lane == 0) { *abortf = 1; atomicExch(p.err, 1); }
+            } else {
+                int8_t pT[4] = {0, 0, 0, 0};
+                const bool has_tp = p.prev_mv != nullptr;
+                if (has_tp) { uint32_t v = *(const uint32_t*)(p.prev_mv + (size_t)p.prev_stride * idx); for (int c = 0; c < 4; c++) pT[c] = (int8_t)(v >> (8 * c)); }
+                // prefetch the next block's window while this block is walked
+                const bool pf_next = bx + 1 < bw;
+                if (pf_next) load_window<NPF>(p, bx + 1, row, lane, WS, pf, pcur);
+
+                const uint32_t* win32 = (const uint32_t*)(winb0 + (bx & 1) * WINB);
+                const uint32_t cur4 = ((const uint32_t*)(curb0 + (bx & 1) * 256))[lane];
+
+                // ---- ME predictors: gather_local_motion_candidates (motion/scalar.rs:56-156) ----
+                // slots in predictor order: left, top, top-right, top-left, temporal
+                const bool pv[5] = {has_l, has_t, has_tr, has_tl, has_tp};
+                const int pdx[5] = {lS[0], tS[1][0], tS[2][0], tS[0][0], pT[0]};
+                const int pdy[5] = {lS[1], tS[1][1], tS[2][1], tS[0][1], pT[1]};
+                int ux0 = 0, uy0 = 0, ux1 = 0, uy1 = 0, nu = 0;
+#pragma unroll
+                for (int i = 0; i < 5; i++) {
+                    if (pv[i]) {
+                        bool dup = (nu >= 1 && ux0 == pdx[i] && uy0 == pdy[i]) || (nu >= 2 && ux1 == pdx[i] && uy1 == pdy[i]);
+                        if (!dup) { if (nu == 0) { ux0 = pdx[i]; uy0 = pdy[i]; } else if (nu == 1) { ux1 = pdx[i]; uy1 = pdy[i]; } nu++; }
+                    }
+                }
+                CostCtx cc;
+                cc.nx = ux0; cc.ny = uy0; cc.rx = ux1; cc.ry = uy1;
+                {
+                    int lx = has_l ? lS[0] : 0, ly = has_l ? lS[1] : 0;
+                    int tx = has_t ? tS[1][0] : 0, ty = has_t ? tS[1][1] : 0;
+                    int trx = has_tr ? tS[2][0] : (has_tl ? tS[0][0] : 0), try_ = has_tr ? tS[2][1] : (has_tl ? tS[0][1] : 0);
+                    cc.mx = median3(lx, tx, trx); cc.my = median3(ly, ty, try_);
+                }
+                cc.bits = bits; cc.lam2 = p.lam2; cc.c_zero = p.c_zero; cc.c_pred = p.c_pred; cc.lam_zero = p.lam_zero;
+                Cand c;
+                c.n = 1; c.dx[0] = 0; c.dy[0] = 0;
+#pragma unroll
+                for (int i = 0; i < 5; i++) {   // push_candidate (motion/scalar.rs:46-53)
+                    if (pv[i] && abs(pdx[i]) <= sr && abs(pdy[i]) <= sr) {
+                        bool dup = false;
+#pragma unroll
+                        for (int q = 0; q < 6; q++) if (q < c.n) dup |= (c.dx[q] == pdx[i] && c.dy[q] == pdy[i]);
+                        if (!dup) {
+#pragma unroll
+                            for (int q = 1; q < 6; q++) if (q == c.n) { c.dx[q] = pdx[i]; c.dy[q] = pdy[i]; }
+                            c.n++;
+                        }
+                    }
+                }
+                // ---- process_block_hex (motion/scalar.rs:546-636) ----
+                int sads[6];
+                int_sads(win32, WSP, cur4, lane, x0, y0, p.W, p.H, sr, c, sads);
+                const int zero_sad = sads[0];
+                int sdx = 0, sdy = 0;
+                long long scost = (long long)zero_sad + cc.cost(0, 0);
+                bool skip_int = p.zthr > 0 && zero_sad <= p.zthr;
+                if (!skip_int) {   // evaluate_seed_candidates (:638-678)
+#pragma unroll
+                    for (int k = 1; k < 6; k++) {
+                        if (k < c.n) {
+                            int mc = cc.cost(c.dx[k], c.dy[k]);
+                            long long t = (long long)sads[k] + mc;
+                            if (scost - mc >= 0 && t < scost) { sdx = c.dx[k]; sdy = c.dy[k]; scost = t; }
+                        }
+                    }
+                    long long cur_sad = scost - cc.cost(sdx, sdy);
+                    if (p.pthr > 0 && cur_sad <= p.pthr) skip_int = true;
+                }
+                if (!skip_int) {   // refine_integer_motion (:680-757): large hexagon, then small diamond
+                    const int HX[6] = {2, 1, -1, -2, -1, 1}, HY[6] = {0, 2, 2, 0, -2, -2};
+                    for (int it = 0; it < 4096; it++) {
+                        Cand h; h.n = 0;
+                        const int cx = sdx, cy = sdy;
+#pragma unroll
+                        for (int k = 0; k < 6; k++) {
+                            int dx = cx + HX[k], dy = cy + HY[k];
+                            if (abs(dx) <= sr && abs(dy) <= sr) { h.dx[h.n] = dx; h.dy[h.n] = dy; h.n++; }
+                        }
+                        int hs[6];
+                        int_sads(win32, WSP, cur4, lane, x0, y0, p.W, p.H, sr, h, hs);
+                        bool improved = false;
+#pragma unroll
+                        for (int k = 0; k < 6; k++) {
+                            if (k < h.n) {
+                                int mc = cc.cost(h.dx[k], h.dy[k]);
+                                long long t = (long long)hs[k] + mc;
+                                if (scost - mc >= 0 && t < scost) { sdx = h.dx[k]; sdy = h.dy[k]; scost = t; improved = true; }
+                            }
+                        }
+                        if (!improved) break;
+                    }
+                    const int DX[4] = {1, -1, 0, 0}, DY[4] = {0, 0, 1, -1};
+                    Cand d; d.n = 0;
+                    const int cx = sdx, cy = sdy;
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        int dx = cx + DX[k], dy = cy + DY[k];
+                        if (abs(dx) <= sr && abs(dy) <= sr) { d.dx[d.n] = dx; d.dy[d.n] = dy; d.n++; }
+                    }
+                    int ds[6];
+                    int_sads(win32, WSP, cur4, lane, x0, y0, p.W, p.H, sr, d, ds);
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        if (k < d.n) {
+                            int mc = cc.cost(d.dx[k], d.dy[k]);
+                            long long t = (long long)ds[k] + mc;
+                            if (scost - mc >= 0 && t < scost) { sdx = d.dx[k]; sdy = d.dy[k]; scost = t; }
+                        }
+                    }
+                }
+                // ---- refine_halfpel (:759-839): base re-scored with the half-pel kernel, then
+                //      the 8 neighbours in (sy outer, sx inner) order, penalty c_pred ----
+                const int ordx[9] = {0, -1, 0, 1, -1, 1, -1, 0, 1}, ordy[9] = {0, -1, -1, -1, 0, 0, 1, 1, 1};
+                int hsad[9];
+#pragma unroll
+                for (int k = 0; k < 9; k += 2) {
+                    uint32_t pk = sad4(cur4, hp_ref4(win32, WSP, lane, sr, 2 * sdx + ordx[k], 2 * sdy + ordy[k]), 0u);
+                    if (k + 1 < 9) pk |= sad4(cur4, hp_ref4(win32, WSP, lane, sr, 2 * sdx + ordx[k + 1], 2 * sdy + ordy[k + 1]), 0u) << 16;
+                    uint32_t t = wave_sum(pk);
+                    hsad[k] = (int)(t & 0xffff);
+                    if (k + 1 < 9) hsad[k + 1] = (int)(t >> 16);
+                }
+                int best_hx = 2 * sdx, best_hy = 2 * sdy;
+                long long best_total = hsad[0];
+                int best_sad = hsad[0];
+#pragma unroll
+                for (int k = 1; k < 9; k++) {
+                    const int pen = p.c_pred;   // every neighbour of an even base is fractional
+                    if (best_total - pen >= 0) {
+                        long long t = (long long)hsad[k] + pen;
+                        if (t < best_total) { best_total = t; best_sad = hsad[k]; best_hx = 2 * sdx + ordx[k]; best_hy = 2 * sdy + ordy[k]; }
+                    }
+                }
+                // canonicalised search MV (motion/scalar.rs:619-634, motion_vector.rs:103-116)
+                int S0, S1, S2, S3;
+                {
+                    int hx = dclamp(best_hx, -255, 255), hy = dclamp(best_hy, -255, 255);
+                    int dx = dclamp(hx / 2, -128, 127), dy = dclamp(hy / 2, -128, 127), fx = hx % 2, fy = hy % 2;
+                    if (dx == -128 && fx == -1) fx = 0;
+                    if (dy == -128 && fy == -1) fy = 0;
+                    if (fx == -1) { fx = 1; dx = max(-128, dx - 1); }
+                    if (fy == -1) { fy = 1; dy = max(-128, dy - 1); }
+                    S0 = dx; S1 = dy; S2 = fx; S3 = fy;
+                }
+
+                // ---- mode decision on the reconstructed prefix (encoder.rs:447-733) ----
+                // derive_mv_predictors_with_stats / gather_mv_candidates (mv_predictor.rs:135-272):
+                // deduplicated (dx,dy,sx,sy) of left, top, top-right, top-left, temporal; the first
+                // two are Nearest / Near, padded with zero. MVs are packed as 32-bit words.
+                auto pack = [](int a, int b, int c, int d) -> uint32_t {
+                    return (uint32_t)(uint8_t)a | ((uint32_t)(uint8_t)b << 8) | ((uint32_t)(uint8_t)c << 16) | ((uint32_t)(uint8_t)d << 24);
+                };
+                auto pk8 = [&](const int8_t* q) -> uint32_t { return pack(q[0], q[1], q[2], q[3]); };
+                const uint32_t wL = pk8(lR), wT = pk8(tR[1]), wTR = pk8(tR[2]), wTL = pk8(tR[0]), wP = pk8(pT);
+                uint32_t c0 = 0, c1 = 0; int ncand = 0;
+                auto addc = [&](bool present, uint32_t q) {
+                    if (!present) return;
+                    if (ncand >= 1 && c0 == q) return;
+                    if (ncand >= 2 && c1 == q) return;
+                    if (ncand == 0) c0 = q; else if (ncand == 1) c1 = q;
+                    ncand++;   // later distinct entries only matter for the count
+                };
+                addc(has_l, wL); addc(has_t, wT); addc(has_tr, wTR); addc(has_tl, wTL); addc(has_tp, wP);
+                const uint32_t nearest = ncand > 0 ? c0 : 0u, nearp = ncand > 1 ? c1 : 0u, temporal = has_tp ? wP : 0u;
+                auto hpx = [](uint32_t q) { return (int)(int8_t)(q & 0xff) * 2 + (int)(int8_t)((q >> 16) & 0xff); };
+                auto hpy = [](uint32_t q) { return (int)(int8_t)((q >> 8) & 0xff) * 2 + (int)(int8_t)(q >> 24); };
+                const int mhx = S0 * 2 + S2, mhy = S1 * 2 + S3;
+                auto hpeq = [&](uint32_t q) { return mhx == hpx(q) && mhy == hpy(q); };
+                uint8_t mode;
+                if (S0 == 0 && S1 == 0 && S2 == 0 && S3 == 0) mode = riv::M_ZERO;
+                else if (hpeq(nearest)) mode = riv::M_NEAREST;
+                else if (hpeq(nearp)) mode = riv::M_NEAR;
+                else if (has_tr && hpeq(wTR)) mode = riv::M_TOPRIGHT;
+                else if (has_tl && hpeq(wTL)) mode = riv::M_TOPLEFT;
+                else if (hpeq(temporal)) mode = riv::M_TEMPORAL;
+                else mode = riv::M_NEW;
+                const uint32_t wS = pack(S0, S1, S2, S3);
+                uint32_t wR = wS;
+                uint8_t chosen = 0; int rdx = 0, rdy = 0;
+                if (mode == riv::M_NEW) {   // NEW base by estimated bits (encoder.rs:567-661, :128-159)
+                    const uint32_t bases[5] = {nearest, nearp, wTR, wTL, temporal};
+                    const bool have[5] = {true, true, has_tr, has_tl, true};
+                    const unsigned selbits[5] = {1, 2, 3, 4, 4};
+                    unsigned best = 0xffffffffu; int bbx = (int)(int8_t)(nearest & 0xff), bby = (int)(int8_t)((nearest >> 8) & 0xff);
+#pragma unroll
+                    for (int k = 0; k < 5; k++) {
+                        if (have[k]) {
+                            int kx = (int)(int8_t)(bases[k] & 0xff), ky = (int)(int8_t)((bases[k] >> 8) & 0xff);
+                            int ddx = dclamp(S0 - kx, -128, 127), ddy = dclamp(S1 - ky, -128, 127);
+                            int cxs = mv_class(abs(ddx)), cys = mv_class(abs(ddy));
+                            unsigned cost = selbits[k] + (cxs == 0 ? 4u : 4u + cxs) + (cys == 0 ? 4u : 4u + cys);
+                            if (cost < best) { best = cost; chosen = (uint8_t)k; bbx = kx; bby = ky; }
+                        }
+                    }
+                    rdx = dclamp(S0 - bbx, -128, 127);
+                    rdy = dclamp(S1 - bby, -128, 127);
+                    wR = pack(dclamp(bbx + rdx, -128, 127), dclamp(bby + rdy, -128, 127), S2, S3);
+                } else if (mode == riv::M_NEAREST) wR = nearest;
+                else if (mode == riv::M_NEAR) wR = nearp;
+                else if (mode == riv::M_TOPRIGHT) wR = wTR;
+                else if (mode == riv::M_TOPLEFT) wR = wTL;
+                else if (mode == riv::M_TEMPORAL) wR = temporal;
+                // RdoContext::decide via the host-built table (rdo.rs:140-178)
+                const bool aligned = S2 == 0 && S3 == 0;
+                uint32_t rdo = 0;
+                if (aligned && best_sad <= p.rdo_max) rdo = (p.rdo_bits[best_sad >> 5] >> (best_sad & 31)) & 1u;
+
+                if (lane == 0) {
+                    uint32_t w0 = wS, w1 = wR;
+                    uint32_t w2 = (uint32_t)mode | ((uint32_t)chosen << 8) | ((uint32_t)(uint8_t)rdx << 16) | ((uint32_t)(uint8_t)rdy << 24);
+                    uint32_t w3 = (uint32_t)best_sad | (rdo << 16);
+                    uint4* dst = (uint4*)&p.rec[idx];
+                    *dst = make_uint4(w0, w1, w2, w3);
+                    uint32_t* rb = (uint32_t*)(rowbuf + (w * bw + bx) * 8);
+                    rb[0] = w0; rb[1] = w1;
+                    __hip_atomic_store(&p.progress[row], p.base + (unsigned long long)(bx + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                lS[0] = (int8_t)S0; lS[1] = (int8_t)S1; lS[2] = (int8_t)S2; lS[3] = (int8_t)S3;
+                lR[0] = (int8_t)(wR & 0xff); lR[1] = (int8_t)((wR >> 8) & 0xff); lR[2] = (int8_t)((wR >> 16) & 0xff); lR[3] = (int8_t)(wR >> 24);
+                if (pf_next) store_window<NPF>(winb0 + ((bx + 1) & 1) * WINB, curb0 + ((bx + 1) & 1) * 256, lane, WS, WSP, pf, pcur);
+            }
+        }
+        __syncthreads();
+        if (*abortf) break;
+    }
+}
+
+// ============================================================================================
+// Motion compensation + prediction-variance sums
+// ============================================================================================
+// build_predicted (reitero_video_common/src/fast_motion.rs:103-352; samplers motion.rs:28-117):
+// rounding half-pel bilinear with per-pixel clamping; integer MVs whose luma block lies inside
+// the frame take the fast path, which copies chroma at floor((x0+dx)/2) and so drops the
+// chroma half-pel phase of odd integer MVs. One 256-thread workgroup per MB; also emits
+// (sum, sum of squares) of the predicted luma block for the adaptive quantiser.
+__device__ __forceinline__ uint8_t sample_hp_round(const uint8_t* p, int W, int H, int xh, int yh) {
+    xh = dclamp(xh, 0, (W - 1) * 2); yh = dclamp(yh, 0, (H - 1) * 2);
+    int x0 = xh >> 1, y0 = yh >> 1;
+    bool xo = xh & 1, yo = yh & 1;
+    int x1 = x0 + 1 < W ? x0 + 1 : x0, y1 = y0 + 1 < H ? y0 + 1 : y0;
+    int a = p[(size_t)y0 * W + x0];
+    if (!xo && !yo) return (uint8_t)a;
+    int b = p[(size_t)y0 * W + x1], d = p[(size_t)y1 * W + x0], e = p[(size_t)y1 * W + x1];
+    if (xo && !yo) return (uint8_t)((a + b + 1) >> 1);
+    if (!xo && yo) return (uint8_t)((a + d + 1) >> 1);
+    return (uint8_t)((a + b + d + e + 2) >> 2);
+}
+
+__global__ __launch_bounds__(256) void k_mc(const uint8_t* __restrict__ rY, const uint8_t* __restrict__ rU, const uint8_t* __restrict__ rV,
+                                            int W, int H, const int8_t* __restrict__ mvs, int mv_stride,
+                                            uint8_t* __restrict__ pY, uint8_t* __restrict__ pU, uint8_t* __restrict__ pV,
+                                            uint32_t* __restrict__ var_sums) {
+    __shared__ uint32_t red[8];
+    const int b = blockIdx.x, bw = W >> 4, bx = b % bw, by = b / bw, t = threadIdx.x;
+    const int cw = W >> 1, ch = H >> 1;
+    const int8_t* m = mvs + (size_t)b * mv_stride;
+    const int mvx = m[0] * 2 + m[2], mvy = m[1] * 2 + m[3];
+    const int x0 = bx * 16, y0 = by * 16;
+    const int x = x0 + (t & 15), y = y0 + (t >> 4);
+    bool fast = false;
+    int sx0 = 0, sy0 = 0;
+    if (((mvx | mvy) & 1) == 0) {
+        sx0 = x0 + mvx / 2; sy0 = y0 + mvy / 2;
+        fast = sx0 >= 0 && sy0 >= 0 && sx0 + 15 < W && sy0 + 15 < H;
+    }
+    uint8_t v;
+    if (fast) v = rY[(size_t)(sy0 + (t >> 4)) * W + sx0 + (t & 15)];
+    else v = sample_hp_round(rY, W, H, 2 * x + mvx, 2 * y + mvy);
+    pY[(size_t)y * W + x] = v;
+    if (t < 64) {
+        const int ci = t & 7, cj = t >> 3, cx = (x0 >> 1) + ci, cy = (y0 >> 1) + cj;
+        uint8_t u, vv;
+        if (fast) {
+            size_t o = (size_t)((sy0 >> 1) + cj) * cw + (sx0 >> 1) + ci;
+            u = rU[o]; vv = rV[o];
+        } else {
+            int cxh = (4 * cx + mvx) >> 1, cyh = (4 * cy + mvy) >> 1;
+            u = sample_hp_round(rU, cw, ch, cxh, cyh);
+            vv = sample_hp_round(rV, cw, ch, cxh, cyh);
+        }
+        pU[(size_t)cy * cw + cx] = u; pV[(size_t)cy * cw + cx] = vv;
+    }
+    if (var_sums) {
+        uint32_t s1 = wave_sum(v), s2 = wave_sum((uint32_t)v * v);
+        if ((t & 63) == 0) { red[(t >> 6) * 2] = s1; red[(t >> 6) * 2 + 1] = s2; }
+        __syncthreads();
+        if (t == 0) {
+            var_sums[2 * b] = red[0] + red[2] + red[4] + red[6];
+            var_sums[2 * b + 1] = red[1] + red[3] + red[5] + red[7];
+        }
+    }
+}
+
+// ============================================================================================
+// Adaptive quantisation (compute_aq_quant_steps, reitero_residual/src/residual.rs:116-167)
+// ============================================================================================
+// variance = ((sum_sq*256 - sum^2) as f32) / 65536, floored at 4; ln with glibc's logf;
+// sequential f32 sum in raster order (one lane: the rounding sequence is part of the result);
+// step = base * exp2f((0.8 * (ln - avg)) / 6). Single 1024-thread workgroup; `lnv` is NB floats
+// of scratch.
+__global__ __launch_bounds__(1024) void k_aq(const uint32_t* __restrict__ var_sums, int nb, float base, float* __restrict__ lnv,
+                                             float* __restrict__ steps) {
+    __shared__ float s_avg;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        uint64_t s = var_sums[2 * b], s2 = var_sums[2 * b + 1];
+        float var = (float)(s2 * 256ull - s * s) / 65536.0f;
+        var = fmaxf(var, 4.0f);
+        lnv[b] = glibc_logf(var);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float sum = 0.0f;
+        for (int b = 0; b < nb; b++) sum = __fadd_rn(sum, lnv[b]);
+        s_avg = __fdiv_rn(sum, (float)nb);
+    }
+    __syncthreads();
+    const float avg = s_avg;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        float off = __fmul_rn(0.8f, __fsub_rn(lnv[b], avg));
+        steps[b] = __fmul_rn(base, glibc_exp2f(__fdiv_rn(off, 6.0f)));
+    }
+}
+
+// elementwise logf/exp2f for parity tests of the transcendental restatements
+__global__ void k_logf_exp2f(const float* in, size_t n, int which, float* out) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = which == 0 ? glibc_logf(in[i]) : glibc_exp2f(in[i]);
+}
+
+// ============================================================================================
+// Residual transforms (reitero_dct/src/dct16.rs, dct8.rs, fast_dct8.rs)
+// ============================================================================================
+namespace {
+// FDCT: rows then columns, i64 accumulation, +2^17 >> 18 (dct16.rs:151-194, dct8.rs:32-54).
+// `res` is the 16x16 residual (row-major, int16 in LDS); coefficient (u,v) -> out[v*16+u].
+__device__ __forceinline__ void fdct16(const int16_t* res, int32_t* tmp, int32_t* out, int t) {
+    {
+        int y = t >> 4, u = t & 15; long long acc = 0;
+#pragma unroll
+        for (int n = 0; n < 16; n++) acc += (long long)res[y * 16 + n] * c_B16[n][u];
+        tmp[y * 16 + u] = (int32_t)((acc + (1ll << 17)) >> 18);
+    }
+    __syncthreads();
+    {
+        int v = t >> 4, u = t & 15; long long acc = 0;
+#pragma unroll
+        for (int y = 0; y < 16; y++) acc += (long long)tmp[y * 16 + u] * c_B16[y][v];
+        out[v * 16 + u] = (int32_t)((acc + (1ll << 17)) >> 18);
+    }
+}
+// 8x8 FDCT for one plane by 64 threads (tt = 0..63)
+__device__ __forceinline__ void fdct8_rows(const int16_t* res, int32_t* tmp, int tt) {
+    int y = tt >> 3, u = tt & 7; long long acc = 0;
+#pragma unroll
+    for (int n = 0; n < 8; n++) acc += (long long)res[y * 8 + n] * c_B8[n][u];
+    tmp[y * 8 + u] = (int32_t)((acc + (1ll << 17)) >> 18);
+}
+__device__ __forceinline__ int32_t fdct8_cols(const int32_t* tmp, int tt) {
+    int v = tt >> 3, u = tt & 7; long long acc = 0;
+#pragma unroll
+    for (int y = 0; y < 8; y++) acc += (long long)tmp[y * 8 + u] * c_B8[y][v];
+    return (int32_t)((acc + (1ll << 17)) >> 18);
+}
+// IDCT16: rows then columns, wrapping i32 (dct16.rs:201-233); d = dequantised coefficients.
+__device__ __forceinline__ int16_t idct16(const int32_t* d, int32_t* tmp, int t) {
+    {
+        int v = t >> 4, x = t & 15; int32_t acc = 0;
+#pragma unroll
+        for (int u = 0; u < 16; u++) acc = wadd(acc, wmul(d[v * 16 + u], c_B16[x][u]));
+        tmp[v * 16 + x] = wadd(acc, 1 << 17) >> 18;
+    }
+    __syncthreads();
+    int y = t >> 4, x = t & 15; int32_t acc = 0;
+#pragma unroll
+    for (int v = 0; v < 16; v++) acc = wadd(acc, wmul(tmp[v * 16 + x], c_B16[y][v]));
+    return (int16_t)dclamp(wadd(acc, 1 << 17) >> 18, -32768, 32767);
+}
+// AAN/stb 8x8 IDCT with the reference's F2F constants (fast_dct8.rs:123-174, :254-330)
+__device__ __forceinline__ void aan1d(const int32_t* s, int32_t xsc, int32_t* x, int32_t* tt) {
+    const int32_t F0541 = 2218, F1848 = -7572, F0765 = 3136, F1176 = 4817, F0299 = 1223, F2053 = 8411, F3073 = 12595,
+                  F1501 = 6150, F0900 = -3685, F2563 = -10496, F1962 = -8035, F0390 = -1598;
+    int32_t p2 = s[2], p3 = s[6];
+    int32_t p1 = wmul(wadd(p2, p3), F0541);
+    int32_t t2 = wadd(p1, wmul(p3, F1848)), t3 = wadd(p1, wmul(p2, F0765));
+    p2 = s[0]; p3 = s[4];
+    int32_t t0 = wshl(wadd(p2, p3), 12), t1 = wshl(wsub(p2, p3), 12);
+    x[0] = wadd(wadd(t0, t3), xsc); x[3] = wadd(wsub(t0, t3), xsc);
+    x[1] = wadd(wadd(t1, t2), xsc); x[2] = wadd(wsub(t1, t2), xsc);
+    int32_t a0 = s[7], a1 = s[5], a2 = s[3], a3 = s[1];
+    int32_t q3 = wadd(a0, a2), q4 = wadd(a1, a3), q1 = wadd(a0, a3), q2 = wadd(a1, a2);
+    int32_t p5 = wmul(wadd(q3, q4), F1176);
+    a0 = wmul(a0, F0299); a1 = wmul(a1, F2053); a2 = wmul(a2, F3073); a3 = wmul(a3, F1501);
+    q1 = wadd(p5, wmul(q1, F0900)); q2 = wadd(p5, wmul(q2, F2563));
+    q3 = wmul(q3, F1962); q4 = wmul(q4, F0390);
+    tt[3] = wadd(wadd(a3, q1), q4); tt[2] = wadd(wadd(a2, q2), q3); tt[1] = wadd(wadd(a1, q2), q4); tt[0] = wadd(wadd(a0, q1), q3);
+}
+// first pass over row r (horizontal frequencies) -> tmp[x*8 + r]
+__device__ __forceinline__ void aan_pass1(const int16_t* coef, const uint16_t* q, int r, int32_t* tmp) {
+    int32_t s[8], x[4], t[4];
+#pragma unroll
+    for (int u = 0; u < 8; u++) s[u] = wmul((int32_t)coef[r * 8 + u], (int32_t)q[r * 8 + u]);
+    aan1d(s, 512, x, t);
+    tmp[0 * 8 + r] = wadd(x[0], t[3]) >> 10; tmp[7 * 8 + r] = wsub(x[0], t[3]) >> 10;
+    tmp[1 * 8 + r] = wadd(x[1], t[2]) >> 10; tmp[6 * 8 + r] = wsub(x[1], t[2]) >> 10;
+    tmp[2 * 8 + r] = wadd(x[2], t[1]) >> 10; tmp[5 * 8 + r] = wsub(x[2], t[1]) >> 10;
+    tmp[3 * 8 + r] = wadd(x[3], t[0]) >> 10; tmp[4 * 8 + r] = wsub(x[3], t[0]) >> 10;
+}
+// second pass over column xc -> out[y*8 + xc]
+__device__ __forceinline__ void aan_pass2(const int32_t* tmp, int xc, int16_t* out) {
+    int32_t s[8], x[4], t[4];
+#pragma unroll
+    for (int r = 0; r < 8; r++) s[r] = tmp[xc * 8 + r];
+    aan1d(s, 65536, x, t);
+    int32_t o[8] = {wadd(x[0], t[3]) >> 17, wadd(x[1], t[2]) >> 17, wadd(x[2], t[1]) >> 17, wadd(x[3], t[0]) >> 17,
+                    wsub(x[3], t[0]) >> 17, wsub(x[2], t[1]) >> 17, wsub(x[1], t[2]) >> 17, wsub(x[0], t[3]) >> 17};
+#pragma unroll
+    for (int y = 0; y < 8; y++) out[y * 8 + xc] = (int16_t)dclamp(o[y], -32768, 32767);
+}
+__device__ __forceinline__ int16_t quant_aq(int32_t c, float qs, float dz) {   // dct16.rs:321-333
+    float r = __fdiv_rn((float)c, qs);
+    return fabsf(r) < dz ? 0 : f32_to_i16_sat(roundf(r));
+}
+__device__ __forceinline__ int16_t quant_matrix(int32_t c, uint16_t q) {   // dct16.rs:467-480
+    float qs = (float)(q < 1 ? 1 : q);
+    return f32_to_i16_sat(roundf(__fdiv_rn((float)c, qs)));
+}
+
+struct ResidLds {
+    int16_t res[384];       // Y 256 | U 64 | V 64 residual (or pixel-128 for intra)
+    int32_t tmp[256];
+    int32_t tmpc[128];
+    int32_t coef32[384];
+    int16_t coef[384];
+    int32_t deq[256];
+    int16_t rc[128];        // chroma residual reconstruction
+    uint16_t qc[64];
+};
+
+// dequant + IDCT16 + add/clamp on luma, AAN on chroma, for one MB whose int16 coefficients
+// are in L.coef; writes the reconstruction (pred + residual, clamped) to out planes.
+// Luma: (c*round(qs*16384) + 8192) >> 14 (dct16.rs:371-381); chroma: c*max(1, round(qs))
+// (fast_dct8.rs:95). With `intra`, the tables yt/ct replace the AQ step and +128 replaces pred.
+__device__ void recon_mb(ResidLds& L, int t, float qs, bool intra, const uint16_t* yt, const uint16_t* ct,
+                         const uint8_t* pY, const uint8_t* pU, const uint8_t* pV, uint8_t* oY, uint8_t* oU, uint8_t* oV,
+                         int W, int x0, int y0) {
+    const int cw = W >> 1;
+    if (intra) {
+        L.deq[t] = wmul((int32_t)L.coef[t], (int32_t)(yt[t] < 1 ? 1 : yt[t]));
+        if (t < 64) L.qc[t] = ct[t];
+    } else {
+        int32_t qfp = f32_to_i32_sat(roundf(__fmul_rn(qs, 16384.0f)));
+        L.deq[t] = wadd(wmul((int32_t)L.coef[t], qfp), 1 << 13) >> 14;
+        if (t < 64) L.qc[t] = f32_to_u16_sat(fmaxf(roundf(qs), 1.0f));
+    }
+    __syncthreads();
+    int16_t r = idct16(L.deq, L.tmp, t);
+    {
+        int yy = t >> 4, xx = t & 15;
+        size_t o = (size_t)(y0 + yy) * W + x0 + xx;
+        int base = intra ? 128 : pY[o];
+        oY[o] = (uint8_t)dclamp(base + (int)r, 0, 255);
+    }
+    if (t < 16) aan_pass1(L.coef + 256 + (t >> 3) * 64, L.qc, t & 7, L.tmpc + (t >> 3) * 64);
+    __syncthreads();
+    if (t < 16) aan_pass2(L.tmpc + (t >> 3) * 64, t & 7, L.rc + (t >> 3) * 64);
+    __syncthreads();
+    if (t < 128) {
+        int pl = t >> 6, tt = t & 63, yy = tt >> 3, xx = tt & 7;
+        size_t o = (size_t)((y0 >> 1) + yy) * cw + (x0 >> 1) + xx;
+        const uint8_t* ps = pl ? pV : pU;
+        uint8_t* od = pl ? oV : oU;
+        int base = intra ? 128 : ps[o];
+        od[o] = (uint8_t)dclamp(base + (int)L.rc[pl * 64 + tt], 0, 255);
+    }
+}
+
+// FDCT of L.res (luma 16x16 + two chroma 8x8) into L.coef32
+__device__ void fdct_mb(ResidLds& L, int t) {
+    fdct16(L.res, L.tmp, L.coef32, t);
+    if (t < 128) fdct8_rows(L.res + 256 + (t >> 6) * 64, L.tmpc + (t >> 6) * 64, t & 63);
+    __syncthreads();
+    if (t < 128) L.coef32[256 + t] = fdct8_cols(L.tmpc + (t >> 6) * 64, t & 63);
+    __syncthreads();
+}
+}  // namespace
+
+// P-frame residual: ResidualEncoder::encode_inter (reitero_residual/src/residual.rs:396-809)
+// minus the rANS: residual = cur - pred for MBs the RDO did not skip, FDCT, dead-zone AQ
+// quantisation, all-zero -> optimised skip, else dequant + IDCT reconstruction. Writes the
+// pre-deblock reconstruction, the coefficients of coded MBs and the optimised skip flags.
+__global__ __launch_bounds__(256) void k_inter_residual(const uint8_t* cY, const uint8_t* cU, const uint8_t* cV,
+                                                        const uint8_t* pY, const uint8_t* pU, const uint8_t* pV,
+                                                        int W, const BlockRec* rec, const float* steps, float dz,
+                                                        uint8_t* oY, uint8_t* oU, uint8_t* oV, int16_t* coeffs, uint8_t* opt_skip) {
+    __shared__ ResidLds L;
+    const int b = blockIdx.x, bw = W >> 4, bx = b % bw, by = b / bw, t = threadIdx.x, cw = W >> 1;
+    const int x0 = bx * 16, y0 = by * 16;
+    if (rec[b].rdo_skip) {   // RDO skip: prediction only
+        size_t o = (size_t)(y0 + (t >> 4)) * W + x0 + (t & 15);
+        oY[o] = pY[o];
+        if (t < 128) {
+            int pl = t >> 6, tt = t & 63;
+            size_t oc = (size_t)((y0 >> 1) + (tt >> 3)) * cw + (x0 >> 1) + (tt & 7);
+            if (pl) oV[oc] = pV[oc]; else oU[oc] = pU[oc];
+        }
+        if (t == 0) opt_skip[b] = 1;
+        return;
+    }
+    {
+        size_t o = (size_t)(y0 + (t >> 4)) * W + x0 + (t & 15);
+        L.res[t] = (int16_t)((int)cY[o] - (int)pY[o]);
+        if (t < 128) {
+            int pl = t >> 6, tt = t & 63;
+            size_t oc = (size_t)((y0 >> 1) + (tt >> 3)) * cw + (x0 >> 1) + (tt & 7);
+            L.res[256 + t] = (int16_t)(pl ? (int)cV[oc] - (int)pV[oc] : (int)cU[oc] - (int)pU[oc]);
+        }
+    }
+    __syncthreads();
+    fdct_mb(L, t);
+    const float qs = steps[b];
+    int nz = 0;
+    for (int i = t; i < 384; i += 256) { int16_t q = quant_aq(L.coef32[i], qs, dz); L.coef[i] = q; nz |= q != 0; }
+    nz = __syncthreads_or(nz);
+    if (!nz) {   // optimised skip (residual.rs:618-674)
+        size_t o = (size_t)(y0 + (t >> 4)) * W + x0 + (t & 15);
+        oY[o] = pY[o];
+        if (t < 128) {
+            int pl = t >> 6, tt = t & 63;
+            size_t oc = (size_t)((y0 >> 1) + (tt >> 3)) * cw + (x0 >> 1) + (tt & 7);
+            if (pl) oV[oc] = pV[oc]; else oU[oc] = pU[oc];
+        }
+        if (t == 0) opt_skip[b] = 1;
+        return;
+    }
+    if (t == 0) opt_skip[b] = 0;
+    for (int i = t; i < 384; i += 256) coeffs[(size_t)b * 384 + i] = L.coef[i];
+    recon_mb(L, t, qs, false, nullptr, nullptr, pY, pU, pV, oY, oU, oV, W, x0, y0);
+}
+
+// I-frame: ResidualEncoder::encode_intra (residual.rs:267-388) minus the rANS: pixel - 128,
+// FDCT, per-coefficient matrix quantisation (no dead zone), integer-table dequant + IDCT, +128.
+__global__ __launch_bounds__(256) void k_intra(const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, int W,
+                                               const uint16_t* yt, const uint16_t* ct, uint8_t* oY, uint8_t* oU, uint8_t* oV,
+                                               int16_t* coeffs) {
+    __shared__ ResidLds L;
+    __shared__ uint16_t syt[256], sct[64];
+    const int b = blockIdx.x, bw = W >> 4, bx = b % bw, by = b / bw, t = threadIdx.x, cw = W >> 1;
+    const int x0 = bx * 16, y0 = by * 16;
+    syt[t] = yt[t];
+    if (t < 64) sct[t] = ct[t];
+    {
+        size_t o = (size_t)(y0 + (t >> 4)) * W + x0 + (t & 15);
+        L.res[t] = (int16_t)((int)cY[o] - 128);
+        if (t < 128) {
+            int pl = t >> 6, tt = t & 63;
+            size_t oc = (size_t)((y0 >> 1) + (tt >> 3)) * cw + (x0 >> 1) + (tt & 7);
+            L.res[256 + t] = (int16_t)((pl ? (int)cV[oc] : (int)cU[oc]) - 128);
+        }
+    }
+    __syncthreads();
+    fdct_mb(L, t);
+    for (int i = t; i < 384; i += 256) {
+        int16_t q = quant_matrix(L.coef32[i], i < 256 ? syt[i] : sct[(i - 256) & 63]);
+        L.coef[i] = q;
+        coeffs[(size_t)b * 384 + i] = q;
+    }
+    __syncthreads();
+    recon_mb(L, t, 0.0f, true, syt, sct, nullptr, nullptr, nullptr, oY, oU, oV, W, x0, y0);
+}
+
+// Decoder P-frame reconstruction: ResidualDecoder::decode_inter (residual.rs:889-1079) after
+// the rANS: skipped MBs keep the prediction, coded MBs add dequant + IDCT of their coefficients.
+__global__ __launch_bounds__(256) void k_inter_recon(const uint8_t* pY, const uint8_t* pU, const uint8_t* pV, int W,
+                                                     const uint8_t* skip, const float* steps, const int16_t* coeffs,
+                                                     uint8_t* oY, uint8_t* oU, uint8_t* oV) {
+    __shared__ ResidLds L;
+    const int b = blockIdx.x, bw = W >> 4, bx = b % bw, by = b / bw, t = threadIdx.x, cw = W >> 1;
+    const int x0 = bx * 16, y0 = by * 16;
+    if (skip[b]) {
+        size_t o = (size_t)(y0 + (t >> 4)) * W + x0 + (t & 15);
+        oY[o] = pY[o];
+        if (t < 128) {
+            int pl = t >> 6, tt = t & 63;
+            size_t oc = (size_t)((y0 >> 1) + (tt >> 3)) * cw + (x0 >> 1) + (tt & 7);
+            if (pl) oV[oc] = pV[oc]; else oU[oc] = pU[oc];
+        }
+        return;
+    }
+    for (int i = t; i < 384; i += 256) L.coef[i] = coeffs[(size_t)b * 384 + i];
+    __syncthreads();
+    recon_mb(L, t, steps[b], false, nullptr, nullptr, pY, pU, pV, oY, oU, oV, W, x0, y0);
+}
+
+// Decoder I-frame reconstruction (residual.rs:815-887 after the rANS)
+__global__ __launch_bounds__(256) void k_intra_recon(int W, const uint16_t* yt, const uint16_t* ct, const int16_t* coeffs,
+                                                     uint8_t* oY, uint8_t* oU, uint8_t* oV) {
+    __shared__ ResidLds L;
+    __shared__ uint16_t syt[256], sct[64];
+    const int b = blockIdx.x, bw = W >> 4, bx = b % bw, by = b / bw, t = threadIdx.x;
+    syt[t] = yt[t];
+    if (t < 64) sct[t] = ct[t];
+    for (int i = t; i < 384; i += 256) L.coef[i] = coeffs[(size_t)b * 384 + i];
+    __syncthreads();
+    recon_mb(L, t, 0.0f, true, syt, sct, nullptr, nullptr, nullptr, oY, oU, oV, W, bx * 16, by * 16);
+}
+
+// Plane-level FDCT/quant/IDCT for the reitero_dct seams (one MB-sized block per workgroup;
+// `n8` selects the 8x8 path, which processes one 8x8 block per workgroup).
+__global__ __launch_bounds__(256) void k_dct_plane(const int16_t* plane, int w, int n8, const float* qs, float dz, int16_t* coeffs, int16_t* recon) {
+    __shared__ ResidLds L;
+    const int b = blockIdx.x, t = threadIdx.x;
+    if (!n8) {
+        const int bw = w / 16, bx = b % bw, by = b / bw;
+        L.res[t] = plane[(size_t)(by * 16 + (t >> 4)) * w + bx * 16 + (t & 15)];
+        __syncthreads();
+        fdct16(L.res, L.tmp, L.coef32, t);
+        __syncthreads();
+        int16_t q = quant_aq(L.coef32[t], qs[b], dz);
+        L.coef[t] = q; coeffs[(size_t)b * 256 + t] = q;
+        int32_t qfp = f32_to_i32_sat(roundf(__fmul_rn(qs[b], 16384.0f)));
+        L.deq[t] = wadd(wmul((int32_t)q, qfp), 1 << 13) >> 14;
+        __syncthreads();
+        int16_t r = idct16(L.deq, L.tmp, t);
+        recon[(size_t)(by * 16 + (t >> 4)) * w + bx * 16 + (t & 15)] = r;
+    } else {
+        const int bw = w / 8, bx = b % bw, by = b / bw;
+        if (t < 64) L.res[t] = plane[(size_t)(by * 8 + (t >> 3)) * w + bx * 8 + (t & 7)];
+        __syncthreads();
+        if (t < 64) fdct8_rows(L.res, L.tmpc, t);
+        __syncthreads();
+        if (t < 64) {
+            int16_t q = quant_aq(fdct8_cols(L.tmpc, t), qs[b], dz);
+            L.coef[t] = q; coeffs[(size_t)b * 64 + t] = q;
+            L.qc[t] = f32_to_u16_sat(fmaxf(roundf(qs[b]), 1.0f));
+        }
+        __syncthreads();
+        if (t < 8) aan_pass1(L.coef, L.qc, t, L.tmpc);
+        __syncthreads();
+        if (t < 8) aan_pass2(L.tmpc, t, L.rc);
+        __syncthreads();
+        if (t < 64) recon[(size_t)(by * 8 + (t >> 3)) * w + bx * 8 + (t & 7)] = L.rc[t];
+    }
+}
+
+// ============================================================================================
+// In-loop deblock (reitero_video_common/src/deblock.rs): VP8 normal macroblock-edge filter,
+// all vertical edges of a plane, then all horizontal ones. Edges are 16 (8 for chroma) pixels
+// apart and a filter touches 4 pixels on each side, so every edge line of one pass is
+// independent: one thread per (edge, line). Y, U and V run in the same launch.
+// ============================================================================================
+__device__ __forceinline__ int c8(int x) { return dclamp(x, -128, 127); }
+__device__ __forceinline__ void mb_filter_line(int* px, int E, int I, int HEV) {   // deblock.rs:68-115
+    int p3 = px[0], p2 = px[1], p1 = px[2], p0 = px[3], q0 = px[4], q1 = px[5], q2 = px[6], q3 = px[7];
+    bool mask = 2 * abs(p0 - q0) + abs(q1 - p1) / 2 <= E && abs(p3 - p2) <= I && abs(p2 - p1) <= I && abs(p1 - p0) <= I &&
+                abs(q1 - q0) <= I && abs(q2 - q1) <= I && abs(q3 - q2) <= I;
+    if (!mask) return;
+    int ps2 = p2 - 128, ps1 = p1 - 128, ps0 = p0 - 128, qs0 = q0 - 128, qs1 = q1 - 128, qs2 = q2 - 128;
+    bool hev = abs(p1 - p0) > HEV || abs(q1 - q0) > HEV;
+    int w = c8(c8(ps1 - qs1) + 3 * (qs0 - ps0));
+    if (hev) {
+        int f1 = c8(w + 4) >> 3, f2 = c8(w + 3) >> 3;
+        px[4] = dclamp(c8(qs0 - f1) + 128, 0, 255);
+        px[3] = dclamp(c8(ps0 + f2) + 128, 0, 255);
+    } else {
+        int a = c8((27 * w + 63) >> 7);
+        int nq0 = c8(qs0 - a), np0 = c8(ps0 + a);
+        a = c8((18 * w + 63) >> 7);
+        int nq1 = c8(qs1 - a), np1 = c8(ps1 + a);
+        a = c8((9 * w + 63) >> 7);
+        int nq2 = c8(qs2 - a), np2 = c8(ps2 + a);
+        px[1] = dclamp(np2 + 128, 0, 255); px[2] = dclamp(np1 + 128, 0, 255); px[3] = dclamp(np0 + 128, 0, 255);
+        px[4] = dclamp(nq0 + 128, 0, 255); px[5] = dclamp(nq1 + 128, 0, 255); px[6] = dclamp(nq2 + 128, 0, 255);
+    }
+}
+// mask: one byte per MB (1 = filter edges touching it), nullptr = every edge (intra)
+__global__ void k_deblock(uint8_t* Y, uint8_t* U, uint8_t* V, int W, int H, int E, int I, int HEV, const uint8_t* mask, int horizontal) {
+    const int bw = W >> 4, bh = H >> 4;
+    const long long nY = horizontal ? (long long)(bh - 1) * W : (long long)(bw - 1) * H;
+    const long long nC = horizontal ? (long long)(bh - 1) * (W >> 1) : (long long)(bw - 1) * (H >> 1);
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    uint8_t* P; int w, grid;
+    if (i < nY) { P = Y; w = W; grid = 16; }
+    else if (i < nY + nC) { P = U; w = W >> 1; grid = 8; i -= nY; }
+    else if (i < nY + 2 * nC) { P = V; w = W >> 1; grid = 8; i -= nY + nC; }
+    else return;
+    int px[8];
+    if (!horizontal) {
+        int h = grid == 16 ? H : H >> 1;
+        int e = (int)(i / h) + 1, y = (int)(i % h);
+        int x = e * grid, by = y / grid;
+        if (mask && !(mask[by * bw + e - 1] || mask[by * bw + e])) return;
+        uint8_t* p = P + (size_t)y * w + x - 4;
+#pragma unroll
+        for (int k = 0; k < 8; k++) px[k] = p[k];
+        mb_filter_line(px, E, I, HEV);
+#pragma unroll
+        for (int k = 0; k < 8; k++) p[k] = (uint8_t)px[k];
+    } else {
+        int e = (int)(i / w) + 1, x = (int)(i % w);
+        int y = e * grid, bxx = x / grid;
+        if (mask && !(mask[(e - 1) * bw + bxx] || mask[e * bw + bxx])) return;
+        uint8_t* p = P + (size_t)(y - 4) * w + x;
+#pragma unroll
+        for (int k = 0; k < 8; k++) px[k] = p[(size_t)k * w];
+        mb_filter_line(px, E, I, HEV);
+#pragma unroll
+        for (int k = 0; k < 8; k++) p[(size_t)k * w] = (uint8_t)px[k];
+    }
+}
+
+// Deblock mask of a P-frame (encoder.rs:831-845): !optimised_skip || reconstructed MV != 0
+__global__ void k_filter_mask(const BlockRec* rec, const uint8_t* opt_skip, int nb, uint8_t* mask) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const int8_t* r = rec[b].r;
+    mask[b] = !opt_skip[b] || r[0] != 0 || r[1] != 0 || r[2] != 0 || r[3] != 0;
+}
+
+// ============================================================================================
+// Host-side launch wrappers
+// ============================================================================================
+namespace rivk {
+
+hipError_t upload_bases(const int32_t* b16, const int32_t* b8) {
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_B16), b16, sizeof(int32_t) * 256);
+    if (e != hipSuccess) return e;
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_B8), b8, sizeof(int32_t) * 64);
+}
+
+void rgb_to_yuv420(hipStream_t st, const uint8_t* rgb, int dw, int dh, int sw, int sh, uint8_t* Y, uint8_t* U, uint8_t* V) {
+    dim3 g((sw / 2 + 127) / 128, sh / 2);
+    hipLaunchKernelGGL(k_rgb_to_yuv420, g, dim3(128), 0, st, rgb, dw, dh, sw, sh, Y, U, V);
+}
+void yuv420_to_rgb(hipStream_t st, const uint8_t* Y, const uint8_t* U, const uint8_t* V, int sw, int dw, int dh, uint8_t* rgb) {
+    dim3 g((dw + 255) / 256, dh);
+    hipLaunchKernelGGL(k_yuv420_to_rgb, g, dim3(256), 0, st, Y, U, V, sw, dw, dh, rgb);
+}
+
+size_t walk_lds_bytes(int sr, int bw, int R) {
+    int WS = 16 + 2 * sr + 2, WSP = (WS + 4 + 3) & ~3, WINB = (WS * WSP + 15) & ~15;
+    return (size_t)R * (2 * WINB + 512) + (size_t)((R * bw * 8 + 15) & ~15) + 256 * sizeof(double) + 16;
+}
+int walk_npf(int sr) {
+    int WS = 16 + 2 * sr + 2;
+    return (WS * WS + 63) / 64;
+}
+hipError_t me_walk(hipStream_t st, const WalkParams& p) {
+    int nbands = (p.bh + p.R - 1) / p.R;
+    size_t lds = walk_lds_bytes(p.sr, p.bw, p.R);
+    int npf = walk_npf(p.sr);
+    if (npf <= 40) hipLaunchKernelGGL(k_me_walk<40>, dim3(nbands), dim3(64 * p.R), lds, st, p);
+    else if (npf <= 106) hipLaunchKernelGGL(k_me_walk<106>, dim3(nbands), dim3(64 * p.R), lds, st, p);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+void mc(hipStream_t st, const uint8_t* rY, const uint8_t* rU, const uint8_t* rV, int W, int H, const int8_t* mvs, int mv_stride,
+        uint8_t* pY, uint8_t* pU, uint8_t* pV, uint32_t* var_sums) {
+    hipLaunchKernelGGL(k_mc, dim3((W / 16) * (H / 16)), dim3(256), 0, st, rY, rU, rV, W, H, mvs, mv_stride, pY, pU, pV, var_sums);
+}
+void aq(hipStream_t st, const uint32_t* var_sums, int nb, float base, float* lnv, float* steps) {
+    hipLaunchKernelGGL(k_aq, dim3(1), dim3(1024), 0, st, var_sums, nb, base, lnv, steps);
+}
+void logf_exp2f(hipStream_t st, const float* in, size_t n, int which, float* out) {
+    hipLaunchKernelGGL(k_logf_exp2f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, n, which, out);
+}
+void inter_residual(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, const uint8_t* pY, const uint8_t* pU,
+                    const uint8_t* pV, int W, int H, const BlockRec* rec, const float* steps, float dz, uint8_t* oY, uint8_t* oU,
+                    uint8_t* oV, int16_t* coeffs, uint8_t* opt_skip) {
+    hipLaunchKernelGGL(k_inter_residual, dim3((W / 16) * (H / 16)), dim3(256), 0, st, cY, cU, cV, pY, pU, pV, W, rec, steps, dz, oY, oU, oV,
+                       coeffs, opt_skip);
+}
+void intra(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, int W, int H, const uint16_t* yt, const uint16_t* ct,
+           uint8_t* oY, uint8_t* oU, uint8_t* oV, int16_t* coeffs) {
+    hipLaunchKernelGGL(k_intra, dim3((W / 16) * (H / 16)), dim3(256), 0, st, cY, cU, cV, W, yt, ct, oY, oU, oV, coeffs);
+}
+void inter_recon(hipStream_t st, const uint8_t* pY, const uint8_t* pU, const uint8_t* pV, int W, int H, const uint8_t* skip,
+                 const float* steps, const int16_t* coeffs, uint8_t* oY, uint8_t* oU, uint8_t* oV) {
+    hipLaunchKernelGGL(k_inter_recon, dim3((W / 16) * (H / 16)), dim3(256), 0, st, pY, pU, pV, W, skip, steps, coeffs, oY, oU, oV);
+}
+void intra_recon(hipStream_t st, int W, int H, const uint16_t* yt, const uint16_t* ct, const int16_t* coeffs, uint8_t* oY, uint8_t* oU,
+                 uint8_t* oV) {
+    hipLaunchKernelGGL(k_intra_recon, dim3((W / 16) * (H / 16)), dim3(256), 0, st, W, yt, ct, coeffs, oY, oU, oV);
+}
+void dct_plane(hipStream_t st, const int16_t* plane, int w, int h, int n8, const float* qs, float dz, int16_t* coeffs, int16_t* recon) {
+    int nb = n8 ? (w / 8) * (h / 8) : (w / 16) * (h / 16);
+    hipLaunchKernelGGL(k_dct_plane, dim3(nb), dim3(256), 0, st, plane, w, n8, qs, dz, coeffs, recon);
+}
+void deblock(hipStream_t st, uint8_t* Y, uint8_t* U, uint8_t* V, int W, int H, int level, bool intra, const uint8_t* mask) {
+    if (level <= 0) return;
+    // thresholds (deblock.rs:40-58)
+    int I = level > 1 ? level : 1, E = (level + 2) * 2 + I, HEV;
+    if (intra) HEV = level >= 40 ? 2 : (level >= 15 ? 1 : 0);
+    else HEV = level >= 40 ? 3 : (level >= 20 ? 2 : (level >= 15 ? 1 : 0));
+    int bw = W / 16, bh = H / 16;
+    long long nv = (long long)(bw - 1) * H + 2ll * (bw - 1) * (H / 2);
+    long long nh = (long long)(bh - 1) * W + 2ll * (bh - 1) * (W / 2);
+    if (nv > 0) hipLaunchKernelGGL(k_deblock, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st, Y, U, V, W, H, E, I, HEV, mask, 0);
+    if (nh > 0) hipLaunchKernelGGL(k_deblock, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, st, Y, U, V, W, H, E, I, HEV, mask, 1);
+}
+void filter_mask(hipStream_t st, const BlockRec* rec, const uint8_t* opt_skip, int nb, uint8_t* mask) {
+    hipLaunchKernelGGL(k_filter_mask, dim3((nb + 255) / 256), dim3(256), 0, st, rec, opt_skip, nb, mask);
+}
+
+}  // namespace rivk

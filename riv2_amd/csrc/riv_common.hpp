@@ -1,0 +1,45 @@
+// Structures shared by the host driver and the gfx950 kernels.
+#pragma once
+#include <cstdint>
+
+namespace riv {
+
+// MV prediction modes (reitero_residual/src/mv_predictor.rs:5-33, as_u8 order)
+enum Mode : uint8_t { M_ZERO = 0, M_NEAREST = 1, M_NEAR = 2, M_TOPRIGHT = 3, M_TOPLEFT = 4, M_NEW = 5, M_TEMPORAL = 6 };
+
+// One record per 16x16 macroblock, written by the motion-search / mode-decision wavefront.
+// The first 8 bytes are what the next MB row (and the next frame) read.
+struct BlockRec {
+    int8_t s[4];        // search MV (dx, dy, sx, sy), canonical (motion/scalar.rs:619-634)
+    int8_t r[4];        // reconstructed MV as the decoder rebuilds it (encoder.rs:667-718)
+    uint8_t mode;       // Mode
+    uint8_t new_base;   // 0 nearest, 1 near, 2 top-right, 3 top-left, 4 temporal
+    int8_t raw_dx, raw_dy;   // NEW delta before the frame bias (encoder.rs:651-660)
+    uint16_t sad;       // best half-pel SAD (motion/scalar.rs:759-839)
+    uint8_t rdo_skip;   // RdoContext::decide (rdo.rs:140-178)
+    uint8_t flags;
+};
+static_assert(sizeof(BlockRec) == 16, "BlockRec layout");
+
+// Parameters of the motion-search + mode-decision wavefront for one frame.
+struct WalkParams {
+    const uint8_t* cur;          // current luma, W x H
+    const uint8_t* ref;          // reference luma (previous reconstruction), W x H
+    int W, H, bw, bh, sr;
+    const int8_t* prev_mv;       // previous P-frame's reconstructed MVs (dx,dy,sx,sy) or nullptr
+    int prev_stride;             // bytes between consecutive MVs of prev_mv
+    BlockRec* rec;               // bw*bh records out
+    unsigned long long* progress;   // per MB row, monotonic: base + number of finished blocks
+    unsigned long long base;
+    const double* bits;          // 256 entries: 1.0 for 0, else 2*log2(a)+2 (motion/scalar.rs:495-525)
+    double lam2;                 // lambda * 2.0
+    int c_zero, c_pred;          // (lam2*0.5) as i64, (lam2*1.5) as i64; c_pred is also the half-pel penalty
+    int lam_zero;                // lambda == 0.0 -> every MV cost is 0
+    const uint32_t* rdo_bits;    // bit s set -> RDO chooses skip at SAD s (integer-aligned MV)
+    int rdo_max;                 // largest SAD that can skip, -1 when skipping is disabled
+    long long zthr, pthr;        // me_zero_mv_threshold, me_predictor_threshold
+    int R;                       // MB rows per workgroup
+    int* err;                    // set non-zero on a wavefront timeout
+};
+
+}  // namespace riv

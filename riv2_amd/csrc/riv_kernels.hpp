@@ -1,0 +1,31 @@
+// Host-side launch wrappers of the gfx950 kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include "riv_common.hpp"
+
+namespace rivk {
+hipError_t upload_bases(const int32_t* b16, const int32_t* b8);
+void rgb_to_yuv420(hipStream_t st, const uint8_t* rgb, int dw, int dh, int sw, int sh, uint8_t* Y, uint8_t* U, uint8_t* V);
+void yuv420_to_rgb(hipStream_t st, const uint8_t* Y, const uint8_t* U, const uint8_t* V, int sw, int dw, int dh, uint8_t* rgb);
+size_t walk_lds_bytes(int sr, int bw, int R);
+int walk_npf(int sr);
+hipError_t me_walk(hipStream_t st, const riv::WalkParams& p);
+void mc(hipStream_t st, const uint8_t* rY, const uint8_t* rU, const uint8_t* rV, int W, int H, const int8_t* mvs, int mv_stride,
+        uint8_t* pY, uint8_t* pU, uint8_t* pV, uint32_t* var_sums);
+void aq(hipStream_t st, const uint32_t* var_sums, int nb, float base, float* lnv, float* steps);
+void logf_exp2f(hipStream_t st, const float* in, size_t n, int which, float* out);
+void inter_residual(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, const uint8_t* pY, const uint8_t* pU,
+                    const uint8_t* pV, int W, int H, const riv::BlockRec* rec, const float* steps, float dz, uint8_t* oY, uint8_t* oU,
+                    uint8_t* oV, int16_t* coeffs, uint8_t* opt_skip);
+void intra(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, int W, int H, const uint16_t* yt, const uint16_t* ct,
+           uint8_t* oY, uint8_t* oU, uint8_t* oV, int16_t* coeffs);
+void inter_recon(hipStream_t st, const uint8_t* pY, const uint8_t* pU, const uint8_t* pV, int W, int H, const uint8_t* skip,
+                 const float* steps, const int16_t* coeffs, uint8_t* oY, uint8_t* oU, uint8_t* oV);
+void intra_recon(hipStream_t st, int W, int H, const uint16_t* yt, const uint16_t* ct, const int16_t* coeffs, uint8_t* oY, uint8_t* oU,
+                 uint8_t* oV);
+void dct_plane(hipStream_t st, const int16_t* plane, int w, int h, int n8, const float* qs, float dz, int16_t* coeffs, int16_t* recon);
+void deblock(hipStream_t st, uint8_t* Y, uint8_t* U, uint8_t* V, int W, int H, int level, bool intra, const uint8_t* mask);
+void filter_mask(hipStream_t st, const riv::BlockRec* rec, const uint8_t* opt_skip, int nb, uint8_t* mask);
+}  // namespace rivk

@@ -1,0 +1,102 @@
+// Kernel-level seams of the C ABI: single-stage GPU entry points on host buffers, used to check
+// each stage against the reference functions it replaces.
+#include <hip/hip_runtime.h>
+#include <vector>
+#include "host_common.hpp"
+#include "riv_kernels.hpp"
+
+using namespace rivhost;
+
+extern "C" {
+
+int riv_gpu_build_predicted(int32_t dev, const uint8_t* y, const uint8_t* u, const uint8_t* v, uint32_t w, uint32_t h, const int8_t* mv4,
+                            uint8_t* oy, uint8_t* ou, uint8_t* ov) {
+    if (!y || !u || !v || !mv4 || !oy || !ou || !ov || w % 16 || h % 16 || !w || !h) return RIV_E_ARG;
+    int r = ensure_device(dev);
+    if (r) return r;
+    const size_t ysz = (size_t)w * h, csz = ysz / 4, nb = ysz / 256;
+    DevBuf ref, pred, mvs;
+    RIV_HIP(ref.alloc(ysz + 2 * csz)); RIV_HIP(pred.alloc(ysz + 2 * csz)); RIV_HIP(mvs.alloc(4 * nb));
+    uint8_t* R = ref.as<uint8_t>(); uint8_t* P = pred.as<uint8_t>();
+    RIV_HIP(hipMemcpy(R, y, ysz, hipMemcpyHostToDevice)); RIV_HIP(hipMemcpy(R + ysz, u, csz, hipMemcpyHostToDevice));
+    RIV_HIP(hipMemcpy(R + ysz + csz, v, csz, hipMemcpyHostToDevice)); RIV_HIP(hipMemcpy(mvs.p, mv4, 4 * nb, hipMemcpyHostToDevice));
+    rivk::mc(0, R, R + ysz, R + ysz + csz, (int)w, (int)h, mvs.as<int8_t>(), 4, P, P + ysz, P + ysz + csz, nullptr);
+    RIV_HIP(hipGetLastError());
+    RIV_HIP(hipMemcpy(oy, P, ysz, hipMemcpyDeviceToHost)); RIV_HIP(hipMemcpy(ou, P + ysz, csz, hipMemcpyDeviceToHost));
+    RIV_HIP(hipMemcpy(ov, P + ysz + csz, csz, hipMemcpyDeviceToHost));
+    return RIV_OK;
+}
+
+int riv_gpu_deblock_yuv420(int32_t dev, uint8_t* y, uint8_t* u, uint8_t* v, uint32_t w, uint32_t h, int32_t level, int32_t intra, const uint8_t* mask) {
+    if (!y || !u || !v || w % 16 || h % 16 || !w || !h) return RIV_E_ARG;
+    int r = ensure_device(dev);
+    if (r) return r;
+    const size_t ysz = (size_t)w * h, csz = ysz / 4, nb = ysz / 256;
+    DevBuf f, m;
+    RIV_HIP(f.alloc(ysz + 2 * csz));
+    uint8_t* F = f.as<uint8_t>();
+    RIV_HIP(hipMemcpy(F, y, ysz, hipMemcpyHostToDevice)); RIV_HIP(hipMemcpy(F + ysz, u, csz, hipMemcpyHostToDevice));
+    RIV_HIP(hipMemcpy(F + ysz + csz, v, csz, hipMemcpyHostToDevice));
+    if (mask) { RIV_HIP(m.alloc(nb)); RIV_HIP(hipMemcpy(m.p, mask, nb, hipMemcpyHostToDevice)); }
+    rivk::deblock(0, F, F + ysz, F + ysz + csz, (int)w, (int)h, level, intra != 0, mask ? m.as<uint8_t>() : nullptr);
+    RIV_HIP(hipGetLastError());
+    RIV_HIP(hipMemcpy(y, F, ysz, hipMemcpyDeviceToHost)); RIV_HIP(hipMemcpy(u, F + ysz, csz, hipMemcpyDeviceToHost));
+    RIV_HIP(hipMemcpy(v, F + ysz + csz, csz, hipMemcpyDeviceToHost));
+    return RIV_OK;
+}
+
+static int dct_plane(int32_t dev, const int16_t* plane, uint32_t w, uint32_t h, int n8, const float* qs, float dz, int16_t* coeffs, int16_t* recon) {
+    const int bs = n8 ? 8 : 16;
+    if (!plane || !qs || !coeffs || !recon || !w || !h || w % bs || h % bs) return RIV_E_ARG;
+    int r = ensure_device(dev);
+    if (r) return r;
+    const size_t n = (size_t)w * h, nb = n / (bs * bs);
+    DevBuf dp, dq, dc, dr;
+    RIV_HIP(dp.alloc(2 * n)); RIV_HIP(dq.alloc(4 * nb)); RIV_HIP(dc.alloc(2 * n)); RIV_HIP(dr.alloc(2 * n));
+    RIV_HIP(hipMemcpy(dp.p, plane, 2 * n, hipMemcpyHostToDevice)); RIV_HIP(hipMemcpy(dq.p, qs, 4 * nb, hipMemcpyHostToDevice));
+    rivk::dct_plane(0, dp.as<int16_t>(), (int)w, (int)h, n8, dq.as<float>(), dz, dc.as<int16_t>(), dr.as<int16_t>());
+    RIV_HIP(hipGetLastError());
+    RIV_HIP(hipMemcpy(coeffs, dc.p, 2 * n, hipMemcpyDeviceToHost)); RIV_HIP(hipMemcpy(recon, dr.p, 2 * n, hipMemcpyDeviceToHost));
+    return RIV_OK;
+}
+int riv_gpu_dct8_aq_plane(int32_t dev, const int16_t* plane, uint32_t w, uint32_t h, const float* qs, float dz, int16_t* coeffs, int16_t* recon) {
+    return dct_plane(dev, plane, w, h, 1, qs, dz, coeffs, recon);
+}
+int riv_gpu_dct16_aq_plane(int32_t dev, const int16_t* plane, uint32_t w, uint32_t h, const float* qs, float dz, int16_t* coeffs, int16_t* recon) {
+    return dct_plane(dev, plane, w, h, 0, qs, dz, coeffs, recon);
+}
+
+int riv_gpu_aq_steps(int32_t dev, const uint8_t* y, uint32_t w, uint32_t h, float base, float* out) {
+    if (!y || !out || w % 16 || h % 16 || !w || !h) return RIV_E_ARG;
+    int r = ensure_device(dev);
+    if (r) return r;
+    const size_t ysz = (size_t)w * h, nb = ysz / 256;
+    // the variance sums come from the MC kernel; feed it the plane as its own zero-MV prediction
+    DevBuf ref, pred, mvs, var, lnv, steps;
+    RIV_HIP(ref.alloc(ysz * 3 / 2)); RIV_HIP(pred.alloc(ysz * 3 / 2)); RIV_HIP(mvs.alloc(4 * nb));
+    RIV_HIP(var.alloc(8 * nb)); RIV_HIP(lnv.alloc(4 * nb)); RIV_HIP(steps.alloc(4 * nb));
+    RIV_HIP(hipMemset(ref.p, 0, ysz * 3 / 2)); RIV_HIP(hipMemset(mvs.p, 0, 4 * nb));
+    RIV_HIP(hipMemcpy(ref.p, y, ysz, hipMemcpyHostToDevice));
+    uint8_t* R = ref.as<uint8_t>(); uint8_t* P = pred.as<uint8_t>();
+    rivk::mc(0, R, R + ysz, R + ysz + ysz / 4, (int)w, (int)h, mvs.as<int8_t>(), 4, P, P + ysz, P + ysz + ysz / 4, var.as<uint32_t>());
+    rivk::aq(0, var.as<uint32_t>(), (int)nb, base, lnv.as<float>(), steps.as<float>());
+    RIV_HIP(hipGetLastError());
+    RIV_HIP(hipMemcpy(out, steps.p, 4 * nb, hipMemcpyDeviceToHost));
+    return RIV_OK;
+}
+
+int riv_gpu_logf_exp2f(int32_t dev, const float* in, size_t n, int32_t which, float* out) {
+    if (!in || !out) return RIV_E_ARG;
+    if (!n) return RIV_OK;
+    int r = ensure_device(dev);
+    if (r) return r;
+    DevBuf di, dout;
+    RIV_HIP(di.alloc(4 * n)); RIV_HIP(dout.alloc(4 * n));
+    RIV_HIP(hipMemcpy(di.p, in, 4 * n, hipMemcpyHostToDevice));
+    rivk::logf_exp2f(0, di.as<float>(), n, which, dout.as<float>());
+    RIV_HIP(hipGetLastError());
+    RIV_HIP(hipMemcpy(out, dout.p, 4 * n, hipMemcpyDeviceToHost));
+    return RIV_OK;
+}
+
+}  // extern "C"
