@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""1080p P-frame encode throughput of the gfx950 RIV encoder (plain form, bit-exact bitstream).
+
+Workload: 1920x1080 RGB24 synthetic sequence (seeded generator: 1/f^2 background, global motion up
+to 8 px/frame at half-pel steps, 12 textured rectangles of 64-320 px at up to 16 px/frame, noise
+sigma 2), intra/inter quality 85/80, search range 12, skip threshold 3, one I-frame then P-frames.
+A step is one P-frame encode_frame call (GPU analysis + host rANS + packing) with its RGB input
+already resident in HBM. Prints one JSON line.
+
+The roofline entry is for the dominant kernel (the motion-search + mode-decision wavefront):
+achieved = 2*Ws*Hs algorithmic bytes (current + reference luma, each read once) / its average
+duration from HIP events on the encoder's stream. cpu_baseline times the oracle (a C++
+restatement of the reference CPU encoder) on the host cores over a bounded sample.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+W, H = 1920, 1080
+SW, SH = 1920, 1088
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except Exception:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return platform.processor()
+
+
+def make_config(riv2_amd):
+    return riv2_amd.EncoderConfig(W, H, 50).with_intra_quality(85).with_inter_quality(80).with_search_range(12) \
+        .with_skip_threshold(3).with_keyframe_interval(riv2_amd.KeyframeInterval.fixed(1000))
+
+
+def run_gpu(frames, steps, warmup):
+    import torch
+    import riv2_amd
+
+    dev = torch.device("cuda:0")
+    d_frames = torch.from_numpy(frames).to(dev)
+    torch.cuda.synchronize()
+    fbytes = W * H * 3
+    enc = riv2_amd.Encoder(make_config(riv2_amd))
+    ptr = lambda i: d_frames[i].data_ptr()
+    enc.encode_frame_device(ptr(0), fbytes, W, H, 0)
+    for i in range(1, 1 + warmup):
+        enc.encode_frame_device(ptr(i), fbytes, W, H, i * 20)
+    enc.set_profiling(True)
+    stats = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(1 + warmup, 1 + warmup + steps):
+        stats.append(enc.encode_frame_device(ptr(i), fbytes, W, H, i * 20))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    stages = enc.stage_times()
+    enc.finish()
+    data = enc.output()
+    enc.close()
+    return t1 - t0, stats, stages, data
+
+
+def check_prefix_bit_exact(frames, gpu_stream, nframes):
+    """Re-encodes the first frames with the oracle and compares the leading frame records."""
+    import oracle
+    ref = oracle.OracleEncoder(oracle.config(W, H))
+    for i in range(nframes):
+        ref.encode(frames[i], i * 20)
+    ref_stream = ref.finish()
+    body = len(ref_stream) - 36
+    return gpu_stream[36:36 + body] == ref_stream[36:]
+
+
+def cpu_baseline(frames, pframes, threads):
+    """Oracle encoder on the host cores: serial-raster ME (bit-exact mode) is the reference library
+    default; threads > 0 emulates its `threads` feature (rows_per_strip = ceil(MB_rows / (4 T)))."""
+    import oracle
+    enc = oracle.OracleEncoder(oracle.config(W, H, me_threads=threads))
+    enc.encode(frames[0], 0)
+    t0 = time.perf_counter()
+    for i in range(1, 1 + pframes):
+        enc.encode(frames[i], i * 20)
+    return pframes / (time.perf_counter() - t0)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-pframes", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one plain 256 MiB device copy kernel (HBM counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every box here has a single MI355X; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is described in DESIGN.md, not measured).", file=sys.stderr)
+        sys.exit(2)
+
+    from riv2_amd import synth
+    nframes = 1 + args.warmup + args.steps
+    print(f"[bench] generating {nframes} 1080p frames", file=sys.stderr, flush=True)
+    frames = synth.Sequence(synth.hd_params(seed=args.seed, frames=max(nframes, 50))).frames(0, nframes)
+
+    print(f"[bench] GPU encode: 1 I + {args.warmup} warmup + {args.steps} timed P-frames", file=sys.stderr, flush=True)
+    elapsed, stats, stages, stream = run_gpu(frames, args.steps, args.warmup)
+    fps = args.steps / elapsed
+    ms_per_step = 1000.0 * elapsed / args.steps
+
+    walk_ms, walk_n = stages["me_walk"]
+    walk_avg_ms = walk_ms / max(1, walk_n)
+    me_bytes = 2 * SW * SH   # current + reference luma planes, each read once
+    achieved = me_bytes / (walk_avg_ms * 1e-3) / 1e9 if walk_avg_ms > 0 else 0.0
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "me_walk_1080p_traffic.json")
+    if os.path.exists(tpath):
+        try:
+            with open(tpath) as f:
+                traffic = json.load(f).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    if args.copy_calibration:
+        import torch
+        src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda:0").fill_(1)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    print("[bench] oracle prefix check", file=sys.stderr, flush=True)
+    bit_exact = check_prefix_bit_exact(frames, stream, 3)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cores = host_cores()
+        print(f"[bench] CPU baseline: oracle, {args.cpu_pframes} P-frames, {cores} threads", file=sys.stderr, flush=True)
+        cpu_fps = cpu_baseline(frames, args.cpu_pframes, cores)
+        cpu = {"value": round(cpu_fps, 3), "unit": "fps", "cores": cores, "kind": "port",
+               "sample": f"1920x1080 synthetic, 1 I + {args.cpu_pframes} P-frames timed, oracle C++ restatement of the "
+                         f"reference encoder with its `threads` strip ME on {cores} threads ({cpu_model()})"}
+
+    stage_avg = {k: round(v[0] / v[1], 4) for k, v in stages.items() if v[1]}
+    out = {
+        "metric": "1080p P-frame encode fps/GPU: plain bit-exact vs learned-MV at ≤1% bitrate loss",
+        "value": round(fps, 3),
+        "unit": "fps",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": "1080p P-frame encode, plain form: 1920x1080 RGB24, q85/80, search range 12, skip 3, "
+                               "1 I + P-frames, inputs resident in HBM", "width": W, "height": H, "form": "plain",
+                   "bit_exact_prefix_vs_oracle": bool(bit_exact)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+                     "kernel": "k_me_walk", "algorithmic_bytes_per_launch": me_bytes, "avg_launch_ms": round(walk_avg_ms, 4)},
+        "cpu_baseline": cpu,
+        "gpu_stage_avg_ms": stage_avg,
+        "host_entropy_avg_ms": round(sum(s.host_entropy_ms for s in stats) / len(stats), 4),
+        "bytes_per_pframe": round(sum(s.total_bytes for s in stats) / len(stats), 1),
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = round(fps / cpu["value"], 2)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -175,6 +175,15 @@ int riv_gpu_aq_steps(int32_t device_id, const uint8_t* y, uint32_t w, uint32_t h
 /* glibc-identical logf / exp2f on the device (used by AQ), n values */
 int riv_gpu_logf_exp2f(int32_t device_id, const float* in, size_t n, int32_t which, float* out);
 
+/* Per-stage device timing of the encoder (HIP events on the encoder's stream around each stage;
+ * no reference counterpart). Stages: 0 RGB->YUV, 1 motion search + mode decision wavefront,
+ * 2 motion compensation, 3 adaptive quantiser, 4 residual transform/quantise/reconstruct,
+ * 5 deblock, 6 intra transform, 7 device->host copies. */
+#define RIV_NUM_STAGES 8
+int riv_encoder_set_profiling(riv_encoder* enc, int on);
+/* Accumulated milliseconds and launch counts per stage since profiling was switched on. */
+int riv_encoder_stage_times(riv_encoder* enc, double* ms, uint32_t* counts);
+
 /* Last error message of the calling thread (static storage). */
 const char* riv_last_error(void);
 

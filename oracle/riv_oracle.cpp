@@ -714,6 +714,16 @@ static inline int median3(int a, int b, int c) {   // scalar.rs:24-43
 // sad_block_int_limit (full SAD; the early exit is result-invariant)  satd_scalar.rs:4-99
 static int64_t sad_int(const uint8_t* prev, const uint8_t* cur, int W, int H, int x0, int y0, int dx, int dy) {
     int64_t sad = 0;
+    if (x0 >= 0 && y0 >= 0 && x0 + 16 <= W && y0 + 16 <= H && x0 + dx >= 0 && y0 + dy >= 0 && x0 + dx + 16 <= W && y0 + dy + 16 <= H) {
+        // both blocks inside: no clamping applies (satd_simd.rs:15-111 in-bounds path)
+        int acc = 0;
+        for (int j = 0; j < 16; j++) {
+            const uint8_t* c = cur + (size_t)(y0 + j) * W + x0;
+            const uint8_t* r = prev + (size_t)(y0 + dy + j) * W + x0 + dx;
+            for (int i = 0; i < 16; i++) acc += std::abs((int)c[i] - (int)r[i]);
+        }
+        return acc;
+    }
     for (int by = 0; by < 4; by++)
         for (int bx = 0; bx < 4; bx++) {
             int sx0 = x0 + bx * 4, sy0 = y0 + by * 4;
@@ -748,6 +758,21 @@ static inline int sample_luma_hp_floor(const uint8_t* p, int W, int H, int xh, i
 // sad_block_halfpel_limit_luma (full SAD)  satd_scalar.rs:168-223
 static int64_t sad_hp(const uint8_t* prev, const uint8_t* cur, int W, int H, int x0, int y0, int dxh, int dyh) {
     int64_t sad = 0;
+    if (2 * x0 + dxh >= 0 && 2 * (x0 + 15) + dxh + 1 <= 2 * (W - 1) && 2 * y0 + dyh >= 0 && 2 * (y0 + 15) + dyh + 1 <= 2 * (H - 1)) {
+        // every sample and its interpolation partner inside: no clamping (satd_simd.rs:184-246 path)
+        const int bx = (2 * x0 + dxh) >> 1, by = (2 * y0 + dyh) >> 1, ax = dxh & 1, ay = dyh & 1;
+        int acc = 0;
+        for (int j = 0; j < 16; j++) {
+            const uint8_t* c = cur + (size_t)(y0 + j) * W + x0;
+            const uint8_t* r0 = prev + (size_t)(by + j) * W + bx;
+            const uint8_t* r1 = r0 + (ay ? W : 0);
+            for (int i = 0; i < 16; i++) {
+                int v = (r0[i] + r0[i + ax] + r1[i] + r1[i + ax]) >> 2;   // floor average of 1, 2 or 4 samples
+                acc += std::abs((int)c[i] - v);
+            }
+        }
+        return acc;
+    }
     for (int yy = 0; yy < 16; yy++)
         for (int xx = 0; xx < 16; xx++) {
             int cx = x0 + xx, cy = y0 + yy;

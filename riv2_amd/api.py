@@ -268,6 +268,17 @@ class Encoder:
         _check(L.lib().riv_encoder_output(self._h, byref(p), byref(n)), EncodeError)
         return ctypes.string_at(p, n.value)
 
+    STAGES = ["rgb_to_yuv", "me_walk", "mc", "aq", "residual", "deblock", "intra", "d2h"]
+
+    def set_profiling(self, on: bool = True):
+        _check(L.lib().riv_encoder_set_profiling(self._h, int(bool(on))))
+
+    def stage_times(self):
+        ms = np.zeros(8, np.float64)
+        n = np.zeros(8, np.uint32)
+        _check(L.lib().riv_encoder_stage_times(self._h, ms.ctypes.data, n.ctypes.data))
+        return {name: (float(ms[i]), int(n[i])) for i, name in enumerate(self.STAGES)}
+
     # ---- last-frame introspection for parity tests ----
     def debug_blocks(self):
         nb = self._bw * self._bh

@@ -37,6 +37,20 @@ struct riv_encoder {
     int c_zero = 0, c_pred = 0, rdo_max = -1;
     float inter_qs = 0, intra_qs = 0;
     rivh::MvCoder mvc;
+    // optional per-stage timing
+    bool profiling = false;
+    hipEvent_t sev[RIV_NUM_STAGES + 1] = {};
+    double stage_ms[RIV_NUM_STAGES] = {};
+    uint32_t stage_n[RIV_NUM_STAGES] = {};
+    int n_marks = 0;
+    int mark_stage[RIV_NUM_STAGES + 1] = {};
+    // records an event closing the stage that started at the previous mark
+    void mark(int stage) {
+        if (!profiling || n_marks > RIV_NUM_STAGES) return;
+        (void)hipEventRecord(sev[n_marks], st);
+        mark_stage[n_marks] = stage;
+        n_marks++;
+    }
 
     int write(const uint8_t* d, size_t n) {
         if (mem) {
@@ -216,18 +230,24 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
     hipStream_t st = e->st;
 
     RIV_HIP(hipEventRecord(e->ev0, st));
+    e->n_marks = 0;
     const uint8_t* d_rgb = (const uint8_t*)rgb;
     if (!on_device) {
         RIV_HIP(hipMemcpyAsync(e->rgb.p, rgb, len, hipMemcpyHostToDevice, st));
         d_rgb = e->rgb.as<uint8_t>();
     }
+    e->mark(-1);
     rivk::rgb_to_yuv420(st, d_rgb, (int)w, (int)h, W, H, cY, cU, cV);
+    e->mark(0);
 
     BlockRec* rec = e->rec[e->rec_idx].as<BlockRec>();
     if (intra) {
         rivk::intra(st, cY, cU, cV, W, H, e->yt.as<uint16_t>(), e->ct.as<uint16_t>(), nY, nU, nV, e->coeffs.as<int16_t>());
+        e->mark(6);
         rivk::deblock(st, nY, nU, nV, W, H, deblock_level(e->intra_qs), true, nullptr);
+        e->mark(5);
         RIV_HIP(hipMemcpyAsync(e->h_coeffs.p, e->coeffs.p, 768 * (size_t)nb, hipMemcpyDeviceToHost, st));
+        e->mark(7);
     } else {
         riv::WalkParams p{};
         p.cur = cY; p.ref = rY; p.W = W; p.H = H; p.bw = e->bw; p.bh = e->bh; p.sr = c.search_range;
@@ -242,15 +262,21 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
         p.zthr = c.me_zero_mv_threshold; p.pthr = c.me_predictor_threshold;
         p.R = e->R; p.err = e->err.as<int>();
         RIV_HIP(rivk::me_walk(st, p));
+        e->mark(1);
         rivk::mc(st, rY, rU, rV, W, H, (const int8_t*)rec, sizeof(BlockRec), pY, pU, pV, e->var.as<uint32_t>());
+        e->mark(2);
         rivk::aq(st, e->var.as<uint32_t>(), nb, e->inter_qs, e->lnv.as<float>(), e->steps.as<float>());
+        e->mark(3);
         rivk::inter_residual(st, cY, cU, cV, pY, pU, pV, W, H, rec, e->steps.as<float>(), c.inter_dead_zone, nY, nU, nV,
                              e->coeffs.as<int16_t>(), e->optskip.as<uint8_t>());
+        e->mark(4);
         rivk::filter_mask(st, rec, e->optskip.as<uint8_t>(), nb, e->mask.as<uint8_t>());
         rivk::deblock(st, nY, nU, nV, W, H, deblock_level(e->inter_qs), false, e->mask.as<uint8_t>());
+        e->mark(5);
         RIV_HIP(hipMemcpyAsync(e->h_rec.p, rec, sizeof(BlockRec) * nb, hipMemcpyDeviceToHost, st));
         RIV_HIP(hipMemcpyAsync(e->h_optskip.p, e->optskip.p, nb, hipMemcpyDeviceToHost, st));
         RIV_HIP(hipMemcpyAsync(e->h_coeffs.p, e->coeffs.p, 768 * (size_t)nb, hipMemcpyDeviceToHost, st));
+        e->mark(7);
     }
     RIV_HIP(hipGetLastError());
     RIV_HIP(hipEventRecord(e->ev1, st));
@@ -262,6 +288,12 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
     }
     float gpu_ms = 0;
     RIV_HIP(hipEventElapsedTime(&gpu_ms, e->ev0, e->ev1));
+    for (int k = 1; k < e->n_marks; k++) {
+        float ms = 0;
+        RIV_HIP(hipEventElapsedTime(&ms, e->sev[k - 1], e->sev[k]));
+        int sidx = e->mark_stage[k];
+        if (sidx >= 0) { e->stage_ms[sidx] += ms; e->stage_n[sidx]++; }
+    }
 
     auto t0 = std::chrono::steady_clock::now();
     std::vector<uint8_t> bytes;
@@ -370,8 +402,24 @@ void riv_encoder_destroy(riv_encoder* e) {
     if (e->st) (void)hipStreamSynchronize(e->st);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
+    for (int i = 0; i <= RIV_NUM_STAGES; i++) if (e->sev[i]) (void)hipEventDestroy(e->sev[i]);
     if (e->st) (void)hipStreamDestroy(e->st);
     delete e;
+}
+
+int riv_encoder_set_profiling(riv_encoder* e, int on) {
+    if (!e) return RIV_E_ARG;
+    RIV_HIP(hipSetDevice(e->cfg.device_id));
+    if (on && !e->sev[0])
+        for (int i = 0; i <= RIV_NUM_STAGES; i++) RIV_HIP(hipEventCreate(&e->sev[i]));
+    e->profiling = on != 0;
+    for (int i = 0; i < RIV_NUM_STAGES; i++) { e->stage_ms[i] = 0; e->stage_n[i] = 0; }
+    return RIV_OK;
+}
+int riv_encoder_stage_times(riv_encoder* e, double* ms, uint32_t* counts) {
+    if (!e || !ms || !counts) return RIV_E_ARG;
+    for (int i = 0; i < RIV_NUM_STAGES; i++) { ms[i] = e->stage_ms[i]; counts[i] = e->stage_n[i]; }
+    return RIV_OK;
 }
 
 int riv_encoder_debug_blocks(riv_encoder* e, int8_t* rec12, int64_t* sad, uint8_t* rdo_skip, uint8_t* opt_skip) {
