@@ -55,6 +55,10 @@ typedef struct {
     double rdo_lambda_mult;
     float inter_dead_zone;
     int32_t device_id;                        /* HIP device ordinal */
+    /* 0: encode_frame returns after the frame is written (reference semantics). N > 0: up to N
+     * frames in flight; the GPU analyses frame k+1 while host threads entropy-code earlier frames,
+     * frames are written in order, stats arrive through riv_encoder_pop_stats. Same bitstream. */
+    int32_t pipeline_depth;
 } riv_encoder_config;
 
 /* EncoderConfig::new(w, h, fps) defaults (config.rs:56-77) */
@@ -113,6 +117,10 @@ int riv_encode_frame(riv_encoder* enc, const uint8_t* rgb, size_t len, uint32_t 
 /* Same, with the RGB24 frame already resident in device memory of the encoder's device. */
 int riv_encode_frame_device(riv_encoder* enc, const void* d_rgb, size_t len, uint32_t width, uint32_t height,
                             uint64_t timestamp_ms, riv_frame_stats* stats);
+/* Blocks until every submitted frame has been written (pipelined mode; no-op otherwise). */
+int riv_encoder_flush(riv_encoder* enc);
+/* Completed frames' stats in encode order (pipelined mode); *n receives the number copied. */
+int riv_encoder_pop_stats(riv_encoder* enc, riv_frame_stats* out, size_t cap, size_t* n);
 /* Encoder::finish (encoder.rs:950-961): patches frame_count into the header, flushes. */
 int riv_encoder_finish(riv_encoder* enc);
 /* Encoder::frame_count / header (encoder.rs:969-981) */

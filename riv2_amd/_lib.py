@@ -32,6 +32,7 @@ class EncoderConfigC(Structure):
         ("rdo_lambda_mult", c_double),
         ("inter_dead_zone", c_float),
         ("device_id", c_int32),
+        ("pipeline_depth", c_int32),
     ]
 
 
@@ -85,6 +86,8 @@ PROTOTYPES = [
     ("riv_encoder_create_mem", c_int, [POINTER(EncoderConfigC), POINTER(c_void_p)]),
     ("riv_encode_frame", c_int, [c_void_p, c_void_p, c_size_t, c_uint32, c_uint32, c_uint64, POINTER(FrameStatsC)]),
     ("riv_encode_frame_device", c_int, [c_void_p, c_void_p, c_size_t, c_uint32, c_uint32, c_uint64, POINTER(FrameStatsC)]),
+    ("riv_encoder_flush", c_int, [c_void_p]),
+    ("riv_encoder_pop_stats", c_int, [c_void_p, POINTER(FrameStatsC), c_size_t, POINTER(c_size_t)]),
     ("riv_encoder_finish", c_int, [c_void_p]),
     ("riv_encoder_frame_count", c_uint64, [c_void_p]),
     ("riv_encoder_header", c_int, [c_void_p, POINTER(VideoHeaderC)]),

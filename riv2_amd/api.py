@@ -101,6 +101,7 @@ class EncoderConfig:
     def with_me_zero_mv_threshold(self, t): return self._with(me_zero_mv_threshold=t)
     def with_me_predictor_threshold(self, t): return self._with(me_predictor_threshold=t)
     def with_device(self, d): return self._with(device_id=d)
+    def with_pipeline_depth(self, n): return self._with(pipeline_depth=n)
 
     def validate(self):
         _check(L.lib().riv_encoder_config_validate(byref(self._c)), EncodeError)
@@ -250,6 +251,17 @@ class Encoder:
         s = L.FrameStatsC()
         _check(L.lib().riv_encode_frame_device(self._h, c_void_p(ptr), nbytes, width, height, timestamp, byref(s)), EncodeError)
         return EncodeFrameStats.from_c(s)
+
+    def flush(self):
+        """Waits until every submitted frame is written (pipelined mode)."""
+        _check(L.lib().riv_encoder_flush(self._h), EncodeError)
+
+    def pop_stats(self, cap: int = 256):
+        """Stats of completed frames in encode order (pipelined mode)."""
+        arr = (L.FrameStatsC * cap)()
+        n = c_size_t()
+        _check(L.lib().riv_encoder_pop_stats(self._h, arr, cap, byref(n)), EncodeError)
+        return [EncodeFrameStats.from_c(arr[i]) for i in range(n.value)]
 
     def finish(self):
         _check(L.lib().riv_encoder_finish(self._h), EncodeError)

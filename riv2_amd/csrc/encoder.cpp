@@ -1,10 +1,18 @@
 // Encoder driver of the C ABI: Encoder / EncoderConfig / encode_frame of reitero_encode
-// (reitero_encode/src/encoder.rs, config.rs), with the per-frame analysis on the GPU and the
-// rANS entropy coding on the host.
+// (reitero_encode/src/encoder.rs, config.rs). Per-frame analysis runs on the GPU; rANS entropy
+// coding and packing run on the host. Frames flow through a ring of slots: the GPU writes a
+// frame's block records and coefficients into its slot, host workers entropy-code slots
+// concurrently (residual coding has fresh contexts per frame), and the ordered tail (MV coding,
+// whose contexts carry across frames, packing, writing) runs in frame order.
 #include <hip/hip_runtime.h>
 #include <chrono>
 #include <cmath>
 #include <algorithm>
+#include <condition_variable>
+#include <deque>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <vector>
 #include "host_common.hpp"
 #include "riv_kernels.hpp"
@@ -12,6 +20,57 @@
 
 using namespace rivhost;
 using riv::BlockRec;
+
+namespace {
+
+struct Slot {
+    DevBuf rec, coeffs, optskip;         // device outputs of the frame (rec also seeds the next frame's temporal MVs)
+    HostBuf h_rec, h_coeffs, h_optskip;  // pinned copies for the host entropy stage
+    hipEvent_t done = nullptr;           // D2H copies of this slot finished
+    hipEvent_t marks[RIV_NUM_STAGES + 1] = {};
+    int mark_stage[RIV_NUM_STAGES + 1] = {};
+    int n_marks = 0;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    uint64_t index = 0, ts = 0;
+    bool intra = false, busy = false;
+};
+
+class Pool {
+public:
+    void start(int n) {
+        for (int i = 0; i < n; i++) th_.emplace_back([this] { run(); });
+    }
+    void submit(std::function<void()> f) {
+        { std::lock_guard<std::mutex> lk(mu_); q_.push_back(std::move(f)); }
+        cv_.notify_one();
+    }
+    ~Pool() {
+        { std::lock_guard<std::mutex> lk(mu_); stop_ = true; }
+        cv_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+private:
+    void run() {
+        for (;;) {
+            std::function<void()> f;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || !q_.empty(); });
+                if (stop_ && q_.empty()) return;
+                f = std::move(q_.front());
+                q_.pop_front();
+            }
+            f();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::deque<std::function<void()>> q_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    bool stop_ = false;
+};
+
+}  // namespace
 
 struct riv_encoder {
     riv_encoder_config cfg{};
@@ -22,35 +81,37 @@ struct riv_encoder {
     uint64_t mempos = 0;
     uint64_t header_pos = 0;
     riv_video_header hdr{};
-    uint64_t frame_count = 0;
+    uint64_t frame_count = 0;      // frames submitted
 
     int W = 0, H = 0, bw = 0, bh = 0, nb = 0, R = 4;
     hipStream_t st = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    DevBuf rgb, cur, pred, ref[2], rec[2], var, lnv, steps, coeffs, optskip, mask, progress, bits, rdo, err, yt, ct;
-    HostBuf h_rec, h_optskip, h_coeffs;
-    int ref_idx = 0, rec_idx = 0;
-    bool have_ref = false, have_prev_mvs = false, last_inter = false;
+    DevBuf rgb, cur, pred, ref[2], var, lnv, steps, mask, progress, bits, rdo, err, yt, ct;
+    int ref_idx = 0;
+    bool have_ref = false;
+    int prev_inter_slot = -1;      // slot holding the previous P-frame's records, -1 after an intra frame
     unsigned long long epoch = 0;
-    // frame constants
     double lambda = 0;
     int c_zero = 0, c_pred = 0, rdo_max = -1;
     float inter_qs = 0, intra_qs = 0;
+
+    std::vector<Slot> slots;
+    int depth = 0;                 // configured pipeline depth (0 = synchronous)
+    std::unique_ptr<Pool> pool;
+
+    // ordered tail + completion state
+    std::mutex mu;
+    std::condition_variable cv;
+    uint64_t next_tail = 0;        // next frame index allowed into the ordered tail
+    uint64_t completed = 0;        // frames fully written
+    int err_code = 0;
+    std::string err_msg;
     rivh::MvCoder mvc;
-    // optional per-stage timing
+    std::deque<riv_frame_stats> done_stats;
+    int last_slot = -1;            // slot of the most recently completed frame (debug accessors)
+
     bool profiling = false;
-    hipEvent_t sev[RIV_NUM_STAGES + 1] = {};
     double stage_ms[RIV_NUM_STAGES] = {};
     uint32_t stage_n[RIV_NUM_STAGES] = {};
-    int n_marks = 0;
-    int mark_stage[RIV_NUM_STAGES + 1] = {};
-    // records an event closing the stage that started at the previous mark
-    void mark(int stage) {
-        if (!profiling || n_marks > RIV_NUM_STAGES) return;
-        (void)hipEventRecord(sev[n_marks], st);
-        mark_stage[n_marks] = stage;
-        n_marks++;
-    }
 
     int write(const uint8_t* d, size_t n) {
         if (mem) {
@@ -65,6 +126,11 @@ struct riv_encoder {
     int seek(uint64_t pos) {
         if (mem) { if (pos > membuf.size()) membuf.resize(pos); mempos = pos; return RIV_OK; }
         return wv.seek(wuser, pos) == 0 ? RIV_OK : RIV_E_IO;
+    }
+    void fail(int code, const std::string& msg) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!err_code) { err_code = code; err_msg = msg; }
+        cv.notify_all();
     }
 };
 
@@ -81,6 +147,7 @@ void riv_encoder_config_default(riv_encoder_config* c, uint32_t w, uint32_t h, u
     c->me_zero_mv_threshold = 0; c->me_predictor_threshold = 0;
     c->rdo_lambda_mult = 0.49; c->inter_dead_zone = 0.75f;
     c->device_id = 0;
+    c->pipeline_depth = 0;
 }
 
 int riv_encoder_config_validate(const riv_encoder_config* c) {
@@ -91,6 +158,7 @@ int riv_encoder_config_validate(const riv_encoder_config* c) {
     if (c->search_range > 127) { set_error("search_range must be <= 127 for current MV encoding"); return RIV_E_INVALID_CONFIG; }
     if (c->mv_deflate_level > 10) { set_error("mv_deflate_level must be in 0..=10"); return RIV_E_INVALID_CONFIG; }
     if (c->keyframe_mode == RIV_KEYFRAME_FIXED && c->keyframe_interval == 0) { set_error("Keyframe interval must be greater than 0"); return RIV_E_INVALID_CONFIG; }
+    if (c->pipeline_depth < 0 || c->pipeline_depth > 64) { set_error("pipeline_depth must be in 0..=64"); return RIV_E_INVALID_CONFIG; }
     return RIV_OK;
 }
 
@@ -102,21 +170,29 @@ static int encoder_init(riv_encoder* e) {
     if (r) return r;
     e->W = (int)c.storage_width; e->H = (int)c.storage_height;
     e->bw = e->W / 16; e->bh = e->H / 16; e->nb = e->bw * e->bh;
-    // rows per wavefront workgroup: as many as fit the LDS budget, at most 4
-    e->R = 4;
+    e->R = 4;   // MB rows per wavefront workgroup: as many as fit the LDS budget, at most 4
     while (e->R > 1 && rivk::walk_lds_bytes(c.search_range, e->bw, e->R) > 60000) e->R--;
     const size_t ysz = (size_t)e->W * e->H, csz = ysz / 4, fsz = ysz + 2 * csz;
     RIV_HIP(hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking));
-    RIV_HIP(hipEventCreate(&e->ev0)); RIV_HIP(hipEventCreate(&e->ev1));
     RIV_HIP(e->rgb.alloc((size_t)c.display_width * c.display_height * 3));
     RIV_HIP(e->cur.alloc(fsz)); RIV_HIP(e->pred.alloc(fsz));
     RIV_HIP(e->ref[0].alloc(fsz)); RIV_HIP(e->ref[1].alloc(fsz));
-    RIV_HIP(e->rec[0].alloc(sizeof(BlockRec) * e->nb)); RIV_HIP(e->rec[1].alloc(sizeof(BlockRec) * e->nb));
     RIV_HIP(e->var.alloc(8 * (size_t)e->nb)); RIV_HIP(e->lnv.alloc(4 * (size_t)e->nb)); RIV_HIP(e->steps.alloc(4 * (size_t)e->nb));
-    RIV_HIP(e->coeffs.alloc(768 * (size_t)e->nb)); RIV_HIP(e->optskip.alloc(e->nb)); RIV_HIP(e->mask.alloc(e->nb));
+    RIV_HIP(e->mask.alloc(e->nb));
     RIV_HIP(e->progress.alloc(8 * (size_t)e->bh)); RIV_HIP(hipMemset(e->progress.p, 0, 8 * (size_t)e->bh));
     RIV_HIP(e->err.alloc(4)); RIV_HIP(hipMemset(e->err.p, 0, 4));
-    RIV_HIP(e->h_rec.alloc(sizeof(BlockRec) * e->nb)); RIV_HIP(e->h_optskip.alloc(e->nb)); RIV_HIP(e->h_coeffs.alloc(768 * (size_t)e->nb));
+    e->depth = c.pipeline_depth;
+    e->slots.resize(std::max(2, e->depth));
+    for (Slot& s : e->slots) {
+        RIV_HIP(s.rec.alloc(sizeof(BlockRec) * e->nb)); RIV_HIP(s.coeffs.alloc(768 * (size_t)e->nb)); RIV_HIP(s.optskip.alloc(e->nb));
+        RIV_HIP(s.h_rec.alloc(sizeof(BlockRec) * e->nb)); RIV_HIP(s.h_coeffs.alloc(768 * (size_t)e->nb)); RIV_HIP(s.h_optskip.alloc(e->nb));
+        RIV_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+        RIV_HIP(hipEventCreate(&s.t0)); RIV_HIP(hipEventCreate(&s.t1));
+    }
+    if (e->depth > 0) {
+        e->pool.reset(new Pool());
+        e->pool->start(std::min(e->depth, 16));
+    }
     // MV-cost bit table (motion/scalar.rs:495-525) and lambda-derived costs
     rivhost::RdoModel rdo(c.inter_quality, c.rdo_lambda_mult, c.skip_threshold);
     e->lambda = rdo.lambda;
@@ -162,12 +238,31 @@ static int create_common(const riv_encoder_config* cfg, riv_encoder* e) {
     return write_header(e);   // Encoder::new writes the header immediately (encoder.rs:181-185)
 }
 
+static void destroy_encoder(riv_encoder* e) {
+    if (!e) return;
+    (void)hipSetDevice(e->cfg.device_id);
+    if (e->pool) {   // let in-flight frames finish before the buffers go away
+        std::unique_lock<std::mutex> lk(e->mu);
+        e->cv.wait(lk, [&] { return e->completed == e->frame_count || e->err_code; });
+    }
+    e->pool.reset();
+    if (e->st) (void)hipStreamSynchronize(e->st);
+    for (Slot& s : e->slots) {
+        if (s.done) (void)hipEventDestroy(s.done);
+        if (s.t0) (void)hipEventDestroy(s.t0);
+        if (s.t1) (void)hipEventDestroy(s.t1);
+        for (auto& m : s.marks) if (m) (void)hipEventDestroy(m);
+    }
+    if (e->st) (void)hipStreamDestroy(e->st);
+    delete e;
+}
+
 int riv_encoder_create(const riv_encoder_config* cfg, const riv_writer_vtbl* writer, void* user, riv_encoder** out) {
     if (!cfg || !writer || !out || !writer->write || !writer->position || !writer->seek || !writer->flush) return RIV_E_ARG;
     riv_encoder* e = new riv_encoder();
     e->wv = *writer; e->wuser = user;
     int r = create_common(cfg, e);
-    if (r) { delete e; return r; }
+    if (r) { destroy_encoder(e); return r; }
     *out = e;
     return RIV_OK;
 }
@@ -177,7 +272,7 @@ int riv_encoder_create_mem(const riv_encoder_config* cfg, riv_encoder** out) {
     riv_encoder* e = new riv_encoder();
     e->mem = true;
     int r = create_common(cfg, e);
-    if (r) { delete e; return r; }
+    if (r) { destroy_encoder(e); return r; }
     *out = e;
     return RIV_OK;
 }
@@ -209,10 +304,116 @@ static void compute_bias(const BlockRec* rec, int nb, int& bx_out, int& by_out) 
     }
 }
 
+// Host stage of one frame: residual rANS (any order), then the ordered tail: MV rANS (contexts
+// carry across frames), packing (format.rs:171-205) and writing.
+static void entropy_job(riv_encoder* e, int si) {
+    Slot& s = e->slots[si];
+    const riv_encoder_config& c = e->cfg;
+    const int nb = e->nb;
+    if (hipEventSynchronize(s.done) != hipSuccess) { e->fail(RIV_E_GPU, "GPU frame failed"); return; }
+    auto t0 = std::chrono::steady_clock::now();
+    riv_frame_stats st{};
+    st.frame_index = s.index; st.timestamp_ms = s.ts;
+    st.storage_width = (uint32_t)e->W; st.storage_height = (uint32_t)e->H; st.blocks_total = (uint32_t)nb;
+    float gpu_ms = 0;
+    if (hipEventElapsedTime(&gpu_ms, s.t0, s.t1) == hipSuccess) st.gpu_ms = gpu_ms;
+    double smark[RIV_NUM_STAGES] = {};
+    uint32_t nmark[RIV_NUM_STAGES] = {};
+    for (int k = 1; k < s.n_marks; k++) {
+        float ms = 0;
+        if (s.mark_stage[k] >= 0 && hipEventElapsedTime(&ms, s.marks[k - 1], s.marks[k]) == hipSuccess) { smark[s.mark_stage[k]] += ms; nmark[s.mark_stage[k]]++; }
+    }
+    const int16_t* hco = s.h_coeffs.as<int16_t>();
+    std::vector<uint8_t> res, bytes;
+    std::vector<rivh::MvBlock> blocks;
+    int bdx = 0, bdy = 0;
+    try {
+        if (s.intra) {
+            std::vector<uint8_t> coded(nb, 1);
+            res = rivh::encode_residuals(hco, coded.data(), nb, true);
+            st.frame_type = 1;
+        } else {
+            const BlockRec* hr = s.h_rec.as<BlockRec>();
+            const uint8_t* os = s.h_optskip.as<uint8_t>();
+            compute_bias(hr, nb, bdx, bdy);
+            blocks.resize(nb);
+            std::vector<uint8_t> coded(nb);
+            for (int i = 0; i < nb; i++) {
+                rivh::MvBlock& b = blocks[i];
+                b.mode = hr[i].mode; b.new_base = hr[i].new_base;
+                b.dx = b.dy = 0; b.sx = b.sy = 0;
+                if (b.mode == riv::M_NEW) {   // delta after the frame bias (encoder.rs:775-818)
+                    b.dx = (int8_t)std::max(-128, std::min(127, (int)hr[i].raw_dx - bdx));
+                    b.dy = (int8_t)std::max(-128, std::min(127, (int)hr[i].raw_dy - bdy));
+                    b.sx = hr[i].s[2]; b.sy = hr[i].s[3];
+                    st.mv_new_blocks++;
+                }
+                b.skip = os[i] != 0;
+                coded[i] = !os[i];
+                st.blocks_skipped += b.skip;
+                st.mv_mode_counts[b.mode]++;
+                st.rdo_blocks_selected_skip += hr[i].rdo_skip;
+                if (hr[i].s[2] || hr[i].s[3]) st.rdo_forced_fractional++;
+            }
+            res = rivh::encode_residuals(hco, coded.data(), nb, false);
+            st.frame_type = 2;
+            st.mv_bias_dx = bdx; st.mv_bias_dy = bdy;
+            st.rdo_lambda = e->lambda; st.rdo_blocks_evaluated = (uint32_t)nb;
+            if (c.skip_threshold == 0) st.rdo_forced_disabled = (uint32_t)nb;
+        }
+    } catch (const std::exception& ex) {
+        e->fail(RIV_E_ENCODING_FAILED, std::string("entropy coding failed: ") + ex.what());
+        return;
+    }
+    // ordered tail
+    std::unique_lock<std::mutex> lk(e->mu);
+    e->cv.wait(lk, [&] { return e->next_tail == s.index || e->err_code; });
+    if (e->err_code) { e->next_tail++; e->cv.notify_all(); return; }
+    try {
+        if (s.intra) {
+            e->mvc.reset();   // MV contexts restart at every I-frame (encoder.rs:335-337)
+            put_u64(bytes, s.ts); bytes.push_back(1); bytes.push_back(c.intra_quality);
+            put_u32(bytes, (uint32_t)res.size());
+            bytes.insert(bytes.end(), res.begin(), res.end());
+            st.residual_bytes = (uint32_t)res.size();
+        } else {
+            std::vector<uint8_t> mvb = e->mvc.encode(blocks, e->bw, e->bh);
+            put_u64(bytes, s.ts); bytes.push_back(2); bytes.push_back(c.inter_quality);
+            bytes.push_back((uint8_t)(int8_t)bdx); bytes.push_back((uint8_t)(int8_t)bdy); bytes.push_back(0);
+            put_u32(bytes, (uint32_t)mvb.size()); bytes.insert(bytes.end(), mvb.begin(), mvb.end());
+            put_u32(bytes, (uint32_t)res.size()); bytes.insert(bytes.end(), res.begin(), res.end());
+            st.mv_bytes = (uint32_t)mvb.size(); st.residual_bytes = (uint32_t)res.size();
+        }
+    } catch (const std::exception& ex) {
+        if (!e->err_code) { e->err_code = RIV_E_ENCODING_FAILED; e->err_msg = std::string("MV coding failed: ") + ex.what(); }
+        e->next_tail++;
+        e->cv.notify_all();
+        return;
+    }
+    int r = e->write(bytes.data(), bytes.size());
+    if (r && !e->err_code) { e->err_code = r; e->err_msg = "writer failed"; }
+    st.total_bytes = (uint32_t)bytes.size();
+    st.host_entropy_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (e->profiling)
+        for (int k = 0; k < RIV_NUM_STAGES; k++) { e->stage_ms[k] += smark[k]; e->stage_n[k] += nmark[k]; }
+    e->done_stats.push_back(st);
+    if (e->done_stats.size() > 4096) e->done_stats.pop_front();
+    e->hdr.frame_count = s.index + 1;
+    e->last_slot = si;
+    e->next_tail++;
+    e->completed++;
+    s.busy = false;
+    e->cv.notify_all();
+}
+
 static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t len, uint32_t w, uint32_t h, uint64_t ts, riv_frame_stats* stats) {
     const riv_encoder_config& c = e->cfg;
     if (w != c.display_width || h != c.display_height) { set_error("Frame dimensions do not match config"); return RIV_E_FRAME; }
     if (len != (size_t)w * h * 3) { set_error("Display RGB24 size mismatch"); return RIV_E_FRAME; }
+    {
+        std::lock_guard<std::mutex> lk(e->mu);
+        if (e->err_code) { set_error(e->err_msg); return e->err_code; }
+    }
     RIV_HIP(hipSetDevice(c.device_id));
     // keyframe decision (encoder.rs:265-291)
     bool intra;
@@ -220,6 +421,22 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
     else if (c.keyframe_mode == RIV_KEYFRAME_AUTOMATIC) intra = e->frame_count % 30 == 0;
     else intra = e->frame_count % c.keyframe_interval == 0;
     if (!e->have_ref) intra = true;
+
+    const int si = (int)(e->frame_count % e->slots.size());
+    Slot& s = e->slots[si];
+    {   // wait for the slot's previous frame to leave the host stage
+        std::unique_lock<std::mutex> lk(e->mu);
+        e->cv.wait(lk, [&] { return !s.busy || e->err_code; });
+        if (e->err_code) { set_error(e->err_msg); return e->err_code; }
+    }
+    s.index = e->frame_count; s.ts = ts; s.intra = intra; s.busy = true; s.n_marks = 0;
+    if (e->profiling && !s.marks[0])
+        for (auto& m : s.marks) RIV_HIP(hipEventCreate(&m));
+    auto mark = [&](int stage) {
+        if (!e->profiling || s.n_marks > RIV_NUM_STAGES) return;
+        (void)hipEventRecord(s.marks[s.n_marks], e->st);
+        s.mark_stage[s.n_marks++] = stage;
+    };
 
     const int W = e->W, H = e->H, nb = e->nb;
     const size_t ysz = (size_t)W * H, csz = ysz / 4;
@@ -229,29 +446,27 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
     uint8_t* pY = e->pred.as<uint8_t>(); uint8_t* pU = pY + ysz; uint8_t* pV = pU + csz;
     hipStream_t st = e->st;
 
-    RIV_HIP(hipEventRecord(e->ev0, st));
-    e->n_marks = 0;
+    RIV_HIP(hipEventRecord(s.t0, st));
     const uint8_t* d_rgb = (const uint8_t*)rgb;
     if (!on_device) {
         RIV_HIP(hipMemcpyAsync(e->rgb.p, rgb, len, hipMemcpyHostToDevice, st));
         d_rgb = e->rgb.as<uint8_t>();
     }
-    e->mark(-1);
+    mark(-1);
     rivk::rgb_to_yuv420(st, d_rgb, (int)w, (int)h, W, H, cY, cU, cV);
-    e->mark(0);
-
-    BlockRec* rec = e->rec[e->rec_idx].as<BlockRec>();
+    mark(0);
+    BlockRec* rec = s.rec.as<BlockRec>();
     if (intra) {
-        rivk::intra(st, cY, cU, cV, W, H, e->yt.as<uint16_t>(), e->ct.as<uint16_t>(), nY, nU, nV, e->coeffs.as<int16_t>());
-        e->mark(6);
+        rivk::intra(st, cY, cU, cV, W, H, e->yt.as<uint16_t>(), e->ct.as<uint16_t>(), nY, nU, nV, s.coeffs.as<int16_t>());
+        mark(6);
         rivk::deblock(st, nY, nU, nV, W, H, deblock_level(e->intra_qs), true, nullptr);
-        e->mark(5);
-        RIV_HIP(hipMemcpyAsync(e->h_coeffs.p, e->coeffs.p, 768 * (size_t)nb, hipMemcpyDeviceToHost, st));
-        e->mark(7);
+        mark(5);
+        RIV_HIP(hipMemcpyAsync(s.h_coeffs.p, s.coeffs.p, 768 * (size_t)nb, hipMemcpyDeviceToHost, st));
+        mark(7);
     } else {
         riv::WalkParams p{};
         p.cur = cY; p.ref = rY; p.W = W; p.H = H; p.bw = e->bw; p.bh = e->bh; p.sr = c.search_range;
-        p.prev_mv = e->have_prev_mvs ? (const int8_t*)e->rec[1 - e->rec_idx].p + 4 : nullptr;
+        p.prev_mv = e->prev_inter_slot >= 0 ? (const int8_t*)e->slots[e->prev_inter_slot].rec.p + 4 : nullptr;
         p.prev_stride = sizeof(BlockRec);
         p.rec = rec; p.progress = e->progress.as<unsigned long long>();
         e->epoch++;
@@ -262,108 +477,45 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
         p.zthr = c.me_zero_mv_threshold; p.pthr = c.me_predictor_threshold;
         p.R = e->R; p.err = e->err.as<int>();
         RIV_HIP(rivk::me_walk(st, p));
-        e->mark(1);
+        mark(1);
         rivk::mc(st, rY, rU, rV, W, H, (const int8_t*)rec, sizeof(BlockRec), pY, pU, pV, e->var.as<uint32_t>());
-        e->mark(2);
+        mark(2);
         rivk::aq(st, e->var.as<uint32_t>(), nb, e->inter_qs, e->lnv.as<float>(), e->steps.as<float>());
-        e->mark(3);
+        mark(3);
         rivk::inter_residual(st, cY, cU, cV, pY, pU, pV, W, H, rec, e->steps.as<float>(), c.inter_dead_zone, nY, nU, nV,
-                             e->coeffs.as<int16_t>(), e->optskip.as<uint8_t>());
-        e->mark(4);
-        rivk::filter_mask(st, rec, e->optskip.as<uint8_t>(), nb, e->mask.as<uint8_t>());
+                             s.coeffs.as<int16_t>(), s.optskip.as<uint8_t>());
+        mark(4);
+        rivk::filter_mask(st, rec, s.optskip.as<uint8_t>(), nb, e->mask.as<uint8_t>());
         rivk::deblock(st, nY, nU, nV, W, H, deblock_level(e->inter_qs), false, e->mask.as<uint8_t>());
-        e->mark(5);
-        RIV_HIP(hipMemcpyAsync(e->h_rec.p, rec, sizeof(BlockRec) * nb, hipMemcpyDeviceToHost, st));
-        RIV_HIP(hipMemcpyAsync(e->h_optskip.p, e->optskip.p, nb, hipMemcpyDeviceToHost, st));
-        RIV_HIP(hipMemcpyAsync(e->h_coeffs.p, e->coeffs.p, 768 * (size_t)nb, hipMemcpyDeviceToHost, st));
-        e->mark(7);
+        mark(5);
+        RIV_HIP(hipMemcpyAsync(s.h_rec.p, rec, sizeof(BlockRec) * nb, hipMemcpyDeviceToHost, st));
+        RIV_HIP(hipMemcpyAsync(s.h_optskip.p, s.optskip.p, nb, hipMemcpyDeviceToHost, st));
+        RIV_HIP(hipMemcpyAsync(s.h_coeffs.p, s.coeffs.p, 768 * (size_t)nb, hipMemcpyDeviceToHost, st));
+        mark(7);
     }
     RIV_HIP(hipGetLastError());
-    RIV_HIP(hipEventRecord(e->ev1, st));
-    RIV_HIP(hipStreamSynchronize(st));
+    RIV_HIP(hipEventRecord(s.t1, st));
+    RIV_HIP(hipEventRecord(s.done, st));
+    e->prev_inter_slot = intra ? -1 : si;
+    e->ref_idx = 1 - e->ref_idx;
+    e->have_ref = true;
+    e->frame_count++;
+
+    if (e->depth > 0) {
+        e->pool->submit([e, si] { entropy_job(e, si); });
+        if (stats) { std::memset(stats, 0, sizeof(*stats)); stats->frame_index = s.index; stats->timestamp_ms = ts; stats->frame_type = intra ? 1 : 2; }
+        return RIV_OK;
+    }
+    entropy_job(e, si);
     if (!intra) {
         int werr = 0;
         RIV_HIP(hipMemcpy(&werr, e->err.p, 4, hipMemcpyDeviceToHost));
         if (werr) { set_error("motion-search wavefront timed out"); return RIV_E_GPU; }
     }
-    float gpu_ms = 0;
-    RIV_HIP(hipEventElapsedTime(&gpu_ms, e->ev0, e->ev1));
-    for (int k = 1; k < e->n_marks; k++) {
-        float ms = 0;
-        RIV_HIP(hipEventElapsedTime(&ms, e->sev[k - 1], e->sev[k]));
-        int sidx = e->mark_stage[k];
-        if (sidx >= 0) { e->stage_ms[sidx] += ms; e->stage_n[sidx]++; }
-    }
-
-    auto t0 = std::chrono::steady_clock::now();
-    std::vector<uint8_t> bytes;
-    riv_frame_stats s{};
-    s.frame_index = e->frame_count; s.timestamp_ms = ts;
-    s.storage_width = (uint32_t)W; s.storage_height = (uint32_t)H; s.blocks_total = (uint32_t)nb;
-    const int16_t* hco = e->h_coeffs.as<int16_t>();
-    try {
-        if (intra) {
-            std::vector<uint8_t> coded(nb, 1);
-            std::vector<uint8_t> res = rivh::encode_residuals(hco, coded.data(), nb, true);
-            put_u64(bytes, ts); bytes.push_back(1); bytes.push_back(c.intra_quality);   // PackedFrame intra (format.rs:171-181)
-            put_u32(bytes, (uint32_t)res.size());
-            bytes.insert(bytes.end(), res.begin(), res.end());
-            e->have_prev_mvs = false;
-            e->mvc.reset();
-            s.frame_type = 1; s.residual_bytes = (uint32_t)res.size();
-            e->last_inter = false;
-        } else {
-            const BlockRec* hr = e->h_rec.as<BlockRec>();
-            const uint8_t* os = e->h_optskip.as<uint8_t>();
-            int bdx, bdy;
-            compute_bias(hr, nb, bdx, bdy);
-            std::vector<rivh::MvBlock> blocks(nb);
-            std::vector<uint8_t> coded(nb);
-            for (int i = 0; i < nb; i++) {
-                rivh::MvBlock& b = blocks[i];
-                b.mode = hr[i].mode; b.new_base = hr[i].new_base;
-                b.dx = b.dy = 0; b.sx = b.sy = 0;
-                if (b.mode == riv::M_NEW) {
-                    b.dx = (int8_t)std::max(-128, std::min(127, (int)hr[i].raw_dx - bdx));
-                    b.dy = (int8_t)std::max(-128, std::min(127, (int)hr[i].raw_dy - bdy));
-                    b.sx = hr[i].s[2]; b.sy = hr[i].s[3];
-                    s.mv_new_blocks++;
-                }
-                b.skip = os[i] != 0;
-                coded[i] = !os[i];
-                s.blocks_skipped += b.skip;
-                s.mv_mode_counts[b.mode]++;
-                s.rdo_blocks_selected_skip += hr[i].rdo_skip;
-                if (hr[i].s[2] || hr[i].s[3]) s.rdo_forced_fractional++;
-            }
-            std::vector<uint8_t> mvb = e->mvc.encode(blocks, e->bw, e->bh);
-            std::vector<uint8_t> res = rivh::encode_residuals(hco, coded.data(), nb, false);
-            put_u64(bytes, ts); bytes.push_back(2); bytes.push_back(c.inter_quality);   // PackedFrame inter (format.rs:183-205)
-            bytes.push_back((uint8_t)(int8_t)bdx); bytes.push_back((uint8_t)(int8_t)bdy); bytes.push_back(0);
-            put_u32(bytes, (uint32_t)mvb.size()); bytes.insert(bytes.end(), mvb.begin(), mvb.end());
-            put_u32(bytes, (uint32_t)res.size()); bytes.insert(bytes.end(), res.begin(), res.end());
-            e->have_prev_mvs = true;
-            s.frame_type = 2; s.mv_bytes = (uint32_t)mvb.size(); s.residual_bytes = (uint32_t)res.size();
-            s.mv_bias_dx = bdx; s.mv_bias_dy = bdy;
-            s.rdo_lambda = e->lambda; s.rdo_blocks_evaluated = (uint32_t)nb;
-            if (c.skip_threshold == 0) s.rdo_forced_disabled = (uint32_t)nb;
-            e->rec_idx = 1 - e->rec_idx;
-            e->last_inter = true;
-        }
-    } catch (const std::exception& ex) {
-        set_error(std::string("entropy coding failed: ") + ex.what());
-        return RIV_E_ENCODING_FAILED;
-    }
-    e->ref_idx = 1 - e->ref_idx;
-    e->have_ref = true;
-    int r = e->write(bytes.data(), bytes.size());
-    if (r) { set_error("writer failed"); return r; }
-    e->frame_count++;
-    e->hdr.frame_count = e->frame_count;
-    s.total_bytes = (uint32_t)bytes.size();
-    s.gpu_ms = gpu_ms;
-    s.host_entropy_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    if (stats) *stats = s;
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (e->err_code) { set_error(e->err_msg); return e->err_code; }
+    if (stats && !e->done_stats.empty()) *stats = e->done_stats.back();
+    e->done_stats.clear();
     return RIV_OK;
 }
 
@@ -376,10 +528,35 @@ int riv_encode_frame_device(riv_encoder* e, const void* d_rgb, size_t len, uint3
     return encode_impl(e, d_rgb, true, len, w, h, ts, stats);
 }
 
+int riv_encoder_flush(riv_encoder* e) {
+    if (!e) return RIV_E_ARG;
+    std::unique_lock<std::mutex> lk(e->mu);
+    e->cv.wait(lk, [&] { return e->completed == e->frame_count || e->err_code; });
+    if (e->err_code) { set_error(e->err_msg); return e->err_code; }
+    int werr = 0;
+    lk.unlock();
+    RIV_HIP(hipSetDevice(e->cfg.device_id));
+    RIV_HIP(hipMemcpy(&werr, e->err.p, 4, hipMemcpyDeviceToHost));
+    if (werr) { set_error("motion-search wavefront timed out"); return RIV_E_GPU; }
+    return RIV_OK;
+}
+
+int riv_encoder_pop_stats(riv_encoder* e, riv_frame_stats* out, size_t cap, size_t* n) {
+    if (!e || !n || (!out && cap)) return RIV_E_ARG;
+    std::lock_guard<std::mutex> lk(e->mu);
+    size_t k = 0;
+    while (k < cap && !e->done_stats.empty()) { out[k++] = e->done_stats.front(); e->done_stats.pop_front(); }
+    *n = k;
+    return RIV_OK;
+}
+
 int riv_encoder_finish(riv_encoder* e) {   // encoder.rs:204-221, :950-961
     if (!e) return RIV_E_ARG;
+    int r = riv_encoder_flush(e);
+    if (r) return r;
+    std::lock_guard<std::mutex> lk(e->mu);
     uint64_t end = e->position();
-    int r = e->seek(e->header_pos);
+    r = e->seek(e->header_pos);
     if (!r) r = write_header(e);
     if (!r) r = e->seek(end);
     if (!r && !e->mem && e->wv.flush(e->wuser) != 0) r = RIV_E_IO;
@@ -396,48 +573,41 @@ int riv_encoder_output(const riv_encoder* e, const uint8_t** data, size_t* len) 
     *data = e->membuf.data(); *len = e->membuf.size();
     return RIV_OK;
 }
-void riv_encoder_destroy(riv_encoder* e) {
-    if (!e) return;
-    (void)hipSetDevice(e->cfg.device_id);
-    if (e->st) (void)hipStreamSynchronize(e->st);
-    if (e->ev0) (void)hipEventDestroy(e->ev0);
-    if (e->ev1) (void)hipEventDestroy(e->ev1);
-    for (int i = 0; i <= RIV_NUM_STAGES; i++) if (e->sev[i]) (void)hipEventDestroy(e->sev[i]);
-    if (e->st) (void)hipStreamDestroy(e->st);
-    delete e;
-}
+void riv_encoder_destroy(riv_encoder* e) { destroy_encoder(e); }
 
 int riv_encoder_set_profiling(riv_encoder* e, int on) {
     if (!e) return RIV_E_ARG;
-    RIV_HIP(hipSetDevice(e->cfg.device_id));
-    if (on && !e->sev[0])
-        for (int i = 0; i <= RIV_NUM_STAGES; i++) RIV_HIP(hipEventCreate(&e->sev[i]));
+    std::lock_guard<std::mutex> lk(e->mu);
     e->profiling = on != 0;
     for (int i = 0; i < RIV_NUM_STAGES; i++) { e->stage_ms[i] = 0; e->stage_n[i] = 0; }
     return RIV_OK;
 }
 int riv_encoder_stage_times(riv_encoder* e, double* ms, uint32_t* counts) {
     if (!e || !ms || !counts) return RIV_E_ARG;
+    std::lock_guard<std::mutex> lk(e->mu);
     for (int i = 0; i < RIV_NUM_STAGES; i++) { ms[i] = e->stage_ms[i]; counts[i] = e->stage_n[i]; }
     return RIV_OK;
 }
 
+// ---- last completed frame introspection (meant for the synchronous mode) ----
 int riv_encoder_debug_blocks(riv_encoder* e, int8_t* rec12, int64_t* sad, uint8_t* rdo_skip, uint8_t* opt_skip) {
-    if (!e || !e->last_inter) return RIV_E_ARG;
-    const BlockRec* hr = e->h_rec.as<BlockRec>();
+    if (!e || e->last_slot < 0 || e->slots[e->last_slot].intra) return RIV_E_ARG;
+    const Slot& s = e->slots[e->last_slot];
+    const BlockRec* hr = s.h_rec.as<BlockRec>();
     for (int i = 0; i < e->nb; i++) {
         int8_t* r = rec12 + 12 * i;
         std::memcpy(r, hr[i].s, 4); std::memcpy(r + 4, hr[i].r, 4);
         r[8] = (int8_t)hr[i].mode; r[9] = (int8_t)hr[i].new_base; r[10] = hr[i].raw_dx; r[11] = hr[i].raw_dy;
         if (sad) sad[i] = hr[i].sad;
         if (rdo_skip) rdo_skip[i] = hr[i].rdo_skip;
-        if (opt_skip) opt_skip[i] = e->h_optskip.as<uint8_t>()[i];
+        if (opt_skip) opt_skip[i] = s.h_optskip.as<uint8_t>()[i];
     }
     return RIV_OK;
 }
 int riv_encoder_debug_yuv(riv_encoder* e, int which, uint8_t* y, uint8_t* u, uint8_t* v) {
     if (!e || which < 0 || which > 2) return RIV_E_ARG;
     RIV_HIP(hipSetDevice(e->cfg.device_id));
+    RIV_HIP(hipStreamSynchronize(e->st));
     const size_t ysz = (size_t)e->W * e->H, csz = ysz / 4;
     const uint8_t* src = which == 0 ? e->cur.as<uint8_t>() : which == 1 ? e->pred.as<uint8_t>() : e->ref[e->ref_idx].as<uint8_t>();
     RIV_HIP(hipMemcpy(y, src, ysz, hipMemcpyDeviceToHost));
@@ -446,19 +616,21 @@ int riv_encoder_debug_yuv(riv_encoder* e, int which, uint8_t* y, uint8_t* u, uin
     return RIV_OK;
 }
 int riv_encoder_debug_coeffs(riv_encoder* e, int16_t* out) {
-    if (!e) return RIV_E_ARG;
-    const int16_t* hco = e->h_coeffs.as<int16_t>();
-    const uint8_t* os = e->h_optskip.as<uint8_t>();
+    if (!e || e->last_slot < 0) return RIV_E_ARG;
+    const Slot& s = e->slots[e->last_slot];
+    const int16_t* hco = s.h_coeffs.as<int16_t>();
+    const uint8_t* os = s.h_optskip.as<uint8_t>();
     for (int b = 0; b < e->nb; b++) {
-        bool zero = e->last_inter && os[b];
+        bool zero = !s.intra && os[b];
         if (zero) std::memset(out + (size_t)b * 384, 0, 768);
         else std::memcpy(out + (size_t)b * 384, hco + (size_t)b * 384, 768);
     }
     return RIV_OK;
 }
 int riv_encoder_debug_aq(riv_encoder* e, float* steps) {
-    if (!e || !e->last_inter) return RIV_E_ARG;
+    if (!e || e->last_slot < 0 || e->slots[e->last_slot].intra) return RIV_E_ARG;
     RIV_HIP(hipSetDevice(e->cfg.device_id));
+    RIV_HIP(hipStreamSynchronize(e->st));
     RIV_HIP(hipMemcpy(steps, e->steps.p, 4 * (size_t)e->nb, hipMemcpyDeviceToHost));
     return RIV_OK;
 }

@@ -78,7 +78,7 @@ __global__ void k_yuv420_to_rgb(const uint8_t* __restrict__ Y, const uint8_t* __
 // candidates packed per 32-bit word; a 16x16 SAD is <= 65280 so the halves never carry).
 namespace {
 
-struct Cand { int n; int dx[6]; int dy[6]; };
+struct Cand { int n; unsigned valid; int dx[6]; int dy[6]; };   // fixed slots; invalid slots hold (0,0)
 
 __device__ __forceinline__ int median3(int a, int b, int c) {   // motion/scalar.rs:24-43
     if (a > b) { if (b > c) return b; else if (a > c) return c; else return a; }
@@ -111,11 +111,11 @@ struct CostCtx {
 // Integer SAD with the reference's per-4x4 edge handling (satd_scalar.rs:4-99: each 4x4 sub-
 // block's reference origin is clamped to [0,W-4] x [0,H-4] as a unit). Evaluates c.n <= 6
 // candidates; returns the SADs in out[].
-__device__ __forceinline__ void int_sads(const uint32_t* win32, int WSP, uint32_t cur4, int lane, int x0, int y0, int W, int H,
+__device__ __forceinline__ void int_sads(const uint32_t* win32, int WSP, int xo, uint32_t cur4, int lane, int x0, int y0, int W, int H,
                                          int sr, const Cand& c, int* out) {
     const int g = lane & 3, j = lane >> 2;
     const int sxo = x0 + 4 * g, syo = y0 + 4 * (j >> 2), jr = j & 3;
-    const int wx0 = x0 - sr - 1, wy0 = y0 - sr - 1;
+    const int wx0 = x0 - sr - 1 - xo, wy0 = y0 - sr - 1;
 #pragma unroll
     for (int k = 0; k < 6; k += 2) {
         if (k < c.n) {
@@ -136,9 +136,9 @@ __device__ __forceinline__ void int_sads(const uint32_t* win32, int WSP, uint32_
 // interpolation (satd_scalar.rs:113-166). The per-pixel clamp of the reference's sampler
 // equals plain interpolation on the replicate-padded window (averaging replicated samples
 // returns the edge sample), so no clamping is needed here.
-__device__ __forceinline__ uint32_t hp_ref4(const uint32_t* win32, int WSP, int lane, int sr, int hx, int hy) {
+__device__ __forceinline__ uint32_t hp_ref4(const uint32_t* win32, int WSP, int xo, int lane, int sr, int hx, int hy) {
     const int g = lane & 3, j = lane >> 2;
-    int xs = 4 * g + (hx >> 1) + sr + 1, ys = j + (hy >> 1) + sr + 1;
+    int xs = 4 * g + (hx >> 1) + sr + 1 + xo, ys = j + (hy >> 1) + sr + 1;
     int ax = hx & 1, ay = hy & 1;
     uint32_t a = lds_read4(win32, ys * WSP + xs);
     if (!ax && !ay) return a;
@@ -147,47 +147,50 @@ __device__ __forceinline__ uint32_t hp_ref4(const uint32_t* win32, int WSP, int 
     return avg4f(a, lds_read4(win32, ys * WSP + xs + 1), lds_read4(win32, (ys + 1) * WSP + xs), lds_read4(win32, (ys + 1) * WSP + xs + 1));
 }
 
-__device__ __forceinline__ bool q4eq(const int8_t* a, const int8_t* b) { return a[0] == b[0] && a[1] == b[1] && a[2] == b[2] && a[3] == b[3]; }
-__device__ __forceinline__ void q4cpy(int8_t* d, const int8_t* s) { d[0] = s[0]; d[1] = s[1]; d[2] = s[2]; d[3] = s[3]; }
+__device__ __forceinline__ int mvx(uint32_t w) { return (int)(int8_t)(w & 0xff); }
+__device__ __forceinline__ int mvy(uint32_t w) { return (int)(int8_t)((w >> 8) & 0xff); }
 __device__ __forceinline__ int mv_class(int mag) {   // mv_class_from_magnitude (mv_rans.rs:81-87)
     if (mag == 0) return 0;
     return min(32 - __clz(mag), 10);
 }
 
-template <int NPF>
+// Window geometry: LDS row r holds reference row clamp(wy0 + r) over the dword-aligned column
+// span starting at wxa = floor4(wx0) (NDW dwords, pitch NDW+1 dwords). Frame widths are multiples
+// of 16, so each aligned dword is either fully inside the row or fully outside it; outside
+// dwords replicate the edge sample, which makes the LDS copy the replicate-padded window.
+template <int NPF, int NDW>
 __device__ __forceinline__ void load_window(const WalkParams& p, int bx, int by, int lane, int WS, uint32_t* pf, uint32_t& pcur) {
-    const int x0 = bx * 16, y0 = by * 16, wx0 = x0 - p.sr - 1, wy0 = y0 - p.sr - 1;
-    const int tot = WS * WS;
+    const int x0 = bx * 16, y0 = by * 16, wx0 = x0 - p.sr - 1, wy0 = y0 - p.sr - 1, wxa = wx0 & ~3;
 #pragma unroll
     for (int k = 0; k < NPF; k++) {
-        int e = lane + 64 * k;
-        if (e < tot) {
-            int wy = e / WS, wx = e - wy * WS;
-            int yy = dclamp(wy0 + wy, 0, p.H - 1), xx = dclamp(wx0 + wx, 0, p.W - 1);
-            pf[k] = p.ref[(size_t)yy * p.W + xx];
+        const int e = lane + 64 * k, r = e / NDW, c = e % NDW;
+        if (r < WS) {
+            const int gy = dclamp(wy0 + r, 0, p.H - 1), gx = wxa + 4 * c;
+            const uint8_t* row = p.ref + (size_t)gy * p.W;
+            if (gx >= 0 && gx < p.W) pf[k] = *(const uint32_t*)(row + gx);
+            else pf[k] = 0x01010101u * (gx < 0 ? row[0] : row[p.W - 1]);
         }
     }
     pcur = *(const uint32_t*)(p.cur + (size_t)(y0 + (lane >> 2)) * p.W + x0 + 4 * (lane & 3));
 }
-template <int NPF>
-__device__ __forceinline__ void store_window(uint8_t* win, uint8_t* curb, int lane, int WS, int WSP, const uint32_t* pf, uint32_t pcur) {
-    const int tot = WS * WS;
+template <int NPF, int NDW>
+__device__ __forceinline__ void store_window(uint32_t* win32, uint32_t* cur32, int lane, int WS, const uint32_t* pf, uint32_t pcur) {
 #pragma unroll
     for (int k = 0; k < NPF; k++) {
-        int e = lane + 64 * k;
-        if (e < tot) { int wy = e / WS, wx = e - wy * WS; win[wy * WSP + wx] = (uint8_t)pf[k]; }
+        const int e = lane + 64 * k, r = e / NDW, c = e % NDW;
+        if (r < WS) win32[r * (NDW + 1) + c] = pf[k];
     }
-    ((uint32_t*)curb)[lane] = pcur;
+    cur32[lane] = pcur;
 }
 
 }  // namespace
 
-template <int NPF>
+template <int NPF, int NDW>
 __global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
     extern __shared__ __align__(16) uint8_t lds[];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int R = p.R, r0 = blockIdx.x * R, row = r0 + w, bw = p.bw, sr = p.sr;
-    const int WS = 16 + 2 * sr + 2, WSP = (WS + 4 + 3) & ~3, WINB = (WS * WSP + 15) & ~15;
+    const int WS = 16 + 2 * sr + 2, WSP = 4 * (NDW + 1), WINB = (WS * WSP + 15) & ~15;
     uint8_t* wbase = lds + w * (2 * WINB + 512);
     uint8_t* winb0 = wbase;
     uint8_t* curb0 = wbase + 2 * WINB;
@@ -200,22 +203,24 @@ __global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
     uint32_t pf[NPF];
     uint32_t pcur = 0;
     if (row_valid) {
-        load_window<NPF>(p, 0, row, lane, WS, pf, pcur);
-        store_window<NPF>(winb0, curb0, lane, WS, WSP, pf, pcur);
+        load_window<NPF, NDW>(p, 0, row, lane, WS, pf, pcur);
+        store_window<NPF, NDW>((uint32_t*)winb0, (uint32_t*)curb0, lane, WS, pf, pcur);
     }
     __syncthreads();
 
-    int8_t lS[4] = {0, 0, 0, 0}, lR[4] = {0, 0, 0, 0};   // left neighbour (search, reconstructed)
+    // MVs are carried packed: bytes (dx, dy, sx, sy) of one 32-bit word
+    uint32_t lS = 0, lR = 0;   // left neighbour (search, reconstructed)
     const int nsteps = bw + 2 * (R - 1);
     for (int s = 0; s < nsteps; s++) {
         const int bx = s - 2 * w;
         if (row_valid && bx >= 0 && bx < bw) {
             const int idx = row * bw + bx, x0 = bx * 16, y0 = row * 16;
             const bool has_l = bx > 0, has_t = row > 0, has_tr = has_t && bx + 1 < bw, has_tl = has_t && bx > 0;
-            // ---- top neighbours (search + reconstructed MVs) ----
-            int8_t tS[3][4] = {{0}}, tR[3][4] = {{0}};   // [0] = top-left, [1] = top, [2] = top-right
+            // ---- top neighbours (search + reconstructed MVs): top-left, top, top-right ----
+            uint32_t tlS = 0, tlR = 0, tS = 0, tR = 0, trS = 0, trR = 0;
             bool timed_out = false;
             if (has_t) {
+                uint64_t v0 = 0, v1 = 0, v2 = 0;
                 if (w == 0) {
                     unsigned long long need = p.base + (unsigned long long)min(bx + 2, bw);
                     unsigned spins = 0;
@@ -224,46 +229,39 @@ __global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
                         __builtin_amdgcn_s_sleep(2);
                     }
                     if (!timed_out) {
-#pragma unroll
-                        for (int k = 0; k < 3; k++) {
-                            int xx = bx - 1 + k;
-                            if (xx >= 0 && xx < bw) {
-                                const int8_t* q = (const int8_t*)&p.rec[(row - 1) * bw + xx];
-                                uint64_t v = *(const uint64_t*)q;
-#pragma unroll
-                                for (int c = 0; c < 4; c++) { tS[k][c] = (int8_t)(v >> (8 * c)); tR[k][c] = (int8_t)(v >> (32 + 8 * c)); }
-                            }
-                        }
+                        const uint64_t* q = (const uint64_t*)&p.rec[(row - 1) * bw + bx];   // 16-byte records
+                        if (bx > 0) v0 = q[-2];
+                        v1 = q[0];
+                        if (bx + 1 < bw) v2 = q[2];
                     }
                 } else {
-#pragma unroll
-                    for (int k = 0; k < 3; k++) {
-                        int xx = bx - 1 + k;
-                        if (xx >= 0 && xx < bw) {
-                            const int8_t* q = rowbuf + ((w - 1) * bw + xx) * 8;
-                            q4cpy(tS[k], q); q4cpy(tR[k], q + 4);
-                        }
-                    }
+                    const uint64_t* q = (const uint64_t*)(rowbuf + ((w - 1) * bw + bx) * 8);
+                    if (bx > 0) v0 = q[-1];
+                    v1 = q[0];
+                    if (bx + 1 < bw) v2 = q[1];
                 }
+                tlS = (uint32_t)v0; tlR = (uint32_t)(v0 >> 32);
+                tS = (uint32_t)v1; tR = (uint32_t)(v1 >> 32);
+                trS = (uint32_t)v2; trR = (uint32_t)(v2 >> 32);
             }
             if (timed_out) {
                 if (lane == 0) { *abortf = 1; atomicExch(p.err, 1); }
             } else {
-                int8_t pT[4] = {0, 0, 0, 0};
                 const bool has_tp = p.prev_mv != nullptr;
-                if (has_tp) { uint32_t v = *(const uint32_t*)(p.prev_mv + (size_t)p.prev_stride * idx); for (int c = 0; c < 4; c++) pT[c] = (int8_t)(v >> (8 * c)); }
+                const uint32_t pT = has_tp ? *(const uint32_t*)(p.prev_mv + (size_t)p.prev_stride * idx) : 0u;
                 // prefetch the next block's window while this block is walked
                 const bool pf_next = bx + 1 < bw;
-                if (pf_next) load_window<NPF>(p, bx + 1, row, lane, WS, pf, pcur);
+                if (pf_next) load_window<NPF, NDW>(p, bx + 1, row, lane, WS, pf, pcur);
 
                 const uint32_t* win32 = (const uint32_t*)(winb0 + (bx & 1) * WINB);
+                const int xo = (x0 - sr - 1) - ((x0 - sr - 1) & ~3);   // byte offset of window column 0
                 const uint32_t cur4 = ((const uint32_t*)(curb0 + (bx & 1) * 256))[lane];
 
                 // ---- ME predictors: gather_local_motion_candidates (motion/scalar.rs:56-156) ----
                 // slots in predictor order: left, top, top-right, top-left, temporal
                 const bool pv[5] = {has_l, has_t, has_tr, has_tl, has_tp};
-                const int pdx[5] = {lS[0], tS[1][0], tS[2][0], tS[0][0], pT[0]};
-                const int pdy[5] = {lS[1], tS[1][1], tS[2][1], tS[0][1], pT[1]};
+                const int pdx[5] = {mvx(lS), mvx(tS), mvx(trS), mvx(tlS), mvx(pT)};
+                const int pdy[5] = {mvy(lS), mvy(tS), mvy(trS), mvy(tlS), mvy(pT)};
                 int ux0 = 0, uy0 = 0, ux1 = 0, uy1 = 0, nu = 0;
 #pragma unroll
                 for (int i = 0; i < 5; i++) {
@@ -275,30 +273,40 @@ __global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
                 CostCtx cc;
                 cc.nx = ux0; cc.ny = uy0; cc.rx = ux1; cc.ry = uy1;
                 {
-                    int lx = has_l ? lS[0] : 0, ly = has_l ? lS[1] : 0;
-                    int tx = has_t ? tS[1][0] : 0, ty = has_t ? tS[1][1] : 0;
-                    int trx = has_tr ? tS[2][0] : (has_tl ? tS[0][0] : 0), try_ = has_tr ? tS[2][1] : (has_tl ? tS[0][1] : 0);
+                    int lx = has_l ? mvx(lS) : 0, ly = has_l ? mvy(lS) : 0;
+                    int tx = has_t ? mvx(tS) : 0, ty = has_t ? mvy(tS) : 0;
+                    int trx = has_tr ? mvx(trS) : (has_tl ? mvx(tlS) : 0), try_ = has_tr ? mvy(trS) : (has_tl ? mvy(tlS) : 0);
                     cc.mx = median3(lx, tx, trx); cc.my = median3(ly, ty, try_);
                 }
                 cc.bits = bits; cc.lam2 = p.lam2; cc.c_zero = p.c_zero; cc.c_pred = p.c_pred; cc.lam_zero = p.lam_zero;
+                // push_candidate (motion/scalar.rs:46-53): (0,0) first, then every in-range predictor
+                // not already listed; built as a select network so no private array is indexed at run time
                 Cand c;
-                c.n = 1; c.dx[0] = 0; c.dy[0] = 0;
+                {
+                    bool inc[5];
+                    int slot[5], cnt = 1;
 #pragma unroll
-                for (int i = 0; i < 5; i++) {   // push_candidate (motion/scalar.rs:46-53)
-                    if (pv[i] && abs(pdx[i]) <= sr && abs(pdy[i]) <= sr) {
-                        bool dup = false;
+                    for (int i = 0; i < 5; i++) {
+                        bool ok = pv[i] && abs(pdx[i]) <= sr && abs(pdy[i]) <= sr && !(pdx[i] == 0 && pdy[i] == 0);
 #pragma unroll
-                        for (int q = 0; q < 6; q++) if (q < c.n) dup |= (c.dx[q] == pdx[i] && c.dy[q] == pdy[i]);
-                        if (!dup) {
+                        for (int j = 0; j < i; j++) ok = ok && !(inc[j] && pdx[j] == pdx[i] && pdy[j] == pdy[i]);
+                        inc[i] = ok;
+                        slot[i] = cnt;
+                        cnt += ok ? 1 : 0;
+                    }
+                    c.n = cnt; c.valid = 0x3f;
+                    c.dx[0] = 0; c.dy[0] = 0;
 #pragma unroll
-                            for (int q = 1; q < 6; q++) if (q == c.n) { c.dx[q] = pdx[i]; c.dy[q] = pdy[i]; }
-                            c.n++;
-                        }
+                    for (int q = 1; q < 6; q++) {
+                        int vx = 0, vy = 0;
+#pragma unroll
+                        for (int i = 0; i < 5; i++) { bool hit = inc[i] && slot[i] == q; vx = hit ? pdx[i] : vx; vy = hit ? pdy[i] : vy; }
+                        c.dx[q] = vx; c.dy[q] = vy;
                     }
                 }
                 // ---- process_block_hex (motion/scalar.rs:546-636) ----
                 int sads[6];
-                int_sads(win32, WSP, cur4, lane, x0, y0, p.W, p.H, sr, c, sads);
+                int_sads(win32, WSP, xo, cur4, lane, x0, y0, p.W, p.H, sr, c, sads);
                 const int zero_sad = sads[0];
                 int sdx = 0, sdy = 0;
                 long long scost = (long long)zero_sad + cc.cost(0, 0);
@@ -318,19 +326,21 @@ __global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
                 if (!skip_int) {   // refine_integer_motion (:680-757): large hexagon, then small diamond
                     const int HX[6] = {2, 1, -1, -2, -1, 1}, HY[6] = {0, 2, 2, 0, -2, -2};
                     for (int it = 0; it < 4096; it++) {
-                        Cand h; h.n = 0;
+                        Cand h; h.n = 6; h.valid = 0;
                         const int cx = sdx, cy = sdy;
 #pragma unroll
                         for (int k = 0; k < 6; k++) {
                             int dx = cx + HX[k], dy = cy + HY[k];
-                            if (abs(dx) <= sr && abs(dy) <= sr) { h.dx[h.n] = dx; h.dy[h.n] = dy; h.n++; }
+                            bool ok = abs(dx) <= sr && abs(dy) <= sr;
+                            h.valid |= ok ? 1u << k : 0u;
+                            h.dx[k] = ok ? dx : 0; h.dy[k] = ok ? dy : 0;
                         }
                         int hs[6];
-                        int_sads(win32, WSP, cur4, lane, x0, y0, p.W, p.H, sr, h, hs);
+                        int_sads(win32, WSP, xo, cur4, lane, x0, y0, p.W, p.H, sr, h, hs);
                         bool improved = false;
 #pragma unroll
                         for (int k = 0; k < 6; k++) {
-                            if (k < h.n) {
+                            if ((h.valid >> k) & 1) {
                                 int mc = cc.cost(h.dx[k], h.dy[k]);
                                 long long t = (long long)hs[k] + mc;
                                 if (scost - mc >= 0 && t < scost) { sdx = h.dx[k]; sdy = h.dy[k]; scost = t; improved = true; }
@@ -339,18 +349,20 @@ __global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
                         if (!improved) break;
                     }
                     const int DX[4] = {1, -1, 0, 0}, DY[4] = {0, 0, 1, -1};
-                    Cand d; d.n = 0;
+                    Cand d; d.n = 4; d.valid = 0;
                     const int cx = sdx, cy = sdy;
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
                         int dx = cx + DX[k], dy = cy + DY[k];
-                        if (abs(dx) <= sr && abs(dy) <= sr) { d.dx[d.n] = dx; d.dy[d.n] = dy; d.n++; }
+                        bool ok = abs(dx) <= sr && abs(dy) <= sr;
+                        d.valid |= ok ? 1u << k : 0u;
+                        d.dx[k] = ok ? dx : 0; d.dy[k] = ok ? dy : 0;
                     }
                     int ds[6];
-                    int_sads(win32, WSP, cur4, lane, x0, y0, p.W, p.H, sr, d, ds);
+                    int_sads(win32, WSP, xo, cur4, lane, x0, y0, p.W, p.H, sr, d, ds);
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
-                        if (k < d.n) {
+                        if ((d.valid >> k) & 1) {
                             int mc = cc.cost(d.dx[k], d.dy[k]);
                             long long t = (long long)ds[k] + mc;
                             if (scost - mc >= 0 && t < scost) { sdx = d.dx[k]; sdy = d.dy[k]; scost = t; }
@@ -363,8 +375,8 @@ __global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
                 int hsad[9];
 #pragma unroll
                 for (int k = 0; k < 9; k += 2) {
-                    uint32_t pk = sad4(cur4, hp_ref4(win32, WSP, lane, sr, 2 * sdx + ordx[k], 2 * sdy + ordy[k]), 0u);
-                    if (k + 1 < 9) pk |= sad4(cur4, hp_ref4(win32, WSP, lane, sr, 2 * sdx + ordx[k + 1], 2 * sdy + ordy[k + 1]), 0u) << 16;
+                    uint32_t pk = sad4(cur4, hp_ref4(win32, WSP, xo, lane, sr, 2 * sdx + ordx[k], 2 * sdy + ordy[k]), 0u);
+                    if (k + 1 < 9) pk |= sad4(cur4, hp_ref4(win32, WSP, xo, lane, sr, 2 * sdx + ordx[k + 1], 2 * sdy + ordy[k + 1]), 0u) << 16;
                     uint32_t t = wave_sum(pk);
                     hsad[k] = (int)(t & 0xffff);
                     if (k + 1 < 9) hsad[k + 1] = (int)(t >> 16);
@@ -399,17 +411,19 @@ __global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
                 auto pack = [](int a, int b, int c, int d) -> uint32_t {
                     return (uint32_t)(uint8_t)a | ((uint32_t)(uint8_t)b << 8) | ((uint32_t)(uint8_t)c << 16) | ((uint32_t)(uint8_t)d << 24);
                 };
-                auto pk8 = [&](const int8_t* q) -> uint32_t { return pack(q[0], q[1], q[2], q[3]); };
-                const uint32_t wL = pk8(lR), wT = pk8(tR[1]), wTR = pk8(tR[2]), wTL = pk8(tR[0]), wP = pk8(pT);
+                const uint32_t wL = lR, wT = tR, wTR = trR, wTL = tlR, wP = pT;
                 uint32_t c0 = 0, c1 = 0; int ncand = 0;
-                auto addc = [&](bool present, uint32_t q) {
-                    if (!present) return;
-                    if (ncand >= 1 && c0 == q) return;
-                    if (ncand >= 2 && c1 == q) return;
-                    if (ncand == 0) c0 = q; else if (ncand == 1) c1 = q;
-                    ncand++;   // later distinct entries only matter for the count
-                };
-                addc(has_l, wL); addc(has_t, wT); addc(has_tr, wTR); addc(has_tl, wTL); addc(has_tp, wP);
+                // first two distinct present entries, selects only (no indexed storage)
+#define RIV_ADDC(present, q)                                                         \
+                {                                                                    \
+                    const uint32_t q_ = (q);                                         \
+                    const bool add_ = (present) && !(ncand >= 1 && c0 == q_) && !(ncand >= 2 && c1 == q_); \
+                    c1 = (add_ && ncand == 1) ? q_ : c1;                             \
+                    c0 = (add_ && ncand == 0) ? q_ : c0;                             \
+                    ncand += add_ ? 1 : 0;                                           \
+                }
+                RIV_ADDC(has_l, wL) RIV_ADDC(has_t, wT) RIV_ADDC(has_tr, wTR) RIV_ADDC(has_tl, wTL) RIV_ADDC(has_tp, wP)
+#undef RIV_ADDC
                 const uint32_t nearest = ncand > 0 ? c0 : 0u, nearp = ncand > 1 ? c1 : 0u, temporal = has_tp ? wP : 0u;
                 auto hpx = [](uint32_t q) { return (int)(int8_t)(q & 0xff) * 2 + (int)(int8_t)((q >> 16) & 0xff); };
                 auto hpy = [](uint32_t q) { return (int)(int8_t)((q >> 8) & 0xff) * 2 + (int)(int8_t)(q >> 24); };
@@ -464,9 +478,8 @@ __global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
                     rb[0] = w0; rb[1] = w1;
                     __hip_atomic_store(&p.progress[row], p.base + (unsigned long long)(bx + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 }
-                lS[0] = (int8_t)S0; lS[1] = (int8_t)S1; lS[2] = (int8_t)S2; lS[3] = (int8_t)S3;
-                lR[0] = (int8_t)(wR & 0xff); lR[1] = (int8_t)((wR >> 8) & 0xff); lR[2] = (int8_t)((wR >> 16) & 0xff); lR[3] = (int8_t)(wR >> 24);
-                if (pf_next) store_window<NPF>(winb0 + ((bx + 1) & 1) * WINB, curb0 + ((bx + 1) & 1) * 256, lane, WS, WSP, pf, pcur);
+                lS = wS; lR = wR;
+                if (pf_next) store_window<NPF, NDW>((uint32_t*)(winb0 + ((bx + 1) & 1) * WINB), (uint32_t*)(curb0 + ((bx + 1) & 1) * 256), lane, WS, pf, pcur);
             }
         }
         __syncthreads();
@@ -550,18 +563,30 @@ __global__ __launch_bounds__(256) void k_mc(const uint8_t* __restrict__ rY, cons
 __global__ __launch_bounds__(1024) void k_aq(const uint32_t* __restrict__ var_sums, int nb, float base, float* __restrict__ lnv,
                                              float* __restrict__ steps) {
     __shared__ float s_avg;
+    __shared__ float4 s_ln[2048];   // 8192 logs per LDS pass
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
         uint64_t s = var_sums[2 * b], s2 = var_sums[2 * b + 1];
         float var = (float)(s2 * 256ull - s * s) / 65536.0f;
         var = fmaxf(var, 4.0f);
         lnv[b] = glibc_logf(var);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float sum = 0.0f;
-        for (int b = 0; b < nb; b++) sum = __fadd_rn(sum, lnv[b]);
-        s_avg = __fdiv_rn(sum, (float)nb);
+    float sum = 0.0f;   // lives in thread 0
+    for (int c0 = 0; c0 < nb; c0 += 8192) {
+        __syncthreads();
+        const int cn = min(8192, nb - c0);
+        float* sl = (float*)s_ln;
+        for (int i = threadIdx.x; i < cn; i += blockDim.x) sl[i] = lnv[c0 + i];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int i = 0;
+            for (; i + 4 <= cn; i += 4) {
+                float4 q = s_ln[i >> 2];
+                sum = __fadd_rn(sum, q.x); sum = __fadd_rn(sum, q.y); sum = __fadd_rn(sum, q.z); sum = __fadd_rn(sum, q.w);
+            }
+            for (; i < cn; i++) sum = __fadd_rn(sum, sl[i]);
+        }
     }
+    if (threadIdx.x == 0) s_avg = __fdiv_rn(sum, (float)nb);
     __syncthreads();
     const float avg = s_avg;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
@@ -985,20 +1010,23 @@ void yuv420_to_rgb(hipStream_t st, const uint8_t* Y, const uint8_t* U, const uin
     hipLaunchKernelGGL(k_yuv420_to_rgb, g, dim3(256), 0, st, Y, U, V, sw, dw, dh, rgb);
 }
 
+// dwords per LDS window row: the aligned span of 16 + 2*sr + 2 bytes starting anywhere in a dword
+static int walk_ndw(int sr) { return (16 + 2 * sr + 2 + 3 + 3) / 4 <= 16 ? 16 : 32; }
 size_t walk_lds_bytes(int sr, int bw, int R) {
-    int WS = 16 + 2 * sr + 2, WSP = (WS + 4 + 3) & ~3, WINB = (WS * WSP + 15) & ~15;
+    int WS = 16 + 2 * sr + 2, WSP = 4 * (walk_ndw(sr) + 1), WINB = (WS * WSP + 15) & ~15;
     return (size_t)R * (2 * WINB + 512) + (size_t)((R * bw * 8 + 15) & ~15) + 256 * sizeof(double) + 16;
 }
 int walk_npf(int sr) {
     int WS = 16 + 2 * sr + 2;
-    return (WS * WS + 63) / 64;
+    return (WS * walk_ndw(sr) + 63) / 64;
 }
 hipError_t me_walk(hipStream_t st, const WalkParams& p) {
     int nbands = (p.bh + p.R - 1) / p.R;
     size_t lds = walk_lds_bytes(p.sr, p.bw, p.R);
-    int npf = walk_npf(p.sr);
-    if (npf <= 40) hipLaunchKernelGGL(k_me_walk<40>, dim3(nbands), dim3(64 * p.R), lds, st, p);
-    else if (npf <= 106) hipLaunchKernelGGL(k_me_walk<106>, dim3(nbands), dim3(64 * p.R), lds, st, p);
+    const int npf = walk_npf(p.sr), ndw = walk_ndw(p.sr);
+    if (ndw == 16 && npf <= 11) hipLaunchKernelGGL((k_me_walk<11, 16>), dim3(nbands), dim3(64 * p.R), lds, st, p);
+    else if (ndw == 16) hipLaunchKernelGGL((k_me_walk<14, 16>), dim3(nbands), dim3(64 * p.R), lds, st, p);
+    else if (npf <= 41) hipLaunchKernelGGL((k_me_walk<41, 32>), dim3(nbands), dim3(64 * p.R), lds, st, p);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

@@ -76,3 +76,30 @@ def test_search_range_and_quality_variants():
     for sr, qi, qp, sk in ((4, 60, 40, 3), (24, 95, 90, 0), (16, 30, 10, 12)):
         data = _run_pair(frames, 320, 176, sr=sr, q_intra=qi, q_inter=qp, skip=sk, label=f"sr{sr}")
         _decode_pair(data, len(frames), label=f"sr{sr}")
+
+
+def test_pipelined_encoder_same_bitstream():
+    frames = synth.Sequence(synth.cif_params(seed=11, frames=12)).frames()
+    outs = []
+    for depth in (0, 1, 3, 8):
+        cfg = _cfg(352, 288).with_pipeline_depth(depth)
+        if depth == 0:
+            cfg = cfg.with_keyframe_interval(riv2_amd.KeyframeInterval.fixed(5))
+        else:
+            cfg = cfg.with_keyframe_interval(riv2_amd.KeyframeInterval.fixed(5))
+        enc = riv2_amd.Encoder(cfg)
+        for i, f in enumerate(frames):
+            print(f"[pipe{depth}] frame {i}", flush=True)
+            enc.encode_frame(riv2_amd.Frame(f, 352, 288, i))
+        enc.finish()
+        stats = enc.pop_stats()
+        if depth:
+            assert [s.frame_index for s in stats] == list(range(len(frames)))
+        outs.append(enc.output())
+        enc.close()
+    ref = oracle.OracleEncoder(oracle.config(352, 288, keyframe_interval=5))
+    for i, f in enumerate(frames):
+        ref.encode(f, i)
+    want = ref.finish()
+    for o in outs:
+        assert o == want
