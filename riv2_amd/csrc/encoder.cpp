@@ -83,9 +83,9 @@ struct riv_encoder {
     riv_video_header hdr{};
     uint64_t frame_count = 0;      // frames submitted
 
-    int W = 0, H = 0, bw = 0, bh = 0, nb = 0, R = 4;
+    int W = 0, H = 0, bw = 0, bh = 0, nb = 0, R = 4, hp_slots = 2;
     hipStream_t st = nullptr;
-    DevBuf rgb, cur, pred, ref[2], var, lnv, steps, mask, progress, bits, rdo, err, yt, ct;
+    DevBuf rgb, cur, pred, ref[2], var, lnv, steps, mask, pub, sadt, hpt, bits, rdo, err, yt, ct;
     int ref_idx = 0;
     bool have_ref = false;
     int prev_inter_slot = -1;      // slot holding the previous P-frame's records, -1 after an intra frame
@@ -170,8 +170,19 @@ static int encoder_init(riv_encoder* e) {
     if (r) return r;
     e->W = (int)c.storage_width; e->H = (int)c.storage_height;
     e->bw = e->W / 16; e->bh = e->H / 16; e->nb = e->bw * e->bh;
-    e->R = 4;   // MB rows per wavefront workgroup: as many as fit the LDS budget, at most 4
-    while (e->R > 1 && rivk::walk_lds_bytes(c.search_range, e->bw, e->R) > 60000) e->R--;
+    // wavefront geometry: up to 4 MB rows per workgroup and a half-pel table prefetched a block
+    // ahead, as far as the 64 KiB of LDS of one workgroup allow
+    {
+        const size_t budget = 65536;
+        e->hp_slots = 2;
+        if (rivk::walk_lds_bytes(c.search_range, e->bw, 1, 2) > budget) e->hp_slots = 1;
+        e->R = 4;
+        while (e->R > 1 && rivk::walk_lds_bytes(c.search_range, e->bw, e->R, e->hp_slots) > budget) e->R--;
+        if (rivk::walk_lds_bytes(c.search_range, e->bw, e->R, e->hp_slots) > budget) {
+            set_error("frame too wide for the motion-search workgroup's LDS at this search range");
+            return RIV_E_UNSUPPORTED;
+        }
+    }
     const size_t ysz = (size_t)e->W * e->H, csz = ysz / 4, fsz = ysz + 2 * csz;
     RIV_HIP(hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking));
     RIV_HIP(e->rgb.alloc((size_t)c.display_width * c.display_height * 3));
@@ -179,7 +190,9 @@ static int encoder_init(riv_encoder* e) {
     RIV_HIP(e->ref[0].alloc(fsz)); RIV_HIP(e->ref[1].alloc(fsz));
     RIV_HIP(e->var.alloc(8 * (size_t)e->nb)); RIV_HIP(e->lnv.alloc(4 * (size_t)e->nb)); RIV_HIP(e->steps.alloc(4 * (size_t)e->nb));
     RIV_HIP(e->mask.alloc(e->nb));
-    RIV_HIP(e->progress.alloc(8 * (size_t)e->bh)); RIV_HIP(hipMemset(e->progress.p, 0, 8 * (size_t)e->bh));
+    RIV_HIP(e->pub.alloc(8 * (size_t)e->nb)); RIV_HIP(hipMemset(e->pub.p, 0, 8 * (size_t)e->nb));
+    RIV_HIP(e->sadt.alloc(2 * (size_t)e->nb * rivk::sad_table_stride(c.search_range)));
+    RIV_HIP(e->hpt.alloc(2 * (size_t)e->nb * rivk::hp_table_stride(c.search_range)));
     RIV_HIP(e->err.alloc(4)); RIV_HIP(hipMemset(e->err.p, 0, 4));
     e->depth = c.pipeline_depth;
     e->slots.resize(std::max(2, e->depth));
@@ -201,6 +214,12 @@ static int encoder_init(riv_encoder* e) {
     RIV_HIP(e->bits.alloc(sizeof(bitsv)));
     RIV_HIP(hipMemcpy(e->bits.p, bitsv, sizeof(bitsv), hipMemcpyHostToDevice));
     const double lam2 = e->lambda * 2.0;
+    // every MV cost is lam2 * (6 + bits_x + bits_y) <= lam2 * 42; the kernels keep SAD + cost in
+    // 27 bits of a search key
+    if (!(e->lambda >= 0.0) || lam2 * 42.0 >= 67108864.0) {
+        set_error("rdo_lambda_mult gives a lambda outside 0 <= lambda < 7.9e5, which this backend does not support");
+        return RIV_E_UNSUPPORTED;
+    }
     e->c_zero = (int)std::min(1e9, std::trunc(lam2 * 0.5));
     e->c_pred = (int)std::min(1e9, std::trunc(lam2 * 1.5));
     // RDO skip table over every reachable SAD (a 16x16 SAD is at most 65280)
@@ -464,13 +483,17 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
         RIV_HIP(hipMemcpyAsync(s.h_coeffs.p, s.coeffs.p, 768 * (size_t)nb, hipMemcpyDeviceToHost, st));
         mark(7);
     } else {
+        rivk::sad_table(st, cY, rY, W, H, c.search_range, e->sadt.as<uint16_t>());
+        rivk::hp_table(st, cY, rY, W, H, c.search_range, e->hpt.as<uint16_t>());
         riv::WalkParams p{};
-        p.cur = cY; p.ref = rY; p.W = W; p.H = H; p.bw = e->bw; p.bh = e->bh; p.sr = c.search_range;
+        p.W = W; p.H = H; p.bw = e->bw; p.bh = e->bh; p.sr = c.search_range;
         p.prev_mv = e->prev_inter_slot >= 0 ? (const int8_t*)e->slots[e->prev_inter_slot].rec.p + 4 : nullptr;
         p.prev_stride = sizeof(BlockRec);
-        p.rec = rec; p.progress = e->progress.as<unsigned long long>();
-        e->epoch++;
-        p.base = e->epoch * 65536ull;
+        p.sadt = e->sadt.as<uint16_t>(); p.sadt_stride = rivk::sad_table_stride(c.search_range);
+        p.hpt = e->hpt.as<uint16_t>(); p.hpt_stride = rivk::hp_table_stride(c.search_range); p.hp_slots = e->hp_slots;
+        p.rec = rec; p.pub = e->pub.as<unsigned long long>();
+        e->epoch = e->epoch % 0x0fffffffull + 1;   // 1 .. 2^28-1, never the zero of a fresh buffer
+        p.tag = (unsigned)e->epoch;
         p.bits = e->bits.as<double>(); p.lam2 = e->lambda * 2.0; p.c_zero = e->c_zero; p.c_pred = e->c_pred;
         p.lam_zero = e->lambda == 0.0;
         p.rdo_bits = e->rdo.as<uint32_t>(); p.rdo_max = e->rdo_max;

@@ -61,24 +61,171 @@ __global__ void k_yuv420_to_rgb(const uint8_t* __restrict__ Y, const uint8_t* __
 }
 
 // ============================================================================================
+// Integer SAD table
+// ============================================================================================
+// Every integer candidate a block's search can visit lies in [-sr, sr]^2, and its SAD does not
+// depend on the search path, so all (2sr+1)^2 of them are computed up front, fully parallel:
+// one workgroup per MB, one position per thread. SAD semantics of satd_scalar.rs:4-99: each 4x4
+// sub-block's reference origin is clamped to [0,W-4] x [0,H-4] as a unit, no per-pixel clamp.
+// The reference window (frame rows and columns the clamped origins can touch) is staged in LDS
+// as four byte-shifted dword copies, so every 4-pixel reference row is one aligned LDS read.
+// Entry (dy+sr)*(2sr+1) + (dx+sr) of block b sits at out[b*stride + ...], stride a multiple of 8.
+__global__ __launch_bounds__(256) void k_sad_table(const uint8_t* __restrict__ cur, const uint8_t* __restrict__ ref, int W, int H, int bw,
+                                                   int sr, int ndwt, uint16_t* __restrict__ out, int stride) {
+    extern __shared__ __align__(16) uint32_t sm[];
+    const int b = blockIdx.x, bx = b % bw, by = b / bw, x0 = bx * 16, y0 = by * 16, t = threadIdx.x;
+    const int D = 2 * sr + 1, P = D * D, MAXR = 16 + 2 * sr, CS = MAXR * ndwt;
+    const int wx = max(0, x0 - sr) & ~3, wy = max(0, y0 - sr);
+    const int NR = min(H, y0 + 16 + sr) - wy;
+    const int wxe = min(W, (x0 + 16 + sr + 3) & ~3);   // exclusive, a multiple of 4 (W is a multiple of 16)
+    const int nd = (wxe - wx) >> 2;
+    uint32_t* curs = sm;        // 64 dwords: the current block
+    uint32_t* cp = sm + 64;     // copy s, row r, dword i = bytes wx+4i+s .. +3 of reference row wy+r
+    if (t < 64) curs[t] = *(const uint32_t*)(cur + (size_t)(y0 + (t >> 2)) * W + x0 + 4 * (t & 3));
+    for (int e = t; e < NR * nd; e += 256) {
+        const int r = e / nd, i = e - r * nd;
+        const uint8_t* row = ref + (size_t)(wy + r) * W + wx;
+        const uint32_t d0 = *(const uint32_t*)(row + 4 * i);
+        const uint32_t d1 = i + 1 < nd ? *(const uint32_t*)(row + 4 * i + 4) : 0u;
+        uint32_t* q = cp + r * ndwt + i;
+        q[0] = d0;
+        q[CS] = __builtin_amdgcn_alignbyte(d1, d0, 1);
+        q[2 * CS] = __builtin_amdgcn_alignbyte(d1, d0, 2);
+        q[3 * CS] = __builtin_amdgcn_alignbyte(d1, d0, 3);
+    }
+    __syncthreads();
+    uint32_t c[64];
+#pragma unroll
+    for (int k = 0; k < 64; k++) c[k] = curs[k];
+    for (int p = t; p < P; p += 256) {
+        const int dy = p / D - sr, dx = p % D - sr;
+        int base[4];
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+            const int ox = dclamp(x0 + 4 * g + dx, 0, W - 4) - wx;
+            base[g] = (ox & 3) * CS + (ox >> 2);
+        }
+        uint32_t acc = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int oy = dclamp(y0 + 4 * j + dy, 0, H - 4) - wy;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+#pragma unroll
+                for (int g = 0; g < 4; g++) acc = sad4(c[(4 * j + i) * 4 + g], cp[base[g] + (oy + i) * ndwt], acc);
+            }
+        }
+        out[(size_t)b * stride + p] = (uint16_t)acc;
+    }
+}
+
+// ============================================================================================
+// Half-pel SAD table
+// ============================================================================================
+// refine_halfpel (motion/scalar.rs:759-839) scores half-pel positions (hx, hy) in
+// [-2sr-1, 2sr+1]^2 around the integer result with floor interpolation and per-pixel clamping
+// (satd_scalar.rs:113-166). Per-pixel clamping equals plain interpolation on the replicate-
+// padded reference window, so the window is staged once per MB as four interpolated planes
+// (integer, x-half, y-half, xy-half); position (hx, hy) is then a plain 16x16 SAD against plane
+// (hx&1, hy&1) at integer offset (hx>>1, hy>>1). A thread scores 4 x-offsets x K y-offsets of one
+// plane: v_qsad_pk_u16_u8 gives the SADs of 4 consecutive byte offsets per instruction, and each
+// reference row is read once for the K y-offsets it serves. Entry (hy+2sr+1)*(4sr+3) + (hx+2sr+1)
+// of block b sits at out[b*stride + ...], stride a multiple of 8.
+template <int K>
+__global__ __launch_bounds__(128) void k_hp_table(const uint8_t* __restrict__ cur, const uint8_t* __restrict__ ref, int W, int H, int bw,
+                                                  int sr, int ndp, uint16_t* __restrict__ out, int stride) {
+    extern __shared__ __align__(16) uint32_t sm[];
+    const int b = blockIdx.x, bx = b % bw, by = b / bw, x0 = bx * 16, y0 = by * 16, t = threadIdx.x;
+    const int WS = 16 + 2 * sr + 2, NRP = WS + K, NRA = NRP + 1, NDA = ndp + 1, PS = NRP * ndp, D2 = 4 * sr + 3, HR = 2 * sr + 1;
+    const int wx0 = x0 - sr - 1, wxa = wx0 & ~3, xo = wx0 - wxa, wy0 = y0 - sr - 1;
+    uint32_t* curs = sm;            // 64 dwords: the current block
+    uint32_t* A = sm + 64;          // NRA x NDA: replicate-padded window, dword-aligned span from wxa
+    uint32_t* P = A + NRA * NDA;    // 4 planes of NRP x ndp dwords
+    if (t < 64) curs[t] = *(const uint32_t*)(cur + (size_t)(y0 + (t >> 2)) * W + x0 + 4 * (t & 3));
+    for (int e = t; e < NRA * NDA; e += blockDim.x) {
+        const int r = e / NDA, k = e - r * NDA, gx = wxa + 4 * k;
+        const uint8_t* row = ref + (size_t)dclamp(wy0 + r, 0, H - 1) * W;
+        // W is a multiple of 16: an aligned dword is wholly inside or wholly outside the row
+        A[e] = (gx >= 0 && gx < W) ? *(const uint32_t*)(row + gx) : 0x01010101u * (gx < 0 ? row[0] : row[W - 1]);
+    }
+    __syncthreads();
+    for (int e = t; e < NRP * ndp; e += blockDim.x) {
+        const int r = e / ndp, k = e - r * ndp;
+        const uint32_t a0 = A[r * NDA + k], a1 = __builtin_amdgcn_alignbyte(A[r * NDA + k + 1], a0, 1);
+        const uint32_t b0 = A[(r + 1) * NDA + k], b1 = __builtin_amdgcn_alignbyte(A[(r + 1) * NDA + k + 1], b0, 1);
+        P[e] = a0;
+        P[PS + e] = avg2f(a0, a1);
+        P[2 * PS + e] = avg2f(a0, b0);
+        P[3 * PS + e] = avg4f(a0, a1, b0, b1);
+    }
+    __syncthreads();
+    uint32_t c[64];
+#pragma unroll
+    for (int q = 0; q < 64; q++) c[q] = curs[q];
+    // x-offset groups: parity 0 has qx in [-sr, sr], parity 1 has qx in [-sr-1, sr] (hx = 2qx + parity)
+    const int gx0 = (2 * sr + 1 + 3) / 4, gx1 = (2 * sr + 2 + 3) / 4, cy0 = (2 * sr + 1 + K - 1) / K, cy1 = (2 * sr + 2 + K - 1) / K;
+    const int ntask = (gx0 + gx1) * (cy0 + cy1);
+    for (int task = t; task < ntask; task += blockDim.x) {
+        const int xi = task % (gx0 + gx1), yi = task / (gx0 + gx1);
+        const int px = xi >= gx0, py = yi >= cy0;
+        const int qx0 = (px ? -sr - 1 + 4 * (xi - gx0) : -sr + 4 * xi);
+        const int qy0 = (py ? -sr - 1 + K * (yi - cy0) : -sr + K * yi);
+        const uint32_t* pl = P + (px + 2 * py) * PS;
+        const int u0 = xo + qx0 + sr + 1, m = u0 >> 2, sh = u0 & 3, r0 = qy0 + sr + 1;
+        unsigned long long acc[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[k] = 0;
+#pragma unroll
+        for (int tt = 0; tt < K + 15; tt++) {
+            const uint32_t* rw = pl + (r0 + tt) * ndp + m;
+            uint32_t d[6], al[5];
+#pragma unroll
+            for (int i = 0; i < 6; i++) d[i] = rw[i];
+#pragma unroll
+            for (int i = 0; i < 5; i++) al[i] = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int j = tt - k;
+                if (j >= 0 && j < 16) {
+#pragma unroll
+                    for (int g = 0; g < 4; g++)
+                        acc[k] = __builtin_amdgcn_qsad_pk_u16_u8((unsigned long long)al[g] | ((unsigned long long)al[g + 1] << 32), c[j * 4 + g], acc[k]);
+                }
+            }
+        }
+        uint16_t* ob = out + (size_t)b * stride;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int hy = 2 * (qy0 + k) + py;
+            if (hy <= HR) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int hx = 2 * (qx0 + q) + px;
+                    if (hx <= HR) ob[(hy + HR) * D2 + hx + HR] = (uint16_t)(acc[k] >> (16 * q));
+                }
+            }
+        }
+    }
+}
+
+// ============================================================================================
 // Motion search + MV mode decision wavefront
 // ============================================================================================
 // One workgroup per band of R MB rows, one wave64 per row. Block (bx, by) depends on the
 // left block and on (bx-1..bx+1, by-1) through the ME seeds / cost predictors
-// (motion/scalar.rs:56-156) and the mode-decision predictors (mv_predictor.rs:59-272), so
-// wave w processes block bx = s - 2w at step s; rows of one band synchronise with s_barrier,
-// the first row of a band waits on the previous band's last row through a release/acquire
-// progress counter in HBM (bounded spin: a timeout sets p.err and every wave exits).
+// (motion/scalar.rs:56-156) and the mode-decision predictors (mv_predictor.rs:59-272), so a
+// row runs two blocks behind the row above. Inside a band, a wave waits on the LDS count of
+// finished blocks of the row above and reads that row's MVs from LDS. Across bands, the last
+// row of a band publishes each block's MV pair as one 64-bit word tagged with the frame
+// (relaxed agent-scope atomics, so no cache write-back or invalidate is needed: the word is
+// its own flag); the first row of the next band polls the top-right word it needs, having
+// loaded it speculatively one block early. Every wait is bounded: a timeout sets p.err, which
+// ends the waits of every other workgroup.
 //
-// Each wave stages its block's replicate-padded reference window
-// ref[clamp(y0-sr-1 .. y0+16+sr)][clamp(x0-sr-1 .. x0+16+sr)] and its 16x16 current block in
-// LDS, one block ahead (double buffer), and evaluates every SAD of the hex/diamond/half-pel
-// walk on the fly: lane l owns current row l>>2, columns 4*(l&3)..+3, computes one packed
-// v_sad_u8 per candidate, and the 64 partial sums are reduced with DPP + readlane (two
-// candidates packed per 32-bit word; a 16x16 SAD is <= 65280 so the halves never carry).
+// All SADs come from the integer and half-pel tables built beforehand; each wave copies the
+// next block's tables from HBM into LDS while it walks the current one. The search itself is
+// then a chain of table lookups and MV-cost evaluations, candidate sets evaluated one per lane.
 namespace {
-
-struct Cand { int n; unsigned valid; int dx[6]; int dy[6]; };   // fixed slots; invalid slots hold (0,0)
 
 __device__ __forceinline__ int median3(int a, int b, int c) {   // motion/scalar.rs:24-43
     if (a > b) { if (b > c) return b; else if (a > c) return c; else return a; }
@@ -90,61 +237,34 @@ struct CostCtx {
     const double* bits;
     double lam2;
     int c_zero, c_pred, lam_zero;
-    // estimate_mv_cost (motion/scalar.rs:453-544); the min over the applicable terms
+    // estimate_mv_cost (motion/scalar.rs:453-544): the min over the applicable terms. The host
+    // keeps lam2 * 42 below 2^26, so every term truncates exactly in 32 bits and SAD + cost
+    // leaves room for a lane index in the low bits of a search key.
     __device__ __forceinline__ int cost(int dx, int dy) const {
         if (lam_zero) return 0;
-        long long best = 0x7fffffffffffffffll;
+        int best = 0x3ffffff;
         if ((dx == nx && dy == ny) || (dx == rx && dy == ry) || (dx == mx && dy == my)) best = c_pred;
-        if (dx == 0 && dy == 0) best = min(best, (long long)c_zero);
-        {
-            double t = __dadd_rn(__dadd_rn(6.0, bits[abs(dx - nx)]), bits[abs(dy - ny)]);
-            best = min(best, (long long)__dmul_rn(lam2, t));
-        }
-        {
-            double t = __dadd_rn(__dadd_rn(6.0, bits[abs(dx - rx)]), bits[abs(dy - ry)]);
-            best = min(best, (long long)__dmul_rn(lam2, t));
-        }
-        return (int)min(best, (long long)0x3fffffff);
+        if (dx == 0 && dy == 0) best = min(best, c_zero);
+        const int t0 = (int)__dmul_rn(lam2, __dadd_rn(__dadd_rn(6.0, bits[abs(dx - nx)]), bits[abs(dy - ny)]));
+        const int t1 = (int)__dmul_rn(lam2, __dadd_rn(__dadd_rn(6.0, bits[abs(dx - rx)]), bits[abs(dy - ry)]));
+        return min(best, min(t0, t1));
     }
 };
 
-// Integer SAD with the reference's per-4x4 edge handling (satd_scalar.rs:4-99: each 4x4 sub-
-// block's reference origin is clamped to [0,W-4] x [0,H-4] as a unit). Evaluates c.n <= 6
-// candidates; returns the SADs in out[].
-__device__ __forceinline__ void int_sads(const uint32_t* win32, int WSP, int xo, uint32_t cur4, int lane, int x0, int y0, int W, int H,
-                                         int sr, const Cand& c, int* out) {
-    const int g = lane & 3, j = lane >> 2;
-    const int sxo = x0 + 4 * g, syo = y0 + 4 * (j >> 2), jr = j & 3;
-    const int wx0 = x0 - sr - 1 - xo, wy0 = y0 - sr - 1;
-#pragma unroll
-    for (int k = 0; k < 6; k += 2) {
-        if (k < c.n) {
-            int ox = dclamp(sxo + c.dx[k], 0, W - 4) - wx0, oy = dclamp(syo + c.dy[k], 0, H - 4) - wy0 + jr;
-            uint32_t pk = sad4(cur4, lds_read4(win32, oy * WSP + ox), 0u);
-            if (k + 1 < c.n) {
-                int ox1 = dclamp(sxo + c.dx[k + 1], 0, W - 4) - wx0, oy1 = dclamp(syo + c.dy[k + 1], 0, H - 4) - wy0 + jr;
-                pk |= sad4(cur4, lds_read4(win32, oy1 * WSP + ox1), 0u) << 16;
-            }
-            uint32_t t = wave_sum(pk);
-            out[k] = (int)(t & 0xffff);
-            if (k + 1 < c.n) out[k + 1] = (int)(t >> 16);
-        }
-    }
+// Minimum of v over lanes 0..7 (0..15), returned wave-uniform; the other lanes of each row of 16
+// must hold values that do not win. Used as an argmin: keys carry the lane index in their low bits.
+__device__ __forceinline__ int lane_min8(int v) {
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true));    // quad_perm [1,0,3,2]
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true));    // quad_perm [2,3,0,1]
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true));   // row_half_mirror
+    return __builtin_amdgcn_readlane(v, 0);
 }
-
-// Half-pel reference samples for the lane's 4 pixels at half-pel offset (hx, hy), floor
-// interpolation (satd_scalar.rs:113-166). The per-pixel clamp of the reference's sampler
-// equals plain interpolation on the replicate-padded window (averaging replicated samples
-// returns the edge sample), so no clamping is needed here.
-__device__ __forceinline__ uint32_t hp_ref4(const uint32_t* win32, int WSP, int xo, int lane, int sr, int hx, int hy) {
-    const int g = lane & 3, j = lane >> 2;
-    int xs = 4 * g + (hx >> 1) + sr + 1 + xo, ys = j + (hy >> 1) + sr + 1;
-    int ax = hx & 1, ay = hy & 1;
-    uint32_t a = lds_read4(win32, ys * WSP + xs);
-    if (!ax && !ay) return a;
-    if (ax && !ay) return avg2f(a, lds_read4(win32, ys * WSP + xs + 1));
-    if (!ax && ay) return avg2f(a, lds_read4(win32, (ys + 1) * WSP + xs));
-    return avg4f(a, lds_read4(win32, ys * WSP + xs + 1), lds_read4(win32, (ys + 1) * WSP + xs), lds_read4(win32, (ys + 1) * WSP + xs + 1));
+__device__ __forceinline__ int lane_min16(int v) {
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true));
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true));
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true));
+    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true));   // row_mirror
+    return __builtin_amdgcn_readlane(v, 0);
 }
 
 __device__ __forceinline__ int mvx(uint32_t w) { return (int)(int8_t)(w & 0xff); }
@@ -154,337 +274,305 @@ __device__ __forceinline__ int mv_class(int mag) {   // mv_class_from_magnitude 
     return min(32 - __clz(mag), 10);
 }
 
-// Window geometry: LDS row r holds reference row clamp(wy0 + r) over the dword-aligned column
-// span starting at wxa = floor4(wx0) (NDW dwords, pitch NDW+1 dwords). Frame widths are multiples
-// of 16, so each aligned dword is either fully inside the row or fully outside it; outside
-// dwords replicate the edge sample, which makes the LDS copy the replicate-padded window.
-template <int NPF, int NDW>
-__device__ __forceinline__ void load_window(const WalkParams& p, int bx, int by, int lane, int WS, uint32_t* pf, uint32_t& pcur) {
-    const int x0 = bx * 16, y0 = by * 16, wx0 = x0 - p.sr - 1, wy0 = y0 - p.sr - 1, wxa = wx0 & ~3;
-#pragma unroll
-    for (int k = 0; k < NPF; k++) {
-        const int e = lane + 64 * k, r = e / NDW, c = e % NDW;
-        if (r < WS) {
-            const int gy = dclamp(wy0 + r, 0, p.H - 1), gx = wxa + 4 * c;
-            const uint8_t* row = p.ref + (size_t)gy * p.W;
-            if (gx >= 0 && gx < p.W) pf[k] = *(const uint32_t*)(row + gx);
-            else pf[k] = 0x01010101u * (gx < 0 ? row[0] : row[p.W - 1]);
-        }
-    }
-    pcur = *(const uint32_t*)(p.cur + (size_t)(y0 + (lane >> 2)) * p.W + x0 + 4 * (lane & 3));
+// Published MV pair of a band's last row: 18-bit (dx, dy, sx, sy) words of the search and the
+// reconstructed MV, plus the frame tag in bits 36..63 so that a reader can tell this frame's
+// value from the previous frame's without any fence.
+__device__ __forceinline__ uint32_t mv18(uint32_t w) { return (w & 0xffffu) | (((w >> 16) & 1u) << 16) | (((w >> 24) & 1u) << 17); }
+__device__ __forceinline__ uint32_t mv32(uint32_t v) { return (v & 0xffffu) | (((v >> 16) & 1u) << 16) | (((v >> 17) & 1u) << 24); }
+
+// Copies one block's table (nb bytes, a multiple of 16) from HBM straight into LDS at dst: each
+// lane moves 16 bytes per 1 KB step. Completion is tracked by the wave's vector-memory counter;
+// the consumer waits for it before reading dst.
+__device__ __forceinline__ void fetch_table(const uint16_t* src, uint8_t* dst, int nb, int lane) {
+    for (int o = 0; o < nb; o += 1024)
+        if (o + 16 * lane < nb)
+            __builtin_amdgcn_global_load_lds((const void*)((const uint8_t*)src + o + 16 * lane), (void*)(dst + o), 16, 0, 0);
 }
-template <int NPF, int NDW>
-__device__ __forceinline__ void store_window(uint32_t* win32, uint32_t* cur32, int lane, int WS, const uint32_t* pf, uint32_t pcur) {
-#pragma unroll
-    for (int k = 0; k < NPF; k++) {
-        const int e = lane + 64 * k, r = e / NDW, c = e % NDW;
-        if (r < WS) win32[r * (NDW + 1) + c] = pf[k];
+
+// A wave-uniform 64-bit value moved to scalar registers (readfirstlane returns a signed int:
+// each half is widened as unsigned).
+__device__ __forceinline__ unsigned long long uniform64(unsigned long long x) {
+    return ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32)) << 32) |
+           (unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)x);
+}
+
+// Spins until the published word at a carries this frame's tag; false after the spin bound or
+// once another workgroup has reported a timeout.
+__device__ __forceinline__ bool wait_pub(const unsigned long long* a, unsigned tag, const int* err, unsigned long long& v) {
+    for (unsigned spins = 0;; spins++) {
+        unsigned long long x = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        v = uniform64(x);
+        if ((unsigned)(v >> 36) == tag) return true;
+        if (spins > (1u << 21)) return false;
+        if ((spins & 63) == 63 && __builtin_amdgcn_readfirstlane(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) return false;
+        __builtin_amdgcn_s_sleep(1);
     }
-    cur32[lane] = pcur;
 }
 
 }  // namespace
 
-template <int NPF, int NDW>
 __global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
     extern __shared__ __align__(16) uint8_t lds[];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int R = p.R, r0 = blockIdx.x * R, row = r0 + w, bw = p.bw, sr = p.sr;
-    const int WS = 16 + 2 * sr + 2, WSP = 4 * (NDW + 1), WINB = (WS * WSP + 15) & ~15;
-    uint8_t* wbase = lds + w * (2 * WINB + 512);
-    uint8_t* winb0 = wbase;
-    uint8_t* curb0 = wbase + 2 * WINB;
-    int8_t* rowbuf = (int8_t*)(lds + R * (2 * WINB + 512));
-    double* bits = (double*)((uint8_t*)rowbuf + ((R * bw * 8 + 15) & ~15));
-    int* abortf = (int*)(bits + 256);
+    const int R = p.R, row = blockIdx.x * R + w, bw = p.bw, sr = p.sr;
+    const int TSB = 2 * p.sadt_stride, HSB = 2 * p.hpt_stride, NHP = p.hp_slots, D2 = 4 * sr + 3, HR = 2 * sr + 1;
+    const int WAVEB = 2 * TSB + NHP * HSB;
+    uint8_t* tabb0 = lds + w * WAVEB;         // two integer-table slots
+    uint8_t* hpb0 = tabb0 + 2 * TSB;          // NHP half-pel-table slots
+    uint32_t* rowbuf = (uint32_t*)(lds + R * WAVEB);   // per row and block: search MV, recon MV, mode word, SAD word
+    double* bits = (double*)(rowbuf + R * bw * 4);
+    uint32_t* rdol = (uint32_t*)(bits + 256);  // first 256 words of the RDO skip bitmask
+    int* abortf = (int*)(rdol + 256);
+    int* lprog = abortf + 1;   // per wave: number of finished blocks of its row
     for (int i = threadIdx.x; i < 256; i += blockDim.x) bits[i] = p.bits[i];
+    const int nrdo = min(256, (p.rdo_max >> 5) + 1);
+    for (int i = threadIdx.x; i < nrdo; i += blockDim.x) rdol[i] = p.rdo_bits[i];
     if (threadIdx.x == 0) *abortf = 0;
+    if (threadIdx.x < R) lprog[threadIdx.x] = 0;
     const bool row_valid = row < p.bh;
-    uint32_t pf[NPF];
-    uint32_t pcur = 0;
+    const bool has_tp = p.prev_mv != nullptr;
+    uint32_t pTn = 0;
     if (row_valid) {
-        load_window<NPF, NDW>(p, 0, row, lane, WS, pf, pcur);
-        store_window<NPF, NDW>((uint32_t*)winb0, (uint32_t*)curb0, lane, WS, pf, pcur);
+        fetch_table(p.sadt + (size_t)row * bw * p.sadt_stride, tabb0, TSB, lane);
+        fetch_table(p.hpt + (size_t)row * bw * p.hpt_stride, hpb0, HSB, lane);
+        if (has_tp) pTn = *(const uint32_t*)(p.prev_mv + (size_t)p.prev_stride * row * bw);
     }
     __syncthreads();
+    if (!row_valid) return;
 
-    // MVs are carried packed: bytes (dx, dy, sx, sy) of one 32-bit word
-    uint32_t lS = 0, lR = 0;   // left neighbour (search, reconstructed)
-    const int nsteps = bw + 2 * (R - 1);
-    for (int s = 0; s < nsteps; s++) {
-        const int bx = s - 2 * w;
-        if (row_valid && bx >= 0 && bx < bw) {
-            const int idx = row * bw + bx, x0 = bx * 16, y0 = row * 16;
-            const bool has_l = bx > 0, has_t = row > 0, has_tr = has_t && bx + 1 < bw, has_tl = has_t && bx > 0;
-            // ---- top neighbours (search + reconstructed MVs): top-left, top, top-right ----
-            uint32_t tlS = 0, tlR = 0, tS = 0, tR = 0, trS = 0, trR = 0;
-            bool timed_out = false;
-            if (has_t) {
-                uint64_t v0 = 0, v1 = 0, v2 = 0;
-                if (w == 0) {
-                    unsigned long long need = p.base + (unsigned long long)min(bx + 2, bw);
-                    unsigned spins = 0;
-                    while (__hip_atomic_load(&p.progress[row - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
-                        if (++spins > (1u << 22)) { timed_out = true; break; }
-                        __builtin_amdgcn_s_sleep(2);
-                    }
-                    if (!timed_out) {
-                        const uint64_t* q = (const uint64_t*)&p.rec[(row - 1) * bw + bx];   // 16-byte records
-                        if (bx > 0) v0 = q[-2];
-                        v1 = q[0];
-                        if (bx + 1 < bw) v2 = q[2];
-                    }
-                } else {
-                    const uint64_t* q = (const uint64_t*)(rowbuf + ((w - 1) * bw + bx) * 8);
-                    if (bx > 0) v0 = q[-1];
-                    v1 = q[0];
-                    if (bx + 1 < bw) v2 = q[1];
-                }
-                tlS = (uint32_t)v0; tlR = (uint32_t)(v0 >> 32);
-                tS = (uint32_t)v1; tR = (uint32_t)(v1 >> 32);
-                trS = (uint32_t)v2; trR = (uint32_t)(v2 >> 32);
+    const bool has_t = row > 0, from_hbm = has_t && w == 0, publish = w == R - 1 && row + 1 < p.bh;
+    const unsigned long long* above = p.pub + (size_t)(has_t ? row - 1 : 0) * bw;
+    bool dead = false;
+    // MVs are carried packed: bytes (dx, dy, sx, sy) of one 32-bit word. Top neighbours slide
+    // along the row: top-left <- top <- top-right.
+    uint32_t lS = 0, lR = 0, tlS = 0, tlR = 0, tS = 0, tR = 0, trS = 0, trR = 0;
+    unsigned long long spec = 0;   // speculative load of the next top-right published word
+    if (from_hbm) {
+        unsigned long long v;
+        dead = !wait_pub(above, p.tag, p.err, v);
+        tS = mv32((uint32_t)v & 0x3ffffu); tR = mv32((uint32_t)(v >> 18) & 0x3ffffu);
+        if (bw > 1) spec = __hip_atomic_load(above + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    for (int bx = 0; bx < bw && !dead; bx++) {
+        const int idx = row * bw + bx;
+        const bool has_l = bx > 0, has_tr = has_t && bx + 1 < bw, has_tl = has_t && bx > 0;
+        // ---- top neighbours (search + reconstructed MVs) ----
+        if (from_hbm) {
+            if (bx > 0) { tlS = tS; tlR = tR; tS = trS; tR = trR; }
+            trS = 0; trR = 0;
+            if (has_tr) {
+                unsigned long long v = uniform64(spec);
+                if ((unsigned)(v >> 36) != p.tag && !wait_pub(above + bx + 1, p.tag, p.err, v)) { dead = true; break; }
+                trS = mv32((uint32_t)v & 0x3ffffu); trR = mv32((uint32_t)(v >> 18) & 0x3ffffu);
+                if (bx + 2 < bw) spec = __hip_atomic_load(above + bx + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-            if (timed_out) {
-                if (lane == 0) { *abortf = 1; atomicExch(p.err, 1); }
+        } else if (has_t) {
+            const int need = min(bx + 2, bw);
+            unsigned spins = 0;
+            while (__hip_atomic_load(&lprog[w - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < need) {
+                if (__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) || ++spins > (1u << 24)) { dead = true; break; }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (dead) break;
+            const uint32_t* q = rowbuf + ((w - 1) * bw + bx) * 4;
+            tlS = bx > 0 ? q[-4] : 0u; tlR = bx > 0 ? q[-3] : 0u;
+            tS = q[0]; tR = q[1];
+            trS = bx + 1 < bw ? q[4] : 0u; trR = bx + 1 < bw ? q[5] : 0u;
+        }
+        const uint32_t pT = pTn;
+        // this block's tables have landed in LDS once every load issued so far is done
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        const uint16_t* sadt = (const uint16_t*)(tabb0 + (bx & 1) * TSB);
+        const uint16_t* hpt = (const uint16_t*)(hpb0 + (NHP == 2 ? (bx & 1) * HSB : 0));
+        // prefetch the next block's tables and temporal MV while this block is walked
+        const bool pf_next = bx + 1 < bw;
+        if (pf_next) {
+            fetch_table(p.sadt + (size_t)(idx + 1) * p.sadt_stride, tabb0 + ((bx + 1) & 1) * TSB, TSB, lane);
+            if (NHP == 2) fetch_table(p.hpt + (size_t)(idx + 1) * p.hpt_stride, hpb0 + ((bx + 1) & 1) * HSB, HSB, lane);
+            if (has_tp) pTn = *(const uint32_t*)(p.prev_mv + (size_t)p.prev_stride * (idx + 1));
+        }
+
+        // ---- ME predictors: gather_local_motion_candidates (motion/scalar.rs:56-156) ----
+        // in predictor order: left, top, top-right, top-left, temporal; (dx, dy) packed in 16 bits
+        const uint32_t q0 = lS & 0xffffu, q1 = tS & 0xffffu, q2 = trS & 0xffffu, q3 = tlS & 0xffffu, q4 = pT & 0xffffu;
+        CostCtx cc;
+        {
+            // nearest / near of the cost model: the first two distinct present predictors
+            uint32_t n0 = 0, n1 = 0;
+            int nu = 0;
+#define RIV_UNIQ(present, q)                                        \
+            if (present) {                                          \
+                if (nu == 0) { n0 = (q); nu = 1; }                  \
+                else if (nu == 1 && (q) != n0) { n1 = (q); nu = 2; } \
+            }
+            RIV_UNIQ(has_l, q0) RIV_UNIQ(has_t, q1) RIV_UNIQ(has_tr, q2) RIV_UNIQ(has_tl, q3) RIV_UNIQ(has_tp, q4)
+#undef RIV_UNIQ
+            cc.nx = mvx(n0); cc.ny = mvy(n0); cc.rx = mvx(n1); cc.ry = mvy(n1);
+            const uint32_t qm = has_tr ? q2 : (has_tl ? q3 : 0u);
+            cc.mx = median3(has_l ? mvx(q0) : 0, has_t ? mvx(q1) : 0, mvx(qm));
+            cc.my = median3(has_l ? mvy(q0) : 0, has_t ? mvy(q1) : 0, mvy(qm));
+        }
+        cc.bits = bits; cc.lam2 = p.lam2; cc.c_zero = p.c_zero; cc.c_pred = p.c_pred; cc.lam_zero = p.lam_zero;
+        // push_candidate (motion/scalar.rs:46-53) + evaluate_seed_candidates (:638-678): (0,0),
+        // then each in-range non-zero predictor not already listed. Lane 0 holds (0,0), lane i+1
+        // predictor i; the sequential "first strictly better" scan is an argmin with ties going
+        // to the lower lane, i.e. the earlier candidate.
+        unsigned incm = 1;
+        {
+            auto inr = [&](uint32_t q) { return q != 0u && abs(mvx(q)) <= sr && abs(mvy(q)) <= sr; };
+            const bool i0 = has_l && inr(q0);
+            const bool i1 = has_t && inr(q1) && !(i0 && q1 == q0);
+            const bool i2 = has_tr && inr(q2) && !(i0 && q2 == q0) && !(i1 && q2 == q1);
+            const bool i3 = has_tl && inr(q3) && !(i0 && q3 == q0) && !(i1 && q3 == q1) && !(i2 && q3 == q2);
+            const bool i4 = has_tp && inr(q4) && !(i0 && q4 == q0) && !(i1 && q4 == q1) && !(i2 && q4 == q2) && !(i3 && q4 == q3);
+            incm |= (i0 ? 2u : 0u) | (i1 ? 4u : 0u) | (i2 ? 8u : 0u) | (i3 ? 16u : 0u) | (i4 ? 32u : 0u);
+        }
+        const int D = 2 * sr + 1;
+        int sdx = 0, sdy = 0;
+        int scost;   // SAD + MV cost of the current integer best
+        bool skip_int;
+        {
+            const uint32_t myq = lane == 1 ? q0 : lane == 2 ? q1 : lane == 3 ? q2 : lane == 4 ? q3 : lane == 5 ? q4 : 0u;
+            const bool myv = lane < 6 && ((incm >> lane) & 1u);
+            const int dx = myv ? mvx(myq) : 0, dy = myv ? mvy(myq) : 0;
+            const int sv = sadt[(dy + sr) * D + dx + sr];
+            const int key = myv ? ((sv + cc.cost(dx, dy)) << 3) | lane : 0x7fffffff;
+            skip_int = p.zthr > 0 && __builtin_amdgcn_readlane(sv, 0) <= p.zthr;   // process_block_hex (:546-636)
+            if (skip_int) {
+                scost = __builtin_amdgcn_readlane(key, 0) >> 3;
             } else {
-                const bool has_tp = p.prev_mv != nullptr;
-                const uint32_t pT = has_tp ? *(const uint32_t*)(p.prev_mv + (size_t)p.prev_stride * idx) : 0u;
-                // prefetch the next block's window while this block is walked
-                const bool pf_next = bx + 1 < bw;
-                if (pf_next) load_window<NPF, NDW>(p, bx + 1, row, lane, WS, pf, pcur);
-
-                const uint32_t* win32 = (const uint32_t*)(winb0 + (bx & 1) * WINB);
-                const int xo = (x0 - sr - 1) - ((x0 - sr - 1) & ~3);   // byte offset of window column 0
-                const uint32_t cur4 = ((const uint32_t*)(curb0 + (bx & 1) * 256))[lane];
-
-                // ---- ME predictors: gather_local_motion_candidates (motion/scalar.rs:56-156) ----
-                // slots in predictor order: left, top, top-right, top-left, temporal
-                const bool pv[5] = {has_l, has_t, has_tr, has_tl, has_tp};
-                const int pdx[5] = {mvx(lS), mvx(tS), mvx(trS), mvx(tlS), mvx(pT)};
-                const int pdy[5] = {mvy(lS), mvy(tS), mvy(trS), mvy(tlS), mvy(pT)};
-                int ux0 = 0, uy0 = 0, ux1 = 0, uy1 = 0, nu = 0;
-#pragma unroll
-                for (int i = 0; i < 5; i++) {
-                    if (pv[i]) {
-                        bool dup = (nu >= 1 && ux0 == pdx[i] && uy0 == pdy[i]) || (nu >= 2 && ux1 == pdx[i] && uy1 == pdy[i]);
-                        if (!dup) { if (nu == 0) { ux0 = pdx[i]; uy0 = pdy[i]; } else if (nu == 1) { ux1 = pdx[i]; uy1 = pdy[i]; } nu++; }
-                    }
-                }
-                CostCtx cc;
-                cc.nx = ux0; cc.ny = uy0; cc.rx = ux1; cc.ry = uy1;
-                {
-                    int lx = has_l ? mvx(lS) : 0, ly = has_l ? mvy(lS) : 0;
-                    int tx = has_t ? mvx(tS) : 0, ty = has_t ? mvy(tS) : 0;
-                    int trx = has_tr ? mvx(trS) : (has_tl ? mvx(tlS) : 0), try_ = has_tr ? mvy(trS) : (has_tl ? mvy(tlS) : 0);
-                    cc.mx = median3(lx, tx, trx); cc.my = median3(ly, ty, try_);
-                }
-                cc.bits = bits; cc.lam2 = p.lam2; cc.c_zero = p.c_zero; cc.c_pred = p.c_pred; cc.lam_zero = p.lam_zero;
-                // push_candidate (motion/scalar.rs:46-53): (0,0) first, then every in-range predictor
-                // not already listed; built as a select network so no private array is indexed at run time
-                Cand c;
-                {
-                    bool inc[5];
-                    int slot[5], cnt = 1;
-#pragma unroll
-                    for (int i = 0; i < 5; i++) {
-                        bool ok = pv[i] && abs(pdx[i]) <= sr && abs(pdy[i]) <= sr && !(pdx[i] == 0 && pdy[i] == 0);
-#pragma unroll
-                        for (int j = 0; j < i; j++) ok = ok && !(inc[j] && pdx[j] == pdx[i] && pdy[j] == pdy[i]);
-                        inc[i] = ok;
-                        slot[i] = cnt;
-                        cnt += ok ? 1 : 0;
-                    }
-                    c.n = cnt; c.valid = 0x3f;
-                    c.dx[0] = 0; c.dy[0] = 0;
-#pragma unroll
-                    for (int q = 1; q < 6; q++) {
-                        int vx = 0, vy = 0;
-#pragma unroll
-                        for (int i = 0; i < 5; i++) { bool hit = inc[i] && slot[i] == q; vx = hit ? pdx[i] : vx; vy = hit ? pdy[i] : vy; }
-                        c.dx[q] = vx; c.dy[q] = vy;
-                    }
-                }
-                // ---- process_block_hex (motion/scalar.rs:546-636) ----
-                int sads[6];
-                int_sads(win32, WSP, xo, cur4, lane, x0, y0, p.W, p.H, sr, c, sads);
-                const int zero_sad = sads[0];
-                int sdx = 0, sdy = 0;
-                long long scost = (long long)zero_sad + cc.cost(0, 0);
-                bool skip_int = p.zthr > 0 && zero_sad <= p.zthr;
-                if (!skip_int) {   // evaluate_seed_candidates (:638-678)
-#pragma unroll
-                    for (int k = 1; k < 6; k++) {
-                        if (k < c.n) {
-                            int mc = cc.cost(c.dx[k], c.dy[k]);
-                            long long t = (long long)sads[k] + mc;
-                            if (scost - mc >= 0 && t < scost) { sdx = c.dx[k]; sdy = c.dy[k]; scost = t; }
-                        }
-                    }
-                    long long cur_sad = scost - cc.cost(sdx, sdy);
-                    if (p.pthr > 0 && cur_sad <= p.pthr) skip_int = true;
-                }
-                if (!skip_int) {   // refine_integer_motion (:680-757): large hexagon, then small diamond
-                    const int HX[6] = {2, 1, -1, -2, -1, 1}, HY[6] = {0, 2, 2, 0, -2, -2};
-                    for (int it = 0; it < 4096; it++) {
-                        Cand h; h.n = 6; h.valid = 0;
-                        const int cx = sdx, cy = sdy;
-#pragma unroll
-                        for (int k = 0; k < 6; k++) {
-                            int dx = cx + HX[k], dy = cy + HY[k];
-                            bool ok = abs(dx) <= sr && abs(dy) <= sr;
-                            h.valid |= ok ? 1u << k : 0u;
-                            h.dx[k] = ok ? dx : 0; h.dy[k] = ok ? dy : 0;
-                        }
-                        int hs[6];
-                        int_sads(win32, WSP, xo, cur4, lane, x0, y0, p.W, p.H, sr, h, hs);
-                        bool improved = false;
-#pragma unroll
-                        for (int k = 0; k < 6; k++) {
-                            if ((h.valid >> k) & 1) {
-                                int mc = cc.cost(h.dx[k], h.dy[k]);
-                                long long t = (long long)hs[k] + mc;
-                                if (scost - mc >= 0 && t < scost) { sdx = h.dx[k]; sdy = h.dy[k]; scost = t; improved = true; }
-                            }
-                        }
-                        if (!improved) break;
-                    }
-                    const int DX[4] = {1, -1, 0, 0}, DY[4] = {0, 0, 1, -1};
-                    Cand d; d.n = 4; d.valid = 0;
-                    const int cx = sdx, cy = sdy;
-#pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        int dx = cx + DX[k], dy = cy + DY[k];
-                        bool ok = abs(dx) <= sr && abs(dy) <= sr;
-                        d.valid |= ok ? 1u << k : 0u;
-                        d.dx[k] = ok ? dx : 0; d.dy[k] = ok ? dy : 0;
-                    }
-                    int ds[6];
-                    int_sads(win32, WSP, xo, cur4, lane, x0, y0, p.W, p.H, sr, d, ds);
-#pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        if ((d.valid >> k) & 1) {
-                            int mc = cc.cost(d.dx[k], d.dy[k]);
-                            long long t = (long long)ds[k] + mc;
-                            if (scost - mc >= 0 && t < scost) { sdx = d.dx[k]; sdy = d.dy[k]; scost = t; }
-                        }
-                    }
-                }
-                // ---- refine_halfpel (:759-839): base re-scored with the half-pel kernel, then
-                //      the 8 neighbours in (sy outer, sx inner) order, penalty c_pred ----
-                const int ordx[9] = {0, -1, 0, 1, -1, 1, -1, 0, 1}, ordy[9] = {0, -1, -1, -1, 0, 0, 1, 1, 1};
-                int hsad[9];
-#pragma unroll
-                for (int k = 0; k < 9; k += 2) {
-                    uint32_t pk = sad4(cur4, hp_ref4(win32, WSP, xo, lane, sr, 2 * sdx + ordx[k], 2 * sdy + ordy[k]), 0u);
-                    if (k + 1 < 9) pk |= sad4(cur4, hp_ref4(win32, WSP, xo, lane, sr, 2 * sdx + ordx[k + 1], 2 * sdy + ordy[k + 1]), 0u) << 16;
-                    uint32_t t = wave_sum(pk);
-                    hsad[k] = (int)(t & 0xffff);
-                    if (k + 1 < 9) hsad[k + 1] = (int)(t >> 16);
-                }
-                int best_hx = 2 * sdx, best_hy = 2 * sdy;
-                long long best_total = hsad[0];
-                int best_sad = hsad[0];
-#pragma unroll
-                for (int k = 1; k < 9; k++) {
-                    const int pen = p.c_pred;   // every neighbour of an even base is fractional
-                    if (best_total - pen >= 0) {
-                        long long t = (long long)hsad[k] + pen;
-                        if (t < best_total) { best_total = t; best_sad = hsad[k]; best_hx = 2 * sdx + ordx[k]; best_hy = 2 * sdy + ordy[k]; }
-                    }
-                }
-                // canonicalised search MV (motion/scalar.rs:619-634, motion_vector.rs:103-116)
-                int S0, S1, S2, S3;
-                {
-                    int hx = dclamp(best_hx, -255, 255), hy = dclamp(best_hy, -255, 255);
-                    int dx = dclamp(hx / 2, -128, 127), dy = dclamp(hy / 2, -128, 127), fx = hx % 2, fy = hy % 2;
-                    if (dx == -128 && fx == -1) fx = 0;
-                    if (dy == -128 && fy == -1) fy = 0;
-                    if (fx == -1) { fx = 1; dx = max(-128, dx - 1); }
-                    if (fy == -1) { fy = 1; dy = max(-128, dy - 1); }
-                    S0 = dx; S1 = dy; S2 = fx; S3 = fy;
-                }
-
-                // ---- mode decision on the reconstructed prefix (encoder.rs:447-733) ----
-                // derive_mv_predictors_with_stats / gather_mv_candidates (mv_predictor.rs:135-272):
-                // deduplicated (dx,dy,sx,sy) of left, top, top-right, top-left, temporal; the first
-                // two are Nearest / Near, padded with zero. MVs are packed as 32-bit words.
-                auto pack = [](int a, int b, int c, int d) -> uint32_t {
-                    return (uint32_t)(uint8_t)a | ((uint32_t)(uint8_t)b << 8) | ((uint32_t)(uint8_t)c << 16) | ((uint32_t)(uint8_t)d << 24);
-                };
-                const uint32_t wL = lR, wT = tR, wTR = trR, wTL = tlR, wP = pT;
-                uint32_t c0 = 0, c1 = 0; int ncand = 0;
-                // first two distinct present entries, selects only (no indexed storage)
-#define RIV_ADDC(present, q)                                                         \
-                {                                                                    \
-                    const uint32_t q_ = (q);                                         \
-                    const bool add_ = (present) && !(ncand >= 1 && c0 == q_) && !(ncand >= 2 && c1 == q_); \
-                    c1 = (add_ && ncand == 1) ? q_ : c1;                             \
-                    c0 = (add_ && ncand == 0) ? q_ : c0;                             \
-                    ncand += add_ ? 1 : 0;                                           \
-                }
-                RIV_ADDC(has_l, wL) RIV_ADDC(has_t, wT) RIV_ADDC(has_tr, wTR) RIV_ADDC(has_tl, wTL) RIV_ADDC(has_tp, wP)
-#undef RIV_ADDC
-                const uint32_t nearest = ncand > 0 ? c0 : 0u, nearp = ncand > 1 ? c1 : 0u, temporal = has_tp ? wP : 0u;
-                auto hpx = [](uint32_t q) { return (int)(int8_t)(q & 0xff) * 2 + (int)(int8_t)((q >> 16) & 0xff); };
-                auto hpy = [](uint32_t q) { return (int)(int8_t)((q >> 8) & 0xff) * 2 + (int)(int8_t)(q >> 24); };
-                const int mhx = S0 * 2 + S2, mhy = S1 * 2 + S3;
-                auto hpeq = [&](uint32_t q) { return mhx == hpx(q) && mhy == hpy(q); };
-                uint8_t mode;
-                if (S0 == 0 && S1 == 0 && S2 == 0 && S3 == 0) mode = riv::M_ZERO;
-                else if (hpeq(nearest)) mode = riv::M_NEAREST;
-                else if (hpeq(nearp)) mode = riv::M_NEAR;
-                else if (has_tr && hpeq(wTR)) mode = riv::M_TOPRIGHT;
-                else if (has_tl && hpeq(wTL)) mode = riv::M_TOPLEFT;
-                else if (hpeq(temporal)) mode = riv::M_TEMPORAL;
-                else mode = riv::M_NEW;
-                const uint32_t wS = pack(S0, S1, S2, S3);
-                uint32_t wR = wS;
-                uint8_t chosen = 0; int rdx = 0, rdy = 0;
-                if (mode == riv::M_NEW) {   // NEW base by estimated bits (encoder.rs:567-661, :128-159)
-                    const uint32_t bases[5] = {nearest, nearp, wTR, wTL, temporal};
-                    const bool have[5] = {true, true, has_tr, has_tl, true};
-                    const unsigned selbits[5] = {1, 2, 3, 4, 4};
-                    unsigned best = 0xffffffffu; int bbx = (int)(int8_t)(nearest & 0xff), bby = (int)(int8_t)((nearest >> 8) & 0xff);
-#pragma unroll
-                    for (int k = 0; k < 5; k++) {
-                        if (have[k]) {
-                            int kx = (int)(int8_t)(bases[k] & 0xff), ky = (int)(int8_t)((bases[k] >> 8) & 0xff);
-                            int ddx = dclamp(S0 - kx, -128, 127), ddy = dclamp(S1 - ky, -128, 127);
-                            int cxs = mv_class(abs(ddx)), cys = mv_class(abs(ddy));
-                            unsigned cost = selbits[k] + (cxs == 0 ? 4u : 4u + cxs) + (cys == 0 ? 4u : 4u + cys);
-                            if (cost < best) { best = cost; chosen = (uint8_t)k; bbx = kx; bby = ky; }
-                        }
-                    }
-                    rdx = dclamp(S0 - bbx, -128, 127);
-                    rdy = dclamp(S1 - bby, -128, 127);
-                    wR = pack(dclamp(bbx + rdx, -128, 127), dclamp(bby + rdy, -128, 127), S2, S3);
-                } else if (mode == riv::M_NEAREST) wR = nearest;
-                else if (mode == riv::M_NEAR) wR = nearp;
-                else if (mode == riv::M_TOPRIGHT) wR = wTR;
-                else if (mode == riv::M_TOPLEFT) wR = wTL;
-                else if (mode == riv::M_TEMPORAL) wR = temporal;
-                // RdoContext::decide via the host-built table (rdo.rs:140-178)
-                const bool aligned = S2 == 0 && S3 == 0;
-                uint32_t rdo = 0;
-                if (aligned && best_sad <= p.rdo_max) rdo = (p.rdo_bits[best_sad >> 5] >> (best_sad & 31)) & 1u;
-
-                if (lane == 0) {
-                    uint32_t w0 = wS, w1 = wR;
-                    uint32_t w2 = (uint32_t)mode | ((uint32_t)chosen << 8) | ((uint32_t)(uint8_t)rdx << 16) | ((uint32_t)(uint8_t)rdy << 24);
-                    uint32_t w3 = (uint32_t)best_sad | (rdo << 16);
-                    uint4* dst = (uint4*)&p.rec[idx];
-                    *dst = make_uint4(w0, w1, w2, w3);
-                    uint32_t* rb = (uint32_t*)(rowbuf + (w * bw + bx) * 8);
-                    rb[0] = w0; rb[1] = w1;
-                    __hip_atomic_store(&p.progress[row], p.base + (unsigned long long)(bx + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                }
-                lS = wS; lR = wR;
-                if (pf_next) store_window<NPF, NDW>((uint32_t*)(winb0 + ((bx + 1) & 1) * WINB), (uint32_t*)(curb0 + ((bx + 1) & 1) * 256), lane, WS, pf, pcur);
+                const int m = lane_min8(key), win = m & 7;
+                scost = m >> 3;
+                const uint32_t wq = (uint32_t)__builtin_amdgcn_readlane((int)myq, win);
+                sdx = mvx(wq); sdy = mvy(wq);
+                // predictor threshold on the winner's SAD (cost - its MV cost)
+                if (p.pthr > 0 && __builtin_amdgcn_readlane(sv, win) <= p.pthr) skip_int = true;
             }
         }
-        __syncthreads();
-        if (*abortf) break;
+        if (!skip_int) {   // refine_integer_motion (:680-757): large hexagon until no improvement, then small diamond
+            const int hox = lane < 6 ? (int)((0x310134u >> (4 * lane)) & 15u) - 2 : 0;
+            const int hoy = lane < 6 ? (int)((0x002442u >> (4 * lane)) & 15u) - 2 : 0;
+            for (int it = 0; it < 4096; it++) {
+                const int dx = sdx + hox, dy = sdy + hoy;
+                const bool v = lane < 6 && abs(dx) <= sr && abs(dy) <= sr;
+                const int sv = sadt[((v ? dy : sdy) + sr) * D + (v ? dx : sdx) + sr];
+                const int key = v ? ((sv + cc.cost(dx, dy)) << 3) | lane : 0x7fffffff;
+                const int m = lane_min8(key);
+                if ((m >> 3) >= scost) break;
+                scost = m >> 3;
+                sdx = __builtin_amdgcn_readlane(dx, m & 7); sdy = __builtin_amdgcn_readlane(dy, m & 7);
+            }
+            const int dox = lane < 4 ? (int)((0x2213u >> (4 * lane)) & 15u) - 2 : 0;
+            const int doy = lane < 4 ? (int)((0x1322u >> (4 * lane)) & 15u) - 2 : 0;
+            const int dx = sdx + dox, dy = sdy + doy;
+            const bool v = lane < 4 && abs(dx) <= sr && abs(dy) <= sr;
+            const int sv = sadt[((v ? dy : sdy) + sr) * D + (v ? dx : sdx) + sr];
+            const int key = v ? ((sv + cc.cost(dx, dy)) << 3) | lane : 0x7fffffff;
+            const int m = lane_min8(key);
+            if ((m >> 3) < scost) { sdx = __builtin_amdgcn_readlane(dx, m & 7); sdy = __builtin_amdgcn_readlane(dy, m & 7); }
+        }
+        // ---- refine_halfpel (:759-839): the base re-scored with the half-pel kernel, then the 8
+        //      neighbours in (sy outer, sx inner) order at penalty c_pred (every neighbour of an
+        //      even base is fractional). Lane 0 holds the base, lanes 1..8 the neighbours.
+        int best_hx, best_hy, best_sad;
+        {
+            const int sel = lane == 0 ? 4 : (lane <= 4 ? lane - 1 : lane);   // 3x3 raster index
+            const bool v = lane < 9;
+            const int hx = 2 * sdx + (v ? sel % 3 - 1 : 0), hy = 2 * sdy + (v ? sel / 3 - 1 : 0);
+            const int sv = hpt[(hy + HR) * D2 + hx + HR];
+            const int key = v ? ((sv + (lane == 0 ? 0 : p.c_pred)) << 4) | lane : 0x7fffffff;
+            const int m = lane_min16(key), k = m & 15;
+            best_sad = __builtin_amdgcn_readlane(sv, k);
+            best_hx = __builtin_amdgcn_readlane(hx, k); best_hy = __builtin_amdgcn_readlane(hy, k);
+        }
+        if (pf_next && NHP == 1) fetch_table(p.hpt + (size_t)(idx + 1) * p.hpt_stride, hpb0, HSB, lane);
+        // canonicalised search MV (motion/scalar.rs:619-634, motion_vector.rs:103-116)
+        int S0, S1, S2, S3;
+        {
+            int hx = dclamp(best_hx, -255, 255), hy = dclamp(best_hy, -255, 255);
+            int dx = dclamp(hx / 2, -128, 127), dy = dclamp(hy / 2, -128, 127), fx = hx % 2, fy = hy % 2;
+            if (dx == -128 && fx == -1) fx = 0;
+            if (dy == -128 && fy == -1) fy = 0;
+            if (fx == -1) { fx = 1; dx = max(-128, dx - 1); }
+            if (fy == -1) { fy = 1; dy = max(-128, dy - 1); }
+            S0 = dx; S1 = dy; S2 = fx; S3 = fy;
+        }
+
+        // ---- mode decision on the reconstructed prefix (encoder.rs:447-733) ----
+        // derive_mv_predictors_with_stats / gather_mv_candidates (mv_predictor.rs:135-272):
+        // deduplicated (dx,dy,sx,sy) of left, top, top-right, top-left, temporal; the first two
+        // are Nearest / Near, padded with zero. Every MV word here is canonical (sx, sy in {0,1}),
+        // so equal half-pel positions are equal words.
+        auto pack = [](int a, int b, int c, int d) -> uint32_t {
+            return (uint32_t)(uint8_t)a | ((uint32_t)(uint8_t)b << 8) | ((uint32_t)(uint8_t)c << 16) | ((uint32_t)(uint8_t)d << 24);
+        };
+        const uint32_t wL = lR, wT = tR, wTR = trR, wTL = tlR, wP = pT;
+        uint32_t c0 = 0, c1 = 0; int ncand = 0;
+#define RIV_ADDC(present, q)                                                         \
+        {                                                                    \
+            const uint32_t q_ = (q);                                         \
+            const bool add_ = (present) && !(ncand >= 1 && c0 == q_) && !(ncand >= 2 && c1 == q_); \
+            c1 = (add_ && ncand == 1) ? q_ : c1;                             \
+            c0 = (add_ && ncand == 0) ? q_ : c0;                             \
+            ncand += add_ ? 1 : 0;                                           \
+        }
+        RIV_ADDC(has_l, wL) RIV_ADDC(has_t, wT) RIV_ADDC(has_tr, wTR) RIV_ADDC(has_tl, wTL) RIV_ADDC(has_tp, wP)
+#undef RIV_ADDC
+        const uint32_t nearest = ncand > 0 ? c0 : 0u, nearp = ncand > 1 ? c1 : 0u, temporal = has_tp ? wP : 0u;
+        const uint32_t wS = pack(S0, S1, S2, S3);
+        uint8_t mode;
+        if (wS == 0u) mode = riv::M_ZERO;
+        else if (wS == nearest) mode = riv::M_NEAREST;
+        else if (wS == nearp) mode = riv::M_NEAR;
+        else if (has_tr && wS == wTR) mode = riv::M_TOPRIGHT;
+        else if (has_tl && wS == wTL) mode = riv::M_TOPLEFT;
+        else if (wS == temporal) mode = riv::M_TEMPORAL;
+        else mode = riv::M_NEW;
+        uint32_t wR = wS;
+        uint8_t chosen = 0; int rdx = 0, rdy = 0;
+        if (mode == riv::M_NEW) {   // NEW base by estimated bits (encoder.rs:567-661, :128-159): lane k scores base k
+            const uint32_t myb = lane == 0 ? nearest : lane == 1 ? nearp : lane == 2 ? wTR : lane == 3 ? wTL : temporal;
+            const bool have = lane < 2 || (lane == 2 && has_tr) || (lane == 3 && has_tl) || lane == 4;
+            const int ddx = dclamp(S0 - mvx(myb), -128, 127), ddy = dclamp(S1 - mvy(myb), -128, 127);
+            const int cxs = mv_class(abs(ddx)), cys = mv_class(abs(ddy));
+            const int bits_k = (lane < 4 ? lane + 1 : 4) + (cxs == 0 ? 4 : 4 + cxs) + (cys == 0 ? 4 : 4 + cys);
+            const int m = lane_min8(have ? (bits_k << 3) | lane : 0x7fffffff);
+            chosen = (uint8_t)(m & 7);
+            const uint32_t bw_ = (uint32_t)__builtin_amdgcn_readlane((int)myb, m & 7);
+            const int bbx = mvx(bw_), bby = mvy(bw_);
+            rdx = dclamp(S0 - bbx, -128, 127);
+            rdy = dclamp(S1 - bby, -128, 127);
+            wR = pack(dclamp(bbx + rdx, -128, 127), dclamp(bby + rdy, -128, 127), S2, S3);
+        } else if (mode == riv::M_NEAREST) wR = nearest;
+        else if (mode == riv::M_NEAR) wR = nearp;
+        else if (mode == riv::M_TOPRIGHT) wR = wTR;
+        else if (mode == riv::M_TOPLEFT) wR = wTL;
+        else if (mode == riv::M_TEMPORAL) wR = temporal;
+        // RdoContext::decide via the host-built table (rdo.rs:140-178)
+        const bool aligned = S2 == 0 && S3 == 0;
+        uint32_t rdo = 0;
+        if (aligned && best_sad <= p.rdo_max)
+            rdo = ((best_sad >> 5) < 256 ? rdol[best_sad >> 5] : p.rdo_bits[best_sad >> 5]) >> (best_sad & 31) & 1u;
+        const uint32_t w2 = (uint32_t)mode | ((uint32_t)chosen << 8) | ((uint32_t)(uint8_t)rdx << 16) | ((uint32_t)(uint8_t)rdy << 24);
+        const uint32_t w3 = (uint32_t)best_sad | (rdo << 16);
+        if (lane == 0) {
+            if (publish)
+                __hip_atomic_store(&p.pub[idx], (unsigned long long)mv18(wS) | ((unsigned long long)mv18(wR) << 18) | ((unsigned long long)p.tag << 36),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            *(uint4*)(rowbuf + (w * bw + bx) * 4) = make_uint4(wS, wR, w2, w3);
+            if (w + 1 < R) __hip_atomic_store(&lprog[w], bx + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        lS = wS; lR = wR;
     }
+    if (dead) {
+        if (lane == 0) {
+            __hip_atomic_store(abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_store(p.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        return;
+    }
+    // the row's records, LDS -> HBM in 16-byte lanes
+    for (int i = lane; i < bw; i += 64) *(uint4*)&p.rec[row * bw + i] = *(const uint4*)(rowbuf + (w * bw + i) * 4);
 }
 
 // ============================================================================================
@@ -1010,24 +1098,26 @@ void yuv420_to_rgb(hipStream_t st, const uint8_t* Y, const uint8_t* U, const uin
     hipLaunchKernelGGL(k_yuv420_to_rgb, g, dim3(256), 0, st, Y, U, V, sw, dw, dh, rgb);
 }
 
-// dwords per LDS window row: the aligned span of 16 + 2*sr + 2 bytes starting anywhere in a dword
-static int walk_ndw(int sr) { return (16 + 2 * sr + 2 + 3 + 3) / 4 <= 16 ? 16 : 32; }
-size_t walk_lds_bytes(int sr, int bw, int R) {
-    int WS = 16 + 2 * sr + 2, WSP = 4 * (walk_ndw(sr) + 1), WINB = (WS * WSP + 15) & ~15;
-    return (size_t)R * (2 * WINB + 512) + (size_t)((R * bw * 8 + 15) & ~15) + 256 * sizeof(double) + 16;
+int sad_table_stride(int sr) { return ((2 * sr + 1) * (2 * sr + 1) + 7) & ~7; }
+int hp_table_stride(int sr) { return ((4 * sr + 3) * (4 * sr + 3) + 7) & ~7; }
+size_t walk_lds_bytes(int sr, int bw, int R, int hp_slots) {
+    return (size_t)R * (4 * sad_table_stride(sr) + 2 * hp_slots * hp_table_stride(sr)) + (size_t)R * bw * 16 + 256 * sizeof(double) + 256 * 4 + 64;
 }
-int walk_npf(int sr) {
-    int WS = 16 + 2 * sr + 2;
-    return (WS * walk_ndw(sr) + 63) / 64;
+void sad_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int H, int sr, uint16_t* out) {
+    const int ndwt = (16 + 2 * sr + 6) / 4 + 1;
+    const size_t lds = (size_t)(64 + 4 * (16 + 2 * sr) * ndwt) * 4;
+    hipLaunchKernelGGL(k_sad_table, dim3((W / 16) * (H / 16)), dim3(256), lds, st, cur, ref, W, H, W / 16, sr, ndwt, out, sad_table_stride(sr));
+}
+static constexpr int HPK = 8;   // y-offsets per half-pel-table thread
+void hp_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int H, int sr, uint16_t* out) {
+    const int ndp = (2 * sr + 4) / 4 + 6, WS = 16 + 2 * sr + 2;
+    const size_t lds = (size_t)(64 + (WS + HPK + 1) * (ndp + 1) + 4 * (WS + HPK) * ndp) * 4;
+    hipLaunchKernelGGL(k_hp_table<HPK>, dim3((W / 16) * (H / 16)), dim3(128), lds, st, cur, ref, W, H, W / 16, sr, ndp, out, hp_table_stride(sr));
 }
 hipError_t me_walk(hipStream_t st, const WalkParams& p) {
     int nbands = (p.bh + p.R - 1) / p.R;
-    size_t lds = walk_lds_bytes(p.sr, p.bw, p.R);
-    const int npf = walk_npf(p.sr), ndw = walk_ndw(p.sr);
-    if (ndw == 16 && npf <= 11) hipLaunchKernelGGL((k_me_walk<11, 16>), dim3(nbands), dim3(64 * p.R), lds, st, p);
-    else if (ndw == 16) hipLaunchKernelGGL((k_me_walk<14, 16>), dim3(nbands), dim3(64 * p.R), lds, st, p);
-    else if (npf <= 41) hipLaunchKernelGGL((k_me_walk<41, 32>), dim3(nbands), dim3(64 * p.R), lds, st, p);
-    else return hipErrorInvalidValue;
+    size_t lds = walk_lds_bytes(p.sr, p.bw, p.R, p.hp_slots);
+    hipLaunchKernelGGL(k_me_walk, dim3(nbands), dim3(64 * p.R), lds, st, p);
     return hipGetLastError();
 }
 void mc(hipStream_t st, const uint8_t* rY, const uint8_t* rU, const uint8_t* rV, int W, int H, const int8_t* mvs, int mv_stride,

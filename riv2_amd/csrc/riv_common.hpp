@@ -23,14 +23,17 @@ static_assert(sizeof(BlockRec) == 16, "BlockRec layout");
 
 // Parameters of the motion-search + mode-decision wavefront for one frame.
 struct WalkParams {
-    const uint8_t* cur;          // current luma, W x H
-    const uint8_t* ref;          // reference luma (previous reconstruction), W x H
     int W, H, bw, bh, sr;
     const int8_t* prev_mv;       // previous P-frame's reconstructed MVs (dx,dy,sx,sy) or nullptr
     int prev_stride;             // bytes between consecutive MVs of prev_mv
+    const uint16_t* sadt;        // per block, integer SAD of every (dx,dy) in [-sr,sr]^2, row-major by dy
+    int sadt_stride;             // entries between consecutive blocks' tables, a multiple of 8
+    const uint16_t* hpt;         // per block, half-pel SAD of every (hx,hy) in [-2sr-1,2sr+1]^2, row-major by hy
+    int hpt_stride;              // entries between consecutive blocks' tables, a multiple of 8
+    int hp_slots;                // LDS slots per wave for the half-pel table: 2 (prefetched a block ahead) or 1
     BlockRec* rec;               // bw*bh records out
-    unsigned long long* progress;   // per MB row, monotonic: base + number of finished blocks
-    unsigned long long base;
+    unsigned long long* pub;     // bw*bh published MV pairs (bits 0-17 search, 18-35 reconstructed, 36-63 tag)
+    unsigned tag;                // this frame's tag, 1 .. 2^28-1; differs from the previous launch's
     const double* bits;          // 256 entries: 1.0 for 0, else 2*log2(a)+2 (motion/scalar.rs:495-525)
     double lam2;                 // lambda * 2.0
     int c_zero, c_pred;          // (lam2*0.5) as i64, (lam2*1.5) as i64; c_pred is also the half-pel penalty
@@ -38,7 +41,7 @@ struct WalkParams {
     const uint32_t* rdo_bits;    // bit s set -> RDO chooses skip at SAD s (integer-aligned MV)
     int rdo_max;                 // largest SAD that can skip, -1 when skipping is disabled
     long long zthr, pthr;        // me_zero_mv_threshold, me_predictor_threshold
-    int R;                       // MB rows per workgroup
+    int R;                       // MB rows per workgroup, at most 15
     int* err;                    // set non-zero on a wavefront timeout
 };
 

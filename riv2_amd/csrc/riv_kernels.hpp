@@ -9,8 +9,11 @@ namespace rivk {
 hipError_t upload_bases(const int32_t* b16, const int32_t* b8);
 void rgb_to_yuv420(hipStream_t st, const uint8_t* rgb, int dw, int dh, int sw, int sh, uint8_t* Y, uint8_t* U, uint8_t* V);
 void yuv420_to_rgb(hipStream_t st, const uint8_t* Y, const uint8_t* U, const uint8_t* V, int sw, int dw, int dh, uint8_t* rgb);
-size_t walk_lds_bytes(int sr, int bw, int R);
-int walk_npf(int sr);
+int sad_table_stride(int sr);
+void sad_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int H, int sr, uint16_t* out);
+int hp_table_stride(int sr);
+void hp_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int H, int sr, uint16_t* out);
+size_t walk_lds_bytes(int sr, int bw, int R, int hp_slots);
 hipError_t me_walk(hipStream_t st, const riv::WalkParams& p);
 void mc(hipStream_t st, const uint8_t* rY, const uint8_t* rU, const uint8_t* rV, int W, int H, const int8_t* mvs, int mv_stride,
         uint8_t* pY, uint8_t* pU, uint8_t* pV, uint32_t* var_sums);
