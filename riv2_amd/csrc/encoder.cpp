@@ -83,9 +83,9 @@ struct riv_encoder {
     riv_video_header hdr{};
     uint64_t frame_count = 0;      // frames submitted
 
-    int W = 0, H = 0, bw = 0, bh = 0, nb = 0, R = 4, hp_slots = 2;
+    int W = 0, H = 0, bw = 0, bh = 0, nb = 0, R = 4, tab_slots = 2;
     hipStream_t st = nullptr;
-    DevBuf rgb, cur, pred, ref[2], var, lnv, steps, mask, pub, sadt, hpt, bits, rdo, err, yt, ct;
+    DevBuf rgb, cur, pred, ref[2], var, lnv, steps, mask, pub, tabs, smv, bits, rdo, err, yt, ct;
     int ref_idx = 0;
     bool have_ref = false;
     int prev_inter_slot = -1;      // slot holding the previous P-frame's records, -1 after an intra frame
@@ -170,15 +170,15 @@ static int encoder_init(riv_encoder* e) {
     if (r) return r;
     e->W = (int)c.storage_width; e->H = (int)c.storage_height;
     e->bw = e->W / 16; e->bh = e->H / 16; e->nb = e->bw * e->bh;
-    // wavefront geometry: up to 4 MB rows per workgroup and a half-pel table prefetched a block
-    // ahead, as far as the 64 KiB of LDS of one workgroup allow
+    // wavefront geometry: up to 4 MB rows per workgroup, each with the next block's SAD tables
+    // prefetched into a second LDS slot, as far as the 64 KiB of LDS of one workgroup allow
     {
         const size_t budget = 65536;
-        e->hp_slots = 2;
-        if (rivk::walk_lds_bytes(c.search_range, e->bw, 1, 2) > budget) e->hp_slots = 1;
+        e->tab_slots = 2;
+        if (rivk::walk_lds_bytes(c.search_range, e->bw, 1, 2) > budget) e->tab_slots = 1;
         e->R = 4;
-        while (e->R > 1 && rivk::walk_lds_bytes(c.search_range, e->bw, e->R, e->hp_slots) > budget) e->R--;
-        if (rivk::walk_lds_bytes(c.search_range, e->bw, e->R, e->hp_slots) > budget) {
+        while (e->R > 1 && rivk::walk_lds_bytes(c.search_range, e->bw, e->R, e->tab_slots) > budget) e->R--;
+        if (rivk::walk_lds_bytes(c.search_range, e->bw, e->R, e->tab_slots) > budget) {
             set_error("frame too wide for the motion-search workgroup's LDS at this search range");
             return RIV_E_UNSUPPORTED;
         }
@@ -191,8 +191,8 @@ static int encoder_init(riv_encoder* e) {
     RIV_HIP(e->var.alloc(8 * (size_t)e->nb)); RIV_HIP(e->lnv.alloc(4 * (size_t)e->nb)); RIV_HIP(e->steps.alloc(4 * (size_t)e->nb));
     RIV_HIP(e->mask.alloc(e->nb));
     RIV_HIP(e->pub.alloc(8 * (size_t)e->nb)); RIV_HIP(hipMemset(e->pub.p, 0, 8 * (size_t)e->nb));
-    RIV_HIP(e->sadt.alloc(2 * (size_t)e->nb * rivk::sad_table_stride(c.search_range)));
-    RIV_HIP(e->hpt.alloc(2 * (size_t)e->nb * rivk::hp_table_stride(c.search_range)));
+    RIV_HIP(e->tabs.alloc((size_t)e->nb * rivk::table_region_bytes(c.search_range)));
+    RIV_HIP(e->smv.alloc(8 * (size_t)e->nb));
     RIV_HIP(e->err.alloc(4)); RIV_HIP(hipMemset(e->err.p, 0, 4));
     e->depth = c.pipeline_depth;
     e->slots.resize(std::max(2, e->depth));
@@ -483,23 +483,27 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
         RIV_HIP(hipMemcpyAsync(s.h_coeffs.p, s.coeffs.p, 768 * (size_t)nb, hipMemcpyDeviceToHost, st));
         mark(7);
     } else {
-        rivk::sad_table(st, cY, rY, W, H, c.search_range, e->sadt.as<uint16_t>());
-        rivk::hp_table(st, cY, rY, W, H, c.search_range, e->hpt.as<uint16_t>());
+        const int sr = c.search_range, RB = rivk::table_region_bytes(sr), TSB = 2 * rivk::sad_table_stride(sr);
+        uint8_t* tabs = e->tabs.as<uint8_t>();
+        rivk::sad_table(st, cY, rY, W, H, sr, (uint16_t*)tabs, RB / 2);
+        rivk::hp_table(st, cY, rY, W, H, sr, (uint16_t*)(tabs + TSB), RB / 2);
+        const int8_t* prev_mv = e->prev_inter_slot >= 0 ? (const int8_t*)e->slots[e->prev_inter_slot].rec.p + 4 : nullptr;
         riv::WalkParams p{};
-        p.W = W; p.H = H; p.bw = e->bw; p.bh = e->bh; p.sr = c.search_range;
-        p.prev_mv = e->prev_inter_slot >= 0 ? (const int8_t*)e->slots[e->prev_inter_slot].rec.p + 4 : nullptr;
-        p.prev_stride = sizeof(BlockRec);
-        p.sadt = e->sadt.as<uint16_t>(); p.sadt_stride = rivk::sad_table_stride(c.search_range);
-        p.hpt = e->hpt.as<uint16_t>(); p.hpt_stride = rivk::hp_table_stride(c.search_range); p.hp_slots = e->hp_slots;
-        p.rec = rec; p.pub = e->pub.as<unsigned long long>();
+        p.W = W; p.H = H; p.bw = e->bw; p.bh = e->bh; p.sr = sr;
+        p.prev_mv = prev_mv; p.prev_stride = sizeof(BlockRec);
+        p.tabs = tabs; p.tab_bytes = RB; p.tsb = TSB; p.slots = e->tab_slots;
+        p.smv = e->smv.as<unsigned long long>(); p.pub = e->pub.as<unsigned long long>();
         e->epoch = e->epoch % 0x0fffffffull + 1;   // 1 .. 2^28-1, never the zero of a fresh buffer
         p.tag = (unsigned)e->epoch;
         p.bits = e->bits.as<double>(); p.lam2 = e->lambda * 2.0; p.c_zero = e->c_zero; p.c_pred = e->c_pred;
         p.lam_zero = e->lambda == 0.0;
-        p.rdo_bits = e->rdo.as<uint32_t>(); p.rdo_max = e->rdo_max;
         p.zthr = c.me_zero_mv_threshold; p.pthr = c.me_predictor_threshold;
         p.R = e->R; p.err = e->err.as<int>();
         RIV_HIP(rivk::me_walk(st, p));
+        riv::ModeParams mp{};
+        mp.bw = e->bw; mp.bh = e->bh; mp.smv = p.smv; mp.prev_mv = prev_mv; mp.prev_stride = sizeof(BlockRec);
+        mp.rdo_bits = e->rdo.as<uint32_t>(); mp.rdo_max = e->rdo_max; mp.rec = rec; mp.err = e->err.as<int>();
+        rivk::mode(st, mp);
         mark(1);
         rivk::mc(st, rY, rU, rV, W, H, (const int8_t*)rec, sizeof(BlockRec), pY, pU, pV, e->var.as<uint32_t>());
         mark(2);
@@ -533,7 +537,7 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
     if (!intra) {
         int werr = 0;
         RIV_HIP(hipMemcpy(&werr, e->err.p, 4, hipMemcpyDeviceToHost));
-        if (werr) { set_error("motion-search wavefront timed out"); return RIV_E_GPU; }
+        if (werr) { set_error(werr == 2 ? "mode decision: reconstructed MV differs from search MV" : "motion-search wavefront timed out"); return RIV_E_GPU; }
     }
     std::lock_guard<std::mutex> lk(e->mu);
     if (e->err_code) { set_error(e->err_msg); return e->err_code; }
@@ -560,7 +564,7 @@ int riv_encoder_flush(riv_encoder* e) {
     lk.unlock();
     RIV_HIP(hipSetDevice(e->cfg.device_id));
     RIV_HIP(hipMemcpy(&werr, e->err.p, 4, hipMemcpyDeviceToHost));
-    if (werr) { set_error("motion-search wavefront timed out"); return RIV_E_GPU; }
+    if (werr) { set_error(werr == 2 ? "mode decision: reconstructed MV differs from search MV" : "motion-search wavefront timed out"); return RIV_E_GPU; }
     return RIV_OK;
 }
 

@@ -8,6 +8,7 @@
 using namespace rivdev;
 using riv::BlockRec;
 using riv::WalkParams;
+using riv::ModeParams;
 
 namespace rivdev {
 __constant__ int32_t c_B16[16][16];
@@ -267,6 +268,16 @@ __device__ __forceinline__ int lane_min16(int v) {
     return __builtin_amdgcn_readlane(v, 0);
 }
 
+__device__ __forceinline__ uint32_t sgpr(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+// per-lane selection among wave-uniform values: lane k of the result is vals[k] for k < N, else
+// vals[0] (a chain of lane compares and selects, no branches)
+template <int N>
+__device__ __forceinline__ uint32_t lane_pick(int lane, const uint32_t (&vals)[N]) {
+    uint32_t r = vals[0] & (0u - (uint32_t)(lane == 0 || lane >= N));
+#pragma unroll
+    for (int k = 1; k < N; k++) r |= vals[k] & (0u - (uint32_t)(lane == k));   // masks, not a table lookup
+    return r;
+}
 __device__ __forceinline__ int mvx(uint32_t w) { return (int)(int8_t)(w & 0xff); }
 __device__ __forceinline__ int mvy(uint32_t w) { return (int)(int8_t)((w >> 8) & 0xff); }
 __device__ __forceinline__ int mv_class(int mag) {   // mv_class_from_magnitude (mv_rans.rs:81-87)
@@ -280,13 +291,14 @@ __device__ __forceinline__ int mv_class(int mag) {   // mv_class_from_magnitude 
 __device__ __forceinline__ uint32_t mv18(uint32_t w) { return (w & 0xffffu) | (((w >> 16) & 1u) << 16) | (((w >> 24) & 1u) << 17); }
 __device__ __forceinline__ uint32_t mv32(uint32_t v) { return (v & 0xffffu) | (((v >> 16) & 1u) << 16) | (((v >> 17) & 1u) << 24); }
 
-// Copies one block's table (nb bytes, a multiple of 16) from HBM straight into LDS at dst: each
-// lane moves 16 bytes per 1 KB step. Completion is tracked by the wave's vector-memory counter;
-// the consumer waits for it before reading dst.
-__device__ __forceinline__ void fetch_table(const uint16_t* src, uint8_t* dst, int nb, int lane) {
-    for (int o = 0; o < nb; o += 1024)
-        if (o + 16 * lane < nb)
-            __builtin_amdgcn_global_load_lds((const void*)((const uint8_t*)src + o + 16 * lane), (void*)(dst + o), 16, 0, 0);
+// Copies one block's table region (nb bytes, a multiple of 16) from HBM straight into LDS at dst:
+// each lane moves 16 bytes per 1 KB step. Completion is tracked by the wave's vector-memory
+// counter; the consumer waits for it before reading dst.
+__device__ __forceinline__ void fetch_region(const uint8_t* src, uint8_t* dst, int nb, int lane) {
+    const uint8_t* s8 = src + 16 * lane;
+    int o = 0;
+    for (; o + 1024 <= nb; o += 1024) __builtin_amdgcn_global_load_lds((const void*)(s8 + o), (void*)(dst + o), 16, 0, 0);
+    if (o < nb && 16 * lane < nb - o) __builtin_amdgcn_global_load_lds((const void*)(s8 + o), (void*)(dst + o), 16, 0, 0);
 }
 
 // A wave-uniform 64-bit value moved to scalar registers (readfirstlane returns a signed int:
@@ -313,28 +325,23 @@ __device__ __forceinline__ bool wait_pub(const unsigned long long* a, unsigned t
 
 __global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
     extern __shared__ __align__(16) uint8_t lds[];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));   // wave index, kept scalar
     const int R = p.R, row = blockIdx.x * R + w, bw = p.bw, sr = p.sr;
-    const int TSB = 2 * p.sadt_stride, HSB = 2 * p.hpt_stride, NHP = p.hp_slots, D2 = 4 * sr + 3, HR = 2 * sr + 1;
-    const int WAVEB = 2 * TSB + NHP * HSB;
-    uint8_t* tabb0 = lds + w * WAVEB;         // two integer-table slots
-    uint8_t* hpb0 = tabb0 + 2 * TSB;          // NHP half-pel-table slots
-    uint32_t* rowbuf = (uint32_t*)(lds + R * WAVEB);   // per row and block: search MV, recon MV, mode word, SAD word
-    double* bits = (double*)(rowbuf + R * bw * 4);
-    uint32_t* rdol = (uint32_t*)(bits + 256);  // first 256 words of the RDO skip bitmask
-    int* abortf = (int*)(rdol + 256);
+    const int RB = p.tab_bytes, TSB = p.tsb, NS = p.slots, D2 = 4 * sr + 3, HR = 2 * sr + 1;
+    uint8_t* slot0 = lds + w * NS * RB;                        // NS table slots of this wave
+    uint32_t* rowbuf = (uint32_t*)(lds + R * NS * RB);         // per row and block: search MV word, SAD
+    double* bits = (double*)(rowbuf + R * bw * 2);
+    int* abortf = (int*)(bits + 256);
     int* lprog = abortf + 1;   // per wave: number of finished blocks of its row
     for (int i = threadIdx.x; i < 256; i += blockDim.x) bits[i] = p.bits[i];
-    const int nrdo = min(256, (p.rdo_max >> 5) + 1);
-    for (int i = threadIdx.x; i < nrdo; i += blockDim.x) rdol[i] = p.rdo_bits[i];
     if (threadIdx.x == 0) *abortf = 0;
     if (threadIdx.x < R) lprog[threadIdx.x] = 0;
     const bool row_valid = row < p.bh;
     const bool has_tp = p.prev_mv != nullptr;
     uint32_t pTn = 0;
     if (row_valid) {
-        fetch_table(p.sadt + (size_t)row * bw * p.sadt_stride, tabb0, TSB, lane);
-        fetch_table(p.hpt + (size_t)row * bw * p.hpt_stride, hpb0, HSB, lane);
+        fetch_region(p.tabs + (size_t)row * bw * RB, slot0, RB, lane);
         if (has_tp) pTn = *(const uint32_t*)(p.prev_mv + (size_t)p.prev_stride * row * bw);
     }
     __syncthreads();
@@ -343,27 +350,27 @@ __global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
     const bool has_t = row > 0, from_hbm = has_t && w == 0, publish = w == R - 1 && row + 1 < p.bh;
     const unsigned long long* above = p.pub + (size_t)(has_t ? row - 1 : 0) * bw;
     bool dead = false;
-    // MVs are carried packed: bytes (dx, dy, sx, sy) of one 32-bit word. Top neighbours slide
-    // along the row: top-left <- top <- top-right.
-    uint32_t lS = 0, lR = 0, tlS = 0, tlR = 0, tS = 0, tR = 0, trS = 0, trR = 0;
+    // search MVs are carried packed: bytes (dx, dy, sx, sy) of one 32-bit word. Top neighbours
+    // slide along the row: top-left <- top <- top-right.
+    uint32_t lS = 0, tlS = 0, tS = 0, trS = 0;
     unsigned long long spec = 0;   // speculative load of the next top-right published word
     if (from_hbm) {
         unsigned long long v;
         dead = !wait_pub(above, p.tag, p.err, v);
-        tS = mv32((uint32_t)v & 0x3ffffu); tR = mv32((uint32_t)(v >> 18) & 0x3ffffu);
+        tS = mv32((uint32_t)v & 0x3ffffu);
         if (bw > 1) spec = __hip_atomic_load(above + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     for (int bx = 0; bx < bw && !dead; bx++) {
         const int idx = row * bw + bx;
         const bool has_l = bx > 0, has_tr = has_t && bx + 1 < bw, has_tl = has_t && bx > 0;
-        // ---- top neighbours (search + reconstructed MVs) ----
+        // ---- top neighbours ----
         if (from_hbm) {
-            if (bx > 0) { tlS = tS; tlR = tR; tS = trS; tR = trR; }
-            trS = 0; trR = 0;
+            if (bx > 0) { tlS = tS; tS = trS; }
+            trS = 0;
             if (has_tr) {
                 unsigned long long v = uniform64(spec);
                 if ((unsigned)(v >> 36) != p.tag && !wait_pub(above + bx + 1, p.tag, p.err, v)) { dead = true; break; }
-                trS = mv32((uint32_t)v & 0x3ffffu); trR = mv32((uint32_t)(v >> 18) & 0x3ffffu);
+                trS = mv32((uint32_t)v & 0x3ffffu);
                 if (bx + 2 < bw) spec = __hip_atomic_load(above + bx + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         } else if (has_t) {
@@ -374,21 +381,22 @@ __global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
                 __builtin_amdgcn_s_sleep(1);
             }
             if (dead) break;
-            const uint32_t* q = rowbuf + ((w - 1) * bw + bx) * 4;
-            tlS = bx > 0 ? q[-4] : 0u; tlR = bx > 0 ? q[-3] : 0u;
-            tS = q[0]; tR = q[1];
-            trS = bx + 1 < bw ? q[4] : 0u; trR = bx + 1 < bw ? q[5] : 0u;
+            // the row above's MVs are wave-uniform: keep them in scalar registers
+            const uint32_t* q = rowbuf + ((w - 1) * bw + bx) * 2;
+            tlS = bx > 0 ? sgpr(q[-2]) : 0u;
+            tS = sgpr(q[0]);
+            trS = bx + 1 < bw ? sgpr(q[2]) : 0u;
         }
-        const uint32_t pT = pTn;
+        const uint32_t pT = sgpr(pTn);
         // this block's tables have landed in LDS once every load issued so far is done
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        const uint16_t* sadt = (const uint16_t*)(tabb0 + (bx & 1) * TSB);
-        const uint16_t* hpt = (const uint16_t*)(hpb0 + (NHP == 2 ? (bx & 1) * HSB : 0));
+        const uint8_t* cslot = slot0 + (NS == 2 ? (bx & 1) * RB : 0);
+        const uint16_t* sadt = (const uint16_t*)cslot;
+        const uint16_t* hpt = (const uint16_t*)(cslot + TSB);
         // prefetch the next block's tables and temporal MV while this block is walked
         const bool pf_next = bx + 1 < bw;
         if (pf_next) {
-            fetch_table(p.sadt + (size_t)(idx + 1) * p.sadt_stride, tabb0 + ((bx + 1) & 1) * TSB, TSB, lane);
-            if (NHP == 2) fetch_table(p.hpt + (size_t)(idx + 1) * p.hpt_stride, hpb0 + ((bx + 1) & 1) * HSB, HSB, lane);
+            if (NS == 2) fetch_region(p.tabs + (size_t)(idx + 1) * RB, slot0 + ((bx + 1) & 1) * RB, RB, lane);
             if (has_tp) pTn = *(const uint32_t*)(p.prev_mv + (size_t)p.prev_stride * (idx + 1));
         }
 
@@ -432,7 +440,8 @@ __global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
         int scost;   // SAD + MV cost of the current integer best
         bool skip_int;
         {
-            const uint32_t myq = lane == 1 ? q0 : lane == 2 ? q1 : lane == 3 ? q2 : lane == 4 ? q3 : lane == 5 ? q4 : 0u;
+            const uint32_t cand[6] = {0u, q0, q1, q2, q3, q4};   // lane i+1 <- predictor i
+            const uint32_t myq = lane_pick(lane, cand);
             const bool myv = lane < 6 && ((incm >> lane) & 1u);
             const int dx = myv ? mvx(myq) : 0, dy = myv ? mvy(myq) : 0;
             const int sv = sadt[(dy + sr) * D + dx + sr];
@@ -485,7 +494,7 @@ __global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
             best_sad = __builtin_amdgcn_readlane(sv, k);
             best_hx = __builtin_amdgcn_readlane(hx, k); best_hy = __builtin_amdgcn_readlane(hy, k);
         }
-        if (pf_next && NHP == 1) fetch_table(p.hpt + (size_t)(idx + 1) * p.hpt_stride, hpb0, HSB, lane);
+        if (pf_next && NS == 1) fetch_region(p.tabs + (size_t)(idx + 1) * RB, slot0, RB, lane);
         // canonicalised search MV (motion/scalar.rs:619-634, motion_vector.rs:103-116)
         int S0, S1, S2, S3;
         {
@@ -498,71 +507,14 @@ __global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
             S0 = dx; S1 = dy; S2 = fx; S3 = fy;
         }
 
-        // ---- mode decision on the reconstructed prefix (encoder.rs:447-733) ----
-        // derive_mv_predictors_with_stats / gather_mv_candidates (mv_predictor.rs:135-272):
-        // deduplicated (dx,dy,sx,sy) of left, top, top-right, top-left, temporal; the first two
-        // are Nearest / Near, padded with zero. Every MV word here is canonical (sx, sy in {0,1}),
-        // so equal half-pel positions are equal words.
-        auto pack = [](int a, int b, int c, int d) -> uint32_t {
-            return (uint32_t)(uint8_t)a | ((uint32_t)(uint8_t)b << 8) | ((uint32_t)(uint8_t)c << 16) | ((uint32_t)(uint8_t)d << 24);
-        };
-        const uint32_t wL = lR, wT = tR, wTR = trR, wTL = tlR, wP = pT;
-        uint32_t c0 = 0, c1 = 0; int ncand = 0;
-#define RIV_ADDC(present, q)                                                         \
-        {                                                                    \
-            const uint32_t q_ = (q);                                         \
-            const bool add_ = (present) && !(ncand >= 1 && c0 == q_) && !(ncand >= 2 && c1 == q_); \
-            c1 = (add_ && ncand == 1) ? q_ : c1;                             \
-            c0 = (add_ && ncand == 0) ? q_ : c0;                             \
-            ncand += add_ ? 1 : 0;                                           \
-        }
-        RIV_ADDC(has_l, wL) RIV_ADDC(has_t, wT) RIV_ADDC(has_tr, wTR) RIV_ADDC(has_tl, wTL) RIV_ADDC(has_tp, wP)
-#undef RIV_ADDC
-        const uint32_t nearest = ncand > 0 ? c0 : 0u, nearp = ncand > 1 ? c1 : 0u, temporal = has_tp ? wP : 0u;
-        const uint32_t wS = pack(S0, S1, S2, S3);
-        uint8_t mode;
-        if (wS == 0u) mode = riv::M_ZERO;
-        else if (wS == nearest) mode = riv::M_NEAREST;
-        else if (wS == nearp) mode = riv::M_NEAR;
-        else if (has_tr && wS == wTR) mode = riv::M_TOPRIGHT;
-        else if (has_tl && wS == wTL) mode = riv::M_TOPLEFT;
-        else if (wS == temporal) mode = riv::M_TEMPORAL;
-        else mode = riv::M_NEW;
-        uint32_t wR = wS;
-        uint8_t chosen = 0; int rdx = 0, rdy = 0;
-        if (mode == riv::M_NEW) {   // NEW base by estimated bits (encoder.rs:567-661, :128-159): lane k scores base k
-            const uint32_t myb = lane == 0 ? nearest : lane == 1 ? nearp : lane == 2 ? wTR : lane == 3 ? wTL : temporal;
-            const bool have = lane < 2 || (lane == 2 && has_tr) || (lane == 3 && has_tl) || lane == 4;
-            const int ddx = dclamp(S0 - mvx(myb), -128, 127), ddy = dclamp(S1 - mvy(myb), -128, 127);
-            const int cxs = mv_class(abs(ddx)), cys = mv_class(abs(ddy));
-            const int bits_k = (lane < 4 ? lane + 1 : 4) + (cxs == 0 ? 4 : 4 + cxs) + (cys == 0 ? 4 : 4 + cys);
-            const int m = lane_min8(have ? (bits_k << 3) | lane : 0x7fffffff);
-            chosen = (uint8_t)(m & 7);
-            const uint32_t bw_ = (uint32_t)__builtin_amdgcn_readlane((int)myb, m & 7);
-            const int bbx = mvx(bw_), bby = mvy(bw_);
-            rdx = dclamp(S0 - bbx, -128, 127);
-            rdy = dclamp(S1 - bby, -128, 127);
-            wR = pack(dclamp(bbx + rdx, -128, 127), dclamp(bby + rdy, -128, 127), S2, S3);
-        } else if (mode == riv::M_NEAREST) wR = nearest;
-        else if (mode == riv::M_NEAR) wR = nearp;
-        else if (mode == riv::M_TOPRIGHT) wR = wTR;
-        else if (mode == riv::M_TOPLEFT) wR = wTL;
-        else if (mode == riv::M_TEMPORAL) wR = temporal;
-        // RdoContext::decide via the host-built table (rdo.rs:140-178)
-        const bool aligned = S2 == 0 && S3 == 0;
-        uint32_t rdo = 0;
-        if (aligned && best_sad <= p.rdo_max)
-            rdo = ((best_sad >> 5) < 256 ? rdol[best_sad >> 5] : p.rdo_bits[best_sad >> 5]) >> (best_sad & 31) & 1u;
-        const uint32_t w2 = (uint32_t)mode | ((uint32_t)chosen << 8) | ((uint32_t)(uint8_t)rdx << 16) | ((uint32_t)(uint8_t)rdy << 24);
-        const uint32_t w3 = (uint32_t)best_sad | (rdo << 16);
+        const uint32_t wS = sgpr((uint32_t)(uint8_t)S0 | ((uint32_t)(uint8_t)S1 << 8) | ((uint32_t)S2 << 16) | ((uint32_t)S3 << 24));
         if (lane == 0) {
             if (publish)
-                __hip_atomic_store(&p.pub[idx], (unsigned long long)mv18(wS) | ((unsigned long long)mv18(wR) << 18) | ((unsigned long long)p.tag << 36),
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            *(uint4*)(rowbuf + (w * bw + bx) * 4) = make_uint4(wS, wR, w2, w3);
+                __hip_atomic_store(&p.pub[idx], (unsigned long long)mv18(wS) | ((unsigned long long)p.tag << 36), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            *(uint2*)(rowbuf + (w * bw + bx) * 2) = make_uint2(wS, (uint32_t)best_sad);
             if (w + 1 < R) __hip_atomic_store(&lprog[w], bx + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
-        lS = wS; lR = wR;
+        lS = wS;
     }
     if (dead) {
         if (lane == 0) {
@@ -571,8 +523,80 @@ __global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
         }
         return;
     }
-    // the row's records, LDS -> HBM in 16-byte lanes
-    for (int i = lane; i < bw; i += 64) *(uint4*)&p.rec[row * bw + i] = *(const uint4*)(rowbuf + (w * bw + i) * 4);
+    // the row's search results, LDS -> HBM
+    for (int i = lane; i < bw; i += 64) p.smv[row * bw + i] = *(const unsigned long long*)(rowbuf + (w * bw + i) * 2);
+}
+
+// ============================================================================================
+// MV mode decision
+// ============================================================================================
+// Mode decision on the reconstructed prefix (encoder.rs:447-733), one thread per MB. It needs
+// the reconstructed MVs of the left, top, top-right and top-left MBs and of the co-located MB
+// of the previous P-frame (derive_mv_predictors_with_stats / gather_mv_candidates,
+// mv_predictor.rs:135-272). A reconstructed MV equals the search MV: for a matched mode it is
+// the matched candidate, which equals the search MV; for NEW it is base + clamp(search - base),
+// and with every MV inside [-sr-1, sr] (sr <= 32) the difference never reaches the clamp. So
+// the whole frame's decisions follow from the search field in parallel. The identity is still
+// checked for every NEW block: a violation sets *err = 2.
+__global__ __launch_bounds__(256) void k_mode(ModeParams p) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x, bw = p.bw;
+    if (idx >= bw * p.bh) return;
+    const int bx = idx % bw, by = idx / bw;
+    const bool has_l = bx > 0, has_t = by > 0, has_tr = has_t && bx + 1 < bw, has_tl = has_t && bx > 0, has_tp = p.prev_mv != nullptr;
+    const unsigned long long me = p.smv[idx];
+    const uint32_t wS = (uint32_t)me;
+    const int best_sad = (int)(me >> 32);
+    const uint32_t wL = has_l ? (uint32_t)p.smv[idx - 1] : 0u, wT = has_t ? (uint32_t)p.smv[idx - bw] : 0u;
+    const uint32_t wTR = has_tr ? (uint32_t)p.smv[idx - bw + 1] : 0u, wTL = has_tl ? (uint32_t)p.smv[idx - bw - 1] : 0u;
+    const uint32_t wP = has_tp ? *(const uint32_t*)(p.prev_mv + (size_t)p.prev_stride * idx) : 0u;
+    // deduplicated (dx,dy,sx,sy) of left, top, top-right, top-left, temporal; the first two are
+    // Nearest / Near, padded with zero. Every MV word is canonical (sx, sy in {0,1}), so equal
+    // half-pel positions are equal words.
+    uint32_t c0 = 0, c1 = 0; int ncand = 0;
+#define RIV_ADDC(present, q)                                                         \
+    {                                                                        \
+        const uint32_t q_ = (q);                                             \
+        const bool add_ = (present) && !(ncand >= 1 && c0 == q_) && !(ncand >= 2 && c1 == q_); \
+        c1 = (add_ && ncand == 1) ? q_ : c1;                                 \
+        c0 = (add_ && ncand == 0) ? q_ : c0;                                 \
+        ncand += add_ ? 1 : 0;                                               \
+    }
+    RIV_ADDC(has_l, wL) RIV_ADDC(has_t, wT) RIV_ADDC(has_tr, wTR) RIV_ADDC(has_tl, wTL) RIV_ADDC(has_tp, wP)
+#undef RIV_ADDC
+    const uint32_t nearest = ncand > 0 ? c0 : 0u, nearp = ncand > 1 ? c1 : 0u, temporal = has_tp ? wP : 0u;
+    const int S0 = mvx(wS), S1 = mvy(wS), S2 = (wS >> 16) & 0xff, S3 = wS >> 24;
+    uint8_t mode;
+    if (wS == 0u) mode = riv::M_ZERO;
+    else if (wS == nearest) mode = riv::M_NEAREST;
+    else if (wS == nearp) mode = riv::M_NEAR;
+    else if (has_tr && wS == wTR) mode = riv::M_TOPRIGHT;
+    else if (has_tl && wS == wTL) mode = riv::M_TOPLEFT;
+    else if (wS == temporal) mode = riv::M_TEMPORAL;
+    else mode = riv::M_NEW;
+    uint8_t chosen = 0; int rdx = 0, rdy = 0;
+    if (mode == riv::M_NEW) {   // NEW base by estimated bits (encoder.rs:567-661, :128-159): the first cheapest
+        const uint32_t bases[5] = {nearest, nearp, wTR, wTL, temporal};
+        const bool have[5] = {true, true, has_tr, has_tl, true};
+        unsigned best = 0xffffffffu;
+        int bbx = mvx(nearest), bby = mvy(nearest);
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            if (have[k]) {
+                const int kx = mvx(bases[k]), ky = mvy(bases[k]);
+                const int cxs = mv_class(abs(dclamp(S0 - kx, -128, 127))), cys = mv_class(abs(dclamp(S1 - ky, -128, 127)));
+                const unsigned cost = (k < 4 ? k + 1 : 4) + (cxs == 0 ? 4u : 4u + cxs) + (cys == 0 ? 4u : 4u + cys);
+                if (cost < best) { best = cost; chosen = (uint8_t)k; bbx = kx; bby = ky; }
+            }
+        }
+        rdx = dclamp(S0 - bbx, -128, 127);
+        rdy = dclamp(S1 - bby, -128, 127);
+        if (dclamp(bbx + rdx, -128, 127) != S0 || dclamp(bby + rdy, -128, 127) != S1) atomicExch(p.err, 2);
+    }
+    // RdoContext::decide via the host-built table (rdo.rs:140-178)
+    uint32_t rdo = 0;
+    if (S2 == 0 && S3 == 0 && best_sad <= p.rdo_max) rdo = (p.rdo_bits[best_sad >> 5] >> (best_sad & 31)) & 1u;
+    const uint32_t w2 = (uint32_t)mode | ((uint32_t)chosen << 8) | ((uint32_t)(uint8_t)rdx << 16) | ((uint32_t)(uint8_t)rdy << 24);
+    *(uint4*)&p.rec[idx] = make_uint4(wS, wS, w2, (uint32_t)best_sad | (rdo << 16));
 }
 
 // ============================================================================================
@@ -1100,25 +1124,29 @@ void yuv420_to_rgb(hipStream_t st, const uint8_t* Y, const uint8_t* U, const uin
 
 int sad_table_stride(int sr) { return ((2 * sr + 1) * (2 * sr + 1) + 7) & ~7; }
 int hp_table_stride(int sr) { return ((4 * sr + 3) * (4 * sr + 3) + 7) & ~7; }
-size_t walk_lds_bytes(int sr, int bw, int R, int hp_slots) {
-    return (size_t)R * (4 * sad_table_stride(sr) + 2 * hp_slots * hp_table_stride(sr)) + (size_t)R * bw * 16 + 256 * sizeof(double) + 256 * 4 + 64;
+int table_region_bytes(int sr) { return 2 * (sad_table_stride(sr) + hp_table_stride(sr)); }
+size_t walk_lds_bytes(int sr, int bw, int R, int slots) {
+    return (size_t)R * slots * table_region_bytes(sr) + (size_t)R * bw * 8 + 256 * sizeof(double) + 64;
 }
-void sad_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int H, int sr, uint16_t* out) {
+void sad_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int H, int sr, uint16_t* out, int stride) {
     const int ndwt = (16 + 2 * sr + 6) / 4 + 1;
     const size_t lds = (size_t)(64 + 4 * (16 + 2 * sr) * ndwt) * 4;
-    hipLaunchKernelGGL(k_sad_table, dim3((W / 16) * (H / 16)), dim3(256), lds, st, cur, ref, W, H, W / 16, sr, ndwt, out, sad_table_stride(sr));
+    hipLaunchKernelGGL(k_sad_table, dim3((W / 16) * (H / 16)), dim3(256), lds, st, cur, ref, W, H, W / 16, sr, ndwt, out, stride);
 }
 static constexpr int HPK = 8;   // y-offsets per half-pel-table thread
-void hp_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int H, int sr, uint16_t* out) {
+void hp_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int H, int sr, uint16_t* out, int stride) {
     const int ndp = (2 * sr + 4) / 4 + 6, WS = 16 + 2 * sr + 2;
     const size_t lds = (size_t)(64 + (WS + HPK + 1) * (ndp + 1) + 4 * (WS + HPK) * ndp) * 4;
-    hipLaunchKernelGGL(k_hp_table<HPK>, dim3((W / 16) * (H / 16)), dim3(128), lds, st, cur, ref, W, H, W / 16, sr, ndp, out, hp_table_stride(sr));
+    hipLaunchKernelGGL(k_hp_table<HPK>, dim3((W / 16) * (H / 16)), dim3(128), lds, st, cur, ref, W, H, W / 16, sr, ndp, out, stride);
 }
 hipError_t me_walk(hipStream_t st, const WalkParams& p) {
     int nbands = (p.bh + p.R - 1) / p.R;
-    size_t lds = walk_lds_bytes(p.sr, p.bw, p.R, p.hp_slots);
+    size_t lds = walk_lds_bytes(p.sr, p.bw, p.R, p.slots);
     hipLaunchKernelGGL(k_me_walk, dim3(nbands), dim3(64 * p.R), lds, st, p);
     return hipGetLastError();
+}
+void mode(hipStream_t st, const ModeParams& p) {
+    hipLaunchKernelGGL(k_mode, dim3((p.bw * p.bh + 255) / 256), dim3(256), 0, st, p);
 }
 void mc(hipStream_t st, const uint8_t* rY, const uint8_t* rU, const uint8_t* rV, int W, int H, const int8_t* mvs, int mv_stride,
         uint8_t* pY, uint8_t* pU, uint8_t* pV, uint32_t* var_sums) {

@@ -21,28 +21,37 @@ struct BlockRec {
 };
 static_assert(sizeof(BlockRec) == 16, "BlockRec layout");
 
-// Parameters of the motion-search + mode-decision wavefront for one frame.
+// Parameters of the motion-search wavefront for one frame.
 struct WalkParams {
     int W, H, bw, bh, sr;
     const int8_t* prev_mv;       // previous P-frame's reconstructed MVs (dx,dy,sx,sy) or nullptr
     int prev_stride;             // bytes between consecutive MVs of prev_mv
-    const uint16_t* sadt;        // per block, integer SAD of every (dx,dy) in [-sr,sr]^2, row-major by dy
-    int sadt_stride;             // entries between consecutive blocks' tables, a multiple of 8
-    const uint16_t* hpt;         // per block, half-pel SAD of every (hx,hy) in [-2sr-1,2sr+1]^2, row-major by hy
-    int hpt_stride;              // entries between consecutive blocks' tables, a multiple of 8
-    int hp_slots;                // LDS slots per wave for the half-pel table: 2 (prefetched a block ahead) or 1
-    BlockRec* rec;               // bw*bh records out
-    unsigned long long* pub;     // bw*bh published MV pairs (bits 0-17 search, 18-35 reconstructed, 36-63 tag)
+    const uint8_t* tabs;         // per block: integer SAD table (tsb bytes), then half-pel SAD table
+    int tab_bytes;               // bytes of one block's table region, a multiple of 16
+    int tsb;                     // bytes of the integer table, a multiple of 16
+    int slots;                   // LDS table slots per wave: 2 (next block prefetched) or 1
+    unsigned long long* smv;     // per block out: search MV word | best half-pel SAD << 32
+    unsigned long long* pub;     // bw*bh published search MVs (bits 0-17) with the frame tag (bits 36-63)
     unsigned tag;                // this frame's tag, 1 .. 2^28-1; differs from the previous launch's
     const double* bits;          // 256 entries: 1.0 for 0, else 2*log2(a)+2 (motion/scalar.rs:495-525)
     double lam2;                 // lambda * 2.0
     int c_zero, c_pred;          // (lam2*0.5) as i64, (lam2*1.5) as i64; c_pred is also the half-pel penalty
     int lam_zero;                // lambda == 0.0 -> every MV cost is 0
-    const uint32_t* rdo_bits;    // bit s set -> RDO chooses skip at SAD s (integer-aligned MV)
-    int rdo_max;                 // largest SAD that can skip, -1 when skipping is disabled
     long long zthr, pthr;        // me_zero_mv_threshold, me_predictor_threshold
-    int R;                       // MB rows per workgroup, at most 15
+    int R;                       // MB rows per workgroup, at most 4
     int* err;                    // set non-zero on a wavefront timeout
+};
+
+// Parameters of the per-MB mode decision that follows the search.
+struct ModeParams {
+    int bw, bh;
+    const unsigned long long* smv;   // search MV word | SAD << 32 per block
+    const int8_t* prev_mv;           // previous P-frame's reconstructed MVs or nullptr
+    int prev_stride;
+    const uint32_t* rdo_bits;        // bit s set -> RDO chooses skip at SAD s (integer-aligned MV)
+    int rdo_max;                     // largest SAD that can skip, -1 when skipping is disabled
+    BlockRec* rec;                   // bw*bh records out
+    int* err;                        // set to 2 if a reconstructed MV would differ from its search MV
 };
 
 }  // namespace riv

@@ -10,11 +10,13 @@ hipError_t upload_bases(const int32_t* b16, const int32_t* b8);
 void rgb_to_yuv420(hipStream_t st, const uint8_t* rgb, int dw, int dh, int sw, int sh, uint8_t* Y, uint8_t* U, uint8_t* V);
 void yuv420_to_rgb(hipStream_t st, const uint8_t* Y, const uint8_t* U, const uint8_t* V, int sw, int dw, int dh, uint8_t* rgb);
 int sad_table_stride(int sr);
-void sad_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int H, int sr, uint16_t* out);
 int hp_table_stride(int sr);
-void hp_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int H, int sr, uint16_t* out);
-size_t walk_lds_bytes(int sr, int bw, int R, int hp_slots);
+int table_region_bytes(int sr);
+void sad_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int H, int sr, uint16_t* out, int stride);
+void hp_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int H, int sr, uint16_t* out, int stride);
+size_t walk_lds_bytes(int sr, int bw, int R, int slots);
 hipError_t me_walk(hipStream_t st, const riv::WalkParams& p);
+void mode(hipStream_t st, const riv::ModeParams& p);
 void mc(hipStream_t st, const uint8_t* rY, const uint8_t* rU, const uint8_t* rV, int W, int H, const int8_t* mvs, int mv_stride,
         uint8_t* pY, uint8_t* pU, uint8_t* pV, uint32_t* var_sums);
 void aq(hipStream_t st, const uint32_t* var_sums, int nb, float base, float* lnv, float* steps);
