@@ -71,7 +71,7 @@ static int decoder_init(riv_decoder* d) {
     RIV_HIP(d->ref[0].alloc(fsz)); RIV_HIP(d->ref[1].alloc(fsz)); RIV_HIP(d->pred.alloc(fsz));
     RIV_HIP(d->coeffs.alloc(768 * (size_t)d->nb)); RIV_HIP(d->mvs.alloc(4 * (size_t)d->nb));
     RIV_HIP(d->skip.alloc(d->nb)); RIV_HIP(d->mask.alloc(d->nb));
-    RIV_HIP(d->var.alloc(8 * (size_t)d->nb)); RIV_HIP(d->lnv.alloc(4 * (size_t)d->nb)); RIV_HIP(d->steps.alloc(4 * (size_t)d->nb));
+    RIV_HIP(d->var.alloc(8 * (size_t)d->nb)); RIV_HIP(d->lnv.alloc(4 * ((size_t)d->nb + 1))); RIV_HIP(d->steps.alloc(4 * (size_t)d->nb));
     size_t rgbsz = (size_t)d->hdr.display_width * d->hdr.display_height * 3;
     RIV_HIP(d->rgb.alloc(rgbsz)); RIV_HIP(d->h_rgb.alloc(rgbsz));
     RIV_HIP(d->h_coeffs.alloc(768 * (size_t)d->nb));
@@ -185,7 +185,7 @@ static int decode_next(riv_decoder* d, uint8_t* rgb_out, riv_decoded_info* info)
                 mask[i] = !b.skip || m.v[0] || m.v[1] || m.v[2] || m.v[3];   // decoder.rs:412-423
             }
             RIV_HIP(hipMemcpyAsync(d->mvs.p, mv.data(), 4 * (size_t)nb, hipMemcpyHostToDevice, st));
-            rivk::mc(st, rY, rU, rV, W, H, d->mvs.as<int8_t>(), 4, pY, pU, pV, d->var.as<uint32_t>());
+            rivk::mc(st, rY, rU, rV, W, H, d->mvs.as<int8_t>(), 4, pY, pU, pV, d->var.as<uint32_t>(), d->lnv.as<float>());
             if (d->skip_residuals) {
                 RIV_HIP(hipMemcpyAsync(nY, pY, ysz + 2 * csz, hipMemcpyDeviceToDevice, st));
             } else {
@@ -194,7 +194,7 @@ static int decode_next(riv_decoder* d, uint8_t* rgb_out, riv_decoded_info* info)
                 RIV_HIP(hipMemcpyAsync(d->coeffs.p, hco, 768 * (size_t)nb, hipMemcpyHostToDevice, st));
                 RIV_HIP(hipMemcpyAsync(d->skip.p, skip.data(), nb, hipMemcpyHostToDevice, st));
                 RIV_HIP(hipMemcpyAsync(d->mask.p, mask.data(), nb, hipMemcpyHostToDevice, st));
-                rivk::aq(st, d->var.as<uint32_t>(), nb, quant_step(q), d->lnv.as<float>(), d->steps.as<float>());
+                rivk::aq(st, nb, quant_step(q), d->lnv.as<float>(), d->steps.as<float>());
                 rivk::inter_recon(st, pY, pU, pV, W, H, d->skip.as<uint8_t>(), d->steps.as<float>(), d->coeffs.as<int16_t>(), nY, nU, nV);
                 rivk::deblock(st, nY, nU, nV, W, H, deblock_level(quant_step(q)), false, d->mask.as<uint8_t>());
             }

@@ -26,6 +26,7 @@ namespace {
 struct Slot {
     DevBuf rec, coeffs, optskip;         // device outputs of the frame (rec also seeds the next frame's temporal MVs)
     HostBuf h_rec, h_coeffs, h_optskip;  // pinned copies for the host entropy stage
+    hipEvent_t kdone = nullptr;          // the frame's kernels finished (the copy stream waits on it)
     hipEvent_t done = nullptr;           // D2H copies of this slot finished
     hipEvent_t marks[RIV_NUM_STAGES + 1] = {};
     int mark_stage[RIV_NUM_STAGES + 1] = {};
@@ -84,7 +85,8 @@ struct riv_encoder {
     uint64_t frame_count = 0;      // frames submitted
 
     int W = 0, H = 0, bw = 0, bh = 0, nb = 0, R = 4, tab_slots = 2;
-    hipStream_t st = nullptr;
+    hipStream_t st = nullptr;    // kernels, frame after frame
+    hipStream_t cst = nullptr;   // device-to-host copies of finished frames, overlapping the next frame's kernels
     DevBuf rgb, cur, pred, ref[2], var, lnv, steps, mask, pub, tabs, smv, bits, rdo, err, yt, ct;
     int ref_idx = 0;
     bool have_ref = false;
@@ -185,10 +187,11 @@ static int encoder_init(riv_encoder* e) {
     }
     const size_t ysz = (size_t)e->W * e->H, csz = ysz / 4, fsz = ysz + 2 * csz;
     RIV_HIP(hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking));
+    RIV_HIP(hipStreamCreateWithFlags(&e->cst, hipStreamNonBlocking));
     RIV_HIP(e->rgb.alloc((size_t)c.display_width * c.display_height * 3));
     RIV_HIP(e->cur.alloc(fsz)); RIV_HIP(e->pred.alloc(fsz));
     RIV_HIP(e->ref[0].alloc(fsz)); RIV_HIP(e->ref[1].alloc(fsz));
-    RIV_HIP(e->var.alloc(8 * (size_t)e->nb)); RIV_HIP(e->lnv.alloc(4 * (size_t)e->nb)); RIV_HIP(e->steps.alloc(4 * (size_t)e->nb));
+    RIV_HIP(e->var.alloc(8 * (size_t)e->nb)); RIV_HIP(e->lnv.alloc(4 * ((size_t)e->nb + 1))); RIV_HIP(e->steps.alloc(4 * (size_t)e->nb));
     RIV_HIP(e->mask.alloc(e->nb));
     RIV_HIP(e->pub.alloc(8 * (size_t)e->nb)); RIV_HIP(hipMemset(e->pub.p, 0, 8 * (size_t)e->nb));
     RIV_HIP(e->tabs.alloc((size_t)e->nb * rivk::table_region_bytes(c.search_range)));
@@ -200,6 +203,7 @@ static int encoder_init(riv_encoder* e) {
         RIV_HIP(s.rec.alloc(sizeof(BlockRec) * e->nb)); RIV_HIP(s.coeffs.alloc(768 * (size_t)e->nb)); RIV_HIP(s.optskip.alloc(e->nb));
         RIV_HIP(s.h_rec.alloc(sizeof(BlockRec) * e->nb)); RIV_HIP(s.h_coeffs.alloc(768 * (size_t)e->nb)); RIV_HIP(s.h_optskip.alloc(e->nb));
         RIV_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+        RIV_HIP(hipEventCreateWithFlags(&s.kdone, hipEventDisableTiming));
         RIV_HIP(hipEventCreate(&s.t0)); RIV_HIP(hipEventCreate(&s.t1));
     }
     if (e->depth > 0) {
@@ -266,13 +270,16 @@ static void destroy_encoder(riv_encoder* e) {
     }
     e->pool.reset();
     if (e->st) (void)hipStreamSynchronize(e->st);
+    if (e->cst) (void)hipStreamSynchronize(e->cst);
     for (Slot& s : e->slots) {
         if (s.done) (void)hipEventDestroy(s.done);
+        if (s.kdone) (void)hipEventDestroy(s.kdone);
         if (s.t0) (void)hipEventDestroy(s.t0);
         if (s.t1) (void)hipEventDestroy(s.t1);
         for (auto& m : s.marks) if (m) (void)hipEventDestroy(m);
     }
     if (e->st) (void)hipStreamDestroy(e->st);
+    if (e->cst) (void)hipStreamDestroy(e->cst);
     delete e;
 }
 
@@ -451,11 +458,12 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
     s.index = e->frame_count; s.ts = ts; s.intra = intra; s.busy = true; s.n_marks = 0;
     if (e->profiling && !s.marks[0])
         for (auto& m : s.marks) RIV_HIP(hipEventCreate(&m));
-    auto mark = [&](int stage) {
+    auto mark_on = [&](int stage, hipStream_t q) {
         if (!e->profiling || s.n_marks > RIV_NUM_STAGES) return;
-        (void)hipEventRecord(s.marks[s.n_marks], e->st);
+        (void)hipEventRecord(s.marks[s.n_marks], q);
         s.mark_stage[s.n_marks++] = stage;
     };
+    auto mark = [&](int stage) { mark_on(stage, e->st); };
 
     const int W = e->W, H = e->H, nb = e->nb;
     const size_t ysz = (size_t)W * H, csz = ysz / 4;
@@ -480,8 +488,6 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
         mark(6);
         rivk::deblock(st, nY, nU, nV, W, H, deblock_level(e->intra_qs), true, nullptr);
         mark(5);
-        RIV_HIP(hipMemcpyAsync(s.h_coeffs.p, s.coeffs.p, 768 * (size_t)nb, hipMemcpyDeviceToHost, st));
-        mark(7);
     } else {
         const int sr = c.search_range, RB = rivk::table_region_bytes(sr), TSB = 2 * rivk::sad_table_stride(sr);
         uint8_t* tabs = e->tabs.as<uint8_t>();
@@ -505,9 +511,9 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
         mp.rdo_bits = e->rdo.as<uint32_t>(); mp.rdo_max = e->rdo_max; mp.rec = rec; mp.err = e->err.as<int>();
         rivk::mode(st, mp);
         mark(1);
-        rivk::mc(st, rY, rU, rV, W, H, (const int8_t*)rec, sizeof(BlockRec), pY, pU, pV, e->var.as<uint32_t>());
+        rivk::mc(st, rY, rU, rV, W, H, (const int8_t*)rec, sizeof(BlockRec), pY, pU, pV, e->var.as<uint32_t>(), e->lnv.as<float>());
         mark(2);
-        rivk::aq(st, e->var.as<uint32_t>(), nb, e->inter_qs, e->lnv.as<float>(), e->steps.as<float>());
+        rivk::aq(st, nb, e->inter_qs, e->lnv.as<float>(), e->steps.as<float>());
         mark(3);
         rivk::inter_residual(st, cY, cU, cV, pY, pU, pV, W, H, rec, e->steps.as<float>(), c.inter_dead_zone, nY, nU, nV,
                              s.coeffs.as<int16_t>(), s.optskip.as<uint8_t>());
@@ -515,14 +521,19 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
         rivk::filter_mask(st, rec, s.optskip.as<uint8_t>(), nb, e->mask.as<uint8_t>());
         rivk::deblock(st, nY, nU, nV, W, H, deblock_level(e->inter_qs), false, e->mask.as<uint8_t>());
         mark(5);
-        RIV_HIP(hipMemcpyAsync(s.h_rec.p, rec, sizeof(BlockRec) * nb, hipMemcpyDeviceToHost, st));
-        RIV_HIP(hipMemcpyAsync(s.h_optskip.p, s.optskip.p, nb, hipMemcpyDeviceToHost, st));
-        RIV_HIP(hipMemcpyAsync(s.h_coeffs.p, s.coeffs.p, 768 * (size_t)nb, hipMemcpyDeviceToHost, st));
-        mark(7);
     }
     RIV_HIP(hipGetLastError());
-    RIV_HIP(hipEventRecord(s.t1, st));
-    RIV_HIP(hipEventRecord(s.done, st));
+    // the frame's results leave on the copy stream while the next frame's kernels run
+    RIV_HIP(hipEventRecord(s.kdone, st));
+    RIV_HIP(hipStreamWaitEvent(e->cst, s.kdone, 0));
+    if (!intra) {
+        RIV_HIP(hipMemcpyAsync(s.h_rec.p, rec, sizeof(BlockRec) * nb, hipMemcpyDeviceToHost, e->cst));
+        RIV_HIP(hipMemcpyAsync(s.h_optskip.p, s.optskip.p, nb, hipMemcpyDeviceToHost, e->cst));
+    }
+    RIV_HIP(hipMemcpyAsync(s.h_coeffs.p, s.coeffs.p, 768 * (size_t)nb, hipMemcpyDeviceToHost, e->cst));
+    mark_on(7, e->cst);
+    RIV_HIP(hipEventRecord(s.t1, e->cst));
+    RIV_HIP(hipEventRecord(s.done, e->cst));
     e->prev_inter_slot = intra ? -1 : si;
     e->ref_idx = 1 - e->ref_idx;
     e->have_ref = true;

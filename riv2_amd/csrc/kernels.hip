@@ -599,6 +599,14 @@ __global__ __launch_bounds__(256) void k_mode(ModeParams p) {
     *(uint4*)&p.rec[idx] = make_uint4(wS, wS, w2, (uint32_t)best_sad | (rdo << 16));
 }
 
+// Natural log of a 16x16 block's variance from its sum and sum of squares, as the adaptive
+// quantiser defines it (reitero_residual/src/residual.rs:116-167): ((sum_sq*256 - sum^2) as f32)
+// / 65536, floored at 4, then glibc's logf.
+__device__ __forceinline__ float aq_log_var(uint32_t s, uint32_t s2) {
+    float var = (float)((uint64_t)s2 * 256ull - (uint64_t)s * s) / 65536.0f;
+    return glibc_logf(fmaxf(var, 4.0f));
+}
+
 // ============================================================================================
 // Motion compensation + prediction-variance sums
 // ============================================================================================
@@ -623,7 +631,7 @@ __device__ __forceinline__ uint8_t sample_hp_round(const uint8_t* p, int W, int 
 __global__ __launch_bounds__(256) void k_mc(const uint8_t* __restrict__ rY, const uint8_t* __restrict__ rU, const uint8_t* __restrict__ rV,
                                             int W, int H, const int8_t* __restrict__ mvs, int mv_stride,
                                             uint8_t* __restrict__ pY, uint8_t* __restrict__ pU, uint8_t* __restrict__ pV,
-                                            uint32_t* __restrict__ var_sums) {
+                                            uint32_t* __restrict__ var_sums, float* __restrict__ lnv) {
     __shared__ uint32_t red[8];
     const int b = blockIdx.x, bw = W >> 4, bx = b % bw, by = b / bw, t = threadIdx.x;
     const int cw = W >> 1, ch = H >> 1;
@@ -659,8 +667,10 @@ __global__ __launch_bounds__(256) void k_mc(const uint8_t* __restrict__ rY, cons
         if ((t & 63) == 0) { red[(t >> 6) * 2] = s1; red[(t >> 6) * 2 + 1] = s2; }
         __syncthreads();
         if (t == 0) {
-            var_sums[2 * b] = red[0] + red[2] + red[4] + red[6];
-            var_sums[2 * b + 1] = red[1] + red[3] + red[5] + red[7];
+            const uint32_t s = red[0] + red[2] + red[4] + red[6], s2 = red[1] + red[3] + red[5] + red[7];
+            var_sums[2 * b] = s;
+            var_sums[2 * b + 1] = s2;
+            if (lnv) lnv[b] = aq_log_var(s, s2);
         }
     }
 }
@@ -668,21 +678,16 @@ __global__ __launch_bounds__(256) void k_mc(const uint8_t* __restrict__ rY, cons
 // ============================================================================================
 // Adaptive quantisation (compute_aq_quant_steps, reitero_residual/src/residual.rs:116-167)
 // ============================================================================================
-// variance = ((sum_sq*256 - sum^2) as f32) / 65536, floored at 4; ln with glibc's logf;
+// variance = ((sum_sq*256 - sum^2) as f32) / 65536, floored at 4, and its natural log with
+// glibc's logf (computed per MB where the sums are formed, in k_mc); the mean of the logs is a
 // sequential f32 sum in raster order (one lane: the rounding sequence is part of the result);
-// step = base * exp2f((0.8 * (ln - avg)) / 6). Single 1024-thread workgroup; `lnv` is NB floats
-// of scratch.
-__global__ __launch_bounds__(1024) void k_aq(const uint32_t* __restrict__ var_sums, int nb, float base, float* __restrict__ lnv,
-                                             float* __restrict__ steps) {
-    __shared__ float s_avg;
-    __shared__ float4 s_ln[2048];   // 8192 logs per LDS pass
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-        uint64_t s = var_sums[2 * b], s2 = var_sums[2 * b + 1];
-        float var = (float)(s2 * 256ull - s * s) / 65536.0f;
-        var = fmaxf(var, 4.0f);
-        lnv[b] = glibc_logf(var);
-    }
-    float sum = 0.0f;   // lives in thread 0
+// step = base * exp2f((0.8 * (ln - avg)) / 6).
+
+// One workgroup: the logs pass through LDS in chunks; lane 0 adds them in order, reading 64
+// values ahead so that LDS latency hides behind the dependent adds. The mean goes to lnv[nb].
+__global__ __launch_bounds__(256) void k_aq_mean(float* __restrict__ lnv, int nb) {
+    __shared__ float4 s_ln[2048];   // 8192 logs per pass
+    float sum = 0.0f;               // lives in thread 0
     for (int c0 = 0; c0 < nb; c0 += 8192) {
         __syncthreads();
         const int cn = min(8192, nb - c0);
@@ -691,20 +696,27 @@ __global__ __launch_bounds__(1024) void k_aq(const uint32_t* __restrict__ var_su
         __syncthreads();
         if (threadIdx.x == 0) {
             int i = 0;
-            for (; i + 4 <= cn; i += 4) {
-                float4 q = s_ln[i >> 2];
-                sum = __fadd_rn(sum, q.x); sum = __fadd_rn(sum, q.y); sum = __fadd_rn(sum, q.z); sum = __fadd_rn(sum, q.w);
+            for (; i + 64 <= cn; i += 64) {
+                float4 q[16];
+#pragma unroll
+                for (int k = 0; k < 16; k++) q[k] = s_ln[(i >> 2) + k];
+#pragma unroll
+                for (int k = 0; k < 16; k++) {
+                    sum = __fadd_rn(sum, q[k].x); sum = __fadd_rn(sum, q[k].y);
+                    sum = __fadd_rn(sum, q[k].z); sum = __fadd_rn(sum, q[k].w);
+                }
             }
             for (; i < cn; i++) sum = __fadd_rn(sum, sl[i]);
         }
     }
-    if (threadIdx.x == 0) s_avg = __fdiv_rn(sum, (float)nb);
-    __syncthreads();
-    const float avg = s_avg;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-        float off = __fmul_rn(0.8f, __fsub_rn(lnv[b], avg));
-        steps[b] = __fmul_rn(base, glibc_exp2f(__fdiv_rn(off, 6.0f)));
-    }
+    if (threadIdx.x == 0) lnv[nb] = __fdiv_rn(sum, (float)nb);
+}
+
+__global__ __launch_bounds__(256) void k_aq_steps(const float* __restrict__ lnv, int nb, float base, float* __restrict__ steps) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const float off = __fmul_rn(0.8f, __fsub_rn(lnv[b], lnv[nb]));
+    steps[b] = __fmul_rn(base, glibc_exp2f(__fdiv_rn(off, 6.0f)));
 }
 
 // elementwise logf/exp2f for parity tests of the transcendental restatements
@@ -1149,11 +1161,12 @@ void mode(hipStream_t st, const ModeParams& p) {
     hipLaunchKernelGGL(k_mode, dim3((p.bw * p.bh + 255) / 256), dim3(256), 0, st, p);
 }
 void mc(hipStream_t st, const uint8_t* rY, const uint8_t* rU, const uint8_t* rV, int W, int H, const int8_t* mvs, int mv_stride,
-        uint8_t* pY, uint8_t* pU, uint8_t* pV, uint32_t* var_sums) {
-    hipLaunchKernelGGL(k_mc, dim3((W / 16) * (H / 16)), dim3(256), 0, st, rY, rU, rV, W, H, mvs, mv_stride, pY, pU, pV, var_sums);
+        uint8_t* pY, uint8_t* pU, uint8_t* pV, uint32_t* var_sums, float* lnv) {
+    hipLaunchKernelGGL(k_mc, dim3((W / 16) * (H / 16)), dim3(256), 0, st, rY, rU, rV, W, H, mvs, mv_stride, pY, pU, pV, var_sums, lnv);
 }
-void aq(hipStream_t st, const uint32_t* var_sums, int nb, float base, float* lnv, float* steps) {
-    hipLaunchKernelGGL(k_aq, dim3(1), dim3(1024), 0, st, var_sums, nb, base, lnv, steps);
+void aq(hipStream_t st, int nb, float base, float* lnv, float* steps) {
+    hipLaunchKernelGGL(k_aq_mean, dim3(1), dim3(256), 0, st, lnv, nb);
+    hipLaunchKernelGGL(k_aq_steps, dim3((nb + 255) / 256), dim3(256), 0, st, lnv, nb, base, steps);
 }
 void logf_exp2f(hipStream_t st, const float* in, size_t n, int which, float* out) {
     hipLaunchKernelGGL(k_logf_exp2f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, n, which, out);

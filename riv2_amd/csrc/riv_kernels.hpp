@@ -18,8 +18,9 @@ size_t walk_lds_bytes(int sr, int bw, int R, int slots);
 hipError_t me_walk(hipStream_t st, const riv::WalkParams& p);
 void mode(hipStream_t st, const riv::ModeParams& p);
 void mc(hipStream_t st, const uint8_t* rY, const uint8_t* rU, const uint8_t* rV, int W, int H, const int8_t* mvs, int mv_stride,
-        uint8_t* pY, uint8_t* pU, uint8_t* pV, uint32_t* var_sums);
-void aq(hipStream_t st, const uint32_t* var_sums, int nb, float base, float* lnv, float* steps);
+        uint8_t* pY, uint8_t* pU, uint8_t* pV, uint32_t* var_sums, float* lnv);
+// lnv: nb logs written by mc, plus one float for their mean
+void aq(hipStream_t st, int nb, float base, float* lnv, float* steps);
 void logf_exp2f(hipStream_t st, const float* in, size_t n, int which, float* out);
 void inter_residual(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, const uint8_t* pY, const uint8_t* pU,
                     const uint8_t* pV, int W, int H, const riv::BlockRec* rec, const float* steps, float dz, uint8_t* oY, uint8_t* oU,

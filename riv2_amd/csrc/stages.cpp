@@ -20,7 +20,7 @@ int riv_gpu_build_predicted(int32_t dev, const uint8_t* y, const uint8_t* u, con
     uint8_t* R = ref.as<uint8_t>(); uint8_t* P = pred.as<uint8_t>();
     RIV_HIP(hipMemcpy(R, y, ysz, hipMemcpyHostToDevice)); RIV_HIP(hipMemcpy(R + ysz, u, csz, hipMemcpyHostToDevice));
     RIV_HIP(hipMemcpy(R + ysz + csz, v, csz, hipMemcpyHostToDevice)); RIV_HIP(hipMemcpy(mvs.p, mv4, 4 * nb, hipMemcpyHostToDevice));
-    rivk::mc(0, R, R + ysz, R + ysz + csz, (int)w, (int)h, mvs.as<int8_t>(), 4, P, P + ysz, P + ysz + csz, nullptr);
+    rivk::mc(0, R, R + ysz, R + ysz + csz, (int)w, (int)h, mvs.as<int8_t>(), 4, P, P + ysz, P + ysz + csz, nullptr, nullptr);
     RIV_HIP(hipGetLastError());
     RIV_HIP(hipMemcpy(oy, P, ysz, hipMemcpyDeviceToHost)); RIV_HIP(hipMemcpy(ou, P + ysz, csz, hipMemcpyDeviceToHost));
     RIV_HIP(hipMemcpy(ov, P + ysz + csz, csz, hipMemcpyDeviceToHost));
@@ -74,12 +74,12 @@ int riv_gpu_aq_steps(int32_t dev, const uint8_t* y, uint32_t w, uint32_t h, floa
     // the variance sums come from the MC kernel; feed it the plane as its own zero-MV prediction
     DevBuf ref, pred, mvs, var, lnv, steps;
     RIV_HIP(ref.alloc(ysz * 3 / 2)); RIV_HIP(pred.alloc(ysz * 3 / 2)); RIV_HIP(mvs.alloc(4 * nb));
-    RIV_HIP(var.alloc(8 * nb)); RIV_HIP(lnv.alloc(4 * nb)); RIV_HIP(steps.alloc(4 * nb));
+    RIV_HIP(var.alloc(8 * nb)); RIV_HIP(lnv.alloc(4 * (nb + 1))); RIV_HIP(steps.alloc(4 * nb));
     RIV_HIP(hipMemset(ref.p, 0, ysz * 3 / 2)); RIV_HIP(hipMemset(mvs.p, 0, 4 * nb));
     RIV_HIP(hipMemcpy(ref.p, y, ysz, hipMemcpyHostToDevice));
     uint8_t* R = ref.as<uint8_t>(); uint8_t* P = pred.as<uint8_t>();
-    rivk::mc(0, R, R + ysz, R + ysz + ysz / 4, (int)w, (int)h, mvs.as<int8_t>(), 4, P, P + ysz, P + ysz + ysz / 4, var.as<uint32_t>());
-    rivk::aq(0, var.as<uint32_t>(), (int)nb, base, lnv.as<float>(), steps.as<float>());
+    rivk::mc(0, R, R + ysz, R + ysz + ysz / 4, (int)w, (int)h, mvs.as<int8_t>(), 4, P, P + ysz, P + ysz + ysz / 4, var.as<uint32_t>(), lnv.as<float>());
+    rivk::aq(0, (int)nb, base, lnv.as<float>(), steps.as<float>());
     RIV_HIP(hipGetLastError());
     RIV_HIP(hipMemcpy(out, steps.p, 4 * nb, hipMemcpyDeviceToHost));
     return RIV_OK;
