@@ -64,6 +64,13 @@ __global__ void k_yuv420_to_rgb(const uint8_t* __restrict__ Y, const uint8_t* __
 // ============================================================================================
 // Integer SAD table
 // ============================================================================================
+// An MB whose whole search area [x0-sr, x0+15+sr] x [y0-sr, y0+15+sr] lies inside the frame
+// clamps no 4x4 origin and no pixel, so its integer SAD at (dx, dy) equals the half-pel SAD at
+// (2dx, 2dy); only the other ("edge") MBs get an integer table of their own.
+__host__ __device__ inline bool mb_interior(int x0, int y0, int W, int H, int sr) {
+    return x0 >= sr && y0 >= sr && x0 + 16 + sr <= W && y0 + 16 + sr <= H;
+}
+
 // Every integer candidate a block's search can visit lies in [-sr, sr]^2, and its SAD does not
 // depend on the search path, so all (2sr+1)^2 of them are computed up front, fully parallel:
 // one workgroup per MB, one position per thread. SAD semantics of satd_scalar.rs:4-99: each 4x4
@@ -75,6 +82,7 @@ __global__ __launch_bounds__(256) void k_sad_table(const uint8_t* __restrict__ c
                                                    int sr, int ndwt, uint16_t* __restrict__ out, int stride) {
     extern __shared__ __align__(16) uint32_t sm[];
     const int b = blockIdx.x, bx = b % bw, by = b / bw, x0 = bx * 16, y0 = by * 16, t = threadIdx.x;
+    if (mb_interior(x0, y0, W, H, sr)) return;   // its SADs are the half-pel table's even entries
     const int D = 2 * sr + 1, P = D * D, MAXR = 16 + 2 * sr, CS = MAXR * ndwt;
     const int wx = max(0, x0 - sr) & ~3, wy = max(0, y0 - sr);
     const int NR = min(H, y0 + 16 + sr) - wy;
@@ -179,18 +187,22 @@ __global__ __launch_bounds__(128) void k_hp_table(const uint8_t* __restrict__ cu
 #pragma unroll
         for (int tt = 0; tt < K + 15; tt++) {
             const uint32_t* rw = pl + (r0 + tt) * ndp + m;
-            uint32_t d[6], al[5];
+            uint32_t d[6];
 #pragma unroll
             for (int i = 0; i < 6; i++) d[i] = rw[i];
+            // group g's 8 reference bytes, built in place as a 64-bit operand
+            unsigned long long pr[4];
 #pragma unroll
-            for (int i = 0; i < 5; i++) al[i] = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
+            for (int g = 0; g < 4; g++)
+                pr[g] = (unsigned long long)__builtin_amdgcn_alignbyte(d[g + 1], d[g], sh) |
+                        ((unsigned long long)__builtin_amdgcn_alignbyte(d[g + 2], d[g + 1], sh) << 32);
+            // consecutive instructions feed different accumulators, so they do not wait on each other
 #pragma unroll
-            for (int k = 0; k < K; k++) {
-                const int j = tt - k;
-                if (j >= 0 && j < 16) {
+            for (int g = 0; g < 4; g++) {
 #pragma unroll
-                    for (int g = 0; g < 4; g++)
-                        acc[k] = __builtin_amdgcn_qsad_pk_u16_u8((unsigned long long)al[g] | ((unsigned long long)al[g + 1] << 32), c[j * 4 + g], acc[k]);
+                for (int k = 0; k < K; k++) {
+                    const int j = tt - k;
+                    if (j >= 0 && j < 16) acc[k] = __builtin_amdgcn_qsad_pk_u16_u8(pr[g], c[j * 4 + g], acc[k]);
                 }
             }
         }
@@ -229,8 +241,7 @@ __global__ __launch_bounds__(128) void k_hp_table(const uint8_t* __restrict__ cu
 namespace {
 
 __device__ __forceinline__ int median3(int a, int b, int c) {   // motion/scalar.rs:24-43
-    if (a > b) { if (b > c) return b; else if (a > c) return c; else return a; }
-    else { if (a > c) return a; else if (b > c) return c; else return b; }
+    return max(min(a, b), min(max(a, b), c));
 }
 
 struct CostCtx {
@@ -242,13 +253,12 @@ struct CostCtx {
     // keeps lam2 * 42 below 2^26, so every term truncates exactly in 32 bits and SAD + cost
     // leaves room for a lane index in the low bits of a search key.
     __device__ __forceinline__ int cost(int dx, int dy) const {
-        if (lam_zero) return 0;
-        int best = 0x3ffffff;
-        if ((dx == nx && dy == ny) || (dx == rx && dy == ry) || (dx == mx && dy == my)) best = c_pred;
-        if (dx == 0 && dy == 0) best = min(best, c_zero);
+        const bool pm = ((dx == nx) & (dy == ny)) | ((dx == rx) & (dy == ry)) | ((dx == mx) & (dy == my));
+        int best = pm ? c_pred : 0x3ffffff;
+        best = ((dx == 0) & (dy == 0)) ? min(best, c_zero) : best;
         const int t0 = (int)__dmul_rn(lam2, __dadd_rn(__dadd_rn(6.0, bits[abs(dx - nx)]), bits[abs(dy - ny)]));
         const int t1 = (int)__dmul_rn(lam2, __dadd_rn(__dadd_rn(6.0, bits[abs(dx - rx)]), bits[abs(dy - ry)]));
-        return min(best, min(t0, t1));
+        return lam_zero ? 0 : min(best, min(t0, t1));
     }
 };
 
@@ -301,6 +311,13 @@ __device__ __forceinline__ void fetch_region(const uint8_t* src, uint8_t* dst, i
     if (o < nb && 16 * lane < nb - o) __builtin_amdgcn_global_load_lds((const void*)(s8 + o), (void*)(dst + o), 16, 0, 0);
 }
 
+// An MB's tables into an LDS slot: the half-pel table always, the integer table only at an edge.
+__device__ __forceinline__ void fetch_tables(const WalkParams& p, int bx, int by, uint8_t* slot, int lane) {
+    const uint8_t* src = p.tabs + (size_t)(by * p.bw + bx) * p.tab_bytes;
+    if (mb_interior(bx * 16, by * 16, p.W, p.H, p.sr)) fetch_region(src + p.tsb, slot + p.tsb, p.tab_bytes - p.tsb, lane);
+    else fetch_region(src, slot, p.tab_bytes, lane);
+}
+
 // A wave-uniform 64-bit value moved to scalar registers (readfirstlane returns a signed int:
 // each half is widened as unsigned).
 __device__ __forceinline__ unsigned long long uniform64(unsigned long long x) {
@@ -323,29 +340,66 @@ __device__ __forceinline__ bool wait_pub(const unsigned long long* a, unsigned t
 
 }  // namespace
 
-__global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
+// Table fetcher: the band's extra wave. For each row it copies the next MB's tables into the
+// row's LDS slot as soon as the row has finished the MB that last used that slot, waits for the
+// copies, then raises the row's ready count. The row waves thus issue no memory traffic of their
+// own for the tables and never wait on the vector-memory counter.
+__device__ void walk_fetcher(const WalkParams& p, int R, int NS, int RB, uint8_t* slots, const int* lprog, int* ready, const int* abortf, int lane) {
+    const int row0 = blockIdx.x * R, bw = p.bw;
+    int nf[4] = {0, 0, 0, 0};   // next MB to fetch, per row
+    unsigned spins = 0;
+    for (;;) {
+        unsigned issued = 0, pending = 0;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            if (r < R && row0 + r < p.bh && nf[r] < bw) {
+                pending |= 1u << r;
+                if (nf[r] - NS < __hip_atomic_load(&lprog[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                    fetch_tables(p, nf[r], row0 + r, slots + (r * NS + (NS == 2 ? (nf[r] & 1) : 0)) * RB, lane);
+                    issued |= 1u << r;
+                }
+            }
+        }
+        if (!pending) return;
+        if (issued) {
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                if ((issued >> r) & 1u) { nf[r]++; if (lane == 0) __hip_atomic_store(&ready[r], nf[r], __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP); }
+            spins = 0;
+        } else {
+            if (__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) || ++spins > (1u << 24)) return;
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+}
+
+__global__ __launch_bounds__(320) void k_me_walk(WalkParams p) {
     extern __shared__ __align__(16) uint8_t lds[];
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));   // wave index, kept scalar
     const int R = p.R, row = blockIdx.x * R + w, bw = p.bw, sr = p.sr;
     const int RB = p.tab_bytes, TSB = p.tsb, NS = p.slots, D2 = 4 * sr + 3, HR = 2 * sr + 1;
-    uint8_t* slot0 = lds + w * NS * RB;                        // NS table slots of this wave
+    uint8_t* slots = lds;                                      // per row: NS table slots
     uint32_t* rowbuf = (uint32_t*)(lds + R * NS * RB);         // per row and block: search MV word, SAD
-    double* bits = (double*)(rowbuf + R * bw * 2);
+    uint32_t* ptl = rowbuf + R * bw * 2;                       // per row and block: temporal MV word
+    double* bits = (double*)(ptl + R * bw);
     int* abortf = (int*)(bits + 256);
-    int* lprog = abortf + 1;   // per wave: number of finished blocks of its row
+    int* lprog = abortf + 1;   // per row: number of finished MBs
+    int* ready = lprog + 4;    // per row: number of MBs whose tables are in LDS
     for (int i = threadIdx.x; i < 256; i += blockDim.x) bits[i] = p.bits[i];
-    if (threadIdx.x == 0) *abortf = 0;
-    if (threadIdx.x < R) lprog[threadIdx.x] = 0;
-    const bool row_valid = row < p.bh;
     const bool has_tp = p.prev_mv != nullptr;
-    uint32_t pTn = 0;
-    if (row_valid) {
-        fetch_region(p.tabs + (size_t)row * bw * RB, slot0, RB, lane);
-        if (has_tp) pTn = *(const uint32_t*)(p.prev_mv + (size_t)p.prev_stride * row * bw);
+    for (int i = threadIdx.x; i < R * bw; i += blockDim.x) {
+        const int r = i / bw;
+        ptl[i] = (has_tp && blockIdx.x * R + r < p.bh) ? *(const uint32_t*)(p.prev_mv + (size_t)p.prev_stride * ((blockIdx.x * R + r) * bw + i % bw)) : 0u;
     }
+    if (threadIdx.x == 0) *abortf = 0;
+    if (threadIdx.x < 8) lprog[threadIdx.x] = 0;   // lprog and ready
     __syncthreads();
+    if (w == R) { walk_fetcher(p, R, NS, RB, slots, lprog, ready, abortf, lane); return; }
+    const bool row_valid = row < p.bh;
     if (!row_valid) return;
+    uint8_t* slot0 = slots + w * NS * RB;
 
     const bool has_t = row > 0, from_hbm = has_t && w == 0, publish = w == R - 1 && row + 1 < p.bh;
     const unsigned long long* above = p.pub + (size_t)(has_t ? row - 1 : 0) * bw;
@@ -387,64 +441,60 @@ __global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
             tS = sgpr(q[0]);
             trS = bx + 1 < bw ? sgpr(q[2]) : 0u;
         }
-        const uint32_t pT = sgpr(pTn);
-        // this block's tables have landed in LDS once every load issued so far is done
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        const uint8_t* cslot = slot0 + (NS == 2 ? (bx & 1) * RB : 0);
-        const uint16_t* sadt = (const uint16_t*)cslot;
-        const uint16_t* hpt = (const uint16_t*)(cslot + TSB);
-        // prefetch the next block's tables and temporal MV while this block is walked
-        const bool pf_next = bx + 1 < bw;
-        if (pf_next) {
-            if (NS == 2) fetch_region(p.tabs + (size_t)(idx + 1) * RB, slot0 + ((bx + 1) & 1) * RB, RB, lane);
-            if (has_tp) pTn = *(const uint32_t*)(p.prev_mv + (size_t)p.prev_stride * (idx + 1));
+        // this MB's tables are in LDS once the fetcher has counted it
+        {
+            unsigned spins = 0;
+            while (__hip_atomic_load(&ready[w], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) <= bx) {
+                if (__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) || ++spins > (1u << 24)) { dead = true; break; }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (dead) break;
         }
+        const uint32_t pT = sgpr(ptl[w * bw + bx]);
+        const uint8_t* cslot = slot0 + (NS == 2 ? (bx & 1) * RB : 0);
+        const uint16_t* hpt = (const uint16_t*)(cslot + TSB);
+        // integer SADs: the MB's own table at an edge, else the half-pel table's even entries;
+        // entry (dx, dy) is itab[dy * irs + dx * ics]
+        const bool edge = !mb_interior(bx * 16, row * 16, p.W, p.H, sr);
+        const uint16_t* itab = edge ? (const uint16_t*)cslot + sr * (2 * sr + 1) + sr : hpt + HR * D2 + HR;
+        const int irs = edge ? 2 * sr + 1 : 2 * D2, ics = edge ? 1 : 2;
 
         // ---- ME predictors: gather_local_motion_candidates (motion/scalar.rs:56-156) ----
-        // in predictor order: left, top, top-right, top-left, temporal; (dx, dy) packed in 16 bits
+        // predictor i in lane i+1, in predictor order: left, top, top-right, top-left, temporal;
+        // (dx, dy) packed in 16 bits. Lane 0 holds the (0,0) candidate.
         const uint32_t q0 = lS & 0xffffu, q1 = tS & 0xffffu, q2 = trS & 0xffffu, q3 = tlS & 0xffffu, q4 = pT & 0xffffu;
+        const uint32_t presm = (has_l ? 2u : 0u) | (has_t ? 4u : 0u) | (has_tr ? 8u : 0u) | (has_tl ? 16u : 0u) | (has_tp ? 32u : 0u);
+        const uint32_t cand[6] = {0u, q0, q1, q2, q3, q4};
+        const uint32_t myq = lane_pick(lane, cand);
+        // a predictor is a duplicate when an earlier present predictor has the same (dx, dy)
+        const bool dup = ((lane > 1) & ((presm >> 1) & 1u) & (myq == q0)) | ((lane > 2) & ((presm >> 2) & 1u) & (myq == q1)) |
+                         ((lane > 3) & ((presm >> 3) & 1u) & (myq == q2)) | ((lane > 4) & ((presm >> 4) & 1u) & (myq == q3));
+        const bool first = (lane < 6) & ((presm >> lane) & 1u) & !dup;
         CostCtx cc;
         {
             // nearest / near of the cost model: the first two distinct present predictors
-            uint32_t n0 = 0, n1 = 0;
-            int nu = 0;
-#define RIV_UNIQ(present, q)                                        \
-            if (present) {                                          \
-                if (nu == 0) { n0 = (q); nu = 1; }                  \
-                else if (nu == 1 && (q) != n0) { n1 = (q); nu = 2; } \
-            }
-            RIV_UNIQ(has_l, q0) RIV_UNIQ(has_t, q1) RIV_UNIQ(has_tr, q2) RIV_UNIQ(has_tl, q3) RIV_UNIQ(has_tp, q4)
-#undef RIV_UNIQ
+            const unsigned long long um = __ballot(first), um2 = um & (um - 1);
+            const uint32_t n0 = um ? sgpr((uint32_t)__builtin_amdgcn_readlane((int)myq, __builtin_ctzll(um))) : 0u;
+            const uint32_t n1 = um2 ? sgpr((uint32_t)__builtin_amdgcn_readlane((int)myq, __builtin_ctzll(um2))) : 0u;
             cc.nx = mvx(n0); cc.ny = mvy(n0); cc.rx = mvx(n1); cc.ry = mvy(n1);
             const uint32_t qm = has_tr ? q2 : (has_tl ? q3 : 0u);
             cc.mx = median3(has_l ? mvx(q0) : 0, has_t ? mvx(q1) : 0, mvx(qm));
             cc.my = median3(has_l ? mvy(q0) : 0, has_t ? mvy(q1) : 0, mvy(qm));
         }
         cc.bits = bits; cc.lam2 = p.lam2; cc.c_zero = p.c_zero; cc.c_pred = p.c_pred; cc.lam_zero = p.lam_zero;
-        // push_candidate (motion/scalar.rs:46-53) + evaluate_seed_candidates (:638-678): (0,0),
-        // then each in-range non-zero predictor not already listed. Lane 0 holds (0,0), lane i+1
-        // predictor i; the sequential "first strictly better" scan is an argmin with ties going
-        // to the lower lane, i.e. the earlier candidate.
-        unsigned incm = 1;
-        {
-            auto inr = [&](uint32_t q) { return q != 0u && abs(mvx(q)) <= sr && abs(mvy(q)) <= sr; };
-            const bool i0 = has_l && inr(q0);
-            const bool i1 = has_t && inr(q1) && !(i0 && q1 == q0);
-            const bool i2 = has_tr && inr(q2) && !(i0 && q2 == q0) && !(i1 && q2 == q1);
-            const bool i3 = has_tl && inr(q3) && !(i0 && q3 == q0) && !(i1 && q3 == q1) && !(i2 && q3 == q2);
-            const bool i4 = has_tp && inr(q4) && !(i0 && q4 == q0) && !(i1 && q4 == q1) && !(i2 && q4 == q2) && !(i3 && q4 == q3);
-            incm |= (i0 ? 2u : 0u) | (i1 ? 4u : 0u) | (i2 ? 8u : 0u) | (i3 ? 16u : 0u) | (i4 ? 32u : 0u);
-        }
-        const int D = 2 * sr + 1;
         int sdx = 0, sdy = 0;
         int scost;   // SAD + MV cost of the current integer best
         bool skip_int;
         {
-            const uint32_t cand[6] = {0u, q0, q1, q2, q3, q4};   // lane i+1 <- predictor i
-            const uint32_t myq = lane_pick(lane, cand);
-            const bool myv = lane < 6 && ((incm >> lane) & 1u);
-            const int dx = myv ? mvx(myq) : 0, dy = myv ? mvy(myq) : 0;
-            const int sv = sadt[(dy + sr) * D + dx + sr];
+            // push_candidate (motion/scalar.rs:46-53) + evaluate_seed_candidates (:638-678): (0,0),
+            // then each in-range non-zero predictor not already listed (a predictor equal to an
+            // earlier excluded one is excluded by the same test, so "first occurrence" suffices).
+            // The sequential "first strictly better" scan is an argmin with ties going to the
+            // lower lane, i.e. the earlier candidate.
+            const int qx = mvx(myq), qy = mvy(myq);
+            const bool myv = lane == 0 || (first & (myq != 0u) & (abs(qx) <= sr) & (abs(qy) <= sr));
+            const int dx = myv ? qx : 0, dy = myv ? qy : 0;
+            const int sv = itab[dy * irs + dx * ics];
             const int key = myv ? ((sv + cc.cost(dx, dy)) << 3) | lane : 0x7fffffff;
             skip_int = p.zthr > 0 && __builtin_amdgcn_readlane(sv, 0) <= p.zthr;   // process_block_hex (:546-636)
             if (skip_int) {
@@ -464,7 +514,7 @@ __global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
             for (int it = 0; it < 4096; it++) {
                 const int dx = sdx + hox, dy = sdy + hoy;
                 const bool v = lane < 6 && abs(dx) <= sr && abs(dy) <= sr;
-                const int sv = sadt[((v ? dy : sdy) + sr) * D + (v ? dx : sdx) + sr];
+                const int sv = itab[(v ? dy : sdy) * irs + (v ? dx : sdx) * ics];
                 const int key = v ? ((sv + cc.cost(dx, dy)) << 3) | lane : 0x7fffffff;
                 const int m = lane_min8(key);
                 if ((m >> 3) >= scost) break;
@@ -475,7 +525,7 @@ __global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
             const int doy = lane < 4 ? (int)((0x1322u >> (4 * lane)) & 15u) - 2 : 0;
             const int dx = sdx + dox, dy = sdy + doy;
             const bool v = lane < 4 && abs(dx) <= sr && abs(dy) <= sr;
-            const int sv = sadt[((v ? dy : sdy) + sr) * D + (v ? dx : sdx) + sr];
+            const int sv = itab[(v ? dy : sdy) * irs + (v ? dx : sdx) * ics];
             const int key = v ? ((sv + cc.cost(dx, dy)) << 3) | lane : 0x7fffffff;
             const int m = lane_min8(key);
             if ((m >> 3) < scost) { sdx = __builtin_amdgcn_readlane(dx, m & 7); sdy = __builtin_amdgcn_readlane(dy, m & 7); }
@@ -494,7 +544,6 @@ __global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
             best_sad = __builtin_amdgcn_readlane(sv, k);
             best_hx = __builtin_amdgcn_readlane(hx, k); best_hy = __builtin_amdgcn_readlane(hy, k);
         }
-        if (pf_next && NS == 1) fetch_region(p.tabs + (size_t)(idx + 1) * RB, slot0, RB, lane);
         // canonicalised search MV (motion/scalar.rs:619-634, motion_vector.rs:103-116)
         int S0, S1, S2, S3;
         {
@@ -512,7 +561,7 @@ __global__ __launch_bounds__(256) void k_me_walk(WalkParams p) {
             if (publish)
                 __hip_atomic_store(&p.pub[idx], (unsigned long long)mv18(wS) | ((unsigned long long)p.tag << 36), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             *(uint2*)(rowbuf + (w * bw + bx) * 2) = make_uint2(wS, (uint32_t)best_sad);
-            if (w + 1 < R) __hip_atomic_store(&lprog[w], bx + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_store(&lprog[w], bx + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
         lS = wS;
     }
@@ -1138,7 +1187,7 @@ int sad_table_stride(int sr) { return ((2 * sr + 1) * (2 * sr + 1) + 7) & ~7; }
 int hp_table_stride(int sr) { return ((4 * sr + 3) * (4 * sr + 3) + 7) & ~7; }
 int table_region_bytes(int sr) { return 2 * (sad_table_stride(sr) + hp_table_stride(sr)); }
 size_t walk_lds_bytes(int sr, int bw, int R, int slots) {
-    return (size_t)R * slots * table_region_bytes(sr) + (size_t)R * bw * 8 + 256 * sizeof(double) + 64;
+    return (size_t)R * slots * table_region_bytes(sr) + (size_t)R * bw * 12 + 256 * sizeof(double) + 64;
 }
 void sad_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int H, int sr, uint16_t* out, int stride) {
     const int ndwt = (16 + 2 * sr + 6) / 4 + 1;
@@ -1154,7 +1203,7 @@ void hp_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int
 hipError_t me_walk(hipStream_t st, const WalkParams& p) {
     int nbands = (p.bh + p.R - 1) / p.R;
     size_t lds = walk_lds_bytes(p.sr, p.bw, p.R, p.slots);
-    hipLaunchKernelGGL(k_me_walk, dim3(nbands), dim3(64 * p.R), lds, st, p);
+    hipLaunchKernelGGL(k_me_walk, dim3(nbands), dim3(64 * (p.R + 1)), lds, st, p);   // R row waves + the table fetcher
     return hipGetLastError();
 }
 void mode(hipStream_t st, const ModeParams& p) {
