@@ -24,10 +24,9 @@ using riv::BlockRec;
 namespace {
 
 struct Slot {
-    DevBuf rec, coeffs, optskip;         // device outputs of the frame (rec also seeds the next frame's temporal MVs)
-    HostBuf h_rec, h_coeffs, h_optskip;  // pinned copies for the host entropy stage
-    hipEvent_t kdone = nullptr;          // the frame's kernels finished (the copy stream waits on it)
-    hipEvent_t done = nullptr;           // D2H copies of this slot finished
+    DevBuf rec, optskip;                 // device outputs of the frame (rec also seeds the next frame's temporal MVs)
+    HostBuf h_rec, h_pack, h_optskip;    // mapped host memory the kernels write for the host entropy stage
+    hipEvent_t done = nullptr;           // the frame's kernels (and so its host-side outputs) finished
     hipEvent_t marks[RIV_NUM_STAGES + 1] = {};
     int mark_stage[RIV_NUM_STAGES + 1] = {};
     int n_marks = 0;
@@ -86,7 +85,6 @@ struct riv_encoder {
 
     int W = 0, H = 0, bw = 0, bh = 0, nb = 0, R = 4, tab_slots = 2;
     hipStream_t st = nullptr;    // kernels, frame after frame
-    hipStream_t cst = nullptr;   // device-to-host copies of finished frames, overlapping the next frame's kernels
     DevBuf rgb, cur, pred, ref[2], var, lnv, steps, mask, pub, tabs, smv, bits, rdo, err, yt, ct;
     int ref_idx = 0;
     bool have_ref = false;
@@ -187,7 +185,6 @@ static int encoder_init(riv_encoder* e) {
     }
     const size_t ysz = (size_t)e->W * e->H, csz = ysz / 4, fsz = ysz + 2 * csz;
     RIV_HIP(hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking));
-    RIV_HIP(hipStreamCreateWithFlags(&e->cst, hipStreamNonBlocking));
     RIV_HIP(e->rgb.alloc((size_t)c.display_width * c.display_height * 3));
     RIV_HIP(e->cur.alloc(fsz)); RIV_HIP(e->pred.alloc(fsz));
     RIV_HIP(e->ref[0].alloc(fsz)); RIV_HIP(e->ref[1].alloc(fsz));
@@ -200,10 +197,10 @@ static int encoder_init(riv_encoder* e) {
     e->depth = c.pipeline_depth;
     e->slots.resize(std::max(2, e->depth));
     for (Slot& s : e->slots) {
-        RIV_HIP(s.rec.alloc(sizeof(BlockRec) * e->nb)); RIV_HIP(s.coeffs.alloc(768 * (size_t)e->nb)); RIV_HIP(s.optskip.alloc(e->nb));
-        RIV_HIP(s.h_rec.alloc(sizeof(BlockRec) * e->nb)); RIV_HIP(s.h_coeffs.alloc(768 * (size_t)e->nb)); RIV_HIP(s.h_optskip.alloc(e->nb));
+        RIV_HIP(s.rec.alloc(sizeof(BlockRec) * e->nb)); RIV_HIP(s.optskip.alloc(e->nb));
+        RIV_HIP(s.h_rec.alloc(sizeof(BlockRec) * e->nb)); RIV_HIP(s.h_pack.alloc((size_t)riv::PACK_STRIDE * e->nb));
+        RIV_HIP(s.h_optskip.alloc(e->nb));
         RIV_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
-        RIV_HIP(hipEventCreateWithFlags(&s.kdone, hipEventDisableTiming));
         RIV_HIP(hipEventCreate(&s.t0)); RIV_HIP(hipEventCreate(&s.t1));
     }
     if (e->depth > 0) {
@@ -270,16 +267,13 @@ static void destroy_encoder(riv_encoder* e) {
     }
     e->pool.reset();
     if (e->st) (void)hipStreamSynchronize(e->st);
-    if (e->cst) (void)hipStreamSynchronize(e->cst);
     for (Slot& s : e->slots) {
         if (s.done) (void)hipEventDestroy(s.done);
-        if (s.kdone) (void)hipEventDestroy(s.kdone);
         if (s.t0) (void)hipEventDestroy(s.t0);
         if (s.t1) (void)hipEventDestroy(s.t1);
         for (auto& m : s.marks) if (m) (void)hipEventDestroy(m);
     }
     if (e->st) (void)hipStreamDestroy(e->st);
-    if (e->cst) (void)hipStreamDestroy(e->cst);
     delete e;
 }
 
@@ -349,14 +343,14 @@ static void entropy_job(riv_encoder* e, int si) {
         float ms = 0;
         if (s.mark_stage[k] >= 0 && hipEventElapsedTime(&ms, s.marks[k - 1], s.marks[k]) == hipSuccess) { smark[s.mark_stage[k]] += ms; nmark[s.mark_stage[k]]++; }
     }
-    const int16_t* hco = s.h_coeffs.as<int16_t>();
+    const uint8_t* pack = s.h_pack.as<uint8_t>();
     std::vector<uint8_t> res, bytes;
     std::vector<rivh::MvBlock> blocks;
     int bdx = 0, bdy = 0;
     try {
         if (s.intra) {
             std::vector<uint8_t> coded(nb, 1);
-            res = rivh::encode_residuals(hco, coded.data(), nb, true);
+            res = rivh::encode_residuals_packed(pack, riv::PACK_STRIDE, coded.data(), nb, true);
             st.frame_type = 1;
         } else {
             const BlockRec* hr = s.h_rec.as<BlockRec>();
@@ -381,7 +375,7 @@ static void entropy_job(riv_encoder* e, int si) {
                 st.rdo_blocks_selected_skip += hr[i].rdo_skip;
                 if (hr[i].s[2] || hr[i].s[3]) st.rdo_forced_fractional++;
             }
-            res = rivh::encode_residuals(hco, coded.data(), nb, false);
+            res = rivh::encode_residuals_packed(pack, riv::PACK_STRIDE, coded.data(), nb, false);
             st.frame_type = 2;
             st.mv_bias_dx = bdx; st.mv_bias_dy = bdy;
             st.rdo_lambda = e->lambda; st.rdo_blocks_evaluated = (uint32_t)nb;
@@ -458,12 +452,11 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
     s.index = e->frame_count; s.ts = ts; s.intra = intra; s.busy = true; s.n_marks = 0;
     if (e->profiling && !s.marks[0])
         for (auto& m : s.marks) RIV_HIP(hipEventCreate(&m));
-    auto mark_on = [&](int stage, hipStream_t q) {
+    auto mark = [&](int stage) {
         if (!e->profiling || s.n_marks > RIV_NUM_STAGES) return;
-        (void)hipEventRecord(s.marks[s.n_marks], q);
+        (void)hipEventRecord(s.marks[s.n_marks], e->st);
         s.mark_stage[s.n_marks++] = stage;
     };
-    auto mark = [&](int stage) { mark_on(stage, e->st); };
 
     const int W = e->W, H = e->H, nb = e->nb;
     const size_t ysz = (size_t)W * H, csz = ysz / 4;
@@ -484,7 +477,7 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
     mark(0);
     BlockRec* rec = s.rec.as<BlockRec>();
     if (intra) {
-        rivk::intra(st, cY, cU, cV, W, H, e->yt.as<uint16_t>(), e->ct.as<uint16_t>(), nY, nU, nV, s.coeffs.as<int16_t>());
+        rivk::intra(st, cY, cU, cV, W, H, e->yt.as<uint16_t>(), e->ct.as<uint16_t>(), nY, nU, nV, s.h_pack.dev<uint8_t>());
         mark(6);
         rivk::deblock(st, nY, nU, nV, W, H, deblock_level(e->intra_qs), true, nullptr);
         mark(5);
@@ -508,7 +501,8 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
         RIV_HIP(rivk::me_walk(st, p));
         riv::ModeParams mp{};
         mp.bw = e->bw; mp.bh = e->bh; mp.smv = p.smv; mp.prev_mv = prev_mv; mp.prev_stride = sizeof(BlockRec);
-        mp.rdo_bits = e->rdo.as<uint32_t>(); mp.rdo_max = e->rdo_max; mp.rec = rec; mp.err = e->err.as<int>();
+        mp.rdo_bits = e->rdo.as<uint32_t>(); mp.rdo_max = e->rdo_max; mp.rec = rec; mp.host_rec = s.h_rec.dev<BlockRec>();
+        mp.err = e->err.as<int>();
         rivk::mode(st, mp);
         mark(1);
         rivk::mc(st, rY, rU, rV, W, H, (const int8_t*)rec, sizeof(BlockRec), pY, pU, pV, e->var.as<uint32_t>(), e->lnv.as<float>());
@@ -516,24 +510,17 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
         rivk::aq(st, nb, e->inter_qs, e->lnv.as<float>(), e->steps.as<float>());
         mark(3);
         rivk::inter_residual(st, cY, cU, cV, pY, pU, pV, W, H, rec, e->steps.as<float>(), c.inter_dead_zone, nY, nU, nV,
-                             s.coeffs.as<int16_t>(), s.optskip.as<uint8_t>());
+                             s.h_pack.dev<uint8_t>(), s.optskip.as<uint8_t>(), s.h_optskip.dev<uint8_t>());
         mark(4);
         rivk::filter_mask(st, rec, s.optskip.as<uint8_t>(), nb, e->mask.as<uint8_t>());
         rivk::deblock(st, nY, nU, nV, W, H, deblock_level(e->inter_qs), false, e->mask.as<uint8_t>());
         mark(5);
     }
     RIV_HIP(hipGetLastError());
-    // the frame's results leave on the copy stream while the next frame's kernels run
-    RIV_HIP(hipEventRecord(s.kdone, st));
-    RIV_HIP(hipStreamWaitEvent(e->cst, s.kdone, 0));
-    if (!intra) {
-        RIV_HIP(hipMemcpyAsync(s.h_rec.p, rec, sizeof(BlockRec) * nb, hipMemcpyDeviceToHost, e->cst));
-        RIV_HIP(hipMemcpyAsync(s.h_optskip.p, s.optskip.p, nb, hipMemcpyDeviceToHost, e->cst));
-    }
-    RIV_HIP(hipMemcpyAsync(s.h_coeffs.p, s.coeffs.p, 768 * (size_t)nb, hipMemcpyDeviceToHost, e->cst));
-    mark_on(7, e->cst);
-    RIV_HIP(hipEventRecord(s.t1, e->cst));
-    RIV_HIP(hipEventRecord(s.done, e->cst));
+    // the kernels wrote the records, skip flags and packed coefficients straight to the slot's
+    // mapped host memory: the frame is ready for the host stage once the stream reaches here
+    RIV_HIP(hipEventRecord(s.t1, st));
+    RIV_HIP(hipEventRecord(s.done, st));
     e->prev_inter_slot = intra ? -1 : si;
     e->ref_idx = 1 - e->ref_idx;
     e->have_ref = true;
@@ -656,12 +643,11 @@ int riv_encoder_debug_yuv(riv_encoder* e, int which, uint8_t* y, uint8_t* u, uin
 int riv_encoder_debug_coeffs(riv_encoder* e, int16_t* out) {
     if (!e || e->last_slot < 0) return RIV_E_ARG;
     const Slot& s = e->slots[e->last_slot];
-    const int16_t* hco = s.h_coeffs.as<int16_t>();
     const uint8_t* os = s.h_optskip.as<uint8_t>();
     for (int b = 0; b < e->nb; b++) {
-        bool zero = !s.intra && os[b];
-        if (zero) std::memset(out + (size_t)b * 384, 0, 768);
-        else std::memcpy(out + (size_t)b * 384, hco + (size_t)b * 384, 768);
+        int16_t* q = out + (size_t)b * 384;
+        if (!s.intra && os[b]) std::memset(q, 0, 768);
+        else rivh::unpack_coeffs(s.h_pack.as<uint8_t>() + (size_t)b * riv::PACK_STRIDE, q);
     }
     return RIV_OK;
 }

@@ -185,6 +185,39 @@ std::vector<uint8_t> encode_residuals(const int16_t* coeffs, const uint8_t* code
     return out;
 }
 
+void unpack_coeffs(const uint8_t* slot, int16_t* q) {
+    std::memset(q, 0, 384 * sizeof(int16_t));
+    const int16_t* v = (const int16_t*)(slot + 48);
+    for (int k = 0; k < 6; k++) {
+        uint64_t m;
+        std::memcpy(&m, slot + 8 * k, 8);
+        while (m) {
+            const int bit = __builtin_ctzll(m);
+            q[64 * k + bit] = *v++;
+            m &= m - 1;
+        }
+    }
+}
+
+std::vector<uint8_t> encode_residuals_packed(const uint8_t* pack, size_t stride, const uint8_t* coded, int nb, bool dc_pred) {
+    std::vector<uint8_t> out;
+    bool any = false;
+    for (int b = 0; b < nb && !any; b++) any = coded[b] != 0;
+    if (!any) return out;
+    ResidualCtx ctx;
+    RansWriter w(out);
+    int16_t q[384];
+    for (int b = 0; b < nb; b++) {
+        if (!coded[b]) continue;
+        unpack_coeffs(pack + (size_t)b * stride, q);
+        put_plane(w, ctx, q, 256, 0, dc_pred);
+        put_plane(w, ctx, q + 256, 64, 1, dc_pred);
+        put_plane(w, ctx, q + 320, 64, 2, dc_pred);
+    }
+    w.flush();
+    return out;
+}
+
 void decode_residuals(const uint8_t* data, size_t len, const uint8_t* coded, int nb, bool dc_pred, int16_t* coeffs) {
     bool any = false;
     for (int b = 0; b < nb && !any; b++) any = coded[b] != 0;

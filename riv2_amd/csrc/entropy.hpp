@@ -67,6 +67,10 @@ struct ResidualCtx {
 };
 // Codes the coefficients of the MBs with coded[b] != 0, in raster order. coeffs: nb*384.
 std::vector<uint8_t> encode_residuals(const int16_t* coeffs, const uint8_t* coded, int nb, bool dc_pred);
+// The same coding from packed per-MB slots (six 64-bit non-zero masks, then the non-zero values
+// in index order), `stride` bytes apart.
+std::vector<uint8_t> encode_residuals_packed(const uint8_t* pack, size_t stride, const uint8_t* coded, int nb, bool dc_pred);
+void unpack_coeffs(const uint8_t* slot, int16_t* q384);
 // Inverse; fills coeffs (nb*384, caller-zeroed) of the MBs with coded[b] != 0.
 void decode_residuals(const uint8_t* data, size_t len, const uint8_t* coded, int nb, bool dc_pred, int16_t* coeffs);
 

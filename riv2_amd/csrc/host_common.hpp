@@ -76,14 +76,19 @@ struct DevBuf {
     }
     template <typename T> T* as() const { return (T*)p; }
 };
+// pinned host allocation that kernels may also write (mapped; d is its device-side address)
 struct HostBuf {
     void* p = nullptr;
+    void* d = nullptr;
     ~HostBuf() { if (p) (void)hipHostFree(p); }
     hipError_t alloc(size_t bytes) {
-        if (p) { (void)hipHostFree(p); p = nullptr; }
-        return hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault);
+        if (p) { (void)hipHostFree(p); p = nullptr; d = nullptr; }
+        hipError_t r = hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocMapped);
+        if (r != hipSuccess) return r;
+        return hipHostGetDevicePointer(&d, p, 0);
     }
     template <typename T> T* as() const { return (T*)p; }
+    template <typename T> T* dev() const { return (T*)d; }
 };
 
 }  // namespace rivhost

@@ -645,7 +645,9 @@ __global__ __launch_bounds__(256) void k_mode(ModeParams p) {
     uint32_t rdo = 0;
     if (S2 == 0 && S3 == 0 && best_sad <= p.rdo_max) rdo = (p.rdo_bits[best_sad >> 5] >> (best_sad & 31)) & 1u;
     const uint32_t w2 = (uint32_t)mode | ((uint32_t)chosen << 8) | ((uint32_t)(uint8_t)rdx << 16) | ((uint32_t)(uint8_t)rdy << 24);
-    *(uint4*)&p.rec[idx] = make_uint4(wS, wS, w2, (uint32_t)best_sad | (rdo << 16));
+    const uint4 r = make_uint4(wS, wS, w2, (uint32_t)best_sad | (rdo << 16));
+    *(uint4*)&p.rec[idx] = r;
+    *(uint4*)&p.host_rec[idx] = r;
 }
 
 // Natural log of a 16x16 block's variance from its sum and sum of squares, as the adaptive
@@ -931,6 +933,21 @@ __device__ void fdct_mb(ResidLds& L, int t) {
 }
 }  // namespace
 
+// Writes the MB's quantised coefficients (LDS) to its packed slot in mapped host memory: wave 0
+// forms the six non-zero masks with ballots, and each non-zero lane stores its value at its rank.
+__device__ __forceinline__ void write_packed(const int16_t* coef, uint8_t* slot, int t) {
+    if (t >= 64) return;
+    int base = 0;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const int16_t v = coef[64 * k + t];
+        const unsigned long long m = __ballot(v != 0);
+        if (t == 0) ((unsigned long long*)slot)[k] = m;
+        if (v != 0) ((int16_t*)(slot + 48))[base + __popcll(m & ((1ull << t) - 1ull))] = v;
+        base += __popcll(m);
+    }
+}
+
 // P-frame residual: ResidualEncoder::encode_inter (reitero_residual/src/residual.rs:396-809)
 // minus the rANS: residual = cur - pred for MBs the RDO did not skip, FDCT, dead-zone AQ
 // quantisation, all-zero -> optimised skip, else dequant + IDCT reconstruction. Writes the
@@ -938,7 +955,8 @@ __device__ void fdct_mb(ResidLds& L, int t) {
 __global__ __launch_bounds__(256) void k_inter_residual(const uint8_t* cY, const uint8_t* cU, const uint8_t* cV,
                                                         const uint8_t* pY, const uint8_t* pU, const uint8_t* pV,
                                                         int W, const BlockRec* rec, const float* steps, float dz,
-                                                        uint8_t* oY, uint8_t* oU, uint8_t* oV, int16_t* coeffs, uint8_t* opt_skip) {
+                                                        uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* pack, uint8_t* opt_skip,
+                                                        uint8_t* host_opt_skip) {
     __shared__ ResidLds L;
     const int b = blockIdx.x, bw = W >> 4, bx = b % bw, by = b / bw, t = threadIdx.x, cw = W >> 1;
     const int x0 = bx * 16, y0 = by * 16;
@@ -950,7 +968,7 @@ __global__ __launch_bounds__(256) void k_inter_residual(const uint8_t* cY, const
             size_t oc = (size_t)((y0 >> 1) + (tt >> 3)) * cw + (x0 >> 1) + (tt & 7);
             if (pl) oV[oc] = pV[oc]; else oU[oc] = pU[oc];
         }
-        if (t == 0) opt_skip[b] = 1;
+        if (t == 0) { opt_skip[b] = 1; host_opt_skip[b] = 1; }
         return;
     }
     {
@@ -976,11 +994,11 @@ __global__ __launch_bounds__(256) void k_inter_residual(const uint8_t* cY, const
             size_t oc = (size_t)((y0 >> 1) + (tt >> 3)) * cw + (x0 >> 1) + (tt & 7);
             if (pl) oV[oc] = pV[oc]; else oU[oc] = pU[oc];
         }
-        if (t == 0) opt_skip[b] = 1;
+        if (t == 0) { opt_skip[b] = 1; host_opt_skip[b] = 1; }
         return;
     }
-    if (t == 0) opt_skip[b] = 0;
-    for (int i = t; i < 384; i += 256) coeffs[(size_t)b * 384 + i] = L.coef[i];
+    if (t == 0) { opt_skip[b] = 0; host_opt_skip[b] = 0; }
+    write_packed(L.coef, pack + (size_t)b * riv::PACK_STRIDE, t);
     recon_mb(L, t, qs, false, nullptr, nullptr, pY, pU, pV, oY, oU, oV, W, x0, y0);
 }
 
@@ -988,7 +1006,7 @@ __global__ __launch_bounds__(256) void k_inter_residual(const uint8_t* cY, const
 // FDCT, per-coefficient matrix quantisation (no dead zone), integer-table dequant + IDCT, +128.
 __global__ __launch_bounds__(256) void k_intra(const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, int W,
                                                const uint16_t* yt, const uint16_t* ct, uint8_t* oY, uint8_t* oU, uint8_t* oV,
-                                               int16_t* coeffs) {
+                                               uint8_t* pack) {
     __shared__ ResidLds L;
     __shared__ uint16_t syt[256], sct[64];
     const int b = blockIdx.x, bw = W >> 4, bx = b % bw, by = b / bw, t = threadIdx.x, cw = W >> 1;
@@ -1009,9 +1027,9 @@ __global__ __launch_bounds__(256) void k_intra(const uint8_t* cY, const uint8_t*
     for (int i = t; i < 384; i += 256) {
         int16_t q = quant_matrix(L.coef32[i], i < 256 ? syt[i] : sct[(i - 256) & 63]);
         L.coef[i] = q;
-        coeffs[(size_t)b * 384 + i] = q;
     }
     __syncthreads();
+    write_packed(L.coef, pack + (size_t)b * riv::PACK_STRIDE, t);
     recon_mb(L, t, 0.0f, true, syt, sct, nullptr, nullptr, nullptr, oY, oU, oV, W, x0, y0);
 }
 
@@ -1222,13 +1240,13 @@ void logf_exp2f(hipStream_t st, const float* in, size_t n, int which, float* out
 }
 void inter_residual(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, const uint8_t* pY, const uint8_t* pU,
                     const uint8_t* pV, int W, int H, const BlockRec* rec, const float* steps, float dz, uint8_t* oY, uint8_t* oU,
-                    uint8_t* oV, int16_t* coeffs, uint8_t* opt_skip) {
+                    uint8_t* oV, uint8_t* pack, uint8_t* opt_skip, uint8_t* host_opt_skip) {
     hipLaunchKernelGGL(k_inter_residual, dim3((W / 16) * (H / 16)), dim3(256), 0, st, cY, cU, cV, pY, pU, pV, W, rec, steps, dz, oY, oU, oV,
-                       coeffs, opt_skip);
+                       pack, opt_skip, host_opt_skip);
 }
 void intra(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, int W, int H, const uint16_t* yt, const uint16_t* ct,
-           uint8_t* oY, uint8_t* oU, uint8_t* oV, int16_t* coeffs) {
-    hipLaunchKernelGGL(k_intra, dim3((W / 16) * (H / 16)), dim3(256), 0, st, cY, cU, cV, W, yt, ct, oY, oU, oV, coeffs);
+           uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* pack) {
+    hipLaunchKernelGGL(k_intra, dim3((W / 16) * (H / 16)), dim3(256), 0, st, cY, cU, cV, W, yt, ct, oY, oU, oV, pack);
 }
 void inter_recon(hipStream_t st, const uint8_t* pY, const uint8_t* pU, const uint8_t* pV, int W, int H, const uint8_t* skip,
                  const float* steps, const int16_t* coeffs, uint8_t* oY, uint8_t* oU, uint8_t* oV) {

@@ -21,6 +21,12 @@ struct BlockRec {
 };
 static_assert(sizeof(BlockRec) == 16, "BlockRec layout");
 
+// Quantised coefficients of one MB as the host entropy stage receives them: six 64-bit words
+// marking the non-zero coefficients (bit i of word k: coefficient 64k+i of the MB's 384, Y then
+// U then V, row-major), then the non-zero values as int16 in index order. Slots are PACK_STRIDE
+// bytes apart; only the used part of a slot is written.
+constexpr int PACK_STRIDE = 48 + 768;
+
 // Parameters of the motion-search wavefront for one frame.
 struct WalkParams {
     int W, H, bw, bh, sr;
@@ -51,6 +57,7 @@ struct ModeParams {
     const uint32_t* rdo_bits;        // bit s set -> RDO chooses skip at SAD s (integer-aligned MV)
     int rdo_max;                     // largest SAD that can skip, -1 when skipping is disabled
     BlockRec* rec;                   // bw*bh records out
+    BlockRec* host_rec;              // the same records, written to mapped host memory
     int* err;                        // set to 2 if a reconstructed MV would differ from its search MV
 };
 

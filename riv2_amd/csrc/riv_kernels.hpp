@@ -24,9 +24,10 @@ void aq(hipStream_t st, int nb, float base, float* lnv, float* steps);
 void logf_exp2f(hipStream_t st, const float* in, size_t n, int which, float* out);
 void inter_residual(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, const uint8_t* pY, const uint8_t* pU,
                     const uint8_t* pV, int W, int H, const riv::BlockRec* rec, const float* steps, float dz, uint8_t* oY, uint8_t* oU,
-                    uint8_t* oV, int16_t* coeffs, uint8_t* opt_skip);
+                    uint8_t* oV, uint8_t* pack, uint8_t* opt_skip, uint8_t* host_opt_skip);
+// pack: riv::PACK_STRIDE bytes per MB (mapped host memory); the layout is in riv_common.hpp
 void intra(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, int W, int H, const uint16_t* yt, const uint16_t* ct,
-           uint8_t* oY, uint8_t* oU, uint8_t* oV, int16_t* coeffs);
+           uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* pack);
 void inter_recon(hipStream_t st, const uint8_t* pY, const uint8_t* pU, const uint8_t* pV, int W, int H, const uint8_t* skip,
                  const float* steps, const int16_t* coeffs, uint8_t* oY, uint8_t* oU, uint8_t* oV);
 void intra_recon(hipStream_t st, int W, int H, const uint16_t* yt, const uint16_t* ct, const int16_t* coeffs, uint8_t* oY, uint8_t* oU,
