@@ -5,12 +5,14 @@ Workload: 1920x1080 RGB24 synthetic sequence (seeded generator: 1/f^2 background
 to 8 px/frame at half-pel steps, 12 textured rectangles of 64-320 px at up to 16 px/frame, noise
 sigma 2), intra/inter quality 85/80, search range 12, skip threshold 3, one I-frame then P-frames.
 A step is one P-frame encode_frame call (GPU analysis + host rANS + packing) with its RGB input
-already resident in HBM. Prints one JSON line.
+already resident in HBM; the default run is the 50-frame sequence (1 I + 4 warm-up P + 45 timed
+P). Prints one JSON line.
 
-The roofline entry is for the dominant kernel (the motion-search + mode-decision wavefront):
-achieved = 2*Ws*Hs algorithmic bytes (current + reference luma, each read once) / its average
-duration from HIP events on the encoder's stream. cpu_baseline times the oracle (a C++
-restatement of the reference CPU encoder) on the host cores over a bounded sample.
+The roofline entry is for the dominant stage, the motion search (SAD tables, wavefront, mode
+decision): achieved = 2*Ws*Hs algorithmic bytes (current + reference luma, each read once) / its
+average duration from HIP events on the encoder's stream over the timed frames. cpu_baseline
+times the oracle (a C++ restatement of the reference CPU encoder) on the host cores over a
+bounded sample.
 """
 import argparse
 import json
@@ -52,6 +54,9 @@ def make_config(riv2_amd, depth):
 
 
 def run_gpu(frames, steps, warmup, depth):
+    """Times `steps` P-frames (after one I-frame and `warmup` P-frames) through the pipelined
+    encoder, inputs resident in HBM; the motion-search stage is bracketed by HIP events on the
+    encoder's stream. Returns elapsed seconds, per-frame stats, stage times and the stream."""
     import torch
     import riv2_amd
 
@@ -66,7 +71,7 @@ def run_gpu(frames, steps, warmup, depth):
         enc.encode_frame_device(ptr(i), fbytes, W, H, i * 20)
     enc.flush()
     enc.pop_stats(4096)
-    enc.set_profiling(True)
+    enc.set_profiling(1)   # two events per frame, around the motion search
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(1 + warmup, 1 + warmup + steps):
@@ -79,7 +84,22 @@ def run_gpu(frames, steps, warmup, depth):
     enc.finish()
     data = enc.output()
     enc.close()
-    return t1 - t0, stats, stages, data
+    return t1 - t0, stats, stages, data, d_frames
+
+
+def stage_breakdown(d_frames, n, depth):
+    """Per-stage device times from a separate, untimed pass with events around every stage."""
+    import riv2_amd
+    enc = riv2_amd.Encoder(make_config(riv2_amd, depth))
+    enc.encode_frame_device(d_frames[0].data_ptr(), W * H * 3, W, H, 0)
+    enc.flush()
+    enc.set_profiling(2)
+    for i in range(1, n):
+        enc.encode_frame_device(d_frames[i].data_ptr(), W * H * 3, W, H, i * 20)
+    enc.flush()
+    st = enc.stage_times()
+    enc.close()
+    return {k: round(v[0] / v[1], 4) for k, v in st.items() if v[1]}
 
 
 def check_prefix_bit_exact(frames, gpu_stream, nframes):
@@ -108,7 +128,7 @@ def cpu_baseline(frames, pframes, threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=45)
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-pframes", type=int, default=8)
@@ -129,17 +149,19 @@ def main():
     frames = synth.Sequence(synth.hd_params(seed=args.seed, frames=max(nframes, 50))).frames(0, nframes)
 
     print(f"[bench] GPU encode: 1 I + {args.warmup} warmup + {args.steps} timed P-frames", file=sys.stderr, flush=True)
-    elapsed, stats, stages, stream = run_gpu(frames, args.steps, args.warmup, args.pipeline_depth)
+    elapsed, stats, stages, stream, d_frames = run_gpu(frames, args.steps, args.warmup, args.pipeline_depth)
+    stage_avg = stage_breakdown(d_frames, min(nframes, 12), args.pipeline_depth)
+    del d_frames
     fps = args.steps / elapsed
     ms_per_step = 1000.0 * elapsed / args.steps
 
-    walk_ms, walk_n = stages["me_walk"]
+    walk_ms, walk_n = stages["motion_search"]
     walk_avg_ms = walk_ms / max(1, walk_n)
     me_bytes = 2 * SW * SH   # current + reference luma planes, each read once
     achieved = me_bytes / (walk_avg_ms * 1e-3) / 1e9 if walk_avg_ms > 0 else 0.0
 
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "me_walk_1080p_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "motion_search_1080p_traffic.json")
     if os.path.exists(tpath):
         try:
             with open(tpath) as f:
@@ -168,7 +190,6 @@ def main():
                "sample": f"1920x1080 synthetic, 1 I + {args.cpu_pframes} P-frames timed, oracle C++ restatement of the "
                          f"reference encoder with its `threads` strip ME on {cores} threads ({cpu_model()})"}
 
-    stage_avg = {k: round(v[0] / v[1], 4) for k, v in stages.items() if v[1]}
     out = {
         "metric": "1080p P-frame encode fps/GPU: plain bit-exact vs learned-MV at ≤1% bitrate loss",
         "value": round(fps, 3),
@@ -187,11 +208,13 @@ def main():
                    "pipeline_depth": args.pipeline_depth, "bit_exact_prefix_vs_oracle": bool(bit_exact)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
-                     "kernel": "k_me_walk", "algorithmic_bytes_per_launch": me_bytes, "avg_launch_ms": round(walk_avg_ms, 4)},
+                     "kernel": "motion search (k_sad_table + k_hp_table + k_me_walk + k_mode, one frame)",
+                     "algorithmic_bytes_per_launch": me_bytes, "avg_launch_ms": round(walk_avg_ms, 4)},
         "cpu_baseline": cpu,
         "gpu_stage_avg_ms": stage_avg,
         "gpu_frame_avg_ms": round(sum(s.gpu_ms for s in stats) / len(stats), 4),
         "host_entropy_avg_ms": round(sum(s.host_entropy_ms for s in stats) / len(stats), 4),
+        "host_tail_avg_ms": round(sum(s.host_tail_ms for s in stats) / len(stats), 4),
         "bytes_per_pframe": round(sum(s.total_bytes for s in stats) / len(stats), 1),
     }
     if cpu:

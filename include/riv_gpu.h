@@ -100,7 +100,8 @@ typedef struct {
     double rdo_lambda;
     uint32_t rdo_blocks_evaluated, rdo_blocks_selected_skip, rdo_forced_fractional, rdo_forced_threshold, rdo_forced_disabled;
     double gpu_ms;                            /* device time of this frame's kernels (no reference counterpart) */
-    double host_entropy_ms;                   /* host rANS + packing time (no reference counterpart) */
+    double host_entropy_ms;                   /* host residual rANS time of this frame (no reference counterpart) */
+    double host_tail_ms;                      /* host ordered tail: MV rANS, packing, write (no reference counterpart) */
 } riv_frame_stats;
 
 typedef struct riv_encoder riv_encoder;
@@ -180,13 +181,21 @@ int riv_gpu_dct16_aq_plane(int32_t device_id, const int16_t* plane, uint32_t w, 
                            int16_t* coeffs, int16_t* recon);
 /* compute_aq_quant_steps (reitero_residual/src/residual.rs:116-167) on a luma plane */
 int riv_gpu_aq_steps(int32_t device_id, const uint8_t* y, uint32_t w, uint32_t h, float base_step, float* out);
+/* The motion search's SAD tables (sad_block_int_limit, satd_scalar.rs:4-99, and
+ * sad_block_halfpel_limit_luma, satd_scalar.rs:171-223, at every candidate the search can visit):
+ * per 16x16 block, int_sad[(dy+sr)*(2sr+1) + dx+sr] for dx,dy in [-sr,sr] and
+ * hp_sad[(hy+2sr+1)*(4sr+3) + hx+2sr+1] for hx,hy in [-2sr-1,2sr+1] (half-pel units), blocks in
+ * raster order. `cur` and `ref` are w x h luma planes (multiples of 16); sr <= 32. */
+int riv_gpu_me_tables(int32_t device_id, const uint8_t* cur, const uint8_t* ref, uint32_t w, uint32_t h, int32_t sr,
+                      uint16_t* int_sad, uint16_t* hp_sad);
 /* glibc-identical logf / exp2f on the device (used by AQ), n values */
 int riv_gpu_logf_exp2f(int32_t device_id, const float* in, size_t n, int32_t which, float* out);
 
 /* Per-stage device timing of the encoder (HIP events on the encoder's stream around each stage;
- * no reference counterpart). Stages: 0 RGB->YUV, 1 motion search + mode decision wavefront,
- * 2 motion compensation, 3 adaptive quantiser, 4 residual transform/quantise/reconstruct,
- * 5 deblock, 6 intra transform, 7 device->host copies. */
+ * no reference counterpart). Stages: 0 RGB->YUV, 1 motion search (SAD tables, wavefront, mode
+ * decision), 2 motion compensation, 3 adaptive quantiser, 4 residual transform/quantise/
+ * reconstruct, 5 deblock, 6 intra transform, 7 unused. `on`: 0 off, 1 the motion-search stage
+ * only (two events per frame), 2 every stage. */
 #define RIV_NUM_STAGES 8
 int riv_encoder_set_profiling(riv_encoder* enc, int on);
 /* Accumulated milliseconds and launch counts per stage since profiling was switched on. */

@@ -54,7 +54,7 @@ class FrameStatsC(Structure):
         ("rdo_lambda", c_double),
         ("rdo_blocks_evaluated", c_uint32), ("rdo_blocks_selected_skip", c_uint32), ("rdo_forced_fractional", c_uint32),
         ("rdo_forced_threshold", c_uint32), ("rdo_forced_disabled", c_uint32),
-        ("gpu_ms", c_double), ("host_entropy_ms", c_double),
+        ("gpu_ms", c_double), ("host_entropy_ms", c_double), ("host_tail_ms", c_double),
     ]
 
 
@@ -111,6 +111,7 @@ PROTOTYPES = [
     ("riv_gpu_dct8_aq_plane", c_int, [c_int32, c_void_p, c_uint32, c_uint32, c_void_p, c_float, c_void_p, c_void_p]),
     ("riv_gpu_dct16_aq_plane", c_int, [c_int32, c_void_p, c_uint32, c_uint32, c_void_p, c_float, c_void_p, c_void_p]),
     ("riv_gpu_aq_steps", c_int, [c_int32, c_void_p, c_uint32, c_uint32, c_float, c_void_p]),
+    ("riv_gpu_me_tables", c_int, [c_int32, c_void_p, c_void_p, c_uint32, c_uint32, c_int32, c_void_p, c_void_p]),
     ("riv_gpu_logf_exp2f", c_int, [c_int32, c_void_p, c_size_t, c_int32, c_void_p]),
     ("riv_encoder_set_profiling", c_int, [c_void_p, c_int]),
     ("riv_encoder_stage_times", c_int, [c_void_p, c_void_p, c_void_p]),

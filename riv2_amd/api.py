@@ -137,13 +137,14 @@ class EncodeFrameStats:
     rdo_blocks_selected_skip: int
     gpu_ms: float
     host_entropy_ms: float
+    host_tail_ms: float = 0.0
 
     @staticmethod
     def from_c(s):
         return EncodeFrameStats(s.frame_index, s.timestamp_ms, s.frame_type, s.total_bytes, s.mv_bytes, s.residual_bytes,
                                 s.storage_width, s.storage_height, s.blocks_total, s.blocks_skipped, list(s.mv_mode_counts),
                                 s.mv_new_blocks, s.mv_bias_dx, s.mv_bias_dy, s.rdo_lambda, s.rdo_blocks_evaluated,
-                                s.rdo_blocks_selected_skip, s.gpu_ms, s.host_entropy_ms)
+                                s.rdo_blocks_selected_skip, s.gpu_ms, s.host_entropy_ms, s.host_tail_ms)
 
 
 class VecWriter:
@@ -280,10 +281,12 @@ class Encoder:
         _check(L.lib().riv_encoder_output(self._h, byref(p), byref(n)), EncodeError)
         return ctypes.string_at(p, n.value)
 
-    STAGES = ["rgb_to_yuv", "me_walk", "mc", "aq", "residual", "deblock", "intra", "d2h"]
+    STAGES = ["rgb_to_yuv", "motion_search", "mc", "aq", "residual", "deblock", "intra", "unused"]
 
-    def set_profiling(self, on: bool = True):
-        _check(L.lib().riv_encoder_set_profiling(self._h, int(bool(on))))
+    def set_profiling(self, on=True):
+        """0/False off, 1 the motion-search stage only, 2/True every stage."""
+        level = 2 if on is True else 0 if on is False else int(on)
+        _check(L.lib().riv_encoder_set_profiling(self._h, level))
 
     def stage_times(self):
         ms = np.zeros(8, np.float64)

@@ -109,7 +109,7 @@ struct riv_encoder {
     std::deque<riv_frame_stats> done_stats;
     int last_slot = -1;            // slot of the most recently completed frame (debug accessors)
 
-    bool profiling = false;
+    int profiling = 0;             // 0 off, 1 motion-search stage only, 2 every stage
     double stage_ms[RIV_NUM_STAGES] = {};
     uint32_t stage_n[RIV_NUM_STAGES] = {};
 
@@ -385,9 +385,11 @@ static void entropy_job(riv_encoder* e, int si) {
         e->fail(RIV_E_ENCODING_FAILED, std::string("entropy coding failed: ") + ex.what());
         return;
     }
+    st.host_entropy_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     // ordered tail
     std::unique_lock<std::mutex> lk(e->mu);
     e->cv.wait(lk, [&] { return e->next_tail == s.index || e->err_code; });
+    const auto t_tail = std::chrono::steady_clock::now();
     if (e->err_code) { e->next_tail++; e->cv.notify_all(); return; }
     try {
         if (s.intra) {
@@ -413,7 +415,7 @@ static void entropy_job(riv_encoder* e, int si) {
     int r = e->write(bytes.data(), bytes.size());
     if (r && !e->err_code) { e->err_code = r; e->err_msg = "writer failed"; }
     st.total_bytes = (uint32_t)bytes.size();
-    st.host_entropy_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    st.host_tail_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_tail).count();
     if (e->profiling)
         for (int k = 0; k < RIV_NUM_STAGES; k++) { e->stage_ms[k] += smark[k]; e->stage_n[k] += nmark[k]; }
     e->done_stats.push_back(st);
@@ -454,6 +456,7 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
         for (auto& m : s.marks) RIV_HIP(hipEventCreate(&m));
     auto mark = [&](int stage) {
         if (!e->profiling || s.n_marks > RIV_NUM_STAGES) return;
+        if (e->profiling == 1 && stage != 0 && stage != 1) return;   // two events per frame: around the search
         (void)hipEventRecord(s.marks[s.n_marks], e->st);
         s.mark_stage[s.n_marks++] = stage;
     };
@@ -603,7 +606,7 @@ void riv_encoder_destroy(riv_encoder* e) { destroy_encoder(e); }
 int riv_encoder_set_profiling(riv_encoder* e, int on) {
     if (!e) return RIV_E_ARG;
     std::lock_guard<std::mutex> lk(e->mu);
-    e->profiling = on != 0;
+    e->profiling = on < 0 ? 0 : on > 2 ? 2 : on;
     for (int i = 0; i < RIV_NUM_STAGES; i++) { e->stage_ms[i] = 0; e->stage_n[i] = 0; }
     return RIV_OK;
 }

@@ -23,6 +23,15 @@ struct P0Table {
 };
 const P0Table& p0table() { static P0Table tab; return tab; }
 inline uint16_t swap_bytes(uint16_t v) { return (uint16_t)((v << 8) | (v >> 8)); }
+// x / d for x < 2^32 and 1 <= d <= 256 without a divide: with M = floor(2^40 / d) + 1,
+// x*M / 2^40 = x/d + e where 0 < e < x / 2^40 < 1/256 <= 1/d, and frac(x/d) <= 1 - 1/d, so the
+// floor is exact.
+struct RcpTable {
+    uint64_t m[257];
+    RcpTable() { m[0] = 0; for (int d = 1; d <= 256; d++) m[d] = (((uint64_t)1) << 40) / (uint64_t)d + 1; }
+};
+const RcpTable g_rcp;
+inline uint32_t div_small(uint32_t x, uint32_t d) { return (uint32_t)(((unsigned __int128)x * g_rcp.m[d]) >> 40); }
 constexpr uint32_t kLow = 1u << 16;   // normalisation interval [2^16, 2^24)
 }  // namespace
 
@@ -54,7 +63,8 @@ void RansWriter::drain() {
         uint32_t p = sym_[k] ? sym_[k] : 1u;
         uint32_t start = bits_[k] ? p : 0u, freq = bits_[k] ? 256u - p : p;
         if (x >= (freq << 24)) { emitted.push_back((uint16_t)x); x >>= 16; }
-        x = ((x / freq) << 8) + (x % freq) + start;
+        const uint32_t q = div_small(x, freq);
+        x = (q << 8) + (x - q * freq) + start;
     };
     int k = n_ - 1;
     int cur = 0;

@@ -734,11 +734,20 @@ __global__ __launch_bounds__(256) void k_mc(const uint8_t* __restrict__ rY, cons
 // sequential f32 sum in raster order (one lane: the rounding sequence is part of the result);
 // step = base * exp2f((0.8 * (ln - avg)) / 6).
 
-// One workgroup: the logs pass through LDS in chunks; lane 0 adds them in order, reading 64
-// values ahead so that LDS latency hides behind the dependent adds. The mean goes to lnv[nb].
+// One workgroup: the logs pass through LDS in chunks; lane 0 adds them in order while the next
+// 64 values are already on their way from LDS (two register buffers), so LDS latency hides behind
+// the dependent adds. The mean goes to lnv[nb].
+__device__ __forceinline__ float add64(float sum, const float4* q) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        sum = __fadd_rn(sum, q[k].x); sum = __fadd_rn(sum, q[k].y);
+        sum = __fadd_rn(sum, q[k].z); sum = __fadd_rn(sum, q[k].w);
+    }
+    return sum;
+}
 __global__ __launch_bounds__(256) void k_aq_mean(float* __restrict__ lnv, int nb) {
-    __shared__ float4 s_ln[2048];   // 8192 logs per pass
-    float sum = 0.0f;               // lives in thread 0
+    __shared__ float4 s_ln[2048 + 16];   // 8192 logs per pass, plus one chunk of slack for the read-ahead
+    float sum = 0.0f;                    // lives in thread 0
     for (int c0 = 0; c0 < nb; c0 += 8192) {
         __syncthreads();
         const int cn = min(8192, nb - c0);
@@ -746,17 +755,22 @@ __global__ __launch_bounds__(256) void k_aq_mean(float* __restrict__ lnv, int nb
         for (int i = threadIdx.x; i < cn; i += blockDim.x) sl[i] = lnv[c0 + i];
         __syncthreads();
         if (threadIdx.x == 0) {
-            int i = 0;
-            for (; i + 64 <= cn; i += 64) {
-                float4 q[16];
+            const int full = cn & ~63;
+            float4 a[16], b[16];
+            if (full) {
 #pragma unroll
-                for (int k = 0; k < 16; k++) q[k] = s_ln[(i >> 2) + k];
-#pragma unroll
-                for (int k = 0; k < 16; k++) {
-                    sum = __fadd_rn(sum, q[k].x); sum = __fadd_rn(sum, q[k].y);
-                    sum = __fadd_rn(sum, q[k].z); sum = __fadd_rn(sum, q[k].w);
-                }
+                for (int k = 0; k < 16; k++) a[k] = s_ln[k];
             }
+            int i = 0;
+            for (; i + 128 <= full; i += 128) {
+#pragma unroll
+                for (int k = 0; k < 16; k++) b[k] = s_ln[((i + 64) >> 2) + k];
+                sum = add64(sum, a);
+#pragma unroll
+                for (int k = 0; k < 16; k++) a[k] = s_ln[((i + 128) >> 2) + k];   // slack covers the last read-ahead
+                sum = add64(sum, b);
+            }
+            if (i < full) { sum = add64(sum, a); i += 64; }
             for (; i < cn; i++) sum = __fadd_rn(sum, sl[i]);
         }
     }
@@ -1204,6 +1218,7 @@ void yuv420_to_rgb(hipStream_t st, const uint8_t* Y, const uint8_t* U, const uin
 int sad_table_stride(int sr) { return ((2 * sr + 1) * (2 * sr + 1) + 7) & ~7; }
 int hp_table_stride(int sr) { return ((4 * sr + 3) * (4 * sr + 3) + 7) & ~7; }
 int table_region_bytes(int sr) { return 2 * (sad_table_stride(sr) + hp_table_stride(sr)); }
+bool mb_is_interior(int x0, int y0, int W, int H, int sr) { return mb_interior(x0, y0, W, H, sr); }
 size_t walk_lds_bytes(int sr, int bw, int R, int slots) {
     return (size_t)R * slots * table_region_bytes(sr) + (size_t)R * bw * 12 + 256 * sizeof(double) + 64;
 }

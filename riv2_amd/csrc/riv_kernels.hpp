@@ -12,6 +12,7 @@ void yuv420_to_rgb(hipStream_t st, const uint8_t* Y, const uint8_t* U, const uin
 int sad_table_stride(int sr);
 int hp_table_stride(int sr);
 int table_region_bytes(int sr);
+bool mb_is_interior(int x0, int y0, int W, int H, int sr);   // its integer SADs are the half-pel table's even entries
 void sad_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int H, int sr, uint16_t* out, int stride);
 void hp_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int H, int sr, uint16_t* out, int stride);
 size_t walk_lds_bytes(int sr, int bw, int R, int slots);

@@ -1,6 +1,7 @@
 // Kernel-level seams of the C ABI: single-stage GPU entry points on host buffers, used to check
 // each stage against the reference functions it replaces.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <vector>
 #include "host_common.hpp"
 #include "riv_kernels.hpp"
@@ -42,6 +43,37 @@ int riv_gpu_deblock_yuv420(int32_t dev, uint8_t* y, uint8_t* u, uint8_t* v, uint
     RIV_HIP(hipGetLastError());
     RIV_HIP(hipMemcpy(y, F, ysz, hipMemcpyDeviceToHost)); RIV_HIP(hipMemcpy(u, F + ysz, csz, hipMemcpyDeviceToHost));
     RIV_HIP(hipMemcpy(v, F + ysz + csz, csz, hipMemcpyDeviceToHost));
+    return RIV_OK;
+}
+
+int riv_gpu_me_tables(int32_t dev, const uint8_t* cur, const uint8_t* ref, uint32_t w, uint32_t h, int32_t sr, uint16_t* int_sad,
+                      uint16_t* hp_sad) {
+    if (!cur || !ref || !int_sad || !hp_sad || !w || !h || w % 16 || h % 16 || sr < 0 || sr > 32) return RIV_E_ARG;
+    int r = ensure_device(dev);
+    if (r) return r;
+    const size_t ysz = (size_t)w * h, nb = ysz / 256;
+    const int D = 2 * sr + 1, D2 = 4 * sr + 3, RB = rivk::table_region_bytes(sr), TSB = 2 * rivk::sad_table_stride(sr);
+    DevBuf dc, dr, dt;
+    RIV_HIP(dc.alloc(ysz)); RIV_HIP(dr.alloc(ysz)); RIV_HIP(dt.alloc((size_t)RB * nb));
+    RIV_HIP(hipMemcpy(dc.p, cur, ysz, hipMemcpyHostToDevice)); RIV_HIP(hipMemcpy(dr.p, ref, ysz, hipMemcpyHostToDevice));
+    uint8_t* T = dt.as<uint8_t>();
+    rivk::sad_table(0, dc.as<uint8_t>(), dr.as<uint8_t>(), (int)w, (int)h, sr, (uint16_t*)T, RB / 2);
+    rivk::hp_table(0, dc.as<uint8_t>(), dr.as<uint8_t>(), (int)w, (int)h, sr, (uint16_t*)(T + TSB), RB / 2);
+    RIV_HIP(hipGetLastError());
+    std::vector<uint8_t> t((size_t)RB * nb);
+    RIV_HIP(hipMemcpy(t.data(), T, t.size(), hipMemcpyDeviceToHost));
+    const int bw = (int)w / 16;
+    for (size_t b = 0; b < nb; b++) {
+        const uint16_t* it = (const uint16_t*)(t.data() + b * RB);
+        const uint16_t* hp = (const uint16_t*)(t.data() + b * RB + TSB);
+        std::copy(hp, hp + (size_t)D2 * D2, hp_sad + b * D2 * D2);
+        // the search reads an interior block's integer SADs from the half-pel table's even entries
+        const bool interior = rivk::mb_is_interior((int)(b % bw) * 16, (int)(b / bw) * 16, (int)w, (int)h, sr);
+        for (int dy = -sr; dy <= sr; dy++)
+            for (int dx = -sr; dx <= sr; dx++)
+                int_sad[b * D * D + (dy + sr) * D + dx + sr] =
+                    interior ? hp[(2 * dy + 2 * sr + 1) * D2 + 2 * dx + 2 * sr + 1] : it[(dy + sr) * D + dx + sr];
+    }
     return RIV_OK;
 }
 
