@@ -102,6 +102,7 @@ typedef struct {
     double gpu_ms;                            /* device time of this frame's kernels (no reference counterpart) */
     double host_entropy_ms;                   /* host residual rANS time of this frame (no reference counterpart) */
     double host_tail_ms;                      /* host ordered tail: MV rANS, packing, write (no reference counterpart) */
+    uint32_t me_passes;                       /* relaxation passes of the motion search, 0 for intra (no reference counterpart) */
 } riv_frame_stats;
 
 typedef struct riv_encoder riv_encoder;
@@ -192,7 +193,7 @@ int riv_gpu_me_tables(int32_t device_id, const uint8_t* cur, const uint8_t* ref,
 int riv_gpu_logf_exp2f(int32_t device_id, const float* in, size_t n, int32_t which, float* out);
 
 /* Per-stage device timing of the encoder (HIP events on the encoder's stream around each stage;
- * no reference counterpart). Stages: 0 RGB->YUV, 1 motion search (SAD tables, wavefront, mode
+ * no reference counterpart). Stages: 0 RGB->YUV, 1 motion search (relaxation passes and mode
  * decision), 2 motion compensation, 3 adaptive quantiser, 4 residual transform/quantise/
  * reconstruct, 5 deblock, 6 intra transform, 7 unused. `on`: 0 off, 1 the motion-search stage
  * only (two events per frame), 2 every stage. */

@@ -55,6 +55,7 @@ class FrameStatsC(Structure):
         ("rdo_blocks_evaluated", c_uint32), ("rdo_blocks_selected_skip", c_uint32), ("rdo_forced_fractional", c_uint32),
         ("rdo_forced_threshold", c_uint32), ("rdo_forced_disabled", c_uint32),
         ("gpu_ms", c_double), ("host_entropy_ms", c_double), ("host_tail_ms", c_double),
+        ("me_passes", c_uint32),
     ]
 
 

@@ -138,13 +138,14 @@ class EncodeFrameStats:
     gpu_ms: float
     host_entropy_ms: float
     host_tail_ms: float = 0.0
+    me_passes: int = 0
 
     @staticmethod
     def from_c(s):
         return EncodeFrameStats(s.frame_index, s.timestamp_ms, s.frame_type, s.total_bytes, s.mv_bytes, s.residual_bytes,
                                 s.storage_width, s.storage_height, s.blocks_total, s.blocks_skipped, list(s.mv_mode_counts),
                                 s.mv_new_blocks, s.mv_bias_dx, s.mv_bias_dy, s.rdo_lambda, s.rdo_blocks_evaluated,
-                                s.rdo_blocks_selected_skip, s.gpu_ms, s.host_entropy_ms, s.host_tail_ms)
+                                s.rdo_blocks_selected_skip, s.gpu_ms, s.host_entropy_ms, s.host_tail_ms, s.me_passes)
 
 
 class VecWriter:

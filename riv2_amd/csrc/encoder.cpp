@@ -26,6 +26,7 @@ namespace {
 struct Slot {
     DevBuf rec, optskip;                 // device outputs of the frame (rec also seeds the next frame's temporal MVs)
     HostBuf h_rec, h_pack, h_optskip;    // mapped host memory the kernels write for the host entropy stage
+    HostBuf h_passes;                    // mapped: the motion search's pass count
     hipEvent_t done = nullptr;           // the frame's kernels (and so its host-side outputs) finished
     hipEvent_t marks[RIV_NUM_STAGES + 1] = {};
     int mark_stage[RIV_NUM_STAGES + 1] = {};
@@ -83,13 +84,13 @@ struct riv_encoder {
     riv_video_header hdr{};
     uint64_t frame_count = 0;      // frames submitted
 
-    int W = 0, H = 0, bw = 0, bh = 0, nb = 0, R = 4, tab_slots = 2;
+    int W = 0, H = 0, bw = 0, bh = 0, nb = 0;
+    int s_waves = 1, s_groups = 1, s_maxp = 0, s_bank = 0;   // motion-search launch shape, pass limit, counter bank
     hipStream_t st = nullptr;    // kernels, frame after frame
-    DevBuf rgb, cur, pred, ref[2], var, lnv, steps, mask, pub, tabs, smv, bits, rdo, err, yt, ct;
+    DevBuf rgb, cur, pred, ref[2], var, lnv, steps, mask, field, arrive, smv, bits, rdo, err, yt, ct;
     int ref_idx = 0;
     bool have_ref = false;
     int prev_inter_slot = -1;      // slot holding the previous P-frame's records, -1 after an intra frame
-    unsigned long long epoch = 0;
     double lambda = 0;
     int c_zero = 0, c_pred = 0, rdo_max = -1;
     float inter_qs = 0, intra_qs = 0;
@@ -170,18 +171,20 @@ static int encoder_init(riv_encoder* e) {
     if (r) return r;
     e->W = (int)c.storage_width; e->H = (int)c.storage_height;
     e->bw = e->W / 16; e->bh = e->H / 16; e->nb = e->bw * e->bh;
-    // wavefront geometry: up to 4 MB rows per workgroup, each with the next block's SAD tables
-    // prefetched into a second LDS slot, as far as the 64 KiB of LDS of one workgroup allow
+    // motion-search geometry: one resident workgroup per CU at most, as many waves per workgroup
+    // as 64 KiB of LDS holds (one reference window each), at least one wave per 64 blocks
     {
         const size_t budget = 65536;
-        e->tab_slots = 2;
-        if (rivk::walk_lds_bytes(c.search_range, e->bw, 1, 2) > budget) e->tab_slots = 1;
-        e->R = 4;
-        while (e->R > 1 && rivk::walk_lds_bytes(c.search_range, e->bw, e->R, e->tab_slots) > budget) e->R--;
-        if (rivk::walk_lds_bytes(c.search_range, e->bw, e->R, e->tab_slots) > budget) {
-            set_error("frame too wide for the motion-search workgroup's LDS at this search range");
+        e->s_waves = 16;
+        while (e->s_waves > 1 && rivk::search_lds_bytes(c.search_range, e->s_waves) > budget) e->s_waves--;
+        hipDeviceProp_t prop;
+        RIV_HIP(hipGetDeviceProperties(&prop, c.device_id));
+        e->s_groups = std::max(1, std::min(prop.multiProcessorCount, (e->nb + e->s_waves - 1) / e->s_waves));
+        if ((long long)e->s_groups * e->s_waves * 64 < e->nb) {
+            set_error("frame has more macroblocks than the motion search's resident waves can hold");
             return RIV_E_UNSUPPORTED;
         }
+        e->s_maxp = 2 * (e->bh - 1) + e->bw + 2;
     }
     const size_t ysz = (size_t)e->W * e->H, csz = ysz / 4, fsz = ysz + 2 * csz;
     RIV_HIP(hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking));
@@ -190,8 +193,8 @@ static int encoder_init(riv_encoder* e) {
     RIV_HIP(e->ref[0].alloc(fsz)); RIV_HIP(e->ref[1].alloc(fsz));
     RIV_HIP(e->var.alloc(8 * (size_t)e->nb)); RIV_HIP(e->lnv.alloc(4 * ((size_t)e->nb + 1))); RIV_HIP(e->steps.alloc(4 * (size_t)e->nb));
     RIV_HIP(e->mask.alloc(e->nb));
-    RIV_HIP(e->pub.alloc(8 * (size_t)e->nb)); RIV_HIP(hipMemset(e->pub.p, 0, 8 * (size_t)e->nb));
-    RIV_HIP(e->tabs.alloc((size_t)e->nb * rivk::table_region_bytes(c.search_range)));
+    RIV_HIP(e->field.alloc(8 * (size_t)e->nb));
+    RIV_HIP(e->arrive.alloc(8 * (size_t)e->s_maxp)); RIV_HIP(hipMemset(e->arrive.p, 0, 8 * (size_t)e->s_maxp));
     RIV_HIP(e->smv.alloc(8 * (size_t)e->nb));
     RIV_HIP(e->err.alloc(4)); RIV_HIP(hipMemset(e->err.p, 0, 4));
     e->depth = c.pipeline_depth;
@@ -199,7 +202,7 @@ static int encoder_init(riv_encoder* e) {
     for (Slot& s : e->slots) {
         RIV_HIP(s.rec.alloc(sizeof(BlockRec) * e->nb)); RIV_HIP(s.optskip.alloc(e->nb));
         RIV_HIP(s.h_rec.alloc(sizeof(BlockRec) * e->nb)); RIV_HIP(s.h_pack.alloc((size_t)riv::PACK_STRIDE * e->nb));
-        RIV_HIP(s.h_optskip.alloc(e->nb));
+        RIV_HIP(s.h_optskip.alloc(e->nb)); RIV_HIP(s.h_passes.alloc(16));
         RIV_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
         RIV_HIP(hipEventCreate(&s.t0)); RIV_HIP(hipEventCreate(&s.t1));
     }
@@ -379,6 +382,7 @@ static void entropy_job(riv_encoder* e, int si) {
             st.frame_type = 2;
             st.mv_bias_dx = bdx; st.mv_bias_dy = bdy;
             st.rdo_lambda = e->lambda; st.rdo_blocks_evaluated = (uint32_t)nb;
+            st.me_passes = (uint32_t)*s.h_passes.as<int>();
             if (c.skip_threshold == 0) st.rdo_forced_disabled = (uint32_t)nb;
         }
     } catch (const std::exception& ex) {
@@ -485,23 +489,19 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
         rivk::deblock(st, nY, nU, nV, W, H, deblock_level(e->intra_qs), true, nullptr);
         mark(5);
     } else {
-        const int sr = c.search_range, RB = rivk::table_region_bytes(sr), TSB = 2 * rivk::sad_table_stride(sr);
-        uint8_t* tabs = e->tabs.as<uint8_t>();
-        rivk::sad_table(st, cY, rY, W, H, sr, (uint16_t*)tabs, RB / 2);
-        rivk::hp_table(st, cY, rY, W, H, sr, (uint16_t*)(tabs + TSB), RB / 2);
         const int8_t* prev_mv = e->prev_inter_slot >= 0 ? (const int8_t*)e->slots[e->prev_inter_slot].rec.p + 4 : nullptr;
-        riv::WalkParams p{};
-        p.W = W; p.H = H; p.bw = e->bw; p.bh = e->bh; p.sr = sr;
+        riv::SearchParams p{};
+        p.cur = cY; p.ref = rY;
+        p.W = W; p.H = H; p.bw = e->bw; p.bh = e->bh; p.sr = c.search_range;
         p.prev_mv = prev_mv; p.prev_stride = sizeof(BlockRec);
-        p.tabs = tabs; p.tab_bytes = RB; p.tsb = TSB; p.slots = e->tab_slots;
-        p.smv = e->smv.as<unsigned long long>(); p.pub = e->pub.as<unsigned long long>();
-        e->epoch = e->epoch % 0x0fffffffull + 1;   // 1 .. 2^28-1, never the zero of a fresh buffer
-        p.tag = (unsigned)e->epoch;
+        p.field = e->field.as<uint32_t>(); p.smv = e->smv.as<unsigned long long>();
+        p.arrive = e->arrive.as<unsigned>(); p.bank = e->s_bank; p.maxp = e->s_maxp;
+        e->s_bank ^= 1;
         p.bits = e->bits.as<double>(); p.lam2 = e->lambda * 2.0; p.c_zero = e->c_zero; p.c_pred = e->c_pred;
         p.lam_zero = e->lambda == 0.0;
         p.zthr = c.me_zero_mv_threshold; p.pthr = c.me_predictor_threshold;
-        p.R = e->R; p.err = e->err.as<int>();
-        RIV_HIP(rivk::me_walk(st, p));
+        p.err = e->err.as<int>(); p.passes = s.h_passes.dev<int>();
+        RIV_HIP(rivk::me_search(st, p, e->s_groups, e->s_waves));
         riv::ModeParams mp{};
         mp.bw = e->bw; mp.bh = e->bh; mp.smv = p.smv; mp.prev_mv = prev_mv; mp.prev_stride = sizeof(BlockRec);
         mp.rdo_bits = e->rdo.as<uint32_t>(); mp.rdo_max = e->rdo_max; mp.rec = rec; mp.host_rec = s.h_rec.dev<BlockRec>();
@@ -538,7 +538,7 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
     if (!intra) {
         int werr = 0;
         RIV_HIP(hipMemcpy(&werr, e->err.p, 4, hipMemcpyDeviceToHost));
-        if (werr) { set_error(werr == 2 ? "mode decision: reconstructed MV differs from search MV" : "motion-search wavefront timed out"); return RIV_E_GPU; }
+        if (werr) { set_error(werr == 2 ? "mode decision: reconstructed MV differs from search MV" : werr == 3 ? "motion search did not settle within its pass limit" : "motion-search barrier timed out"); return RIV_E_GPU; }
     }
     std::lock_guard<std::mutex> lk(e->mu);
     if (e->err_code) { set_error(e->err_msg); return e->err_code; }
@@ -565,7 +565,7 @@ int riv_encoder_flush(riv_encoder* e) {
     lk.unlock();
     RIV_HIP(hipSetDevice(e->cfg.device_id));
     RIV_HIP(hipMemcpy(&werr, e->err.p, 4, hipMemcpyDeviceToHost));
-    if (werr) { set_error(werr == 2 ? "mode decision: reconstructed MV differs from search MV" : "motion-search wavefront timed out"); return RIV_E_GPU; }
+    if (werr) { set_error(werr == 2 ? "mode decision: reconstructed MV differs from search MV" : werr == 3 ? "motion search did not settle within its pass limit" : "motion-search barrier timed out"); return RIV_E_GPU; }
     return RIV_OK;
 }
 

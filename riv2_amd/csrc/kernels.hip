@@ -1,5 +1,5 @@
-// gfx950 kernels of the RIV backend: colour conversion, motion search + mode decision
-// wavefront, motion compensation, adaptive quantisation, residual transforms, deblock.
+// gfx950 kernels of the RIV backend: colour conversion, motion search (relaxation passes) and
+// mode decision, motion compensation, adaptive quantisation, residual transforms, deblock.
 // Each kernel states the reference function it reproduces bit-exactly (paths relative
 // to the reference tree, under reitero_video/).
 #include "riv_dev.hpp"
@@ -7,8 +7,8 @@
 
 using namespace rivdev;
 using riv::BlockRec;
-using riv::WalkParams;
 using riv::ModeParams;
+using riv::SearchParams;
 
 namespace rivdev {
 __constant__ int32_t c_B16[16][16];
@@ -222,22 +222,8 @@ __global__ __launch_bounds__(128) void k_hp_table(const uint8_t* __restrict__ cu
 }
 
 // ============================================================================================
-// Motion search + MV mode decision wavefront
+// Motion search: shared helpers (MV-cost model, lane argmin, lane selection)
 // ============================================================================================
-// One workgroup per band of R MB rows, one wave64 per row. Block (bx, by) depends on the
-// left block and on (bx-1..bx+1, by-1) through the ME seeds / cost predictors
-// (motion/scalar.rs:56-156) and the mode-decision predictors (mv_predictor.rs:59-272), so a
-// row runs two blocks behind the row above. Inside a band, a wave waits on the LDS count of
-// finished blocks of the row above and reads that row's MVs from LDS. Across bands, the last
-// row of a band publishes each block's MV pair as one 64-bit word tagged with the frame
-// (relaxed agent-scope atomics, so no cache write-back or invalidate is needed: the word is
-// its own flag); the first row of the next band polls the top-right word it needs, having
-// loaded it speculatively one block early. Every wait is bounded: a timeout sets p.err, which
-// ends the waits of every other workgroup.
-//
-// All SADs come from the integer and half-pel tables built beforehand; each wave copies the
-// next block's tables from HBM into LDS while it walks the current one. The search itself is
-// then a chain of table lookups and MV-cost evaluations, candidate sets evaluated one per lane.
 namespace {
 
 __device__ __forceinline__ int median3(int a, int b, int c) {   // motion/scalar.rs:24-43
@@ -295,285 +281,319 @@ __device__ __forceinline__ int mv_class(int mag) {   // mv_class_from_magnitude 
     return min(32 - __clz(mag), 10);
 }
 
-// Published MV pair of a band's last row: 18-bit (dx, dy, sx, sy) words of the search and the
-// reconstructed MV, plus the frame tag in bits 36..63 so that a reader can tell this frame's
-// value from the previous frame's without any fence.
-__device__ __forceinline__ uint32_t mv18(uint32_t w) { return (w & 0xffffu) | (((w >> 16) & 1u) << 16) | (((w >> 24) & 1u) << 17); }
-__device__ __forceinline__ uint32_t mv32(uint32_t v) { return (v & 0xffffu) | (((v >> 16) & 1u) << 16) | (((v >> 17) & 1u) << 24); }
+}  // namespace
 
-// Copies one block's table region (nb bytes, a multiple of 16) from HBM straight into LDS at dst:
-// each lane moves 16 bytes per 1 KB step. Completion is tracked by the wave's vector-memory
-// counter; the consumer waits for it before reading dst.
-__device__ __forceinline__ void fetch_region(const uint8_t* src, uint8_t* dst, int nb, int lane) {
-    const uint8_t* s8 = src + 16 * lane;
-    int o = 0;
-    for (; o + 1024 <= nb; o += 1024) __builtin_amdgcn_global_load_lds((const void*)(s8 + o), (void*)(dst + o), 16, 0, 0);
-    if (o < nb && 16 * lane < nb - o) __builtin_amdgcn_global_load_lds((const void*)(s8 + o), (void*)(dst + o), 16, 0, 0);
+// ============================================================================================
+// Motion search by relaxation passes
+// ============================================================================================
+// A block's search result is a function of constant data and of the integer (dx, dy) of its
+// left, top-left, top and top-right neighbours' results (ME seeds and MV-cost predictors,
+// motion/scalar.rs:56-156). Every one of those neighbours comes earlier in raster order, so the
+// dependencies form a DAG and the reference's raster-order result is the unique fixed point of
+// x = F(x). Passes of F, each computed from the previous pass's field, reach it exactly from any
+// start field: a block on anti-diagonal t = bx + 2*by is final after pass t + 1, so 2*(bh-1) + bw
+// passes always suffice, and a pass that changes no integer MV proves that the fixed point has
+// been reached. A block is re-evaluated only when one of its neighbours changed in the previous
+// pass. On camera-like frames the field settles in 10-20 passes, against the 2*(bh-1) + bw
+// dependent steps of a wavefront.
+//
+// One persistent launch, every workgroup resident at once (the host sizes the grid to one
+// workgroup per CU). Wave w owns blocks w + k * nwaves, lane k holding block k's current word,
+// so that a pass's few changed blocks land on different waves. Passes end at a grid barrier on a
+// per-pass arrival counter that also counts the workgroups whose blocks changed. The field words
+// and the counters are read and written with relaxed agent-scope atomics (coherent across the
+// chip without cache maintenance); each wave waits for its own stores before arriving.
+//
+// An evaluation stages the block's replicate-padded reference window in LDS as four byte-shifted
+// copies (so every 4-pixel read is one aligned LDS read) and scores each candidate set with
+// direct SADs: every lane holds 4 pixels of the block, the 64 partial sums of two candidates are
+// reduced at once as 16-bit halves of one word.
+namespace {
+
+struct JWin {
+    const uint32_t* cp;   // 4 copies of NR x ND dwords, copy s holding bytes 4i+s .. 4i+s+3 at dword i
+    int CS, ND;
+    int W, H, wxa, wy0;
+    uint32_t curd;        // this lane's 4 current pixels: row lane>>2, columns 4*(lane&3) .. +3
+    int lx, ly, li;       // its 4x4 sub-block's x origin and y origin, and its row inside the sub-block
+    int hcx, hry;         // window column and row of its first pixel
+};
+
+__device__ __forceinline__ uint32_t jw_read(const JWin& w, int row, int col) { return w.cp[(col & 3) * w.CS + row * w.ND + (col >> 2)]; }
+
+// Partial SAD at integer MV (dx, dy): each 4x4 sub-block's reference origin is clamped to
+// [0,W-4] x [0,H-4] as a whole (sad_block_int_limit, satd_scalar.rs:4-99).
+__device__ __forceinline__ uint32_t jw_int(const JWin& w, int dx, int dy) {
+    const int ox = dclamp(w.lx + dx, 0, w.W - 4) - w.wxa;
+    const int oy = dclamp(w.ly + dy, 0, w.H - 4) + w.li - w.wy0;
+    return __builtin_amdgcn_sad_u8(w.curd, jw_read(w, oy, ox), 0u);
 }
 
-// An MB's tables into an LDS slot: the half-pel table always, the integer table only at an edge.
-__device__ __forceinline__ void fetch_tables(const WalkParams& p, int bx, int by, uint8_t* slot, int lane) {
-    const uint8_t* src = p.tabs + (size_t)(by * p.bw + bx) * p.tab_bytes;
-    if (mb_interior(bx * 16, by * 16, p.W, p.H, p.sr)) fetch_region(src + p.tsb, slot + p.tsb, p.tab_bytes - p.tsb, lane);
-    else fetch_region(src, slot, p.tab_bytes, lane);
+// Partial SAD at half-pel MV (hx, hy): floor interpolation with every sample position clamped to
+// the frame (sample_luma_halfpel, satd_scalar.rs:113-166), which is plain interpolation on the
+// replicate-padded window. (hx, hy) are wave-uniform, so the branches are scalar.
+__device__ __forceinline__ uint32_t jw_hp(const JWin& w, int hx, int hy) {
+    const int col = w.hcx + (hx >> 1), row = w.hry + (hy >> 1);
+    const uint32_t a0 = jw_read(w, row, col);
+    uint32_t v;
+    if (hx & 1) {
+        const uint32_t a1 = jw_read(w, row, col + 1);
+        if (hy & 1) v = avg4f(a0, a1, jw_read(w, row + 1, col), jw_read(w, row + 1, col + 1));
+        else v = avg2f(a0, a1);
+    } else {
+        v = (hy & 1) ? avg2f(a0, jw_read(w, row + 1, col)) : a0;
+    }
+    return __builtin_amdgcn_sad_u8(w.curd, v, 0u);
 }
 
-// A wave-uniform 64-bit value moved to scalar registers (readfirstlane returns a signed int:
-// each half is widened as unsigned).
-__device__ __forceinline__ unsigned long long uniform64(unsigned long long x) {
-    return ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32)) << 32) |
-           (unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)x);
+// SADs of the candidates held by lanes 0..N-1 (cx, cy); lane k receives candidate k's SAD.
+// Two candidates' partial sums (each at most 64 * 1020 < 2^16 in total) share one reduction.
+template <int N, bool HP>
+__device__ __forceinline__ int jw_set(const JWin& w, int lane, int cx, int cy) {
+    uint32_t part[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        const int x = __builtin_amdgcn_readlane(cx, k), y = __builtin_amdgcn_readlane(cy, k);
+        part[k] = HP ? jw_hp(w, x, y) : jw_int(w, x, y);
+    }
+    uint32_t tot[N];
+#pragma unroll
+    for (int k = 0; k < N; k += 2) {
+        if (k + 1 < N) {
+            const uint32_t s = wave_sum((part[k] << 16) | part[k + 1]);
+            tot[k] = s >> 16; tot[k + 1] = s & 0xffffu;
+        } else {
+            tot[k] = wave_sum(part[k]);
+        }
+    }
+    uint32_t r = 0;
+#pragma unroll
+    for (int k = 0; k < N; k++) r = lane == k ? tot[k] : r;
+    return (int)r;
 }
 
-// Spins until the published word at a carries this frame's tag; false after the spin bound or
-// once another workgroup has reported a timeout.
-__device__ __forceinline__ bool wait_pub(const unsigned long long* a, unsigned tag, const int* err, unsigned long long& v) {
-    for (unsigned spins = 0;; spins++) {
-        unsigned long long x = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        v = uniform64(x);
-        if ((unsigned)(v >> 36) == tag) return true;
-        if (spins > (1u << 21)) return false;
-        if ((spins & 63) == 63 && __builtin_amdgcn_readfirstlane(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) return false;
-        __builtin_amdgcn_s_sleep(1);
+// The window of MB (x0, y0): rows y0-sr-1 .. y0+16+sr and columns from the dword-aligned
+// wxa <= x0-sr-1, replicate-padded outside the frame (W is a multiple of 16, so an aligned
+// dword is wholly inside or wholly outside a row).
+__device__ __forceinline__ void jw_load(const SearchParams& p, uint32_t* cp, int CS, int ND, int NR, int x0, int y0, int lane) {
+    const int wxa = (x0 - p.sr - 1) & ~3, wy0 = y0 - p.sr - 1;
+    const int rstep = 64 / ND, jr = lane / ND, ji = lane - jr * ND;
+    if (jr >= rstep) return;
+    for (int r = jr; r < NR; r += rstep) {
+        const uint8_t* row = p.ref + (size_t)dclamp(wy0 + r, 0, p.H - 1) * p.W;
+        const int g0 = wxa + 4 * ji, g1 = g0 + 4;
+        const uint32_t d0 = (g0 >= 0 && g0 < p.W) ? *(const uint32_t*)(row + g0) : 0x01010101u * (g0 < 0 ? row[0] : row[p.W - 1]);
+        const uint32_t d1 = (g1 >= 0 && g1 < p.W) ? *(const uint32_t*)(row + g1) : 0x01010101u * (g1 < 0 ? row[0] : row[p.W - 1]);
+        uint32_t* q = cp + r * ND + ji;
+        q[0] = d0;
+        q[CS] = __builtin_amdgcn_alignbyte(d1, d0, 1);
+        q[2 * CS] = __builtin_amdgcn_alignbyte(d1, d0, 2);
+        q[3 * CS] = __builtin_amdgcn_alignbyte(d1, d0, 3);
     }
 }
+
+// One block's search: process_block_hex (motion/scalar.rs:546-636) with the seeds and cost
+// predictors of gather_local_motion_candidates (:56-156) taken from the neighbour words lS, tS,
+// trS, tlS and the temporal word pT. Returns the canonical search MV word; *sad_out = its
+// half-pel SAD.
+__device__ uint32_t jsearch(const SearchParams& p, const double* bits, uint32_t* cp, int CS, int ND, int NR, int bx, int by,
+                            uint32_t lS, uint32_t tS, uint32_t trS, uint32_t tlS, uint32_t pT, int lane, int* sad_out) {
+    const int sr = p.sr, x0 = bx * 16, y0 = by * 16;
+    jw_load(p, cp, CS, ND, NR, x0, y0, lane);
+    JWin w;
+    w.cp = cp; w.CS = CS; w.ND = ND; w.W = p.W; w.H = p.H;
+    w.wxa = (x0 - sr - 1) & ~3; w.wy0 = y0 - sr - 1;
+    w.curd = *(const uint32_t*)(p.cur + (size_t)(y0 + (lane >> 2)) * p.W + x0 + 4 * (lane & 3));
+    w.lx = x0 + 4 * (lane & 3); w.ly = y0 + 4 * (lane >> 4); w.li = (lane >> 2) & 3;
+    w.hcx = w.lx - w.wxa; w.hry = y0 + (lane >> 2) - w.wy0;
+
+    const bool has_l = bx > 0, has_t = by > 0, has_tr = has_t && bx + 1 < p.bw, has_tl = has_t && bx > 0, has_tp = p.prev_mv != nullptr;
+    // predictor i in lane i+1, in predictor order: left, top, top-right, top-left, temporal;
+    // (dx, dy) packed in 16 bits. Lane 0 holds the (0,0) candidate.
+    const uint32_t q0 = lS & 0xffffu, q1 = tS & 0xffffu, q2 = trS & 0xffffu, q3 = tlS & 0xffffu, q4 = pT & 0xffffu;
+    const uint32_t presm = (has_l ? 2u : 0u) | (has_t ? 4u : 0u) | (has_tr ? 8u : 0u) | (has_tl ? 16u : 0u) | (has_tp ? 32u : 0u);
+    const uint32_t cand[6] = {0u, q0, q1, q2, q3, q4};
+    const uint32_t myq = lane_pick(lane, cand);
+    // a predictor is a duplicate when an earlier present predictor has the same (dx, dy)
+    const bool dup = ((lane > 1) & ((presm >> 1) & 1u) & (myq == q0)) | ((lane > 2) & ((presm >> 2) & 1u) & (myq == q1)) |
+                     ((lane > 3) & ((presm >> 3) & 1u) & (myq == q2)) | ((lane > 4) & ((presm >> 4) & 1u) & (myq == q3));
+    const bool first = (lane < 6) & ((presm >> lane) & 1u) & !dup;
+    CostCtx cc;
+    {
+        // nearest / near of the cost model: the first two distinct present predictors
+        const unsigned long long um = __ballot(first), um2 = um & (um - 1);
+        const uint32_t n0 = um ? sgpr((uint32_t)__builtin_amdgcn_readlane((int)myq, __builtin_ctzll(um))) : 0u;
+        const uint32_t n1 = um2 ? sgpr((uint32_t)__builtin_amdgcn_readlane((int)myq, __builtin_ctzll(um2))) : 0u;
+        cc.nx = mvx(n0); cc.ny = mvy(n0); cc.rx = mvx(n1); cc.ry = mvy(n1);
+        const uint32_t qm = has_tr ? q2 : (has_tl ? q3 : 0u);
+        cc.mx = median3(has_l ? mvx(q0) : 0, has_t ? mvx(q1) : 0, mvx(qm));
+        cc.my = median3(has_l ? mvy(q0) : 0, has_t ? mvy(q1) : 0, mvy(qm));
+    }
+    cc.bits = bits; cc.lam2 = p.lam2; cc.c_zero = p.c_zero; cc.c_pred = p.c_pred; cc.lam_zero = p.lam_zero;
+    int sdx = 0, sdy = 0;
+    int scost;   // SAD + MV cost of the current integer best
+    bool skip_int;
+    {
+        // push_candidate (:46-53) + evaluate_seed_candidates (:638-678): (0,0), then each in-range
+        // non-zero predictor not already listed. The sequential "first strictly better" scan is an
+        // argmin with ties going to the lower lane, i.e. the earlier candidate.
+        const int qx = mvx(myq), qy = mvy(myq);
+        const bool myv = lane == 0 || (first & (myq != 0u) & (abs(qx) <= sr) & (abs(qy) <= sr));
+        const int dx = myv ? qx : 0, dy = myv ? qy : 0;
+        const int sv = jw_set<6, false>(w, lane, dx, dy);
+        const int key = myv ? ((sv + cc.cost(dx, dy)) << 3) | lane : 0x7fffffff;
+        skip_int = p.zthr > 0 && __builtin_amdgcn_readlane(sv, 0) <= p.zthr;
+        if (skip_int) {
+            scost = __builtin_amdgcn_readlane(key, 0) >> 3;
+        } else {
+            const int m = lane_min8(key), win = m & 7;
+            scost = m >> 3;
+            const uint32_t wq = (uint32_t)__builtin_amdgcn_readlane((int)myq, win);
+            sdx = mvx(wq); sdy = mvy(wq);
+            if (p.pthr > 0 && __builtin_amdgcn_readlane(sv, win) <= p.pthr) skip_int = true;
+        }
+    }
+    if (!skip_int) {   // refine_integer_motion (:680-757): large hexagon until no improvement, then small diamond
+        const int hox = lane < 6 ? (int)((0x310134u >> (4 * lane)) & 15u) - 2 : 0;
+        const int hoy = lane < 6 ? (int)((0x002442u >> (4 * lane)) & 15u) - 2 : 0;
+        for (int it = 0; it < 4096; it++) {
+            const int dx = sdx + hox, dy = sdy + hoy;
+            const bool v = lane < 6 && abs(dx) <= sr && abs(dy) <= sr;
+            const int ex = v ? dx : sdx, ey = v ? dy : sdy;
+            const int sv = jw_set<6, false>(w, lane, ex, ey);
+            const int key = v ? ((sv + cc.cost(dx, dy)) << 3) | lane : 0x7fffffff;
+            const int m = lane_min8(key);
+            if ((m >> 3) >= scost) break;
+            scost = m >> 3;
+            sdx = __builtin_amdgcn_readlane(dx, m & 7); sdy = __builtin_amdgcn_readlane(dy, m & 7);
+        }
+        const int dox = lane < 4 ? (int)((0x2213u >> (4 * lane)) & 15u) - 2 : 0;
+        const int doy = lane < 4 ? (int)((0x1322u >> (4 * lane)) & 15u) - 2 : 0;
+        const int dx = sdx + dox, dy = sdy + doy;
+        const bool v = lane < 4 && abs(dx) <= sr && abs(dy) <= sr;
+        const int ex = v ? dx : sdx, ey = v ? dy : sdy;
+        const int sv = jw_set<4, false>(w, lane, ex, ey);
+        const int key = v ? ((sv + cc.cost(dx, dy)) << 3) | lane : 0x7fffffff;
+        const int m = lane_min8(key);
+        if ((m >> 3) < scost) { sdx = __builtin_amdgcn_readlane(dx, m & 7); sdy = __builtin_amdgcn_readlane(dy, m & 7); }
+    }
+    // refine_halfpel (:759-839): the base re-scored with the half-pel kernel, then the 8
+    // neighbours in (sy outer, sx inner) order at penalty c_pred (every neighbour of an even base
+    // is fractional). Lane 0 holds the base, lanes 1..8 the neighbours.
+    int best_hx, best_hy, best_sad;
+    {
+        const int sel = lane == 0 ? 4 : (lane <= 4 ? lane - 1 : lane);   // 3x3 raster index
+        const bool v = lane < 9;
+        const int hx = 2 * sdx + (v ? sel % 3 - 1 : 0), hy = 2 * sdy + (v ? sel / 3 - 1 : 0);
+        const int sv = jw_set<9, true>(w, lane, hx, hy);
+        const int key = v ? ((sv + (lane == 0 ? 0 : p.c_pred)) << 4) | lane : 0x7fffffff;
+        const int m = lane_min16(key), k = m & 15;
+        best_sad = __builtin_amdgcn_readlane(sv, k);
+        best_hx = __builtin_amdgcn_readlane(hx, k); best_hy = __builtin_amdgcn_readlane(hy, k);
+    }
+    // canonicalised search MV (motion/scalar.rs:619-634, motion_vector.rs:103-116)
+    int hx = dclamp(best_hx, -255, 255), hy = dclamp(best_hy, -255, 255);
+    int dx = dclamp(hx / 2, -128, 127), dy = dclamp(hy / 2, -128, 127), fx = hx % 2, fy = hy % 2;
+    if (dx == -128 && fx == -1) fx = 0;
+    if (dy == -128 && fy == -1) fy = 0;
+    if (fx == -1) { fx = 1; dx = max(-128, dx - 1); }
+    if (fy == -1) { fy = 1; dy = max(-128, dy - 1); }
+    *sad_out = best_sad;
+    return sgpr((uint32_t)(uint8_t)dx | ((uint32_t)(uint8_t)dy << 8) | ((uint32_t)fx << 16) | ((uint32_t)fy << 24));
+}
+
+__device__ __forceinline__ uint32_t field_load(const uint32_t* a) { return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ void field_store(uint32_t* a, uint32_t v) { __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
 }  // namespace
 
-// Table fetcher: the band's extra wave. For each row it copies the next MB's tables into the
-// row's LDS slot as soon as the row has finished the MB that last used that slot, waits for the
-// copies, then raises the row's ready count. The row waves thus issue no memory traffic of their
-// own for the tables and never wait on the vector-memory counter.
-__device__ void walk_fetcher(const WalkParams& p, int R, int NS, int RB, uint8_t* slots, const int* lprog, int* ready, const int* abortf, int lane) {
-    const int row0 = blockIdx.x * R, bw = p.bw;
-    int nf[4] = {0, 0, 0, 0};   // next MB to fetch, per row
-    unsigned spins = 0;
-    for (;;) {
-        unsigned issued = 0, pending = 0;
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            if (r < R && row0 + r < p.bh && nf[r] < bw) {
-                pending |= 1u << r;
-                if (nf[r] - NS < __hip_atomic_load(&lprog[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
-                    fetch_tables(p, nf[r], row0 + r, slots + (r * NS + (NS == 2 ? (nf[r] & 1) : 0)) * RB, lane);
-                    issued |= 1u << r;
-                }
-            }
-        }
-        if (!pending) return;
-        if (issued) {
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-                if ((issued >> r) & 1u) { nf[r]++; if (lane == 0) __hip_atomic_store(&ready[r], nf[r], __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP); }
-            spins = 0;
-        } else {
-            if (__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) || ++spins > (1u << 24)) return;
-            __builtin_amdgcn_s_sleep(1);
-        }
-    }
-}
-
-__global__ __launch_bounds__(320) void k_me_walk(WalkParams p) {
-    extern __shared__ __align__(16) uint8_t lds[];
+__global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
+    extern __shared__ __align__(16) uint32_t lds32[];
     const int lane = threadIdx.x & 63;
-    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));   // wave index, kept scalar
-    const int R = p.R, row = blockIdx.x * R + w, bw = p.bw, sr = p.sr;
-    const int RB = p.tab_bytes, TSB = p.tsb, NS = p.slots, D2 = 4 * sr + 3, HR = 2 * sr + 1;
-    uint8_t* slots = lds;                                      // per row: NS table slots
-    uint32_t* rowbuf = (uint32_t*)(lds + R * NS * RB);         // per row and block: search MV word, SAD
-    uint32_t* ptl = rowbuf + R * bw * 2;                       // per row and block: temporal MV word
-    double* bits = (double*)(ptl + R * bw);
-    int* abortf = (int*)(bits + 256);
-    int* lprog = abortf + 1;   // per row: number of finished MBs
-    int* ready = lprog + 4;    // per row: number of MBs whose tables are in LDS
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int NW = gridDim.x * (blockDim.x >> 6), gw = blockIdx.x * (blockDim.x >> 6) + wv;
+    const int bw = p.bw, nb = bw * p.bh, sr = p.sr;
+    const int NR = 2 * sr + 18, ND = ((2 * sr + 20) >> 2) + 1, CS = NR * ND;
+    double* bits = (double*)lds32;           // 256 doubles
+    int* wflag = (int*)(bits + 256);         // [0] a wave of this workgroup changed a block, [1] pass outcome
+    uint32_t* cp = (uint32_t*)(wflag + 4) + (size_t)wv * 4 * CS;
     for (int i = threadIdx.x; i < 256; i += blockDim.x) bits[i] = p.bits[i];
-    const bool has_tp = p.prev_mv != nullptr;
-    for (int i = threadIdx.x; i < R * bw; i += blockDim.x) {
-        const int r = i / bw;
-        ptl[i] = (has_tp && blockIdx.x * R + r < p.bh) ? *(const uint32_t*)(p.prev_mv + (size_t)p.prev_stride * ((blockIdx.x * R + r) * bw + i % bw)) : 0u;
-    }
-    if (threadIdx.x == 0) *abortf = 0;
-    if (threadIdx.x < 8) lprog[threadIdx.x] = 0;   // lprog and ready
+    if (blockIdx.x == 0)   // the other bank was the previous launch's; it is next launch's, so clear it
+        for (int i = threadIdx.x; i < p.maxp; i += blockDim.x) p.arrive[(size_t)(p.bank ^ 1) * p.maxp + i] = 0u;
+    unsigned* arrive = p.arrive + (size_t)p.bank * p.maxp;
+    if (threadIdx.x == 0) wflag[0] = 0;
     __syncthreads();
-    if (w == R) { walk_fetcher(p, R, NS, RB, slots, lprog, ready, abortf, lane); return; }
-    const bool row_valid = row < p.bh;
-    if (!row_valid) return;
-    uint8_t* slot0 = slots + w * NS * RB;
 
-    const bool has_t = row > 0, from_hbm = has_t && w == 0, publish = w == R - 1 && row + 1 < p.bh;
-    const unsigned long long* above = p.pub + (size_t)(has_t ? row - 1 : 0) * bw;
-    bool dead = false;
-    // search MVs are carried packed: bytes (dx, dy, sx, sy) of one 32-bit word. Top neighbours
-    // slide along the row: top-left <- top <- top-right.
-    uint32_t lS = 0, tlS = 0, tS = 0, trS = 0;
-    unsigned long long spec = 0;   // speculative load of the next top-right published word
-    if (from_hbm) {
-        unsigned long long v;
-        dead = !wait_pub(above, p.tag, p.err, v);
-        tS = mv32((uint32_t)v & 0x3ffffu);
-        if (bw > 1) spec = __hip_atomic_load(above + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    for (int bx = 0; bx < bw && !dead; bx++) {
-        const int idx = row * bw + bx;
-        const bool has_l = bx > 0, has_tr = has_t && bx + 1 < bw, has_tl = has_t && bx > 0;
-        // ---- top neighbours ----
-        if (from_hbm) {
-            if (bx > 0) { tlS = tS; tS = trS; }
-            trS = 0;
-            if (has_tr) {
-                unsigned long long v = uniform64(spec);
-                if ((unsigned)(v >> 36) != p.tag && !wait_pub(above + bx + 1, p.tag, p.err, v)) { dead = true; break; }
-                trS = mv32((uint32_t)v & 0x3ffffu);
-                if (bx + 2 < bw) spec = __hip_atomic_load(above + bx + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // lane k owns block gw + k*NW (the host keeps nb <= 64*NW)
+    const int myb = gw + lane * NW;
+    const bool valid = myb < nb;
+    const int mbx = valid ? myb % bw : 0, mby = valid ? myb / bw : 0;
+    const uint32_t mypT = (valid && p.prev_mv) ? *(const uint32_t*)(p.prev_mv + (size_t)p.prev_stride * myb) : 0u;
+    uint32_t myw = 0u;   // start field: every MV (0,0)
+    int mysad = 0;
+    int outcome = 0;     // 1 converged, -1 barrier timeout, 0 pass limit
+    int pass = 0;
+    for (; pass < p.maxp; pass++) {
+        const uint32_t* src = p.field + (size_t)(pass & 1) * nb;
+        uint32_t* dst = p.field + (size_t)((pass + 1) & 1) * nb;
+        uint32_t nL = 0u, nT = 0u, nTR = 0u, nTL = 0u;
+        bool dirty = valid;
+        if (pass > 0 && valid) {
+            if (mbx > 0) nL = field_load(src + myb - 1);
+            if (mby > 0) {
+                nT = field_load(src + myb - bw);
+                if (mbx + 1 < bw) nTR = field_load(src + myb - bw + 1);
+                if (mbx > 0) nTL = field_load(src + myb - bw - 1);
             }
-        } else if (has_t) {
-            const int need = min(bx + 2, bw);
-            unsigned spins = 0;
-            while (__hip_atomic_load(&lprog[w - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < need) {
-                if (__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) || ++spins > (1u << 24)) { dead = true; break; }
+            dirty = ((nL | nT | nTR | nTL) >> 31) != 0u;
+        }
+        // blocks whose inputs did not change keep their word (flag cleared) in this pass's field
+        if (valid && !dirty) field_store(dst + myb, myw);
+        bool chg = false;
+        for (unsigned long long dm = __ballot(dirty); dm; dm &= dm - 1) {
+            const int k = __builtin_ctzll(dm);
+            const int b = __builtin_amdgcn_readlane(myb, k);
+            const int bx = __builtin_amdgcn_readlane(mbx, k), by = __builtin_amdgcn_readlane(mby, k);
+            const uint32_t lS = sgpr((uint32_t)__builtin_amdgcn_readlane((int)nL, k)) & 0x7fffffffu;
+            const uint32_t tS = sgpr((uint32_t)__builtin_amdgcn_readlane((int)nT, k)) & 0x7fffffffu;
+            const uint32_t trS = sgpr((uint32_t)__builtin_amdgcn_readlane((int)nTR, k)) & 0x7fffffffu;
+            const uint32_t tlS = sgpr((uint32_t)__builtin_amdgcn_readlane((int)nTL, k)) & 0x7fffffffu;
+            const uint32_t pT = sgpr((uint32_t)__builtin_amdgcn_readlane((int)mypT, k));
+            int sad;
+            const uint32_t wS = jsearch(p, bits, cp, CS, ND, NR, bx, by, lS, tS, trS, tlS, pT, lane, &sad);
+            const uint32_t old = sgpr((uint32_t)__builtin_amdgcn_readlane((int)myw, k));
+            const bool c = ((wS ^ old) & 0xffffu) != 0u;
+            chg |= c;
+            if (lane == k) { myw = wS; mysad = sad; }
+            if (lane == 0) field_store(dst + b, wS | (c ? 0x80000000u : 0u));
+        }
+        // ---- grid barrier: arrival count in bits 0-15, workgroups with a change in bits 16-31 ----
+        if (chg && lane == 0) __hip_atomic_store(&wflag[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // this wave's field stores are performed
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned add = 1u + (wflag[0] ? 0x10000u : 0u);
+            wflag[0] = 0;
+            __hip_atomic_fetch_add(arrive + pass, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int res = -1;
+            for (unsigned spins = 0; spins < (1u << 20); spins++) {
+                const unsigned v = __hip_atomic_load(arrive + pass, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((v & 0xffffu) >= gridDim.x) { res = (int)(v >> 16); break; }
                 __builtin_amdgcn_s_sleep(1);
             }
-            if (dead) break;
-            // the row above's MVs are wave-uniform: keep them in scalar registers
-            const uint32_t* q = rowbuf + ((w - 1) * bw + bx) * 2;
-            tlS = bx > 0 ? sgpr(q[-2]) : 0u;
-            tS = sgpr(q[0]);
-            trS = bx + 1 < bw ? sgpr(q[2]) : 0u;
+            wflag[1] = res;
         }
-        // this MB's tables are in LDS once the fetcher has counted it
-        {
-            unsigned spins = 0;
-            while (__hip_atomic_load(&ready[w], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) <= bx) {
-                if (__hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) || ++spins > (1u << 24)) { dead = true; break; }
-                __builtin_amdgcn_s_sleep(1);
-            }
-            if (dead) break;
-        }
-        const uint32_t pT = sgpr(ptl[w * bw + bx]);
-        const uint8_t* cslot = slot0 + (NS == 2 ? (bx & 1) * RB : 0);
-        const uint16_t* hpt = (const uint16_t*)(cslot + TSB);
-        // integer SADs: the MB's own table at an edge, else the half-pel table's even entries;
-        // entry (dx, dy) is itab[dy * irs + dx * ics]
-        const bool edge = !mb_interior(bx * 16, row * 16, p.W, p.H, sr);
-        const uint16_t* itab = edge ? (const uint16_t*)cslot + sr * (2 * sr + 1) + sr : hpt + HR * D2 + HR;
-        const int irs = edge ? 2 * sr + 1 : 2 * D2, ics = edge ? 1 : 2;
-
-        // ---- ME predictors: gather_local_motion_candidates (motion/scalar.rs:56-156) ----
-        // predictor i in lane i+1, in predictor order: left, top, top-right, top-left, temporal;
-        // (dx, dy) packed in 16 bits. Lane 0 holds the (0,0) candidate.
-        const uint32_t q0 = lS & 0xffffu, q1 = tS & 0xffffu, q2 = trS & 0xffffu, q3 = tlS & 0xffffu, q4 = pT & 0xffffu;
-        const uint32_t presm = (has_l ? 2u : 0u) | (has_t ? 4u : 0u) | (has_tr ? 8u : 0u) | (has_tl ? 16u : 0u) | (has_tp ? 32u : 0u);
-        const uint32_t cand[6] = {0u, q0, q1, q2, q3, q4};
-        const uint32_t myq = lane_pick(lane, cand);
-        // a predictor is a duplicate when an earlier present predictor has the same (dx, dy)
-        const bool dup = ((lane > 1) & ((presm >> 1) & 1u) & (myq == q0)) | ((lane > 2) & ((presm >> 2) & 1u) & (myq == q1)) |
-                         ((lane > 3) & ((presm >> 3) & 1u) & (myq == q2)) | ((lane > 4) & ((presm >> 4) & 1u) & (myq == q3));
-        const bool first = (lane < 6) & ((presm >> lane) & 1u) & !dup;
-        CostCtx cc;
-        {
-            // nearest / near of the cost model: the first two distinct present predictors
-            const unsigned long long um = __ballot(first), um2 = um & (um - 1);
-            const uint32_t n0 = um ? sgpr((uint32_t)__builtin_amdgcn_readlane((int)myq, __builtin_ctzll(um))) : 0u;
-            const uint32_t n1 = um2 ? sgpr((uint32_t)__builtin_amdgcn_readlane((int)myq, __builtin_ctzll(um2))) : 0u;
-            cc.nx = mvx(n0); cc.ny = mvy(n0); cc.rx = mvx(n1); cc.ry = mvy(n1);
-            const uint32_t qm = has_tr ? q2 : (has_tl ? q3 : 0u);
-            cc.mx = median3(has_l ? mvx(q0) : 0, has_t ? mvx(q1) : 0, mvx(qm));
-            cc.my = median3(has_l ? mvy(q0) : 0, has_t ? mvy(q1) : 0, mvy(qm));
-        }
-        cc.bits = bits; cc.lam2 = p.lam2; cc.c_zero = p.c_zero; cc.c_pred = p.c_pred; cc.lam_zero = p.lam_zero;
-        int sdx = 0, sdy = 0;
-        int scost;   // SAD + MV cost of the current integer best
-        bool skip_int;
-        {
-            // push_candidate (motion/scalar.rs:46-53) + evaluate_seed_candidates (:638-678): (0,0),
-            // then each in-range non-zero predictor not already listed (a predictor equal to an
-            // earlier excluded one is excluded by the same test, so "first occurrence" suffices).
-            // The sequential "first strictly better" scan is an argmin with ties going to the
-            // lower lane, i.e. the earlier candidate.
-            const int qx = mvx(myq), qy = mvy(myq);
-            const bool myv = lane == 0 || (first & (myq != 0u) & (abs(qx) <= sr) & (abs(qy) <= sr));
-            const int dx = myv ? qx : 0, dy = myv ? qy : 0;
-            const int sv = itab[dy * irs + dx * ics];
-            const int key = myv ? ((sv + cc.cost(dx, dy)) << 3) | lane : 0x7fffffff;
-            skip_int = p.zthr > 0 && __builtin_amdgcn_readlane(sv, 0) <= p.zthr;   // process_block_hex (:546-636)
-            if (skip_int) {
-                scost = __builtin_amdgcn_readlane(key, 0) >> 3;
-            } else {
-                const int m = lane_min8(key), win = m & 7;
-                scost = m >> 3;
-                const uint32_t wq = (uint32_t)__builtin_amdgcn_readlane((int)myq, win);
-                sdx = mvx(wq); sdy = mvy(wq);
-                // predictor threshold on the winner's SAD (cost - its MV cost)
-                if (p.pthr > 0 && __builtin_amdgcn_readlane(sv, win) <= p.pthr) skip_int = true;
-            }
-        }
-        if (!skip_int) {   // refine_integer_motion (:680-757): large hexagon until no improvement, then small diamond
-            const int hox = lane < 6 ? (int)((0x310134u >> (4 * lane)) & 15u) - 2 : 0;
-            const int hoy = lane < 6 ? (int)((0x002442u >> (4 * lane)) & 15u) - 2 : 0;
-            for (int it = 0; it < 4096; it++) {
-                const int dx = sdx + hox, dy = sdy + hoy;
-                const bool v = lane < 6 && abs(dx) <= sr && abs(dy) <= sr;
-                const int sv = itab[(v ? dy : sdy) * irs + (v ? dx : sdx) * ics];
-                const int key = v ? ((sv + cc.cost(dx, dy)) << 3) | lane : 0x7fffffff;
-                const int m = lane_min8(key);
-                if ((m >> 3) >= scost) break;
-                scost = m >> 3;
-                sdx = __builtin_amdgcn_readlane(dx, m & 7); sdy = __builtin_amdgcn_readlane(dy, m & 7);
-            }
-            const int dox = lane < 4 ? (int)((0x2213u >> (4 * lane)) & 15u) - 2 : 0;
-            const int doy = lane < 4 ? (int)((0x1322u >> (4 * lane)) & 15u) - 2 : 0;
-            const int dx = sdx + dox, dy = sdy + doy;
-            const bool v = lane < 4 && abs(dx) <= sr && abs(dy) <= sr;
-            const int sv = itab[(v ? dy : sdy) * irs + (v ? dx : sdx) * ics];
-            const int key = v ? ((sv + cc.cost(dx, dy)) << 3) | lane : 0x7fffffff;
-            const int m = lane_min8(key);
-            if ((m >> 3) < scost) { sdx = __builtin_amdgcn_readlane(dx, m & 7); sdy = __builtin_amdgcn_readlane(dy, m & 7); }
-        }
-        // ---- refine_halfpel (:759-839): the base re-scored with the half-pel kernel, then the 8
-        //      neighbours in (sy outer, sx inner) order at penalty c_pred (every neighbour of an
-        //      even base is fractional). Lane 0 holds the base, lanes 1..8 the neighbours.
-        int best_hx, best_hy, best_sad;
-        {
-            const int sel = lane == 0 ? 4 : (lane <= 4 ? lane - 1 : lane);   // 3x3 raster index
-            const bool v = lane < 9;
-            const int hx = 2 * sdx + (v ? sel % 3 - 1 : 0), hy = 2 * sdy + (v ? sel / 3 - 1 : 0);
-            const int sv = hpt[(hy + HR) * D2 + hx + HR];
-            const int key = v ? ((sv + (lane == 0 ? 0 : p.c_pred)) << 4) | lane : 0x7fffffff;
-            const int m = lane_min16(key), k = m & 15;
-            best_sad = __builtin_amdgcn_readlane(sv, k);
-            best_hx = __builtin_amdgcn_readlane(hx, k); best_hy = __builtin_amdgcn_readlane(hy, k);
-        }
-        // canonicalised search MV (motion/scalar.rs:619-634, motion_vector.rs:103-116)
-        int S0, S1, S2, S3;
-        {
-            int hx = dclamp(best_hx, -255, 255), hy = dclamp(best_hy, -255, 255);
-            int dx = dclamp(hx / 2, -128, 127), dy = dclamp(hy / 2, -128, 127), fx = hx % 2, fy = hy % 2;
-            if (dx == -128 && fx == -1) fx = 0;
-            if (dy == -128 && fy == -1) fy = 0;
-            if (fx == -1) { fx = 1; dx = max(-128, dx - 1); }
-            if (fy == -1) { fy = 1; dy = max(-128, dy - 1); }
-            S0 = dx; S1 = dy; S2 = fx; S3 = fy;
-        }
-
-        const uint32_t wS = sgpr((uint32_t)(uint8_t)S0 | ((uint32_t)(uint8_t)S1 << 8) | ((uint32_t)S2 << 16) | ((uint32_t)S3 << 24));
-        if (lane == 0) {
-            if (publish)
-                __hip_atomic_store(&p.pub[idx], (unsigned long long)mv18(wS) | ((unsigned long long)p.tag << 36), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            *(uint2*)(rowbuf + (w * bw + bx) * 2) = make_uint2(wS, (uint32_t)best_sad);
-            __hip_atomic_store(&lprog[w], bx + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
-        lS = wS;
+        __syncthreads();
+        const int res = wflag[1];
+        if (res < 0) { outcome = -1; break; }
+        if (res == 0) { outcome = 1; break; }
     }
-    if (dead) {
-        if (lane == 0) {
-            __hip_atomic_store(abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            __hip_atomic_store(p.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+    if (outcome != 1) {
+        if (threadIdx.x == 0) __hip_atomic_store(p.err, outcome < 0 ? 1 : 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
-    // the row's search results, LDS -> HBM
-    for (int i = lane; i < bw; i += 64) p.smv[row * bw + i] = *(const unsigned long long*)(rowbuf + (w * bw + i) * 2);
+    if (valid) p.smv[myb] = (unsigned long long)myw | ((unsigned long long)(uint32_t)mysad << 32);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && p.passes) *p.passes = pass + 1;
 }
 
 // ============================================================================================
@@ -1219,9 +1239,6 @@ int sad_table_stride(int sr) { return ((2 * sr + 1) * (2 * sr + 1) + 7) & ~7; }
 int hp_table_stride(int sr) { return ((4 * sr + 3) * (4 * sr + 3) + 7) & ~7; }
 int table_region_bytes(int sr) { return 2 * (sad_table_stride(sr) + hp_table_stride(sr)); }
 bool mb_is_interior(int x0, int y0, int W, int H, int sr) { return mb_interior(x0, y0, W, H, sr); }
-size_t walk_lds_bytes(int sr, int bw, int R, int slots) {
-    return (size_t)R * slots * table_region_bytes(sr) + (size_t)R * bw * 12 + 256 * sizeof(double) + 64;
-}
 void sad_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int H, int sr, uint16_t* out, int stride) {
     const int ndwt = (16 + 2 * sr + 6) / 4 + 1;
     const size_t lds = (size_t)(64 + 4 * (16 + 2 * sr) * ndwt) * 4;
@@ -1233,10 +1250,12 @@ void hp_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int
     const size_t lds = (size_t)(64 + (WS + HPK + 1) * (ndp + 1) + 4 * (WS + HPK) * ndp) * 4;
     hipLaunchKernelGGL(k_hp_table<HPK>, dim3((W / 16) * (H / 16)), dim3(128), lds, st, cur, ref, W, H, W / 16, sr, ndp, out, stride);
 }
-hipError_t me_walk(hipStream_t st, const WalkParams& p) {
-    int nbands = (p.bh + p.R - 1) / p.R;
-    size_t lds = walk_lds_bytes(p.sr, p.bw, p.R, p.slots);
-    hipLaunchKernelGGL(k_me_walk, dim3(nbands), dim3(64 * (p.R + 1)), lds, st, p);   // R row waves + the table fetcher
+size_t search_lds_bytes(int sr, int waves) {
+    const int NR = 2 * sr + 18, ND = ((2 * sr + 20) >> 2) + 1;
+    return 256 * sizeof(double) + 16 + (size_t)waves * 4 * NR * ND * sizeof(uint32_t);
+}
+hipError_t me_search(hipStream_t st, const SearchParams& p, int groups, int waves) {
+    hipLaunchKernelGGL(k_me_search, dim3(groups), dim3(64 * waves), search_lds_bytes(p.sr, waves), st, p);
     return hipGetLastError();
 }
 void mode(hipStream_t st, const ModeParams& p) {

@@ -7,8 +7,8 @@ namespace riv {
 // MV prediction modes (reitero_residual/src/mv_predictor.rs:5-33, as_u8 order)
 enum Mode : uint8_t { M_ZERO = 0, M_NEAREST = 1, M_NEAR = 2, M_TOPRIGHT = 3, M_TOPLEFT = 4, M_NEW = 5, M_TEMPORAL = 6 };
 
-// One record per 16x16 macroblock, written by the motion-search / mode-decision wavefront.
-// The first 8 bytes are what the next MB row (and the next frame) read.
+// One record per 16x16 macroblock, written by the mode decision after the motion search.
+// Bytes 4..7 (the reconstructed MV) seed the next frame's temporal candidates.
 struct BlockRec {
     int8_t s[4];        // search MV (dx, dy, sx, sy), canonical (motion/scalar.rs:619-634)
     int8_t r[4];        // reconstructed MV as the decoder rebuilds it (encoder.rs:667-718)
@@ -27,25 +27,26 @@ static_assert(sizeof(BlockRec) == 16, "BlockRec layout");
 // bytes apart; only the used part of a slot is written.
 constexpr int PACK_STRIDE = 48 + 768;
 
-// Parameters of the motion-search wavefront for one frame.
-struct WalkParams {
+// Parameters of the motion search by relaxation passes (one persistent launch per frame).
+struct SearchParams {
+    const uint8_t* cur;          // current luma plane, W x H
+    const uint8_t* ref;          // reference luma plane (previous reconstruction), W x H
     int W, H, bw, bh, sr;
     const int8_t* prev_mv;       // previous P-frame's reconstructed MVs (dx,dy,sx,sy) or nullptr
     int prev_stride;             // bytes between consecutive MVs of prev_mv
-    const uint8_t* tabs;         // per block: integer SAD table (tsb bytes), then half-pel SAD table
-    int tab_bytes;               // bytes of one block's table region, a multiple of 16
-    int tsb;                     // bytes of the integer table, a multiple of 16
-    int slots;                   // LDS table slots per wave: 2 (next block prefetched) or 1
+    uint32_t* field;             // 2 * bw*bh MV words: the field after even / odd passes; bytes
+                                 // (dx, dy, sx, sy), bit 31 set when the pass changed (dx, dy)
     unsigned long long* smv;     // per block out: search MV word | best half-pel SAD << 32
-    unsigned long long* pub;     // bw*bh published search MVs (bits 0-17) with the frame tag (bits 36-63)
-    unsigned tag;                // this frame's tag, 1 .. 2^28-1; differs from the previous launch's
+    unsigned* arrive;            // 2 banks of maxp pass-arrival counters
+    int bank;                    // bank of this launch (zero at launch); the launch clears the other
+    int maxp;                    // pass limit: 2*(bh-1) + bw + 2 passes always reach the fixed point
     const double* bits;          // 256 entries: 1.0 for 0, else 2*log2(a)+2 (motion/scalar.rs:495-525)
     double lam2;                 // lambda * 2.0
     int c_zero, c_pred;          // (lam2*0.5) as i64, (lam2*1.5) as i64; c_pred is also the half-pel penalty
     int lam_zero;                // lambda == 0.0 -> every MV cost is 0
     long long zthr, pthr;        // me_zero_mv_threshold, me_predictor_threshold
-    int R;                       // MB rows per workgroup, at most 4
-    int* err;                    // set non-zero on a wavefront timeout
+    int* err;                    // 1 on a barrier timeout, 3 if the pass limit is hit
+    int* passes;                 // out: passes the launch ran
 };
 
 // Parameters of the per-MB mode decision that follows the search.
