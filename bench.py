@@ -133,7 +133,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-pframes", type=int, default=8)
-    ap.add_argument("--pipeline-depth", type=int, default=16,
+    ap.add_argument("--pipeline-depth", type=int, default=32,
                     help="frames in flight (GPU analysis of frame k+1 overlaps host entropy coding of earlier frames)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",

@@ -33,7 +33,17 @@ struct Slot {
     int n_marks = 0;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     uint64_t index = 0, ts = 0;
-    bool intra = false, busy = false;
+    bool intra = false;
+    bool busy = false;       // holds a frame that is not yet written
+    // host-stage products of the frame, passed from its coding job to the ordered tail
+    bool coded = false;      // the coding job finished (products below are valid unless failed)
+    bool failed = false;
+    riv_frame_stats st{};
+    std::vector<uint8_t> res;
+    std::vector<rivh::MvBlock> blocks;
+    int bdx = 0, bdy = 0;
+    double smark[RIV_NUM_STAGES] = {};
+    uint32_t nmark[RIV_NUM_STAGES] = {};
 };
 
 class Pool {
@@ -104,6 +114,14 @@ struct riv_encoder {
     std::condition_variable cv;
     uint64_t next_tail = 0;        // next frame index allowed into the ordered tail
     uint64_t completed = 0;        // frames fully written
+    bool tail_active = false;      // a thread is running ready tails in frame order
+    // GPU completion: one thread waits for frames in submission order and hands each finished
+    // frame to the pool, so no pool worker ever blocks on the GPU
+    std::thread waiter;
+    std::mutex wmu;
+    std::condition_variable wcv;
+    std::deque<int> waitq;
+    bool wstop = false;
     int err_code = 0;
     std::string err_msg;
     rivh::MvCoder mvc;
@@ -134,6 +152,8 @@ struct riv_encoder {
         cv.notify_all();
     }
 };
+
+static void waiter_loop(riv_encoder* e);
 
 extern "C" {
 
@@ -203,12 +223,15 @@ static int encoder_init(riv_encoder* e) {
         RIV_HIP(s.rec.alloc(sizeof(BlockRec) * e->nb)); RIV_HIP(s.optskip.alloc(e->nb));
         RIV_HIP(s.h_rec.alloc(sizeof(BlockRec) * e->nb)); RIV_HIP(s.h_pack.alloc((size_t)riv::PACK_STRIDE * e->nb));
         RIV_HIP(s.h_optskip.alloc(e->nb)); RIV_HIP(s.h_passes.alloc(16));
-        RIV_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+        RIV_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming | hipEventBlockingSync));   // waiting workers sleep instead of spinning
         RIV_HIP(hipEventCreate(&s.t0)); RIV_HIP(hipEventCreate(&s.t1));
     }
     if (e->depth > 0) {
         e->pool.reset(new Pool());
-        e->pool->start(std::min(e->depth, 16));
+        // workers: up to 16, leaving two cores to the submitting thread and the completion thread
+        const int cores = (int)std::max(3u, std::thread::hardware_concurrency());
+        e->pool->start(std::max(1, std::min({e->depth, 16, cores - 2})));
+        e->waiter = std::thread(waiter_loop, e);
     }
     // MV-cost bit table (motion/scalar.rs:495-525) and lambda-derived costs
     rivhost::RdoModel rdo(c.inter_quality, c.rdo_lambda_mult, c.skip_threshold);
@@ -267,6 +290,11 @@ static void destroy_encoder(riv_encoder* e) {
     if (e->pool) {   // let in-flight frames finish before the buffers go away
         std::unique_lock<std::mutex> lk(e->mu);
         e->cv.wait(lk, [&] { return e->completed == e->frame_count || e->err_code; });
+    }
+    if (e->waiter.joinable()) {
+        { std::lock_guard<std::mutex> lk(e->wmu); e->wstop = true; }
+        e->wcv.notify_all();
+        e->waiter.join();
     }
     e->pool.reset();
     if (e->st) (void)hipStreamSynchronize(e->st);
@@ -327,42 +355,40 @@ static void compute_bias(const BlockRec* rec, int nb, int& bx_out, int& by_out) 
     }
 }
 
-// Host stage of one frame: residual rANS (any order), then the ordered tail: MV rANS (contexts
-// carry across frames), packing (format.rs:171-205) and writing.
-static void entropy_job(riv_encoder* e, int si) {
+// Host stage of one frame, part 1 (any order, any thread): residual rANS. The frame's GPU work
+// has finished. The products stay in the slot for the ordered tail.
+static void run_tails(riv_encoder* e, int si);
+static void code_frame(riv_encoder* e, int si) {
     Slot& s = e->slots[si];
-    const riv_encoder_config& c = e->cfg;
     const int nb = e->nb;
-    if (hipEventSynchronize(s.done) != hipSuccess) { e->fail(RIV_E_GPU, "GPU frame failed"); return; }
     auto t0 = std::chrono::steady_clock::now();
-    riv_frame_stats st{};
+    riv_frame_stats& st = s.st;
+    st = riv_frame_stats{};
     st.frame_index = s.index; st.timestamp_ms = s.ts;
     st.storage_width = (uint32_t)e->W; st.storage_height = (uint32_t)e->H; st.blocks_total = (uint32_t)nb;
     float gpu_ms = 0;
     if (hipEventElapsedTime(&gpu_ms, s.t0, s.t1) == hipSuccess) st.gpu_ms = gpu_ms;
-    double smark[RIV_NUM_STAGES] = {};
-    uint32_t nmark[RIV_NUM_STAGES] = {};
+    for (int k = 0; k < RIV_NUM_STAGES; k++) { s.smark[k] = 0; s.nmark[k] = 0; }
     for (int k = 1; k < s.n_marks; k++) {
         float ms = 0;
-        if (s.mark_stage[k] >= 0 && hipEventElapsedTime(&ms, s.marks[k - 1], s.marks[k]) == hipSuccess) { smark[s.mark_stage[k]] += ms; nmark[s.mark_stage[k]]++; }
+        if (s.mark_stage[k] >= 0 && hipEventElapsedTime(&ms, s.marks[k - 1], s.marks[k]) == hipSuccess) { s.smark[s.mark_stage[k]] += ms; s.nmark[s.mark_stage[k]]++; }
     }
     const uint8_t* pack = s.h_pack.as<uint8_t>();
-    std::vector<uint8_t> res, bytes;
-    std::vector<rivh::MvBlock> blocks;
-    int bdx = 0, bdy = 0;
+    s.res.clear(); s.blocks.clear(); s.bdx = s.bdy = 0;
     try {
         if (s.intra) {
             std::vector<uint8_t> coded(nb, 1);
-            res = rivh::encode_residuals_packed(pack, riv::PACK_STRIDE, coded.data(), nb, true);
+            s.res = rivh::encode_residuals_packed(pack, riv::PACK_STRIDE, coded.data(), nb, true);
             st.frame_type = 1;
         } else {
             const BlockRec* hr = s.h_rec.as<BlockRec>();
             const uint8_t* os = s.h_optskip.as<uint8_t>();
+            int bdx = 0, bdy = 0;
             compute_bias(hr, nb, bdx, bdy);
-            blocks.resize(nb);
+            s.blocks.resize(nb);
             std::vector<uint8_t> coded(nb);
             for (int i = 0; i < nb; i++) {
-                rivh::MvBlock& b = blocks[i];
+                rivh::MvBlock& b = s.blocks[i];
                 b.mode = hr[i].mode; b.new_base = hr[i].new_base;
                 b.dx = b.dy = 0; b.sx = b.sy = 0;
                 if (b.mode == riv::M_NEW) {   // delta after the frame bias (encoder.rs:775-818)
@@ -378,58 +404,108 @@ static void entropy_job(riv_encoder* e, int si) {
                 st.rdo_blocks_selected_skip += hr[i].rdo_skip;
                 if (hr[i].s[2] || hr[i].s[3]) st.rdo_forced_fractional++;
             }
-            res = rivh::encode_residuals_packed(pack, riv::PACK_STRIDE, coded.data(), nb, false);
+            s.res = rivh::encode_residuals_packed(pack, riv::PACK_STRIDE, coded.data(), nb, false);
+            s.bdx = bdx; s.bdy = bdy;
             st.frame_type = 2;
             st.mv_bias_dx = bdx; st.mv_bias_dy = bdy;
             st.rdo_lambda = e->lambda; st.rdo_blocks_evaluated = (uint32_t)nb;
+            if (e->cfg.skip_threshold == 0) st.rdo_forced_disabled = (uint32_t)nb;
             st.me_passes = (uint32_t)*s.h_passes.as<int>();
-            if (c.skip_threshold == 0) st.rdo_forced_disabled = (uint32_t)nb;
         }
+        s.failed = false;
     } catch (const std::exception& ex) {
+        s.failed = true;
         e->fail(RIV_E_ENCODING_FAILED, std::string("entropy coding failed: ") + ex.what());
-        return;
     }
     st.host_entropy_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    // ordered tail
-    std::unique_lock<std::mutex> lk(e->mu);
-    e->cv.wait(lk, [&] { return e->next_tail == s.index || e->err_code; });
+    run_tails(e, si);
+}
+
+// Host stage, part 2 (frame order, one thread at a time): MV rANS (contexts carry across frames),
+// packing (format.rs:171-205) and writing. Returns a status.
+static int tail_frame(riv_encoder* e, Slot& s) {
+    const riv_encoder_config& c = e->cfg;
     const auto t_tail = std::chrono::steady_clock::now();
-    if (e->err_code) { e->next_tail++; e->cv.notify_all(); return; }
+    std::vector<uint8_t> bytes;
     try {
         if (s.intra) {
             e->mvc.reset();   // MV contexts restart at every I-frame (encoder.rs:335-337)
             put_u64(bytes, s.ts); bytes.push_back(1); bytes.push_back(c.intra_quality);
-            put_u32(bytes, (uint32_t)res.size());
-            bytes.insert(bytes.end(), res.begin(), res.end());
-            st.residual_bytes = (uint32_t)res.size();
+            put_u32(bytes, (uint32_t)s.res.size());
+            bytes.insert(bytes.end(), s.res.begin(), s.res.end());
+            s.st.residual_bytes = (uint32_t)s.res.size();
         } else {
-            std::vector<uint8_t> mvb = e->mvc.encode(blocks, e->bw, e->bh);
+            std::vector<uint8_t> mvb = e->mvc.encode(s.blocks, e->bw, e->bh);
+            bytes.reserve(20 + mvb.size() + s.res.size());
             put_u64(bytes, s.ts); bytes.push_back(2); bytes.push_back(c.inter_quality);
-            bytes.push_back((uint8_t)(int8_t)bdx); bytes.push_back((uint8_t)(int8_t)bdy); bytes.push_back(0);
+            bytes.push_back((uint8_t)(int8_t)s.bdx); bytes.push_back((uint8_t)(int8_t)s.bdy); bytes.push_back(0);
             put_u32(bytes, (uint32_t)mvb.size()); bytes.insert(bytes.end(), mvb.begin(), mvb.end());
-            put_u32(bytes, (uint32_t)res.size()); bytes.insert(bytes.end(), res.begin(), res.end());
-            st.mv_bytes = (uint32_t)mvb.size(); st.residual_bytes = (uint32_t)res.size();
+            put_u32(bytes, (uint32_t)s.res.size()); bytes.insert(bytes.end(), s.res.begin(), s.res.end());
+            s.st.mv_bytes = (uint32_t)mvb.size(); s.st.residual_bytes = (uint32_t)s.res.size();
         }
     } catch (const std::exception& ex) {
-        if (!e->err_code) { e->err_code = RIV_E_ENCODING_FAILED; e->err_msg = std::string("MV coding failed: ") + ex.what(); }
-        e->next_tail++;
-        e->cv.notify_all();
-        return;
+        e->fail(RIV_E_ENCODING_FAILED, std::string("MV coding failed: ") + ex.what());
+        return RIV_E_ENCODING_FAILED;
     }
-    int r = e->write(bytes.data(), bytes.size());
-    if (r && !e->err_code) { e->err_code = r; e->err_msg = "writer failed"; }
-    st.total_bytes = (uint32_t)bytes.size();
-    st.host_tail_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_tail).count();
-    if (e->profiling)
-        for (int k = 0; k < RIV_NUM_STAGES; k++) { e->stage_ms[k] += smark[k]; e->stage_n[k] += nmark[k]; }
-    e->done_stats.push_back(st);
-    if (e->done_stats.size() > 4096) e->done_stats.pop_front();
-    e->hdr.frame_count = s.index + 1;
-    e->last_slot = si;
-    e->next_tail++;
-    e->completed++;
-    s.busy = false;
-    e->cv.notify_all();
+    const int r = e->write(bytes.data(), bytes.size());
+    if (r) { e->fail(r, "writer failed"); return r; }
+    s.st.total_bytes = (uint32_t)bytes.size();
+    s.st.host_tail_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_tail).count();
+    return RIV_OK;
+}
+
+// Marks slot si coded, then, unless another thread is already doing so, runs every ready tail in
+// frame order. The tails run outside the lock; only the thread that set tail_active runs them.
+static void run_tails(riv_encoder* e, int si) {
+    std::unique_lock<std::mutex> lk(e->mu);
+    e->slots[si].coded = true;
+    if (e->tail_active) return;
+    e->tail_active = true;
+    for (;;) {
+        const int ti = (int)(e->next_tail % e->slots.size());
+        Slot& t = e->slots[ti];
+        if (!t.busy || !t.coded || t.index != e->next_tail) break;
+        t.coded = false;
+        const bool skip = t.failed || e->err_code;
+        lk.unlock();
+        const int r = skip ? RIV_E_ENCODING_FAILED : tail_frame(e, t);
+        lk.lock();
+        if (r == RIV_OK) {
+            if (e->profiling)
+                for (int k = 0; k < RIV_NUM_STAGES; k++) { e->stage_ms[k] += t.smark[k]; e->stage_n[k] += t.nmark[k]; }
+            e->done_stats.push_back(t.st);
+            if (e->done_stats.size() > 4096) e->done_stats.pop_front();
+            e->hdr.frame_count = t.index + 1;
+            e->last_slot = ti;
+        }
+        e->next_tail++;
+        e->completed++;
+        t.busy = false;
+        e->cv.notify_all();
+    }
+    e->tail_active = false;
+}
+
+// The GPU-completion thread of the pipelined mode.
+static void waiter_loop(riv_encoder* e) {
+    (void)hipSetDevice(e->cfg.device_id);
+    for (;;) {
+        int si;
+        {
+            std::unique_lock<std::mutex> lk(e->wmu);
+            e->wcv.wait(lk, [&] { return e->wstop || !e->waitq.empty(); });
+            if (e->waitq.empty()) return;
+            si = e->waitq.front();
+            e->waitq.pop_front();
+        }
+        if (hipEventSynchronize(e->slots[si].done) != hipSuccess) {
+            e->fail(RIV_E_GPU, "GPU frame failed");
+            e->slots[si].failed = true;
+            run_tails(e, si);
+            continue;
+        }
+        e->pool->submit([e, si] { code_frame(e, si); });
+    }
 }
 
 static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t len, uint32_t w, uint32_t h, uint64_t ts, riv_frame_stats* stats) {
@@ -454,8 +530,9 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
         std::unique_lock<std::mutex> lk(e->mu);
         e->cv.wait(lk, [&] { return !s.busy || e->err_code; });
         if (e->err_code) { set_error(e->err_msg); return e->err_code; }
+        s.index = e->frame_count; s.ts = ts; s.intra = intra; s.busy = true; s.coded = false; s.failed = false;
     }
-    s.index = e->frame_count; s.ts = ts; s.intra = intra; s.busy = true; s.n_marks = 0;
+    s.n_marks = 0;
     if (e->profiling && !s.marks[0])
         for (auto& m : s.marks) RIV_HIP(hipEventCreate(&m));
     auto mark = [&](int stage) {
@@ -530,11 +607,13 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
     e->frame_count++;
 
     if (e->depth > 0) {
-        e->pool->submit([e, si] { entropy_job(e, si); });
+        { std::lock_guard<std::mutex> lk(e->wmu); e->waitq.push_back(si); }
+        e->wcv.notify_one();
         if (stats) { std::memset(stats, 0, sizeof(*stats)); stats->frame_index = s.index; stats->timestamp_ms = ts; stats->frame_type = intra ? 1 : 2; }
         return RIV_OK;
     }
-    entropy_job(e, si);
+    if (hipEventSynchronize(s.done) != hipSuccess) { e->fail(RIV_E_GPU, "GPU frame failed"); s.failed = true; run_tails(e, si); }
+    else code_frame(e, si);
     if (!intra) {
         int werr = 0;
         RIV_HIP(hipMemcpy(&werr, e->err.p, 4, hipMemcpyDeviceToHost));

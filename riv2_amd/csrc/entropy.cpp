@@ -4,25 +4,18 @@
 #include <algorithm>
 #include <cstring>
 #include <cstdlib>
+#include <immintrin.h>
 
 namespace rivh {
 
 namespace {
-// P(0) table of the packed counter pair (reitero_video_common/src/rans.rs:30-44):
-// zeros*256/(zeros+ones) truncated to u8; entries that truncate to 0 read as 1.
-struct P0Table {
-    uint8_t t[65536];
-    P0Table() {
-        t[0] = 1;
-        for (uint32_t i = 1; i < 65536; i++) {
-            uint32_t z = i >> 8, o = i & 0xff;
-            uint8_t v = (uint8_t)((z << 8) / (z + o));
-            t[i] = v ? v : 1;
-        }
+struct P0Rcp {
+    uint32_t m[512];
+    constexpr P0Rcp() : m() {
+        for (int d = 1; d < 512; d++) m[d] = (uint32_t)((((uint64_t)1) << 32) / (uint64_t)d + 1);
     }
 };
-const P0Table& p0table() { static P0Table tab; return tab; }
-inline uint16_t swap_bytes(uint16_t v) { return (uint16_t)((v << 8) | (v >> 8)); }
+constexpr P0Rcp kRcpInit{};
 // x / d for x < 2^32 and 1 <= d <= 256 without a divide: with M = floor(2^40 / d) + 1,
 // x*M / 2^40 = x/d + e where 0 < e < x / 2^40 < 1/256 <= 1/d, and frac(x/d) <= 1 - 1/d, so the
 // floor is exact.
@@ -35,50 +28,112 @@ inline uint32_t div_small(uint32_t x, uint32_t d) { return (uint32_t)(((unsigned
 constexpr uint32_t kLow = 1u << 16;   // normalisation interval [2^16, 2^24)
 }  // namespace
 
-uint8_t Ctx::p0() const { return p0table().t[c]; }
+const uint32_t kP0Rcp[512] = {
+#define R8(i) kRcpInit.m[i], kRcpInit.m[i + 1], kRcpInit.m[i + 2], kRcpInit.m[i + 3], kRcpInit.m[i + 4], kRcpInit.m[i + 5], kRcpInit.m[i + 6], kRcpInit.m[i + 7]
+#define R64(i) R8(i), R8(i + 8), R8(i + 16), R8(i + 24), R8(i + 32), R8(i + 40), R8(i + 48), R8(i + 56)
+    R64(0), R64(64), R64(128), R64(192), R64(256), R64(320), R64(384), R64(448)
+#undef R64
+#undef R8
+};
 
-// BinProb::observe (rans.rs:73-84): the counter of the observed symbol sits in the high
-// byte; on overflow both counters are halved (rounding up), the observed one re-based.
-void Ctx::update(bool bit) {
-    uint16_t v = bit ? swap_bytes(c) : c;
-    if (v >= 0xff00) {
-        uint16_t keep = v == 0xff01 ? 0xff00 : 0x8100;
-        v = (uint16_t)((uint16_t)(v + 0x101) >> 1) | keep;
-    } else {
-        v = (uint16_t)(v + 0x100);
+namespace {
+struct WriterBuffers { std::vector<uint16_t> buf, tmp[4]; };
+thread_local std::vector<WriterBuffers> tl_pool;
+}  // namespace
+
+RansWriter::RansWriter(std::vector<uint8_t>& out) : out_(out) {
+    if (!tl_pool.empty()) {
+        WriterBuffers& b = tl_pool.back();
+        buf_.swap(b.buf);
+        for (int j = 0; j < kLanes; j++) tmp_[j].swap(b.tmp[j]);
+        tl_pool.pop_back();
+        sym_ = buf_.data(); cap_ = buf_.size();
     }
-    c = bit ? swap_bytes(v) : v;
+}
+RansWriter::~RansWriter() {
+    if (tl_pool.size() < 4) {
+        tl_pool.emplace_back();
+        WriterBuffers& b = tl_pool.back();
+        b.buf.swap(buf_);
+        for (int j = 0; j < kLanes; j++) b.tmp[j].swap(tmp_[j]);
+    }
 }
 
-RansWriter::RansWriter(std::vector<uint8_t>& out) : out_(out), sym_(kBatch), bits_(kBatch) {}
+void RansWriter::grow() {
+    buf_.resize(std::max<size_t>(cap_ * 2, 1u << 16));
+    sym_ = buf_.data();
+    cap_ = buf_.size();
+}
 
-// Encode the buffered block last-symbol-first with two alternating states; emitted u16
-// words are written in reverse emission order after both final states (state 1 first),
-// which is the byte order of the reference's front-pushed deque (rans.rs:231-269).
-void RansWriter::drain() {
-    uint32_t st[2] = {kLow, kLow};
-    std::vector<uint16_t> emitted;
-    emitted.reserve(n_ / 4 + 8);
-    auto code = [&](uint32_t& x, int k) {
-        uint32_t p = sym_[k] ? sym_[k] : 1u;
-        uint32_t start = bits_[k] ? p : 0u, freq = bits_[k] ? 256u - p : p;
-        if (x >= (freq << 24)) { emitted.push_back((uint16_t)x); x >>= 16; }
+// Codes blocks last-symbol-first with two alternating states; a block's emitted u16 words go
+// out in reverse emission order after both final states (state 1 first), which is the byte order
+// of the reference's front-pushed deque (rans.rs:231-269). Renormalisation is branch-free: the
+// low word is always stored below the write pointer, which moves only when the word is emitted.
+// Blocks are independent, so `nblk` of them (at most kLanes) are coded in one interleaved loop to
+// give the CPU independent state chains to overlap; block j holds n[j] symbols at sym[j].
+namespace {
+struct BlockCoder {
+    const uint16_t* sym;
+    uint16_t* wp;       // next emitted word goes to wp[-1]
+    uint32_t A, B;      // A codes the higher symbol of each pair, B the lower; they end as r0, r1
+    int k;              // next symbol to code
+    void start(const uint16_t* s, int n, uint16_t* end) {
+        sym = s; wp = end; A = kLow; B = kLow; k = n - 1;
+        if (n & 1) { code(B, k); k--; }   // odd block: first symbol alone, then the roles swap
+    }
+    inline void code(uint32_t& x, int i) {
+        const uint32_t s = sym[i], freq = s & 0xffu, start = s >> 8;
+        const uint32_t e = x >= (freq << 24);
+        wp[-1] = (uint16_t)x;
+        wp -= e;
+        x = e ? x >> 16 : x;
         const uint32_t q = div_small(x, freq);
-        x = (q << 8) + (x - q * freq) + start;
-    };
-    int k = n_ - 1;
-    int cur = 0;
-    if (n_ & 1) { code(st[0], k--); cur = 1; }   // odd block: first coded symbol alone, then roles swap
-    while (k >= 1) { code(st[cur], k); code(st[cur ^ 1], k - 1); k -= 2; }
-    // after an odd start the states are swapped: st[cur] plays r0, st[cur^1] plays r1
-    uint32_t r0 = st[cur], r1 = st[cur ^ 1];
-    auto put16 = [&](uint16_t w) { out_.push_back((uint8_t)w); out_.push_back((uint8_t)(w >> 8)); };
-    put16((uint16_t)r1); put16((uint16_t)(r1 >> 16));
-    put16((uint16_t)r0); put16((uint16_t)(r0 >> 16));
-    for (size_t i = emitted.size(); i-- > 0;) put16(emitted[i]);
+        x += q * (256u - freq) + start;
+    }
+    inline void pair() { code(A, k); code(B, k - 1); k -= 2; }
+};
+}  // namespace
+
+void RansWriter::drain(const uint16_t* const* sym, const int* n, int nblk) {
+    BlockCoder bc[kLanes];
+    for (int j = 0; j < nblk; j++) {
+        if (tmp_[j].size() < (size_t)n[j] + 8) tmp_[j].resize((size_t)n[j] + 8);
+        bc[j].start(sym[j], n[j], tmp_[j].data() + tmp_[j].size());
+    }
+    if (nblk == kLanes) {   // full blocks have equal lengths: step all of them together
+        while (bc[0].k >= 1 && bc[kLanes - 1].k >= 1) {
+#pragma GCC unroll 4
+            for (int j = 0; j < kLanes; j++) bc[j].pair();
+        }
+    }
+    for (int j = 0; j < nblk; j++)
+        while (bc[j].k >= 1) bc[j].pair();
+    for (int j = 0; j < nblk; j++) {
+        const uint32_t r0 = bc[j].A, r1 = bc[j].B;
+        const uint16_t* wp = bc[j].wp;
+        const size_t ne = (size_t)(tmp_[j].data() + tmp_[j].size() - wp), o = out_.size();
+        out_.resize(o + 8 + 2 * ne);
+        uint8_t* d = out_.data() + o;
+        const uint16_t head[4] = {(uint16_t)r1, (uint16_t)(r1 >> 16), (uint16_t)r0, (uint16_t)(r0 >> 16)};
+        for (int i = 0; i < 4; i++) { d[2 * i] = (uint8_t)head[i]; d[2 * i + 1] = (uint8_t)(head[i] >> 8); }
+        for (size_t i = 0; i < ne; i++) { d[8 + 2 * i] = (uint8_t)wp[i]; d[9 + 2 * i] = (uint8_t)(wp[i] >> 8); }
+    }
+}
+
+void RansWriter::flush() {
+    size_t b = 0;
+    do {   // an empty stream still carries its two initial states
+        const uint16_t* s[kLanes];
+        int n[kLanes], nblk = 0;
+        for (; nblk < kLanes && (nblk == 0 || b < n_); nblk++) {
+            s[nblk] = sym_ + b;
+            n[nblk] = (int)std::min<size_t>(kBatch, n_ - b);
+            b += (size_t)n[nblk];
+        }
+        drain(s, n, nblk);
+    } while (b < n_);
     n_ = 0;
 }
-void RansWriter::flush() { drain(); }
 
 RansReader::RansReader(const uint8_t* data, size_t len) : d_(data), len_(len) { a_ = state(); b_ = state(); }
 uint16_t RansReader::word() {
@@ -209,6 +264,99 @@ void unpack_coeffs(const uint8_t* slot, int16_t* q) {
     }
 }
 
+namespace {
+// Significance symbols of positions 0..last of one plane, into out[0..last], with the contexts
+// updated as the scalar coder would leave them: each position has its own context, so 16
+// positions are modelled at once. P(0) = (zeros << 8) / (zeros + ones) in single precision is
+// exact here: the numerator is below 2^16 and the divisor at most 510, so a non-integer quotient
+// lies at least 1/510 from the next integer, far beyond the division's rounding error.
+__attribute__((target("avx2"))) void sig_symbols_avx2(Ctx* sig, const uint64_t* mask, int last, uint16_t* out) {
+    const __m256i bsel = _mm256_setr_epi16(1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384, (short)0x8000);
+    const __m256i iota = _mm256_setr_epi16(0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15);
+    const __m256i swap = _mm256_setr_epi8(1, 0, 3, 2, 5, 4, 7, 6, 9, 8, 11, 10, 13, 12, 15, 14, 1, 0, 3, 2, 5, 4, 7, 6, 9, 8, 11, 10, 13, 12, 15, 14);
+    const __m256i ff = _mm256_set1_epi16(0xff), one = _mm256_set1_epi16(1), c256 = _mm256_set1_epi16(256);
+    const __m256i xff00 = _mm256_set1_epi16((short)0xff00), xff01 = _mm256_set1_epi16((short)0xff01), x8100 = _mm256_set1_epi16((short)0x8100);
+    const __m256i x100 = _mm256_set1_epi16(0x100), x101 = _mm256_set1_epi16(0x101);
+    const __m256i lim = _mm256_set1_epi16((short)(last + 1));
+    for (int g = 0; 16 * g <= last; g++) {
+        const __m256i st = _mm256_loadu_si256((const __m256i*)(sig + 16 * g));
+        const uint32_t b16 = (uint32_t)(mask[g >> 2] >> (16 * (g & 3))) & 0xffffu;
+        const __m256i bit = _mm256_cmpeq_epi16(_mm256_and_si256(_mm256_set1_epi16((short)b16), bsel), bsel);
+        const __m256i act = _mm256_cmpgt_epi16(lim, _mm256_add_epi16(_mm256_set1_epi16((short)(16 * g)), iota));
+        // P(0): (z << 8) / (z + o) truncated to u8, 0 read as 1 (0/0 gives NaN -> 0 -> 1, as the table)
+        const __m256i z = _mm256_srli_epi16(st, 8), d = _mm256_add_epi16(z, _mm256_and_si256(st, ff));
+        const __m256i zl = _mm256_cvtepu16_epi32(_mm256_castsi256_si128(z)), zh = _mm256_cvtepu16_epi32(_mm256_extracti128_si256(z, 1));
+        const __m256i dl = _mm256_cvtepu16_epi32(_mm256_castsi256_si128(d)), dh = _mm256_cvtepu16_epi32(_mm256_extracti128_si256(d, 1));
+        const __m256i ql = _mm256_cvttps_epi32(_mm256_div_ps(_mm256_cvtepi32_ps(_mm256_slli_epi32(zl, 8)), _mm256_cvtepi32_ps(dl)));
+        const __m256i qh = _mm256_cvttps_epi32(_mm256_div_ps(_mm256_cvtepi32_ps(_mm256_slli_epi32(zh, 8)), _mm256_cvtepi32_ps(dh)));
+        const __m256i m8 = _mm256_set1_epi32(0xff);
+        __m256i p = _mm256_permute4x64_epi64(_mm256_packus_epi32(_mm256_and_si256(ql, m8), _mm256_and_si256(qh, m8)), 0xD8);
+        p = _mm256_blendv_epi8(p, one, _mm256_cmpeq_epi16(p, _mm256_setzero_si256()));
+        // symbol: bit 0 -> (start 0, freq p); bit 1 -> (start p, freq 256 - p)
+        const __m256i s1 = _mm256_or_si256(_mm256_slli_epi16(p, 8), _mm256_sub_epi16(c256, p));
+        _mm256_storeu_si256((__m256i*)(out + 16 * g), _mm256_blendv_epi8(p, s1, bit));
+        // BinProb::observe
+        const __m256i v = _mm256_blendv_epi8(st, _mm256_shuffle_epi8(st, swap), bit);
+        const __m256i sat = _mm256_cmpeq_epi16(_mm256_max_epu16(v, xff00), v);
+        const __m256i keep = _mm256_blendv_epi8(x8100, xff00, _mm256_cmpeq_epi16(v, xff01));
+        const __m256i halved = _mm256_or_si256(_mm256_srli_epi16(_mm256_add_epi16(v, x101), 1), keep);
+        const __m256i n = _mm256_blendv_epi8(_mm256_add_epi16(v, x100), halved, sat);
+        const __m256i nn = _mm256_blendv_epi8(n, _mm256_shuffle_epi8(n, swap), bit);
+        _mm256_storeu_si256((__m256i*)(sig + 16 * g), _mm256_blendv_epi8(st, nn, act));
+    }
+}
+void sig_symbols_scalar(Ctx* sig, const uint64_t* mask, int last, uint16_t* out) {
+    for (int i = 0; i <= last; i++) {
+        const bool b = (mask[i >> 6] >> (i & 63)) & 1u;
+        const uint32_t p = sig[i].p0();
+        sig[i].update(b);
+        out[i] = (uint16_t)(b ? ((p << 8) | (256u - p)) : p);
+    }
+}
+const bool g_avx2 = __builtin_cpu_supports("avx2");
+
+inline void emit_bit(uint16_t*& o, bool b, Ctx& c) {
+    const uint32_t p = c.p0();
+    c.update(b);
+    *o++ = (uint16_t)(b ? ((p << 8) | (256u - p)) : p);
+}
+inline void emit_bypass(uint16_t*& o, bool b) { *o++ = (uint16_t)(b ? 0x8080u : 0x80u); }
+inline void emit_class_value(uint16_t*& o, int v, Ctx* ctx, int n) {   // put_class_value, into o
+    const int k = value_class(v);
+    if (k > n - 1) throw CorruptStream("coefficient magnitude outside the class table");
+    for (int i = 0; i < k; i++) emit_bit(o, true, ctx[i]);
+    if (k < n - 1) emit_bit(o, false, ctx[k]);
+    for (int b = k - 2; b >= 0; b--) emit_bypass(o, ((v - (1 << (k - 1))) >> b) & 1);
+}
+
+// One inter plane (no DC prediction) straight from its non-zero masks (nw words) and values.
+// Symbol order of put_plane: EOB class value, then per position 0..last the significance bit,
+// and after a non-zero one its sign (bypass) and magnitude class value.
+void put_plane_packed(uint16_t*& o, ResidualCtx& c, const uint64_t* mask, int nw, const int16_t*& vals, bool luma) {
+    int last = 0;
+    for (int k = nw - 1; k >= 0; k--) if (mask[k]) { last = 64 * k + 63 - __builtin_clzll(mask[k]); break; }
+    emit_class_value(o, last, luma ? c.eob_y : c.eob_c, luma ? 9 : 7);
+    uint16_t sym[256];
+    Ctx* sig = luma ? c.sig_y : c.sig_c;
+    if (g_avx2) sig_symbols_avx2(sig, mask, last, sym);
+    else sig_symbols_scalar(sig, mask, last, sym);
+    int prev = 0;
+    for (int k = 0; k < nw; k++) {
+        for (uint64_t m = mask[k]; m; m &= m - 1) {
+            const int i = 64 * k + __builtin_ctzll(m);
+            std::memcpy(o, sym + prev, sizeof(uint16_t) * (size_t)(i + 1 - prev));
+            o += i + 1 - prev;
+            prev = i + 1;
+            const int v = *vals++;
+            emit_bypass(o, v < 0);
+            emit_class_value(o, std::abs(v) - 1, luma ? c.mag_y[band_y(i)] : c.mag_c[band_c(i)], 13);
+        }
+    }
+    std::memcpy(o, sym + prev, sizeof(uint16_t) * (size_t)(last + 1 - prev));
+    o += last + 1 - prev;
+}
+}  // namespace
+
 std::vector<uint8_t> encode_residuals_packed(const uint8_t* pack, size_t stride, const uint8_t* coded, int nb, bool dc_pred) {
     std::vector<uint8_t> out;
     bool any = false;
@@ -216,13 +364,31 @@ std::vector<uint8_t> encode_residuals_packed(const uint8_t* pack, size_t stride,
     if (!any) return out;
     ResidualCtx ctx;
     RansWriter w(out);
-    int16_t q[384];
+    if (dc_pred) {   // intra: DC prediction changes position 0; rare, coded from unpacked blocks
+        int16_t q[384];
+        for (int b = 0; b < nb; b++) {
+            if (!coded[b]) continue;
+            unpack_coeffs(pack + (size_t)b * stride, q);
+            put_plane(w, ctx, q, 256, 0, dc_pred);
+            put_plane(w, ctx, q + 256, 64, 1, dc_pred);
+            put_plane(w, ctx, q + 320, 64, 2, dc_pred);
+        }
+        w.flush();
+        return out;
+    }
+    // a block yields at most 3 EOB codes of 16 symbols and 384 positions of 27 symbols
+    constexpr size_t kMaxBlockSymbols = 3 * 16 + 384 * 27;
     for (int b = 0; b < nb; b++) {
         if (!coded[b]) continue;
-        unpack_coeffs(pack + (size_t)b * stride, q);
-        put_plane(w, ctx, q, 256, 0, dc_pred);
-        put_plane(w, ctx, q + 256, 64, 1, dc_pred);
-        put_plane(w, ctx, q + 320, 64, 2, dc_pred);
+        const uint8_t* slot = pack + (size_t)b * stride;
+        uint64_t m[6];
+        std::memcpy(m, slot, sizeof(m));
+        const int16_t* vals = (const int16_t*)(slot + 48);
+        uint16_t* o = w.reserve(kMaxBlockSymbols);
+        put_plane_packed(o, ctx, m, 4, vals, true);
+        put_plane_packed(o, ctx, m + 4, 1, vals, false);
+        put_plane_packed(o, ctx, m + 5, 1, vals, false);
+        w.commit(o);
     }
     w.flush();
     return out;

@@ -12,35 +12,66 @@ namespace rivh {
 
 struct CorruptStream : std::runtime_error { using std::runtime_error::runtime_error; };
 
+// Reciprocals for P(0) without a divide: x / d == (x * kP0Rcp[d]) >> 32 for x < 2^16, d <= 510
+// (kP0Rcp[d] = floor(2^32 / d) + 1: the error x * (kP0Rcp[d] - 2^32/d) / 2^32 < 2^-16 stays
+// below the distance 1/d from x/d to the next integer above it). kP0Rcp[0] = 0.
+extern const uint32_t kP0Rcp[512];
+
 // Adaptive binary context: two saturating 8-bit counters packed in 16 bits
 // (reitero_video_common/src/rans.rs:25-85).
 struct Ctx {
     uint16_t c = 0x0101;
-    uint8_t p0() const;      // P(bit == 0) * 256, in [1, 255]
-    void update(bool bit);
+    // P(bit == 0) * 256: zeros*256/(zeros+ones) truncated to u8 (rans.rs:30-44), 0 read as 1
+    uint32_t p0() const {
+        const uint32_t z = c >> 8, d = z + (c & 0xffu);
+        const uint32_t p = (uint32_t)(((uint64_t)(z << 8) * kP0Rcp[d]) >> 32) & 0xffu;
+        return p ? p : 1u;
+    }
+    // BinProb::observe (rans.rs:73-84): the counter of the observed symbol sits in the high byte;
+    // on overflow both counters are halved (rounding up), the observed one re-based.
+    void update(bool bit) {
+        const uint32_t v = bit ? (uint32_t)(((c << 8) | (c >> 8)) & 0xffffu) : c;
+        const uint32_t halved = (((v + 0x101u) & 0xffffu) >> 1) | (v == 0xff01u ? 0xff00u : 0x8100u);
+        const uint32_t n = v >= 0xff00u ? halved : v + 0x100u;
+        c = (uint16_t)(bit ? ((n << 8) | (n >> 8)) : n);
+    }
 };
 
 // Symbol-batched rANS encoder with two interleaved 32-bit states and u16 renormalisation
-// (rans.rs:88-298): symbols are buffered, then coded in reverse in blocks of 16386.
+// (rans.rs:88-298): symbols are coded in reverse in blocks of 16386, every block from fresh
+// states, and the blocks' outputs follow each other in order. So the whole stream's symbols are
+// buffered (as start << 8 | freq) and the blocks are coded at flush().
 class RansWriter {
 public:
     explicit RansWriter(std::vector<uint8_t>& out);
-    void bit(bool b, Ctx& c) { uint8_t p = c.p0(); c.update(b); push(b, p); }
-    void bypass(bool b) { push(b, 128); }
+    ~RansWriter();
+    RansWriter(const RansWriter&) = delete;
+    RansWriter& operator=(const RansWriter&) = delete;
+    void bit(bool b, Ctx& c) {
+        const uint32_t p = c.p0();
+        c.update(b);
+        push(b ? ((p << 8) | (256u - p)) : p);
+    }
+    void bypass(bool b) { push(b ? 0x8080u : 0x80u); }
+    // Direct writes: room for n more symbols at the returned pointer; commit(end) adopts them.
+    uint16_t* reserve(size_t n) { while (cap_ - n_ < n) grow(); return sym_ + n_; }
+    void commit(const uint16_t* end) { n_ = (size_t)(end - sym_); }
     void flush();
 private:
-    void push(bool b, uint8_t p) {
-        if (n_ == kBatch) drain();
-        sym_[n_] = p;
-        bits_[n_] = b;
-        n_++;
+    void push(uint32_t s) {
+        if (n_ == cap_) grow();
+        sym_[n_++] = (uint16_t)s;
     }
-    void drain();
+    void grow();
+    void drain(const uint16_t* const* sym, const int* n, int nblk);
     static constexpr int kBatch = 16386;
+    static constexpr int kLanes = 4;   // blocks coded together
     std::vector<uint8_t>& out_;
-    std::vector<uint8_t> sym_;   // probability of each buffered symbol, in coding order
-    std::vector<uint8_t> bits_;
-    int n_ = 0;
+    // buffers come from a per-thread pool, so a worker coding frame after frame reuses warm memory
+    std::vector<uint16_t> buf_;
+    std::vector<uint16_t> tmp_[kLanes];   // emitted words of the blocks being coded, filled from the end
+    uint16_t* sym_ = nullptr;
+    size_t n_ = 0, cap_ = 0;
 };
 
 class RansReader {
