@@ -192,13 +192,14 @@ static int encoder_init(riv_encoder* e) {
     e->W = (int)c.storage_width; e->H = (int)c.storage_height;
     e->bw = e->W / 16; e->bh = e->H / 16; e->nb = e->bw * e->bh;
     // motion-search geometry: one resident workgroup per CU at most, as many waves per workgroup
-    // as 64 KiB of LDS holds (one reference window each), at least one wave per 64 blocks
+    // (up to 16) as the workgroup's LDS holds (one reference window each), at least one wave per
+    // 64 blocks
     {
-        const size_t budget = 65536;
-        e->s_waves = 16;
-        while (e->s_waves > 1 && rivk::search_lds_bytes(c.search_range, e->s_waves) > budget) e->s_waves--;
         hipDeviceProp_t prop;
         RIV_HIP(hipGetDeviceProperties(&prop, c.device_id));
+        const size_t budget = prop.sharedMemPerBlock;
+        e->s_waves = 16;
+        while (e->s_waves > 1 && rivk::search_lds_bytes(c.search_range, e->s_waves) > budget) e->s_waves--;
         e->s_groups = std::max(1, std::min(prop.multiProcessorCount, (e->nb + e->s_waves - 1) / e->s_waves));
         if ((long long)e->s_groups * e->s_waves * 64 < e->nb) {
             set_error("frame has more macroblocks than the motion search's resident waves can hold");
@@ -214,7 +215,10 @@ static int encoder_init(riv_encoder* e) {
     RIV_HIP(e->var.alloc(8 * (size_t)e->nb)); RIV_HIP(e->lnv.alloc(4 * ((size_t)e->nb + 1))); RIV_HIP(e->steps.alloc(4 * (size_t)e->nb));
     RIV_HIP(e->mask.alloc(e->nb));
     RIV_HIP(e->field.alloc(8 * (size_t)e->nb));
-    RIV_HIP(e->arrive.alloc(8 * (size_t)e->s_maxp)); RIV_HIP(hipMemset(e->arrive.p, 0, 8 * (size_t)e->s_maxp));
+    {
+        const size_t bb = 2 * sizeof(unsigned) * (size_t)riv::barrier_words(e->s_maxp);
+        RIV_HIP(e->arrive.alloc(bb)); RIV_HIP(hipMemset(e->arrive.p, 0, bb));
+    }
     RIV_HIP(e->smv.alloc(8 * (size_t)e->nb));
     RIV_HIP(e->err.alloc(4)); RIV_HIP(hipMemset(e->err.p, 0, 4));
     e->depth = c.pipeline_depth;

@@ -375,20 +375,35 @@ __device__ __forceinline__ int jw_set(const JWin& w, int lane, int cx, int cy) {
 // The window of MB (x0, y0): rows y0-sr-1 .. y0+16+sr and columns from the dword-aligned
 // wxa <= x0-sr-1, replicate-padded outside the frame (W is a multiple of 16, so an aligned
 // dword is wholly inside or wholly outside a row).
+// A lane covers rows jr, jr + rstep, ... of dword column ji; the loads of JU rows are all issued
+// before any of them is stored, so a window costs about one memory latency.
 __device__ __forceinline__ void jw_load(const SearchParams& p, uint32_t* cp, int CS, int ND, int NR, int x0, int y0, int lane) {
+    constexpr int JU = 9;
     const int wxa = (x0 - p.sr - 1) & ~3, wy0 = y0 - p.sr - 1;
     const int rstep = 64 / ND, jr = lane / ND, ji = lane - jr * ND;
     if (jr >= rstep) return;
-    for (int r = jr; r < NR; r += rstep) {
-        const uint8_t* row = p.ref + (size_t)dclamp(wy0 + r, 0, p.H - 1) * p.W;
-        const int g0 = wxa + 4 * ji, g1 = g0 + 4;
-        const uint32_t d0 = (g0 >= 0 && g0 < p.W) ? *(const uint32_t*)(row + g0) : 0x01010101u * (g0 < 0 ? row[0] : row[p.W - 1]);
-        const uint32_t d1 = (g1 >= 0 && g1 < p.W) ? *(const uint32_t*)(row + g1) : 0x01010101u * (g1 < 0 ? row[0] : row[p.W - 1]);
-        uint32_t* q = cp + r * ND + ji;
-        q[0] = d0;
-        q[CS] = __builtin_amdgcn_alignbyte(d1, d0, 1);
-        q[2 * CS] = __builtin_amdgcn_alignbyte(d1, d0, 2);
-        q[3 * CS] = __builtin_amdgcn_alignbyte(d1, d0, 3);
+    const int g0 = wxa + 4 * ji, g1 = g0 + 4;
+    const bool in0 = g0 >= 0 && g0 < p.W, in1 = g1 >= 0 && g1 < p.W;
+    for (int rb = jr; rb < NR; rb += JU * rstep) {
+        uint32_t d0[JU], d1[JU];
+#pragma unroll
+        for (int u = 0; u < JU; u++) {
+            const int r = min(rb + u * rstep, NR - 1);
+            const uint8_t* row = p.ref + (size_t)dclamp(wy0 + r, 0, p.H - 1) * p.W;
+            d0[u] = in0 ? *(const uint32_t*)(row + g0) : 0x01010101u * (g0 < 0 ? row[0] : row[p.W - 1]);
+            d1[u] = in1 ? *(const uint32_t*)(row + g1) : 0x01010101u * (g1 < 0 ? row[0] : row[p.W - 1]);
+        }
+#pragma unroll
+        for (int u = 0; u < JU; u++) {
+            const int r = rb + u * rstep;
+            if (r < NR) {
+                uint32_t* q = cp + r * ND + ji;
+                q[0] = d0[u];
+                q[CS] = __builtin_amdgcn_alignbyte(d1[u], d0[u], 1);
+                q[2 * CS] = __builtin_amdgcn_alignbyte(d1[u], d0[u], 2);
+                q[3 * CS] = __builtin_amdgcn_alignbyte(d1[u], d0[u], 3);
+            }
+        }
     }
 }
 
@@ -518,9 +533,14 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
     int* wflag = (int*)(bits + 256);         // [0] a wave of this workgroup changed a block, [1] pass outcome
     uint32_t* cp = (uint32_t*)(wflag + 4) + (size_t)wv * 4 * CS;
     for (int i = threadIdx.x; i < 256; i += blockDim.x) bits[i] = p.bits[i];
+    const int BW = riv::barrier_words(p.maxp);
     if (blockIdx.x == 0)   // the other bank was the previous launch's; it is next launch's, so clear it
-        for (int i = threadIdx.x; i < p.maxp; i += blockDim.x) p.arrive[(size_t)(p.bank ^ 1) * p.maxp + i] = 0u;
-    unsigned* arrive = p.arrive + (size_t)p.bank * p.maxp;
+        for (int i = threadIdx.x; i < BW; i += blockDim.x) p.arrive[(size_t)(p.bank ^ 1) * BW + i] = 0u;
+    unsigned* arrive = p.arrive + (size_t)p.bank * BW;
+    unsigned* release = arrive + BW - 1;   // pass + 1 once every workgroup has finished the pass; bit 31: a block changed
+    constexpr int L = riv::kBarrierLeaves;
+    const int leaf = blockIdx.x % L;
+    const unsigned nleaf = gridDim.x / L + (leaf < (int)(gridDim.x % L) ? 1u : 0u), nlive = min(gridDim.x, (unsigned)L);
     if (threadIdx.x == 0) wflag[0] = 0;
     __syncthreads();
 
@@ -567,18 +587,28 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
             if (lane == k) { myw = wS; mysad = sad; }
             if (lane == 0) field_store(dst + b, wS | (c ? 0x80000000u : 0u));
         }
-        // ---- grid barrier: arrival count in bits 0-15, workgroups with a change in bits 16-31 ----
+        // ---- pass barrier: a two-level tree of arrival counters (arrivals in bits 0-15, arrivals
+        //      that changed a block in bits 16-31); the last arrival at the root publishes the pass
+        //      on the release word, which every workgroup polls ----
         if (chg && lane == 0) __hip_atomic_store(&wflag[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // this wave's field stores are performed
         __syncthreads();
         if (threadIdx.x == 0) {
-            const unsigned add = 1u + (wflag[0] ? 0x10000u : 0u);
+            const unsigned mine = wflag[0] ? 1u : 0u;
             wflag[0] = 0;
-            __hip_atomic_fetch_add(arrive + pass, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            unsigned* cnt = arrive + (size_t)pass * (L + 1);
+            const unsigned old = __hip_atomic_fetch_add(cnt + leaf, 1u + (mine << 16), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((old & 0xffffu) + 1u == nleaf) {   // last at this leaf: carry the leaf to the root
+                const unsigned lchg = (old >> 16) + mine;
+                const unsigned rold = __hip_atomic_fetch_add(cnt + L, 1u + ((lchg ? 1u : 0u) << 16), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((rold & 0xffffu) + 1u == nlive)   // last overall
+                    __hip_atomic_store(release, (unsigned)(pass + 1) | (((rold >> 16) + (lchg ? 1u : 0u)) ? 0x80000000u : 0u),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
             int res = -1;
             for (unsigned spins = 0; spins < (1u << 20); spins++) {
-                const unsigned v = __hip_atomic_load(arrive + pass, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if ((v & 0xffffu) >= gridDim.x) { res = (int)(v >> 16); break; }
+                const unsigned v = __hip_atomic_load(release, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((v & 0x7fffffffu) == (unsigned)(pass + 1)) { res = (int)(v >> 31); break; }
                 __builtin_amdgcn_s_sleep(1);
             }
             wflag[1] = res;

@@ -1,5 +1,6 @@
 // Structures shared by the host driver and the gfx950 kernels.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <cstdint>
 
 namespace riv {
@@ -27,6 +28,12 @@ static_assert(sizeof(BlockRec) == 16, "BlockRec layout");
 // bytes apart; only the used part of a slot is written.
 constexpr int PACK_STRIDE = 48 + 768;
 
+// Pass barrier of the motion search: per pass, kBarrierLeaves leaf counters (workgroup w arrives
+// at leaf w % kBarrierLeaves) and a root counter; then one release word. A bank holds
+// barrier_words(maxp) counters; the search alternates between two banks.
+constexpr int kBarrierLeaves = 16;
+__host__ __device__ inline int barrier_words(int maxp) { return maxp * (kBarrierLeaves + 1) + 1; }
+
 // Parameters of the motion search by relaxation passes (one persistent launch per frame).
 struct SearchParams {
     const uint8_t* cur;          // current luma plane, W x H
@@ -37,7 +44,7 @@ struct SearchParams {
     uint32_t* field;             // 2 * bw*bh MV words: the field after even / odd passes; bytes
                                  // (dx, dy, sx, sy), bit 31 set when the pass changed (dx, dy)
     unsigned long long* smv;     // per block out: search MV word | best half-pel SAD << 32
-    unsigned* arrive;            // 2 banks of maxp pass-arrival counters
+    unsigned* arrive;            // 2 banks of barrier_words(maxp) pass-barrier counters
     int bank;                    // bank of this launch (zero at launch); the launch clears the other
     int maxp;                    // pass limit: 2*(bh-1) + bw + 2 passes always reach the fixed point
     const double* bits;          // 256 entries: 1.0 for 0, else 2*log2(a)+2 (motion/scalar.rs:495-525)
