@@ -230,10 +230,18 @@ __device__ __forceinline__ int median3(int a, int b, int c) {   // motion/scalar
     return max(min(a, b), min(max(a, b), c));
 }
 
+// The NEW-vector term of estimate_mv_cost for component distances (a, b) to a predictor:
+// (lam2 * ((6 + bits[a]) + bits[b])) as i64, evaluated in f64 in the reference's order
+// (motion/scalar.rs:453-544). A search tabulates it once per launch for a, b < 2*sr + 4, which
+// covers every candidate (|d| <= sr + 2) against every predictor (|d| <= sr + 1).
+__device__ __forceinline__ int new_mv_cost(double lam2, const double* bits, int a, int b) {
+    return (int)__dmul_rn(lam2, __dadd_rn(__dadd_rn(6.0, bits[a]), bits[b]));
+}
+
 struct CostCtx {
     int nx, ny, rx, ry, mx, my;
-    const double* bits;
-    double lam2;
+    const int* tab;   // tab[a * td + b] = new_mv_cost(lam2, bits, a, b)
+    int td;
     int c_zero, c_pred, lam_zero;
     // estimate_mv_cost (motion/scalar.rs:453-544): the min over the applicable terms. The host
     // keeps lam2 * 42 below 2^26, so every term truncates exactly in 32 bits and SAD + cost
@@ -242,8 +250,8 @@ struct CostCtx {
         const bool pm = ((dx == nx) & (dy == ny)) | ((dx == rx) & (dy == ry)) | ((dx == mx) & (dy == my));
         int best = pm ? c_pred : 0x3ffffff;
         best = ((dx == 0) & (dy == 0)) ? min(best, c_zero) : best;
-        const int t0 = (int)__dmul_rn(lam2, __dadd_rn(__dadd_rn(6.0, bits[abs(dx - nx)]), bits[abs(dy - ny)]));
-        const int t1 = (int)__dmul_rn(lam2, __dadd_rn(__dadd_rn(6.0, bits[abs(dx - rx)]), bits[abs(dy - ry)]));
+        const int t0 = tab[__mul24(abs(dx - nx), td) + abs(dy - ny)];
+        const int t1 = tab[__mul24(abs(dx - rx), td) + abs(dy - ry)];
         return lam_zero ? 0 : min(best, min(t0, t1));
     }
 };
@@ -310,66 +318,117 @@ __device__ __forceinline__ int mv_class(int mag) {   // mv_class_from_magnitude 
 // reduced at once as 16-bit halves of one word.
 namespace {
 
+// Candidate SADs are computed one candidate per 16-lane row: lane j of a row holds row j of the
+// block (16 pixels, 4 dwords) and sums its 16 absolute differences; a 4-step DPP reduction
+// inside the row gives the candidate's SAD in all 16 lanes. A wave thus scores 4 candidates per
+// round with no cross-row traffic.
 struct JWin {
     const uint32_t* cp;   // 4 copies of NR x ND dwords, copy s holding bytes 4i+s .. 4i+s+3 at dword i
     int CS, ND;
-    int W, H, wxa, wy0;
-    uint32_t curd;        // this lane's 4 current pixels: row lane>>2, columns 4*(lane&3) .. +3
-    int lx, ly, li;       // its 4x4 sub-block's x origin and y origin, and its row inside the sub-block
-    int hcx, hry;         // window column and row of its first pixel
+    int W, H, x0, y0, wxa, wy0;
+    uint4 cq;             // row j = lane & 15 of the current block
+    int j;
+    // Block row j, dword k at offset (d, e) reads the window at column cb + d + 4k and row
+    // j + sr + 1 + e, i.e. dword ((cb+d) & 3) * CS + ((cb+d) >> 2) + k + e * ND + rowc.
+    int cb;               // x0 - wxa (>= sr + 1)
+    int rowc;             // (j + sr + 1) * ND
 };
 
-__device__ __forceinline__ uint32_t jw_read(const JWin& w, int row, int col) { return w.cp[(col & 3) * w.CS + row * w.ND + (col >> 2)]; }
-
-// Partial SAD at integer MV (dx, dy): each 4x4 sub-block's reference origin is clamped to
-// [0,W-4] x [0,H-4] as a whole (sad_block_int_limit, satd_scalar.rs:4-99).
-__device__ __forceinline__ uint32_t jw_int(const JWin& w, int dx, int dy) {
-    const int ox = dclamp(w.lx + dx, 0, w.W - 4) - w.wxa;
-    const int oy = dclamp(w.ly + dy, 0, w.H - 4) + w.li - w.wy0;
-    return __builtin_amdgcn_sad_u8(w.curd, jw_read(w, oy, ox), 0u);
+__device__ __forceinline__ uint32_t row_sum(uint32_t v) {   // sum over the lane's 16-lane row, in every lane of it
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, true);    // quad_perm [1,0,3,2]
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, true);    // quad_perm [2,3,0,1]
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, true);   // row_half_mirror
+    v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xF, 0xF, true);   // row_mirror
+    return v;
 }
 
-// Partial SAD at half-pel MV (hx, hy): floor interpolation with every sample position clamped to
-// the frame (sample_luma_halfpel, satd_scalar.rs:113-166), which is plain interpolation on the
-// replicate-padded window. (hx, hy) are wave-uniform, so the branches are scalar.
-__device__ __forceinline__ uint32_t jw_hp(const JWin& w, int hx, int hy) {
-    const int col = w.hcx + (hx >> 1), row = w.hry + (hy >> 1);
-    const uint32_t a0 = jw_read(w, row, col);
-    uint32_t v;
-    if (hx & 1) {
-        const uint32_t a1 = jw_read(w, row, col + 1);
-        if (hy & 1) v = avg4f(a0, a1, jw_read(w, row + 1, col), jw_read(w, row + 1, col + 1));
-        else v = avg2f(a0, a1);
+// Row SAD at integer MV (dx, dy): each 4x4 sub-block's reference origin is clamped to
+// [0,W-4] x [0,H-4] as a whole (sad_block_int_limit, satd_scalar.rs:4-99); inside the frame's
+// interior no origin is clamped and the 4 dwords of a row are consecutive.
+template <bool INTERIOR>
+__device__ __forceinline__ uint32_t jw_row_int(const JWin& w, int dx, int dy) {
+    uint32_t r[4];
+    if (INTERIOR) {
+        const int c = w.cb + dx;
+        const uint32_t* q = w.cp + __mul24(c & 3, w.CS) + (c >> 2) + __mul24(dy, w.ND) + w.rowc;
+#pragma unroll
+        for (int k = 0; k < 4; k++) r[k] = q[k];
     } else {
-        v = (hy & 1) ? avg2f(a0, jw_read(w, row + 1, col)) : a0;
-    }
-    return __builtin_amdgcn_sad_u8(w.curd, v, 0u);
-}
-
-// SADs of the candidates held by lanes 0..N-1 (cx, cy); lane k receives candidate k's SAD.
-// Two candidates' partial sums (each at most 64 * 1020 < 2^16 in total) share one reduction.
-template <int N, bool HP>
-__device__ __forceinline__ int jw_set(const JWin& w, int lane, int cx, int cy) {
-    uint32_t part[N];
+        const int oy = dclamp(w.y0 + 4 * (w.j >> 2) + dy, 0, w.H - 4) + (w.j & 3) - w.wy0;
 #pragma unroll
-    for (int k = 0; k < N; k++) {
-        const int x = __builtin_amdgcn_readlane(cx, k), y = __builtin_amdgcn_readlane(cy, k);
-        part[k] = HP ? jw_hp(w, x, y) : jw_int(w, x, y);
-    }
-    uint32_t tot[N];
-#pragma unroll
-    for (int k = 0; k < N; k += 2) {
-        if (k + 1 < N) {
-            const uint32_t s = wave_sum((part[k] << 16) | part[k + 1]);
-            tot[k] = s >> 16; tot[k + 1] = s & 0xffffu;
-        } else {
-            tot[k] = wave_sum(part[k]);
+        for (int k = 0; k < 4; k++) {
+            const int ox = dclamp(w.x0 + 4 * k + dx, 0, w.W - 4) - w.wxa;
+            r[k] = w.cp[__mul24(ox & 3, w.CS) + __mul24(oy, w.ND) + (ox >> 2)];
         }
     }
-    uint32_t r = 0;
+    uint32_t acc = __builtin_amdgcn_sad_u8(w.cq.x, r[0], 0u);
+    acc = __builtin_amdgcn_sad_u8(w.cq.y, r[1], acc);
+    acc = __builtin_amdgcn_sad_u8(w.cq.z, r[2], acc);
+    acc = __builtin_amdgcn_sad_u8(w.cq.w, r[3], acc);
+    return row_sum(acc);
+}
+
+// Row SAD at half-pel MV (hx, hy): floor interpolation with every sample position clamped to the
+// frame (sample_luma_halfpel, satd_scalar.rs:113-166), which is plain interpolation on the
+// replicate-padded window. TYPE: 0 integer position, 1 one half-pel component (x when hx is odd,
+// else y), 2 both components half-pel.
+template <int TYPE>
+__device__ __forceinline__ uint32_t jw_row_hp(const JWin& w, int hx, int hy) {
+    const int c = w.cb + (hx >> 1), base = __mul24(hy >> 1, w.ND) + w.rowc;
+    const uint32_t* q0 = w.cp + __mul24(c & 3, w.CS) + (c >> 2) + base;
+    const uint32_t* q1 = w.cp + __mul24((c + 1) & 3, w.CS) + ((c + 1) >> 2) + base;   // one column right
+    uint32_t v[4];
+    if (TYPE == 0) {
 #pragma unroll
-    for (int k = 0; k < N; k++) r = lane == k ? tot[k] : r;
-    return (int)r;
+        for (int k = 0; k < 4; k++) v[k] = q0[k];
+    } else if (TYPE == 1) {
+        const uint32_t* q2 = (hx & 1) ? q1 : q0 + w.ND;   // the second sample: right or below
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = avg2f(q0[k], q2[k]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = avg4f(q0[k], q1[k], q0[k + w.ND], q1[k + w.ND]);
+    }
+    uint32_t acc = __builtin_amdgcn_sad_u8(w.cq.x, v[0], 0u);
+    acc = __builtin_amdgcn_sad_u8(w.cq.y, v[1], acc);
+    acc = __builtin_amdgcn_sad_u8(w.cq.z, v[2], acc);
+    acc = __builtin_amdgcn_sad_u8(w.cq.w, v[3], acc);
+    return row_sum(acc);
+}
+
+__device__ __forceinline__ int lane_bcast(int v, int src) { return __builtin_amdgcn_ds_bpermute(src << 2, v); }
+
+// SADs of the integer candidates held by lanes 0..N-1 (cx, cy); lane k receives candidate k's SAD.
+// Round R puts candidate 4R + r on row r (rows past N repeat candidate 0, which is in range).
+template <int N, bool INTERIOR>
+__device__ __forceinline__ int jw_set_int(const JWin& w, int lane, int cx, int cy) {
+    const int r = lane >> 4;
+    int sv = 0;
+#pragma unroll
+    for (int R = 0; R < (N + 3) / 4; R++) {
+        const int k = 4 * R + r < N ? 4 * R + r : 0;
+        const int s = (int)jw_row_int<INTERIOR>(w, lane_bcast(cx, k), lane_bcast(cy, k));
+        const int got = lane_bcast(s, (lane - 4 * R) << 4);   // row lane - 4R holds candidate `lane`
+        sv = (lane >= 4 * R && lane < 4 * R + 4) ? got : sv;
+    }
+    return sv;
+}
+
+// SADs of the 9 half-pel candidates of refine_halfpel held by lanes 0..8 (lane = the reference's
+// evaluation order: base, then the 8 neighbours sy-major). Rounds group them by interpolation
+// type: the 4 diagonal neighbours (lanes 1, 3, 6, 8), the 4 axial ones (2, 4, 5, 7), the base.
+__device__ __forceinline__ int jw_set_hp(const JWin& w, int lane, int cx, int cy) {
+    const int r = lane >> 4;
+    const int kd = 0x8631 >> (4 * r) & 15, ka = 0x7542 >> (4 * r) & 15;   // row r's candidate per round
+    const int sd = (int)jw_row_hp<2>(w, lane_bcast(cx, kd), lane_bcast(cy, kd));
+    const int sa = (int)jw_row_hp<1>(w, lane_bcast(cx, ka), lane_bcast(cy, ka));
+    const int s0 = (int)jw_row_hp<0>(w, __builtin_amdgcn_readlane(cx, 0), __builtin_amdgcn_readlane(cy, 0));
+    // lane k's row in its round: diagonals 1,3,6,8 -> rows 0..3, axials 2,4,5,7 -> rows 0..3
+    const int rd = (0x3020100 >> (8 * ((lane == 3) + 2 * (lane == 6) + 3 * (lane == 8)))) & 3;
+    const int ra = (lane == 4) + 2 * (lane == 5) + 3 * (lane == 7);
+    const bool diag = lane == 1 || lane == 3 || lane == 6 || lane == 8;
+    const int gd = lane_bcast(sd, rd << 4), ga = lane_bcast(sa, ra << 4);
+    return lane == 0 ? s0 : (diag ? gd : ga);
 }
 
 // The window of MB (x0, y0): rows y0-sr-1 .. y0+16+sr and columns from the dword-aligned
@@ -411,16 +470,17 @@ __device__ __forceinline__ void jw_load(const SearchParams& p, uint32_t* cp, int
 // predictors of gather_local_motion_candidates (:56-156) taken from the neighbour words lS, tS,
 // trS, tlS and the temporal word pT. Returns the canonical search MV word; *sad_out = its
 // half-pel SAD.
-__device__ uint32_t jsearch(const SearchParams& p, const double* bits, uint32_t* cp, int CS, int ND, int NR, int bx, int by,
+template <bool INTERIOR>
+__device__ uint32_t jsearch(const SearchParams& p, const int* ctab, uint32_t* cp, int CS, int ND, int NR, int bx, int by,
                             uint32_t lS, uint32_t tS, uint32_t trS, uint32_t tlS, uint32_t pT, int lane, int* sad_out) {
     const int sr = p.sr, x0 = bx * 16, y0 = by * 16;
     jw_load(p, cp, CS, ND, NR, x0, y0, lane);
     JWin w;
-    w.cp = cp; w.CS = CS; w.ND = ND; w.W = p.W; w.H = p.H;
+    w.cp = cp; w.CS = CS; w.ND = ND; w.W = p.W; w.H = p.H; w.x0 = x0; w.y0 = y0;
     w.wxa = (x0 - sr - 1) & ~3; w.wy0 = y0 - sr - 1;
-    w.curd = *(const uint32_t*)(p.cur + (size_t)(y0 + (lane >> 2)) * p.W + x0 + 4 * (lane & 3));
-    w.lx = x0 + 4 * (lane & 3); w.ly = y0 + 4 * (lane >> 4); w.li = (lane >> 2) & 3;
-    w.hcx = w.lx - w.wxa; w.hry = y0 + (lane >> 2) - w.wy0;
+    w.j = lane & 15;
+    w.cq = *(const uint4*)(p.cur + (size_t)(y0 + w.j) * p.W + x0);
+    w.cb = x0 - w.wxa; w.rowc = (w.j + sr + 1) * ND;
 
     const bool has_l = bx > 0, has_t = by > 0, has_tr = has_t && bx + 1 < p.bw, has_tl = has_t && bx > 0, has_tp = p.prev_mv != nullptr;
     // predictor i in lane i+1, in predictor order: left, top, top-right, top-left, temporal;
@@ -444,7 +504,7 @@ __device__ uint32_t jsearch(const SearchParams& p, const double* bits, uint32_t*
         cc.mx = median3(has_l ? mvx(q0) : 0, has_t ? mvx(q1) : 0, mvx(qm));
         cc.my = median3(has_l ? mvy(q0) : 0, has_t ? mvy(q1) : 0, mvy(qm));
     }
-    cc.bits = bits; cc.lam2 = p.lam2; cc.c_zero = p.c_zero; cc.c_pred = p.c_pred; cc.lam_zero = p.lam_zero;
+    cc.tab = ctab; cc.td = 2 * sr + 4; cc.c_zero = p.c_zero; cc.c_pred = p.c_pred; cc.lam_zero = p.lam_zero;
     int sdx = 0, sdy = 0;
     int scost;   // SAD + MV cost of the current integer best
     bool skip_int;
@@ -455,7 +515,7 @@ __device__ uint32_t jsearch(const SearchParams& p, const double* bits, uint32_t*
         const int qx = mvx(myq), qy = mvy(myq);
         const bool myv = lane == 0 || (first & (myq != 0u) & (abs(qx) <= sr) & (abs(qy) <= sr));
         const int dx = myv ? qx : 0, dy = myv ? qy : 0;
-        const int sv = jw_set<6, false>(w, lane, dx, dy);
+        const int sv = jw_set_int<6, INTERIOR>(w, lane, dx, dy);
         const int key = myv ? ((sv + cc.cost(dx, dy)) << 3) | lane : 0x7fffffff;
         skip_int = p.zthr > 0 && __builtin_amdgcn_readlane(sv, 0) <= p.zthr;
         if (skip_int) {
@@ -475,7 +535,7 @@ __device__ uint32_t jsearch(const SearchParams& p, const double* bits, uint32_t*
             const int dx = sdx + hox, dy = sdy + hoy;
             const bool v = lane < 6 && abs(dx) <= sr && abs(dy) <= sr;
             const int ex = v ? dx : sdx, ey = v ? dy : sdy;
-            const int sv = jw_set<6, false>(w, lane, ex, ey);
+            const int sv = jw_set_int<6, INTERIOR>(w, lane, ex, ey);
             const int key = v ? ((sv + cc.cost(dx, dy)) << 3) | lane : 0x7fffffff;
             const int m = lane_min8(key);
             if ((m >> 3) >= scost) break;
@@ -487,7 +547,7 @@ __device__ uint32_t jsearch(const SearchParams& p, const double* bits, uint32_t*
         const int dx = sdx + dox, dy = sdy + doy;
         const bool v = lane < 4 && abs(dx) <= sr && abs(dy) <= sr;
         const int ex = v ? dx : sdx, ey = v ? dy : sdy;
-        const int sv = jw_set<4, false>(w, lane, ex, ey);
+        const int sv = jw_set_int<4, INTERIOR>(w, lane, ex, ey);
         const int key = v ? ((sv + cc.cost(dx, dy)) << 3) | lane : 0x7fffffff;
         const int m = lane_min8(key);
         if ((m >> 3) < scost) { sdx = __builtin_amdgcn_readlane(dx, m & 7); sdy = __builtin_amdgcn_readlane(dy, m & 7); }
@@ -500,7 +560,7 @@ __device__ uint32_t jsearch(const SearchParams& p, const double* bits, uint32_t*
         const int sel = lane == 0 ? 4 : (lane <= 4 ? lane - 1 : lane);   // 3x3 raster index
         const bool v = lane < 9;
         const int hx = 2 * sdx + (v ? sel % 3 - 1 : 0), hy = 2 * sdy + (v ? sel / 3 - 1 : 0);
-        const int sv = jw_set<9, true>(w, lane, hx, hy);
+        const int sv = jw_set_hp(w, lane, hx, hy);
         const int key = v ? ((sv + (lane == 0 ? 0 : p.c_pred)) << 4) | lane : 0x7fffffff;
         const int m = lane_min16(key), k = m & 15;
         best_sad = __builtin_amdgcn_readlane(sv, k);
@@ -529,10 +589,11 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
     const int NW = gridDim.x * (blockDim.x >> 6), gw = blockIdx.x * (blockDim.x >> 6) + wv;
     const int bw = p.bw, nb = bw * p.bh, sr = p.sr;
     const int NR = 2 * sr + 18, ND = ((2 * sr + 20) >> 2) + 1, CS = NR * ND;
-    double* bits = (double*)lds32;           // 256 doubles
-    int* wflag = (int*)(bits + 256);         // [0] a wave of this workgroup changed a block, [1] pass outcome
+    const int TD = 2 * sr + 4;
+    int* ctab = (int*)lds32;                           // TD x TD NEW-vector cost terms
+    int* wflag = ctab + ((TD * TD + 3) & ~3);          // [0] a wave of this workgroup changed a block, [1] pass outcome
     uint32_t* cp = (uint32_t*)(wflag + 4) + (size_t)wv * 4 * CS;
-    for (int i = threadIdx.x; i < 256; i += blockDim.x) bits[i] = p.bits[i];
+    for (int i = threadIdx.x; i < TD * TD; i += blockDim.x) ctab[i] = new_mv_cost(p.lam2, p.bits, i / TD, i % TD);
     const int BW = riv::barrier_words(p.maxp);
     if (blockIdx.x == 0)   // the other bank was the previous launch's; it is next launch's, so clear it
         for (int i = threadIdx.x; i < BW; i += blockDim.x) p.arrive[(size_t)(p.bank ^ 1) * BW + i] = 0u;
@@ -580,7 +641,9 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
             const uint32_t tlS = sgpr((uint32_t)__builtin_amdgcn_readlane((int)nTL, k)) & 0x7fffffffu;
             const uint32_t pT = sgpr((uint32_t)__builtin_amdgcn_readlane((int)mypT, k));
             int sad;
-            const uint32_t wS = jsearch(p, bits, cp, CS, ND, NR, bx, by, lS, tS, trS, tlS, pT, lane, &sad);
+            const uint32_t wS = mb_interior(bx * 16, by * 16, p.W, p.H, sr)
+                                    ? jsearch<true>(p, ctab, cp, CS, ND, NR, bx, by, lS, tS, trS, tlS, pT, lane, &sad)
+                                    : jsearch<false>(p, ctab, cp, CS, ND, NR, bx, by, lS, tS, trS, tlS, pT, lane, &sad);
             const uint32_t old = sgpr((uint32_t)__builtin_amdgcn_readlane((int)myw, k));
             const bool c = ((wS ^ old) & 0xffffu) != 0u;
             chg |= c;
@@ -1281,8 +1344,8 @@ void hp_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int
     hipLaunchKernelGGL(k_hp_table<HPK>, dim3((W / 16) * (H / 16)), dim3(128), lds, st, cur, ref, W, H, W / 16, sr, ndp, out, stride);
 }
 size_t search_lds_bytes(int sr, int waves) {
-    const int NR = 2 * sr + 18, ND = ((2 * sr + 20) >> 2) + 1;
-    return 256 * sizeof(double) + 16 + (size_t)waves * 4 * NR * ND * sizeof(uint32_t);
+    const int NR = 2 * sr + 18, ND = ((2 * sr + 20) >> 2) + 1, TD = 2 * sr + 4;
+    return (size_t)((TD * TD + 3) & ~3) * sizeof(int) + 16 + (size_t)waves * 4 * NR * ND * sizeof(uint32_t);
 }
 hipError_t me_search(hipStream_t st, const SearchParams& p, int groups, int waves) {
     hipLaunchKernelGGL(k_me_search, dim3(groups), dim3(64 * waves), search_lds_bytes(p.sr, waves), st, p);
