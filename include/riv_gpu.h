@@ -119,6 +119,16 @@ int riv_encode_frame(riv_encoder* enc, const uint8_t* rgb, size_t len, uint32_t 
 /* Same, with the RGB24 frame already resident in device memory of the encoder's device. */
 int riv_encode_frame_device(riv_encoder* enc, const void* d_rgb, size_t len, uint32_t width, uint32_t height,
                             uint64_t timestamp_ms, riv_frame_stats* stats);
+/* Batched form (no reference counterpart; it stands for running one reference Encoder per stream):
+ * one frame for each of n encoders, whose streams may differ in size. The GPU work of all n frames
+ * is queued on encs[0]'s stream, and the motion searches of the inter frames run as one persistent
+ * launch over all their blocks. Every stream's output is byte-identical to encoding it alone.
+ * The encoders must share device, search range, inter quality / rdo_lambda_mult and the ME
+ * thresholds, and appear once each; rgb[i] is a host (on_device = 0) or device (on_device = 1)
+ * RGB24 frame of len[i] bytes at encs[i]'s display size. timestamp_ms and stats (n entries) may be
+ * NULL. With pipeline_depth > 0 the call returns once the work is queued. */
+int riv_encode_batch_step(riv_encoder* const* encs, size_t n, const void* const* rgb, const size_t* len, const uint64_t* timestamp_ms,
+                          int on_device, riv_frame_stats* stats);
 /* Blocks until every submitted frame has been written (pipelined mode; no-op otherwise). */
 int riv_encoder_flush(riv_encoder* enc);
 /* Completed frames' stats in encode order (pipelined mode); *n receives the number copied. */

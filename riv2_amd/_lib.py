@@ -87,6 +87,8 @@ PROTOTYPES = [
     ("riv_encoder_create_mem", c_int, [POINTER(EncoderConfigC), POINTER(c_void_p)]),
     ("riv_encode_frame", c_int, [c_void_p, c_void_p, c_size_t, c_uint32, c_uint32, c_uint64, POINTER(FrameStatsC)]),
     ("riv_encode_frame_device", c_int, [c_void_p, c_void_p, c_size_t, c_uint32, c_uint32, c_uint64, POINTER(FrameStatsC)]),
+    ("riv_encode_batch_step", c_int, [POINTER(c_void_p), c_size_t, POINTER(c_void_p), POINTER(c_size_t), POINTER(c_uint64), c_int,
+                                      POINTER(FrameStatsC)]),
     ("riv_encoder_flush", c_int, [c_void_p]),
     ("riv_encoder_pop_stats", c_int, [c_void_p, POINTER(FrameStatsC), c_size_t, POINTER(c_size_t)]),
     ("riv_encoder_finish", c_int, [c_void_p]),

@@ -5,7 +5,7 @@ Frame, Encoder, VecWriter; reitero_encode/src/{config,encoder,writer,error}.rs) 
 (Decoder, DecodedFrame; reitero_decode/src/decoder.rs). Every call runs the gfx950 backend.
 """
 import ctypes
-from ctypes import byref, c_size_t, c_uint8, c_void_p, POINTER
+from ctypes import byref, c_size_t, c_uint8, c_uint64, c_void_p, POINTER
 from dataclasses import dataclass, field
 from typing import List, Optional
 
@@ -146,6 +146,30 @@ class EncodeFrameStats:
                                 s.storage_width, s.storage_height, s.blocks_total, s.blocks_skipped, list(s.mv_mode_counts),
                                 s.mv_new_blocks, s.mv_bias_dx, s.mv_bias_dy, s.rdo_lambda, s.rdo_blocks_evaluated,
                                 s.rdo_blocks_selected_skip, s.gpu_ms, s.host_entropy_ms, s.host_tail_ms, s.me_passes)
+
+
+def encode_batch_step(encoders, frames, timestamps=None, on_device=False):
+    """One frame for each encoder (streams of any sizes) in one batched GPU step: the inter frames'
+    motion searches run as one launch. frames[i]: HxWx3 uint8 arrays, or device pointers (ints) with
+    on_device=True. Returns the per-frame stats (complete only for synchronous encoders)."""
+    n = len(encoders)
+    if n == 0 or len(frames) != n:
+        raise ValueError("one frame per encoder")
+    hs = (c_void_p * n)(*[e._h for e in encoders])
+    keep, ptrs, lens = [], (c_void_p * n)(), (c_size_t * n)()
+    for i, (e, f) in enumerate(zip(encoders, frames)):
+        if on_device:
+            ptrs[i] = c_void_p(int(f))
+            lens[i] = e.config.display_width * e.config.display_height * 3
+        else:
+            buf = _as_u8(f)
+            keep.append(buf)
+            ptrs[i] = c_void_p(buf.ctypes.data)
+            lens[i] = buf.size
+    ts = (c_uint64 * n)(*(timestamps if timestamps is not None else [0] * n))
+    st = (L.FrameStatsC * n)()
+    _check(L.lib().riv_encode_batch_step(hs, n, ptrs, lens, ts, 1 if on_device else 0, st), EncodeError)
+    return [EncodeFrameStats.from_c(st[i]) for i in range(n)]
 
 
 class VecWriter:

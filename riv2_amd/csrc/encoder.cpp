@@ -83,6 +83,25 @@ private:
 
 }  // namespace
 
+struct riv_encoder;
+namespace {
+// Host stage shared by every pipelined encoder of the process: the worker pool that entropy-codes
+// frames and the thread that waits for the GPU to finish them, in submission order, before handing
+// them to the pool (so no worker ever blocks on the GPU). Created with the first pipelined encoder
+// and joined when the last one is destroyed.
+struct HostStage {
+    Pool pool;
+    std::thread waiter;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<std::pair<riv_encoder*, int>> q;   // (encoder, slot) in submission order
+    bool stop = false;
+};
+std::mutex g_hs_mu;
+HostStage* g_hs = nullptr;
+int g_hs_refs = 0;
+}  // namespace
+
 struct riv_encoder {
     riv_encoder_config cfg{};
     riv_writer_vtbl wv{};
@@ -107,7 +126,15 @@ struct riv_encoder {
 
     std::vector<Slot> slots;
     int depth = 0;                 // configured pipeline depth (0 = synchronous)
-    std::unique_ptr<Pool> pool;
+    HostStage* hs = nullptr;       // the process's host stage, while pipelined
+    hipEvent_t sync_ev = nullptr;  // orders this encoder's stream against a batch step's stream
+    // batch-step scratch (held by the first encoder of a batch): field and barrier counters over all
+    // streams of the batch, the stream descriptors and their pinned staging copies
+    DevBuf b_field, b_arrive, b_desc;
+    size_t b_field_n = 0, b_arrive_n = 0, b_desc_n = 0;
+    int b_bank = 0, b_stage = 0, b_maxp = 0;
+    HostBuf b_stage_buf[2];
+    hipEvent_t b_stage_ev[2] = {nullptr, nullptr};
 
     // ordered tail + completion state
     std::mutex mu;
@@ -115,13 +142,6 @@ struct riv_encoder {
     uint64_t next_tail = 0;        // next frame index allowed into the ordered tail
     uint64_t completed = 0;        // frames fully written
     bool tail_active = false;      // a thread is running ready tails in frame order
-    // GPU completion: one thread waits for frames in submission order and hands each finished
-    // frame to the pool, so no pool worker ever blocks on the GPU
-    std::thread waiter;
-    std::mutex wmu;
-    std::condition_variable wcv;
-    std::deque<int> waitq;
-    bool wstop = false;
     int err_code = 0;
     std::string err_msg;
     rivh::MvCoder mvc;
@@ -153,7 +173,8 @@ struct riv_encoder {
     }
 };
 
-static void waiter_loop(riv_encoder* e);
+static HostStage* acquire_host_stage();
+static void release_host_stage();
 
 extern "C" {
 
@@ -199,7 +220,7 @@ static int encoder_init(riv_encoder* e) {
         RIV_HIP(hipGetDeviceProperties(&prop, c.device_id));
         const size_t budget = prop.sharedMemPerBlock;
         e->s_waves = 16;
-        while (e->s_waves > 1 && rivk::search_lds_bytes(c.search_range, e->s_waves) > budget) e->s_waves--;
+        while (e->s_waves > 1 && rivk::search_lds_bytes(c.search_range, e->s_waves, 1) > budget) e->s_waves--;
         e->s_groups = std::max(1, std::min(prop.multiProcessorCount, (e->nb + e->s_waves - 1) / e->s_waves));
         if ((long long)e->s_groups * e->s_waves * 64 < e->nb) {
             set_error("frame has more macroblocks than the motion search's resident waves can hold");
@@ -230,13 +251,8 @@ static int encoder_init(riv_encoder* e) {
         RIV_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming | hipEventBlockingSync));   // waiting workers sleep instead of spinning
         RIV_HIP(hipEventCreate(&s.t0)); RIV_HIP(hipEventCreate(&s.t1));
     }
-    if (e->depth > 0) {
-        e->pool.reset(new Pool());
-        // workers: up to 16, leaving two cores to the submitting thread and the completion thread
-        const int cores = (int)std::max(3u, std::thread::hardware_concurrency());
-        e->pool->start(std::max(1, std::min({e->depth, 16, cores - 2})));
-        e->waiter = std::thread(waiter_loop, e);
-    }
+    RIV_HIP(hipEventCreateWithFlags(&e->sync_ev, hipEventDisableTiming));
+    if (e->depth > 0) e->hs = acquire_host_stage();
     // MV-cost bit table (motion/scalar.rs:495-525) and lambda-derived costs
     rivhost::RdoModel rdo(c.inter_quality, c.rdo_lambda_mult, c.skip_threshold);
     e->lambda = rdo.lambda;
@@ -291,16 +307,13 @@ static int create_common(const riv_encoder_config* cfg, riv_encoder* e) {
 static void destroy_encoder(riv_encoder* e) {
     if (!e) return;
     (void)hipSetDevice(e->cfg.device_id);
-    if (e->pool) {   // let in-flight frames finish before the buffers go away
+    if (e->hs) {   // every submitted frame reaches its tail (errors included) before the buffers go away
         std::unique_lock<std::mutex> lk(e->mu);
-        e->cv.wait(lk, [&] { return e->completed == e->frame_count || e->err_code; });
+        e->cv.wait(lk, [&] { return e->completed == e->frame_count; });
+        lk.unlock();
+        release_host_stage();
+        e->hs = nullptr;
     }
-    if (e->waiter.joinable()) {
-        { std::lock_guard<std::mutex> lk(e->wmu); e->wstop = true; }
-        e->wcv.notify_all();
-        e->waiter.join();
-    }
-    e->pool.reset();
     if (e->st) (void)hipStreamSynchronize(e->st);
     for (Slot& s : e->slots) {
         if (s.done) (void)hipEventDestroy(s.done);
@@ -308,6 +321,8 @@ static void destroy_encoder(riv_encoder* e) {
         if (s.t1) (void)hipEventDestroy(s.t1);
         for (auto& m : s.marks) if (m) (void)hipEventDestroy(m);
     }
+    if (e->sync_ev) (void)hipEventDestroy(e->sync_ev);
+    for (auto& ev : e->b_stage_ev) if (ev) (void)hipEventDestroy(ev);
     if (e->st) (void)hipStreamDestroy(e->st);
     delete e;
 }
@@ -490,36 +505,84 @@ static void run_tails(riv_encoder* e, int si) {
     e->tail_active = false;
 }
 
-// The GPU-completion thread of the pipelined mode.
-static void waiter_loop(riv_encoder* e) {
-    (void)hipSetDevice(e->cfg.device_id);
+// The GPU-completion thread of the host stage.
+static void waiter_loop(HostStage* hs) {
     for (;;) {
-        int si;
+        std::pair<riv_encoder*, int> job;
         {
-            std::unique_lock<std::mutex> lk(e->wmu);
-            e->wcv.wait(lk, [&] { return e->wstop || !e->waitq.empty(); });
-            if (e->waitq.empty()) return;
-            si = e->waitq.front();
-            e->waitq.pop_front();
+            std::unique_lock<std::mutex> lk(hs->mu);
+            hs->cv.wait(lk, [&] { return hs->stop || !hs->q.empty(); });
+            if (hs->q.empty()) return;
+            job = hs->q.front();
+            hs->q.pop_front();
         }
+        riv_encoder* e = job.first;
+        const int si = job.second;
+        (void)hipSetDevice(e->cfg.device_id);
         if (hipEventSynchronize(e->slots[si].done) != hipSuccess) {
             e->fail(RIV_E_GPU, "GPU frame failed");
             e->slots[si].failed = true;
             run_tails(e, si);
             continue;
         }
-        e->pool->submit([e, si] { code_frame(e, si); });
+        hs->pool.submit([e, si] { code_frame(e, si); });
     }
 }
 
-static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t len, uint32_t w, uint32_t h, uint64_t ts, riv_frame_stats* stats) {
+static HostStage* acquire_host_stage() {
+    std::lock_guard<std::mutex> lk(g_hs_mu);
+    if (!g_hs) {
+        g_hs = new HostStage();
+        // workers: up to 16, leaving two cores to the submitting thread and the completion thread
+        const int cores = (int)std::max(3u, std::thread::hardware_concurrency());
+        g_hs->pool.start(std::max(1, std::min(16, cores - 2)));
+        g_hs->waiter = std::thread(waiter_loop, g_hs);
+    }
+    g_hs_refs++;
+    return g_hs;
+}
+
+static void release_host_stage() {
+    std::lock_guard<std::mutex> lk(g_hs_mu);
+    if (--g_hs_refs > 0) return;
+    { std::lock_guard<std::mutex> q(g_hs->mu); g_hs->stop = true; }
+    g_hs->cv.notify_all();
+    g_hs->waiter.join();
+    delete g_hs;   // joins the workers
+    g_hs = nullptr;
+}
+
+// ---- one frame, in phases: begin (up to the motion search), search, after the search, end ----
+struct FrameCtx {
+    int si = 0;
+    bool intra = false;
+    hipStream_t st = nullptr;
+    uint8_t *cY, *cU, *cV, *rY, *rU, *rV, *nY, *nU, *nV, *pY, *pU, *pV;
+    const int8_t* prev_mv = nullptr;
+};
+
+static void mark(riv_encoder* e, Slot& s, hipStream_t st, int stage) {
+    if (!e->profiling || s.n_marks > RIV_NUM_STAGES) return;
+    if (e->profiling == 1 && stage != 0 && stage != 1) return;   // two events per frame: around the search
+    (void)hipEventRecord(s.marks[s.n_marks], st);
+    s.mark_stage[s.n_marks++] = stage;
+}
+
+static int check_frame(riv_encoder* e, size_t len, uint32_t w, uint32_t h) {
     const riv_encoder_config& c = e->cfg;
     if (w != c.display_width || h != c.display_height) { set_error("Frame dimensions do not match config"); return RIV_E_FRAME; }
     if (len != (size_t)w * h * 3) { set_error("Display RGB24 size mismatch"); return RIV_E_FRAME; }
-    {
-        std::lock_guard<std::mutex> lk(e->mu);
-        if (e->err_code) { set_error(e->err_msg); return e->err_code; }
-    }
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (e->err_code) { set_error(e->err_msg); return e->err_code; }
+    return RIV_OK;
+}
+
+// Takes a slot, converts the frame, and for an intra frame runs all of its GPU work.
+static int frame_begin(riv_encoder* e, const void* rgb, bool on_device, size_t len, uint32_t w, uint32_t h, uint64_t ts,
+                       hipStream_t st, FrameCtx& f) {
+    const riv_encoder_config& c = e->cfg;
+    int r = check_frame(e, len, w, h);
+    if (r) return r;
     RIV_HIP(hipSetDevice(c.device_id));
     // keyframe decision (encoder.rs:265-291)
     bool intra;
@@ -539,20 +602,14 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
     s.n_marks = 0;
     if (e->profiling && !s.marks[0])
         for (auto& m : s.marks) RIV_HIP(hipEventCreate(&m));
-    auto mark = [&](int stage) {
-        if (!e->profiling || s.n_marks > RIV_NUM_STAGES) return;
-        if (e->profiling == 1 && stage != 0 && stage != 1) return;   // two events per frame: around the search
-        (void)hipEventRecord(s.marks[s.n_marks], e->st);
-        s.mark_stage[s.n_marks++] = stage;
-    };
 
-    const int W = e->W, H = e->H, nb = e->nb;
-    const size_t ysz = (size_t)W * H, csz = ysz / 4;
-    uint8_t* cY = e->cur.as<uint8_t>(); uint8_t* cU = cY + ysz; uint8_t* cV = cU + csz;
-    uint8_t* rY = e->ref[e->ref_idx].as<uint8_t>(); uint8_t* rU = rY + ysz; uint8_t* rV = rU + csz;
-    uint8_t* nY = e->ref[1 - e->ref_idx].as<uint8_t>(); uint8_t* nU = nY + ysz; uint8_t* nV = nU + csz;
-    uint8_t* pY = e->pred.as<uint8_t>(); uint8_t* pU = pY + ysz; uint8_t* pV = pU + csz;
-    hipStream_t st = e->st;
+    f.si = si; f.intra = intra; f.st = st;
+    const size_t ysz = (size_t)e->W * e->H, csz = ysz / 4;
+    f.cY = e->cur.as<uint8_t>(); f.cU = f.cY + ysz; f.cV = f.cU + csz;
+    f.rY = e->ref[e->ref_idx].as<uint8_t>(); f.rU = f.rY + ysz; f.rV = f.rU + csz;
+    f.nY = e->ref[1 - e->ref_idx].as<uint8_t>(); f.nU = f.nY + ysz; f.nV = f.nU + csz;
+    f.pY = e->pred.as<uint8_t>(); f.pU = f.pY + ysz; f.pV = f.pU + csz;
+    f.prev_mv = e->prev_inter_slot >= 0 ? (const int8_t*)e->slots[e->prev_inter_slot].rec.p + 4 : nullptr;
 
     RIV_HIP(hipEventRecord(s.t0, st));
     const uint8_t* d_rgb = (const uint8_t*)rgb;
@@ -560,65 +617,87 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
         RIV_HIP(hipMemcpyAsync(e->rgb.p, rgb, len, hipMemcpyHostToDevice, st));
         d_rgb = e->rgb.as<uint8_t>();
     }
-    mark(-1);
-    rivk::rgb_to_yuv420(st, d_rgb, (int)w, (int)h, W, H, cY, cU, cV);
-    mark(0);
-    BlockRec* rec = s.rec.as<BlockRec>();
+    mark(e, s, st, -1);
+    rivk::rgb_to_yuv420(st, d_rgb, (int)w, (int)h, e->W, e->H, f.cY, f.cU, f.cV);
+    mark(e, s, st, 0);
     if (intra) {
-        rivk::intra(st, cY, cU, cV, W, H, e->yt.as<uint16_t>(), e->ct.as<uint16_t>(), nY, nU, nV, s.h_pack.dev<uint8_t>());
-        mark(6);
-        rivk::deblock(st, nY, nU, nV, W, H, deblock_level(e->intra_qs), true, nullptr);
-        mark(5);
-    } else {
-        const int8_t* prev_mv = e->prev_inter_slot >= 0 ? (const int8_t*)e->slots[e->prev_inter_slot].rec.p + 4 : nullptr;
-        riv::SearchParams p{};
-        p.cur = cY; p.ref = rY;
-        p.W = W; p.H = H; p.bw = e->bw; p.bh = e->bh; p.sr = c.search_range;
-        p.prev_mv = prev_mv; p.prev_stride = sizeof(BlockRec);
-        p.field = e->field.as<uint32_t>(); p.smv = e->smv.as<unsigned long long>();
-        p.arrive = e->arrive.as<unsigned>(); p.bank = e->s_bank; p.maxp = e->s_maxp;
-        e->s_bank ^= 1;
-        p.bits = e->bits.as<double>(); p.lam2 = e->lambda * 2.0; p.c_zero = e->c_zero; p.c_pred = e->c_pred;
-        p.lam_zero = e->lambda == 0.0;
-        p.zthr = c.me_zero_mv_threshold; p.pthr = c.me_predictor_threshold;
-        p.err = e->err.as<int>(); p.passes = s.h_passes.dev<int>();
-        RIV_HIP(rivk::me_search(st, p, e->s_groups, e->s_waves));
-        riv::ModeParams mp{};
-        mp.bw = e->bw; mp.bh = e->bh; mp.smv = p.smv; mp.prev_mv = prev_mv; mp.prev_stride = sizeof(BlockRec);
-        mp.rdo_bits = e->rdo.as<uint32_t>(); mp.rdo_max = e->rdo_max; mp.rec = rec; mp.host_rec = s.h_rec.dev<BlockRec>();
-        mp.err = e->err.as<int>();
-        rivk::mode(st, mp);
-        mark(1);
-        rivk::mc(st, rY, rU, rV, W, H, (const int8_t*)rec, sizeof(BlockRec), pY, pU, pV, e->var.as<uint32_t>(), e->lnv.as<float>());
-        mark(2);
-        rivk::aq(st, nb, e->inter_qs, e->lnv.as<float>(), e->steps.as<float>());
-        mark(3);
-        rivk::inter_residual(st, cY, cU, cV, pY, pU, pV, W, H, rec, e->steps.as<float>(), c.inter_dead_zone, nY, nU, nV,
-                             s.h_pack.dev<uint8_t>(), s.optskip.as<uint8_t>(), s.h_optskip.dev<uint8_t>());
-        mark(4);
-        rivk::filter_mask(st, rec, s.optskip.as<uint8_t>(), nb, e->mask.as<uint8_t>());
-        rivk::deblock(st, nY, nU, nV, W, H, deblock_level(e->inter_qs), false, e->mask.as<uint8_t>());
-        mark(5);
+        rivk::intra(st, f.cY, f.cU, f.cV, e->W, e->H, e->yt.as<uint16_t>(), e->ct.as<uint16_t>(), f.nY, f.nU, f.nV, s.h_pack.dev<uint8_t>());
+        mark(e, s, st, 6);
+        rivk::deblock(st, f.nY, f.nU, f.nV, e->W, e->H, deblock_level(e->intra_qs), true, nullptr);
+        mark(e, s, st, 5);
     }
     RIV_HIP(hipGetLastError());
+    return RIV_OK;
+}
+
+// The inter frame's part of a search launch.
+static riv::SearchStream frame_search_stream(riv_encoder* e, const FrameCtx& f, int blk0) {
+    riv::SearchStream S{};
+    S.cur = f.cY; S.ref = f.rY; S.prev_mv = f.prev_mv; S.prev_stride = sizeof(BlockRec);
+    S.smv = e->smv.as<unsigned long long>(); S.passes = e->slots[f.si].h_passes.dev<int>();
+    S.W = e->W; S.H = e->H; S.bw = e->bw; S.bh = e->bh; S.blk0 = blk0;
+    return S;
+}
+
+// Search parameters shared by every stream of a launch (from the launching encoder's config).
+static riv::SearchParams search_params(riv_encoder* e) {
+    const riv_encoder_config& c = e->cfg;
+    riv::SearchParams p{};
+    p.sr = c.search_range;
+    p.bits = e->bits.as<double>(); p.lam2 = e->lambda * 2.0; p.c_zero = e->c_zero; p.c_pred = e->c_pred;
+    p.lam_zero = e->lambda == 0.0;
+    p.zthr = c.me_zero_mv_threshold; p.pthr = c.me_predictor_threshold;
+    p.err = e->err.as<int>();
+    return p;
+}
+
+// Everything of an inter frame after its motion search.
+static int frame_after_search(riv_encoder* e, const FrameCtx& f) {
+    const riv_encoder_config& c = e->cfg;
+    Slot& s = e->slots[f.si];
+    hipStream_t st = f.st;
+    BlockRec* rec = s.rec.as<BlockRec>();
+    riv::ModeParams mp{};
+    mp.bw = e->bw; mp.bh = e->bh; mp.smv = e->smv.as<unsigned long long>(); mp.prev_mv = f.prev_mv; mp.prev_stride = sizeof(BlockRec);
+    mp.rdo_bits = e->rdo.as<uint32_t>(); mp.rdo_max = e->rdo_max; mp.rec = rec; mp.host_rec = s.h_rec.dev<BlockRec>();
+    mp.err = e->err.as<int>();
+    rivk::mode(st, mp);
+    mark(e, s, st, 1);
+    rivk::mc(st, f.rY, f.rU, f.rV, e->W, e->H, (const int8_t*)rec, sizeof(BlockRec), f.pY, f.pU, f.pV, e->var.as<uint32_t>(), e->lnv.as<float>());
+    mark(e, s, st, 2);
+    rivk::aq(st, e->nb, e->inter_qs, e->lnv.as<float>(), e->steps.as<float>());
+    mark(e, s, st, 3);
+    rivk::inter_residual(st, f.cY, f.cU, f.cV, f.pY, f.pU, f.pV, e->W, e->H, rec, e->steps.as<float>(), c.inter_dead_zone, f.nY, f.nU, f.nV,
+                         s.h_pack.dev<uint8_t>(), s.optskip.as<uint8_t>(), s.h_optskip.dev<uint8_t>());
+    mark(e, s, st, 4);
+    rivk::filter_mask(st, rec, s.optskip.as<uint8_t>(), e->nb, e->mask.as<uint8_t>());
+    rivk::deblock(st, f.nY, f.nU, f.nV, e->W, e->H, deblock_level(e->inter_qs), false, e->mask.as<uint8_t>());
+    mark(e, s, st, 5);
+    RIV_HIP(hipGetLastError());
+    return RIV_OK;
+}
+
+// Closes the frame's GPU work and hands it to the host stage (pipelined) or codes it (synchronous).
+static int frame_end(riv_encoder* e, const FrameCtx& f, riv_frame_stats* stats) {
+    Slot& s = e->slots[f.si];
     // the kernels wrote the records, skip flags and packed coefficients straight to the slot's
     // mapped host memory: the frame is ready for the host stage once the stream reaches here
-    RIV_HIP(hipEventRecord(s.t1, st));
-    RIV_HIP(hipEventRecord(s.done, st));
-    e->prev_inter_slot = intra ? -1 : si;
+    RIV_HIP(hipEventRecord(s.t1, f.st));
+    RIV_HIP(hipEventRecord(s.done, f.st));
+    e->prev_inter_slot = f.intra ? -1 : f.si;
     e->ref_idx = 1 - e->ref_idx;
     e->have_ref = true;
     e->frame_count++;
 
-    if (e->depth > 0) {
-        { std::lock_guard<std::mutex> lk(e->wmu); e->waitq.push_back(si); }
-        e->wcv.notify_one();
-        if (stats) { std::memset(stats, 0, sizeof(*stats)); stats->frame_index = s.index; stats->timestamp_ms = ts; stats->frame_type = intra ? 1 : 2; }
+    if (e->hs) {
+        { std::lock_guard<std::mutex> lk(e->hs->mu); e->hs->q.push_back({e, f.si}); }
+        e->hs->cv.notify_one();
+        if (stats) { std::memset(stats, 0, sizeof(*stats)); stats->frame_index = s.index; stats->timestamp_ms = s.ts; stats->frame_type = f.intra ? 1 : 2; }
         return RIV_OK;
     }
-    if (hipEventSynchronize(s.done) != hipSuccess) { e->fail(RIV_E_GPU, "GPU frame failed"); s.failed = true; run_tails(e, si); }
-    else code_frame(e, si);
-    if (!intra) {
+    if (hipEventSynchronize(s.done) != hipSuccess) { e->fail(RIV_E_GPU, "GPU frame failed"); s.failed = true; run_tails(e, f.si); }
+    else code_frame(e, f.si);
+    if (!f.intra) {
         int werr = 0;
         RIV_HIP(hipMemcpy(&werr, e->err.p, 4, hipMemcpyDeviceToHost));
         if (werr) { set_error(werr == 2 ? "mode decision: reconstructed MV differs from search MV" : werr == 3 ? "motion search did not settle within its pass limit" : "motion-search barrier timed out"); return RIV_E_GPU; }
@@ -630,6 +709,122 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
     return RIV_OK;
 }
 
+static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t len, uint32_t w, uint32_t h, uint64_t ts, riv_frame_stats* stats) {
+    FrameCtx f;
+    int r = frame_begin(e, rgb, on_device, len, w, h, ts, e->st, f);
+    if (r) return r;
+    if (!f.intra) {
+        riv::SearchParams p = search_params(e);
+        p.s0 = frame_search_stream(e, f, 0);
+        p.streams = nullptr; p.nstreams = 1; p.nb = e->nb;
+        p.field = e->field.as<uint32_t>(); p.arrive = e->arrive.as<unsigned>(); p.bank = e->s_bank; p.maxp = e->s_maxp;
+        e->s_bank ^= 1;
+        RIV_HIP(rivk::me_search(f.st, p, e->s_groups, e->s_waves));
+        r = frame_after_search(e, f);
+        if (r) return r;
+    }
+    return frame_end(e, f, stats);
+}
+
+// One frame for each of n encoders; their GPU work is queued on the first encoder's stream and the
+// motion searches of all inter frames run as one launch.
+static int batch_step(riv_encoder* const* encs, size_t n, const void* const* rgb, const size_t* len, const uint64_t* ts, bool on_device,
+                      riv_frame_stats* stats) {
+    riv_encoder* e0 = encs[0];
+    const riv_encoder_config& c0 = e0->cfg;
+    for (size_t i = 0; i < n; i++) {
+        riv_encoder* e = encs[i];
+        if (!e || !rgb[i]) return RIV_E_ARG;
+        for (size_t j = 0; j < i; j++) if (encs[j] == e) { set_error("an encoder appears twice in a batch step"); return RIV_E_ARG; }
+        const riv_encoder_config& c = e->cfg;
+        if (c.device_id != c0.device_id || c.search_range != c0.search_range || e->lambda != e0->lambda ||
+            c.me_zero_mv_threshold != c0.me_zero_mv_threshold || c.me_predictor_threshold != c0.me_predictor_threshold) {
+            set_error("batched encoders must share device, search range, inter quality / lambda and ME thresholds");
+            return RIV_E_ARG;
+        }
+        int r = check_frame(e, len[i], c.display_width, c.display_height);
+        if (r) return r;
+    }
+    RIV_HIP(hipSetDevice(c0.device_id));
+    hipStream_t st = e0->st;
+    for (size_t i = 1; i < n; i++) {   // the batch stream follows each encoder's earlier work
+        RIV_HIP(hipEventRecord(encs[i]->sync_ev, encs[i]->st));
+        RIV_HIP(hipStreamWaitEvent(st, encs[i]->sync_ev, 0));
+    }
+    std::vector<FrameCtx> f(n);
+    std::vector<riv::SearchStream> desc;
+    int total = 0, maxp = 0;
+    for (size_t i = 0; i < n; i++) {
+        riv_encoder* e = encs[i];
+        int r = frame_begin(e, rgb[i], on_device, len[i], e->cfg.display_width, e->cfg.display_height, ts ? ts[i] : 0, st, f[i]);
+        if (r) return r;
+        if (!f[i].intra) {
+            desc.push_back(frame_search_stream(e, f[i], total));
+            total += e->nb;
+            maxp = std::max(maxp, e->s_maxp);
+        }
+    }
+    if (!desc.empty()) {
+        riv::SearchParams p = search_params(e0);
+        p.nstreams = (int)desc.size(); p.nb = total; p.maxp = maxp;
+        // launch shape for the whole batch
+        hipDeviceProp_t prop;
+        RIV_HIP(hipGetDeviceProperties(&prop, c0.device_id));
+        int waves = 16;
+        while (waves > 1 && rivk::search_lds_bytes(p.sr, waves, p.nstreams) > prop.sharedMemPerBlock) waves--;
+        if (rivk::search_lds_bytes(p.sr, waves, p.nstreams) > prop.sharedMemPerBlock) { set_error("too many streams in one batch step"); return RIV_E_UNSUPPORTED; }
+        const int groups = std::max(1, std::min(prop.multiProcessorCount, (total + waves - 1) / waves));
+        if ((long long)groups * waves * 64 < total) { set_error("batch has more macroblocks than the motion search's resident waves can hold"); return RIV_E_UNSUPPORTED; }
+        // scratch sized for the batch (grown when a larger batch comes; the stream is drained first)
+        const size_t need_field = 2 * (size_t)total, need_arrive = 2 * (size_t)riv::barrier_words(maxp);
+        if (need_field > e0->b_field_n || need_arrive > e0->b_arrive_n || desc.size() > e0->b_desc_n) {
+            RIV_HIP(hipStreamSynchronize(st));
+            if (need_field > e0->b_field_n) { RIV_HIP(e0->b_field.alloc(4 * need_field)); e0->b_field_n = need_field; }
+            if (need_arrive > e0->b_arrive_n) {
+                RIV_HIP(e0->b_arrive.alloc(4 * need_arrive)); RIV_HIP(hipMemset(e0->b_arrive.p, 0, 4 * need_arrive));
+                e0->b_arrive_n = need_arrive; e0->b_bank = 0;
+            }
+            if (desc.size() > e0->b_desc_n) {
+                const size_t cap = std::max<size_t>(desc.size(), 64);
+                RIV_HIP(e0->b_desc.alloc(cap * sizeof(riv::SearchStream)));
+                for (int k = 0; k < 2; k++) {
+                    RIV_HIP(e0->b_stage_buf[k].alloc(cap * sizeof(riv::SearchStream)));
+                    if (!e0->b_stage_ev[k]) RIV_HIP(hipEventCreateWithFlags(&e0->b_stage_ev[k], hipEventDisableTiming));
+                }
+                e0->b_desc_n = cap;
+            }
+        }
+        // the counter banks are laid out for one pass limit: clear them when it changes
+        if (maxp != e0->b_maxp) {
+            RIV_HIP(hipMemsetAsync(e0->b_arrive.p, 0, 4 * need_arrive, st));
+            e0->b_maxp = maxp; e0->b_bank = 0;
+        }
+        // the descriptors go up through a pinned staging buffer that no pending copy still reads
+        const int k = e0->b_stage;
+        e0->b_stage ^= 1;
+        RIV_HIP(hipEventSynchronize(e0->b_stage_ev[k]));
+        std::memcpy(e0->b_stage_buf[k].p, desc.data(), desc.size() * sizeof(riv::SearchStream));
+        RIV_HIP(hipMemcpyAsync(e0->b_desc.p, e0->b_stage_buf[k].p, desc.size() * sizeof(riv::SearchStream), hipMemcpyHostToDevice, st));
+        RIV_HIP(hipEventRecord(e0->b_stage_ev[k], st));
+        p.streams = e0->b_desc.as<riv::SearchStream>();
+        p.field = e0->b_field.as<uint32_t>(); p.arrive = e0->b_arrive.as<unsigned>(); p.bank = e0->b_bank;
+        e0->b_bank ^= 1;
+        RIV_HIP(rivk::me_search(st, p, groups, waves));
+    }
+    int status = RIV_OK;
+    for (size_t i = 0; i < n; i++) {
+        riv_encoder* e = encs[i];
+        int r = f[i].intra ? RIV_OK : frame_after_search(e, f[i]);
+        if (!r) r = frame_end(e, f[i], stats ? &stats[i] : nullptr);
+        if (r && !status) status = r;
+    }
+    for (size_t i = 1; i < n; i++) {   // each encoder's own stream follows the batch
+        RIV_HIP(hipEventRecord(encs[i]->sync_ev, st));
+        RIV_HIP(hipStreamWaitEvent(encs[i]->st, encs[i]->sync_ev, 0));
+    }
+    return status;
+}
+
 int riv_encode_frame(riv_encoder* e, const uint8_t* rgb, size_t len, uint32_t w, uint32_t h, uint64_t ts, riv_frame_stats* stats) {
     if (!e || !rgb) return RIV_E_ARG;
     return encode_impl(e, rgb, false, len, w, h, ts, stats);
@@ -637,6 +832,12 @@ int riv_encode_frame(riv_encoder* e, const uint8_t* rgb, size_t len, uint32_t w,
 int riv_encode_frame_device(riv_encoder* e, const void* d_rgb, size_t len, uint32_t w, uint32_t h, uint64_t ts, riv_frame_stats* stats) {
     if (!e || !d_rgb) return RIV_E_ARG;
     return encode_impl(e, d_rgb, true, len, w, h, ts, stats);
+}
+
+int riv_encode_batch_step(riv_encoder* const* encs, size_t n, const void* const* rgb, const size_t* len, const uint64_t* timestamp_ms,
+                          int on_device, riv_frame_stats* stats) {
+    if (!encs || !n || !rgb || !len) return RIV_E_ARG;
+    return batch_step(encs, n, rgb, len, timestamp_ms, on_device != 0, stats);
 }
 
 int riv_encoder_flush(riv_encoder* e) {

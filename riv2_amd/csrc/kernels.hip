@@ -9,6 +9,7 @@ using namespace rivdev;
 using riv::BlockRec;
 using riv::ModeParams;
 using riv::SearchParams;
+using riv::SearchStream;
 
 namespace rivdev {
 __constant__ int32_t c_B16[16][16];
@@ -436,21 +437,21 @@ __device__ __forceinline__ int jw_set_hp(const JWin& w, int lane, int cx, int cy
 // dword is wholly inside or wholly outside a row).
 // A lane covers rows jr, jr + rstep, ... of dword column ji; the loads of JU rows are all issued
 // before any of them is stored, so a window costs about one memory latency.
-__device__ __forceinline__ void jw_load(const SearchParams& p, uint32_t* cp, int CS, int ND, int NR, int x0, int y0, int lane) {
+__device__ __forceinline__ void jw_load(const SearchStream& S, int sr, uint32_t* cp, int CS, int ND, int NR, int x0, int y0, int lane) {
     constexpr int JU = 9;
-    const int wxa = (x0 - p.sr - 1) & ~3, wy0 = y0 - p.sr - 1;
+    const int wxa = (x0 - sr - 1) & ~3, wy0 = y0 - sr - 1;
     const int rstep = 64 / ND, jr = lane / ND, ji = lane - jr * ND;
     if (jr >= rstep) return;
     const int g0 = wxa + 4 * ji, g1 = g0 + 4;
-    const bool in0 = g0 >= 0 && g0 < p.W, in1 = g1 >= 0 && g1 < p.W;
+    const bool in0 = g0 >= 0 && g0 < S.W, in1 = g1 >= 0 && g1 < S.W;
     for (int rb = jr; rb < NR; rb += JU * rstep) {
         uint32_t d0[JU], d1[JU];
 #pragma unroll
         for (int u = 0; u < JU; u++) {
             const int r = min(rb + u * rstep, NR - 1);
-            const uint8_t* row = p.ref + (size_t)dclamp(wy0 + r, 0, p.H - 1) * p.W;
-            d0[u] = in0 ? *(const uint32_t*)(row + g0) : 0x01010101u * (g0 < 0 ? row[0] : row[p.W - 1]);
-            d1[u] = in1 ? *(const uint32_t*)(row + g1) : 0x01010101u * (g1 < 0 ? row[0] : row[p.W - 1]);
+            const uint8_t* row = S.ref + (size_t)dclamp(wy0 + r, 0, S.H - 1) * S.W;
+            d0[u] = in0 ? *(const uint32_t*)(row + g0) : 0x01010101u * (g0 < 0 ? row[0] : row[S.W - 1]);
+            d1[u] = in1 ? *(const uint32_t*)(row + g1) : 0x01010101u * (g1 < 0 ? row[0] : row[S.W - 1]);
         }
 #pragma unroll
         for (int u = 0; u < JU; u++) {
@@ -471,18 +472,18 @@ __device__ __forceinline__ void jw_load(const SearchParams& p, uint32_t* cp, int
 // trS, tlS and the temporal word pT. Returns the canonical search MV word; *sad_out = its
 // half-pel SAD.
 template <bool INTERIOR>
-__device__ uint32_t jsearch(const SearchParams& p, const int* ctab, uint32_t* cp, int CS, int ND, int NR, int bx, int by,
+__device__ uint32_t jsearch(const SearchParams& p, const SearchStream& S, const int* ctab, uint32_t* cp, int CS, int ND, int NR, int bx, int by,
                             uint32_t lS, uint32_t tS, uint32_t trS, uint32_t tlS, uint32_t pT, int lane, int* sad_out) {
     const int sr = p.sr, x0 = bx * 16, y0 = by * 16;
-    jw_load(p, cp, CS, ND, NR, x0, y0, lane);
+    jw_load(S, sr, cp, CS, ND, NR, x0, y0, lane);
     JWin w;
-    w.cp = cp; w.CS = CS; w.ND = ND; w.W = p.W; w.H = p.H; w.x0 = x0; w.y0 = y0;
+    w.cp = cp; w.CS = CS; w.ND = ND; w.W = S.W; w.H = S.H; w.x0 = x0; w.y0 = y0;
     w.wxa = (x0 - sr - 1) & ~3; w.wy0 = y0 - sr - 1;
     w.j = lane & 15;
-    w.cq = *(const uint4*)(p.cur + (size_t)(y0 + w.j) * p.W + x0);
+    w.cq = *(const uint4*)(S.cur + (size_t)(y0 + w.j) * S.W + x0);
     w.cb = x0 - w.wxa; w.rowc = (w.j + sr + 1) * ND;
 
-    const bool has_l = bx > 0, has_t = by > 0, has_tr = has_t && bx + 1 < p.bw, has_tl = has_t && bx > 0, has_tp = p.prev_mv != nullptr;
+    const bool has_l = bx > 0, has_t = by > 0, has_tr = has_t && bx + 1 < S.bw, has_tl = has_t && bx > 0, has_tp = S.prev_mv != nullptr;
     // predictor i in lane i+1, in predictor order: left, top, top-right, top-left, temporal;
     // (dx, dy) packed in 16 bits. Lane 0 holds the (0,0) candidate.
     const uint32_t q0 = lS & 0xffffu, q1 = tS & 0xffffu, q2 = trS & 0xffffu, q3 = tlS & 0xffffu, q4 = pT & 0xffffu;
@@ -587,13 +588,16 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int NW = gridDim.x * (blockDim.x >> 6), gw = blockIdx.x * (blockDim.x >> 6) + wv;
-    const int bw = p.bw, nb = bw * p.bh, sr = p.sr;
+    const int nb = p.nb, sr = p.sr, NS = p.nstreams;
     const int NR = 2 * sr + 18, ND = ((2 * sr + 20) >> 2) + 1, CS = NR * ND;
     const int TD = 2 * sr + 4;
     int* ctab = (int*)lds32;                           // TD x TD NEW-vector cost terms
     int* wflag = ctab + ((TD * TD + 3) & ~3);          // [0] a wave of this workgroup changed a block, [1] pass outcome
-    uint32_t* cp = (uint32_t*)(wflag + 4) + (size_t)wv * 4 * CS;
+    SearchStream* sd = (SearchStream*)(wflag + 4);     // the launch's streams
+    uint32_t* cp = (uint32_t*)(sd + NS) + (size_t)wv * 4 * CS;
     for (int i = threadIdx.x; i < TD * TD; i += blockDim.x) ctab[i] = new_mv_cost(p.lam2, p.bits, i / TD, i % TD);
+    if (!p.streams) { if (threadIdx.x == 0) sd[0] = p.s0; }
+    else for (int i = threadIdx.x; i < NS * (int)(sizeof(SearchStream) / 4); i += blockDim.x) ((uint32_t*)sd)[i] = ((const uint32_t*)p.streams)[i];
     const int BW = riv::barrier_words(p.maxp);
     if (blockIdx.x == 0)   // the other bank was the previous launch's; it is next launch's, so clear it
         for (int i = threadIdx.x; i < BW; i += blockDim.x) p.arrive[(size_t)(p.bank ^ 1) * BW + i] = 0u;
@@ -605,11 +609,18 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
     if (threadIdx.x == 0) wflag[0] = 0;
     __syncthreads();
 
-    // lane k owns block gw + k*NW (the host keeps nb <= 64*NW)
+    // lane k owns block gw + k*NW (the host keeps nb <= 64*NW), of stream ms: the last with blk0 <= it
     const int myb = gw + lane * NW;
     const bool valid = myb < nb;
-    const int mbx = valid ? myb % bw : 0, mby = valid ? myb / bw : 0;
-    const uint32_t mypT = (valid && p.prev_mv) ? *(const uint32_t*)(p.prev_mv + (size_t)p.prev_stride * myb) : 0u;
+    int ms = 0;
+    for (int lo = 0, hi = NS - 1; lo < hi;) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sd[mid].blk0 <= myb) lo = mid; else hi = mid - 1;
+        ms = lo;
+    }
+    const int mbw = sd[ms].bw, mloc = valid ? myb - sd[ms].blk0 : 0;
+    const int mbx = mloc % mbw, mby = mloc / mbw;
+    const uint32_t mypT = (valid && sd[ms].prev_mv) ? *(const uint32_t*)(sd[ms].prev_mv + (size_t)sd[ms].prev_stride * mloc) : 0u;
     uint32_t myw = 0u;   // start field: every MV (0,0)
     int mysad = 0;
     int outcome = 0;     // 1 converged, -1 barrier timeout, 0 pass limit
@@ -622,9 +633,9 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
         if (pass > 0 && valid) {
             if (mbx > 0) nL = field_load(src + myb - 1);
             if (mby > 0) {
-                nT = field_load(src + myb - bw);
-                if (mbx + 1 < bw) nTR = field_load(src + myb - bw + 1);
-                if (mbx > 0) nTL = field_load(src + myb - bw - 1);
+                nT = field_load(src + myb - mbw);
+                if (mbx + 1 < mbw) nTR = field_load(src + myb - mbw + 1);
+                if (mbx > 0) nTL = field_load(src + myb - mbw - 1);
             }
             dirty = ((nL | nT | nTR | nTL) >> 31) != 0u;
         }
@@ -635,15 +646,16 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
             const int k = __builtin_ctzll(dm);
             const int b = __builtin_amdgcn_readlane(myb, k);
             const int bx = __builtin_amdgcn_readlane(mbx, k), by = __builtin_amdgcn_readlane(mby, k);
+            const SearchStream& S = sd[__builtin_amdgcn_readlane(ms, k)];
             const uint32_t lS = sgpr((uint32_t)__builtin_amdgcn_readlane((int)nL, k)) & 0x7fffffffu;
             const uint32_t tS = sgpr((uint32_t)__builtin_amdgcn_readlane((int)nT, k)) & 0x7fffffffu;
             const uint32_t trS = sgpr((uint32_t)__builtin_amdgcn_readlane((int)nTR, k)) & 0x7fffffffu;
             const uint32_t tlS = sgpr((uint32_t)__builtin_amdgcn_readlane((int)nTL, k)) & 0x7fffffffu;
             const uint32_t pT = sgpr((uint32_t)__builtin_amdgcn_readlane((int)mypT, k));
             int sad;
-            const uint32_t wS = mb_interior(bx * 16, by * 16, p.W, p.H, sr)
-                                    ? jsearch<true>(p, ctab, cp, CS, ND, NR, bx, by, lS, tS, trS, tlS, pT, lane, &sad)
-                                    : jsearch<false>(p, ctab, cp, CS, ND, NR, bx, by, lS, tS, trS, tlS, pT, lane, &sad);
+            const uint32_t wS = mb_interior(bx * 16, by * 16, S.W, S.H, sr)
+                                    ? jsearch<true>(p, S, ctab, cp, CS, ND, NR, bx, by, lS, tS, trS, tlS, pT, lane, &sad)
+                                    : jsearch<false>(p, S, ctab, cp, CS, ND, NR, bx, by, lS, tS, trS, tlS, pT, lane, &sad);
             const uint32_t old = sgpr((uint32_t)__builtin_amdgcn_readlane((int)myw, k));
             const bool c = ((wS ^ old) & 0xffffu) != 0u;
             chg |= c;
@@ -685,8 +697,10 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
         if (threadIdx.x == 0) __hip_atomic_store(p.err, outcome < 0 ? 1 : 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
-    if (valid) p.smv[myb] = (unsigned long long)myw | ((unsigned long long)(uint32_t)mysad << 32);
-    if (blockIdx.x == 0 && threadIdx.x == 0 && p.passes) *p.passes = pass + 1;
+    if (valid) sd[ms].smv[mloc] = (unsigned long long)myw | ((unsigned long long)(uint32_t)mysad << 32);
+    if (blockIdx.x == 0)
+        for (int i = threadIdx.x; i < NS; i += blockDim.x)
+            if (sd[i].passes) *sd[i].passes = pass + 1;
 }
 
 // ============================================================================================
@@ -1343,12 +1357,13 @@ void hp_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int
     const size_t lds = (size_t)(64 + (WS + HPK + 1) * (ndp + 1) + 4 * (WS + HPK) * ndp) * 4;
     hipLaunchKernelGGL(k_hp_table<HPK>, dim3((W / 16) * (H / 16)), dim3(128), lds, st, cur, ref, W, H, W / 16, sr, ndp, out, stride);
 }
-size_t search_lds_bytes(int sr, int waves) {
+size_t search_lds_bytes(int sr, int waves, int nstreams) {
     const int NR = 2 * sr + 18, ND = ((2 * sr + 20) >> 2) + 1, TD = 2 * sr + 4;
-    return (size_t)((TD * TD + 3) & ~3) * sizeof(int) + 16 + (size_t)waves * 4 * NR * ND * sizeof(uint32_t);
+    return (size_t)((TD * TD + 3) & ~3) * sizeof(int) + 16 + (size_t)nstreams * sizeof(SearchStream) +
+           (size_t)waves * 4 * NR * ND * sizeof(uint32_t);
 }
 hipError_t me_search(hipStream_t st, const SearchParams& p, int groups, int waves) {
-    hipLaunchKernelGGL(k_me_search, dim3(groups), dim3(64 * waves), search_lds_bytes(p.sr, waves), st, p);
+    hipLaunchKernelGGL(k_me_search, dim3(groups), dim3(64 * waves), search_lds_bytes(p.sr, waves, p.nstreams), st, p);
     return hipGetLastError();
 }
 void mode(hipStream_t st, const ModeParams& p) {

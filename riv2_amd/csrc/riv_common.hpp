@@ -34,26 +34,40 @@ constexpr int PACK_STRIDE = 48 + 768;
 constexpr int kBarrierLeaves = 16;
 __host__ __device__ inline int barrier_words(int maxp) { return maxp * (kBarrierLeaves + 1) + 1; }
 
-// Parameters of the motion search by relaxation passes (one persistent launch per frame).
-struct SearchParams {
+// One frame of one stream in a motion-search launch. A launch covers the frames of one or more
+// independent streams; their blocks are numbered consecutively, stream by stream (blk0).
+struct SearchStream {
     const uint8_t* cur;          // current luma plane, W x H
     const uint8_t* ref;          // reference luma plane (previous reconstruction), W x H
-    int W, H, bw, bh, sr;
     const int8_t* prev_mv;       // previous P-frame's reconstructed MVs (dx,dy,sx,sy) or nullptr
-    int prev_stride;             // bytes between consecutive MVs of prev_mv
-    uint32_t* field;             // 2 * bw*bh MV words: the field after even / odd passes; bytes
-                                 // (dx, dy, sx, sy), bit 31 set when the pass changed (dx, dy)
     unsigned long long* smv;     // per block out: search MV word | best half-pel SAD << 32
+    int* passes;                 // out: passes the launch ran (may be nullptr)
+    int W, H, bw, bh;
+    int blk0;                    // number of the stream's first block in the launch
+    int prev_stride;             // bytes between consecutive MVs of prev_mv
+};
+static_assert(sizeof(SearchStream) == 64, "SearchStream layout");
+
+// Parameters of the motion search by relaxation passes (one persistent launch per frame step).
+// Every stream of a launch shares the search range, lambda and thresholds.
+struct SearchParams {
+    SearchStream s0;             // the stream of a single-stream launch (streams == nullptr)
+    const SearchStream* streams; // device array of nstreams streams, ascending blk0, or nullptr
+    int nstreams;
+    int nb;                      // blocks of all streams
+    int sr;
+    uint32_t* field;             // 2 * nb MV words: the field after even / odd passes; bytes
+                                 // (dx, dy, sx, sy), bit 31 set when the pass changed (dx, dy)
     unsigned* arrive;            // 2 banks of barrier_words(maxp) pass-barrier counters
     int bank;                    // bank of this launch (zero at launch); the launch clears the other
-    int maxp;                    // pass limit: 2*(bh-1) + bw + 2 passes always reach the fixed point
+    int maxp;                    // pass limit: 2*(bh-1) + bw + 2 passes of the largest stream always
+                                 // reach the fixed point
     const double* bits;          // 256 entries: 1.0 for 0, else 2*log2(a)+2 (motion/scalar.rs:495-525)
     double lam2;                 // lambda * 2.0
     int c_zero, c_pred;          // (lam2*0.5) as i64, (lam2*1.5) as i64; c_pred is also the half-pel penalty
     int lam_zero;                // lambda == 0.0 -> every MV cost is 0
     long long zthr, pthr;        // me_zero_mv_threshold, me_predictor_threshold
     int* err;                    // 1 on a barrier timeout, 3 if the pass limit is hit
-    int* passes;                 // out: passes the launch ran
 };
 
 // Parameters of the per-MB mode decision that follows the search.

@@ -15,7 +15,7 @@ int table_region_bytes(int sr);
 bool mb_is_interior(int x0, int y0, int W, int H, int sr);   // its integer SADs are the half-pel table's even entries
 void sad_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int H, int sr, uint16_t* out, int stride);
 void hp_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int H, int sr, uint16_t* out, int stride);
-size_t search_lds_bytes(int sr, int waves);
+size_t search_lds_bytes(int sr, int waves, int nstreams);
 hipError_t me_search(hipStream_t st, const riv::SearchParams& p, int groups, int waves);
 void mode(hipStream_t st, const riv::ModeParams& p);
 void mc(hipStream_t st, const uint8_t* rY, const uint8_t* rU, const uint8_t* rV, int W, int H, const int8_t* mvs, int mv_stride,
