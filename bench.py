@@ -126,6 +126,59 @@ def cpu_baseline(frames, pframes, threads):
     return pframes / (time.perf_counter() - t0)
 
 
+BATCH_SIZES = [(352, 288), (640, 360), (1280, 720), (1920, 1080)]
+
+
+def run_batched(per_size, p_frames, depth):
+    """The batched workload: per_size streams of each of BATCH_SIZES, each 1 I + p_frames P-frames
+    from the CIF reference case's generator (global motion up to 3 px, 4 rectangles of 32-96 px at
+    up to 6 px/frame) with its own seed, frames resident in HBM. One batch step encodes one frame of
+    every stream. The I step and the first P step are untimed; the other P steps are timed."""
+    import torch
+    import riv2_amd
+    from riv2_amd import synth
+
+    dev = torch.device("cuda:0")
+    streams, frames = [], []
+    for k, (w, h) in enumerate(BATCH_SIZES):
+        for j in range(per_size):
+            seed = 1000 + 100 * k + j
+            seq = synth.Sequence(synth.custom_params(w, h, 1 + p_frames, seed=seed, global_motion=3.0, n_rects=4, rect_min=32,
+                                                     rect_max=96, rect_velocity=6.0))
+            frames.append(torch.from_numpy(seq.frames(0, 1 + p_frames)).to(dev))
+            streams.append((w, h))
+    torch.cuda.synchronize()
+    encs = [riv2_amd.Encoder(riv2_amd.EncoderConfig(w, h, 30).with_intra_quality(85).with_inter_quality(80).with_search_range(12)
+                             .with_skip_threshold(3).with_keyframe_interval(riv2_amd.KeyframeInterval.fixed(1000))
+                             .with_pipeline_depth(depth)) for w, h in streams]
+    step = lambda i: riv2_amd.encode_batch_step(encs, [f[i].data_ptr() for f in frames], [i * 33] * len(encs), on_device=True)
+    step(0)
+    step(1)
+    for e in encs:
+        e.flush()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(2, 1 + p_frames):
+        step(i)
+    for e in encs:
+        e.flush()
+    elapsed = time.perf_counter() - t0
+    mbs = sum((w + 15) // 16 * ((h + 15) // 16) for w, h in streams)
+    nbytes = 0
+    for e in encs:
+        e.finish()
+        nbytes += len(e.output())
+        e.close()
+    steps = p_frames - 1
+    return {"metric": "batched P-frame encode fps/GPU (64 mixed-resolution streams, one batch step per frame of every stream)",
+            "value": round(steps * len(streams) / elapsed, 3), "unit": "fps", "n_gpus": 1, "steps": steps, "warmup": 2,
+            "ms_per_step": round(1000 * elapsed / steps, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "u8", "data": "synthetic",
+            "config": {"workload": f"{len(streams)} streams ({per_size} each of " + ", ".join(f"{w}x{h}" for w, h in BATCH_SIZES) +
+                                   f"), 1 I + {p_frames} P each, q85/80, sr 12, inputs resident in HBM", "pipeline_depth": depth},
+            "fps_1080p_equivalent": round(steps * mbs / 8160 / elapsed, 3), "stream_bytes": nbytes}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,6 +189,9 @@ def main():
     ap.add_argument("--pipeline-depth", type=int, default=32,
                     help="frames in flight (GPU analysis of frame k+1 overlaps host entropy coding of earlier frames)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batched", action="store_true", help="measure the batched 64-stream workload instead")
+    ap.add_argument("--batch-per-size", type=int, default=16)
+    ap.add_argument("--batch-p-frames", type=int, default=20)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 256 MiB device copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -143,6 +199,11 @@ def main():
         print(f"bench.py measures one GPU: every box here has a single MI355X; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is described in DESIGN.md, not measured).", file=sys.stderr)
         sys.exit(2)
+
+    if args.batched:
+        print(f"[bench] batched: {4 * args.batch_per_size} streams, 1 I + {args.batch_p_frames} P each", file=sys.stderr, flush=True)
+        print(json.dumps(run_batched(args.batch_per_size, args.batch_p_frames, max(2, min(args.pipeline_depth, 4)))))
+        return
 
     from riv2_amd import synth
     nframes = 1 + args.warmup + args.steps
