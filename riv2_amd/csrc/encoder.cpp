@@ -44,6 +44,7 @@ struct Slot {
     int bdx = 0, bdy = 0;
     double smark[RIV_NUM_STAGES] = {};
     uint32_t nmark[RIV_NUM_STAGES] = {};
+    double gpu_ms = 0;
 };
 
 class Pool {
@@ -377,6 +378,18 @@ static void compute_bias(const BlockRec* rec, int nb, int& bx_out, int& by_out) 
 // Host stage of one frame, part 1 (any order, any thread): residual rANS. The frame's GPU work
 // has finished. The products stay in the slot for the ordered tail.
 static void run_tails(riv_encoder* e, int si);
+// Device times of a finished frame, read once its events have completed (by the thread that
+// waited for them, so that the host-stage workers make no runtime calls).
+static void collect_gpu_times(Slot& s) {
+    float gpu_ms = 0;
+    s.gpu_ms = hipEventElapsedTime(&gpu_ms, s.t0, s.t1) == hipSuccess ? gpu_ms : 0.0;
+    for (int k = 0; k < RIV_NUM_STAGES; k++) { s.smark[k] = 0; s.nmark[k] = 0; }
+    for (int k = 1; k < s.n_marks; k++) {
+        float ms = 0;
+        if (s.mark_stage[k] >= 0 && hipEventElapsedTime(&ms, s.marks[k - 1], s.marks[k]) == hipSuccess) { s.smark[s.mark_stage[k]] += ms; s.nmark[s.mark_stage[k]]++; }
+    }
+}
+
 static void code_frame(riv_encoder* e, int si) {
     Slot& s = e->slots[si];
     const int nb = e->nb;
@@ -385,13 +398,7 @@ static void code_frame(riv_encoder* e, int si) {
     st = riv_frame_stats{};
     st.frame_index = s.index; st.timestamp_ms = s.ts;
     st.storage_width = (uint32_t)e->W; st.storage_height = (uint32_t)e->H; st.blocks_total = (uint32_t)nb;
-    float gpu_ms = 0;
-    if (hipEventElapsedTime(&gpu_ms, s.t0, s.t1) == hipSuccess) st.gpu_ms = gpu_ms;
-    for (int k = 0; k < RIV_NUM_STAGES; k++) { s.smark[k] = 0; s.nmark[k] = 0; }
-    for (int k = 1; k < s.n_marks; k++) {
-        float ms = 0;
-        if (s.mark_stage[k] >= 0 && hipEventElapsedTime(&ms, s.marks[k - 1], s.marks[k]) == hipSuccess) { s.smark[s.mark_stage[k]] += ms; s.nmark[s.mark_stage[k]]++; }
-    }
+    st.gpu_ms = s.gpu_ms;
     const uint8_t* pack = s.h_pack.as<uint8_t>();
     s.res.clear(); s.blocks.clear(); s.bdx = s.bdy = 0;
     try {
@@ -525,6 +532,7 @@ static void waiter_loop(HostStage* hs) {
             run_tails(e, si);
             continue;
         }
+        collect_gpu_times(e->slots[si]);
         hs->pool.submit([e, si] { code_frame(e, si); });
     }
 }
@@ -696,7 +704,7 @@ static int frame_end(riv_encoder* e, const FrameCtx& f, riv_frame_stats* stats) 
         return RIV_OK;
     }
     if (hipEventSynchronize(s.done) != hipSuccess) { e->fail(RIV_E_GPU, "GPU frame failed"); s.failed = true; run_tails(e, f.si); }
-    else code_frame(e, f.si);
+    else { collect_gpu_times(s); code_frame(e, f.si); }
     if (!f.intra) {
         int werr = 0;
         RIV_HIP(hipMemcpy(&werr, e->err.p, 4, hipMemcpyDeviceToHost));

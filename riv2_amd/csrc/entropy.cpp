@@ -26,6 +26,7 @@ struct RcpTable {
 const RcpTable g_rcp;
 inline uint32_t div_small(uint32_t x, uint32_t d) { return (uint32_t)(((unsigned __int128)x * g_rcp.m[d]) >> 40); }
 constexpr uint32_t kLow = 1u << 16;   // normalisation interval [2^16, 2^24)
+const bool g_avx2_rans = __builtin_cpu_supports("avx2");
 }  // namespace
 
 const uint32_t kP0Rcp[512] = {
@@ -37,7 +38,7 @@ const uint32_t kP0Rcp[512] = {
 };
 
 namespace {
-struct WriterBuffers { std::vector<uint16_t> buf, tmp[4]; };
+struct WriterBuffers { std::vector<uint16_t> buf, tmp[4], tmp8[8]; };
 thread_local std::vector<WriterBuffers> tl_pool;
 }  // namespace
 
@@ -46,6 +47,7 @@ RansWriter::RansWriter(std::vector<uint8_t>& out) : out_(out) {
         WriterBuffers& b = tl_pool.back();
         buf_.swap(b.buf);
         for (int j = 0; j < kLanes; j++) tmp_[j].swap(b.tmp[j]);
+        for (int j = 0; j < 8; j++) tmp8_[j].swap(b.tmp8[j]);
         tl_pool.pop_back();
         sym_ = buf_.data(); cap_ = buf_.size();
     }
@@ -56,6 +58,7 @@ RansWriter::~RansWriter() {
         WriterBuffers& b = tl_pool.back();
         b.buf.swap(buf_);
         for (int j = 0; j < kLanes; j++) b.tmp[j].swap(tmp_[j]);
+        for (int j = 0; j < 8; j++) b.tmp8[j].swap(tmp8_[j]);
     }
 }
 
@@ -94,6 +97,16 @@ struct BlockCoder {
 };
 }  // namespace
 
+// A block's output: state 1 then state 0 as two u16 words each, then the emitted words [wp, end).
+void RansWriter::emit_block(uint32_t r0, uint32_t r1, const uint16_t* wp, const uint16_t* end) {
+    const size_t ne = (size_t)(end - wp), o = out_.size();
+    out_.resize(o + 8 + 2 * ne);
+    uint8_t* d = out_.data() + o;
+    const uint16_t head[4] = {(uint16_t)r1, (uint16_t)(r1 >> 16), (uint16_t)r0, (uint16_t)(r0 >> 16)};
+    for (int i = 0; i < 4; i++) { d[2 * i] = (uint8_t)head[i]; d[2 * i + 1] = (uint8_t)(head[i] >> 8); }
+    for (size_t i = 0; i < ne; i++) { d[8 + 2 * i] = (uint8_t)wp[i]; d[9 + 2 * i] = (uint8_t)(wp[i] >> 8); }
+}
+
 void RansWriter::drain(const uint16_t* const* sym, const int* n, int nblk) {
     BlockCoder bc[kLanes];
     for (int j = 0; j < nblk; j++) {
@@ -108,20 +121,72 @@ void RansWriter::drain(const uint16_t* const* sym, const int* n, int nblk) {
     }
     for (int j = 0; j < nblk; j++)
         while (bc[j].k >= 1) bc[j].pair();
-    for (int j = 0; j < nblk; j++) {
-        const uint32_t r0 = bc[j].A, r1 = bc[j].B;
-        const uint16_t* wp = bc[j].wp;
-        const size_t ne = (size_t)(tmp_[j].data() + tmp_[j].size() - wp), o = out_.size();
-        out_.resize(o + 8 + 2 * ne);
-        uint8_t* d = out_.data() + o;
-        const uint16_t head[4] = {(uint16_t)r1, (uint16_t)(r1 >> 16), (uint16_t)r0, (uint16_t)(r0 >> 16)};
-        for (int i = 0; i < 4; i++) { d[2 * i] = (uint8_t)head[i]; d[2 * i + 1] = (uint8_t)(head[i] >> 8); }
-        for (size_t i = 0; i < ne; i++) { d[8 + 2 * i] = (uint8_t)wp[i]; d[9 + 2 * i] = (uint8_t)(wp[i] >> 8); }
-    }
+    for (int j = 0; j < nblk; j++) emit_block(bc[j].A, bc[j].B, bc[j].wp, tmp_[j].data() + tmp_[j].size());
 }
+
+namespace {
+// Eight full blocks (16386 symbols, an even count, so every step codes one pair per block) with
+// AVX2, 16 state chains in two vectors: lane 2j holds block j's B state (the lower symbol of each
+// pair), lane 2j+1 its A state. q = x / freq is a double-precision division: x < 2^32 and
+// freq <= 255 convert exactly, the quotient is below 2^24, and a correctly rounded quotient of
+// x/freq never crosses an integer (a non-integer x/freq lies at least 1/freq below the next one).
+// Emitted words go below wp[j], block by block, A before B within a step (the scalar order).
+__attribute__((target("avx2"))) void drain8_avx2(const uint16_t* const* sym, int n, uint16_t** wp, uint32_t* r0, uint32_t* r1) {
+    const __m256i sign = _mm256_set1_epi32((int)0x80000000u), ff = _mm256_set1_epi32(0xff), c256 = _mm256_set1_epi32(256);
+    const __m256d two31 = _mm256_set1_pd(2147483648.0);
+    __m256i X[2] = {_mm256_set1_epi32((int)kLow), _mm256_set1_epi32((int)kLow)};
+    alignas(32) uint32_t xs[16];
+    for (int k = n - 2; k >= 0; k -= 2) {   // pair (k+1, k): A codes k+1, B codes k
+        uint32_t w[8];
+        for (int j = 0; j < 8; j++) std::memcpy(&w[j], sym[j] + k, 4);   // low half: symbol k (B), high: k+1 (A)
+#pragma GCC unroll 2
+        for (int h = 0; h < 2; h++) {
+            const __m256i S = _mm256_cvtepu16_epi32(_mm_set_epi32((int)w[4 * h + 3], (int)w[4 * h + 2], (int)w[4 * h + 1], (int)w[4 * h]));
+            const __m256i freq = _mm256_and_si256(S, ff), start = _mm256_srli_epi32(S, 8);
+            // renormalise where x >= freq << 24 (unsigned compare through the sign flip)
+            const __m256i e = _mm256_xor_si256(_mm256_cmpgt_epi32(_mm256_xor_si256(_mm256_slli_epi32(freq, 24), sign),
+                                                                  _mm256_xor_si256(X[h], sign)), _mm256_set1_epi32(-1));
+            const int m = _mm256_movemask_ps(_mm256_castsi256_ps(e));
+            if (m) {
+                _mm256_store_si256((__m256i*)(xs + 8 * h), X[h]);
+                for (int j = 0; j < 4; j++) {
+                    if (m & (2 << (2 * j))) *--wp[4 * h + j] = (uint16_t)xs[8 * h + 2 * j + 1];   // A first
+                    if (m & (1 << (2 * j))) *--wp[4 * h + j] = (uint16_t)xs[8 * h + 2 * j];
+                }
+            }
+            const __m256i x = _mm256_blendv_epi8(X[h], _mm256_srli_epi32(X[h], 16), e);
+            const __m256i xb = _mm256_xor_si256(x, sign);
+            const __m256d xl = _mm256_add_pd(_mm256_cvtepi32_pd(_mm256_castsi256_si128(xb)), two31);
+            const __m256d xh = _mm256_add_pd(_mm256_cvtepi32_pd(_mm256_extracti128_si256(xb, 1)), two31);
+            const __m256d fl = _mm256_cvtepi32_pd(_mm256_castsi256_si128(freq)), fh = _mm256_cvtepi32_pd(_mm256_extracti128_si256(freq, 1));
+            const __m256i q = _mm256_set_m128i(_mm256_cvttpd_epi32(_mm256_div_pd(xh, fh)), _mm256_cvttpd_epi32(_mm256_div_pd(xl, fl)));
+            X[h] = _mm256_add_epi32(_mm256_add_epi32(x, _mm256_mullo_epi32(q, _mm256_sub_epi32(c256, freq))), start);
+        }
+    }
+    _mm256_store_si256((__m256i*)xs, X[0]);
+    _mm256_store_si256((__m256i*)(xs + 8), X[1]);
+    for (int j = 0; j < 8; j++) { r1[j] = xs[2 * j]; r0[j] = xs[2 * j + 1]; }
+}
+}  // namespace
 
 void RansWriter::flush() {
     size_t b = 0;
+    if (g_avx2_rans) {   // groups of eight full blocks, then the rest below
+        while (n_ - b >= 8 * (size_t)kBatch) {
+            const uint16_t* s8[8];
+            uint16_t* wp[8];
+            uint32_t r0[8], r1[8];
+            for (int j = 0; j < 8; j++) {
+                s8[j] = sym_ + b + (size_t)j * kBatch;
+                if (tmp8_[j].size() < (size_t)kBatch + 8) tmp8_[j].resize((size_t)kBatch + 8);
+                wp[j] = tmp8_[j].data() + tmp8_[j].size();
+            }
+            drain8_avx2(s8, kBatch, wp, r0, r1);
+            for (int j = 0; j < 8; j++) emit_block(r0[j], r1[j], wp[j], tmp8_[j].data() + tmp8_[j].size());
+            b += 8 * (size_t)kBatch;
+        }
+        if (b == n_ && b > 0) { n_ = 0; return; }
+    }
     do {   // an empty stream still carries its two initial states
         const uint16_t* s[kLanes];
         int n[kLanes], nblk = 0;
@@ -441,6 +506,8 @@ int MvCoder::lookup(int8_t dx, int8_t dy, int8_t sx, int8_t sy) const {
 // 7/8, count this frame's NEW (delta, sub-pel) tuples, drop zero counts; below 100 NEW blocks
 // of history the list is empty, else the top 16 by count (ties by key) above total/100.
 void MvCoder::refresh_history(const std::vector<MvBlock>& blocks, bool from_decoder) {
+    // counts are kept in a hash map on a 32-bit key whose unsigned order is the tuple order
+    auto key_of = [](int dx, int dy, int rx, int ry) { return ((uint32_t)(dx + 128) << 24) | ((uint32_t)(dy + 128) << 16) | ((uint32_t)rx << 8) | (uint32_t)ry; };
     for (auto& e : counts_) e.second = e.second * 7 / 8;
     total_ = total_ * 7 / 8;
     for (const MvBlock& b : blocks) {
@@ -449,24 +516,21 @@ void MvCoder::refresh_history(const std::vector<MvBlock>& blocks, bool from_deco
             if (from_decoder) continue;
             throw CorruptStream("-0.5 sub-pel in a NEW vector");
         }
-        std::array<int, 4> k = {b.dx, b.dy, subpel_rank(b.sx), subpel_rank(b.sy)};
-        bool found = false;
-        for (auto& e : counts_) if (e.first == k) { e.second++; found = true; break; }
-        if (!found) counts_.push_back({k, 1});
+        counts_[key_of(b.dx, b.dy, subpel_rank(b.sx), subpel_rank(b.sy))]++;
         total_++;
     }
-    counts_.erase(std::remove_if(counts_.begin(), counts_.end(), [](const std::pair<std::array<int, 4>, uint32_t>& e) { return e.second == 0; }),
-                  counts_.end());
+    for (auto it = counts_.begin(); it != counts_.end();) it = it->second == 0 ? counts_.erase(it) : std::next(it);
     cand_.clear();
     if (total_ < 100) return;
-    std::vector<std::pair<std::array<int, 4>, uint32_t>> order = counts_;
-    std::sort(order.begin(), order.end(), [](const std::pair<std::array<int, 4>, uint32_t>& a, const std::pair<std::array<int, 4>, uint32_t>& b) {
-        return a.second != b.second ? a.second > b.second : a.first < b.first;
-    });
     const uint32_t floor_count = total_ / 100;
-    for (auto& e : order) {
-        if (e.second <= floor_count) continue;
-        cand_.push_back(e.first);
+    std::vector<std::pair<uint32_t, uint32_t>> order;   // (count, key) above the floor
+    for (const auto& e : counts_) if (e.second > floor_count) order.push_back({e.second, e.first});
+    std::sort(order.begin(), order.end(), [](const std::pair<uint32_t, uint32_t>& a, const std::pair<uint32_t, uint32_t>& b) {
+        return a.first != b.first ? a.first > b.first : a.second < b.second;
+    });
+    for (const auto& e : order) {
+        const uint32_t k = e.second;
+        cand_.push_back({(int)(k >> 24) - 128, (int)((k >> 16) & 0xff) - 128, (int)((k >> 8) & 0xff), (int)(k & 0xff)});
         if (cand_.size() == 16) break;
     }
 }

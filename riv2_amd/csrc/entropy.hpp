@@ -6,6 +6,7 @@
 #include <cstddef>
 #include <vector>
 #include <array>
+#include <unordered_map>
 #include <stdexcept>
 
 namespace rivh {
@@ -64,12 +65,14 @@ private:
     }
     void grow();
     void drain(const uint16_t* const* sym, const int* n, int nblk);
+    void emit_block(uint32_t r0, uint32_t r1, const uint16_t* wp, const uint16_t* end);
     static constexpr int kBatch = 16386;
     static constexpr int kLanes = 4;   // blocks coded together
     std::vector<uint8_t>& out_;
     // buffers come from a per-thread pool, so a worker coding frame after frame reuses warm memory
     std::vector<uint16_t> buf_;
     std::vector<uint16_t> tmp_[kLanes];   // emitted words of the blocks being coded, filled from the end
+    std::vector<uint16_t> tmp8_[8];       // the same for the vector coder's eight blocks
     uint16_t* sym_ = nullptr;
     size_t n_ = 0, cap_ = 0;
 };
@@ -129,7 +132,7 @@ private:
     Ctx hit_[3], idx_[4];
     // candidate history: key = (dx, dy, sub-pel order x, sub-pel order y)
     std::vector<std::array<int, 4>> cand_;
-    std::vector<std::pair<std::array<int, 4>, uint32_t>> counts_;
+    std::unordered_map<uint32_t, uint32_t> counts_;   // key: (dx+128, dy+128, sub-pel order x, y) bytes
     uint32_t total_ = 0;
 };
 
