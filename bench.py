@@ -25,7 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-ME_KERNELS = ["k_me_search", "k_mode"]   # the motion-search stage, bracketed by two events per frame
+ME_KERNELS = ["k_me_search"]   # the motion-search stage (search passes and mode decision), one launch per frame
 W, H = 1920, 1080
 SW, SH = 1920, 1088
 

@@ -24,7 +24,7 @@ using riv::BlockRec;
 namespace {
 
 struct Slot {
-    DevBuf rec, optskip;                 // device outputs of the frame (rec also seeds the next frame's temporal MVs)
+    DevBuf rec;                          // the frame's block records (they also seed the next frame's temporal MVs)
     HostBuf h_rec, h_pack, h_optskip;    // mapped host memory the kernels write for the host entropy stage
     HostBuf h_passes;                    // mapped: the motion search's pass count
     hipEvent_t done = nullptr;           // the frame's kernels (and so its host-side outputs) finished
@@ -36,8 +36,12 @@ struct Slot {
     bool intra = false;
     bool busy = false;       // holds a frame that is not yet written
     // host-stage products of the frame, passed from its coding job to the ordered tail
-    bool coded = false;      // the coding job finished (products below are valid unless failed)
+    bool coded = false;      // the residual coding finished (products below are valid unless failed)
+    bool modeled = false;    // the ordered MV modelling finished
+    bool packed = false;     // the frame's bytes are assembled (or it failed)
     bool failed = false;
+    std::vector<uint8_t> mvb, bytes;                 // MV payload, whole frame record
+    std::unique_ptr<rivh::RansWriter> mvw;           // the MV symbols between modelling and packing
     riv_frame_stats st{};
     std::vector<uint8_t> res;
     std::vector<rivh::MvBlock> blocks;
@@ -117,7 +121,7 @@ struct riv_encoder {
     int W = 0, H = 0, bw = 0, bh = 0, nb = 0;
     int s_waves = 1, s_groups = 1, s_maxp = 0, s_bank = 0;   // motion-search launch shape, pass limit, counter bank
     hipStream_t st = nullptr;    // kernels, frame after frame
-    DevBuf rgb, cur, pred, ref[2], var, lnv, steps, mask, field, arrive, smv, bits, rdo, err, yt, ct;
+    DevBuf rgb, cur, pred, ref[2], var, lnv, steps, mask, field, arrive, bits, rdo, err, yt, ct;
     int ref_idx = 0;
     bool have_ref = false;
     int prev_inter_slot = -1;      // slot holding the previous P-frame's records, -1 after an intra frame
@@ -140,9 +144,13 @@ struct riv_encoder {
     // ordered tail + completion state
     std::mutex mu;
     std::condition_variable cv;
-    uint64_t next_tail = 0;        // next frame index allowed into the ordered tail
+    // ordered stages of the host tail: MV modelling (its contexts carry from frame to frame), then,
+    // after the packing that any thread may do, the write
+    uint64_t next_tail = 0;        // next frame for MV modelling
+    uint64_t next_write = 0;       // next frame to write
     uint64_t completed = 0;        // frames fully written
-    bool tail_active = false;      // a thread is running ready tails in frame order
+    bool tail_active = false;      // a thread is modelling ready frames in order
+    bool write_active = false;     // a thread is writing packed frames in order
     int err_code = 0;
     std::string err_msg;
     rivh::MvCoder mvc;
@@ -241,12 +249,11 @@ static int encoder_init(riv_encoder* e) {
         const size_t bb = 2 * sizeof(unsigned) * (size_t)riv::barrier_words(e->s_maxp);
         RIV_HIP(e->arrive.alloc(bb)); RIV_HIP(hipMemset(e->arrive.p, 0, bb));
     }
-    RIV_HIP(e->smv.alloc(8 * (size_t)e->nb));
     RIV_HIP(e->err.alloc(4)); RIV_HIP(hipMemset(e->err.p, 0, 4));
     e->depth = c.pipeline_depth;
     e->slots.resize(std::max(2, e->depth));
     for (Slot& s : e->slots) {
-        RIV_HIP(s.rec.alloc(sizeof(BlockRec) * e->nb)); RIV_HIP(s.optskip.alloc(e->nb));
+        RIV_HIP(s.rec.alloc(sizeof(BlockRec) * e->nb));
         RIV_HIP(s.h_rec.alloc(sizeof(BlockRec) * e->nb)); RIV_HIP(s.h_pack.alloc((size_t)riv::PACK_STRIDE * e->nb));
         RIV_HIP(s.h_optskip.alloc(e->nb)); RIV_HIP(s.h_passes.alloc(16));
         RIV_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming | hipEventBlockingSync));   // waiting workers sleep instead of spinning
@@ -447,41 +454,61 @@ static void code_frame(riv_encoder* e, int si) {
     run_tails(e, si);
 }
 
-// Host stage, part 2 (frame order, one thread at a time): MV rANS (contexts carry across frames),
-// packing (format.rs:171-205) and writing. Returns a status.
-static int tail_frame(riv_encoder* e, Slot& s) {
-    const riv_encoder_config& c = e->cfg;
-    const auto t_tail = std::chrono::steady_clock::now();
-    std::vector<uint8_t> bytes;
+// Host stage, part 2 (frame order, one thread at a time): the MV symbols and history update
+// (mv_rans/encoder.rs; contexts and history carry across frames, and restart at I-frames).
+static int model_frame(riv_encoder* e, Slot& s) {
     try {
         if (s.intra) {
             e->mvc.reset();   // MV contexts restart at every I-frame (encoder.rs:335-337)
-            put_u64(bytes, s.ts); bytes.push_back(1); bytes.push_back(c.intra_quality);
-            put_u32(bytes, (uint32_t)s.res.size());
-            bytes.insert(bytes.end(), s.res.begin(), s.res.end());
-            s.st.residual_bytes = (uint32_t)s.res.size();
         } else {
-            std::vector<uint8_t> mvb = e->mvc.encode(s.blocks, e->bw, e->bh);
-            bytes.reserve(20 + mvb.size() + s.res.size());
-            put_u64(bytes, s.ts); bytes.push_back(2); bytes.push_back(c.inter_quality);
-            bytes.push_back((uint8_t)(int8_t)s.bdx); bytes.push_back((uint8_t)(int8_t)s.bdy); bytes.push_back(0);
-            put_u32(bytes, (uint32_t)mvb.size()); bytes.insert(bytes.end(), mvb.begin(), mvb.end());
-            put_u32(bytes, (uint32_t)s.res.size()); bytes.insert(bytes.end(), s.res.begin(), s.res.end());
-            s.st.mv_bytes = (uint32_t)mvb.size(); s.st.residual_bytes = (uint32_t)s.res.size();
+            s.mvb.clear();
+            s.mvw.reset(new rivh::RansWriter(s.mvb));
+            e->mvc.model(s.blocks, e->bw, e->bh, *s.mvw);
         }
     } catch (const std::exception& ex) {
         e->fail(RIV_E_ENCODING_FAILED, std::string("MV coding failed: ") + ex.what());
         return RIV_E_ENCODING_FAILED;
     }
-    const int r = e->write(bytes.data(), bytes.size());
-    if (r) { e->fail(r, "writer failed"); return r; }
-    s.st.total_bytes = (uint32_t)bytes.size();
-    s.st.host_tail_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_tail).count();
     return RIV_OK;
 }
 
-// Marks slot si coded, then, unless another thread is already doing so, runs every ready tail in
-// frame order. The tails run outside the lock; only the thread that set tail_active runs them.
+// Part 3 (any order, any thread): the MV rANS bytes and the frame record (format.rs:171-205).
+static void pack_frame(riv_encoder* e, Slot& s) {
+    const riv_encoder_config& c = e->cfg;
+    std::vector<uint8_t>& bytes = s.bytes;
+    bytes.clear();
+    if (s.intra) {
+        bytes.reserve(14 + s.res.size());
+        put_u64(bytes, s.ts); bytes.push_back(1); bytes.push_back(c.intra_quality);
+        put_u32(bytes, (uint32_t)s.res.size());
+        bytes.insert(bytes.end(), s.res.begin(), s.res.end());
+        s.st.residual_bytes = (uint32_t)s.res.size();
+    } else {
+        s.mvw->flush();
+        s.mvw.reset();
+        bytes.reserve(21 + s.mvb.size() + s.res.size());
+        put_u64(bytes, s.ts); bytes.push_back(2); bytes.push_back(c.inter_quality);
+        bytes.push_back((uint8_t)(int8_t)s.bdx); bytes.push_back((uint8_t)(int8_t)s.bdy); bytes.push_back(0);
+        put_u32(bytes, (uint32_t)s.mvb.size()); bytes.insert(bytes.end(), s.mvb.begin(), s.mvb.end());
+        put_u32(bytes, (uint32_t)s.res.size()); bytes.insert(bytes.end(), s.res.begin(), s.res.end());
+        s.st.mv_bytes = (uint32_t)s.mvb.size(); s.st.residual_bytes = (uint32_t)s.res.size();
+    }
+    s.st.total_bytes = (uint32_t)bytes.size();
+}
+
+static void run_writes(riv_encoder* e, int si);
+
+// Packs slot si (on the pool when pipelined), then runs the ordered writes.
+static void pack_then_write(riv_encoder* e, int si) {
+    Slot& s = e->slots[si];
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!s.failed) pack_frame(e, s);
+    s.st.host_tail_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    run_writes(e, si);
+}
+
+// Marks slot si coded, then, unless another thread is already doing so, models every ready frame
+// in frame order; each modelled frame goes on to packing. The modelling runs outside the lock.
 static void run_tails(riv_encoder* e, int si) {
     std::unique_lock<std::mutex> lk(e->mu);
     e->slots[si].coded = true;
@@ -494,9 +521,47 @@ static void run_tails(riv_encoder* e, int si) {
         t.coded = false;
         const bool skip = t.failed || e->err_code;
         lk.unlock();
-        const int r = skip ? RIV_E_ENCODING_FAILED : tail_frame(e, t);
+        const auto t0 = std::chrono::steady_clock::now();
+        if (!skip && model_frame(e, t) != RIV_OK) t.failed = true;
+        if (skip) t.failed = true;
+        t.st.host_tail_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         lk.lock();
-        if (r == RIV_OK) {
+        t.modeled = true;
+        e->next_tail++;
+        if (e->hs) {
+            lk.unlock();
+            e->hs->pool.submit([e, ti] { pack_then_write(e, ti); });
+            lk.lock();
+        } else {
+            lk.unlock();
+            pack_then_write(e, ti);
+            lk.lock();
+        }
+    }
+    e->tail_active = false;
+}
+
+// Marks slot si packed, then, unless another thread is already doing so, writes every packed
+// frame in frame order.
+static void run_writes(riv_encoder* e, int si) {
+    std::unique_lock<std::mutex> lk(e->mu);
+    e->slots[si].packed = true;
+    if (e->write_active) return;
+    e->write_active = true;
+    for (;;) {
+        const int ti = (int)(e->next_write % e->slots.size());
+        Slot& t = e->slots[ti];
+        if (!t.busy || !t.packed || t.index != e->next_write) break;
+        t.packed = false;
+        const bool ok = !t.failed && !e->err_code;
+        lk.unlock();
+        int r = RIV_OK;
+        if (ok) {
+            r = e->write(t.bytes.data(), t.bytes.size());
+            if (r) e->fail(r, "writer failed");
+        }
+        lk.lock();
+        if (ok && r == RIV_OK) {
             if (e->profiling)
                 for (int k = 0; k < RIV_NUM_STAGES; k++) { e->stage_ms[k] += t.smark[k]; e->stage_n[k] += t.nmark[k]; }
             e->done_stats.push_back(t.st);
@@ -504,12 +569,12 @@ static void run_tails(riv_encoder* e, int si) {
             e->hdr.frame_count = t.index + 1;
             e->last_slot = ti;
         }
-        e->next_tail++;
+        e->next_write++;
         e->completed++;
         t.busy = false;
         e->cv.notify_all();
     }
-    e->tail_active = false;
+    e->write_active = false;
 }
 
 // The GPU-completion thread of the host stage.
@@ -605,7 +670,8 @@ static int frame_begin(riv_encoder* e, const void* rgb, bool on_device, size_t l
         std::unique_lock<std::mutex> lk(e->mu);
         e->cv.wait(lk, [&] { return !s.busy || e->err_code; });
         if (e->err_code) { set_error(e->err_msg); return e->err_code; }
-        s.index = e->frame_count; s.ts = ts; s.intra = intra; s.busy = true; s.coded = false; s.failed = false;
+        s.index = e->frame_count; s.ts = ts; s.intra = intra; s.busy = true; s.coded = false; s.modeled = false; s.packed = false;
+        s.failed = false;
     }
     s.n_marks = 0;
     if (e->profiling && !s.marks[0])
@@ -642,7 +708,9 @@ static int frame_begin(riv_encoder* e, const void* rgb, bool on_device, size_t l
 static riv::SearchStream frame_search_stream(riv_encoder* e, const FrameCtx& f, int blk0) {
     riv::SearchStream S{};
     S.cur = f.cY; S.ref = f.rY; S.prev_mv = f.prev_mv; S.prev_stride = sizeof(BlockRec);
-    S.smv = e->smv.as<unsigned long long>(); S.passes = e->slots[f.si].h_passes.dev<int>();
+    Slot& s = e->slots[f.si];
+    S.rec = s.rec.as<BlockRec>(); S.host_rec = s.h_rec.dev<BlockRec>(); S.passes = s.h_passes.dev<int>();
+    S.rdo_bits = e->rdo.as<uint32_t>(); S.rdo_max = e->rdo_max;
     S.W = e->W; S.H = e->H; S.bw = e->bw; S.bh = e->bh; S.blk0 = blk0;
     return S;
 }
@@ -664,21 +732,15 @@ static int frame_after_search(riv_encoder* e, const FrameCtx& f) {
     const riv_encoder_config& c = e->cfg;
     Slot& s = e->slots[f.si];
     hipStream_t st = f.st;
-    BlockRec* rec = s.rec.as<BlockRec>();
-    riv::ModeParams mp{};
-    mp.bw = e->bw; mp.bh = e->bh; mp.smv = e->smv.as<unsigned long long>(); mp.prev_mv = f.prev_mv; mp.prev_stride = sizeof(BlockRec);
-    mp.rdo_bits = e->rdo.as<uint32_t>(); mp.rdo_max = e->rdo_max; mp.rec = rec; mp.host_rec = s.h_rec.dev<BlockRec>();
-    mp.err = e->err.as<int>();
-    rivk::mode(st, mp);
+    BlockRec* rec = s.rec.as<BlockRec>();   // written by the search's mode-decision epilogue
     mark(e, s, st, 1);
     rivk::mc(st, f.rY, f.rU, f.rV, e->W, e->H, (const int8_t*)rec, sizeof(BlockRec), f.pY, f.pU, f.pV, e->var.as<uint32_t>(), e->lnv.as<float>());
     mark(e, s, st, 2);
     rivk::aq(st, e->nb, e->inter_qs, e->lnv.as<float>(), e->steps.as<float>());
     mark(e, s, st, 3);
     rivk::inter_residual(st, f.cY, f.cU, f.cV, f.pY, f.pU, f.pV, e->W, e->H, rec, e->steps.as<float>(), c.inter_dead_zone, f.nY, f.nU, f.nV,
-                         s.h_pack.dev<uint8_t>(), s.optskip.as<uint8_t>(), s.h_optskip.dev<uint8_t>());
+                         s.h_pack.dev<uint8_t>(), e->mask.as<uint8_t>(), s.h_optskip.dev<uint8_t>());
     mark(e, s, st, 4);
-    rivk::filter_mask(st, rec, s.optskip.as<uint8_t>(), e->nb, e->mask.as<uint8_t>());
     rivk::deblock(st, f.nY, f.nU, f.nV, e->W, e->H, deblock_level(e->inter_qs), false, e->mask.as<uint8_t>());
     mark(e, s, st, 5);
     RIV_HIP(hipGetLastError());

@@ -536,9 +536,15 @@ void MvCoder::refresh_history(const std::vector<MvBlock>& blocks, bool from_deco
 }
 
 std::vector<uint8_t> MvCoder::encode(const std::vector<MvBlock>& blocks, int bw, int bh) {
-    (void)bh;
     std::vector<uint8_t> out;
     RansWriter w(out);
+    model(blocks, bw, bh, w);
+    w.flush();
+    return out;
+}
+
+void MvCoder::model(const std::vector<MvBlock>& blocks, int bw, int bh, RansWriter& w) {
+    (void)bh;
     std::vector<uint8_t> modes;
     std::vector<uint8_t> hits;
     modes.reserve(blocks.size()); hits.reserve(blocks.size());
@@ -596,9 +602,7 @@ std::vector<uint8_t> MvCoder::encode(const std::vector<MvBlock>& blocks, int bw,
         hits.push_back(hit);
         modes.push_back(b.mode);
     }
-    w.flush();
     refresh_history(blocks, false);
-    return out;
 }
 
 std::vector<MvBlock> MvCoder::decode(const uint8_t* data, size_t len, int bw, int bh) {

@@ -120,6 +120,9 @@ class MvCoder {
 public:
     void reset();
     std::vector<uint8_t> encode(const std::vector<MvBlock>& blocks, int bw, int bh);
+    // The frame's symbols into w (not flushed) and the history update: the part of encode that
+    // depends on the previous frames; w.flush() then yields the bytes encode returns.
+    void model(const std::vector<MvBlock>& blocks, int bw, int bh, RansWriter& w);
     std::vector<MvBlock> decode(const uint8_t* data, size_t len, int bw, int bh);
 private:
     void refresh_history(const std::vector<MvBlock>& blocks, bool from_decoder);

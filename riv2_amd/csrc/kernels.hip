@@ -7,7 +7,6 @@
 
 using namespace rivdev;
 using riv::BlockRec;
-using riv::ModeParams;
 using riv::SearchParams;
 using riv::SearchStream;
 
@@ -291,6 +290,70 @@ __device__ __forceinline__ int mv_class(int mag) {   // mv_class_from_magnitude 
 }
 
 }  // namespace
+
+// ============================================================================================
+// MV mode decision
+// ============================================================================================
+// Mode decision on the reconstructed prefix (encoder.rs:447-733), per MB, run by the motion
+// search's epilogue on the final field. It needs the reconstructed MVs of the left, top,
+// top-right and top-left MBs and of the co-located MB of the previous P-frame (derive_mv_predictors_with_stats / gather_mv_candidates,
+// mv_predictor.rs:135-272). A reconstructed MV equals the search MV: for a matched mode it is
+// the matched candidate, which equals the search MV; for NEW it is base + clamp(search - base),
+// and with every MV inside [-sr-1, sr] (sr <= 32) the difference never reaches the clamp. So
+// the whole frame's decisions follow from the search field in parallel. The identity is still
+// checked for every NEW block: a violation sets *err = 2.
+__device__ __forceinline__ uint4 mode_record(uint32_t wS, int best_sad, bool has_l, bool has_t, bool has_tr, bool has_tl, bool has_tp,
+                                             uint32_t wL, uint32_t wT, uint32_t wTR, uint32_t wTL, uint32_t wP,
+                                             const uint32_t* rdo_bits, int rdo_max, int* err) {
+    // deduplicated (dx,dy,sx,sy) of left, top, top-right, top-left, temporal; the first two are
+    // Nearest / Near, padded with zero. Every MV word is canonical (sx, sy in {0,1}), so equal
+    // half-pel positions are equal words.
+    uint32_t c0 = 0, c1 = 0; int ncand = 0;
+#define RIV_ADDC(present, q)                                                         \
+    {                                                                        \
+        const uint32_t q_ = (q);                                             \
+        const bool add_ = (present) && !(ncand >= 1 && c0 == q_) && !(ncand >= 2 && c1 == q_); \
+        c1 = (add_ && ncand == 1) ? q_ : c1;                                 \
+        c0 = (add_ && ncand == 0) ? q_ : c0;                                 \
+        ncand += add_ ? 1 : 0;                                               \
+    }
+    RIV_ADDC(has_l, wL) RIV_ADDC(has_t, wT) RIV_ADDC(has_tr, wTR) RIV_ADDC(has_tl, wTL) RIV_ADDC(has_tp, wP)
+#undef RIV_ADDC
+    const uint32_t nearest = ncand > 0 ? c0 : 0u, nearp = ncand > 1 ? c1 : 0u, temporal = has_tp ? wP : 0u;
+    const int S0 = mvx(wS), S1 = mvy(wS), S2 = (wS >> 16) & 0xff, S3 = wS >> 24;
+    uint8_t mode;
+    if (wS == 0u) mode = riv::M_ZERO;
+    else if (wS == nearest) mode = riv::M_NEAREST;
+    else if (wS == nearp) mode = riv::M_NEAR;
+    else if (has_tr && wS == wTR) mode = riv::M_TOPRIGHT;
+    else if (has_tl && wS == wTL) mode = riv::M_TOPLEFT;
+    else if (wS == temporal) mode = riv::M_TEMPORAL;
+    else mode = riv::M_NEW;
+    uint8_t chosen = 0; int rdx = 0, rdy = 0;
+    if (mode == riv::M_NEW) {   // NEW base by estimated bits (encoder.rs:567-661, :128-159): the first cheapest
+        const uint32_t bases[5] = {nearest, nearp, wTR, wTL, temporal};
+        const bool have[5] = {true, true, has_tr, has_tl, true};
+        unsigned best = 0xffffffffu;
+        int bbx = mvx(nearest), bby = mvy(nearest);
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            if (have[k]) {
+                const int kx = mvx(bases[k]), ky = mvy(bases[k]);
+                const int cxs = mv_class(abs(dclamp(S0 - kx, -128, 127))), cys = mv_class(abs(dclamp(S1 - ky, -128, 127)));
+                const unsigned cost = (k < 4 ? k + 1 : 4) + (cxs == 0 ? 4u : 4u + cxs) + (cys == 0 ? 4u : 4u + cys);
+                if (cost < best) { best = cost; chosen = (uint8_t)k; bbx = kx; bby = ky; }
+            }
+        }
+        rdx = dclamp(S0 - bbx, -128, 127);
+        rdy = dclamp(S1 - bby, -128, 127);
+        if (dclamp(bbx + rdx, -128, 127) != S0 || dclamp(bby + rdy, -128, 127) != S1) atomicExch(err, 2);
+    }
+    // RdoContext::decide via the host-built table (rdo.rs:140-178)
+    uint32_t rdo = 0;
+    if (S2 == 0 && S3 == 0 && best_sad <= rdo_max) rdo = (rdo_bits[best_sad >> 5] >> (best_sad & 31)) & 1u;
+    const uint32_t w2 = (uint32_t)mode | ((uint32_t)chosen << 8) | ((uint32_t)(uint8_t)rdx << 16) | ((uint32_t)(uint8_t)rdy << 24);
+    return make_uint4(wS, wS, w2, (uint32_t)best_sad | (rdo << 16));
+}
 
 // ============================================================================================
 // Motion search by relaxation passes
@@ -697,84 +760,24 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
         if (threadIdx.x == 0) __hip_atomic_store(p.err, outcome < 0 ? 1 : 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
-    if (valid) sd[ms].smv[mloc] = (unsigned long long)myw | ((unsigned long long)(uint32_t)mysad << 32);
+    // the mode decision, on the final field (every block stored its word in the last pass, which
+    // completed before the last barrier)
+    if (valid) {
+        const SearchStream& S = sd[ms];
+        const uint32_t* fin = p.field + (size_t)((pass + 1) & 1) * nb;
+        const bool has_l = mbx > 0, has_t = mby > 0, has_tr = has_t && mbx + 1 < mbw, has_tl = has_t && mbx > 0;
+        const uint32_t wL = has_l ? field_load(fin + myb - 1) & 0x7fffffffu : 0u;
+        const uint32_t wT = has_t ? field_load(fin + myb - mbw) & 0x7fffffffu : 0u;
+        const uint32_t wTR = has_tr ? field_load(fin + myb - mbw + 1) & 0x7fffffffu : 0u;
+        const uint32_t wTL = has_tl ? field_load(fin + myb - mbw - 1) & 0x7fffffffu : 0u;
+        const uint4 r = mode_record(myw, mysad, has_l, has_t, has_tr, has_tl, S.prev_mv != nullptr, wL, wT, wTR, wTL, mypT,
+                                    S.rdo_bits, S.rdo_max, p.err);
+        *(uint4*)&S.rec[mloc] = r;
+        *(uint4*)&S.host_rec[mloc] = r;
+    }
     if (blockIdx.x == 0)
         for (int i = threadIdx.x; i < NS; i += blockDim.x)
             if (sd[i].passes) *sd[i].passes = pass + 1;
-}
-
-// ============================================================================================
-// MV mode decision
-// ============================================================================================
-// Mode decision on the reconstructed prefix (encoder.rs:447-733), one thread per MB. It needs
-// the reconstructed MVs of the left, top, top-right and top-left MBs and of the co-located MB
-// of the previous P-frame (derive_mv_predictors_with_stats / gather_mv_candidates,
-// mv_predictor.rs:135-272). A reconstructed MV equals the search MV: for a matched mode it is
-// the matched candidate, which equals the search MV; for NEW it is base + clamp(search - base),
-// and with every MV inside [-sr-1, sr] (sr <= 32) the difference never reaches the clamp. So
-// the whole frame's decisions follow from the search field in parallel. The identity is still
-// checked for every NEW block: a violation sets *err = 2.
-__global__ __launch_bounds__(256) void k_mode(ModeParams p) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x, bw = p.bw;
-    if (idx >= bw * p.bh) return;
-    const int bx = idx % bw, by = idx / bw;
-    const bool has_l = bx > 0, has_t = by > 0, has_tr = has_t && bx + 1 < bw, has_tl = has_t && bx > 0, has_tp = p.prev_mv != nullptr;
-    const unsigned long long me = p.smv[idx];
-    const uint32_t wS = (uint32_t)me;
-    const int best_sad = (int)(me >> 32);
-    const uint32_t wL = has_l ? (uint32_t)p.smv[idx - 1] : 0u, wT = has_t ? (uint32_t)p.smv[idx - bw] : 0u;
-    const uint32_t wTR = has_tr ? (uint32_t)p.smv[idx - bw + 1] : 0u, wTL = has_tl ? (uint32_t)p.smv[idx - bw - 1] : 0u;
-    const uint32_t wP = has_tp ? *(const uint32_t*)(p.prev_mv + (size_t)p.prev_stride * idx) : 0u;
-    // deduplicated (dx,dy,sx,sy) of left, top, top-right, top-left, temporal; the first two are
-    // Nearest / Near, padded with zero. Every MV word is canonical (sx, sy in {0,1}), so equal
-    // half-pel positions are equal words.
-    uint32_t c0 = 0, c1 = 0; int ncand = 0;
-#define RIV_ADDC(present, q)                                                         \
-    {                                                                        \
-        const uint32_t q_ = (q);                                             \
-        const bool add_ = (present) && !(ncand >= 1 && c0 == q_) && !(ncand >= 2 && c1 == q_); \
-        c1 = (add_ && ncand == 1) ? q_ : c1;                                 \
-        c0 = (add_ && ncand == 0) ? q_ : c0;                                 \
-        ncand += add_ ? 1 : 0;                                               \
-    }
-    RIV_ADDC(has_l, wL) RIV_ADDC(has_t, wT) RIV_ADDC(has_tr, wTR) RIV_ADDC(has_tl, wTL) RIV_ADDC(has_tp, wP)
-#undef RIV_ADDC
-    const uint32_t nearest = ncand > 0 ? c0 : 0u, nearp = ncand > 1 ? c1 : 0u, temporal = has_tp ? wP : 0u;
-    const int S0 = mvx(wS), S1 = mvy(wS), S2 = (wS >> 16) & 0xff, S3 = wS >> 24;
-    uint8_t mode;
-    if (wS == 0u) mode = riv::M_ZERO;
-    else if (wS == nearest) mode = riv::M_NEAREST;
-    else if (wS == nearp) mode = riv::M_NEAR;
-    else if (has_tr && wS == wTR) mode = riv::M_TOPRIGHT;
-    else if (has_tl && wS == wTL) mode = riv::M_TOPLEFT;
-    else if (wS == temporal) mode = riv::M_TEMPORAL;
-    else mode = riv::M_NEW;
-    uint8_t chosen = 0; int rdx = 0, rdy = 0;
-    if (mode == riv::M_NEW) {   // NEW base by estimated bits (encoder.rs:567-661, :128-159): the first cheapest
-        const uint32_t bases[5] = {nearest, nearp, wTR, wTL, temporal};
-        const bool have[5] = {true, true, has_tr, has_tl, true};
-        unsigned best = 0xffffffffu;
-        int bbx = mvx(nearest), bby = mvy(nearest);
-#pragma unroll
-        for (int k = 0; k < 5; k++) {
-            if (have[k]) {
-                const int kx = mvx(bases[k]), ky = mvy(bases[k]);
-                const int cxs = mv_class(abs(dclamp(S0 - kx, -128, 127))), cys = mv_class(abs(dclamp(S1 - ky, -128, 127)));
-                const unsigned cost = (k < 4 ? k + 1 : 4) + (cxs == 0 ? 4u : 4u + cxs) + (cys == 0 ? 4u : 4u + cys);
-                if (cost < best) { best = cost; chosen = (uint8_t)k; bbx = kx; bby = ky; }
-            }
-        }
-        rdx = dclamp(S0 - bbx, -128, 127);
-        rdy = dclamp(S1 - bby, -128, 127);
-        if (dclamp(bbx + rdx, -128, 127) != S0 || dclamp(bby + rdy, -128, 127) != S1) atomicExch(p.err, 2);
-    }
-    // RdoContext::decide via the host-built table (rdo.rs:140-178)
-    uint32_t rdo = 0;
-    if (S2 == 0 && S3 == 0 && best_sad <= p.rdo_max) rdo = (p.rdo_bits[best_sad >> 5] >> (best_sad & 31)) & 1u;
-    const uint32_t w2 = (uint32_t)mode | ((uint32_t)chosen << 8) | ((uint32_t)(uint8_t)rdx << 16) | ((uint32_t)(uint8_t)rdy << 24);
-    const uint4 r = make_uint4(wS, wS, w2, (uint32_t)best_sad | (rdo << 16));
-    *(uint4*)&p.rec[idx] = r;
-    *(uint4*)&p.host_rec[idx] = r;
 }
 
 // Natural log of a 16x16 block's variance from its sum and sum of squares, as the adaptive
@@ -872,7 +875,7 @@ __device__ __forceinline__ float add64(float sum, const float4* q) {
     }
     return sum;
 }
-__global__ __launch_bounds__(256) void k_aq_mean(float* __restrict__ lnv, int nb) {
+__global__ __launch_bounds__(256) void k_aq(float* __restrict__ lnv, int nb, float base, float* __restrict__ steps) {
     __shared__ float4 s_ln[2048 + 16];   // 8192 logs per pass, plus one chunk of slack for the read-ahead
     float sum = 0.0f;                    // lives in thread 0
     for (int c0 = 0; c0 < nb; c0 += 8192) {
@@ -901,14 +904,15 @@ __global__ __launch_bounds__(256) void k_aq_mean(float* __restrict__ lnv, int nb
             for (; i < cn; i++) sum = __fadd_rn(sum, sl[i]);
         }
     }
-    if (threadIdx.x == 0) lnv[nb] = __fdiv_rn(sum, (float)nb);
-}
-
-__global__ __launch_bounds__(256) void k_aq_steps(const float* __restrict__ lnv, int nb, float base, float* __restrict__ steps) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb) return;
-    const float off = __fmul_rn(0.8f, __fsub_rn(lnv[b], lnv[nb]));
-    steps[b] = __fmul_rn(base, glibc_exp2f(__fdiv_rn(off, 6.0f)));
+    __shared__ float s_mean;
+    if (threadIdx.x == 0) { s_mean = __fdiv_rn(sum, (float)nb); lnv[nb] = s_mean; }
+    __syncthreads();
+    // per-MB steps: base * exp2(0.8 * (ln v - mean) / 6) in f32 (residual.rs:150-166)
+    const float mean = s_mean;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        const float off = __fmul_rn(0.8f, __fsub_rn(lnv[b], mean));
+        steps[b] = __fmul_rn(base, glibc_exp2f(__fdiv_rn(off, 6.0f)));
+    }
 }
 
 // elementwise logf/exp2f for parity tests of the transcendental restatements
@@ -1096,7 +1100,7 @@ __device__ __forceinline__ void write_packed(const int16_t* coef, uint8_t* slot,
 __global__ __launch_bounds__(256) void k_inter_residual(const uint8_t* cY, const uint8_t* cU, const uint8_t* cV,
                                                         const uint8_t* pY, const uint8_t* pU, const uint8_t* pV,
                                                         int W, const BlockRec* rec, const float* steps, float dz,
-                                                        uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* pack, uint8_t* opt_skip,
+                                                        uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* pack, uint8_t* mask,
                                                         uint8_t* host_opt_skip) {
     __shared__ ResidLds L;
     const int b = blockIdx.x, bw = W >> 4, bx = b % bw, by = b / bw, t = threadIdx.x, cw = W >> 1;
@@ -1109,7 +1113,7 @@ __global__ __launch_bounds__(256) void k_inter_residual(const uint8_t* cY, const
             size_t oc = (size_t)((y0 >> 1) + (tt >> 3)) * cw + (x0 >> 1) + (tt & 7);
             if (pl) oV[oc] = pV[oc]; else oU[oc] = pU[oc];
         }
-        if (t == 0) { opt_skip[b] = 1; host_opt_skip[b] = 1; }
+        if (t == 0) { mask[b] = *(const uint32_t*)rec[b].r != 0u; host_opt_skip[b] = 1; }
         return;
     }
     {
@@ -1135,10 +1139,10 @@ __global__ __launch_bounds__(256) void k_inter_residual(const uint8_t* cY, const
             size_t oc = (size_t)((y0 >> 1) + (tt >> 3)) * cw + (x0 >> 1) + (tt & 7);
             if (pl) oV[oc] = pV[oc]; else oU[oc] = pU[oc];
         }
-        if (t == 0) { opt_skip[b] = 1; host_opt_skip[b] = 1; }
+        if (t == 0) { mask[b] = *(const uint32_t*)rec[b].r != 0u; host_opt_skip[b] = 1; }
         return;
     }
-    if (t == 0) { opt_skip[b] = 0; host_opt_skip[b] = 0; }
+    if (t == 0) { mask[b] = 1; host_opt_skip[b] = 0; }
     write_packed(L.coef, pack + (size_t)b * riv::PACK_STRIDE, t);
     recon_mb(L, t, qs, false, nullptr, nullptr, pY, pU, pV, oY, oU, oV, W, x0, y0);
 }
@@ -1315,12 +1319,6 @@ __global__ void k_deblock(uint8_t* Y, uint8_t* U, uint8_t* V, int W, int H, int 
 }
 
 // Deblock mask of a P-frame (encoder.rs:831-845): !optimised_skip || reconstructed MV != 0
-__global__ void k_filter_mask(const BlockRec* rec, const uint8_t* opt_skip, int nb, uint8_t* mask) {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb) return;
-    const int8_t* r = rec[b].r;
-    mask[b] = !opt_skip[b] || r[0] != 0 || r[1] != 0 || r[2] != 0 || r[3] != 0;
-}
 
 // ============================================================================================
 // Host-side launch wrappers
@@ -1366,25 +1364,21 @@ hipError_t me_search(hipStream_t st, const SearchParams& p, int groups, int wave
     hipLaunchKernelGGL(k_me_search, dim3(groups), dim3(64 * waves), search_lds_bytes(p.sr, waves, p.nstreams), st, p);
     return hipGetLastError();
 }
-void mode(hipStream_t st, const ModeParams& p) {
-    hipLaunchKernelGGL(k_mode, dim3((p.bw * p.bh + 255) / 256), dim3(256), 0, st, p);
-}
 void mc(hipStream_t st, const uint8_t* rY, const uint8_t* rU, const uint8_t* rV, int W, int H, const int8_t* mvs, int mv_stride,
         uint8_t* pY, uint8_t* pU, uint8_t* pV, uint32_t* var_sums, float* lnv) {
     hipLaunchKernelGGL(k_mc, dim3((W / 16) * (H / 16)), dim3(256), 0, st, rY, rU, rV, W, H, mvs, mv_stride, pY, pU, pV, var_sums, lnv);
 }
 void aq(hipStream_t st, int nb, float base, float* lnv, float* steps) {
-    hipLaunchKernelGGL(k_aq_mean, dim3(1), dim3(256), 0, st, lnv, nb);
-    hipLaunchKernelGGL(k_aq_steps, dim3((nb + 255) / 256), dim3(256), 0, st, lnv, nb, base, steps);
+    hipLaunchKernelGGL(k_aq, dim3(1), dim3(256), 0, st, lnv, nb, base, steps);
 }
 void logf_exp2f(hipStream_t st, const float* in, size_t n, int which, float* out) {
     hipLaunchKernelGGL(k_logf_exp2f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, n, which, out);
 }
 void inter_residual(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, const uint8_t* pY, const uint8_t* pU,
                     const uint8_t* pV, int W, int H, const BlockRec* rec, const float* steps, float dz, uint8_t* oY, uint8_t* oU,
-                    uint8_t* oV, uint8_t* pack, uint8_t* opt_skip, uint8_t* host_opt_skip) {
+                    uint8_t* oV, uint8_t* pack, uint8_t* mask, uint8_t* host_opt_skip) {
     hipLaunchKernelGGL(k_inter_residual, dim3((W / 16) * (H / 16)), dim3(256), 0, st, cY, cU, cV, pY, pU, pV, W, rec, steps, dz, oY, oU, oV,
-                       pack, opt_skip, host_opt_skip);
+                       pack, mask, host_opt_skip);
 }
 void intra(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, int W, int H, const uint16_t* yt, const uint16_t* ct,
            uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* pack) {
@@ -1413,9 +1407,6 @@ void deblock(hipStream_t st, uint8_t* Y, uint8_t* U, uint8_t* V, int W, int H, i
     long long nh = (long long)(bh - 1) * W + 2ll * (bh - 1) * (W / 2);
     if (nv > 0) hipLaunchKernelGGL(k_deblock, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st, Y, U, V, W, H, E, I, HEV, mask, 0);
     if (nh > 0) hipLaunchKernelGGL(k_deblock, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, st, Y, U, V, W, H, E, I, HEV, mask, 1);
-}
-void filter_mask(hipStream_t st, const BlockRec* rec, const uint8_t* opt_skip, int nb, uint8_t* mask) {
-    hipLaunchKernelGGL(k_filter_mask, dim3((nb + 255) / 256), dim3(256), 0, st, rec, opt_skip, nb, mask);
 }
 
 }  // namespace rivk

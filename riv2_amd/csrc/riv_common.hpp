@@ -40,13 +40,17 @@ struct SearchStream {
     const uint8_t* cur;          // current luma plane, W x H
     const uint8_t* ref;          // reference luma plane (previous reconstruction), W x H
     const int8_t* prev_mv;       // previous P-frame's reconstructed MVs (dx,dy,sx,sy) or nullptr
-    unsigned long long* smv;     // per block out: search MV word | best half-pel SAD << 32
+    BlockRec* rec;               // per block out: the mode decision's record
+    BlockRec* host_rec;          // the same records, written to mapped host memory
+    const uint32_t* rdo_bits;    // bit s set -> RDO chooses skip at SAD s (integer-aligned MV)
     int* passes;                 // out: passes the launch ran (may be nullptr)
     int W, H, bw, bh;
     int blk0;                    // number of the stream's first block in the launch
     int prev_stride;             // bytes between consecutive MVs of prev_mv
+    int rdo_max;                 // largest SAD that can skip, -1 when skipping is disabled
+    int pad_;
 };
-static_assert(sizeof(SearchStream) == 64, "SearchStream layout");
+static_assert(sizeof(SearchStream) == 88, "SearchStream layout");
 
 // Parameters of the motion search by relaxation passes (one persistent launch per frame step).
 // Every stream of a launch shares the search range, lambda and thresholds.
@@ -67,20 +71,8 @@ struct SearchParams {
     int c_zero, c_pred;          // (lam2*0.5) as i64, (lam2*1.5) as i64; c_pred is also the half-pel penalty
     int lam_zero;                // lambda == 0.0 -> every MV cost is 0
     long long zthr, pthr;        // me_zero_mv_threshold, me_predictor_threshold
-    int* err;                    // 1 on a barrier timeout, 3 if the pass limit is hit
-};
-
-// Parameters of the per-MB mode decision that follows the search.
-struct ModeParams {
-    int bw, bh;
-    const unsigned long long* smv;   // search MV word | SAD << 32 per block
-    const int8_t* prev_mv;           // previous P-frame's reconstructed MVs or nullptr
-    int prev_stride;
-    const uint32_t* rdo_bits;        // bit s set -> RDO chooses skip at SAD s (integer-aligned MV)
-    int rdo_max;                     // largest SAD that can skip, -1 when skipping is disabled
-    BlockRec* rec;                   // bw*bh records out
-    BlockRec* host_rec;              // the same records, written to mapped host memory
-    int* err;                        // set to 2 if a reconstructed MV would differ from its search MV
+    int* err;                    // 1 on a barrier timeout, 3 if the pass limit is hit, 2 if a
+                                 // reconstructed MV would differ from its search MV
 };
 
 }  // namespace riv

@@ -17,15 +17,15 @@ void sad_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, in
 void hp_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int H, int sr, uint16_t* out, int stride);
 size_t search_lds_bytes(int sr, int waves, int nstreams);
 hipError_t me_search(hipStream_t st, const riv::SearchParams& p, int groups, int waves);
-void mode(hipStream_t st, const riv::ModeParams& p);
 void mc(hipStream_t st, const uint8_t* rY, const uint8_t* rU, const uint8_t* rV, int W, int H, const int8_t* mvs, int mv_stride,
         uint8_t* pY, uint8_t* pU, uint8_t* pV, uint32_t* var_sums, float* lnv);
-// lnv: nb logs written by mc, plus one float for their mean
+// lnv: nb logs written by mc, plus one float for their mean; one launch computes the mean and the steps
 void aq(hipStream_t st, int nb, float base, float* lnv, float* steps);
 void logf_exp2f(hipStream_t st, const float* in, size_t n, int which, float* out);
 void inter_residual(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, const uint8_t* pY, const uint8_t* pU,
                     const uint8_t* pV, int W, int H, const riv::BlockRec* rec, const float* steps, float dz, uint8_t* oY, uint8_t* oU,
-                    uint8_t* oV, uint8_t* pack, uint8_t* opt_skip, uint8_t* host_opt_skip);
+                    uint8_t* oV, uint8_t* pack, uint8_t* mask, uint8_t* host_opt_skip);
+// mask: per MB the deblock filter flag (not optimised-skip, or a non-zero reconstructed MV; encoder.rs:831-845)
 // pack: riv::PACK_STRIDE bytes per MB (mapped host memory); the layout is in riv_common.hpp
 void intra(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, int W, int H, const uint16_t* yt, const uint16_t* ct,
            uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* pack);
@@ -35,5 +35,4 @@ void intra_recon(hipStream_t st, int W, int H, const uint16_t* yt, const uint16_
                  uint8_t* oV);
 void dct_plane(hipStream_t st, const int16_t* plane, int w, int h, int n8, const float* qs, float dz, int16_t* coeffs, int16_t* recon);
 void deblock(hipStream_t st, uint8_t* Y, uint8_t* U, uint8_t* V, int W, int H, int level, bool intra, const uint8_t* mask);
-void filter_mask(hipStream_t st, const riv::BlockRec* rec, const uint8_t* opt_skip, int nb, uint8_t* mask);
 }  // namespace rivk
