@@ -119,7 +119,8 @@ struct riv_encoder {
     uint64_t frame_count = 0;      // frames submitted
 
     int W = 0, H = 0, bw = 0, bh = 0, nb = 0;
-    int s_waves = 1, s_groups = 1, s_maxp = 0, s_bank = 0;   // motion-search launch shape, pass limit, counter bank
+    rivk::SearchShape shape{};                 // motion-search launch shape
+    int s_maxp = 0, s_bank = 0;                // pass limit, counter bank
     hipStream_t st = nullptr;    // kernels, frame after frame
     DevBuf rgb, cur, pred, ref[2], var, lnv, steps, mask, field, arrive, bits, rdo, err, yt, ct;
     int ref_idx = 0;
@@ -221,17 +222,11 @@ static int encoder_init(riv_encoder* e) {
     if (r) return r;
     e->W = (int)c.storage_width; e->H = (int)c.storage_height;
     e->bw = e->W / 16; e->bh = e->H / 16; e->nb = e->bw * e->bh;
-    // motion-search geometry: one resident workgroup per CU at most, as many waves per workgroup
-    // (up to 16) as the workgroup's LDS holds (one reference window each), at least one wave per
-    // 64 blocks
+    // motion-search launch shape
     {
         hipDeviceProp_t prop;
         RIV_HIP(hipGetDeviceProperties(&prop, c.device_id));
-        const size_t budget = prop.sharedMemPerBlock;
-        e->s_waves = 16;
-        while (e->s_waves > 1 && rivk::search_lds_bytes(c.search_range, e->s_waves, 1) > budget) e->s_waves--;
-        e->s_groups = std::max(1, std::min(prop.multiProcessorCount, (e->nb + e->s_waves - 1) / e->s_waves));
-        if ((long long)e->s_groups * e->s_waves * 64 < e->nb) {
+        if (!rivk::search_shape(c.search_range, e->nb, 1, prop.multiProcessorCount, prop.sharedMemPerBlock, &e->shape)) {
             set_error("frame has more macroblocks than the motion search's resident waves can hold");
             return RIV_E_UNSUPPORTED;
         }
@@ -244,7 +239,7 @@ static int encoder_init(riv_encoder* e) {
     RIV_HIP(e->ref[0].alloc(fsz)); RIV_HIP(e->ref[1].alloc(fsz));
     RIV_HIP(e->var.alloc(8 * (size_t)e->nb)); RIV_HIP(e->lnv.alloc(4 * ((size_t)e->nb + 1))); RIV_HIP(e->steps.alloc(4 * (size_t)e->nb));
     RIV_HIP(e->mask.alloc(e->nb));
-    RIV_HIP(e->field.alloc(8 * (size_t)e->nb));
+    RIV_HIP(e->field.alloc(8 * (size_t)e->nb)); RIV_HIP(hipMemset(e->field.p, 0, 8 * (size_t)e->nb));
     {
         const size_t bb = 2 * sizeof(unsigned) * (size_t)riv::barrier_words(e->s_maxp);
         RIV_HIP(e->arrive.alloc(bb)); RIV_HIP(hipMemset(e->arrive.p, 0, bb));
@@ -788,8 +783,9 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
         p.s0 = frame_search_stream(e, f, 0);
         p.streams = nullptr; p.nstreams = 1; p.nb = e->nb;
         p.field = e->field.as<uint32_t>(); p.arrive = e->arrive.as<unsigned>(); p.bank = e->s_bank; p.maxp = e->s_maxp;
+        p.nslot = e->shape.nslot; p.nres = e->shape.nres;
         e->s_bank ^= 1;
-        RIV_HIP(rivk::me_search(f.st, p, e->s_groups, e->s_waves));
+        RIV_HIP(rivk::me_search(f.st, p, e->shape.groups, e->shape.waves));
         r = frame_after_search(e, f);
         if (r) return r;
     }
@@ -840,16 +836,20 @@ static int batch_step(riv_encoder* const* encs, size_t n, const void* const* rgb
         // launch shape for the whole batch
         hipDeviceProp_t prop;
         RIV_HIP(hipGetDeviceProperties(&prop, c0.device_id));
-        int waves = 16;
-        while (waves > 1 && rivk::search_lds_bytes(p.sr, waves, p.nstreams) > prop.sharedMemPerBlock) waves--;
-        if (rivk::search_lds_bytes(p.sr, waves, p.nstreams) > prop.sharedMemPerBlock) { set_error("too many streams in one batch step"); return RIV_E_UNSUPPORTED; }
-        const int groups = std::max(1, std::min(prop.multiProcessorCount, (total + waves - 1) / waves));
-        if ((long long)groups * waves * 64 < total) { set_error("batch has more macroblocks than the motion search's resident waves can hold"); return RIV_E_UNSUPPORTED; }
+        rivk::SearchShape sh;
+        if (!rivk::search_shape(p.sr, total, p.nstreams, prop.multiProcessorCount, prop.sharedMemPerBlock, &sh)) {
+            set_error("batch has more macroblocks or streams than one motion-search launch can hold");
+            return RIV_E_UNSUPPORTED;
+        }
+        p.nslot = sh.nslot; p.nres = sh.nres;
         // scratch sized for the batch (grown when a larger batch comes; the stream is drained first)
         const size_t need_field = 2 * (size_t)total, need_arrive = 2 * (size_t)riv::barrier_words(maxp);
         if (need_field > e0->b_field_n || need_arrive > e0->b_arrive_n || desc.size() > e0->b_desc_n) {
             RIV_HIP(hipStreamSynchronize(st));
-            if (need_field > e0->b_field_n) { RIV_HIP(e0->b_field.alloc(4 * need_field)); e0->b_field_n = need_field; }
+            if (need_field > e0->b_field_n) {
+                RIV_HIP(e0->b_field.alloc(4 * need_field)); RIV_HIP(hipMemset(e0->b_field.p, 0, 4 * need_field));
+                e0->b_field_n = need_field;
+            }
             if (need_arrive > e0->b_arrive_n) {
                 RIV_HIP(e0->b_arrive.alloc(4 * need_arrive)); RIV_HIP(hipMemset(e0->b_arrive.p, 0, 4 * need_arrive));
                 e0->b_arrive_n = need_arrive; e0->b_bank = 0;
@@ -879,7 +879,7 @@ static int batch_step(riv_encoder* const* encs, size_t n, const void* const* rgb
         p.streams = e0->b_desc.as<riv::SearchStream>();
         p.field = e0->b_field.as<uint32_t>(); p.arrive = e0->b_arrive.as<unsigned>(); p.bank = e0->b_bank;
         e0->b_bank ^= 1;
-        RIV_HIP(rivk::me_search(st, p, groups, waves));
+        RIV_HIP(rivk::me_search(st, p, sh.groups, sh.waves));
     }
     int status = RIV_OK;
     for (size_t i = 0; i < n; i++) {

@@ -376,10 +376,10 @@ __device__ __forceinline__ uint4 mode_record(uint32_t wS, int best_sad, bool has
 // and the counters are read and written with relaxed agent-scope atomics (coherent across the
 // chip without cache maintenance); each wave waits for its own stores before arriving.
 //
-// An evaluation stages the block's replicate-padded reference window in LDS as four byte-shifted
-// copies (so every 4-pixel read is one aligned LDS read) and scores each candidate set with
-// direct SADs: every lane holds 4 pixels of the block, the 64 partial sums of two candidates are
-// reduced at once as 16-bit halves of one word.
+// Each wave keeps its blocks' replicate-padded reference windows, and the blocks' own 16 rows, in
+// LDS slots for the whole launch (the reference frame does not change during the search), so a
+// re-evaluation reads no global memory; when a wave owns more blocks than it has slots, the
+// blocks past the resident ones share one slot that is reloaded for each evaluation.
 namespace {
 
 // Candidate SADs are computed one candidate per 16-lane row: lane j of a row holds row j of the
@@ -387,16 +387,36 @@ namespace {
 // inside the row gives the candidate's SAD in all 16 lanes. A wave thus scores 4 candidates per
 // round with no cross-row traffic.
 struct JWin {
-    const uint32_t* cp;   // 4 copies of NR x ND dwords, copy s holding bytes 4i+s .. 4i+s+3 at dword i
-    int CS, ND;
+    const uint32_t* wp;   // the window: NR rows of ND dwords, dword i of a row = bytes 4i .. 4i+3
+    int ND;
     int W, H, x0, y0, wxa, wy0;
     uint4 cq;             // row j = lane & 15 of the current block
     int j;
-    // Block row j, dword k at offset (d, e) reads the window at column cb + d + 4k and row
-    // j + sr + 1 + e, i.e. dword ((cb+d) & 3) * CS + ((cb+d) >> 2) + k + e * ND + rowc.
+    // Block row j at offset (d, e) reads the window from byte cb + d of row j + sr + 1 + e, i.e.
+    // from dword ((cb+d) >> 2) + e * ND + rowc, shifted by (cb+d) & 3 bytes.
     int cb;               // x0 - wxa (>= sr + 1)
     int rowc;             // (j + sr + 1) * ND
 };
+
+// 16 bytes from byte s (0..3) of q[0]: five dword reads, four byte-aligns.
+__device__ __forceinline__ void jw_read16(const uint32_t* q, int s, uint32_t r[4]) {
+    uint32_t d[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) d[k] = q[k];
+#pragma unroll
+    for (int k = 0; k < 4; k++) r[k] = __builtin_amdgcn_alignbyte(d[k + 1], d[k], (uint32_t)s);
+}
+// The same 16 bytes (a) and the 16 one byte further on (b), from one set of five dword reads.
+__device__ __forceinline__ void jw_read16x2(const uint32_t* q, int s, uint32_t a[4], uint32_t b[4]) {
+    uint32_t d[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) d[k] = q[k];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        a[k] = __builtin_amdgcn_alignbyte(d[k + 1], d[k], (uint32_t)s);
+        b[k] = s == 3 ? d[k + 1] : __builtin_amdgcn_alignbyte(d[k + 1], d[k], (uint32_t)(s + 1));
+    }
+}
 
 __device__ __forceinline__ uint32_t row_sum(uint32_t v) {   // sum over the lane's 16-lane row, in every lane of it
     v += (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, true);    // quad_perm [1,0,3,2]
@@ -414,15 +434,14 @@ __device__ __forceinline__ uint32_t jw_row_int(const JWin& w, int dx, int dy) {
     uint32_t r[4];
     if (INTERIOR) {
         const int c = w.cb + dx;
-        const uint32_t* q = w.cp + __mul24(c & 3, w.CS) + (c >> 2) + __mul24(dy, w.ND) + w.rowc;
-#pragma unroll
-        for (int k = 0; k < 4; k++) r[k] = q[k];
+        jw_read16(w.wp + (c >> 2) + __mul24(dy, w.ND) + w.rowc, c & 3, r);
     } else {
         const int oy = dclamp(w.y0 + 4 * (w.j >> 2) + dy, 0, w.H - 4) + (w.j & 3) - w.wy0;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int ox = dclamp(w.x0 + 4 * k + dx, 0, w.W - 4) - w.wxa;
-            r[k] = w.cp[__mul24(ox & 3, w.CS) + __mul24(oy, w.ND) + (ox >> 2)];
+            const uint32_t* q = w.wp + __mul24(oy, w.ND) + (ox >> 2);
+            r[k] = __builtin_amdgcn_alignbyte(q[1], q[0], (uint32_t)(ox & 3));
         }
     }
     uint32_t acc = __builtin_amdgcn_sad_u8(w.cq.x, r[0], 0u);
@@ -438,20 +457,26 @@ __device__ __forceinline__ uint32_t jw_row_int(const JWin& w, int dx, int dy) {
 // else y), 2 both components half-pel.
 template <int TYPE>
 __device__ __forceinline__ uint32_t jw_row_hp(const JWin& w, int hx, int hy) {
-    const int c = w.cb + (hx >> 1), base = __mul24(hy >> 1, w.ND) + w.rowc;
-    const uint32_t* q0 = w.cp + __mul24(c & 3, w.CS) + (c >> 2) + base;
-    const uint32_t* q1 = w.cp + __mul24((c + 1) & 3, w.CS) + ((c + 1) >> 2) + base;   // one column right
+    const int c = w.cb + (hx >> 1);
+    const uint32_t* q = w.wp + (c >> 2) + __mul24(hy >> 1, w.ND) + w.rowc;
     uint32_t v[4];
     if (TYPE == 0) {
-#pragma unroll
-        for (int k = 0; k < 4; k++) v[k] = q0[k];
+        jw_read16(q, c & 3, v);
     } else if (TYPE == 1) {
-        const uint32_t* q2 = (hx & 1) ? q1 : q0 + w.ND;   // the second sample: right or below
+        // the second sample: one byte right (hx odd) or one row below
+        const int c2 = c + (hx & 1);
+        const uint32_t* q2 = w.wp + (c2 >> 2) + __mul24((hy >> 1) + ((hx & 1) ^ 1), w.ND) + w.rowc;
+        uint32_t a[4], b[4];
+        jw_read16(q, c & 3, a);
+        jw_read16(q2, c2 & 3, b);
 #pragma unroll
-        for (int k = 0; k < 4; k++) v[k] = avg2f(q0[k], q2[k]);
+        for (int k = 0; k < 4; k++) v[k] = avg2f(a[k], b[k]);
     } else {
+        uint32_t a[4], b[4], e[4], f[4];
+        jw_read16x2(q, c & 3, a, b);
+        jw_read16x2(q + w.ND, c & 3, e, f);
 #pragma unroll
-        for (int k = 0; k < 4; k++) v[k] = avg4f(q0[k], q1[k], q0[k + w.ND], q1[k + w.ND]);
+        for (int k = 0; k < 4; k++) v[k] = avg4f(a[k], b[k], e[k], f[k]);
     }
     uint32_t acc = __builtin_amdgcn_sad_u8(w.cq.x, v[0], 0u);
     acc = __builtin_amdgcn_sad_u8(w.cq.y, v[1], acc);
@@ -497,35 +522,33 @@ __device__ __forceinline__ int jw_set_hp(const JWin& w, int lane, int cx, int cy
 
 // The window of MB (x0, y0): rows y0-sr-1 .. y0+16+sr and columns from the dword-aligned
 // wxa <= x0-sr-1, replicate-padded outside the frame (W is a multiple of 16, so an aligned
-// dword is wholly inside or wholly outside a row).
+// dword is wholly inside or wholly outside a row), written to wp as NR rows of ND dwords; the
+// block's own 16 rows follow at wp + riv::search_window_dwords(sr).
 // A lane covers rows jr, jr + rstep, ... of dword column ji; the loads of JU rows are all issued
 // before any of them is stored, so a window costs about one memory latency.
-__device__ __forceinline__ void jw_load(const SearchStream& S, int sr, uint32_t* cp, int CS, int ND, int NR, int x0, int y0, int lane) {
+__device__ __forceinline__ void jw_load(const SearchStream& S, int sr, uint32_t* wp, int ND, int NR, int x0, int y0, int lane) {
     constexpr int JU = 9;
+    if (lane < 16) {
+        const uint4 v = *(const uint4*)(S.cur + (size_t)(y0 + lane) * S.W + x0);
+        *(uint4*)(wp + riv::search_window_dwords(sr) + 4 * lane) = v;
+    }
     const int wxa = (x0 - sr - 1) & ~3, wy0 = y0 - sr - 1;
     const int rstep = 64 / ND, jr = lane / ND, ji = lane - jr * ND;
     if (jr >= rstep) return;
-    const int g0 = wxa + 4 * ji, g1 = g0 + 4;
-    const bool in0 = g0 >= 0 && g0 < S.W, in1 = g1 >= 0 && g1 < S.W;
+    const int g0 = wxa + 4 * ji;
+    const bool in0 = g0 >= 0 && g0 < S.W;
     for (int rb = jr; rb < NR; rb += JU * rstep) {
-        uint32_t d0[JU], d1[JU];
+        uint32_t d0[JU];
 #pragma unroll
         for (int u = 0; u < JU; u++) {
             const int r = min(rb + u * rstep, NR - 1);
             const uint8_t* row = S.ref + (size_t)dclamp(wy0 + r, 0, S.H - 1) * S.W;
             d0[u] = in0 ? *(const uint32_t*)(row + g0) : 0x01010101u * (g0 < 0 ? row[0] : row[S.W - 1]);
-            d1[u] = in1 ? *(const uint32_t*)(row + g1) : 0x01010101u * (g1 < 0 ? row[0] : row[S.W - 1]);
         }
 #pragma unroll
         for (int u = 0; u < JU; u++) {
             const int r = rb + u * rstep;
-            if (r < NR) {
-                uint32_t* q = cp + r * ND + ji;
-                q[0] = d0[u];
-                q[CS] = __builtin_amdgcn_alignbyte(d1[u], d0[u], 1);
-                q[2 * CS] = __builtin_amdgcn_alignbyte(d1[u], d0[u], 2);
-                q[3 * CS] = __builtin_amdgcn_alignbyte(d1[u], d0[u], 3);
-            }
+            if (r < NR) wp[r * ND + ji] = d0[u];
         }
     }
 }
@@ -535,15 +558,15 @@ __device__ __forceinline__ void jw_load(const SearchStream& S, int sr, uint32_t*
 // trS, tlS and the temporal word pT. Returns the canonical search MV word; *sad_out = its
 // half-pel SAD.
 template <bool INTERIOR>
-__device__ uint32_t jsearch(const SearchParams& p, const SearchStream& S, const int* ctab, uint32_t* cp, int CS, int ND, int NR, int bx, int by,
-                            uint32_t lS, uint32_t tS, uint32_t trS, uint32_t tlS, uint32_t pT, int lane, int* sad_out) {
+__device__ uint32_t jsearch(const SearchParams& p, const SearchStream& S, const int* ctab, uint32_t* wp, bool load, int ND, int NR, int bx,
+                            int by, uint32_t lS, uint32_t tS, uint32_t trS, uint32_t tlS, uint32_t pT, int lane, int* sad_out) {
     const int sr = p.sr, x0 = bx * 16, y0 = by * 16;
-    jw_load(S, sr, cp, CS, ND, NR, x0, y0, lane);
+    if (load) jw_load(S, sr, wp, ND, NR, x0, y0, lane);
     JWin w;
-    w.cp = cp; w.CS = CS; w.ND = ND; w.W = S.W; w.H = S.H; w.x0 = x0; w.y0 = y0;
+    w.wp = wp; w.ND = ND; w.W = S.W; w.H = S.H; w.x0 = x0; w.y0 = y0;
     w.wxa = (x0 - sr - 1) & ~3; w.wy0 = y0 - sr - 1;
     w.j = lane & 15;
-    w.cq = *(const uint4*)(S.cur + (size_t)(y0 + w.j) * S.W + x0);
+    w.cq = *(const uint4*)(wp + riv::search_window_dwords(sr) + 4 * w.j);
     w.cb = x0 - w.wxa; w.rowc = (w.j + sr + 1) * ND;
 
     const bool has_l = bx > 0, has_t = by > 0, has_tr = has_t && bx + 1 < S.bw, has_tl = has_t && bx > 0, has_tp = S.prev_mv != nullptr;
@@ -644,6 +667,13 @@ __device__ uint32_t jsearch(const SearchParams& p, const SearchStream& S, const 
 __device__ __forceinline__ uint32_t field_load(const uint32_t* a) { return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 __device__ __forceinline__ void field_store(uint32_t* a, uint32_t v) { __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
+// A start-field word as a search input: its integer vector when that lies within the search range,
+// else (0,0). The start field is only a guess (the relaxation reaches the same fixed point from any
+// start), so words left by another frame or size of batch are safe to use.
+__device__ __forceinline__ uint32_t seed_word(uint32_t w, int sr) {
+    return (abs(mvx(w)) <= sr && abs(mvy(w)) <= sr) ? (w & 0xffffu) : 0u;
+}
+
 }  // namespace
 
 __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
@@ -652,12 +682,14 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int NW = gridDim.x * (blockDim.x >> 6), gw = blockIdx.x * (blockDim.x >> 6) + wv;
     const int nb = p.nb, sr = p.sr, NS = p.nstreams;
-    const int NR = 2 * sr + 18, ND = ((2 * sr + 20) >> 2) + 1, CS = NR * ND;
+    const int NR = 2 * sr + 18, ND = ((2 * sr + 20) >> 2) + 1, SLOT = riv::search_slot_dwords(sr);
     const int TD = 2 * sr + 4;
     int* ctab = (int*)lds32;                           // TD x TD NEW-vector cost terms
     int* wflag = ctab + ((TD * TD + 3) & ~3);          // [0] a wave of this workgroup changed a block, [1] pass outcome
     SearchStream* sd = (SearchStream*)(wflag + 4);     // the launch's streams
-    uint32_t* cp = (uint32_t*)(sd + NS) + (size_t)wv * 4 * CS;
+    // this wave's slots: lane k's block in slot k for k < nres, the other lanes' blocks in slot nres
+    uint32_t* slots = lds32 + riv::search_slots_offset(sr, NS) / 4 + (size_t)wv * p.nslot * SLOT;
+    uint64_t resident = 0;   // slots that hold their block's window
     for (int i = threadIdx.x; i < TD * TD; i += blockDim.x) ctab[i] = new_mv_cost(p.lam2, p.bits, i / TD, i % TD);
     if (!p.streams) { if (threadIdx.x == 0) sd[0] = p.s0; }
     else for (int i = threadIdx.x; i < NS * (int)(sizeof(SearchStream) / 4); i += blockDim.x) ((uint32_t*)sd)[i] = ((const uint32_t*)p.streams)[i];
@@ -684,7 +716,9 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
     const int mbw = sd[ms].bw, mloc = valid ? myb - sd[ms].blk0 : 0;
     const int mbx = mloc % mbw, mby = mloc / mbw;
     const uint32_t mypT = (valid && sd[ms].prev_mv) ? *(const uint32_t*)(sd[ms].prev_mv + (size_t)sd[ms].prev_stride * mloc) : 0u;
-    uint32_t myw = 0u;   // start field: every MV (0,0)
+    // start field: bank 0 as the previous launch over this field left it (each block's final word),
+    // a guess that spares the second pass every block whose guess its first pass confirms
+    uint32_t myw = valid ? seed_word(field_load(p.field + myb), sr) : 0u;
     int mysad = 0;
     int outcome = 0;     // 1 converged, -1 barrier timeout, 0 pass limit
     int pass = 0;
@@ -693,14 +727,18 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
         uint32_t* dst = p.field + (size_t)((pass + 1) & 1) * nb;
         uint32_t nL = 0u, nT = 0u, nTR = 0u, nTL = 0u;
         bool dirty = valid;
-        if (pass > 0 && valid) {
+        if (valid) {
             if (mbx > 0) nL = field_load(src + myb - 1);
             if (mby > 0) {
                 nT = field_load(src + myb - mbw);
                 if (mbx + 1 < mbw) nTR = field_load(src + myb - mbw + 1);
                 if (mbx > 0) nTL = field_load(src + myb - mbw - 1);
             }
-            dirty = ((nL | nT | nTR | nTL) >> 31) != 0u;
+            if (pass == 0) {   // every block searches once, from its neighbours' guesses
+                nL = seed_word(nL, sr); nT = seed_word(nT, sr); nTR = seed_word(nTR, sr); nTL = seed_word(nTL, sr);
+            } else {
+                dirty = ((nL | nT | nTR | nTL) >> 31) != 0u;
+            }
         }
         // blocks whose inputs did not change keep their word (flag cleared) in this pass's field
         if (valid && !dirty) field_store(dst + myb, myw);
@@ -716,9 +754,13 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
             const uint32_t tlS = sgpr((uint32_t)__builtin_amdgcn_readlane((int)nTL, k)) & 0x7fffffffu;
             const uint32_t pT = sgpr((uint32_t)__builtin_amdgcn_readlane((int)mypT, k));
             int sad;
+            const bool own = k < p.nres;
+            uint32_t* wp = slots + (own ? k : p.nres) * SLOT;
+            const bool load = !own || !((resident >> k) & 1u);
+            if (own) resident |= 1ull << k;
             const uint32_t wS = mb_interior(bx * 16, by * 16, S.W, S.H, sr)
-                                    ? jsearch<true>(p, S, ctab, cp, CS, ND, NR, bx, by, lS, tS, trS, tlS, pT, lane, &sad)
-                                    : jsearch<false>(p, S, ctab, cp, CS, ND, NR, bx, by, lS, tS, trS, tlS, pT, lane, &sad);
+                                    ? jsearch<true>(p, S, ctab, wp, load, ND, NR, bx, by, lS, tS, trS, tlS, pT, lane, &sad)
+                                    : jsearch<false>(p, S, ctab, wp, load, ND, NR, bx, by, lS, tS, trS, tlS, pT, lane, &sad);
             const uint32_t old = sgpr((uint32_t)__builtin_amdgcn_readlane((int)myw, k));
             const bool c = ((wS ^ old) & 0xffffu) != 0u;
             chg |= c;
@@ -774,6 +816,8 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
                                     S.rdo_bits, S.rdo_max, p.err);
         *(uint4*)&S.rec[mloc] = r;
         *(uint4*)&S.host_rec[mloc] = r;
+        // the next launch's start field; where bank 0 is the final field it already holds myw
+        field_store(p.field + myb, myw);
     }
     if (blockIdx.x == 0)
         for (int i = threadIdx.x; i < NS; i += blockDim.x)
@@ -1355,13 +1399,30 @@ void hp_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int
     const size_t lds = (size_t)(64 + (WS + HPK + 1) * (ndp + 1) + 4 * (WS + HPK) * ndp) * 4;
     hipLaunchKernelGGL(k_hp_table<HPK>, dim3((W / 16) * (H / 16)), dim3(128), lds, st, cur, ref, W, H, W / 16, sr, ndp, out, stride);
 }
-size_t search_lds_bytes(int sr, int waves, int nstreams) {
-    const int NR = 2 * sr + 18, ND = ((2 * sr + 20) >> 2) + 1, TD = 2 * sr + 4;
-    return (size_t)((TD * TD + 3) & ~3) * sizeof(int) + 16 + (size_t)nstreams * sizeof(SearchStream) +
-           (size_t)waves * 4 * NR * ND * sizeof(uint32_t);
+size_t search_lds_bytes(int sr, int waves, int nstreams, int nslot) {
+    return riv::search_slots_offset(sr, nstreams) + (size_t)waves * nslot * riv::search_slot_dwords(sr) * sizeof(uint32_t);
+}
+bool search_shape(int sr, int nb, int nstreams, int cus, size_t budget, SearchShape* out) {
+    // one resident workgroup per CU at most, as many waves per workgroup (up to 16) as hold one
+    // slot each, at least one wave per 64 blocks; then as many slots per wave as fit, up to the
+    // blocks a wave owns
+    int waves = 16;
+    while (waves > 1 && search_lds_bytes(sr, waves, nstreams, 1) > budget) waves--;
+    if (search_lds_bytes(sr, waves, nstreams, 1) > budget) return false;
+    const int groups = std::max(1, std::min(cus, (nb + waves - 1) / waves));
+    const long long nw = (long long)groups * waves;
+    if (nw * 64 < nb) return false;
+    const int owned = (int)((nb + nw - 1) / nw);
+    const size_t per = (size_t)waves * riv::search_slot_dwords(sr) * sizeof(uint32_t);
+    const int fit = (int)std::min<size_t>(64, (budget - riv::search_slots_offset(sr, nstreams)) / per);
+    out->waves = waves; out->groups = groups;
+    out->nslot = std::min(owned, fit);
+    out->nres = owned <= fit ? owned : fit - 1;
+    out->lds = search_lds_bytes(sr, waves, nstreams, out->nslot);
+    return true;
 }
 hipError_t me_search(hipStream_t st, const SearchParams& p, int groups, int waves) {
-    hipLaunchKernelGGL(k_me_search, dim3(groups), dim3(64 * waves), search_lds_bytes(p.sr, waves, p.nstreams), st, p);
+    hipLaunchKernelGGL(k_me_search, dim3(groups), dim3(64 * waves), search_lds_bytes(p.sr, waves, p.nstreams, p.nslot), st, p);
     return hipGetLastError();
 }
 void mc(hipStream_t st, const uint8_t* rY, const uint8_t* rU, const uint8_t* rV, int W, int H, const int8_t* mvs, int mv_stride,

@@ -52,6 +52,18 @@ struct SearchStream {
 };
 static_assert(sizeof(SearchStream) == 88, "SearchStream layout");
 
+// LDS layout of the motion search: the MV-cost table (TD x TD ints, TD = 2*sr + 4), 4 flag words,
+// the launch's stream descriptors, then per wave its window slots, each an NR x ND dword window
+// (NR = 2*sr + 18 rows, ND = ((2*sr + 20) >> 2) + 1 dwords) and the block's 16 rows (64 dwords).
+__host__ __device__ inline int search_window_dwords(int sr) {
+    return (((2 * sr + 18) * (((2 * sr + 20) >> 2) + 1)) + 3) & ~3;
+}
+__host__ __device__ inline int search_slot_dwords(int sr) { return search_window_dwords(sr) + 64; }
+__host__ __device__ inline size_t search_slots_offset(int sr, int nstreams) {
+    const int TD = 2 * sr + 4;
+    return ((((size_t)((TD * TD + 3) & ~3) + 4) * 4 + (size_t)nstreams * sizeof(SearchStream)) + 15) & ~(size_t)15;
+}
+
 // Parameters of the motion search by relaxation passes (one persistent launch per frame step).
 // Every stream of a launch shares the search range, lambda and thresholds.
 struct SearchParams {
@@ -66,6 +78,8 @@ struct SearchParams {
     int bank;                    // bank of this launch (zero at launch); the launch clears the other
     int maxp;                    // pass limit: 2*(bh-1) + bw + 2 passes of the largest stream always
                                  // reach the fixed point
+    int nslot, nres;             // LDS window slots per wave: lane k < nres keeps its block's window in
+                                 // slot k; when nslot = nres + 1 the other lanes share slot nres
     const double* bits;          // 256 entries: 1.0 for 0, else 2*log2(a)+2 (motion/scalar.rs:495-525)
     double lam2;                 // lambda * 2.0
     int c_zero, c_pred;          // (lam2*0.5) as i64, (lam2*1.5) as i64; c_pred is also the half-pel penalty

@@ -15,7 +15,12 @@ int table_region_bytes(int sr);
 bool mb_is_interior(int x0, int y0, int W, int H, int sr);   // its integer SADs are the half-pel table's even entries
 void sad_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int H, int sr, uint16_t* out, int stride);
 void hp_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int H, int sr, uint16_t* out, int stride);
-size_t search_lds_bytes(int sr, int waves, int nstreams);
+size_t search_lds_bytes(int sr, int waves, int nstreams, int nslot);
+// Launch shape of the motion search for nb blocks over nstreams streams: workgroups (at most one
+// per CU), waves per workgroup, window slots per wave (nres of them resident) and LDS bytes.
+// False when the blocks do not fit in resident waves or the LDS budget.
+struct SearchShape { int waves, groups, nslot, nres; size_t lds; };
+bool search_shape(int sr, int nb, int nstreams, int cus, size_t budget, SearchShape* out);
 hipError_t me_search(hipStream_t st, const riv::SearchParams& p, int groups, int waves);
 void mc(hipStream_t st, const uint8_t* rY, const uint8_t* rU, const uint8_t* rV, int W, int H, const int8_t* mvs, int mv_stride,
         uint8_t* pY, uint8_t* pU, uint8_t* pV, uint32_t* var_sums, float* lnv);
