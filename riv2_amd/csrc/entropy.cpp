@@ -482,6 +482,41 @@ int mv_class(int m) { return m == 0 ? 0 : std::min(32 - __builtin_clz((unsigned)
 
 namespace {
 enum { MZERO = 0, MNEAREST = 1, MNEAR = 2, MTR = 3, MTL = 4, MNEW = 5, MTEMP = 6 };
+// The decisions of the mode tree (New | Zero | Near | Temporal | TopRight | TopLeft | Nearest,
+// mv_rans/encoder.rs:257-301) per mode, and of the class tree (a binary search over 0..10,
+// encode_component, mv_rans/encoder.rs:303-331) per class: the count, and the bits by level.
+struct MvTrees {
+    uint8_t mode_n[8], mode_bits[8], cls_n[11], cls_bits[11];
+    MvTrees() {
+        static const uint8_t order[6] = {MNEW, 0xff, MNEAR, MTEMP, MTR, MTL};
+        for (int m = 0; m < 8; m++) {
+            int n = 0, bits = 0;
+            for (int lvl = 0; lvl < 6; lvl++) {
+                const bool bit = lvl == 1 ? m != MZERO : m == order[lvl];
+                bits |= (int)bit << lvl;
+                n++;
+                if (lvl == 1 ? !bit : bit) break;
+            }
+            mode_n[m] = (uint8_t)n; mode_bits[m] = (uint8_t)bits;
+        }
+        for (int k = 0; k <= 10; k++) {
+            int lo = 0, hi = 10, depth = 0, bits = 0;
+            while (lo < hi) { const int mid = (lo + hi) / 2; const bool up = k > mid; bits |= (int)up << depth++; if (up) lo = mid + 1; else hi = mid; }
+            cls_n[k] = (uint8_t)depth; cls_bits[k] = (uint8_t)bits;
+        }
+    }
+};
+const MvTrees kMvTrees;
+// Ctx::p0 of every context state
+struct P0Table {
+    uint8_t p[65536];
+    P0Table() { for (uint32_t c = 0; c < 65536; c++) { Ctx t; t.c = (uint16_t)c; p[c] = (uint8_t)t.p0(); } }
+};
+const P0Table kP0Table;
+// one coded decision: context index | bit << 8
+inline uint16_t mv_event(int ctx, bool bit) { return (uint16_t)(ctx | ((int)bit << 8)); }
+constexpr int kMvMaxEvents = 42;   // per block: skip 1, mode 6, base 4, hit 1, 2 x (class 4, sign 1, magnitude 9), frac 2
+
 // mv_mode_context (mv_predictor.rs:275-308): left + top, Zero = 2, New = 0, others 1, capped at 3
 int mode_ctx(const std::vector<uint8_t>& modes, int bw, int i) {
     int bx = i % bw, s = 0;
@@ -496,40 +531,73 @@ inline int8_t subpel_of_rank(int r) { return r == 0 ? 0 : r == 1 ? 1 : -1; }
 
 void MvCoder::reset() { *this = MvCoder(); }
 
+namespace {
+inline uint32_t mv_key(int dx, int dy, int rx, int ry) { return ((uint32_t)(dx + 128) << 24) | ((uint32_t)(dy + 128) << 16) | ((uint32_t)rx << 8) | (uint32_t)ry; }
+inline uint32_t mv_hash(uint32_t k) { return (k * 0x9E3779B1u) >> 7; }
+}  // namespace
+
 int MvCoder::lookup(int8_t dx, int8_t dy, int8_t sx, int8_t sy) const {
-    std::array<int, 4> k = {dx, dy, subpel_rank(sx), subpel_rank(sy)};
-    for (size_t i = 0; i < cand_.size(); i++) if (cand_[i] == k) return (int)i;
-    return -1;
+    if (cand_.empty()) return -1;
+    const uint32_t k = mv_key(dx, dy, subpel_rank(sx), subpel_rank(sy));
+    int r = -1;
+    for (int i = 15; i >= 0; i--) r = cand_key_[i] == k ? i : r;   // the first match
+    return r;
+}
+
+// The count of key (nullptr when absent and !insert; a new entry counts 0).
+uint32_t* MvCoder::find_count(uint32_t key, bool insert) {
+    const uint32_t mask = (uint32_t)index_.size() - 1;
+    for (uint32_t h = mv_hash(key) & mask;; h = (h + 1) & mask) {
+        const uint32_t s = index_[h];
+        if (s == 0) {
+            if (!insert) return nullptr;
+            live_.push_back({key, 0u});
+            index_[h] = (uint32_t)live_.size();
+            return &live_.back().second;
+        }
+        if (live_[s - 1].first == key) return &live_[s - 1].second;
+    }
 }
 
 // update_candidates (mv_rans/encoder.rs:370-412, decoder.rs:283-326): decay every count by
 // 7/8, count this frame's NEW (delta, sub-pel) tuples, drop zero counts; below 100 NEW blocks
 // of history the list is empty, else the top 16 by count (ties by key) above total/100.
 void MvCoder::refresh_history(const std::vector<MvBlock>& blocks, bool from_decoder) {
-    // counts are kept in a hash map on a 32-bit key whose unsigned order is the tuple order
-    auto key_of = [](int dx, int dy, int rx, int ry) { return ((uint32_t)(dx + 128) << 24) | ((uint32_t)(dy + 128) << 16) | ((uint32_t)rx << 8) | (uint32_t)ry; };
-    for (auto& e : counts_) e.second = e.second * 7 / 8;
+    for (auto& e : live_) e.second = e.second * 7 / 8;
     total_ = total_ * 7 / 8;
+    // index sized for every live key plus one new key per block at most half full
+    size_t need = 64;
+    while (need < 2 * (live_.size() + blocks.size())) need <<= 1;
+    if (index_.size() != need) index_.assign(need, 0u);
+    for (size_t i = 0; i < live_.size(); i++) {   // index the live keys (each once)
+        const uint32_t mask = (uint32_t)need - 1;
+        uint32_t h = mv_hash(live_[i].first) & mask;
+        while (index_[h]) h = (h + 1) & mask;
+        index_[h] = (uint32_t)i + 1;
+    }
     for (const MvBlock& b : blocks) {
         if (b.mode != MNEW) continue;
         if (b.sx == -1 || b.sy == -1) {
             if (from_decoder) continue;
             throw CorruptStream("-0.5 sub-pel in a NEW vector");
         }
-        counts_[key_of(b.dx, b.dy, subpel_rank(b.sx), subpel_rank(b.sy))]++;
+        (*find_count(mv_key(b.dx, b.dy, subpel_rank(b.sx), subpel_rank(b.sy)), true))++;
         total_++;
     }
-    for (auto it = counts_.begin(); it != counts_.end();) it = it->second == 0 ? counts_.erase(it) : std::next(it);
+    live_.erase(std::remove_if(live_.begin(), live_.end(), [](const std::pair<uint32_t, uint32_t>& e) { return e.second == 0; }), live_.end());
+    std::fill(index_.begin(), index_.end(), 0u);   // re-indexed from the live list at the next refresh
     cand_.clear();
+    for (uint32_t& k : cand_key_) k = ~0u;
     if (total_ < 100) return;
     const uint32_t floor_count = total_ / 100;
     std::vector<std::pair<uint32_t, uint32_t>> order;   // (count, key) above the floor
-    for (const auto& e : counts_) if (e.second > floor_count) order.push_back({e.second, e.first});
+    for (const auto& e : live_) if (e.second > floor_count) order.push_back({e.second, e.first});
     std::sort(order.begin(), order.end(), [](const std::pair<uint32_t, uint32_t>& a, const std::pair<uint32_t, uint32_t>& b) {
         return a.first != b.first ? a.first > b.first : a.second < b.second;
     });
     for (const auto& e : order) {
         const uint32_t k = e.second;
+        cand_key_[cand_.size()] = k;
         cand_.push_back({(int)(k >> 24) - 128, (int)((k >> 16) & 0xff) - 128, (int)((k >> 8) & 0xff), (int)(k & 0xff)});
         if (cand_.size() == 16) break;
     }
@@ -545,63 +613,76 @@ std::vector<uint8_t> MvCoder::encode(const std::vector<MvBlock>& blocks, int bw,
 
 void MvCoder::model(const std::vector<MvBlock>& blocks, int bw, int bh, RansWriter& w) {
     (void)bh;
-    std::vector<uint8_t> modes;
-    std::vector<uint8_t> hits;
-    modes.reserve(blocks.size()); hits.reserve(blocks.size());
-    auto component = [&](int8_t d, int axis) {   // encode_component (mv_rans/encoder.rs:303-331)
-        int mag = std::abs((int)d), k = mv_class(mag);
-        int lo = 0, hi = 10, depth = 0;
-        while (lo < hi) { int mid = (lo + hi) / 2; bool up = k > mid; w.bit(up, cls_[axis][depth++]); if (up) lo = mid + 1; else hi = mid; }
-        if (k == 0) return;
-        w.bit(d < 0, sign_[axis]);
-        int off = mag - (1 << (k - 1));
-        for (int b = 0; b < k - 1; b++) w.bit((off >> b) & 1, mag_[axis][b]);
+    const int nb = (int)blocks.size();
+    // 1. the frame's decisions in coding order, as (context, bit) events; every tree writes its
+    //    full depth and advances by the decisions it takes, so the loop has few branches
+    thread_local std::vector<uint16_t> evs;
+    thread_local std::vector<uint8_t> hitv;
+    if (evs.size() < (size_t)nb * kMvMaxEvents + 16) evs.resize((size_t)nb * kMvMaxEvents + 16);
+    hitv.assign((size_t)nb, 0);
+    uint16_t* e = evs.data();
+    const MvTrees& T = kMvTrees;
+    auto score = [](uint8_t m) { return m == MZERO ? 2 : m == MNEW ? 0 : 1; };
+    auto component = [&T](uint16_t* e, int8_t d, int axis) {   // encode_component (mv_rans/encoder.rs:303-331)
+        const int mag = std::abs((int)d), k = mv_class(mag), cb = T.cls_bits[k];
+        for (int depth = 0; depth < 4; depth++) e[depth] = mv_event(kCls + axis * 4 + depth, (cb >> depth) & 1);
+        e += T.cls_n[k];
+        if (k == 0) return e;
+        *e++ = mv_event(kSign + axis, d < 0);
+        const int off = mag - (1 << (k - 1));
+        for (int b = 0; b < 9; b++) e[b] = mv_event(kMag + axis * 10 + b, (off >> b) & 1);
+        return e + (k - 1);
     };
-    for (size_t i = 0; i < blocks.size(); i++) {
+    const bool have_cand = !cand_.empty();
+    for (int i = 0, bx = 0; i < nb; i++, bx = bx + 1 == bw ? 0 : bx + 1) {
         const MvBlock& b = blocks[i];
-        const int bx = (int)(i % bw);
-        w.bit(b.skip, skip_);
-        const int ctx = mode_ctx(modes, bw, (int)i);
-        // mode tree: New | Zero | Near | Temporal | TopRight | TopLeft | Nearest (mv_rans/encoder.rs:257-301)
-        static const uint8_t order[6] = {MNEW, 0xff, MNEAR, MTEMP, MTR, MTL};
-        for (int lvl = 0; lvl < 6; lvl++) {
-            bool stop;
-            if (lvl == 1) { bool nz = b.mode != MZERO; w.bit(nz, mode_[1][ctx]); stop = !nz; }
-            else { bool is = b.mode == order[lvl]; w.bit(is, mode_[lvl][ctx]); stop = is; }
-            if (stop) break;
-        }
+        *e++ = mv_event(kSkip, b.skip);
+        const int ctx = std::min((bx > 0 ? score(blocks[i - 1].mode) : 0) + (i >= bw ? score(blocks[i - bw].mode) : 0), 3);
+        const int m = b.mode < 8 ? b.mode : 7, mb = T.mode_bits[m];
+        for (int lvl = 0; lvl < 6; lvl++) e[lvl] = mv_event(kMode + lvl * 4 + ctx, (mb >> lvl) & 1);
+        e += T.mode_n[m];
+        if (b.mode != MNEW) continue;
+        // base selector (nearest, near, top-right, top-left | temporal)
+        for (int lvl = 0; lvl < 4; lvl++) e[lvl] = mv_event(kBase + lvl * 4 + ctx, b.new_base == lvl);
+        e += b.new_base < 4 ? b.new_base + 1 : 4;
         bool hit = false;
-        if (b.mode == MNEW) {
-            for (int lvl = 0; lvl < 4; lvl++) {   // base selector (nearest, near, top-right, top-left | temporal)
-                bool is = b.new_base == lvl;
-                w.bit(is, base_[lvl][ctx]);
-                if (is) break;
-            }
-            bool frac_done = false;
-            if (!cand_.empty()) {
-                int ci = lookup(b.dx, b.dy, b.sx, b.sy);
-                hit = ci >= 0;
-                int hctx = (bx > 0 ? hits[i - 1] : 0) + (i >= (size_t)bw ? hits[i - bw] : 0);
-                w.bit(hit, hit_[hctx]);
-                if (hit) {
-                    int lo = 0, hi = 16, depth = 0;
-                    while (hi - lo > 1) { int mid = (lo + hi) / 2; bool up = ci >= mid; w.bit(up, idx_[depth++]); if (up) lo = mid; else hi = mid; }
-                    frac_done = true;
-                } else {
-                    component(b.dx, 0); component(b.dy, 1);
-                }
-            } else {
-                component(b.dx, 0); component(b.dy, 1);
-            }
-            if (!frac_done) {
-                if (b.sx == -1 || b.sy == -1) throw CorruptStream("-0.5 sub-pel in a NEW vector");
-                w.bit(b.sx != 0, frac_[0]);
-                w.bit(b.sy != 0, frac_[1]);
+        if (have_cand) {
+            const int ci = lookup(b.dx, b.dy, b.sx, b.sy);
+            hit = ci >= 0;
+            *e++ = mv_event(kHit + (bx > 0 ? hitv[i - 1] : 0) + (i >= bw ? hitv[i - bw] : 0), hit);
+            if (hit) {   // the index, a binary search over 16: its 4 bits, most significant first
+                for (int d = 0; d < 4; d++) e[d] = mv_event(kIdx + d, (ci >> (3 - d)) & 1);
+                e += 4;
             }
         }
-        hits.push_back(hit);
-        modes.push_back(b.mode);
+        if (!hit) {
+            e = component(e, b.dx, 0);
+            e = component(e, b.dy, 1);
+            if (b.sx == -1 || b.sy == -1) throw CorruptStream("-0.5 sub-pel in a NEW vector");
+            e[0] = mv_event(kFrac, b.sx != 0);
+            e[1] = mv_event(kFrac + 1, b.sy != 0);
+            e += 2;
+        }
+        hitv[i] = hit;
     }
+    // 2. the adaptive contexts over the events, in order, each giving its rANS symbol
+    const size_t n = (size_t)(e - evs.data());
+    uint16_t* o = w.reserve(n);
+    const uint16_t* ev = evs.data();
+    uint16_t* const cs = &ctx_[0].c;
+    static_assert(sizeof(Ctx) == sizeof(uint16_t), "Ctx is its two counters");
+    for (size_t j = 0; j < n; j++) {
+        const uint32_t v = ev[j], bit = v >> 8, id = v & 0xffu, c = cs[id];
+        const uint32_t pz = kP0Table.p[c];                                // Ctx::p0
+        // Ctx::update: the observed counter steps by one, unless it is saturated (rare)
+        if (__builtin_expect(((c >> ((bit ^ 1u) << 3)) & 0xffu) == 0xffu, 0)) {
+            Ctx t; t.c = (uint16_t)c; t.update_saturated(bit != 0); cs[id] = t.c;
+        } else {
+            cs[id] = (uint16_t)(c + 0x100u - 0xffu * bit);
+        }
+        o[j] = (uint16_t)(pz ^ ((pz ^ ((pz << 8) | (256u - pz))) & (0u - bit)));
+    }
+    w.commit(o + n);
     refresh_history(blocks, false);
 }
 
@@ -613,12 +694,12 @@ std::vector<MvBlock> MvCoder::decode(const uint8_t* data, size_t len, int bw, in
     modes.reserve(nb); hits.reserve(nb);
     auto component = [&](int axis) -> int8_t {   // decode_component (mv_rans/decoder.rs:375-401)
         int lo = 0, hi = 10, depth = 0;
-        while (lo < hi) { int mid = (lo + hi) / 2; if (r.bit(cls_[axis][depth++])) lo = mid + 1; else hi = mid; }
+        while (lo < hi) { int mid = (lo + hi) / 2; if (r.bit(ctx_[kCls + axis * 4 + depth++])) lo = mid + 1; else hi = mid; }
         int k = lo;
         if (k == 0) return 0;
-        bool neg = r.bit(sign_[axis]);
+        bool neg = r.bit(ctx_[kSign + axis]);
         int off = 0;
-        for (int b = 0; b < k - 1; b++) if (r.bit(mag_[axis][b])) off |= 1 << b;
+        for (int b = 0; b < k - 1; b++) if (r.bit(ctx_[kMag + axis * 10 + b])) off |= 1 << b;
         int v = (1 << (k - 1)) + off;
         v = neg ? -v : v;
         return (int8_t)std::max(-128, std::min(127, v));
@@ -626,26 +707,26 @@ std::vector<MvBlock> MvCoder::decode(const uint8_t* data, size_t len, int bw, in
     for (int i = 0; i < nb; i++) {
         MvBlock b{};
         const int bx = i % bw;
-        b.skip = r.bit(skip_);
+        b.skip = r.bit(ctx_[kSkip]);
         const int ctx = mode_ctx(modes, bw, i);
-        if (r.bit(mode_[0][ctx])) b.mode = MNEW;
-        else if (!r.bit(mode_[1][ctx])) b.mode = MZERO;
-        else if (r.bit(mode_[2][ctx])) b.mode = MNEAR;
-        else if (r.bit(mode_[3][ctx])) b.mode = MTEMP;
-        else if (r.bit(mode_[4][ctx])) b.mode = MTR;
-        else if (r.bit(mode_[5][ctx])) b.mode = MTL;
+        if (r.bit(ctx_[kMode + ctx])) b.mode = MNEW;
+        else if (!r.bit(ctx_[kMode + 4 + ctx])) b.mode = MZERO;
+        else if (r.bit(ctx_[kMode + 8 + ctx])) b.mode = MNEAR;
+        else if (r.bit(ctx_[kMode + 12 + ctx])) b.mode = MTEMP;
+        else if (r.bit(ctx_[kMode + 16 + ctx])) b.mode = MTR;
+        else if (r.bit(ctx_[kMode + 20 + ctx])) b.mode = MTL;
         else b.mode = MNEAREST;
         bool hit = false;
         if (b.mode == MNEW) {
             b.new_base = 4;
-            for (int lvl = 0; lvl < 4; lvl++) if (r.bit(base_[lvl][ctx])) { b.new_base = (uint8_t)lvl; break; }
+            for (int lvl = 0; lvl < 4; lvl++) if (r.bit(ctx_[kBase + lvl * 4 + ctx])) { b.new_base = (uint8_t)lvl; break; }
             bool frac_done = false;
             if (!cand_.empty()) {
                 int hctx = (bx > 0 ? hits[i - 1] : 0) + (i >= bw ? hits[i - bw] : 0);
-                hit = r.bit(hit_[hctx]);
+                hit = r.bit(ctx_[kHit + hctx]);
                 if (hit) {
                     int lo = 0, hi = 16, depth = 0;
-                    while (hi - lo > 1) { int mid = (lo + hi) / 2; if (r.bit(idx_[depth++])) lo = mid; else hi = mid; }
+                    while (hi - lo > 1) { int mid = (lo + hi) / 2; if (r.bit(ctx_[kIdx + depth++])) lo = mid; else hi = mid; }
                     if (lo < (int)cand_.size()) {
                         const auto& k = cand_[lo];
                         b.dx = (int8_t)k[0]; b.dy = (int8_t)k[1]; b.sx = subpel_of_rank(k[2]); b.sy = subpel_of_rank(k[3]);
@@ -657,7 +738,7 @@ std::vector<MvBlock> MvCoder::decode(const uint8_t* data, size_t len, int bw, in
             } else {
                 b.dx = component(0); b.dy = component(1);
             }
-            if (!frac_done) { b.sx = r.bit(frac_[0]) ? 1 : 0; b.sy = r.bit(frac_[1]) ? 1 : 0; }
+            if (!frac_done) { b.sx = r.bit(ctx_[kFrac]) ? 1 : 0; b.sy = r.bit(ctx_[kFrac + 1]) ? 1 : 0; }
         }
         hits.push_back(hit);
         modes.push_back(b.mode);

@@ -6,7 +6,6 @@
 #include <cstddef>
 #include <vector>
 #include <array>
-#include <unordered_map>
 #include <stdexcept>
 
 namespace rivh {
@@ -30,7 +29,12 @@ struct Ctx {
     }
     // BinProb::observe (rans.rs:73-84): the counter of the observed symbol sits in the high byte;
     // on overflow both counters are halved (rounding up), the observed one re-based.
+    // The observed counter steps by one; only a saturated one (255, rare) takes the general rule.
     void update(bool bit) {
+        if (__builtin_expect(((c >> (bit ? 0 : 8)) & 0xffu) == 0xffu, 0)) update_saturated(bit);
+        else c = (uint16_t)(c + (bit ? 1u : 0x100u));
+    }
+    void update_saturated(bool bit) {
         const uint32_t v = bit ? (uint32_t)(((c << 8) | (c >> 8)) & 0xffffu) : c;
         const uint32_t halved = (((v + 0x101u) & 0xffffu) >> 1) | (v == 0xff01u ? 0xff00u : 0x8100u);
         const uint32_t n = v >= 0xff00u ? halved : v + 0x100u;
@@ -127,16 +131,29 @@ public:
 private:
     void refresh_history(const std::vector<MvBlock>& blocks, bool from_decoder);
     int lookup(int8_t dx, int8_t dy, int8_t sx, int8_t sy) const;
-    Ctx skip_;
-    Ctx mode_[6][4];         // is_new, nonzero, is_near, is_temporal, is_top_right, is_top_left
-    Ctx base_[4][4];         // nearest, near, top_right, top_left
-    Ctx cls_[2][4], sign_[2], mag_[2][10];
-    Ctx frac_[2];
-    Ctx hit_[3], idx_[4];
+    // the adaptive contexts, in one array so that a coded decision can name its context by index
+    enum : int {
+        kSkip = 0,
+        kMode = 1,      // [6][4]: is_new, nonzero, is_near, is_temporal, is_top_right, is_top_left x mode context
+        kBase = 25,     // [4][4]: nearest, near, top_right, top_left x mode context
+        kCls = 41,      // [2][4]: per axis, by depth of the class tree
+        kSign = 49,     // [2]
+        kMag = 51,      // [2][10]
+        kFrac = 71,     // [2]
+        kHit = 73,      // [3]
+        kIdx = 76,      // [4]
+        kNumCtx = 80
+    };
+    Ctx ctx_[kNumCtx];
     // candidate history: key = (dx, dy, sub-pel order x, sub-pel order y)
     std::vector<std::array<int, 4>> cand_;
-    std::unordered_map<uint32_t, uint32_t> counts_;   // key: (dx+128, dy+128, sub-pel order x, y) bytes
+    uint32_t cand_key_[16];   // the candidates' packed keys (below), the unused ones ~0u (matches no key)
+    // counts of the packed keys (dx+128, dy+128, sub-pel order x, y as bytes, high to low, so that
+    // unsigned order is tuple order): a list of the live ones and an open-addressing index into it
+    std::vector<std::pair<uint32_t, uint32_t>> live_;   // (key, count)
+    std::vector<uint32_t> index_;                       // slot: live_ position + 1, 0 = empty
     uint32_t total_ = 0;
+    uint32_t* find_count(uint32_t key, bool insert);
 };
 
 int mv_class(int magnitude);   // mv_class_from_magnitude (mv_rans.rs:81-87)
