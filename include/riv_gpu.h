@@ -192,6 +192,10 @@ int riv_gpu_dct16_aq_plane(int32_t device_id, const int16_t* plane, uint32_t w, 
                            int16_t* coeffs, int16_t* recon);
 /* compute_aq_quant_steps (reitero_residual/src/residual.rs:116-167) on a luma plane */
 int riv_gpu_aq_steps(int32_t device_id, const uint8_t* y, uint32_t w, uint32_t h, float base_step, float* out);
+/* The adaptive quantiser's mean of n per-MB logs (the sequential f32 sum of
+ * reitero_residual/src/residual.rs:150-155, divided by n) and the steps base_step derives from it;
+ * every log must be positive, as the logs of floored variances are. steps may be NULL. */
+int riv_gpu_aq_mean(int32_t device_id, const float* logs, size_t n, float base_step, float* mean, float* steps);
 /* The motion search's SAD tables (sad_block_int_limit, satd_scalar.rs:4-99, and
  * sad_block_halfpel_limit_luma, satd_scalar.rs:171-223, at every candidate the search can visit):
  * per 16x16 block, int_sad[(dy+sr)*(2sr+1) + dx+sr] for dx,dy in [-sr,sr] and

@@ -905,55 +905,184 @@ __global__ __launch_bounds__(256) void k_mc(const uint8_t* __restrict__ rY, cons
 // ============================================================================================
 // variance = ((sum_sq*256 - sum^2) as f32) / 65536, floored at 4, and its natural log with
 // glibc's logf (computed per MB where the sums are formed, in k_mc); the mean of the logs is a
-// sequential f32 sum in raster order (one lane: the rounding sequence is part of the result);
+// sequential f32 sum in raster order (the rounding sequence is part of the result);
 // step = base * exp2f((0.8 * (ln - avg)) / 6).
-
-// One workgroup: the logs pass through LDS in chunks; lane 0 adds them in order while the next
-// 64 values are already on their way from LDS (two register buffers), so LDS latency hides behind
-// the dependent adds. The mean goes to lnv[nb].
-__device__ __forceinline__ float add64(float sum, const float4* q) {
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-        sum = __fadd_rn(sum, q[k].x); sum = __fadd_rn(sum, q[k].y);
-        sum = __fadd_rn(sum, q[k].z); sum = __fadd_rn(sum, q[k].w);
-    }
-    return sum;
+//
+// The sequential sum, exactly, in parallel. Every log is >= ln 4 > 0, so the running sum only
+// grows. While it stays within one binade [2^e, 2^(e+1)) every partial sum is a multiple of that
+// binade's ulp u, and adding x rounds to s + u * (x rounded to a multiple of u), independent of s
+// except when x lies exactly halfway between multiples of u, where ties-to-even makes it depend
+// on the parity of s / u (the parity that follows is determined too). So a run of values adds a
+// fixed K(e, parity) ulps to any start in binade e with that parity, provided the result stays in
+// binade e (the partial sums in between are then in it as well); and adding K ulps to a float of
+// binade e is adding K to its bit pattern, the exponent field unchanged.
+// Runs of 32 values (sub-chunks) get K for the binade their (f64) prefix sum predicts and the
+// adjacent one, both parities, by running them in f32 from 2^e (+u); runs of 8 sub-chunks
+// (chunks) compose their sub-chunks' K the same way in integers. One lane then chains the chunks,
+// adding K to the sum's bits where the binade and parity match and the exponent field stays; a
+// chunk that does not (it crosses a binade, or is among the first) is chained by its sub-chunks,
+// and a sub-chunk that does not by its values one by one, in f32. The result is the sequential
+// sum bit for bit whatever the estimates; they only decide how often the slower levels run.
+constexpr int kAqSub = 32;          // values per sub-chunk
+constexpr int kAqGroup = 8;         // sub-chunks per chunk
+constexpr int kAqStage = 32768;     // most logs the parallel path takes (staged in LDS)
+constexpr int kAqNoK = 0x40000000;  // no K: adding it changes the exponent field
+__device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
+// log j's LDS word: an XOR swizzle inside each group of 32, so that the lanes of a wave, each
+// reading value i of its own sub-chunk, hit 32 different banks
+__device__ __forceinline__ int aq_pos(int j) { return j ^ ((j >> 5) & 31); }
+// the binade of P and its neighbour on the side P is near
+__device__ __forceinline__ int2 aq_binades(double P) {
+    const float pf = (float)P;
+    const int e = (int)(fbits(pf) >> 23);
+    const uint32_t man = fbits(pf) & 0x7fffffu;
+    return make_int2(e, man < (1u << 17) ? e - 1 : (man >= 0x7fffffu - (1u << 17) ? e + 1 : e));
 }
-__global__ __launch_bounds__(256) void k_aq(float* __restrict__ lnv, int nb, float base, float* __restrict__ steps) {
-    __shared__ float4 s_ln[2048 + 16];   // 8192 logs per pass, plus one chunk of slack for the read-ahead
-    float sum = 0.0f;                    // lives in thread 0
-    for (int c0 = 0; c0 < nb; c0 += 8192) {
-        __syncthreads();
-        const int cn = min(8192, nb - c0);
-        float* sl = (float*)s_ln;
-        for (int i = threadIdx.x; i < cn; i += blockDim.x) sl[i] = lnv[c0 + i];
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const int full = cn & ~63;
-            float4 a[16], b[16];
-            if (full) {
-#pragma unroll
-                for (int k = 0; k < 16; k++) a[k] = s_ln[k];
-            }
-            int i = 0;
-            for (; i + 128 <= full; i += 128) {
-#pragma unroll
-                for (int k = 0; k < 16; k++) b[k] = s_ln[((i + 64) >> 2) + k];
-                sum = add64(sum, a);
-#pragma unroll
-                for (int k = 0; k < 16; k++) a[k] = s_ln[((i + 128) >> 2) + k];   // slack covers the last read-ahead
-                sum = add64(sum, b);
-            }
-            if (i < full) { sum = add64(sum, a); i += 64; }
-            for (; i < cn; i++) sum = __fadd_rn(sum, sl[i]);
+// K for binade eb.x and eb.y (even, odd start) at sum bits b (exponent field e), or kAqNoK
+__device__ __forceinline__ int aq_pick(uint32_t b, int4 k, int2 eb) {
+    const int e = (int)(b >> 23);
+    const bool odd = (b & 1u) != 0u;
+    return e == eb.x ? (odd ? k.y : k.x) : (e == eb.y ? (odd ? k.w : k.z) : kAqNoK);
+}
+
+size_t aq_lds_bytes() {
+    const int nsub = kAqStage / kAqSub, nch = nsub / kAqGroup;
+    return (size_t)kAqStage * 4 + (size_t)nsub * (8 + 16) + (size_t)nch * (8 + 16 + 8) + 16;
+}
+
+__global__ __launch_bounds__(1024) void k_aq(float* __restrict__ lnv, int nb, float base, float* __restrict__ steps) {
+    extern __shared__ __align__(16) uint32_t aq_lds[];
+    const int t = threadIdx.x, nt = blockDim.x, lane = t & 63;
+    const int nsub = (nb + kAqSub - 1) / kAqSub, nch = (nsub + kAqGroup - 1) / kAqGroup;
+    const int NSUB = kAqStage / kAqSub, NCH = NSUB / kAqGroup;
+    float* xs = (float*)aq_lds;
+    double* pre = (double*)(xs + kAqStage);      // sub-chunk sums, then their exclusive prefix; then binades
+    int2* sbin = (int2*)pre;
+    int4* sk = (int4*)(pre + NSUB);
+    double* cpre = (double*)(sk + NSUB);
+    int4* ck = (int4*)(cpre + NCH);
+    int2* cbin = (int2*)(ck + NCH);
+    float* s_mean = (float*)(cbin + NCH);
+    if (nb > kAqStage) {   // beyond the staged size: the plain sequential sum
+        if (t == 0) {
+            float s = 0.0f;
+            for (int i = 0; i < nb; i++) s = __fadd_rn(s, lnv[i]);
+            *s_mean = __fdiv_rn(s, (float)nb);
+            lnv[nb] = *s_mean;
         }
+        __syncthreads();
+    } else {
+        for (int j0 = t; j0 < nb; j0 += 8 * nt) {   // 8 loads in flight per thread
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = j0 + u * nt < nb ? lnv[j0 + u * nt] : 0.0f;
+#pragma unroll
+            for (int u = 0; u < 8; u++) if (j0 + u * nt < nb) xs[aq_pos(j0 + u * nt)] = v[u];
+        }
+        __syncthreads();
+        for (int c = t; c < nsub; c += nt) {   // sub-chunk sums (f64: only an estimate, any order will do)
+            const int lo = c * kAqSub, hi = min(lo + kAqSub, nb);
+            double cs = 0.0;
+            for (int i = lo; i < hi; i++) cs += (double)xs[aq_pos(i)];
+            pre[c] = cs;
+        }
+        __syncthreads();
+        if (t < 64) {   // exclusive prefix of the sub-chunk sums, 64 at a time
+            double carry = 0.0;
+            for (int g = 0; g < nsub; g += 64) {
+                const double v = g + lane < nsub ? pre[g + lane] : 0.0;
+                double inc = v;
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const double o = __shfl_up(inc, d);
+                    if (lane >= d) inc += o;
+                }
+                if (g + lane < nsub) pre[g + lane] = carry + inc - v;
+                carry += __shfl(inc, 63);
+            }
+        }
+        __syncthreads();
+        for (int k = t; k < nch; k += nt) cpre[k] = pre[k * kAqGroup];
+        __syncthreads();
+        for (int c = t; c < nsub; c += nt) {   // sub-chunk K: the run in f32 from 2^e (+u)
+            const int lo = c * kAqSub, hi = min(lo + kAqSub, nb);
+            const int2 eb = aq_binades(pre[c]);   // read before sbin[c] overwrites it
+            const uint32_t b0 = (uint32_t)eb.x << 23, b2 = (uint32_t)eb.y << 23;
+            float s0 = __uint_as_float(b0), s1 = __uint_as_float(b0 | 1u), s2 = __uint_as_float(b2), s3 = __uint_as_float(b2 | 1u);
+            bool pos = true;
+            for (int i = lo; i < hi; i++) {
+                const float x = xs[aq_pos(i)];
+                pos &= x > 0.0f;
+                s0 = __fadd_rn(s0, x); s1 = __fadd_rn(s1, x); s2 = __fadd_rn(s2, x); s3 = __fadd_rn(s3, x);
+            }
+            const bool ve = pos && eb.x >= 1 && eb.x <= 253, va = pos && eb.y >= 1 && eb.y <= 253;
+            int4 k;
+            k.x = ve && (fbits(s0) >> 23) == (uint32_t)eb.x ? (int)(fbits(s0) - b0) : kAqNoK;
+            k.y = ve && (fbits(s1) >> 23) == (uint32_t)eb.x ? (int)(fbits(s1) - (b0 | 1u)) : kAqNoK;
+            k.z = va && (fbits(s2) >> 23) == (uint32_t)eb.y ? (int)(fbits(s2) - b2) : kAqNoK;
+            k.w = va && (fbits(s3) >> 23) == (uint32_t)eb.y ? (int)(fbits(s3) - (b2 | 1u)) : kAqNoK;
+            sk[c] = k;
+            sbin[c] = eb;
+        }
+        __syncthreads();
+        for (int q = t; q < 4 * nch; q += nt) {   // chunk K, one (chunk, binade, parity) per thread
+            const int k = q >> 2, which = q & 3;
+            const int2 eb = aq_binades(cpre[k]);
+            const int e = which < 2 ? eb.x : eb.y;
+            const uint32_t b0 = ((uint32_t)e << 23) | (uint32_t)(which & 1);
+            uint32_t b = b0;
+            bool ok = e >= 1 && e <= 253;
+            for (int c = k * kAqGroup; ok && c < min((k + 1) * kAqGroup, nsub); c++) {
+                b += (uint32_t)aq_pick(b, sk[c], sbin[c]);
+                ok = (b >> 23) == (uint32_t)e;
+            }
+            ((int*)&ck[k])[which] = ok ? (int)(b - b0) : kAqNoK;
+            if (which == 0) cbin[k] = eb;
+        }
+        __syncthreads();
+        if (t == 0) {
+            // one lane; every LDS read is issued ahead of the step that needs it (the next chunk's
+            // K, a chunk's 8 sub-chunks together, a sub-chunk's 32 values together)
+            uint32_t b = 0u;   // the sum's bits
+            int4 nk = ck[0];
+            int2 ne = cbin[0];
+            for (int k = 0; k < nch; k++) {
+                const int4 kk = nk;
+                const int2 ee = ne;
+                if (k + 1 < nch) { nk = ck[k + 1]; ne = cbin[k + 1]; }
+                const uint32_t nx = b + (uint32_t)aq_pick(b, kk, ee);
+                if ((nx >> 23) == (b >> 23)) { b = nx; continue; }
+                const int c0 = k * kAqGroup, nsc = min(kAqGroup, nsub - c0);
+                int4 sks[kAqGroup];
+                int2 sbs[kAqGroup];
+#pragma unroll
+                for (int j = 0; j < kAqGroup; j++) {
+                    sks[j] = j < nsc ? sk[c0 + j] : make_int4(0, 0, 0, 0);
+                    sbs[j] = j < nsc ? sbin[c0 + j] : make_int2(-9, -9);
+                }
+#pragma unroll
+                for (int j = 0; j < kAqGroup; j++) {
+                    if (j >= nsc) continue;
+                    const uint32_t ns = b + (uint32_t)aq_pick(b, sks[j], sbs[j]);
+                    if ((ns >> 23) == (b >> 23)) { b = ns; continue; }
+                    const int lo = (c0 + j) * kAqSub;
+                    float v[kAqSub];   // past nb: +0, which leaves a positive sum as it is
+#pragma unroll
+                    for (int i = 0; i < kAqSub; i++) v[i] = lo + i < nb ? xs[aq_pos(lo + i)] : 0.0f;
+                    float sf = __uint_as_float(b);
+#pragma unroll
+                    for (int i = 0; i < kAqSub; i++) sf = __fadd_rn(sf, v[i]);
+                    b = fbits(sf);
+                }
+            }
+            *s_mean = __fdiv_rn(__uint_as_float(b), (float)nb);
+            lnv[nb] = *s_mean;
+        }
+        __syncthreads();
     }
-    __shared__ float s_mean;
-    if (threadIdx.x == 0) { s_mean = __fdiv_rn(sum, (float)nb); lnv[nb] = s_mean; }
-    __syncthreads();
     // per-MB steps: base * exp2(0.8 * (ln v - mean) / 6) in f32 (residual.rs:150-166)
-    const float mean = s_mean;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    const float mean = *s_mean;
+    for (int b = t; b < nb; b += nt) {
         const float off = __fmul_rn(0.8f, __fsub_rn(lnv[b], mean));
         steps[b] = __fmul_rn(base, glibc_exp2f(__fdiv_rn(off, 6.0f)));
     }
@@ -1430,7 +1559,7 @@ void mc(hipStream_t st, const uint8_t* rY, const uint8_t* rU, const uint8_t* rV,
     hipLaunchKernelGGL(k_mc, dim3((W / 16) * (H / 16)), dim3(256), 0, st, rY, rU, rV, W, H, mvs, mv_stride, pY, pU, pV, var_sums, lnv);
 }
 void aq(hipStream_t st, int nb, float base, float* lnv, float* steps) {
-    hipLaunchKernelGGL(k_aq, dim3(1), dim3(256), 0, st, lnv, nb, base, steps);
+    hipLaunchKernelGGL(k_aq, dim3(1), dim3(1024), aq_lds_bytes(), st, lnv, nb, base, steps);
 }
 void logf_exp2f(hipStream_t st, const float* in, size_t n, int which, float* out) {
     hipLaunchKernelGGL(k_logf_exp2f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, n, which, out);

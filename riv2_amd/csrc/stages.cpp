@@ -117,6 +117,20 @@ int riv_gpu_aq_steps(int32_t dev, const uint8_t* y, uint32_t w, uint32_t h, floa
     return RIV_OK;
 }
 
+int riv_gpu_aq_mean(int32_t dev, const float* logs, size_t n, float base, float* mean, float* steps) {
+    if (!logs || !mean || !n || n > (size_t)INT32_MAX - 1) return RIV_E_ARG;
+    int r = ensure_device(dev);
+    if (r) return r;
+    DevBuf lnv, st;
+    RIV_HIP(lnv.alloc(4 * (n + 1))); RIV_HIP(st.alloc(4 * n));
+    RIV_HIP(hipMemcpy(lnv.p, logs, 4 * n, hipMemcpyHostToDevice));
+    rivk::aq(0, (int)n, base, lnv.as<float>(), st.as<float>());
+    RIV_HIP(hipGetLastError());
+    RIV_HIP(hipMemcpy(mean, lnv.as<float>() + n, 4, hipMemcpyDeviceToHost));
+    if (steps) RIV_HIP(hipMemcpy(steps, st.p, 4 * n, hipMemcpyDeviceToHost));
+    return RIV_OK;
+}
+
 int riv_gpu_logf_exp2f(int32_t dev, const float* in, size_t n, int32_t which, float* out) {
     if (!in || !out) return RIV_E_ARG;
     if (!n) return RIV_OK;

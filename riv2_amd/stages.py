@@ -53,6 +53,15 @@ def aq_steps(y, base_step, device=0):
     return out
 
 
+def aq_mean(logs, base_step, device=0):
+    """The adaptive quantiser's mean of positive per-MB logs and the steps derived from it."""
+    x = np.ascontiguousarray(logs, np.float32)
+    mean = np.zeros(1, np.float32)
+    steps = np.zeros_like(x)
+    _check(L.lib().riv_gpu_aq_mean(device, _p(x), x.size, float(base_step), _p(mean), _p(steps)))
+    return mean[0], steps
+
+
 def logf_exp2f(x, which, device=0):
     """which 0: logf, 1: exp2f, elementwise on the device."""
     x = np.ascontiguousarray(x, np.float32)

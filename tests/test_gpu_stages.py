@@ -103,6 +103,23 @@ def test_aq_steps_against_oracle():
         assert np.array_equal(g.view(np.uint32), o.view(np.uint32)), base
 
 
+def test_aq_mean_is_the_sequential_f32_sum():
+    """The device computes the mean of the logs in parallel; it must equal the sequential f32 sum
+    (a left fold, as numpy's add.accumulate is) divided by n, bit for bit, also on inputs made of
+    few distinct values, where halfway cases (ties) are common, and on frames of 4K and above."""
+    rng = np.random.default_rng(9)
+    cases = []
+    for n in (1, 31, 396, 8160, 32400, 130000):
+        cases.append(rng.uniform(np.log(4.0), np.log(65025.0), n).astype(np.float32))
+        cases.append((np.float32(np.log(4.0)) + rng.integers(0, 64, n).astype(np.float32) / np.float32(8)).astype(np.float32))
+        cases.append(np.ldexp(rng.integers(1, 4096, n), -rng.integers(0, 12, n)).astype(np.float32))
+    cases.append(np.full(8160, np.float32(np.log(4.0)), np.float32))
+    for i, x in enumerate(cases):
+        mean, steps = stages.aq_mean(x, 23.424242)
+        want = np.add.accumulate(x, dtype=np.float32)[-1] / np.float32(x.size)
+        assert np.float32(mean).view(np.uint32) == np.float32(want).view(np.uint32), (i, x.size, mean, want)
+
+
 def test_device_logf_exp2f_match_libm():
     """The device restatement of glibc's logf/exp2f agrees with the host libm bit for bit over the
     adaptive quantiser's domains (log of variances in [4, 65025], exp2 of offsets in [-2, 2])."""
