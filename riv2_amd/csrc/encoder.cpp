@@ -53,8 +53,9 @@ struct Slot {
 
 class Pool {
 public:
-    void start(int n) {
-        for (int i = 0; i < n; i++) th_.emplace_back([this] { run(); });
+    // n workers, each running init once before taking jobs
+    void start(int n, std::function<void()> init) {
+        for (int i = 0; i < n; i++) th_.emplace_back([this, init] { if (init) init(); run(); });
     }
     void submit(std::function<void()> f) {
         { std::lock_guard<std::mutex> lk(mu_); q_.push_back(std::move(f)); }
@@ -603,7 +604,8 @@ static HostStage* acquire_host_stage() {
         g_hs = new HostStage();
         // workers: up to 16, leaving two cores to the submitting thread and the completion thread
         const int cores = (int)std::max(3u, std::thread::hardware_concurrency());
-        g_hs->pool.start(std::max(1, std::min(16, cores - 2)));
+        // each worker's rANS buffers sized for a 1080p frame's residual symbols up front
+        g_hs->pool.start(std::max(1, std::min(16, cores - 2)), [] { rivh::RansWriter::prewarm(1u << 20); });
         g_hs->waiter = std::thread(waiter_loop, g_hs);
     }
     g_hs_refs++;

@@ -62,6 +62,17 @@ RansWriter::~RansWriter() {
     }
 }
 
+void RansWriter::prewarm(size_t symbols) {
+    if (!tl_pool.empty()) return;
+    tl_pool.emplace_back();
+    WriterBuffers& b = tl_pool.back();
+    size_t cap = 1u << 16;
+    while (cap < symbols) cap *= 2;
+    b.buf.assign(cap, 0);
+    for (auto& t : b.tmp) t.assign((size_t)kBatch + 8, 0);
+    for (auto& t : b.tmp8) t.assign((size_t)kBatch + 8, 0);
+}
+
 void RansWriter::grow() {
     buf_.resize(std::max<size_t>(cap_ * 2, 1u << 16));
     sym_ = buf_.data();

@@ -62,6 +62,9 @@ public:
     uint16_t* reserve(size_t n) { while (cap_ - n_ < n) grow(); return sym_ + n_; }
     void commit(const uint16_t* end) { n_ = (size_t)(end - sym_); }
     void flush();
+    // Fills the calling thread's buffer pool (when empty) with buffers for `symbols` symbols, their
+    // pages touched, so that the thread's first frames do not pay for growing them.
+    static void prewarm(size_t symbols);
 private:
     void push(uint32_t s) {
         if (n_ == cap_) grow();
