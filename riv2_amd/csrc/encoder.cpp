@@ -26,6 +26,7 @@ namespace {
 struct Slot {
     DevBuf rec;                          // the frame's block records (they also seed the next frame's temporal MVs)
     HostBuf h_rec, h_pack, h_optskip;    // mapped host memory the kernels write for the host entropy stage
+    HostBuf h_main;                      // inter frames: each coded MB's masks (and few values), PACK_MAIN bytes per MB
     HostBuf h_passes;                    // mapped: the motion search's pass count
     hipEvent_t done = nullptr;           // the frame's kernels (and so its host-side outputs) finished
     hipEvent_t marks[RIV_NUM_STAGES + 1] = {};
@@ -251,7 +252,7 @@ static int encoder_init(riv_encoder* e) {
     for (Slot& s : e->slots) {
         RIV_HIP(s.rec.alloc(sizeof(BlockRec) * e->nb));
         RIV_HIP(s.h_rec.alloc(sizeof(BlockRec) * e->nb)); RIV_HIP(s.h_pack.alloc((size_t)riv::PACK_STRIDE * e->nb));
-        RIV_HIP(s.h_optskip.alloc(e->nb)); RIV_HIP(s.h_passes.alloc(16));
+        RIV_HIP(s.h_optskip.alloc(e->nb)); RIV_HIP(s.h_passes.alloc(16)); RIV_HIP(s.h_main.alloc((size_t)riv::PACK_MAIN * e->nb));
         RIV_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming | hipEventBlockingSync));   // waiting workers sleep instead of spinning
         RIV_HIP(hipEventCreate(&s.t0)); RIV_HIP(hipEventCreate(&s.t1));
     }
@@ -433,7 +434,7 @@ static void code_frame(riv_encoder* e, int si) {
                 st.rdo_blocks_selected_skip += hr[i].rdo_skip;
                 if (hr[i].s[2] || hr[i].s[3]) st.rdo_forced_fractional++;
             }
-            s.res = rivh::encode_residuals_packed(pack, riv::PACK_STRIDE, coded.data(), nb, false);
+            s.res = rivh::encode_residuals_split(s.h_main.as<uint8_t>(), pack, coded.data(), nb);
             s.bdx = bdx; s.bdy = bdy;
             st.frame_type = 2;
             st.mv_bias_dx = bdx; st.mv_bias_dy = bdy;
@@ -736,7 +737,7 @@ static int frame_after_search(riv_encoder* e, const FrameCtx& f) {
     rivk::aq(st, e->nb, e->inter_qs, e->lnv.as<float>(), e->steps.as<float>());
     mark(e, s, st, 3);
     rivk::inter_residual(st, f.cY, f.cU, f.cV, f.pY, f.pU, f.pV, e->W, e->H, rec, e->steps.as<float>(), c.inter_dead_zone, f.nY, f.nU, f.nV,
-                         s.h_pack.dev<uint8_t>(), e->mask.as<uint8_t>(), s.h_optskip.dev<uint8_t>());
+                         s.h_main.dev<uint8_t>(), s.h_pack.dev<uint8_t>(), e->mask.as<uint8_t>(), s.h_optskip.dev<uint8_t>());
     mark(e, s, st, 4);
     rivk::deblock(st, f.nY, f.nU, f.nV, e->W, e->H, deblock_level(e->inter_qs), false, e->mask.as<uint8_t>());
     mark(e, s, st, 5);
@@ -1006,6 +1007,7 @@ int riv_encoder_debug_coeffs(riv_encoder* e, int16_t* out) {
     for (int b = 0; b < e->nb; b++) {
         int16_t* q = out + (size_t)b * 384;
         if (!s.intra && os[b]) std::memset(q, 0, 768);
+        else if (!s.intra) rivh::unpack_split(s.h_main.as<uint8_t>(), s.h_pack.as<uint8_t>(), b, q);
         else rivh::unpack_coeffs(s.h_pack.as<uint8_t>() + (size_t)b * riv::PACK_STRIDE, q);
     }
     return RIV_OK;

@@ -1,10 +1,12 @@
 // Host-side entropy coding of the RIV bitstream (see entropy.hpp). Byte-exact with the
 // reference coders; citations are to the reference tree under reitero_video/.
 #include "entropy.hpp"
+#include "riv_common.hpp"
 #include <algorithm>
 #include <cstring>
 #include <cstdlib>
 #include <immintrin.h>
+#include <type_traits>
 
 namespace rivh {
 
@@ -38,7 +40,7 @@ const uint32_t kP0Rcp[512] = {
 };
 
 namespace {
-struct WriterBuffers { std::vector<uint16_t> buf, tmp[4], tmp8[8]; };
+struct WriterBuffers { std::vector<uint16_t> buf, tmp[4], tmp8[16]; };
 thread_local std::vector<WriterBuffers> tl_pool;
 }  // namespace
 
@@ -47,7 +49,7 @@ RansWriter::RansWriter(std::vector<uint8_t>& out) : out_(out) {
         WriterBuffers& b = tl_pool.back();
         buf_.swap(b.buf);
         for (int j = 0; j < kLanes; j++) tmp_[j].swap(b.tmp[j]);
-        for (int j = 0; j < 8; j++) tmp8_[j].swap(b.tmp8[j]);
+        for (int j = 0; j < 16; j++) tmp8_[j].swap(b.tmp8[j]);
         tl_pool.pop_back();
         sym_ = buf_.data(); cap_ = buf_.size();
     }
@@ -58,7 +60,7 @@ RansWriter::~RansWriter() {
         WriterBuffers& b = tl_pool.back();
         b.buf.swap(buf_);
         for (int j = 0; j < kLanes; j++) b.tmp[j].swap(tmp_[j]);
-        for (int j = 0; j < 8; j++) b.tmp8[j].swap(tmp8_[j]);
+        for (int j = 0; j < 16; j++) b.tmp8[j].swap(tmp8_[j]);
     }
 }
 
@@ -136,66 +138,82 @@ void RansWriter::drain(const uint16_t* const* sym, const int* n, int nblk) {
 }
 
 namespace {
-// Eight full blocks (16386 symbols, an even count, so every step codes one pair per block) with
-// AVX2, 16 state chains in two vectors: lane 2j holds block j's B state (the lower symbol of each
-// pair), lane 2j+1 its A state. q = x / freq is a double-precision division: x < 2^32 and
-// freq <= 255 convert exactly, the quotient is below 2^24, and a correctly rounded quotient of
-// x/freq never crosses an integer (a non-integer x/freq lies at least 1/freq below the next one).
+// 4*NV full blocks (16386 symbols, an even count, so every step codes one pair per block) with
+// AVX2, 8*NV state chains in NV vectors: vector v holds blocks 4v .. 4v+3, lane 2j block 4v+j's
+// B state (the lower symbol of each pair), lane 2j+1 its A state. More vectors keep more
+// independent chains in flight: a step's chain (renormalise, divide, multiply, add) is long.
+// q = x / freq is a double-precision division: x < 2^32 and freq <= 255 convert exactly, the
+// quotient is below 2^24, and a correctly rounded quotient of x/freq never crosses an integer
+// (a non-integer x/freq lies at least 1/freq below the next one).
 // Emitted words go below wp[j], block by block, A before B within a step (the scalar order).
-__attribute__((target("avx2"))) void drain8_avx2(const uint16_t* const* sym, int n, uint16_t** wp, uint32_t* r0, uint32_t* r1) {
+template <int NV>
+__attribute__((target("avx2"))) void drain_avx2(const uint16_t* const* sym, int n, uint16_t** wp, uint32_t* r0, uint32_t* r1) {
     const __m256i sign = _mm256_set1_epi32((int)0x80000000u), ff = _mm256_set1_epi32(0xff), c256 = _mm256_set1_epi32(256);
     const __m256d two31 = _mm256_set1_pd(2147483648.0);
-    __m256i X[2] = {_mm256_set1_epi32((int)kLow), _mm256_set1_epi32((int)kLow)};
-    alignas(32) uint32_t xs[16];
+    __m256i X[NV];
+    for (int v = 0; v < NV; v++) X[v] = _mm256_set1_epi32((int)kLow);
+    alignas(32) uint32_t xs[8 * NV];
     for (int k = n - 2; k >= 0; k -= 2) {   // pair (k+1, k): A codes k+1, B codes k
-        uint32_t w[8];
-        for (int j = 0; j < 8; j++) std::memcpy(&w[j], sym[j] + k, 4);   // low half: symbol k (B), high: k+1 (A)
-#pragma GCC unroll 2
-        for (int h = 0; h < 2; h++) {
-            const __m256i S = _mm256_cvtepu16_epi32(_mm_set_epi32((int)w[4 * h + 3], (int)w[4 * h + 2], (int)w[4 * h + 1], (int)w[4 * h]));
-            const __m256i freq = _mm256_and_si256(S, ff), start = _mm256_srli_epi32(S, 8);
+        __m256i S[NV], e[NV];
+        int m[NV], any = 0;
+#pragma GCC unroll 4
+        for (int v = 0; v < NV; v++) {
+            uint32_t w[4];
+            for (int j = 0; j < 4; j++) std::memcpy(&w[j], sym[4 * v + j] + k, 4);   // low half: symbol k (B), high: k+1 (A)
+            S[v] = _mm256_cvtepu16_epi32(_mm_set_epi32((int)w[3], (int)w[2], (int)w[1], (int)w[0]));
             // renormalise where x >= freq << 24 (unsigned compare through the sign flip)
-            const __m256i e = _mm256_xor_si256(_mm256_cmpgt_epi32(_mm256_xor_si256(_mm256_slli_epi32(freq, 24), sign),
-                                                                  _mm256_xor_si256(X[h], sign)), _mm256_set1_epi32(-1));
-            const int m = _mm256_movemask_ps(_mm256_castsi256_ps(e));
-            if (m) {
-                _mm256_store_si256((__m256i*)(xs + 8 * h), X[h]);
+            const __m256i freq = _mm256_and_si256(S[v], ff);
+            e[v] = _mm256_xor_si256(_mm256_cmpgt_epi32(_mm256_xor_si256(_mm256_slli_epi32(freq, 24), sign),
+                                                       _mm256_xor_si256(X[v], sign)), _mm256_set1_epi32(-1));
+            m[v] = _mm256_movemask_ps(_mm256_castsi256_ps(e[v]));
+            any |= m[v];
+        }
+        if (any) {
+            for (int v = 0; v < NV; v++) {
+                if (!m[v]) continue;
+                _mm256_store_si256((__m256i*)(xs + 8 * v), X[v]);
                 for (int j = 0; j < 4; j++) {
-                    if (m & (2 << (2 * j))) *--wp[4 * h + j] = (uint16_t)xs[8 * h + 2 * j + 1];   // A first
-                    if (m & (1 << (2 * j))) *--wp[4 * h + j] = (uint16_t)xs[8 * h + 2 * j];
+                    if (m[v] & (2 << (2 * j))) *--wp[4 * v + j] = (uint16_t)xs[8 * v + 2 * j + 1];   // A first
+                    if (m[v] & (1 << (2 * j))) *--wp[4 * v + j] = (uint16_t)xs[8 * v + 2 * j];
                 }
             }
-            const __m256i x = _mm256_blendv_epi8(X[h], _mm256_srli_epi32(X[h], 16), e);
+        }
+#pragma GCC unroll 4
+        for (int v = 0; v < NV; v++) {
+            const __m256i freq = _mm256_and_si256(S[v], ff), start = _mm256_srli_epi32(S[v], 8);
+            const __m256i x = _mm256_blendv_epi8(X[v], _mm256_srli_epi32(X[v], 16), e[v]);
             const __m256i xb = _mm256_xor_si256(x, sign);
             const __m256d xl = _mm256_add_pd(_mm256_cvtepi32_pd(_mm256_castsi256_si128(xb)), two31);
             const __m256d xh = _mm256_add_pd(_mm256_cvtepi32_pd(_mm256_extracti128_si256(xb, 1)), two31);
             const __m256d fl = _mm256_cvtepi32_pd(_mm256_castsi256_si128(freq)), fh = _mm256_cvtepi32_pd(_mm256_extracti128_si256(freq, 1));
             const __m256i q = _mm256_set_m128i(_mm256_cvttpd_epi32(_mm256_div_pd(xh, fh)), _mm256_cvttpd_epi32(_mm256_div_pd(xl, fl)));
-            X[h] = _mm256_add_epi32(_mm256_add_epi32(x, _mm256_mullo_epi32(q, _mm256_sub_epi32(c256, freq))), start);
+            X[v] = _mm256_add_epi32(_mm256_add_epi32(x, _mm256_mullo_epi32(q, _mm256_sub_epi32(c256, freq))), start);
         }
     }
-    _mm256_store_si256((__m256i*)xs, X[0]);
-    _mm256_store_si256((__m256i*)(xs + 8), X[1]);
-    for (int j = 0; j < 8; j++) { r1[j] = xs[2 * j]; r0[j] = xs[2 * j + 1]; }
+    for (int v = 0; v < NV; v++) _mm256_store_si256((__m256i*)(xs + 8 * v), X[v]);
+    for (int j = 0; j < 4 * NV; j++) { r1[j] = xs[2 * j]; r0[j] = xs[2 * j + 1]; }
 }
 }  // namespace
 
 void RansWriter::flush() {
     size_t b = 0;
-    if (g_avx2_rans) {   // groups of eight full blocks, then the rest below
-        while (n_ - b >= 8 * (size_t)kBatch) {
-            const uint16_t* s8[8];
-            uint16_t* wp[8];
-            uint32_t r0[8], r1[8];
-            for (int j = 0; j < 8; j++) {
-                s8[j] = sym_ + b + (size_t)j * kBatch;
+    if (g_avx2_rans) {   // groups of sixteen full blocks, then of eight, then the rest below
+        auto group = [&](auto nvt) {
+            constexpr int NB = 4 * decltype(nvt)::value;
+            const uint16_t* sg[NB];
+            uint16_t* wp[NB];
+            uint32_t r0[NB], r1[NB];
+            for (int j = 0; j < NB; j++) {
+                sg[j] = sym_ + b + (size_t)j * kBatch;
                 if (tmp8_[j].size() < (size_t)kBatch + 8) tmp8_[j].resize((size_t)kBatch + 8);
                 wp[j] = tmp8_[j].data() + tmp8_[j].size();
             }
-            drain8_avx2(s8, kBatch, wp, r0, r1);
-            for (int j = 0; j < 8; j++) emit_block(r0[j], r1[j], wp[j], tmp8_[j].data() + tmp8_[j].size());
-            b += 8 * (size_t)kBatch;
-        }
+            drain_avx2<decltype(nvt)::value>(sg, kBatch, wp, r0, r1);
+            for (int j = 0; j < NB; j++) emit_block(r0[j], r1[j], wp[j], tmp8_[j].data() + tmp8_[j].size());
+            b += (size_t)NB * kBatch;
+        };
+        while (n_ - b >= 16 * (size_t)kBatch) group(std::integral_constant<int, 4>());
+        while (n_ - b >= 8 * (size_t)kBatch) group(std::integral_constant<int, 2>());
         if (b == n_ && b > 0) { n_ = 0; return; }
     }
     do {   // an empty stream still carries its two initial states
@@ -433,33 +451,23 @@ void put_plane_packed(uint16_t*& o, ResidualCtx& c, const uint64_t* mask, int nw
 }
 }  // namespace
 
-std::vector<uint8_t> encode_residuals_packed(const uint8_t* pack, size_t stride, const uint8_t* coded, int nb, bool dc_pred) {
+namespace {
+// Inter residuals (no DC prediction) from each coded MB's packed record: record(b, m) copies MB b's
+// six masks to m and returns its values.
+template <typename Rec>
+std::vector<uint8_t> encode_inter_records(const uint8_t* coded, int nb, Rec record) {
     std::vector<uint8_t> out;
     bool any = false;
     for (int b = 0; b < nb && !any; b++) any = coded[b] != 0;
-    if (!any) return out;
+    if (!any) return out;   // no coder instantiated when every block is skipped (residual.rs:670-674, :792-798)
     ResidualCtx ctx;
     RansWriter w(out);
-    if (dc_pred) {   // intra: DC prediction changes position 0; rare, coded from unpacked blocks
-        int16_t q[384];
-        for (int b = 0; b < nb; b++) {
-            if (!coded[b]) continue;
-            unpack_coeffs(pack + (size_t)b * stride, q);
-            put_plane(w, ctx, q, 256, 0, dc_pred);
-            put_plane(w, ctx, q + 256, 64, 1, dc_pred);
-            put_plane(w, ctx, q + 320, 64, 2, dc_pred);
-        }
-        w.flush();
-        return out;
-    }
     // a block yields at most 3 EOB codes of 16 symbols and 384 positions of 27 symbols
     constexpr size_t kMaxBlockSymbols = 3 * 16 + 384 * 27;
     for (int b = 0; b < nb; b++) {
         if (!coded[b]) continue;
-        const uint8_t* slot = pack + (size_t)b * stride;
         uint64_t m[6];
-        std::memcpy(m, slot, sizeof(m));
-        const int16_t* vals = (const int16_t*)(slot + 48);
+        const int16_t* vals = record(b, m);
         uint16_t* o = w.reserve(kMaxBlockSymbols);
         put_plane_packed(o, ctx, m, 4, vals, true);
         put_plane_packed(o, ctx, m + 4, 1, vals, false);
@@ -468,6 +476,55 @@ std::vector<uint8_t> encode_residuals_packed(const uint8_t* pack, size_t stride,
     }
     w.flush();
     return out;
+}
+}  // namespace
+
+std::vector<uint8_t> encode_residuals_packed(const uint8_t* pack, size_t stride, const uint8_t* coded, int nb, bool dc_pred) {
+    if (!dc_pred)
+        return encode_inter_records(coded, nb, [&](int b, uint64_t* m) {
+            const uint8_t* slot = pack + (size_t)b * stride;
+            std::memcpy(m, slot, 48);
+            return (const int16_t*)(slot + 48);
+        });
+    std::vector<uint8_t> out;
+    bool any = false;
+    for (int b = 0; b < nb && !any; b++) any = coded[b] != 0;
+    if (!any) return out;
+    ResidualCtx ctx;
+    RansWriter w(out);
+    // intra: DC prediction changes position 0; rare, coded from unpacked blocks
+    int16_t q[384];
+    for (int b = 0; b < nb; b++) {
+        if (!coded[b]) continue;
+        unpack_coeffs(pack + (size_t)b * stride, q);
+        put_plane(w, ctx, q, 256, 0, dc_pred);
+        put_plane(w, ctx, q + 256, 64, 1, dc_pred);
+        put_plane(w, ctx, q + 320, 64, 2, dc_pred);
+    }
+    w.flush();
+    return out;
+}
+
+namespace {
+inline const int16_t* split_record(const uint8_t* main, const uint8_t* over, int b, uint64_t* m) {
+    const uint8_t* slot = main + (size_t)b * riv::PACK_MAIN;
+    std::memcpy(m, slot, 48);
+    int nnz = 0;
+    for (int k = 0; k < 6; k++) nnz += __builtin_popcountll(m[k]);
+    return (const int16_t*)(nnz <= riv::PACK_MAIN_VALUES ? slot + 48 : over + (size_t)b * riv::PACK_STRIDE + 48);
+}
+}  // namespace
+
+std::vector<uint8_t> encode_residuals_split(const uint8_t* main, const uint8_t* over, const uint8_t* coded, int nb) {
+    return encode_inter_records(coded, nb, [&](int b, uint64_t* m) { return split_record(main, over, b, m); });
+}
+
+void unpack_split(const uint8_t* main, const uint8_t* over, int b, int16_t* q) {
+    uint64_t m[6];
+    const int16_t* v = split_record(main, over, b, m);
+    std::memset(q, 0, 384 * sizeof(int16_t));
+    for (int k = 0; k < 6; k++)
+        for (uint64_t w = m[k]; w; w &= w - 1) q[64 * k + __builtin_ctzll(w)] = *v++;
 }
 
 void decode_residuals(const uint8_t* data, size_t len, const uint8_t* coded, int nb, bool dc_pred, int16_t* coeffs) {

@@ -79,7 +79,7 @@ private:
     // buffers come from a per-thread pool, so a worker coding frame after frame reuses warm memory
     std::vector<uint16_t> buf_;
     std::vector<uint16_t> tmp_[kLanes];   // emitted words of the blocks being coded, filled from the end
-    std::vector<uint16_t> tmp8_[8];       // the same for the vector coder's eight blocks
+    std::vector<uint16_t> tmp8_[16];      // the same for the vector coder's blocks (up to sixteen)
     uint16_t* sym_ = nullptr;
     size_t n_ = 0, cap_ = 0;
 };
@@ -111,6 +111,11 @@ std::vector<uint8_t> encode_residuals(const int16_t* coeffs, const uint8_t* code
 // The same coding from packed per-MB slots (six 64-bit non-zero masks, then the non-zero values
 // in index order), `stride` bytes apart.
 std::vector<uint8_t> encode_residuals_packed(const uint8_t* pack, size_t stride, const uint8_t* coded, int nb, bool dc_pred);
+// The inter form from split records: MB b's masks at main + b * PACK_MAIN, its values after them
+// when there are at most PACK_MAIN_VALUES, else at over + b * PACK_STRIDE + 48 (coded MBs only).
+std::vector<uint8_t> encode_residuals_split(const uint8_t* main, const uint8_t* over, const uint8_t* coded, int nb);
+// One MB of the split form as 384 coefficients.
+void unpack_split(const uint8_t* main, const uint8_t* over, int b, int16_t* q);
 void unpack_coeffs(const uint8_t* slot, int16_t* q384);
 // Inverse; fills coeffs (nb*384, caller-zeroed) of the MBs with coded[b] != 0.
 void decode_residuals(const uint8_t* data, size_t len, const uint8_t* coded, int nb, bool dc_pred, int16_t* coeffs);

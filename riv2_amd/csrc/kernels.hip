@@ -1266,6 +1266,29 @@ __device__ __forceinline__ void write_packed(const int16_t* coef, uint8_t* slot,
     }
 }
 
+// The same record split for the host's reads: the six masks, and the values when there are at
+// most PACK_MAIN_VALUES of them, in the MB's compact slot (main, PACK_MAIN bytes per MB, so a
+// frame's records are one short contiguous run); more values go to the MB's full-size slot
+// (over). Mapped memory the device has just written is read from DRAM, so the bytes and pages the
+// host touches are what it costs.
+__device__ __forceinline__ void write_packed_split(const int16_t* coef, uint8_t* main, uint8_t* over, int t) {
+    if (t >= 64) return;
+    unsigned long long m[6];
+    int nnz = 0;
+#pragma unroll
+    for (int k = 0; k < 6; k++) { m[k] = __ballot(coef[64 * k + t] != 0); nnz += __popcll(m[k]); }
+#pragma unroll
+    for (int k = 0; k < 6; k++) if (t == k) ((unsigned long long*)main)[k] = m[k];
+    int16_t* vals = (int16_t*)((nnz <= riv::PACK_MAIN_VALUES ? main : over) + 48);
+    int base = 0;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const int16_t v = coef[64 * k + t];
+        if (v != 0) vals[base + __popcll(m[k] & ((1ull << t) - 1ull))] = v;
+        base += __popcll(m[k]);
+    }
+}
+
 // P-frame residual: ResidualEncoder::encode_inter (reitero_residual/src/residual.rs:396-809)
 // minus the rANS: residual = cur - pred for MBs the RDO did not skip, FDCT, dead-zone AQ
 // quantisation, all-zero -> optimised skip, else dequant + IDCT reconstruction. Writes the
@@ -1273,8 +1296,8 @@ __device__ __forceinline__ void write_packed(const int16_t* coef, uint8_t* slot,
 __global__ __launch_bounds__(256) void k_inter_residual(const uint8_t* cY, const uint8_t* cU, const uint8_t* cV,
                                                         const uint8_t* pY, const uint8_t* pU, const uint8_t* pV,
                                                         int W, const BlockRec* rec, const float* steps, float dz,
-                                                        uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* pack, uint8_t* mask,
-                                                        uint8_t* host_opt_skip) {
+                                                        uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* main, uint8_t* over,
+                                                        uint8_t* mask, uint8_t* host_opt_skip) {
     __shared__ ResidLds L;
     const int b = blockIdx.x, bw = W >> 4, bx = b % bw, by = b / bw, t = threadIdx.x, cw = W >> 1;
     const int x0 = bx * 16, y0 = by * 16;
@@ -1316,7 +1339,7 @@ __global__ __launch_bounds__(256) void k_inter_residual(const uint8_t* cY, const
         return;
     }
     if (t == 0) { mask[b] = 1; host_opt_skip[b] = 0; }
-    write_packed(L.coef, pack + (size_t)b * riv::PACK_STRIDE, t);
+    write_packed_split(L.coef, main + (size_t)b * riv::PACK_MAIN, over + (size_t)b * riv::PACK_STRIDE, t);
     recon_mb(L, t, qs, false, nullptr, nullptr, pY, pU, pV, oY, oU, oV, W, x0, y0);
 }
 
@@ -1566,9 +1589,9 @@ void logf_exp2f(hipStream_t st, const float* in, size_t n, int which, float* out
 }
 void inter_residual(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, const uint8_t* pY, const uint8_t* pU,
                     const uint8_t* pV, int W, int H, const BlockRec* rec, const float* steps, float dz, uint8_t* oY, uint8_t* oU,
-                    uint8_t* oV, uint8_t* pack, uint8_t* mask, uint8_t* host_opt_skip) {
+                    uint8_t* oV, uint8_t* main, uint8_t* over, uint8_t* mask, uint8_t* host_opt_skip) {
     hipLaunchKernelGGL(k_inter_residual, dim3((W / 16) * (H / 16)), dim3(256), 0, st, cY, cU, cV, pY, pU, pV, W, rec, steps, dz, oY, oU, oV,
-                       pack, mask, host_opt_skip);
+                       main, over, mask, host_opt_skip);
 }
 void intra(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, int W, int H, const uint16_t* yt, const uint16_t* ct,
            uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* pack) {

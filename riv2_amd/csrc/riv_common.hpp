@@ -27,6 +27,11 @@ static_assert(sizeof(BlockRec) == 16, "BlockRec layout");
 // U then V, row-major), then the non-zero values as int16 in index order. Slots are PACK_STRIDE
 // bytes apart; only the used part of a slot is written.
 constexpr int PACK_STRIDE = 48 + 768;
+// Inter frames split the record: the masks, and the values when there are at most
+// PACK_MAIN_VALUES, go to a compact slot of PACK_MAIN bytes; more values go to the MB's
+// PACK_STRIDE slot, after its (unwritten) masks.
+constexpr int PACK_MAIN_VALUES = 40;
+constexpr int PACK_MAIN = 48 + 2 * PACK_MAIN_VALUES;
 
 // Pass barrier of the motion search: per pass, kBarrierLeaves leaf counters (workgroup w arrives
 // at leaf w % kBarrierLeaves) and a root counter; then one release word. A bank holds
