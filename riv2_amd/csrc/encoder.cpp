@@ -710,6 +710,7 @@ static riv::SearchStream frame_search_stream(riv_encoder* e, const FrameCtx& f, 
     S.rec = s.rec.as<BlockRec>(); S.host_rec = s.h_rec.dev<BlockRec>(); S.passes = s.h_passes.dev<int>();
     S.rdo_bits = e->rdo.as<uint32_t>(); S.rdo_max = e->rdo_max;
     S.W = e->W; S.H = e->H; S.bw = e->bw; S.bh = e->bh; S.blk0 = blk0;
+    S.lnv = e->lnv.as<float>();
     return S;
 }
 
@@ -732,11 +733,12 @@ static int frame_after_search(riv_encoder* e, const FrameCtx& f) {
     hipStream_t st = f.st;
     BlockRec* rec = s.rec.as<BlockRec>();   // written by the search's mode-decision epilogue
     mark(e, s, st, 1);
-    rivk::mc(st, f.rY, f.rU, f.rV, e->W, e->H, (const int8_t*)rec, sizeof(BlockRec), f.pY, f.pU, f.pV, e->var.as<uint32_t>(), e->lnv.as<float>());
+    // the prediction's variances came from the search (lnv); the residual kernel forms the prediction
     mark(e, s, st, 2);
     rivk::aq(st, e->nb, e->inter_qs, e->lnv.as<float>(), e->steps.as<float>());
     mark(e, s, st, 3);
-    rivk::inter_residual(st, f.cY, f.cU, f.cV, f.pY, f.pU, f.pV, e->W, e->H, rec, e->steps.as<float>(), c.inter_dead_zone, f.nY, f.nU, f.nV,
+    rivk::inter_residual(st, f.cY, f.cU, f.cV, f.rY, f.rU, f.rV, f.pY, f.pU, f.pV, e->W, e->H, rec, e->steps.as<float>(), c.inter_dead_zone,
+                         f.nY, f.nU, f.nV,
                          s.h_main.dev<uint8_t>(), s.h_pack.dev<uint8_t>(), e->mask.as<uint8_t>(), s.h_optskip.dev<uint8_t>());
     mark(e, s, st, 4);
     rivk::deblock(st, f.nY, f.nU, f.nV, e->W, e->H, deblock_level(e->inter_qs), false, e->mask.as<uint8_t>());

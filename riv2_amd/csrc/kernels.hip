@@ -676,6 +676,38 @@ __device__ __forceinline__ uint32_t seed_word(uint32_t w, int sr) {
 
 }  // namespace
 
+// Natural log of a 16x16 block's variance from its sum and sum of squares, as the adaptive
+// quantiser defines it (reitero_residual/src/residual.rs:116-167): ((sum_sq*256 - sum^2) as f32)
+// / 65536, floored at 4, then glibc's logf.
+__device__ __forceinline__ float aq_log_var(uint32_t s, uint32_t s2) {
+    float var = (float)((uint64_t)s2 * 256ull - (uint64_t)s * s) / 65536.0f;
+    return glibc_logf(fmaxf(var, 4.0f));
+}
+
+// Sums (Σv, Σv²) of a block's 16x16 luma prediction at half-pel vector (hx, hy), rounded as the
+// encoder's motion compensation rounds it (sample_hp_round below: (a+b+1)>>1, (a+b+c+d+2)>>2;
+// clamping to the frame equals interpolating the replicate-padded window), from the block's
+// window slot wp; in every lane. Lane l forms pixels 4(l&3) .. 4(l&3)+3 of row l>>2.
+__device__ __forceinline__ void window_pred_sums(const uint32_t* wp, int ND, int sr, int cb, int hx, int hy, int lane, uint32_t* s1, uint32_t* s2) {
+    const int j = lane >> 2, c = cb + (hx >> 1) + 4 * (lane & 3);
+    const uint32_t* q = wp + (j + sr + 1 + (hy >> 1)) * ND + (c >> 2);
+    const uint32_t sh = (uint32_t)(c & 3);
+    const uint32_t a = __builtin_amdgcn_alignbyte(q[1], q[0], sh);
+    const uint32_t b = sh == 3u ? q[1] : __builtin_amdgcn_alignbyte(q[1], q[0], sh + 1u);
+    const uint32_t cc = __builtin_amdgcn_alignbyte(q[ND + 1], q[ND], sh);
+    const uint32_t d = sh == 3u ? q[ND + 1] : __builtin_amdgcn_alignbyte(q[ND + 1], q[ND], sh + 1u);
+    const bool xo = (hx & 1) != 0, yo = (hy & 1) != 0;
+    uint32_t t1 = 0u, t2 = 0u;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const uint32_t pa = (a >> (8 * i)) & 0xffu, pb = (b >> (8 * i)) & 0xffu, pc = (cc >> (8 * i)) & 0xffu, pd = (d >> (8 * i)) & 0xffu;
+        const uint32_t v = !xo ? (!yo ? pa : (pa + pc + 1u) >> 1) : (!yo ? (pa + pb + 1u) >> 1 : (pa + pb + pc + pd + 2u) >> 2);
+        t1 += v; t2 += v * v;
+    }
+    *s1 = wave_sum(t1);
+    *s2 = wave_sum(t2);
+}
+
 __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
     extern __shared__ __align__(16) uint32_t lds32[];
     const int lane = threadIdx.x & 63;
@@ -819,18 +851,27 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
         // the next launch's start field; where bank 0 is the final field it already holds myw
         field_store(p.field + myb, myw);
     }
+    // the adaptive quantiser's input: per block, the log variance of its luma prediction at its
+    // final vector, formed from the block's window (reloaded where the block's slot is shared)
+    for (unsigned long long vm = __ballot(valid); vm; vm &= vm - 1) {
+        const int k = __builtin_ctzll(vm);
+        const SearchStream& S = sd[__builtin_amdgcn_readlane(ms, k)];
+        if (!S.lnv) continue;
+        const int bx = __builtin_amdgcn_readlane(mbx, k), by = __builtin_amdgcn_readlane(mby, k);
+        const uint32_t w = sgpr((uint32_t)__builtin_amdgcn_readlane((int)myw, k));
+        const int hx = 2 * mvx(w) + (int)((w >> 16) & 0xffu), hy = 2 * mvy(w) + (int)(w >> 24);
+        const bool own = k < p.nres;
+        uint32_t* wp = slots + (own ? k : p.nres) * SLOT;
+        if (!own) jw_load(S, sr, wp, ND, NR, bx * 16, by * 16, lane);
+        uint32_t s1, s2;
+        window_pred_sums(wp, ND, sr, bx * 16 - ((bx * 16 - sr - 1) & ~3), hx, hy, lane, &s1, &s2);
+        if (lane == 0) S.lnv[__builtin_amdgcn_readlane(mloc, k)] = aq_log_var(s1, s2);
+    }
     if (blockIdx.x == 0)
         for (int i = threadIdx.x; i < NS; i += blockDim.x)
             if (sd[i].passes) *sd[i].passes = pass + 1;
 }
 
-// Natural log of a 16x16 block's variance from its sum and sum of squares, as the adaptive
-// quantiser defines it (reitero_residual/src/residual.rs:116-167): ((sum_sq*256 - sum^2) as f32)
-// / 65536, floored at 4, then glibc's logf.
-__device__ __forceinline__ float aq_log_var(uint32_t s, uint32_t s2) {
-    float var = (float)((uint64_t)s2 * 256ull - (uint64_t)s * s) / 65536.0f;
-    return glibc_logf(fmaxf(var, 4.0f));
-}
 
 // ============================================================================================
 // Motion compensation + prediction-variance sums
@@ -1207,9 +1248,10 @@ struct ResidLds {
 // are in L.coef; writes the reconstruction (pred + residual, clamped) to out planes.
 // Luma: (c*round(qs*16384) + 8192) >> 14 (dct16.rs:371-381); chroma: c*max(1, round(qs))
 // (fast_dct8.rs:95). With `intra`, the tables yt/ct replace the AQ step and +128 replaces pred.
+// predY / predC >= 0: thread t's luma / chroma prediction sample, used instead of reading pY / pU / pV.
 __device__ void recon_mb(ResidLds& L, int t, float qs, bool intra, const uint16_t* yt, const uint16_t* ct,
                          const uint8_t* pY, const uint8_t* pU, const uint8_t* pV, uint8_t* oY, uint8_t* oU, uint8_t* oV,
-                         int W, int x0, int y0) {
+                         int W, int x0, int y0, int predY = -1, int predC = -1) {
     const int cw = W >> 1;
     if (intra) {
         L.deq[t] = wmul((int32_t)L.coef[t], (int32_t)(yt[t] < 1 ? 1 : yt[t]));
@@ -1224,7 +1266,7 @@ __device__ void recon_mb(ResidLds& L, int t, float qs, bool intra, const uint16_
     {
         int yy = t >> 4, xx = t & 15;
         size_t o = (size_t)(y0 + yy) * W + x0 + xx;
-        int base = intra ? 128 : pY[o];
+        int base = intra ? 128 : (predY >= 0 ? predY : pY[o]);
         oY[o] = (uint8_t)dclamp(base + (int)r, 0, 255);
     }
     if (t < 16) aan_pass1(L.coef + 256 + (t >> 3) * 64, L.qc, t & 7, L.tmpc + (t >> 3) * 64);
@@ -1236,7 +1278,7 @@ __device__ void recon_mb(ResidLds& L, int t, float qs, bool intra, const uint16_
         size_t o = (size_t)((y0 >> 1) + yy) * cw + (x0 >> 1) + xx;
         const uint8_t* ps = pl ? pV : pU;
         uint8_t* od = pl ? oV : oU;
-        int base = intra ? 128 : ps[o];
+        int base = intra ? 128 : (predC >= 0 ? predC : ps[o]);
         od[o] = (uint8_t)dclamp(base + (int)L.rc[pl * 64 + tt], 0, 255);
     }
 }
@@ -1294,33 +1336,47 @@ __device__ __forceinline__ void write_packed_split(const int16_t* coef, uint8_t*
 // quantisation, all-zero -> optimised skip, else dequant + IDCT reconstruction. Writes the
 // pre-deblock reconstruction, the coefficients of coded MBs and the optimised skip flags.
 __global__ __launch_bounds__(256) void k_inter_residual(const uint8_t* cY, const uint8_t* cU, const uint8_t* cV,
-                                                        const uint8_t* pY, const uint8_t* pU, const uint8_t* pV,
-                                                        int W, const BlockRec* rec, const float* steps, float dz,
+                                                        const uint8_t* rY, const uint8_t* rU, const uint8_t* rV,
+                                                        uint8_t* pY, uint8_t* pU, uint8_t* pV,
+                                                        int W, int H, const BlockRec* rec, const float* steps, float dz,
                                                         uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* main, uint8_t* over,
                                                         uint8_t* mask, uint8_t* host_opt_skip) {
     __shared__ ResidLds L;
-    const int b = blockIdx.x, bw = W >> 4, bx = b % bw, by = b / bw, t = threadIdx.x, cw = W >> 1;
+    const int b = blockIdx.x, bw = W >> 4, bx = b % bw, by = b / bw, t = threadIdx.x, cw = W >> 1, ch = H >> 1;
     const int x0 = bx * 16, y0 = by * 16;
+    // the prediction (motion compensation with the search MV, which the mode decision keeps;
+    // fast_motion.rs:103-352): thread t forms luma sample t and, below 128, chroma sample t & 63 of
+    // plane t >> 6, the samples it later uses; they also go to pY / pU / pV
+    const int8_t* m = rec[b].s;
+    const int mvx = m[0] * 2 + m[2], mvy = m[1] * 2 + m[3];
+    int sx0 = 0, sy0 = 0;
+    bool fast = false;
+    if (((mvx | mvy) & 1) == 0) {
+        sx0 = x0 + mvx / 2; sy0 = y0 + mvy / 2;
+        fast = sx0 >= 0 && sy0 >= 0 && sx0 + 15 < W && sy0 + 15 < H;
+    }
+    const size_t o = (size_t)(y0 + (t >> 4)) * W + x0 + (t & 15);
+    const int predY = fast ? rY[(size_t)(sy0 + (t >> 4)) * W + sx0 + (t & 15)]
+                           : sample_hp_round(rY, W, H, 2 * (x0 + (t & 15)) + mvx, 2 * (y0 + (t >> 4)) + mvy);
+    pY[o] = (uint8_t)predY;
+    const int pl = t >> 6, tt = t & 63;
+    size_t oc = 0;
+    int predC = -1;
+    if (t < 128) {
+        const int ci = tt & 7, cj = tt >> 3, cx = (x0 >> 1) + ci, cy = (y0 >> 1) + cj;
+        oc = (size_t)cy * cw + cx;
+        const uint8_t* rc = pl ? rV : rU;
+        predC = fast ? rc[(size_t)((sy0 >> 1) + cj) * cw + (sx0 >> 1) + ci] : sample_hp_round(rc, cw, ch, (4 * cx + mvx) >> 1, (4 * cy + mvy) >> 1);
+        (pl ? pV : pU)[oc] = (uint8_t)predC;
+    }
     if (rec[b].rdo_skip) {   // RDO skip: prediction only
-        size_t o = (size_t)(y0 + (t >> 4)) * W + x0 + (t & 15);
-        oY[o] = pY[o];
-        if (t < 128) {
-            int pl = t >> 6, tt = t & 63;
-            size_t oc = (size_t)((y0 >> 1) + (tt >> 3)) * cw + (x0 >> 1) + (tt & 7);
-            if (pl) oV[oc] = pV[oc]; else oU[oc] = pU[oc];
-        }
+        oY[o] = (uint8_t)predY;
+        if (t < 128) (pl ? oV : oU)[oc] = (uint8_t)predC;
         if (t == 0) { mask[b] = *(const uint32_t*)rec[b].r != 0u; host_opt_skip[b] = 1; }
         return;
     }
-    {
-        size_t o = (size_t)(y0 + (t >> 4)) * W + x0 + (t & 15);
-        L.res[t] = (int16_t)((int)cY[o] - (int)pY[o]);
-        if (t < 128) {
-            int pl = t >> 6, tt = t & 63;
-            size_t oc = (size_t)((y0 >> 1) + (tt >> 3)) * cw + (x0 >> 1) + (tt & 7);
-            L.res[256 + t] = (int16_t)(pl ? (int)cV[oc] - (int)pV[oc] : (int)cU[oc] - (int)pU[oc]);
-        }
-    }
+    L.res[t] = (int16_t)((int)cY[o] - predY);
+    if (t < 128) L.res[256 + t] = (int16_t)((int)(pl ? cV : cU)[oc] - predC);
     __syncthreads();
     fdct_mb(L, t);
     const float qs = steps[b];
@@ -1328,19 +1384,14 @@ __global__ __launch_bounds__(256) void k_inter_residual(const uint8_t* cY, const
     for (int i = t; i < 384; i += 256) { int16_t q = quant_aq(L.coef32[i], qs, dz); L.coef[i] = q; nz |= q != 0; }
     nz = __syncthreads_or(nz);
     if (!nz) {   // optimised skip (residual.rs:618-674)
-        size_t o = (size_t)(y0 + (t >> 4)) * W + x0 + (t & 15);
-        oY[o] = pY[o];
-        if (t < 128) {
-            int pl = t >> 6, tt = t & 63;
-            size_t oc = (size_t)((y0 >> 1) + (tt >> 3)) * cw + (x0 >> 1) + (tt & 7);
-            if (pl) oV[oc] = pV[oc]; else oU[oc] = pU[oc];
-        }
+        oY[o] = (uint8_t)predY;
+        if (t < 128) (pl ? oV : oU)[oc] = (uint8_t)predC;
         if (t == 0) { mask[b] = *(const uint32_t*)rec[b].r != 0u; host_opt_skip[b] = 1; }
         return;
     }
     if (t == 0) { mask[b] = 1; host_opt_skip[b] = 0; }
     write_packed_split(L.coef, main + (size_t)b * riv::PACK_MAIN, over + (size_t)b * riv::PACK_STRIDE, t);
-    recon_mb(L, t, qs, false, nullptr, nullptr, pY, pU, pV, oY, oU, oV, W, x0, y0);
+    recon_mb(L, t, qs, false, nullptr, nullptr, pY, pU, pV, oY, oU, oV, W, x0, y0, predY, predC);
 }
 
 // I-frame: ResidualEncoder::encode_intra (residual.rs:267-388) minus the rANS: pixel - 128,
@@ -1587,11 +1638,11 @@ void aq(hipStream_t st, int nb, float base, float* lnv, float* steps) {
 void logf_exp2f(hipStream_t st, const float* in, size_t n, int which, float* out) {
     hipLaunchKernelGGL(k_logf_exp2f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, n, which, out);
 }
-void inter_residual(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, const uint8_t* pY, const uint8_t* pU,
-                    const uint8_t* pV, int W, int H, const BlockRec* rec, const float* steps, float dz, uint8_t* oY, uint8_t* oU,
-                    uint8_t* oV, uint8_t* main, uint8_t* over, uint8_t* mask, uint8_t* host_opt_skip) {
-    hipLaunchKernelGGL(k_inter_residual, dim3((W / 16) * (H / 16)), dim3(256), 0, st, cY, cU, cV, pY, pU, pV, W, rec, steps, dz, oY, oU, oV,
-                       main, over, mask, host_opt_skip);
+void inter_residual(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, const uint8_t* rY, const uint8_t* rU,
+                    const uint8_t* rV, uint8_t* pY, uint8_t* pU, uint8_t* pV, int W, int H, const BlockRec* rec, const float* steps, float dz,
+                    uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* main, uint8_t* over, uint8_t* mask, uint8_t* host_opt_skip) {
+    hipLaunchKernelGGL(k_inter_residual, dim3((W / 16) * (H / 16)), dim3(256), 0, st, cY, cU, cV, rY, rU, rV, pY, pU, pV, W, H, rec, steps, dz,
+                       oY, oU, oV, main, over, mask, host_opt_skip);
 }
 void intra(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, int W, int H, const uint16_t* yt, const uint16_t* ct,
            uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* pack) {

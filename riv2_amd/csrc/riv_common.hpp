@@ -54,8 +54,9 @@ struct SearchStream {
     int prev_stride;             // bytes between consecutive MVs of prev_mv
     int rdo_max;                 // largest SAD that can skip, -1 when skipping is disabled
     int pad_;
+    float* lnv;                  // out: per block, the log variance of its luma prediction (AQ), or nullptr
 };
-static_assert(sizeof(SearchStream) == 88, "SearchStream layout");
+static_assert(sizeof(SearchStream) == 96, "SearchStream layout");
 
 // LDS layout of the motion search: the MV-cost table (TD x TD ints, TD = 2*sr + 4), 4 flag words,
 // the launch's stream descriptors, then per wave its window slots, each an NR x ND dword window

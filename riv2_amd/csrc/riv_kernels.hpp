@@ -27,9 +27,10 @@ void mc(hipStream_t st, const uint8_t* rY, const uint8_t* rU, const uint8_t* rV,
 // lnv: nb logs written by mc, plus one float for their mean; one launch computes the mean and the steps
 void aq(hipStream_t st, int nb, float base, float* lnv, float* steps);
 void logf_exp2f(hipStream_t st, const float* in, size_t n, int which, float* out);
-void inter_residual(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, const uint8_t* pY, const uint8_t* pU,
-                    const uint8_t* pV, int W, int H, const riv::BlockRec* rec, const float* steps, float dz, uint8_t* oY, uint8_t* oU,
-                    uint8_t* oV, uint8_t* main, uint8_t* over, uint8_t* mask, uint8_t* host_opt_skip);
+void inter_residual(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, const uint8_t* rY, const uint8_t* rU,
+                    const uint8_t* rV, uint8_t* pY, uint8_t* pU, uint8_t* pV, int W, int H, const riv::BlockRec* rec, const float* steps, float dz,
+                    uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* main, uint8_t* over, uint8_t* mask, uint8_t* host_opt_skip);
+// r*: the reference planes; p*: out, the motion-compensated prediction (with the search MV)
 // main/over: coded MBs' packed coefficients, MB b's masks (and its values if there are at most
 // PACK_MAIN_VALUES) at main + b * PACK_MAIN, else its values at over + b * PACK_STRIDE + 48
 // mask: per MB the deblock filter flag (not optimised-skip, or a non-zero reconstructed MV; encoder.rs:831-845)
