@@ -99,7 +99,8 @@ typedef struct {
     int32_t mv_bias_dx, mv_bias_dy;
     double rdo_lambda;
     uint32_t rdo_blocks_evaluated, rdo_blocks_selected_skip, rdo_forced_fractional, rdo_forced_threshold, rdo_forced_disabled;
-    double gpu_ms;                            /* device time of this frame's kernels (no reference counterpart) */
+    double gpu_ms;                            /* device time of this frame's kernels when stage profiling (level 2) is
+                                                 on, else 0 (no reference counterpart) */
     double host_entropy_ms;                   /* host residual rANS time of this frame (no reference counterpart) */
     double host_tail_ms;                      /* host ordered tail: MV rANS, packing, write (no reference counterpart) */
     uint32_t me_passes;                       /* relaxation passes of the motion search, 0 for intra (no reference counterpart) */
@@ -207,10 +208,12 @@ int riv_gpu_me_tables(int32_t device_id, const uint8_t* cur, const uint8_t* ref,
 int riv_gpu_logf_exp2f(int32_t device_id, const float* in, size_t n, int32_t which, float* out);
 
 /* Per-stage device timing of the encoder (HIP events on the encoder's stream around each stage;
- * no reference counterpart). Stages: 0 RGB->YUV, 1 motion search (relaxation passes and mode
- * decision), 2 motion compensation, 3 adaptive quantiser, 4 residual transform/quantise/
- * reconstruct, 5 deblock, 6 intra transform, 7 unused. `on`: 0 off, 1 the motion-search stage
- * only (two events per frame), 2 every stage. */
+ * no reference counterpart). Stages: 0 RGB->YUV, 1 motion search (relaxation passes, mode
+ * decision and the prediction variances), 2 motion compensation (on the encoder's P-frame path
+ * formed inside stage 4, so not recorded), 3 adaptive quantiser, 4 residual prediction/transform/
+ * quantise/reconstruct, 5 deblock, 6 intra transform, 7 unused. `on`: 0 off, 1 the motion-search
+ * stage only (two events per frame), 2 every stage and each frame's gpu_ms. Every event costs the
+ * stream a few microseconds. */
 #define RIV_NUM_STAGES 8
 int riv_encoder_set_profiling(riv_encoder* enc, int on);
 /* Accumulated milliseconds and launch counts per stage since profiling was switched on. */

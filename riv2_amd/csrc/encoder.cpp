@@ -35,6 +35,7 @@ struct Slot {
     hipEvent_t t0 = nullptr, t1 = nullptr;
     uint64_t index = 0, ts = 0;
     bool intra = false;
+    bool timed = false;      // t0 / t1 were recorded (stage profiling): each event costs the stream microseconds
     bool busy = false;       // holds a frame that is not yet written
     // host-stage products of the frame, passed from its coding job to the ordered tail
     bool coded = false;      // the residual coding finished (products below are valid unless failed)
@@ -254,7 +255,10 @@ static int encoder_init(riv_encoder* e) {
         RIV_HIP(s.h_rec.alloc(sizeof(BlockRec) * e->nb)); RIV_HIP(s.h_pack.alloc((size_t)riv::PACK_STRIDE * e->nb));
         RIV_HIP(s.h_optskip.alloc(e->nb)); RIV_HIP(s.h_passes.alloc(16)); RIV_HIP(s.h_main.alloc((size_t)riv::PACK_MAIN * e->nb));
         RIV_HIP(hipEventCreateWithFlags(&s.done, hipEventDisableTiming | hipEventBlockingSync));   // waiting workers sleep instead of spinning
-        RIV_HIP(hipEventCreate(&s.t0)); RIV_HIP(hipEventCreate(&s.t1));
+        // timing-only events skip the system-scope fence (a cache write-back per record); done keeps
+        // it: the host reads what the kernels wrote to mapped memory once done has fired
+        RIV_HIP(hipEventCreateWithFlags(&s.t0, hipEventDisableSystemFence));
+        RIV_HIP(hipEventCreateWithFlags(&s.t1, hipEventDisableSystemFence));
     }
     RIV_HIP(hipEventCreateWithFlags(&e->sync_ev, hipEventDisableTiming));
     if (e->depth > 0) e->hs = acquire_host_stage();
@@ -386,7 +390,7 @@ static void run_tails(riv_encoder* e, int si);
 // waited for them, so that the host-stage workers make no runtime calls).
 static void collect_gpu_times(Slot& s) {
     float gpu_ms = 0;
-    s.gpu_ms = hipEventElapsedTime(&gpu_ms, s.t0, s.t1) == hipSuccess ? gpu_ms : 0.0;
+    s.gpu_ms = s.timed && hipEventElapsedTime(&gpu_ms, s.t0, s.t1) == hipSuccess ? gpu_ms : 0.0;
     for (int k = 0; k < RIV_NUM_STAGES; k++) { s.smark[k] = 0; s.nmark[k] = 0; }
     for (int k = 1; k < s.n_marks; k++) {
         float ms = 0;
@@ -673,7 +677,7 @@ static int frame_begin(riv_encoder* e, const void* rgb, bool on_device, size_t l
     }
     s.n_marks = 0;
     if (e->profiling && !s.marks[0])
-        for (auto& m : s.marks) RIV_HIP(hipEventCreate(&m));
+        for (auto& m : s.marks) RIV_HIP(hipEventCreateWithFlags(&m, hipEventDisableSystemFence));
 
     f.si = si; f.intra = intra; f.st = st;
     const size_t ysz = (size_t)e->W * e->H, csz = ysz / 4;
@@ -683,7 +687,8 @@ static int frame_begin(riv_encoder* e, const void* rgb, bool on_device, size_t l
     f.pY = e->pred.as<uint8_t>(); f.pU = f.pY + ysz; f.pV = f.pU + csz;
     f.prev_mv = e->prev_inter_slot >= 0 ? (const int8_t*)e->slots[e->prev_inter_slot].rec.p + 4 : nullptr;
 
-    RIV_HIP(hipEventRecord(s.t0, st));
+    s.timed = e->profiling >= 2;
+    if (s.timed) RIV_HIP(hipEventRecord(s.t0, st));
     const uint8_t* d_rgb = (const uint8_t*)rgb;
     if (!on_device) {
         RIV_HIP(hipMemcpyAsync(e->rgb.p, rgb, len, hipMemcpyHostToDevice, st));
@@ -734,7 +739,6 @@ static int frame_after_search(riv_encoder* e, const FrameCtx& f) {
     BlockRec* rec = s.rec.as<BlockRec>();   // written by the search's mode-decision epilogue
     mark(e, s, st, 1);
     // the prediction's variances came from the search (lnv); the residual kernel forms the prediction
-    mark(e, s, st, 2);
     rivk::aq(st, e->nb, e->inter_qs, e->lnv.as<float>(), e->steps.as<float>());
     mark(e, s, st, 3);
     rivk::inter_residual(st, f.cY, f.cU, f.cV, f.rY, f.rU, f.rV, f.pY, f.pU, f.pV, e->W, e->H, rec, e->steps.as<float>(), c.inter_dead_zone,
@@ -752,7 +756,7 @@ static int frame_end(riv_encoder* e, const FrameCtx& f, riv_frame_stats* stats) 
     Slot& s = e->slots[f.si];
     // the kernels wrote the records, skip flags and packed coefficients straight to the slot's
     // mapped host memory: the frame is ready for the host stage once the stream reaches here
-    RIV_HIP(hipEventRecord(s.t1, f.st));
+    if (s.timed) RIV_HIP(hipEventRecord(s.t1, f.st));
     RIV_HIP(hipEventRecord(s.done, f.st));
     e->prev_inter_slot = f.intra ? -1 : f.si;
     e->ref_idx = 1 - e->ref_idx;
