@@ -607,10 +607,11 @@ static HostStage* acquire_host_stage() {
     std::lock_guard<std::mutex> lk(g_hs_mu);
     if (!g_hs) {
         g_hs = new HostStage();
-        // workers: up to 16, leaving two cores to the submitting thread and the completion thread
-        const int cores = (int)std::max(3u, std::thread::hardware_concurrency());
+        // workers: up to 16, leaving one core to the submitting and completion threads (both sleep
+        // on the GPU and on free slots most of the time)
+        const int cores = (int)std::max(2u, std::thread::hardware_concurrency());
         // each worker's rANS buffers sized for a 1080p frame's residual symbols up front
-        g_hs->pool.start(std::max(1, std::min(16, cores - 2)), [] { rivh::RansWriter::prewarm(1u << 20); });
+        g_hs->pool.start(std::max(1, std::min(16, cores - 1)), [] { rivh::RansWriter::prewarm(1u << 20); });
         g_hs->waiter = std::thread(waiter_loop, g_hs);
     }
     g_hs_refs++;

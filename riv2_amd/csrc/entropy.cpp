@@ -195,26 +195,37 @@ __attribute__((target("avx2"))) void drain_avx2(const uint16_t* const* sym, int 
 }
 }  // namespace
 
+void RansWriter::drain_group(size_t b, int nblocks) {
+    const uint16_t* sg[16];
+    uint16_t* wp[16];
+    uint32_t r0[16], r1[16];
+    for (int j = 0; j < nblocks; j++) {
+        sg[j] = sym_ + b + (size_t)j * kBatch;
+        if (tmp8_[j].size() < (size_t)kBatch + 8) tmp8_[j].resize((size_t)kBatch + 8);
+        wp[j] = tmp8_[j].data() + tmp8_[j].size();
+    }
+    if (nblocks == 16) drain_avx2<4>(sg, kBatch, wp, r0, r1);
+    else drain_avx2<2>(sg, kBatch, wp, r0, r1);
+    for (int j = 0; j < nblocks; j++) emit_block(r0[j], r1[j], wp[j], tmp8_[j].data() + tmp8_[j].size());
+}
+
+void RansWriter::make_room(size_t n) {
+    if (g_avx2_rans && n_ >= 16 * (size_t)kBatch) {   // code the first sixteen blocks, keep the rest
+        drain_group(0, 16);
+        emitted_ = true;
+        const size_t done = 16 * (size_t)kBatch;
+        std::memmove(sym_, sym_ + done, (n_ - done) * sizeof(uint16_t));
+        n_ -= done;
+    }
+    while (cap_ - n_ < n) grow();
+}
+
 void RansWriter::flush() {
     size_t b = 0;
     if (g_avx2_rans) {   // groups of sixteen full blocks, then of eight, then the rest below
-        auto group = [&](auto nvt) {
-            constexpr int NB = 4 * decltype(nvt)::value;
-            const uint16_t* sg[NB];
-            uint16_t* wp[NB];
-            uint32_t r0[NB], r1[NB];
-            for (int j = 0; j < NB; j++) {
-                sg[j] = sym_ + b + (size_t)j * kBatch;
-                if (tmp8_[j].size() < (size_t)kBatch + 8) tmp8_[j].resize((size_t)kBatch + 8);
-                wp[j] = tmp8_[j].data() + tmp8_[j].size();
-            }
-            drain_avx2<decltype(nvt)::value>(sg, kBatch, wp, r0, r1);
-            for (int j = 0; j < NB; j++) emit_block(r0[j], r1[j], wp[j], tmp8_[j].data() + tmp8_[j].size());
-            b += (size_t)NB * kBatch;
-        };
-        while (n_ - b >= 16 * (size_t)kBatch) group(std::integral_constant<int, 4>());
-        while (n_ - b >= 8 * (size_t)kBatch) group(std::integral_constant<int, 2>());
-        if (b == n_ && b > 0) { n_ = 0; return; }
+        for (; n_ - b >= 16 * (size_t)kBatch; b += 16 * (size_t)kBatch) drain_group(b, 16);
+        for (; n_ - b >= 8 * (size_t)kBatch; b += 8 * (size_t)kBatch) drain_group(b, 8);
+        if (b == n_ && (b > 0 || emitted_)) { n_ = 0; emitted_ = false; return; }
     }
     do {   // an empty stream still carries its two initial states
         const uint16_t* s[kLanes];
@@ -227,6 +238,7 @@ void RansWriter::flush() {
         drain(s, n, nblk);
     } while (b < n_);
     n_ = 0;
+    emitted_ = false;
 }
 
 RansReader::RansReader(const uint8_t* data, size_t len) : d_(data), len_(len) { a_ = state(); b_ = state(); }

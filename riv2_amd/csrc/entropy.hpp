@@ -59,7 +59,12 @@ public:
     }
     void bypass(bool b) { push(b ? 0x8080u : 0x80u); }
     // Direct writes: room for n more symbols at the returned pointer; commit(end) adopts them.
-    uint16_t* reserve(size_t n) { while (cap_ - n_ < n) grow(); return sym_ + n_; }
+    // Once sixteen full blocks are buffered they are coded and dropped from the buffer (the
+    // blocks are coded independently and emitted in order), which keeps the buffer small.
+    uint16_t* reserve(size_t n) {
+        if (cap_ - n_ < n || n_ >= 16 * (size_t)kBatch) make_room(n);
+        return sym_ + n_;
+    }
     void commit(const uint16_t* end) { n_ = (size_t)(end - sym_); }
     void flush();
     // Fills the calling thread's buffer pool (when empty) with buffers for `symbols` symbols, their
@@ -71,6 +76,8 @@ private:
         sym_[n_++] = (uint16_t)s;
     }
     void grow();
+    void make_room(size_t n);
+    void drain_group(size_t b, int nblocks);   // 16 or 8 full blocks from symbol b, vector coder
     void drain(const uint16_t* const* sym, const int* n, int nblk);
     void emit_block(uint32_t r0, uint32_t r1, const uint16_t* wp, const uint16_t* end);
     static constexpr int kBatch = 16386;
@@ -82,6 +89,7 @@ private:
     std::vector<uint16_t> tmp8_[16];      // the same for the vector coder's blocks (up to sixteen)
     uint16_t* sym_ = nullptr;
     size_t n_ = 0, cap_ = 0;
+    bool emitted_ = false;   // blocks were already coded by make_room
 };
 
 class RansReader {
