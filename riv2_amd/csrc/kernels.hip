@@ -256,19 +256,12 @@ struct CostCtx {
     }
 };
 
-// Minimum of v over lanes 0..7 (0..15), returned wave-uniform; the other lanes of each row of 16
+// Minimum of v over lanes 0..7, returned wave-uniform; the other lanes of each row of 16
 // must hold values that do not win. Used as an argmin: keys carry the lane index in their low bits.
 __device__ __forceinline__ int lane_min8(int v) {
     v = min(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true));    // quad_perm [1,0,3,2]
     v = min(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true));    // quad_perm [2,3,0,1]
     v = min(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true));   // row_half_mirror
-    return __builtin_amdgcn_readlane(v, 0);
-}
-__device__ __forceinline__ int lane_min16(int v) {
-    v = min(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true));
-    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true));
-    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true));
-    v = min(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true));   // row_mirror
     return __builtin_amdgcn_readlane(v, 0);
 }
 
@@ -503,23 +496,6 @@ __device__ __forceinline__ int jw_set_int(const JWin& w, int lane, int cx, int c
     return sv;
 }
 
-// SADs of the 9 half-pel candidates of refine_halfpel held by lanes 0..8 (lane = the reference's
-// evaluation order: base, then the 8 neighbours sy-major). Rounds group them by interpolation
-// type: the 4 diagonal neighbours (lanes 1, 3, 6, 8), the 4 axial ones (2, 4, 5, 7), the base.
-__device__ __forceinline__ int jw_set_hp(const JWin& w, int lane, int cx, int cy) {
-    const int r = lane >> 4;
-    const int kd = 0x8631 >> (4 * r) & 15, ka = 0x7542 >> (4 * r) & 15;   // row r's candidate per round
-    const int sd = (int)jw_row_hp<2>(w, lane_bcast(cx, kd), lane_bcast(cy, kd));
-    const int sa = (int)jw_row_hp<1>(w, lane_bcast(cx, ka), lane_bcast(cy, ka));
-    const int s0 = (int)jw_row_hp<0>(w, __builtin_amdgcn_readlane(cx, 0), __builtin_amdgcn_readlane(cy, 0));
-    // lane k's row in its round: diagonals 1,3,6,8 -> rows 0..3, axials 2,4,5,7 -> rows 0..3
-    const int rd = (0x3020100 >> (8 * ((lane == 3) + 2 * (lane == 6) + 3 * (lane == 8)))) & 3;
-    const int ra = (lane == 4) + 2 * (lane == 5) + 3 * (lane == 7);
-    const bool diag = lane == 1 || lane == 3 || lane == 6 || lane == 8;
-    const int gd = lane_bcast(sd, rd << 4), ga = lane_bcast(sa, ra << 4);
-    return lane == 0 ? s0 : (diag ? gd : ga);
-}
-
 // The window of MB (x0, y0): rows y0-sr-1 .. y0+16+sr and columns from the dword-aligned
 // wxa <= x0-sr-1, replicate-padded outside the frame (W is a multiple of 16, so an aligned
 // dword is wholly inside or wholly outside a row), written to wp as NR rows of ND dwords; the
@@ -641,17 +617,25 @@ __device__ uint32_t jsearch(const SearchParams& p, const SearchStream& S, const 
     }
     // refine_halfpel (:759-839): the base re-scored with the half-pel kernel, then the 8
     // neighbours in (sy outer, sx inner) order at penalty c_pred (every neighbour of an even base
-    // is fractional). Lane 0 holds the base, lanes 1..8 the neighbours.
+    // is fractional). Candidate 0 is the base, 1..8 the neighbours; the rounds group them by
+    // interpolation type: the diagonal ones (1, 3, 6, 8), the axial ones (2, 4, 5, 7), the base.
     int best_hx, best_hy, best_sad;
     {
-        const int sel = lane == 0 ? 4 : (lane <= 4 ? lane - 1 : lane);   // 3x3 raster index
-        const bool v = lane < 9;
-        const int hx = 2 * sdx + (v ? sel % 3 - 1 : 0), hy = 2 * sdy + (v ? sel / 3 - 1 : 0);
-        const int sv = jw_set_hp(w, lane, hx, hy);
-        const int key = v ? ((sv + (lane == 0 ? 0 : p.c_pred)) << 4) | lane : 0x7fffffff;
-        const int m = lane_min16(key), k = m & 15;
-        best_sad = __builtin_amdgcn_readlane(sv, k);
-        best_hx = __builtin_amdgcn_readlane(hx, k); best_hy = __builtin_amdgcn_readlane(hy, k);
+        // one candidate per 16-lane row (row r = lane >> 4), the set's argmin read from the rows' keys
+        const int r = lane >> 4;
+        auto rows_min = [](int v) {
+            return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+                       min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+        };
+        auto sel = [](int k) { return k == 0 ? 4 : (k <= 4 ? k - 1 : k); };   // 3x3 raster index
+        const int kd = (0x8631 >> (4 * r)) & 15, ka = (0x7542 >> (4 * r)) & 15;
+        const int sd = (int)jw_row_hp<2>(w, 2 * sdx + sel(kd) % 3 - 1, 2 * sdy + sel(kd) / 3 - 1);
+        const int sa = (int)jw_row_hp<1>(w, 2 * sdx + sel(ka) % 3 - 1, 2 * sdy + sel(ka) / 3 - 1);
+        const int s0 = __builtin_amdgcn_readlane((int)jw_row_hp<0>(w, 2 * sdx, 2 * sdy), 0);
+        const int m1 = rows_min(min(((sd + p.c_pred) << 4) | kd, ((sa + p.c_pred) << 4) | ka));
+        const int m = min(m1, s0 << 4), k = m & 15;
+        best_sad = (m >> 4) - (k == 0 ? 0 : p.c_pred);
+        best_hx = 2 * sdx + sel(k) % 3 - 1; best_hy = 2 * sdy + sel(k) / 3 - 1;
     }
     // canonicalised search MV (motion/scalar.rs:619-634, motion_vector.rs:103-116)
     int hx = dclamp(best_hx, -255, 255), hy = dclamp(best_hy, -255, 255);
