@@ -219,6 +219,57 @@ int riv_encoder_set_profiling(riv_encoder* enc, int on);
 /* Accumulated milliseconds and launch counts per stage since profiling was switched on. */
 int riv_encoder_stage_times(riv_encoder* enc, double* ms, uint32_t* counts);
 
+/* ---- learned form: a BF16 MFMA MV-proposal network seeding the motion search ----
+ * The reference has no learned form; the search it seeds is process_block_hex
+ * (reitero_encode/src/motion/scalar.rs:546-636). Per 16x16 block the network reads the current
+ * block with an 8-pixel border, the co-located reference patch and the previous P-frame's vectors
+ * around the block, and proposes four integer vectors and a confidence. In the encoder, a block
+ * whose confidence clears the threshold scores the four proposals (SAD + lambda * bits, as the
+ * search scores candidates); when the best is no worse than the best predictor seed it starts a
+ * small-diamond refine (at most 8 steps) instead of the hexagon search, then the usual half-pel
+ * refine. Every other block runs the plain search. The bitstream format is unchanged: any
+ * decoder of the plain form decodes it. */
+typedef struct riv_mvnet riv_mvnet;
+/* A frame pair the network reads: luma planes in device memory (width x height, multiples of
+ * 16) and the previous P-frame's vectors (x, y as int8 at prev_mv + block * prev_stride) or NULL. */
+typedef struct {
+    const uint8_t* cur_y;
+    const uint8_t* ref_y;
+    const int8_t* prev_mv;
+    int32_t prev_stride;
+    uint32_t width, height;
+} riv_mv_source;
+/* Number of f32 parameters (weights [K][N] row-major and biases, in layer order). */
+size_t riv_mvnet_param_count(void);
+/* A network with parameters initialised from `seed`, on device_id. */
+int riv_mvnet_create(int32_t device_id, uint64_t seed, riv_mvnet** out);
+void riv_mvnet_destroy(riv_mvnet* net);
+int riv_mvnet_get_params(riv_mvnet* net, float* out, size_t n);
+/* Replaces the parameters (and resets the optimiser state). */
+int riv_mvnet_set_params(riv_mvnet* net, const float* in, size_t n);
+/* The frame pairs that samples index; the planes must stay valid while they are used. */
+int riv_mvnet_set_sources(riv_mvnet* net, const riv_mv_source* src, size_t n);
+/* One Adam step on n samples: samples = n (source, block) int32 pairs, targets = n (x, y) int8
+ * vectors (the plain search's integer vectors). loss (may be NULL) receives the batch's mean
+ * loss; with NULL the call does not wait for the GPU. */
+int riv_mvnet_train_step(riv_mvnet* net, const int32_t* samples, const int8_t* targets, size_t n, float lr, float* loss);
+/* The loss's gradient with respect to every parameter (riv_mvnet_param_count floats), no update. */
+int riv_mvnet_gradients(riv_mvnet* net, const int32_t* samples, const int8_t* targets, size_t n, float* grad, float* loss);
+/* Raw outputs, n x 16 floats (9 used), by the layer-wise kernels (fused 0) or the fused
+ * inference kernel (fused 1). */
+int riv_mvnet_forward(riv_mvnet* net, const int32_t* samples, size_t n, int32_t fused, float* out);
+/* Proposals as the encoder receives them, n x 4 words: candidates 0 and 1, candidates 2 and 3
+ * (x, y bytes, whole pixels clamped to +-sr), the confident flag, the confidence logit's bits. */
+int riv_mvnet_propose(riv_mvnet* net, const int32_t* samples, size_t n, float confidence, int32_t sr, uint32_t* prop);
+/* Switches the encoder to the learned form with `net` (NULL: back to the plain form) from its
+ * next P-frame. confidence in (0, 1] is the probability threshold (1: never confident, so every
+ * block takes the plain search). The network must outlive its use and keep its parameters while
+ * frames are in flight. Single-frame calls only (batch steps search plainly). */
+int riv_encoder_set_mvnet(riv_encoder* enc, riv_mvnet* net, float confidence);
+/* Learned-form counts since the encoder was created: counts[0] P-frame blocks searched, [1] those
+ * with a confident proposal, [2] those that took the proposal (the rest fell back). */
+int riv_encoder_learned_stats(riv_encoder* enc, uint64_t* counts);
+
 /* Last error message of the calling thread (static storage). */
 const char* riv_last_error(void);
 

@@ -71,6 +71,11 @@ FLUSH_FN = ctypes.CFUNCTYPE(c_int, c_void_p)
 READ_FN = ctypes.CFUNCTYPE(c_size_t, c_void_p, POINTER(c_uint8), c_size_t)
 
 
+class MvSourceC(Structure):
+    _fields_ = [("cur_y", c_void_p), ("ref_y", c_void_p), ("prev_mv", c_void_p), ("prev_stride", c_int32),
+                ("width", c_uint32), ("height", c_uint32)]
+
+
 class WriterVtblC(Structure):
     _fields_ = [("write", WRITE_FN), ("position", POS_FN), ("seek", SEEK_FN), ("flush", FLUSH_FN)]
 
@@ -119,6 +124,18 @@ PROTOTYPES = [
     ("riv_gpu_logf_exp2f", c_int, [c_int32, c_void_p, c_size_t, c_int32, c_void_p]),
     ("riv_encoder_set_profiling", c_int, [c_void_p, c_int]),
     ("riv_encoder_stage_times", c_int, [c_void_p, c_void_p, c_void_p]),
+    ("riv_mvnet_param_count", c_size_t, []),
+    ("riv_mvnet_create", c_int, [c_int32, c_uint64, POINTER(c_void_p)]),
+    ("riv_mvnet_destroy", None, [c_void_p]),
+    ("riv_mvnet_get_params", c_int, [c_void_p, c_void_p, c_size_t]),
+    ("riv_mvnet_set_params", c_int, [c_void_p, c_void_p, c_size_t]),
+    ("riv_mvnet_set_sources", c_int, [c_void_p, c_void_p, c_size_t]),
+    ("riv_mvnet_train_step", c_int, [c_void_p, c_void_p, c_void_p, c_size_t, c_float, c_void_p]),
+    ("riv_mvnet_gradients", c_int, [c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_void_p]),
+    ("riv_mvnet_forward", c_int, [c_void_p, c_void_p, c_size_t, c_int32, c_void_p]),
+    ("riv_mvnet_propose", c_int, [c_void_p, c_void_p, c_size_t, c_float, c_int32, c_void_p]),
+    ("riv_encoder_set_mvnet", c_int, [c_void_p, c_void_p, c_float]),
+    ("riv_encoder_learned_stats", c_int, [c_void_p, c_void_p]),
     ("riv_last_error", c_char_p, []),
 ]
 

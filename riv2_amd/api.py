@@ -319,6 +319,18 @@ class Encoder:
         _check(L.lib().riv_encoder_stage_times(self._h, ms.ctypes.data, n.ctypes.data))
         return {name: (float(ms[i]), int(n[i])) for i, name in enumerate(self.STAGES)}
 
+    # ---- learned form (no reference counterpart; see riv2_amd.learned) ----
+    def set_mvnet(self, net, confidence: float = 0.5):
+        """Seeds the motion search with `net`'s proposals from the next P-frame (None: plain form)."""
+        self._mvnet = net   # the network must outlive its use
+        _check(L.lib().riv_encoder_set_mvnet(self._h, net._h if net is not None else None, float(confidence)), EncodeError)
+
+    def learned_stats(self):
+        """(P-frame blocks searched, blocks with a confident proposal, blocks that took it)."""
+        c = np.zeros(3, np.uint64)
+        _check(L.lib().riv_encoder_learned_stats(self._h, c.ctypes.data), EncodeError)
+        return tuple(int(v) for v in c)
+
     # ---- last-frame introspection for parity tests ----
     def debug_blocks(self):
         nb = self._bw * self._bh

@@ -17,6 +17,7 @@
 #include "host_common.hpp"
 #include "riv_kernels.hpp"
 #include "entropy.hpp"
+#include "mvnet_host.hpp"
 
 using namespace rivhost;
 using riv::BlockRec;
@@ -161,6 +162,12 @@ struct riv_encoder {
     std::deque<riv_frame_stats> done_stats;
     int last_slot = -1;            // slot of the most recently completed frame (debug accessors)
 
+    // learned form: the proposal network, its confidence threshold (a logit), the per-block
+    // proposals of the frame being searched and the running counts
+    riv_mvnet* net = nullptr;
+    float net_logit = 0.0f;
+    DevBuf prop, lstat;
+
     int profiling = 0;             // 0 off, 1 motion-search stage only, 2 every stage
     double stage_ms[RIV_NUM_STAGES] = {};
     uint32_t stage_n[RIV_NUM_STAGES] = {};
@@ -248,6 +255,8 @@ static int encoder_init(riv_encoder* e) {
         RIV_HIP(e->arrive.alloc(bb)); RIV_HIP(hipMemset(e->arrive.p, 0, bb));
     }
     RIV_HIP(e->err.alloc(4)); RIV_HIP(hipMemset(e->err.p, 0, 4));
+    RIV_HIP(e->prop.alloc(16 * (size_t)e->nb));
+    RIV_HIP(e->lstat.alloc(3 * sizeof(unsigned long long))); RIV_HIP(hipMemset(e->lstat.p, 0, 3 * sizeof(unsigned long long)));
     e->depth = c.pipeline_depth;
     e->slots.resize(std::max(2, e->depth));
     for (Slot& s : e->slots) {
@@ -794,6 +803,13 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
         p.streams = nullptr; p.nstreams = 1; p.nb = e->nb;
         p.field = e->field.as<uint32_t>(); p.arrive = e->arrive.as<unsigned>(); p.bank = e->s_bank; p.maxp = e->s_maxp;
         p.nslot = e->shape.nslot; p.nres = e->shape.nres;
+        if (e->net) {   // learned form: the network's proposals for every block seed the search
+            const rivnet::Source src{f.cY, f.rY, f.prev_mv, (int)sizeof(BlockRec), e->W, e->H, e->bw, e->bh};
+            RIV_HIP(rivnet::infer(f.st, nullptr, src, nullptr, e->nb, rivnet::net_weights(e->net), e->net_logit, e->cfg.search_range,
+                                  e->prop.as<uint4>(), nullptr));
+            p.prop = e->prop.as<uint4>();
+            p.lstat = e->lstat.as<unsigned long long>();
+        }
         e->s_bank ^= 1;
         RIV_HIP(rivk::me_search(f.st, p, e->shape.groups, e->shape.waves));
         r = frame_after_search(e, f);
@@ -966,6 +982,25 @@ int riv_encoder_output(const riv_encoder* e, const uint8_t** data, size_t* len) 
     return RIV_OK;
 }
 void riv_encoder_destroy(riv_encoder* e) { destroy_encoder(e); }
+
+int riv_encoder_set_mvnet(riv_encoder* e, riv_mvnet* net, float confidence) {
+    if (!e) return RIV_E_ARG;
+    if (net && rivnet::net_device(net) != e->cfg.device_id) { set_error("the network lives on another device"); return RIV_E_ARG; }
+    if (net && !(confidence > 0.0f && confidence <= 1.0f)) { set_error("confidence must be in (0, 1]"); return RIV_E_ARG; }
+    e->net = net;
+    e->net_logit = net ? rivnet::conf_logit(confidence) : 0.0f;
+    return RIV_OK;
+}
+
+int riv_encoder_learned_stats(riv_encoder* e, uint64_t* counts) {
+    if (!e || !counts) return RIV_E_ARG;
+    RIV_HIP(hipSetDevice(e->cfg.device_id));
+    RIV_HIP(hipStreamSynchronize(e->st));
+    unsigned long long h[3];
+    RIV_HIP(hipMemcpy(h, e->lstat.p, sizeof(h), hipMemcpyDeviceToHost));
+    for (int i = 0; i < 3; i++) counts[i] = h[i];
+    return RIV_OK;
+}
 
 int riv_encoder_set_profiling(riv_encoder* e, int on) {
     if (!e) return RIV_E_ARG;

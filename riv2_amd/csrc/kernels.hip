@@ -535,7 +535,8 @@ __device__ __forceinline__ void jw_load(const SearchStream& S, int sr, uint32_t*
 // half-pel SAD.
 template <bool INTERIOR>
 __device__ uint32_t jsearch(const SearchParams& p, const SearchStream& S, const int* ctab, uint32_t* wp, bool load, int ND, int NR, int bx,
-                            int by, uint32_t lS, uint32_t tS, uint32_t trS, uint32_t tlS, uint32_t pT, int lane, int* sad_out) {
+                            int by, uint32_t lS, uint32_t tS, uint32_t trS, uint32_t tlS, uint32_t pT, uint4 pr, int lane, int* sad_out,
+                            bool* took) {
     const int sr = p.sr, x0 = bx * 16, y0 = by * 16;
     if (load) jw_load(S, sr, wp, ND, NR, x0, y0, lane);
     JWin w;
@@ -591,7 +592,38 @@ __device__ uint32_t jsearch(const SearchParams& p, const SearchStream& S, const 
             if (p.pthr > 0 && __builtin_amdgcn_readlane(sv, win) <= p.pthr) skip_int = true;
         }
     }
-    if (!skip_int) {   // refine_integer_motion (:680-757): large hexagon until no improvement, then small diamond
+    // learned form: a confident block scores the network's four proposals; when the best is no
+    // worse than the best seed, a small diamond walks from it (at most 8 steps) in place of the
+    // hexagon search
+    bool learned = false;
+    if (!skip_int && (pr.z & 1u)) {
+        const uint32_t props[4] = {pr.x & 0xffffu, pr.x >> 16, pr.y & 0xffffu, pr.y >> 16};
+        const uint32_t mq = lane_pick(lane, props);
+        const int cx = mvx(mq), cy = mvy(mq);
+        const bool v = lane < 4 && abs(cx) <= sr && abs(cy) <= sr;
+        const int sv = jw_set_int<4, INTERIOR>(w, lane, v ? cx : 0, v ? cy : 0);
+        const int key = v ? ((sv + cc.cost(cx, cy)) << 3) | lane : 0x7fffffff;
+        const int m = lane_min8(key);
+        if ((m >> 3) <= scost) {
+            learned = true;
+            scost = m >> 3;
+            sdx = __builtin_amdgcn_readlane(cx, m & 7); sdy = __builtin_amdgcn_readlane(cy, m & 7);
+            const int dox = lane < 4 ? (int)((0x2213u >> (4 * lane)) & 15u) - 2 : 0;
+            const int doy = lane < 4 ? (int)((0x1322u >> (4 * lane)) & 15u) - 2 : 0;
+            for (int it = 0; it < 8; it++) {
+                const int dx = sdx + dox, dy = sdy + doy;
+                const bool vd = lane < 4 && abs(dx) <= sr && abs(dy) <= sr;
+                const int sd = jw_set_int<4, INTERIOR>(w, lane, vd ? dx : sdx, vd ? dy : sdy);
+                const int kd = vd ? ((sd + cc.cost(dx, dy)) << 3) | lane : 0x7fffffff;
+                const int md = lane_min8(kd);
+                if ((md >> 3) >= scost) break;
+                scost = md >> 3;
+                sdx = __builtin_amdgcn_readlane(dx, md & 7); sdy = __builtin_amdgcn_readlane(dy, md & 7);
+            }
+        }
+    }
+    *took = learned;
+    if (!skip_int && !learned) {   // refine_integer_motion (:680-757): large hexagon until no improvement, then small diamond
         const int hox = lane < 6 ? (int)((0x310134u >> (4 * lane)) & 15u) - 2 : 0;
         const int hoy = lane < 6 ? (int)((0x002442u >> (4 * lane)) & 15u) - 2 : 0;
         for (int it = 0; it < 4096; it++) {
@@ -736,6 +768,7 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
     // a guess that spares the second pass every block whose guess its first pass confirms
     uint32_t myw = valid ? seed_word(field_load(p.field + myb), sr) : 0u;
     int mysad = 0;
+    bool mytook = false;   // the block's last evaluation took the network's proposal
     int outcome = 0;     // 1 converged, -1 barrier timeout, 0 pass limit
     int pass = 0;
     for (; pass < p.maxp; pass++) {
@@ -774,13 +807,15 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
             uint32_t* wp = slots + (own ? k : p.nres) * SLOT;
             const bool load = !own || !((resident >> k) & 1u);
             if (own) resident |= 1ull << k;
+            const uint4 pr = p.prop ? p.prop[b] : make_uint4(0u, 0u, 0u, 0u);
+            bool took;
             const uint32_t wS = mb_interior(bx * 16, by * 16, S.W, S.H, sr)
-                                    ? jsearch<true>(p, S, ctab, wp, load, ND, NR, bx, by, lS, tS, trS, tlS, pT, lane, &sad)
-                                    : jsearch<false>(p, S, ctab, wp, load, ND, NR, bx, by, lS, tS, trS, tlS, pT, lane, &sad);
+                                    ? jsearch<true>(p, S, ctab, wp, load, ND, NR, bx, by, lS, tS, trS, tlS, pT, pr, lane, &sad, &took)
+                                    : jsearch<false>(p, S, ctab, wp, load, ND, NR, bx, by, lS, tS, trS, tlS, pT, pr, lane, &sad, &took);
             const uint32_t old = sgpr((uint32_t)__builtin_amdgcn_readlane((int)myw, k));
             const bool c = ((wS ^ old) & 0xffffu) != 0u;
             chg |= c;
-            if (lane == k) { myw = wS; mysad = sad; }
+            if (lane == k) { myw = wS; mysad = sad; mytook = took; }
             if (lane == 0) field_store(dst + b, wS | (c ? 0x80000000u : 0u));
         }
         // ---- pass barrier: a two-level tree of arrival counters (arrivals in bits 0-15, arrivals
@@ -834,6 +869,14 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
         *(uint4*)&S.host_rec[mloc] = r;
         // the next launch's start field; where bank 0 is the final field it already holds myw
         field_store(p.field + myb, myw);
+    }
+    if (p.lstat) {
+        const unsigned long long vb = __ballot(valid), cb = __ballot(valid && (p.prop[valid ? myb : 0].z & 1u)), tb = __ballot(valid && mytook);
+        if (lane == 0 && vb) {
+            atomicAdd(p.lstat, (unsigned long long)__popcll(vb));
+            atomicAdd(p.lstat + 1, (unsigned long long)__popcll(cb));
+            atomicAdd(p.lstat + 2, (unsigned long long)__popcll(tb));
+        }
     }
     // the adaptive quantiser's input: per block, the log variance of its luma prediction at its
     // final vector, formed from the block's window (reloaded where the block's slot is shared)

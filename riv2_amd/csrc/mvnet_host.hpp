@@ -1,0 +1,12 @@
+// What the encoder needs of a learned MV-proposal network handle (mvnet_api.cpp).
+#pragma once
+#include "mvnet.hpp"
+
+struct riv_mvnet;
+
+namespace rivnet {
+Weights net_weights(const riv_mvnet* n);
+int net_device(const riv_mvnet* n);
+// logit threshold of a confidence in (0, 1): +inf at >= 1 (never confident), -inf at <= 0
+float conf_logit(float confidence);
+}  // namespace rivnet

@@ -93,6 +93,11 @@ struct SearchParams {
     long long zthr, pthr;        // me_zero_mv_threshold, me_predictor_threshold
     int* err;                    // 1 on a barrier timeout, 3 if the pass limit is hit, 2 if a
                                  // reconstructed MV would differ from its search MV
+    // learned form (single-stream launches): per block the network's proposals (candidates 0|1,
+    // 2|3 as (x, y) bytes, confident flag, logit bits), or null for the plain search; lstat
+    // accumulates (blocks, confident, proposal taken)
+    const uint4* prop;
+    unsigned long long* lstat;
 };
 
 }  // namespace riv
