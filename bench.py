@@ -1,5 +1,6 @@
 #!/usr/bin/env python3
-"""1080p P-frame encode throughput of the gfx950 RIV encoder (plain form, bit-exact bitstream).
+"""1080p P-frame encode throughput of the gfx950 RIV encoder (plain form, bit-exact bitstream; the
+learned form on the same frames in the line's `learned` object).
 
 Workload: 1920x1080 RGB24 synthetic sequence (seeded generator: 1/f^2 background, global motion up
 to 8 px/frame at half-pel steps, 12 textured rectangles of 64-320 px at up to 16 px/frame, noise
@@ -54,18 +55,23 @@ def make_config(riv2_amd, depth):
         .with_skip_threshold(3).with_keyframe_interval(riv2_amd.KeyframeInterval.fixed(1000)).with_pipeline_depth(depth)
 
 
-def run_gpu(frames, steps, warmup, depth):
+def run_gpu(frames, steps, warmup, depth, net=None, d_frames=None, confidence=0.5):
     """Times `steps` P-frames (after one I-frame and `warmup` P-frames) through the pipelined
     encoder, inputs resident in HBM; the motion-search stage is bracketed by HIP events on the
-    encoder's stream. Returns elapsed seconds, per-frame stats, stage times and the stream."""
+    encoder's stream. With `net`, the learned form (the network's proposals seed the search).
+    Returns elapsed seconds, per-frame stats, stage times, the stream, the device frames and the
+    learned-form counts."""
     import torch
     import riv2_amd
 
     dev = torch.device("cuda:0")
-    d_frames = torch.from_numpy(frames).to(dev)
+    if d_frames is None:
+        d_frames = torch.from_numpy(frames).to(dev)
     torch.cuda.synchronize()
     fbytes = W * H * 3
     enc = riv2_amd.Encoder(make_config(riv2_amd, depth))
+    if net is not None:
+        enc.set_mvnet(net, confidence)
     ptr = lambda i: d_frames[i].data_ptr()
     enc.encode_frame_device(ptr(0), fbytes, W, H, 0)
     for i in range(1, 1 + warmup):
@@ -82,10 +88,48 @@ def run_gpu(frames, steps, warmup, depth):
     t1 = time.perf_counter()
     stats = enc.pop_stats(4096)
     stages = enc.stage_times()
+    counts = enc.learned_stats()
     enc.finish()
     data = enc.output()
     enc.close()
-    return t1 - t0, stats, stages, data, d_frames
+    return t1 - t0, stats, stages, data, d_frames, counts
+
+
+def mean_pframe_psnr_y(frames, stream):
+    """Mean PSNR-Y (BT.601 luma, as ri-quality computes it) of a stream's decoded P-frames."""
+    import riv2_amd
+    from riv2_amd import learned
+    dec = riv2_amd.Decoder(stream)
+    ps = []
+    for k in range(len(frames)):
+        f = dec.decode_frame()
+        if k > 0:
+            ps.append(learned.psnr_y(frames[k], f.data))
+    return sum(ps) / len(ps)
+
+
+def run_learned(frames, d_frames, steps, warmup, depth, plain_stream, plain_fps):
+    """The learned form on the same frames, timed like the plain form, with the trained network
+    committed beside the package (riv2_amd/mvnet_weights.npz, made by riv2_amd/train_mvnet.py)."""
+    from riv2_amd import learned
+    if not os.path.exists(learned.WEIGHTS_PATH):
+        return {"value": None, "reason": "no trained weights at riv2_amd/mvnet_weights.npz"}
+    net = learned.MvNet.load(learned.WEIGHTS_PATH)
+    elapsed, stats, stages, stream, _, counts = run_gpu(frames, steps, warmup, depth, net, d_frames)
+    fps = steps / elapsed
+    me_ms, me_n = stages["motion_search"]
+    p_plain = mean_pframe_psnr_y(frames, plain_stream)
+    p_learned = mean_pframe_psnr_y(frames, stream)
+    net.close()
+    return {"value": round(fps, 3), "unit": "fps", "vs_plain": round(fps / plain_fps, 4),
+            "ms_per_step": round(1000.0 * elapsed / steps, 4),
+            "motion_search_avg_ms": round(me_ms / max(1, me_n), 4),
+            "bytes_vs_plain": round(len(stream) / len(plain_stream), 5),
+            "psnr_y": round(p_learned, 4), "plain_psnr_y": round(p_plain, 4),
+            "fallback_rate": round(1.0 - counts[2] / max(1, counts[0]), 4),
+            "confident_rate": round(counts[1] / max(1, counts[0]), 4),
+            "note": "same frames and settings as the plain line; bytes and PSNR-Y over the whole 50-frame stream at q80 "
+                    "(rate at equal PSNR over several qualities: profiles/learned_mv_1080p_eval.json)"}
 
 
 def stage_breakdown(d_frames, n, depth):
@@ -190,6 +234,7 @@ def main():
     ap.add_argument("--pipeline-depth", type=int, default=32,
                     help="frames in flight (GPU analysis of frame k+1 overlaps host entropy coding of earlier frames)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned form's line")
     ap.add_argument("--batched", action="store_true", help="measure the batched 64-stream workload instead")
     ap.add_argument("--batch-per-size", type=int, default=16)
     ap.add_argument("--batch-p-frames", type=int, default=20)
@@ -212,10 +257,14 @@ def main():
     frames = synth.Sequence(synth.hd_params(seed=args.seed, frames=max(nframes, 50))).frames(0, nframes)
 
     print(f"[bench] GPU encode: 1 I + {args.warmup} warmup + {args.steps} timed P-frames", file=sys.stderr, flush=True)
-    elapsed, stats, stages, stream, d_frames = run_gpu(frames, args.steps, args.warmup, args.pipeline_depth)
+    elapsed, stats, stages, stream, d_frames, _ = run_gpu(frames, args.steps, args.warmup, args.pipeline_depth)
     stage_avg, gpu_frame_ms = stage_breakdown(d_frames, min(nframes, 12), args.pipeline_depth)
-    del d_frames
     fps = args.steps / elapsed
+    learned_leg = None
+    if not args.no_learned:
+        print("[bench] learned form on the same frames", file=sys.stderr, flush=True)
+        learned_leg = run_learned(frames, d_frames, args.steps, args.warmup, args.pipeline_depth, stream, fps)
+    del d_frames
     ms_per_step = 1000.0 * elapsed / args.steps
 
     me_ms, me_n = stages["motion_search"]
@@ -282,6 +331,7 @@ def main():
         "host_tail_avg_ms": round(sum(s.host_tail_ms for s in stats) / len(stats), 4),
         "me_passes_avg": round(sum(s.me_passes for s in stats) / len(stats), 2),
         "bytes_per_pframe": round(sum(s.total_bytes for s in stats) / len(stats), 1),
+        "learned": learned_leg,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(fps / cpu["value"], 2)

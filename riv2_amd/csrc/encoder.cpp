@@ -256,7 +256,10 @@ static int encoder_init(riv_encoder* e) {
     }
     RIV_HIP(e->err.alloc(4)); RIV_HIP(hipMemset(e->err.p, 0, 4));
     RIV_HIP(e->prop.alloc(16 * (size_t)e->nb));
-    RIV_HIP(e->lstat.alloc(3 * sizeof(unsigned long long))); RIV_HIP(hipMemset(e->lstat.p, 0, 3 * sizeof(unsigned long long)));
+    {   // three learned-form counters per wave of the search launch
+        const size_t lb = 3 * sizeof(unsigned long long) * (size_t)e->shape.groups * e->shape.waves;
+        RIV_HIP(e->lstat.alloc(lb)); RIV_HIP(hipMemset(e->lstat.p, 0, lb));
+    }
     e->depth = c.pipeline_depth;
     e->slots.resize(std::max(2, e->depth));
     for (Slot& s : e->slots) {
@@ -996,9 +999,11 @@ int riv_encoder_learned_stats(riv_encoder* e, uint64_t* counts) {
     if (!e || !counts) return RIV_E_ARG;
     RIV_HIP(hipSetDevice(e->cfg.device_id));
     RIV_HIP(hipStreamSynchronize(e->st));
-    unsigned long long h[3];
-    RIV_HIP(hipMemcpy(h, e->lstat.p, sizeof(h), hipMemcpyDeviceToHost));
-    for (int i = 0; i < 3; i++) counts[i] = h[i];
+    std::vector<unsigned long long> h(e->lstat.n / sizeof(unsigned long long));
+    RIV_HIP(hipMemcpy(h.data(), e->lstat.p, e->lstat.n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < 3; i++) counts[i] = 0;
+    for (size_t w = 0; w + 2 < h.size(); w += 3)
+        for (int i = 0; i < 3; i++) counts[i] += h[w + i];
     return RIV_OK;
 }
 

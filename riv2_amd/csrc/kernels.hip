@@ -592,9 +592,9 @@ __device__ uint32_t jsearch(const SearchParams& p, const SearchStream& S, const 
             if (p.pthr > 0 && __builtin_amdgcn_readlane(sv, win) <= p.pthr) skip_int = true;
         }
     }
-    // learned form: a confident block scores the network's four proposals; when the best is no
-    // worse than the best seed, a small diamond walks from it (at most 8 steps) in place of the
-    // hexagon search
+    // learned form: a confident block scores the network's four proposals and walks a small
+    // diamond (at most 8 steps) from the best; when that walk's cost is no worse than the best
+    // seed's, it replaces the hexagon search
     bool learned = false;
     if (!skip_int && (pr.z & 1u)) {
         const uint32_t props[4] = {pr.x & 0xffffu, pr.x >> 16, pr.y & 0xffffu, pr.y >> 16};
@@ -604,23 +604,21 @@ __device__ uint32_t jsearch(const SearchParams& p, const SearchStream& S, const 
         const int sv = jw_set_int<4, INTERIOR>(w, lane, v ? cx : 0, v ? cy : 0);
         const int key = v ? ((sv + cc.cost(cx, cy)) << 3) | lane : 0x7fffffff;
         const int m = lane_min8(key);
-        if ((m >> 3) <= scost) {
-            learned = true;
-            scost = m >> 3;
-            sdx = __builtin_amdgcn_readlane(cx, m & 7); sdy = __builtin_amdgcn_readlane(cy, m & 7);
-            const int dox = lane < 4 ? (int)((0x2213u >> (4 * lane)) & 15u) - 2 : 0;
-            const int doy = lane < 4 ? (int)((0x1322u >> (4 * lane)) & 15u) - 2 : 0;
-            for (int it = 0; it < 8; it++) {
-                const int dx = sdx + dox, dy = sdy + doy;
-                const bool vd = lane < 4 && abs(dx) <= sr && abs(dy) <= sr;
-                const int sd = jw_set_int<4, INTERIOR>(w, lane, vd ? dx : sdx, vd ? dy : sdy);
-                const int kd = vd ? ((sd + cc.cost(dx, dy)) << 3) | lane : 0x7fffffff;
-                const int md = lane_min8(kd);
-                if ((md >> 3) >= scost) break;
-                scost = md >> 3;
-                sdx = __builtin_amdgcn_readlane(dx, md & 7); sdy = __builtin_amdgcn_readlane(dy, md & 7);
-            }
+        int lcost = m >> 3;
+        int lx = __builtin_amdgcn_readlane(cx, m & 7), ly = __builtin_amdgcn_readlane(cy, m & 7);
+        const int dox = lane < 4 ? (int)((0x2213u >> (4 * lane)) & 15u) - 2 : 0;
+        const int doy = lane < 4 ? (int)((0x1322u >> (4 * lane)) & 15u) - 2 : 0;
+        for (int it = 0; it < 8; it++) {
+            const int dx = lx + dox, dy = ly + doy;
+            const bool vd = lane < 4 && abs(dx) <= sr && abs(dy) <= sr;
+            const int sd = jw_set_int<4, INTERIOR>(w, lane, vd ? dx : lx, vd ? dy : ly);
+            const int kd = vd ? ((sd + cc.cost(dx, dy)) << 3) | lane : 0x7fffffff;
+            const int md = lane_min8(kd);
+            if ((md >> 3) >= lcost) break;
+            lcost = md >> 3;
+            lx = __builtin_amdgcn_readlane(dx, md & 7); ly = __builtin_amdgcn_readlane(dy, md & 7);
         }
+        if (lcost <= scost) { learned = true; scost = lcost; sdx = lx; sdy = ly; }
     }
     *took = learned;
     if (!skip_int && !learned) {   // refine_integer_motion (:680-757): large hexagon until no improvement, then small diamond
@@ -767,6 +765,8 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
     // start field: bank 0 as the previous launch over this field left it (each block's final word),
     // a guess that spares the second pass every block whose guess its first pass confirms
     uint32_t myw = valid ? seed_word(field_load(p.field + myb), sr) : 0u;
+    // learned form: the block's proposals, read once (an evaluation takes them by lane read)
+    const uint4 myprop = (valid && p.prop) ? p.prop[myb] : make_uint4(0u, 0u, 0u, 0u);
     int mysad = 0;
     bool mytook = false;   // the block's last evaluation took the network's proposal
     int outcome = 0;     // 1 converged, -1 barrier timeout, 0 pass limit
@@ -807,7 +807,9 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
             uint32_t* wp = slots + (own ? k : p.nres) * SLOT;
             const bool load = !own || !((resident >> k) & 1u);
             if (own) resident |= 1ull << k;
-            const uint4 pr = p.prop ? p.prop[b] : make_uint4(0u, 0u, 0u, 0u);
+            const uint4 pr = make_uint4(sgpr((uint32_t)__builtin_amdgcn_readlane((int)myprop.x, k)),
+                                        sgpr((uint32_t)__builtin_amdgcn_readlane((int)myprop.y, k)),
+                                        sgpr((uint32_t)__builtin_amdgcn_readlane((int)myprop.z, k)), 0u);
             bool took;
             const uint32_t wS = mb_interior(bx * 16, by * 16, S.W, S.H, sr)
                                     ? jsearch<true>(p, S, ctab, wp, load, ND, NR, bx, by, lS, tS, trS, tlS, pT, pr, lane, &sad, &took)
@@ -870,12 +872,13 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
         // the next launch's start field; where bank 0 is the final field it already holds myw
         field_store(p.field + myb, myw);
     }
-    if (p.lstat) {
-        const unsigned long long vb = __ballot(valid), cb = __ballot(valid && (p.prop[valid ? myb : 0].z & 1u)), tb = __ballot(valid && mytook);
+    if (p.lstat) {   // this wave's own three counters (same-address atomics from every wave would serialise)
+        const unsigned long long vb = __ballot(valid), cb = __ballot(valid && (myprop.z & 1u)), tb = __ballot(valid && mytook);
         if (lane == 0 && vb) {
-            atomicAdd(p.lstat, (unsigned long long)__popcll(vb));
-            atomicAdd(p.lstat + 1, (unsigned long long)__popcll(cb));
-            atomicAdd(p.lstat + 2, (unsigned long long)__popcll(tb));
+            unsigned long long* q = p.lstat + 3 * (size_t)gw;
+            q[0] += (unsigned long long)__popcll(vb);
+            q[1] += (unsigned long long)__popcll(cb);
+            q[2] += (unsigned long long)__popcll(tb);
         }
     }
     // the adaptive quantiser's input: per block, the log variance of its luma prediction at its

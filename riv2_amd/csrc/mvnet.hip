@@ -264,26 +264,58 @@ __global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __re
 // Fused inference: 16 blocks per workgroup, every activation in LDS
 // ============================================================================================
 // Wave w runs the convolutions of blocks 4w .. 4w+3 of its group, one at a time, in its own LDS
-// (input 32x32x4, conv1 out 16x16x16, conv2 out 8x8x16); conv3 writes the block's row of the
-// group's fc1 input. Then the four waves split fc1's 128 outputs (two 16-column tiles each) and
-// wave 0 runs the head and writes the proposals.
+// (the 32x32 current and reference patches as bytes, conv1 out 16x16x16, conv2 out 8x8x16);
+// conv3 writes the block's row of the group's fc1 input. Then the four waves split fc1's 128
+// outputs (two 16-column tiles each) and wave 0 runs the head and writes the proposals. 61.5 KB
+// of LDS per workgroup: two workgroups per CU.
 namespace {
-constexpr int kWaveLds = 4096 + 4096 + 1024;   // uint16 entries: input, conv1 out, conv2 out
+constexpr int kWaveLds = 1024 + 4096 + 1024;   // uint16 entries: patches (2 x 1024 bytes), conv1 out, conv2 out
 constexpr int kInferLds = 4 * kWaveLds * 2 + 16 * rivnet::FC_IN * 2 + 16 * rivnet::FC_H * 2 + 16 * 16 * 4;
 
-// A fragment of a conv layer: output row m of this tile, K step ks, lane's 4 k values
-// (k = 16 ks + 4 (lane >> 4) + j): C == 4 -> tap 4 ks + (lane >> 4), channels 0..3; C == 16 ->
-// tap ks, channels 4 (lane >> 4) .. +3. The 4 values are one 8-byte LDS read.
-template <int C, int HI>
+// The block's 32x32 current and reference patches (8-pixel border, clamped to the frame) as bytes:
+// cur at P[0..1023], ref at P[1024..2047], row-major. Interior blocks read aligned dwords.
+__device__ __forceinline__ void load_patches(const rivnet::Source& S, int bx, int by, uint8_t* P, int lane) {
+    const int x0 = bx * 16 - 8, y0 = by * 16 - 8;
+    if (x0 >= 0 && y0 >= 0 && x0 + 32 <= S.W && y0 + 32 <= S.H) {
+        for (int q = lane; q < 512; q += 64) {
+            const int plane = q >> 8, row = (q >> 3) & 31, c4 = (q & 7) * 4;
+            const uint8_t* src = plane ? S.ref : S.cur;
+            *(uint32_t*)(P + plane * 1024 + row * 32 + c4) = *(const uint32_t*)(src + (size_t)(y0 + row) * S.W + x0 + c4);
+        }
+    } else {
+        for (int q = lane; q < 2048; q += 64) {
+            const int plane = q >> 10, row = (q >> 5) & 31, col = q & 31;
+            const int y = min(max(y0 + row, 0), S.H - 1), x = min(max(x0 + col, 0), S.W - 1);
+            P[q] = (plane ? S.ref : S.cur)[(size_t)y * S.W + x];
+        }
+    }
+}
+
+// conv1's A fragment for output row m, K step ks: tap 4 ks + (lane >> 4), its 4 input channels
+// formed from the patch bytes (zero past tap 8 and outside the patch).
+__device__ __forceinline__ v4s conv1_a(const uint8_t* P, int m, int ks, int lane) {
+    const int oy = m >> 4, ox = m & 15, tap = 4 * ks + (lane >> 4);
+    const int iy = 2 * oy + tap / 3 - 1, ix = 2 * ox + tap % 3 - 1;
+    v4s a = {0, 0, 0, 0};
+    if (tap < 9 && iy >= 0 && iy < 32 && ix >= 0 && ix < 32) {
+        const int c = P[iy * 32 + ix], r = P[1024 + iy * 32 + ix];
+        a[0] = (short)f2bf((float)(c - 128) * (1.0f / 64.0f));
+        a[1] = (short)f2bf((float)(r - 128) * (1.0f / 64.0f));
+        a[2] = (short)f2bf((float)(c - r) * (1.0f / 32.0f));
+    }
+    return a;
+}
+
+// A fragment of conv2 / conv3 (C = 16 input channels in NHWC bf16): output row m, K step ks =
+// tap, channels 4 (lane >> 4) .. +3 as one 8-byte LDS read.
+template <int HI>
 __device__ __forceinline__ v4s conv_a(const uint16_t* x, int m, int ks, int lane) {
     constexpr int HO = HI / 2;
     const int oy = m / HO, ox = m % HO;
-    const int tap = C == 4 ? 4 * ks + (lane >> 4) : ks;
-    const int c0 = C == 4 ? 0 : 4 * (lane >> 4);
-    const int iy = 2 * oy + tap / 3 - 1, ix = 2 * ox + tap % 3 - 1;
+    const int iy = 2 * oy + ks / 3 - 1, ix = 2 * ox + ks % 3 - 1;
     v4s a = {0, 0, 0, 0};
-    if (tap < 9 && iy >= 0 && iy < HI && ix >= 0 && ix < HI) {
-        const uint2 v = *(const uint2*)(x + (iy * HI + ix) * C + c0);
+    if (iy >= 0 && iy < HI && ix >= 0 && ix < HI) {
+        const uint2 v = *(const uint2*)(x + (iy * HI + ix) * 16 + 4 * (lane >> 4));
         a[0] = (short)(v.x & 0xffff); a[1] = (short)(v.x >> 16); a[2] = (short)(v.y & 0xffff); a[3] = (short)(v.y >> 16);
     }
     return a;
@@ -310,8 +342,8 @@ __global__ __launch_bounds__(256) void k_net_infer(const rivnet::Source* __restr
                                                    float* __restrict__ yout) {
     extern __shared__ __align__(16) uint16_t lds16[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    uint16_t* x0 = lds16 + wv * kWaveLds;
-    uint16_t* a1 = x0 + 4096;
+    uint8_t* P = (uint8_t*)(lds16 + wv * kWaveLds);
+    uint16_t* a1 = lds16 + wv * kWaveLds + 1024;
     uint16_t* a2 = a1 + 4096;
     uint16_t* a3 = lds16 + 4 * kWaveLds;            // [16][FC_IN]
     uint16_t* a4 = a3 + 16 * rivnet::FC_IN;         // [16][FC_H]
@@ -335,24 +367,26 @@ __global__ __launch_bounds__(256) void k_net_infer(const rivnet::Source* __restr
         sample_of(samples, i, &si, &blk);
         const rivnet::Source S = srcs ? srcs[si] : s0;
         const int bx = blk % S.bw, by = blk / S.bw;
-        for (int p = lane; p < 1024; p += 64) ((uint2*)x0)[p] = patch_px(S, bx, by, p & 31, p >> 5);
+        load_patches(S, bx, by, P, lane);
         if (lane < 16) row3[256 + lane] = f2bf(pred_input(S, bx, by, lane));
         __builtin_amdgcn_s_waitcnt(0);   // LDS writes of this wave done before its reads (one wave, in order)
         __builtin_amdgcn_wave_barrier();
         // conv1: 256 output pixels (16 tiles), K 48
+#pragma unroll 4
         for (int tm = 0; tm < 16; tm++) {
             v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int ks = 0; ks < 3; ks++) acc = mfma16(conv_a<4, 32>(x0, tm * 16 + (lane & 15), ks, lane), w1[ks], acc);
+            for (int ks = 0; ks < 3; ks++) acc = mfma16(conv1_a(P, tm * 16 + (lane & 15), ks, lane), w1[ks], acc);
             store_relu(acc, wt.b1, 0, tm * 16, a1, 16, lane);
         }
         __builtin_amdgcn_s_waitcnt(0);
         __builtin_amdgcn_wave_barrier();
         // conv2: 64 output pixels (4 tiles), K 144
+#pragma unroll
         for (int tm = 0; tm < 4; tm++) {
             v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int ks = 0; ks < 9; ks++) acc = mfma16(conv_a<16, 16>(a1, tm * 16 + (lane & 15), ks, lane), w2[ks], acc);
+            for (int ks = 0; ks < 9; ks++) acc = mfma16(conv_a<16>(a1, tm * 16 + (lane & 15), ks, lane), w2[ks], acc);
             store_relu(acc, wt.b2, 0, tm * 16, a2, 16, lane);
         }
         __builtin_amdgcn_s_waitcnt(0);
@@ -361,7 +395,7 @@ __global__ __launch_bounds__(256) void k_net_infer(const rivnet::Source* __restr
         {
             v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int ks = 0; ks < 9; ks++) acc = mfma16(conv_a<16, 8>(a2, lane & 15, ks, lane), w3[ks], acc);
+            for (int ks = 0; ks < 9; ks++) acc = mfma16(conv_a<8>(a2, lane & 15, ks, lane), w3[ks], acc);
             store_relu(acc, wt.b3, 0, 0, row3, 16, lane);
         }
         __builtin_amdgcn_s_waitcnt(0);

@@ -94,8 +94,8 @@ struct SearchParams {
     int* err;                    // 1 on a barrier timeout, 3 if the pass limit is hit, 2 if a
                                  // reconstructed MV would differ from its search MV
     // learned form (single-stream launches): per block the network's proposals (candidates 0|1,
-    // 2|3 as (x, y) bytes, confident flag, logit bits), or null for the plain search; lstat
-    // accumulates (blocks, confident, proposal taken)
+    // 2|3 as (x, y) bytes, confident flag, logit bits), or null for the plain search; wave w of
+    // the launch accumulates (blocks, confident, proposal taken) at lstat[3w .. 3w+2]
     const uint4* prop;
     unsigned long long* lstat;
 };

@@ -22,6 +22,7 @@ from dataclasses import dataclass, field
 from typing import List, Optional
 
 import numpy as np
+import torch   # before the backend initialises the HIP runtime, so that torch finds the device
 
 from . import _lib as L
 from .api import EncodeError, Encoder, EncoderConfig, Frame, KeyframeInterval, _check
@@ -141,7 +142,6 @@ class PairSet:
 def collect_pairs(seeds, pframes: int = 4, device: int = 0, log=None) -> PairSet:
     """Encodes 1 I + `pframes` P-frames of each seed's 1080p synthetic sequence with the plain
     GPU encoder; every P-frame block gives one pair, with the frame pair resident on the device."""
-    import torch
     from . import synth
     dev = torch.device("cuda", device)
     ps = PairSet()
@@ -207,7 +207,6 @@ def proposal_accuracy(net: MvNet, ps: PairSet, confidence: float = 0.5, sr: int 
 
 def psnr_y(a: np.ndarray, b: np.ndarray) -> float:
     """PSNR of the BT.601 luma of two RGB24 frames (ri-quality's compute_psnr, y part)."""
-    import torch
     ta = torch.from_numpy(a).to("cuda", torch.float64)
     tb = torch.from_numpy(b).to("cuda", torch.float64)
     w = torch.tensor([0.299, 0.587, 0.114], dtype=torch.float64, device="cuda")

@@ -40,9 +40,21 @@ def main(argv=None):
     ap.add_argument("--qualities", default="70,75,80,85")
     ap.add_argument("--out", default=learned.WEIGHTS_PATH)
     ap.add_argument("--report", default=None)
+    ap.add_argument("--weights", default=None, help="evaluate these trained parameters instead of training")
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
 
+    if a.weights:
+        net = learned.MvNet.load(a.weights)
+        qs = tuple(int(q) for q in a.qualities.split(","))
+        ev = learned.compare(net, seed_range(a.eval_seeds), a.eval_frames, qs, a.confidence, log=log)
+        report = {"weights": os.path.relpath(a.weights, ROOT),
+                  "eval": {"seeds": a.eval_seeds, "frames": a.eval_frames, "confidence": a.confidence, **ev}}
+        log(json.dumps(report["eval"]))
+        if a.report:
+            with open(a.report, "w") as f:
+                json.dump(report, f, indent=1)
+        return report
     t0 = time.time()
     ps = learned.collect_pairs(seed_range(a.seeds), a.pframes, log=log)
     t1 = time.time()
