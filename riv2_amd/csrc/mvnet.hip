@@ -26,14 +26,16 @@ namespace {
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ uint16_t f2bf(float f) {   // round to nearest even
-    uint32_t u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+__device__ __forceinline__ uint16_t f2bf(float f) {   // round to nearest even (v_cvt_pk_bf16_f32)
+    return __builtin_bit_cast(uint16_t, (__bf16)f);
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
 __device__ __forceinline__ v4f mfma16(v4s a, v4s b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+
+// s_waitcnt lgkmcnt(0) with vmcnt and expcnt at their maxima: LDS (and scalar) accesses drain,
+// vector memory loads stay in flight
+__device__ __forceinline__ void lds_wait() { __builtin_amdgcn_s_waitcnt(0xC07F); }
 
 }  // namespace
 
@@ -263,59 +265,67 @@ __global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __re
 // ============================================================================================
 // Fused inference: 16 blocks per workgroup, every activation in LDS
 // ============================================================================================
-// Wave w runs the convolutions of blocks 4w .. 4w+3 of its group, one at a time, in its own LDS
-// (the 32x32 current and reference patches as bytes, conv1 out 16x16x16, conv2 out 8x8x16);
-// conv3 writes the block's row of the group's fc1 input. Then the four waves split fc1's 128
-// outputs (two 16-column tiles each) and wave 0 runs the head and writes the proposals. 61.5 KB
-// of LDS per workgroup: two workgroups per CU.
+// Wave w runs the convolutions of blocks 2w and 2w+1 of its group, one at a time, in its own LDS
+// (the 32x32x4 input in bf16, conv1 out 16x16x16, conv2 out 8x8x16); conv3 writes the block's
+// row of the group's fc1 input. Then the eight waves split fc1's 128 outputs (one 16-column tile
+// each) and wave 0 runs the head and writes the proposals. 157.5 KB of LDS: one workgroup of 8
+// waves per CU. Each pixel is converted to bf16 once; a conv fragment is one 8-byte LDS read.
 namespace {
-constexpr int kWaveLds = 1024 + 4096 + 1024;   // uint16 entries: patches (2 x 1024 bytes), conv1 out, conv2 out
-constexpr int kInferLds = 4 * kWaveLds * 2 + 16 * rivnet::FC_IN * 2 + 16 * rivnet::FC_H * 2 + 16 * 16 * 4;
+constexpr int kInferWaves = 8;
+constexpr int kWaveLds = 4096 + 4096 + 1024;   // uint16 entries: input, conv1 out, conv2 out
+constexpr int kInferLds = kInferWaves * kWaveLds * 2 + 16 * rivnet::FC_IN * 2 + 16 * rivnet::FC_H * 2 + 16 * 16 * 4;
 
-// The block's 32x32 current and reference patches (8-pixel border, clamped to the frame) as bytes:
-// cur at P[0..1023], ref at P[1024..2047], row-major. Interior blocks read aligned dwords.
-__device__ __forceinline__ void load_patches(const rivnet::Source& S, int bx, int by, uint8_t* P, int lane) {
+__device__ __forceinline__ bool patch_interior(const rivnet::Source& S, int bx, int by) {
     const int x0 = bx * 16 - 8, y0 = by * 16 - 8;
-    if (x0 >= 0 && y0 >= 0 && x0 + 32 <= S.W && y0 + 32 <= S.H) {
-        for (int q = lane; q < 512; q += 64) {
-            const int plane = q >> 8, row = (q >> 3) & 31, c4 = (q & 7) * 4;
-            const uint8_t* src = plane ? S.ref : S.cur;
-            *(uint32_t*)(P + plane * 1024 + row * 32 + c4) = *(const uint32_t*)(src + (size_t)(y0 + row) * S.W + x0 + c4);
-        }
-    } else {
-        for (int q = lane; q < 2048; q += 64) {
-            const int plane = q >> 10, row = (q >> 5) & 31, col = q & 31;
-            const int y = min(max(y0 + row, 0), S.H - 1), x = min(max(x0 + col, 0), S.W - 1);
-            P[q] = (plane ? S.ref : S.cur)[(size_t)y * S.W + x];
-        }
+    return x0 >= 0 && y0 >= 0 && x0 + 32 <= S.W && y0 + 32 <= S.H;
+}
+// An interior block's patches as aligned dwords: lane's dword u < 4 holds current-frame pixels
+// (row, c4 .. c4+3) with q = lane + 64u, row = q >> 3, c4 = 4 (q & 7); dword u + 4 the same
+// pixels of the reference.
+__device__ __forceinline__ void patch_fetch(const rivnet::Source& S, int bx, int by, uint32_t (&d)[8], int lane) {
+    const int x0 = bx * 16 - 8, y0 = by * 16 - 8;
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        const int q = lane + 64 * (u & 3), row = q >> 3, c4 = (q & 7) * 4;
+        d[u] = *(const uint32_t*)((u >= 4 ? S.ref : S.cur) + (size_t)(y0 + row) * S.W + x0 + c4);
+    }
+}
+__device__ __forceinline__ uint2 px4(int c, int r) {   // the input's 4 channels of one pixel
+    const uint32_t a = f2bf((float)(c - 128) * (1.0f / 64.0f)), b = f2bf((float)(r - 128) * (1.0f / 64.0f));
+    const uint32_t d = f2bf((float)(c - r) * (1.0f / 32.0f));
+    return make_uint2(a | (b << 16), d);
+}
+__device__ __forceinline__ void patch_store(const uint32_t (&d)[8], uint16_t* x, int lane) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int q = lane + 64 * u, row = q >> 3, c4 = (q & 7) * 4;
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            *(uint2*)(x + (row * 32 + c4 + i) * 4) = px4((int)((d[u] >> (8 * i)) & 0xffu), (int)((d[u + 4] >> (8 * i)) & 0xffu));
+    }
+}
+__device__ __forceinline__ void patch_edge(const rivnet::Source& S, int bx, int by, uint16_t* x, int lane) {
+    const int x0 = bx * 16 - 8, y0 = by * 16 - 8;
+    for (int p = lane; p < 1024; p += 64) {
+        const int yy = min(max(y0 + (p >> 5), 0), S.H - 1), xx = min(max(x0 + (p & 31), 0), S.W - 1);
+        const size_t o = (size_t)yy * S.W + xx;
+        *(uint2*)(x + p * 4) = px4(S.cur[o], S.ref[o]);
     }
 }
 
-// conv1's A fragment for output row m, K step ks: tap 4 ks + (lane >> 4), its 4 input channels
-// formed from the patch bytes (zero past tap 8 and outside the patch).
-__device__ __forceinline__ v4s conv1_a(const uint8_t* P, int m, int ks, int lane) {
-    const int oy = m >> 4, ox = m & 15, tap = 4 * ks + (lane >> 4);
-    const int iy = 2 * oy + tap / 3 - 1, ix = 2 * ox + tap % 3 - 1;
-    v4s a = {0, 0, 0, 0};
-    if (tap < 9 && iy >= 0 && iy < 32 && ix >= 0 && ix < 32) {
-        const int c = P[iy * 32 + ix], r = P[1024 + iy * 32 + ix];
-        a[0] = (short)f2bf((float)(c - 128) * (1.0f / 64.0f));
-        a[1] = (short)f2bf((float)(r - 128) * (1.0f / 64.0f));
-        a[2] = (short)f2bf((float)(c - r) * (1.0f / 32.0f));
-    }
-    return a;
-}
-
-// A fragment of conv2 / conv3 (C = 16 input channels in NHWC bf16): output row m, K step ks =
-// tap, channels 4 (lane >> 4) .. +3 as one 8-byte LDS read.
-template <int HI>
+// A fragment of a conv layer from its NHWC bf16 input x (C = 4: tap 4 ks + (lane >> 4), all 4
+// channels; C = 16: tap ks, channels 4 (lane >> 4) .. +3), output row m: one 8-byte LDS read,
+// zero past tap 8 and outside the input.
+template <int C, int HI>
 __device__ __forceinline__ v4s conv_a(const uint16_t* x, int m, int ks, int lane) {
     constexpr int HO = HI / 2;
     const int oy = m / HO, ox = m % HO;
-    const int iy = 2 * oy + ks / 3 - 1, ix = 2 * ox + ks % 3 - 1;
+    const int tap = C == 4 ? 4 * ks + (lane >> 4) : ks;
+    const int c0 = C == 4 ? 0 : 4 * (lane >> 4);
+    const int iy = 2 * oy + tap / 3 - 1, ix = 2 * ox + tap % 3 - 1;
     v4s a = {0, 0, 0, 0};
-    if (iy >= 0 && iy < HI && ix >= 0 && ix < HI) {
-        const uint2 v = *(const uint2*)(x + (iy * HI + ix) * 16 + 4 * (lane >> 4));
+    if (tap < 9 && iy >= 0 && iy < HI && ix >= 0 && ix < HI) {
+        const uint2 v = *(const uint2*)(x + (iy * HI + ix) * C + c0);
         a[0] = (short)(v.x & 0xffff); a[1] = (short)(v.x >> 16); a[2] = (short)(v.y & 0xffff); a[3] = (short)(v.y >> 16);
     }
     return a;
@@ -337,15 +347,15 @@ __device__ __forceinline__ void store_relu(v4f acc, const float* bias, int n0, i
 }
 }  // namespace
 
-__global__ __launch_bounds__(256) void k_net_infer(const rivnet::Source* __restrict__ srcs, rivnet::Source s0, const int* __restrict__ samples, int n,
+__global__ __launch_bounds__(512) void k_net_infer(const rivnet::Source* __restrict__ srcs, rivnet::Source s0, const int* __restrict__ samples, int n,
                                                    rivnet::Weights wt, float conf_logit, int sr, uint4* __restrict__ prop,
                                                    float* __restrict__ yout) {
     extern __shared__ __align__(16) uint16_t lds16[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    uint8_t* P = (uint8_t*)(lds16 + wv * kWaveLds);
-    uint16_t* a1 = lds16 + wv * kWaveLds + 1024;
+    uint16_t* x0 = lds16 + wv * kWaveLds;
+    uint16_t* a1 = x0 + 4096;
     uint16_t* a2 = a1 + 4096;
-    uint16_t* a3 = lds16 + 4 * kWaveLds;            // [16][FC_IN]
+    uint16_t* a3 = lds16 + kInferWaves * kWaveLds;  // [16][FC_IN]
     uint16_t* a4 = a3 + 16 * rivnet::FC_IN;         // [16][FC_H]
     float* yl = (float*)(a4 + 16 * rivnet::FC_H);   // [16][16]
 
@@ -356,8 +366,22 @@ __global__ __launch_bounds__(256) void k_net_infer(const rivnet::Source* __restr
 #pragma unroll
     for (int ks = 0; ks < 9; ks++) { w2[ks] = w_frag(wt.w2, 16, ks, 0, lane); w3[ks] = w_frag(wt.w3, 16, ks, 0, lane); }
 
-    for (int q = 0; q < 4; q++) {
-        const int slot = 4 * wv + q, i = blockIdx.x * 16 + slot;
+    // the next block's interior patch, fetched while the current one is computed
+    uint32_t pf[8];
+    bool pf_ok = false;
+    auto fetch = [&](int q) {
+        const int i = blockIdx.x * 16 + 2 * wv + q;
+        pf_ok = false;
+        if (q >= 2 || i >= n) return;
+        int si, blk;
+        sample_of(samples, i, &si, &blk);
+        const rivnet::Source S = srcs ? srcs[si] : s0;
+        const int bx = blk % S.bw, by = blk / S.bw;
+        if (patch_interior(S, bx, by)) { patch_fetch(S, bx, by, pf, lane); pf_ok = true; }
+    };
+    fetch(0);
+    for (int q = 0; q < 2; q++) {
+        const int slot = 2 * wv + q, i = blockIdx.x * 16 + slot;
         uint16_t* row3 = a3 + slot * rivnet::FC_IN;
         if (i >= n) {   // past the end: a zero row, never stored
             for (int k = lane; k < rivnet::FC_IN; k += 64) row3[k] = 0;
@@ -367,45 +391,48 @@ __global__ __launch_bounds__(256) void k_net_infer(const rivnet::Source* __restr
         sample_of(samples, i, &si, &blk);
         const rivnet::Source S = srcs ? srcs[si] : s0;
         const int bx = blk % S.bw, by = blk / S.bw;
-        load_patches(S, bx, by, P, lane);
+        if (pf_ok) patch_store(pf, x0, lane);
+        else patch_edge(S, bx, by, x0, lane);
+        fetch(q + 1);
         if (lane < 16) row3[256 + lane] = f2bf(pred_input(S, bx, by, lane));
-        __builtin_amdgcn_s_waitcnt(0);   // LDS writes of this wave done before its reads (one wave, in order)
+        lds_wait();   // this wave's LDS writes done before its reads; the prefetch stays in flight
         __builtin_amdgcn_wave_barrier();
         // conv1: 256 output pixels (16 tiles), K 48
 #pragma unroll 4
         for (int tm = 0; tm < 16; tm++) {
             v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int ks = 0; ks < 3; ks++) acc = mfma16(conv1_a(P, tm * 16 + (lane & 15), ks, lane), w1[ks], acc);
+            for (int ks = 0; ks < 3; ks++) acc = mfma16(conv_a<4, 32>(x0, tm * 16 + (lane & 15), ks, lane), w1[ks], acc);
             store_relu(acc, wt.b1, 0, tm * 16, a1, 16, lane);
         }
-        __builtin_amdgcn_s_waitcnt(0);
+        lds_wait();
         __builtin_amdgcn_wave_barrier();
         // conv2: 64 output pixels (4 tiles), K 144
 #pragma unroll
         for (int tm = 0; tm < 4; tm++) {
             v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int ks = 0; ks < 9; ks++) acc = mfma16(conv_a<16>(a1, tm * 16 + (lane & 15), ks, lane), w2[ks], acc);
+            for (int ks = 0; ks < 9; ks++) acc = mfma16(conv_a<16, 16>(a1, tm * 16 + (lane & 15), ks, lane), w2[ks], acc);
             store_relu(acc, wt.b2, 0, tm * 16, a2, 16, lane);
         }
-        __builtin_amdgcn_s_waitcnt(0);
+        lds_wait();
         __builtin_amdgcn_wave_barrier();
         // conv3: 16 output pixels, K 144 -> features p * 16 + channel of the fc1 row
         {
             v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int ks = 0; ks < 9; ks++) acc = mfma16(conv_a<8>(a2, lane & 15, ks, lane), w3[ks], acc);
+            for (int ks = 0; ks < 9; ks++) acc = mfma16(conv_a<16, 8>(a2, lane & 15, ks, lane), w3[ks], acc);
             store_relu(acc, wt.b3, 0, 0, row3, 16, lane);
         }
-        __builtin_amdgcn_s_waitcnt(0);
+        lds_wait();
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    // fc1: 16 rows x 128 columns, K 272; wave w takes columns 32w .. 32w+31
-    for (int nt = 0; nt < 2; nt++) {
-        const int n0 = 32 * wv + 16 * nt;
+    // fc1: 16 rows x 128 columns, K 272; wave w takes columns 16w .. 16w+15
+    {
+        const int n0 = 16 * wv;
         v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
         for (int ks = 0; ks < rivnet::FC_IN / 16; ks++) {
             const uint2 v = *(const uint2*)(a3 + (lane & 15) * rivnet::FC_IN + 16 * ks + 4 * (lane >> 4));
             v4s a;
@@ -418,6 +445,7 @@ __global__ __launch_bounds__(256) void k_net_infer(const rivnet::Source* __restr
     if (wv != 0) return;
     {   // head: 16 rows x 16 columns, K 128
         v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
         for (int ks = 0; ks < rivnet::FC_H / 16; ks++) {
             const uint2 v = *(const uint2*)(a4 + (lane & 15) * rivnet::FC_H + 16 * ks + 4 * (lane >> 4));
             v4s a;
@@ -433,7 +461,7 @@ __global__ __launch_bounds__(256) void k_net_infer(const rivnet::Source* __restr
             if (yout && i < n) yout[(size_t)i * rivnet::OUT + col] = y;
         }
     }
-    __builtin_amdgcn_s_waitcnt(0);
+    lds_wait();
     __builtin_amdgcn_wave_barrier();
     const int i = blockIdx.x * 16 + lane;
     if (lane < 16 && i < n && prop) {
@@ -501,7 +529,7 @@ hipError_t infer(hipStream_t st, const Source* srcs, const Source& s0, const int
         if (e != hipSuccess) return e;
         attr = true;
     }
-    if (n > 0) k_net_infer<<<(n + 15) / 16, 256, kInferLds, st>>>(srcs, s0, samples, n, w, conf_logit, sr, prop, yout);
+    if (n > 0) k_net_infer<<<(n + 15) / 16, 64 * kInferWaves, kInferLds, st>>>(srcs, s0, samples, n, w, conf_logit, sr, prop, yout);
     return hipGetLastError();
 }
 
