@@ -219,6 +219,17 @@ int riv_encoder_set_profiling(riv_encoder* enc, int on);
 /* Accumulated milliseconds and launch counts per stage since profiling was switched on. */
 int riv_encoder_stage_times(riv_encoder* enc, double* ms, uint32_t* counts);
 
+/* Batch-step launch-shape autotuner (no reference counterpart; the reference has no batched
+ * form). Per (blocks, streams, search range) of a batch, the first batch steps led by `enc` (the
+ * first encoder of the batch) each launch one candidate motion-search shape (waves per
+ * workgroup, workgroups, window slots per wave), timed with events read without waiting; after
+ * three samples of every candidate the fastest is kept. The shape changes no output. On by
+ * default; off keeps the occupancy-derived shape. */
+int riv_encoder_set_batch_autotune(riv_encoder* enc, int32_t on);
+/* The latest batch step's shape: out[0] waves per workgroup, [1] workgroups, [2] window slots per
+ * wave, [3] candidates, [4] 1 once tuning has settled, [5] the chosen candidate (or -1). */
+int riv_encoder_batch_tuning(riv_encoder* enc, int32_t* out);
+
 /* ---- learned form: a BF16 MFMA MV-proposal network seeding the motion search ----
  * The reference has no learned form; the search it seeds is process_block_hex
  * (reitero_encode/src/motion/scalar.rs:546-636). Per 16x16 block the network reads the current

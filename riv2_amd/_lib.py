@@ -124,6 +124,8 @@ PROTOTYPES = [
     ("riv_gpu_logf_exp2f", c_int, [c_int32, c_void_p, c_size_t, c_int32, c_void_p]),
     ("riv_encoder_set_profiling", c_int, [c_void_p, c_int]),
     ("riv_encoder_stage_times", c_int, [c_void_p, c_void_p, c_void_p]),
+    ("riv_encoder_set_batch_autotune", c_int, [c_void_p, c_int32]),
+    ("riv_encoder_batch_tuning", c_int, [c_void_p, c_void_p]),
     ("riv_mvnet_param_count", c_size_t, []),
     ("riv_mvnet_create", c_int, [c_int32, c_uint64, POINTER(c_void_p)]),
     ("riv_mvnet_destroy", None, [c_void_p]),

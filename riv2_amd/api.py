@@ -319,6 +319,18 @@ class Encoder:
         _check(L.lib().riv_encoder_stage_times(self._h, ms.ctypes.data, n.ctypes.data))
         return {name: (float(ms[i]), int(n[i])) for i, name in enumerate(self.STAGES)}
 
+    # ---- batch-step launch-shape autotuner (no reference counterpart) ----
+    def set_batch_autotune(self, on: bool = True):
+        _check(L.lib().riv_encoder_set_batch_autotune(self._h, 1 if on else 0), EncodeError)
+
+    def batch_tuning(self):
+        """The latest batch step led by this encoder: dict of waves, groups, slots, candidates,
+        settled, chosen."""
+        o = np.zeros(6, np.int32)
+        _check(L.lib().riv_encoder_batch_tuning(self._h, o.ctypes.data), EncodeError)
+        return {"waves": int(o[0]), "groups": int(o[1]), "slots": int(o[2]), "candidates": int(o[3]), "settled": bool(o[4]),
+                "chosen": int(o[5])}
+
     # ---- learned form (no reference counterpart; see riv2_amd.learned) ----
     def set_mvnet(self, net, confidence: float = 0.5):
         """Seeds the motion search with `net`'s proposals from the next P-frame (None: plain form)."""

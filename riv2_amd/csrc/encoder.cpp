@@ -10,6 +10,8 @@
 #include <algorithm>
 #include <condition_variable>
 #include <deque>
+#include <map>
+#include <tuple>
 #include <functional>
 #include <mutex>
 #include <thread>
@@ -111,6 +113,54 @@ HostStage* g_hs = nullptr;
 int g_hs_refs = 0;
 }  // namespace
 
+// Online autotuner of the batched motion search's launch shape, per (blocks, streams, search
+// range): the first steps of a key each launch one candidate shape (every shape reaches the same
+// fixed point, so the bitstreams do not change), timed by two events around the launch that are
+// read once they have completed; after kTuneSamples samples of every candidate the fastest
+// mean is kept.
+struct ShapeTuner {
+    static constexpr int kTuneSamples = 3;
+    struct Cand { rivk::SearchShape sh{}; double ms = 0; int n = 0; int issued = 0; };
+    struct Pending { int ci; hipEvent_t a, b; };
+    std::vector<Cand> cands;
+    std::deque<Pending> pending;
+    std::vector<hipEvent_t> spare;
+    int best = -1;
+    int last = -1;   // the candidate of the latest step
+    ~ShapeTuner() {
+        for (const Pending& q : pending) { (void)hipEventDestroy(q.a); (void)hipEventDestroy(q.b); }
+        for (hipEvent_t ev : spare) (void)hipEventDestroy(ev);
+    }
+    void collect() {   // completed timings, in launch order
+        while (!pending.empty() && hipEventQuery(pending.front().b) == hipSuccess) {
+            const Pending q = pending.front();
+            pending.pop_front();
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, q.a, q.b) == hipSuccess) { cands[q.ci].ms += ms; cands[q.ci].n++; }
+            spare.push_back(q.a); spare.push_back(q.b);
+        }
+        if (best >= 0 || !pending.empty()) return;
+        for (const Cand& c : cands) if (c.n < kTuneSamples) return;
+        best = 0;
+        for (int i = 1; i < (int)cands.size(); i++)
+            if (cands[i].ms / cands[i].n < cands[best].ms / cands[best].n) best = i;
+    }
+    // the candidate for the next step; -1 means tuning is over (use `best`)
+    int next() const {
+        if (best >= 0) return -1;
+        int pick = -1;
+        for (int i = 0; i < (int)cands.size(); i++)
+            if (cands[i].issued < kTuneSamples && (pick < 0 || cands[i].issued < cands[pick].issued)) pick = i;
+        return pick;
+    }
+    hipEvent_t event() {
+        hipEvent_t ev = nullptr;
+        if (!spare.empty()) { ev = spare.back(); spare.pop_back(); return ev; }
+        if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+        return ev;
+    }
+};
+
 struct riv_encoder {
     riv_encoder_config cfg{};
     riv_writer_vtbl wv{};
@@ -145,6 +195,9 @@ struct riv_encoder {
     int b_bank = 0, b_stage = 0, b_maxp = 0;
     HostBuf b_stage_buf[2];
     hipEvent_t b_stage_ev[2] = {nullptr, nullptr};
+    bool b_autotune = true;
+    std::map<std::tuple<int, int, int>, ShapeTuner> b_tuners;   // by (blocks, streams, search range)
+    ShapeTuner* b_last_tuner = nullptr;
 
     // ordered tail + completion state
     std::mutex mu;
@@ -862,14 +915,24 @@ static int batch_step(riv_encoder* const* encs, size_t n, const void* const* rgb
     if (!desc.empty()) {
         riv::SearchParams p = search_params(e0);
         p.nstreams = (int)desc.size(); p.nb = total; p.maxp = maxp;
-        // launch shape for the whole batch
-        hipDeviceProp_t prop;
-        RIV_HIP(hipGetDeviceProperties(&prop, c0.device_id));
-        rivk::SearchShape sh;
-        if (!rivk::search_shape(p.sr, total, p.nstreams, prop.multiProcessorCount, prop.sharedMemPerBlock, &sh)) {
-            set_error("batch has more macroblocks or streams than one motion-search launch can hold");
-            return RIV_E_UNSUPPORTED;
+        // launch shape for the whole batch: the autotuner's choice for this batch's key
+        ShapeTuner& T = e0->b_tuners[std::make_tuple(total, p.nstreams, p.sr)];
+        if (T.cands.empty()) {
+            hipDeviceProp_t prop;
+            RIV_HIP(hipGetDeviceProperties(&prop, c0.device_id));
+            for (const rivk::SearchShape& c : rivk::search_shapes(p.sr, total, p.nstreams, prop.multiProcessorCount, prop.sharedMemPerBlock))
+                T.cands.push_back(ShapeTuner::Cand{c});
+            if (T.cands.empty()) {
+                set_error("batch has more macroblocks or streams than one motion-search launch can hold");
+                return RIV_E_UNSUPPORTED;
+            }
+            if (!e0->b_autotune || T.cands.size() == 1) T.best = 0;
         }
+        T.collect();
+        const int ci = T.next();
+        T.last = ci >= 0 ? ci : T.best;
+        e0->b_last_tuner = &T;
+        const rivk::SearchShape sh = T.cands[T.last].sh;
         p.nslot = sh.nslot; p.nres = sh.nres;
         // scratch sized for the batch (grown when a larger batch comes; the stream is drained first)
         const size_t need_field = 2 * (size_t)total, need_arrive = 2 * (size_t)riv::barrier_words(maxp);
@@ -908,7 +971,17 @@ static int batch_step(riv_encoder* const* encs, size_t n, const void* const* rgb
         p.streams = e0->b_desc.as<riv::SearchStream>();
         p.field = e0->b_field.as<uint32_t>(); p.arrive = e0->b_arrive.as<unsigned>(); p.bank = e0->b_bank;
         e0->b_bank ^= 1;
+        hipEvent_t ta = ci >= 0 ? T.event() : nullptr, tb = ci >= 0 ? T.event() : nullptr;
+        if (ta && tb) RIV_HIP(hipEventRecord(ta, st));
         RIV_HIP(rivk::me_search(st, p, sh.groups, sh.waves));
+        if (ta && tb) {
+            RIV_HIP(hipEventRecord(tb, st));
+            T.pending.push_back({ci, ta, tb});
+            T.cands[ci].issued++;
+        } else {
+            if (ta) T.spare.push_back(ta);
+            if (tb) T.spare.push_back(tb);
+        }
     }
     int status = RIV_OK;
     for (size_t i = 0; i < n; i++) {
@@ -937,6 +1010,22 @@ int riv_encode_batch_step(riv_encoder* const* encs, size_t n, const void* const*
                           int on_device, riv_frame_stats* stats) {
     if (!encs || !n || !rgb || !len) return RIV_E_ARG;
     return batch_step(encs, n, rgb, len, timestamp_ms, on_device != 0, stats);
+}
+
+int riv_encoder_set_batch_autotune(riv_encoder* e, int on) {
+    if (!e) return RIV_E_ARG;
+    e->b_autotune = on != 0;
+    return RIV_OK;
+}
+
+int riv_encoder_batch_tuning(riv_encoder* e, int32_t* out) {
+    if (!e || !out) return RIV_E_ARG;
+    const ShapeTuner* T = e->b_last_tuner;
+    if (!T || T->last < 0) { set_error("no batch step has been launched with this encoder first"); return RIV_E_ARG; }
+    const rivk::SearchShape& sh = T->cands[T->last].sh;
+    out[0] = sh.waves; out[1] = sh.groups; out[2] = sh.nslot; out[3] = (int32_t)T->cands.size(); out[4] = T->best >= 0 ? 1 : 0;
+    out[5] = T->best >= 0 ? T->best : -1;
+    return RIV_OK;
 }
 
 int riv_encoder_flush(riv_encoder* e) {

@@ -1654,6 +1654,36 @@ bool search_shape(int sr, int nb, int nstreams, int cus, size_t budget, SearchSh
     out->lds = search_lds_bytes(sr, waves, nstreams, out->nslot);
     return true;
 }
+std::vector<SearchShape> search_shapes(int sr, int nb, int nstreams, int cus, size_t budget) {
+    // the occupancy-derived shape first, then every other feasible (waves, groups) of a small grid,
+    // each with as many window slots per wave as fit
+    std::vector<SearchShape> out;
+    SearchShape d;
+    if (search_shape(sr, nb, nstreams, cus, budget, &d)) out.push_back(d);
+    const int wv[] = {16, 14, 12, 10, 8, 6, 4};
+    const int gq[] = {4, 3, 2};   // groups = cus * gq / 4
+    for (int waves : wv) {
+        if (search_lds_bytes(sr, waves, nstreams, 1) > budget) continue;
+        for (int q : gq) {
+            const int groups = std::max(1, std::min(cus * q / 4, (nb + waves - 1) / waves));
+            const long long nw = (long long)groups * waves;
+            if (nw * 64 < nb) continue;
+            const int owned = (int)((nb + nw - 1) / nw);
+            const size_t per = (size_t)waves * riv::search_slot_dwords(sr) * sizeof(uint32_t);
+            const int fit = (int)std::min<size_t>(64, (budget - riv::search_slots_offset(sr, nstreams)) / per);
+            if (fit < 1) continue;
+            SearchShape sh;
+            sh.waves = waves; sh.groups = groups;
+            sh.nslot = std::min(owned, fit);
+            sh.nres = owned <= fit ? owned : fit - 1;
+            sh.lds = search_lds_bytes(sr, waves, nstreams, sh.nslot);
+            bool dup = false;
+            for (const SearchShape& o : out) dup |= o.waves == sh.waves && o.groups == sh.groups;
+            if (!dup) out.push_back(sh);
+        }
+    }
+    return out;
+}
 hipError_t me_search(hipStream_t st, const SearchParams& p, int groups, int waves) {
     hipLaunchKernelGGL(k_me_search, dim3(groups), dim3(64 * waves), search_lds_bytes(p.sr, waves, p.nstreams, p.nslot), st, p);
     return hipGetLastError();

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstddef>
+#include <vector>
 #include "riv_common.hpp"
 
 namespace rivk {
@@ -21,6 +22,9 @@ size_t search_lds_bytes(int sr, int waves, int nstreams, int nslot);
 // False when the blocks do not fit in resident waves or the LDS budget.
 struct SearchShape { int waves, groups, nslot, nres; size_t lds; };
 bool search_shape(int sr, int nb, int nstreams, int cus, size_t budget, SearchShape* out);
+// The launch shapes an autotuner may try for the same search (search_shape's first); every one
+// reaches the same fixed point, so the choice changes only the time.
+std::vector<SearchShape> search_shapes(int sr, int nb, int nstreams, int cus, size_t budget);
 hipError_t me_search(hipStream_t st, const riv::SearchParams& p, int groups, int waves);
 void mc(hipStream_t st, const uint8_t* rY, const uint8_t* rU, const uint8_t* rV, int W, int H, const int8_t* mvs, int mv_stride,
         uint8_t* pY, uint8_t* pU, uint8_t* pV, uint32_t* var_sums, float* lnv);

@@ -85,3 +85,33 @@ def test_batch_rejects_mismatched_encoders():
         riv2_amd.encode_batch_step([a, a], [f, f])
     a.close()
     b.close()
+
+
+def test_autotuner_settles_without_changing_any_stream():
+    """The batch step's launch-shape autotuner tries every candidate shape on the first steps of a
+    batch (three times each), keeps the fastest, and no stream's bytes change on the way."""
+    streams = [(64, 48, 1000, 31), (200, 120, 1000, 32), (96, 64, 1000, 33)]
+    nframes = 30
+    seqs = [synth.Sequence(synth.custom_params(w, h, nframes, seed=seed, global_motion=5.0, n_rects=2, rect_min=12,
+                                               rect_max=40, rect_velocity=7.0)).frames() for w, h, _, seed in streams]
+    encs = [riv2_amd.Encoder(_cfg(w, h, kf, 0)) for w, h, kf, _ in streams]
+    seen = set()
+    settled_at = None
+    for i in range(nframes):
+        riv2_amd.encode_batch_step(encs, [s[i] for s in seqs], [i * 33] * len(encs))
+        if i > 0:
+            t = encs[0].batch_tuning()
+            seen.add((t["waves"], t["groups"]))
+            if t["settled"] and settled_at is None:
+                settled_at = i
+                print(f"[autotune] settled at step {i}: {t}", flush=True)
+    t = encs[0].batch_tuning()
+    assert t["settled"] and 0 <= t["chosen"] < t["candidates"] and t["candidates"] > 1
+    assert len(seen) == t["candidates"], "every candidate shape was launched"
+    outs = []
+    for e in encs:
+        e.finish()
+        outs.append(e.output())
+        e.close()
+    for k, ((w, h, kf, _), seq, got) in enumerate(zip(streams, seqs, outs)):
+        assert got == _oracle(seq, w, h, kf), f"stream {k} ({w}x{h}) differs from the oracle"
