@@ -22,6 +22,8 @@ import platform
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -235,6 +237,104 @@ def run_batched(per_size, p_frames, depth):
             "search_shape": tuning}
 
 
+def run_stress(per_size, p_frames, depth):
+    """The learned + batched stress workload: per_size streams of each of BATCH_SIZES, each 1 I +
+    p_frames P-frames of high-motion content (riv2_amd.synth.stress_params: global motion up to
+    24 px, rectangles up to 32 px per frame that zoom and rotate, a scene cut every 25 frames,
+    noise sigma 4), encoded by batch steps twice, plain and learned, frames resident in HBM. The
+    P steps after the autotuner has settled are timed. Reports both forms' fps, the learned form's
+    fallback rate, bytes and PSNR-Y against plain, and the decode check of one stream per size
+    (GPU decoder against the oracle decoder)."""
+    import torch
+    import riv2_amd
+    import oracle
+    from riv2_amd import synth, learned
+
+    dev = torch.device("cuda:0")
+    streams, host = [], []
+    for k, (w, h) in enumerate(BATCH_SIZES):
+        for j in range(per_size):
+            host.append(synth.Sequence(synth.stress_params(w, h, 1 + p_frames, seed=5000 + 100 * k + j)).frames(0, 1 + p_frames))
+            streams.append((w, h))
+    frames = [torch.from_numpy(f).to(dev) for f in host]
+    torch.cuda.synchronize()
+    net = learned.MvNet.load(learned.WEIGHTS_PATH)
+
+    def encode(with_net):
+        encs = [riv2_amd.Encoder(riv2_amd.EncoderConfig(w, h, 30).with_intra_quality(85).with_inter_quality(80).with_search_range(12)
+                                 .with_skip_threshold(3).with_keyframe_interval(riv2_amd.KeyframeInterval.fixed(1000))
+                                 .with_pipeline_depth(depth)) for w, h in streams]
+        if with_net:
+            for e in encs:
+                e.set_mvnet(net, 0.5)
+        step = lambda i: riv2_amd.encode_batch_step(encs, [f[i].data_ptr() for f in frames], [i * 33] * len(encs), on_device=True)
+        step(0)
+        first = 1
+        while first < p_frames - 2:
+            step(first)
+            first += 1
+            for e in encs:
+                e.flush()
+            if encs[0].batch_tuning()["settled"]:
+                break
+        for e in encs:
+            e.flush()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(first, 1 + p_frames):
+            step(i)
+        for e in encs:
+            e.flush()
+        elapsed = time.perf_counter() - t0
+        counts = encs[0].learned_stats()
+        outs = []
+        for e in encs:
+            e.finish()
+            outs.append(e.output())
+            e.close()
+        return (1 + p_frames - first) * len(encs) / elapsed, outs, counts
+
+    print("[bench] stress: plain", file=sys.stderr, flush=True)
+    plain_fps, plain_out, _ = encode(False)
+    print("[bench] stress: learned", file=sys.stderr, flush=True)
+    learned_fps, learned_out, counts = encode(True)
+    net.close()
+
+    def psnr(outs):
+        vals = []
+        for f, data in zip(host, outs):
+            dec = riv2_amd.Decoder(data)
+            for k in range(len(f)):
+                d = dec.decode_frame().data
+                if k > 0:
+                    vals.append(learned.psnr_y(f[k], d))
+            dec.close()
+        return sum(vals) / len(vals)
+
+    print("[bench] stress: PSNR-Y of every decoded P-frame", file=sys.stderr, flush=True)
+    p_plain, p_learned = psnr(plain_out), psnr(learned_out)
+    exact = True
+    for k in range(len(BATCH_SIZES)):   # one learned stream of each size through both decoders
+        data = learned_out[k * per_size]
+        dec, rdec = riv2_amd.Decoder(data), oracle.OracleDecoder(data)
+        for _ in range(1 + p_frames):
+            a = dec.decode_frame().data
+            b, _ = rdec.decode()
+            exact &= bool(np.array_equal(a, b))
+        dec.close()
+    pb, lb = sum(len(o) for o in plain_out), sum(len(o) for o in learned_out)
+    return {"metric": "learned + batched stress: P-frame encode fps/GPU (64 mixed-resolution streams, high motion)",
+            "value": round(learned_fps, 3), "unit": "fps", "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+            "config": {"workload": f"{len(streams)} streams ({per_size} each of " + ", ".join(f"{w}x{h}" for w, h in BATCH_SIZES) +
+                                   f"), 1 I + {p_frames} P each, high motion (global 24 px, rectangles 32 px/frame, zoom 0.5%, "
+                                   "rotation 1 deg per frame, scene cut every 25 frames, sigma 4), q85/80, sr 12", "pipeline_depth": depth},
+            "plain_fps": round(plain_fps, 3), "learned_vs_plain_fps": round(learned_fps / plain_fps, 4),
+            "fallback_rate": round(1.0 - counts[2] / max(1, counts[0]), 4), "confident_rate": round(counts[1] / max(1, counts[0]), 4),
+            "bytes_vs_plain": round(lb / pb, 5), "psnr_y": round(p_learned, 4), "plain_psnr_y": round(p_plain, 4),
+            "decode_bit_exact_vs_oracle_decoder": exact, "decode_checked_streams": len(BATCH_SIZES)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -249,6 +349,8 @@ def main():
     ap.add_argument("--batched", action="store_true", help="measure the batched 64-stream workload instead")
     ap.add_argument("--batch-per-size", type=int, default=16)
     ap.add_argument("--batch-p-frames", type=int, default=24)
+    ap.add_argument("--stress", action="store_true", help="with --batched: the learned + batched high-motion workload")
+    ap.add_argument("--stress-p-frames", type=int, default=49)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 256 MiB device copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -257,6 +359,10 @@ def main():
               "(multi-GPU sharding is described in DESIGN.md, not measured).", file=sys.stderr)
         sys.exit(2)
 
+    if args.batched and args.stress:
+        print(f"[bench] stress: {4 * args.batch_per_size} streams, 1 I + {args.stress_p_frames} P each", file=sys.stderr, flush=True)
+        print(json.dumps(run_stress(args.batch_per_size, args.stress_p_frames, max(2, min(args.pipeline_depth, 4)))))
+        return
     if args.batched:
         print(f"[bench] batched: {4 * args.batch_per_size} streams, 1 I + {args.batch_p_frames} P each", file=sys.stderr, flush=True)
         print(json.dumps(run_batched(args.batch_per_size, args.batch_p_frames, max(2, min(args.pipeline_depth, 4)))))

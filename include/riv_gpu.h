@@ -275,10 +275,11 @@ int riv_mvnet_propose(riv_mvnet* net, const int32_t* samples, size_t n, float co
 /* Switches the encoder to the learned form with `net` (NULL: back to the plain form) from its
  * next P-frame. confidence in (0, 1] is the probability threshold (1: never confident, so every
  * block takes the plain search). The network must outlive its use and keep its parameters while
- * frames are in flight. Single-frame calls only (batch steps search plainly). */
+ * frames are in flight. In a batch step each stream follows its own encoder's setting. */
 int riv_encoder_set_mvnet(riv_encoder* enc, riv_mvnet* net, float confidence);
 /* Learned-form counts since the encoder was created: counts[0] P-frame blocks searched, [1] those
- * with a confident proposal, [2] those that took the proposal (the rest fell back). */
+ * with a confident proposal, [2] those that took the proposal (the rest fell back). For the first
+ * encoder of batch steps with learned streams, the counts also cover every stream of those steps. */
 int riv_encoder_learned_stats(riv_encoder* enc, uint64_t* counts);
 
 /* Last error message of the calling thread (static storage). */

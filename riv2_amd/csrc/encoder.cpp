@@ -196,6 +196,8 @@ struct riv_encoder {
     HostBuf b_stage_buf[2];
     hipEvent_t b_stage_ev[2] = {nullptr, nullptr};
     bool b_autotune = true;
+    DevBuf b_prop, b_lstat;   // learned form in batch steps: proposals of every block, per-wave counters
+    size_t b_prop_n = 0;
     std::map<std::tuple<int, int, int>, ShapeTuner> b_tuners;   // by (blocks, streams, search range)
     ShapeTuner* b_last_tuner = nullptr;
 
@@ -970,6 +972,35 @@ static int batch_step(riv_encoder* const* encs, size_t n, const void* const* rgb
         RIV_HIP(hipEventRecord(e0->b_stage_ev[k], st));
         p.streams = e0->b_desc.as<riv::SearchStream>();
         p.field = e0->b_field.as<uint32_t>(); p.arrive = e0->b_arrive.as<unsigned>(); p.bank = e0->b_bank;
+        // learned form: each learned stream's proposals at its first block; the other streams' zero
+        bool any_net = false;
+        for (size_t i = 0; i < n; i++) any_net |= !f[i].intra && encs[i]->net;
+        if (any_net) {
+            if ((size_t)total > e0->b_prop_n) {
+                RIV_HIP(hipStreamSynchronize(st));
+                RIV_HIP(e0->b_prop.alloc(16 * (size_t)total));
+                e0->b_prop_n = total;
+            }
+            if (!e0->b_lstat.p) {   // three counters for each wave any launch shape can have
+                const size_t lb = 3 * sizeof(unsigned long long) * riv::kMaxSearchWaves;
+                RIV_HIP(e0->b_lstat.alloc(lb)); RIV_HIP(hipMemset(e0->b_lstat.p, 0, lb));
+            }
+            uint4* bp = e0->b_prop.as<uint4>();
+            for (size_t i = 0, d = 0; i < n; i++) {
+                if (f[i].intra) continue;
+                riv_encoder* e = encs[i];
+                const riv::SearchStream& S = desc[d++];
+                if (e->net) {
+                    const rivnet::Source src{S.cur, S.ref, f[i].prev_mv, (int)sizeof(BlockRec), e->W, e->H, e->bw, e->bh};
+                    RIV_HIP(rivnet::infer(st, nullptr, src, nullptr, e->nb, rivnet::net_weights(e->net), e->net_logit, p.sr, bp + S.blk0, nullptr));
+                } else {
+                    RIV_HIP(hipMemsetAsync(bp + S.blk0, 0, 16 * (size_t)e->nb, st));
+                }
+            }
+            p.prop = bp;
+            // counters only while the launch fits the counter array (every MI355X shape does)
+            p.lstat = sh.groups * sh.waves <= riv::kMaxSearchWaves ? e0->b_lstat.as<unsigned long long>() : nullptr;
+        }
         e0->b_bank ^= 1;
         hipEvent_t ta = ci >= 0 ? T.event() : nullptr, tb = ci >= 0 ? T.event() : nullptr;
         if (ta && tb) RIV_HIP(hipEventRecord(ta, st));
@@ -1088,11 +1119,14 @@ int riv_encoder_learned_stats(riv_encoder* e, uint64_t* counts) {
     if (!e || !counts) return RIV_E_ARG;
     RIV_HIP(hipSetDevice(e->cfg.device_id));
     RIV_HIP(hipStreamSynchronize(e->st));
-    std::vector<unsigned long long> h(e->lstat.n / sizeof(unsigned long long));
-    RIV_HIP(hipMemcpy(h.data(), e->lstat.p, e->lstat.n, hipMemcpyDeviceToHost));
     for (int i = 0; i < 3; i++) counts[i] = 0;
-    for (size_t w = 0; w + 2 < h.size(); w += 3)
-        for (int i = 0; i < 3; i++) counts[i] += h[w + i];
+    for (const DevBuf* b : {&e->lstat, &e->b_lstat}) {
+        if (!b->p) continue;
+        std::vector<unsigned long long> h(b->n / sizeof(unsigned long long));
+        RIV_HIP(hipMemcpy(h.data(), b->p, b->n, hipMemcpyDeviceToHost));
+        for (size_t w = 0; w + 2 < h.size(); w += 3)
+            for (int i = 0; i < 3; i++) counts[i] += h[w + i];
+    }
     return RIV_OK;
 }
 

@@ -72,6 +72,9 @@ __host__ __device__ inline size_t search_slots_offset(int sr, int nstreams) {
 
 // Parameters of the motion search by relaxation passes (one persistent launch per frame step).
 // Every stream of a launch shares the search range, lambda and thresholds.
+// Waves a motion-search launch can have at most (workgroups <= 256 CUs, 16 waves each).
+constexpr int kMaxSearchWaves = 256 * 16;
+
 struct SearchParams {
     SearchStream s0;             // the stream of a single-stream launch (streams == nullptr)
     const SearchStream* streams; // device array of nstreams streams, ascending blk0, or nullptr

@@ -5,7 +5,9 @@
 //  * chroma: smoother periodic fields (1/f^4 power), scaled to [96, 160];
 //  * global translation per frame drawn uniformly in [-M, M] pixels at half-pel steps;
 //  * K textured rectangles (own 1/f^2 texture, own flat chroma) moving with an integer
-//    velocity drawn in [-V, V] px/frame, reflected at the frame border;
+//    velocity drawn in [-V, V] px/frame, reflected at the frame border; optionally each zooms
+//    and rotates about its centre at its own rate (drawn in [-Z, Z] per frame, [-R, R] degrees
+//    per frame);
 //  * optional hard scene cut every C frames (new background phase and chroma);
 //  * additive Gaussian noise (sigma) on Y, U and V, rounded and clipped;
 //  * RGB24 from the YUV planes through the codec's BT.709 limited-range inverse.
@@ -113,13 +115,15 @@ typedef struct {
     float rect_velocity;        // V: per-rectangle velocity drawn in [-V, V] px/frame (integer)
     float noise_sigma;
     uint32_t scene_cut_interval;   // 0 = none
+    float rect_zoom;               // Z: per-rectangle zoom rate drawn in [-Z, Z] per frame (0.005 = 0.5%)
+    float rect_rotate_deg;         // R: per-rectangle rotation rate drawn in [-R, R] degrees per frame
 } riv_synth_params;
 
 struct riv_synth {
     riv_synth_params p;
     int nbg, nch, nrt;
     std::vector<float> bg, cu, cv;
-    struct Rect { int w, h; std::vector<float> tex; uint8_t u, v; std::vector<int> px, py; };
+    struct Rect { int w, h; std::vector<float> tex; uint8_t u, v; std::vector<int> px, py; double zoom = 0, rot = 0; };
     std::vector<Rect> rects;
     std::vector<int> gx, gy;        // cumulative global offset, half-pel units
     std::vector<int> cut_dx, cut_dy; // background phase after scene cuts (pixels)
@@ -156,6 +160,9 @@ int riv_synth_create(const riv_synth_params* p, riv_synth** out) {
         r.u = (uint8_t)rng.range(64, 192); r.v = (uint8_t)rng.range(64, 192);
         int x = rng.range(0, W - r.w), y = rng.range(0, H - r.h);
         int vx = rng.range(-V, V), vy = rng.range(-V, V);
+        // drawn only when enabled, so sequences without zoom or rotation keep their random stream
+        if (p->rect_zoom > 0) r.zoom = (2.0 * rng.uniform() - 1.0) * p->rect_zoom;
+        if (p->rect_rotate_deg > 0) r.rot = (2.0 * rng.uniform() - 1.0) * p->rect_rotate_deg * M_PI / 180.0;
         r.px.resize(F); r.py.resize(F);
         for (int f = 0; f < F; f++) {
             r.px[f] = x; r.py[f] = y;
@@ -189,6 +196,31 @@ static void render(const riv_synth* s, int f, uint8_t* rgb) {
         }
     for (const auto& r : s->rects) {
         int px = r.px[f], py = r.py[f];
+        if (r.zoom != 0 || r.rot != 0) {
+            // the rectangle scaled by (1+zoom)^f and turned by rot*f about its centre; a pixel
+            // belongs to it when its inverse-mapped position lies inside the w x h texture window
+            const double sc = std::pow(1.0 + r.zoom, f), a = r.rot * f, ca = std::cos(a), sa = std::sin(a);
+            const double cx = px + 0.5 * r.w, cy = py + 0.5 * r.h, rad = 0.5 * sc * std::sqrt((double)r.w * r.w + (double)r.h * r.h);
+            auto inv = [&](double x, double y, double* tx, double* ty) {
+                const double u = x - cx, v = y - cy;
+                *tx = (ca * u + sa * v) / sc + 0.5 * r.w;
+                *ty = (-sa * u + ca * v) / sc + 0.5 * r.h;
+                return *tx >= 0 && *tx < r.w && *ty >= 0 && *ty < r.h;
+            };
+            const int x0 = std::max(0, (int)std::floor(cx - rad)), x1 = std::min(W - 1, (int)std::ceil(cx + rad));
+            const int y0 = std::max(0, (int)std::floor(cy - rad)), y1 = std::min(H - 1, (int)std::ceil(cy + rad));
+            for (int y = y0; y <= y1; y++)
+                for (int x = x0; x <= x1; x++) {
+                    double tx, ty;
+                    if (inv(x + 0.5, y + 0.5, &tx, &ty)) yf[(size_t)y * W + x] = sample_wrap(r.tex, s->nrt, tx, ty);
+                }
+            for (int y = y0 / 2; y <= y1 / 2 && y < ch; y++)
+                for (int x = x0 / 2; x <= x1 / 2 && x < cw; x++) {
+                    double tx, ty;
+                    if (inv(2 * x + 1.0, 2 * y + 1.0, &tx, &ty)) { uf[(size_t)y * cw + x] = r.u; vf[(size_t)y * cw + x] = r.v; }
+                }
+            continue;
+        }
         for (int y = py; y < py + r.h; y++)
             for (int x = px; x < px + r.w; x++) yf[(size_t)y * W + x] = r.tex[(size_t)((y - py) % s->nrt) * s->nrt + (x - px) % s->nrt];
         for (int y = (py + 1) / 2; y < (py + r.h) / 2; y++)

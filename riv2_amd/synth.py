@@ -16,7 +16,8 @@ from ._lib import SYNTH_PATH
 class SynthParams(Structure):
     _fields_ = [("width", c_uint32), ("height", c_uint32), ("frames", c_uint32), ("seed", c_uint64),
                 ("global_motion", c_float), ("n_rects", c_uint32), ("rect_min", c_uint32), ("rect_max", c_uint32),
-                ("rect_velocity", c_float), ("noise_sigma", c_float), ("scene_cut_interval", c_uint32)]
+                ("rect_velocity", c_float), ("noise_sigma", c_float), ("scene_cut_interval", c_uint32),
+                ("rect_zoom", c_float), ("rect_rotate_deg", c_float)]
 
 
 _synth = None
@@ -47,9 +48,17 @@ def hd_params(seed: int = 1, frames: int = 50) -> SynthParams:
 
 
 def custom_params(width, height, frames, seed=1, global_motion=3.0, n_rects=4, rect_min=32, rect_max=96,
-                  rect_velocity=6.0, noise_sigma=2.0, scene_cut_interval=0) -> SynthParams:
+                  rect_velocity=6.0, noise_sigma=2.0, scene_cut_interval=0, rect_zoom=0.0, rect_rotate_deg=0.0) -> SynthParams:
     return SynthParams(width, height, frames, seed, global_motion, n_rects, rect_min, rect_max, rect_velocity,
-                       noise_sigma, scene_cut_interval)
+                       noise_sigma, scene_cut_interval, rect_zoom, rect_rotate_deg)
+
+
+def stress_params(width, height, frames, seed=1) -> SynthParams:
+    """High-motion stress content: global motion up to 24 px per frame, 4 rectangles of 32-96 px
+    moving up to 32 px per frame, each zooming up to 0.5% and rotating up to 1 degree per frame, a
+    hard scene cut every 25 frames, noise sigma 4."""
+    return custom_params(width, height, frames, seed, global_motion=24.0, n_rects=4, rect_min=32, rect_max=96,
+                         rect_velocity=32.0, noise_sigma=4.0, scene_cut_interval=25, rect_zoom=0.005, rect_rotate_deg=1.0)
 
 
 class Sequence:

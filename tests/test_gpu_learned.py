@@ -273,3 +273,36 @@ def test_learned_stream_decodes_identically(depth):
     # the learned form's vectors differ from the plain form's, its bytes decode all the same
     plain, _, _ = encode(fr)
     assert data != plain
+
+
+def test_learned_streams_in_a_batch_equal_their_solo_encodes():
+    """A batch step with learned and plain streams: each learned stream's bytes equal the same
+    stream encoded alone in the learned form, the plain stream's equal the oracle's."""
+    print("[learned] batch", flush=True)
+    net = make_net(23)
+    specs = [(160, 96, 41, True), (96, 64, 42, False), (200, 120, 43, True)]
+    seqs = [synth.Sequence(synth.stress_params(w, h, 6, seed=s)).frames() for w, h, s, _ in specs]
+    encs = []
+    for w, h, _, use in specs:
+        e = riv2_amd.Encoder(learned.hd_config(80, width=w, height=h))
+        if use:
+            e.set_mvnet(net, 1e-6)
+        encs.append(e)
+    for i in range(6):
+        riv2_amd.encode_batch_step(encs, [s[i] for s in seqs], [33 * i] * len(encs))
+    counts = encs[0].learned_stats()
+    outs = []
+    for e in encs:
+        e.finish()
+        outs.append(e.output())
+        e.close()
+    assert counts[0] == 5 * sum(((w + 15) // 16) * ((h + 15) // 16) for w, h, _, _ in specs) and counts[2] > 0
+    for (w, h, s, use), seq, got in zip(specs, seqs, outs):
+        if use:
+            solo, _, _ = encode(seq, net, 1e-6)
+            assert got == solo, f"{w}x{h}: batch learned stream differs from its solo encode"
+        else:
+            ref = oracle.OracleEncoder(oracle.config(w, h))
+            for i, f in enumerate(seq):
+                ref.encode(f, 33 * i)
+            assert got == ref.finish(), f"{w}x{h}: plain stream in a learned batch differs from the oracle"
