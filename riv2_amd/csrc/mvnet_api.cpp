@@ -54,6 +54,10 @@ struct riv_mvnet {
     int bcap = 0;
     DevBuf samp, tgt, x0, c1, a1, c2, a2, c3, a3, a4, z1, z2, z3, z4, y, dy, dyb, da4, dz4, da3, dz3, dc3, dz2, dc2, dz1, part, lossb, prop;
     size_t part_n = 0;
+    // pinned staging of a batch's samples and targets: the caller's arrays may go away as soon as
+    // a step returns, and the previous step's upload must have left the buffer before it is refilled
+    HostBuf stage;
+    hipEvent_t staged = nullptr;
 
     rivnet::Weights weights() const {
         const uint16_t* w = p16.as<uint16_t>();
@@ -98,6 +102,9 @@ int ensure_batch(riv_mvnet* n, int B) {
     RIV_HIP(n->dc3.alloc(4 * b * 16 * K2)); RIV_HIP(n->dz2.alloc(2 * b * 64 * F));
     RIV_HIP(n->dc2.alloc(4 * b * 64 * K2)); RIV_HIP(n->dz1.alloc(2 * b * 256 * F));
     RIV_HIP(n->lossb.alloc(4 * b)); RIV_HIP(n->prop.alloc(16 * b));
+    if (n->staged) RIV_HIP(hipEventSynchronize(n->staged));
+    RIV_HIP(n->stage.alloc(10 * b));
+    if (!n->staged) RIV_HIP(hipEventCreateWithFlags(&n->staged, hipEventDisableTiming));
     // the largest split partials: conv1's weight gradient (K1 x F over B*256 rows) or fc1's
     const size_t splits1 = (b * 256 + kSplitRows - 1) / kSplitRows;
     n->part_n = std::max(splits1 * K1 * F, ((b + kSplitRows - 1) / kSplitRows) * (size_t)FC_IN * FC_H);
@@ -201,8 +208,13 @@ int stage_batch(riv_mvnet* n, const int32_t* samples, const int8_t* targets, siz
     }
     int r = ensure_batch(n, (int)count);
     if (r) return r;
-    RIV_HIP(hipMemcpyAsync(n->samp.p, samples, 8 * count, hipMemcpyHostToDevice, n->st));
-    if (targets) RIV_HIP(hipMemcpyAsync(n->tgt.p, targets, 2 * count, hipMemcpyHostToDevice, n->st));
+    RIV_HIP(hipEventSynchronize(n->staged));   // the previous upload has read the staging buffer
+    uint8_t* h = n->stage.as<uint8_t>();
+    std::memcpy(h, samples, 8 * count);
+    if (targets) std::memcpy(h + 8 * count, targets, 2 * count);
+    RIV_HIP(hipMemcpyAsync(n->samp.p, h, 8 * count, hipMemcpyHostToDevice, n->st));
+    if (targets) RIV_HIP(hipMemcpyAsync(n->tgt.p, h + 8 * count, 2 * count, hipMemcpyHostToDevice, n->st));
+    RIV_HIP(hipEventRecord(n->staged, n->st));
     return RIV_OK;
 }
 
@@ -272,6 +284,7 @@ void riv_mvnet_destroy(riv_mvnet* n) {
     if (!n) return;
     (void)hipSetDevice(n->dev);
     if (n->st) { (void)hipStreamSynchronize(n->st); (void)hipStreamDestroy(n->st); }
+    if (n->staged) (void)hipEventDestroy(n->staged);
     delete n;
 }
 
