@@ -989,7 +989,7 @@ __global__ __launch_bounds__(256) void k_mc(const uint8_t* __restrict__ rY, cons
 // binade e is adding K to its bit pattern, the exponent field unchanged.
 // Runs of 32 values (sub-chunks) get K for the binade their (f64) prefix sum predicts and the
 // adjacent one, both parities, by running them in f32 from 2^e (+u); runs of 8 sub-chunks
-// (chunks) compose their sub-chunks' K the same way in integers. One lane then chains the chunks,
+// (chunks) compose their sub-chunks' K the same way in integers. One wave then chains the chunks,
 // adding K to the sum's bits where the binade and parity match and the exponent field stays; a
 // chunk that does not (it crosses a binade, or is among the first) is chained by its sub-chunks,
 // and a sub-chunk that does not by its values one by one, in f32. The result is the sequential
@@ -1111,9 +1111,10 @@ __global__ __launch_bounds__(1024) void k_aq(float* __restrict__ lnv, int nb, fl
             if (which == 0) cbin[k] = eb;
         }
         __syncthreads();
-        if (t == 0) {
-            // one lane; every LDS read is issued ahead of the step that needs it (the next chunk's
-            // K, a chunk's 8 sub-chunks together, a sub-chunk's 32 values together)
+        if (t < 64) {
+            // one wave, every lane holding the same sum: lanes fetch together what the chain reads
+            // in turn (a chunk's 8 sub-chunk K, a sub-chunk's 32 values, one LDS instruction each)
+            // and the chain takes them by lane reads; the next chunk's K is read ahead
             uint32_t b = 0u;   // the sum's bits
             int4 nk = ck[0];
             int2 ne = cbin[0];
@@ -1124,30 +1125,27 @@ __global__ __launch_bounds__(1024) void k_aq(float* __restrict__ lnv, int nb, fl
                 const uint32_t nx = b + (uint32_t)aq_pick(b, kk, ee);
                 if ((nx >> 23) == (b >> 23)) { b = nx; continue; }
                 const int c0 = k * kAqGroup, nsc = min(kAqGroup, nsub - c0);
-                int4 sks[kAqGroup];
-                int2 sbs[kAqGroup];
-#pragma unroll
-                for (int j = 0; j < kAqGroup; j++) {
-                    sks[j] = j < nsc ? sk[c0 + j] : make_int4(0, 0, 0, 0);
-                    sbs[j] = j < nsc ? sbin[c0 + j] : make_int2(-9, -9);
-                }
-#pragma unroll
-                for (int j = 0; j < kAqGroup; j++) {
-                    if (j >= nsc) continue;
-                    const uint32_t ns = b + (uint32_t)aq_pick(b, sks[j], sbs[j]);
+                const int4 skl = lane < nsc ? sk[c0 + lane] : make_int4(0, 0, 0, 0);
+                const int2 sbl = lane < nsc ? sbin[c0 + lane] : make_int2(-9, -9);
+                for (int j = 0; j < nsc; j++) {
+                    const int4 kj = make_int4(__builtin_amdgcn_readlane(skl.x, j), __builtin_amdgcn_readlane(skl.y, j),
+                                              __builtin_amdgcn_readlane(skl.z, j), __builtin_amdgcn_readlane(skl.w, j));
+                    const int2 ej = make_int2(__builtin_amdgcn_readlane(sbl.x, j), __builtin_amdgcn_readlane(sbl.y, j));
+                    const uint32_t ns = b + (uint32_t)aq_pick(b, kj, ej);
                     if ((ns >> 23) == (b >> 23)) { b = ns; continue; }
                     const int lo = (c0 + j) * kAqSub;
-                    float v[kAqSub];   // past nb: +0, which leaves a positive sum as it is
-#pragma unroll
-                    for (int i = 0; i < kAqSub; i++) v[i] = lo + i < nb ? xs[aq_pos(lo + i)] : 0.0f;
+                    // past nb: +0, which leaves a positive sum as it is
+                    const float vl = (lane < kAqSub && lo + lane < nb) ? xs[aq_pos(lo + lane)] : 0.0f;
                     float sf = __uint_as_float(b);
 #pragma unroll
-                    for (int i = 0; i < kAqSub; i++) sf = __fadd_rn(sf, v[i]);
+                    for (int i = 0; i < kAqSub; i++) sf = __fadd_rn(sf, __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)fbits(vl), i)));
                     b = fbits(sf);
                 }
             }
-            *s_mean = __fdiv_rn(__uint_as_float(b), (float)nb);
-            lnv[nb] = *s_mean;
+            if (t == 0) {
+                *s_mean = __fdiv_rn(__uint_as_float(b), (float)nb);
+                lnv[nb] = *s_mean;
+            }
         }
         __syncthreads();
     }
