@@ -1149,10 +1149,12 @@ __global__ __launch_bounds__(1024) void k_aq(float* __restrict__ lnv, int nb, fl
         }
         __syncthreads();
     }
-    // per-MB steps: base * exp2(0.8 * (ln v - mean) / 6) in f32 (residual.rs:150-166)
+    // per-MB steps: base * exp2(0.8 * (ln v - mean) / 6) in f32 (residual.rs:150-166), the logs
+    // from LDS where they are staged
     const float mean = *s_mean;
+    const bool staged = nb <= kAqStage;
     for (int b = t; b < nb; b += nt) {
-        const float off = __fmul_rn(0.8f, __fsub_rn(lnv[b], mean));
+        const float off = __fmul_rn(0.8f, __fsub_rn(staged ? xs[aq_pos(b)] : lnv[b], mean));
         steps[b] = __fmul_rn(base, glibc_exp2f(__fdiv_rn(off, 6.0f)));
     }
 }
