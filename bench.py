@@ -110,6 +110,31 @@ def mean_pframe_psnr_y(frames, stream):
     return sum(ps) / len(ps)
 
 
+def run_decode(stream, nframes, cpu_sample):
+    """Decoder throughput on the plain bench stream: the GPU decoder (host MV and residual rANS
+    decode feeding the GPU reconstruction, RGB24 back in host memory) over every frame, and the
+    oracle's CPU decoder (one thread) over the first `cpu_sample` frames. Frames per second of
+    decode_frame calls, as ri-quality times decoding (reitero_video_quality_test/src/main.rs:1417-1426)."""
+    import riv2_amd
+    import oracle
+    dec = riv2_amd.Decoder(stream)
+    dec.decode_frame()   # the I-frame, untimed
+    t0 = time.perf_counter()
+    for _ in range(1, nframes):
+        dec.decode_frame()
+    gpu = (nframes - 1) / (time.perf_counter() - t0)
+    dec.close()
+    rdec = oracle.OracleDecoder(stream)
+    rdec.decode()
+    t0 = time.perf_counter()
+    for _ in range(cpu_sample):
+        rdec.decode()
+    cpu = cpu_sample / (time.perf_counter() - t0)
+    return {"value": round(gpu, 3), "unit": "fps", "frames": nframes - 1,
+            "cpu_baseline": {"value": round(cpu, 3), "unit": "fps", "cores": 1, "kind": "port",
+                             "sample": f"the first {cpu_sample} P-frames of the same stream, oracle C++ restatement of the reference decoder"}}
+
+
 def run_learned(frames, d_frames, steps, warmup, depth, plain_stream, plain_fps):
     """The learned form on the same frames, timed like the plain form, with the trained network
     committed beside the package (riv2_amd/mvnet_weights.npz, made by riv2_amd/train_mvnet.py)."""
@@ -346,6 +371,7 @@ def main():
                     help="frames in flight (GPU analysis of frame k+1 overlaps host entropy coding of earlier frames)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned form's line")
+    ap.add_argument("--cpu-decode-frames", type=int, default=6, help="P-frames the oracle decoder is timed on")
     ap.add_argument("--batched", action="store_true", help="measure the batched 64-stream workload instead")
     ap.add_argument("--batch-per-size", type=int, default=16)
     ap.add_argument("--batch-p-frames", type=int, default=24)
@@ -411,6 +437,8 @@ def main():
 
     print("[bench] oracle prefix check", file=sys.stderr, flush=True)
     bit_exact = check_prefix_bit_exact(frames, stream, 3)
+    print("[bench] decode", file=sys.stderr, flush=True)
+    decode = run_decode(stream, nframes, args.cpu_decode_frames)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -449,6 +477,7 @@ def main():
         "me_passes_avg": round(sum(s.me_passes for s in stats) / len(stats), 2),
         "bytes_per_pframe": round(sum(s.total_bytes for s in stats) / len(stats), 1),
         "learned": learned_leg,
+        "decode": decode,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(fps / cpu["value"], 2)
