@@ -3,12 +3,46 @@
 // motion compensation, dequantisation, IDCT, reconstruction, deblock and YUV -> RGB on the GPU.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 #include "host_common.hpp"
 #include "riv_kernels.hpp"
 #include "entropy.hpp"
 
 using namespace rivhost;
+
+// One frame's host stage: the parsed record, its vectors and masks (inter), and its quantised
+// coefficients in a pinned buffer, ready for upload.
+struct HostFrame {
+    int status = RIV_OK;
+    std::string err;
+    uint64_t ts = 0;
+    uint8_t ft = 0, q = 0;
+    std::vector<Mv4> mv;
+    std::vector<uint8_t> skip, coded, mask, payload;
+    uint16_t yt[256], ct[64];
+    HostBuf coeffs;   // nb * 384 int16
+};
+
+// Look-ahead of an in-memory stream: one thread parses the records in order and decodes their
+// vectors (the MV contexts and the previous frame's vectors carry from frame to frame); worker
+// threads decode each frame's residuals (fresh contexts per frame) into its slot. decode_frame
+// takes the slots in order. Slot states: 0 free, 1 being filled, 2 ready.
+struct DecodePipe {
+    static constexpr int kSlots = 8;
+    std::thread producer;
+    std::vector<std::thread> workers;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<int> jobs;
+    bool stop = false;
+    HostFrame slots[kSlots];
+    int state[kSlots] = {};
+};
 
 struct riv_decoder {
     riv_reader_vtbl rv{};
@@ -23,12 +57,15 @@ struct riv_decoder {
     int W = 0, H = 0, bw = 0, bh = 0, nb = 0;
     hipStream_t st = nullptr;
     DevBuf ref[2], pred, coeffs, mvs, skip, mask, var, lnv, steps, rgb, yt, ct;
-    HostBuf h_coeffs, h_rgb;
+    HostBuf h_rgb;
     int ref_idx = 0;
     bool have_ref = false, have_prev_mvs = false;
     std::vector<Mv4> prev_mvs;
     rivh::MvCoder mvc;
     uint8_t last_yt_q = 0;
+    bool parsed_intra = false;        // the host stage has seen the first (intra) frame
+    DecodePipe* pipe = nullptr;       // in-memory streams, once decoding starts
+    HostFrame inline_frame;           // reader streams: the one frame in flight
 
     // read_exact (decoder.rs:502-511): a short read is EndOfStream
     bool read_exact(uint8_t* dst, size_t n) {
@@ -74,7 +111,6 @@ static int decoder_init(riv_decoder* d) {
     RIV_HIP(d->var.alloc(8 * (size_t)d->nb)); RIV_HIP(d->lnv.alloc(4 * ((size_t)d->nb + 1))); RIV_HIP(d->steps.alloc(4 * (size_t)d->nb));
     size_t rgbsz = (size_t)d->hdr.display_width * d->hdr.display_height * 3;
     RIV_HIP(d->rgb.alloc(rgbsz)); RIV_HIP(d->h_rgb.alloc(rgbsz));
-    RIV_HIP(d->h_coeffs.alloc(768 * (size_t)d->nb));
     RIV_HIP(d->yt.alloc(512)); RIV_HIP(d->ct.alloc(128));
     return RIV_OK;
 }
@@ -98,65 +134,48 @@ int riv_decoder_create_mem(const uint8_t* data, size_t len, int32_t device_id, r
     return RIV_OK;
 }
 
-// decode_next (decoder.rs:195-459)
-static int decode_next(riv_decoder* d, uint8_t* rgb_out, riv_decoded_info* info) {
+// decode_next (decoder.rs:195-459), in two stages.
+// Host stage, part 1 (frame order): reads the next record, decodes and reconstructs its vectors
+// (decoder.rs:195-382). The residual payload is kept for part 2.
+static int parse_frame(riv_decoder* d, HostFrame& f) {
     uint8_t head[9];
     if (!d->read_exact(head, 9)) { set_error("end of stream"); return RIV_E_END_OF_STREAM; }
-    const uint64_t ts = get_u64(head);
-    const uint8_t ft = head[8];
-    if (ft != 1 && ft != 2) { set_error("Bad frame_type"); return RIV_E_INVALID_FRAME; }
-    RIV_HIP(hipSetDevice(d->dev));
-    const int W = d->W, H = d->H, nb = d->nb, bw = d->bw, bh = d->bh;
-    const size_t ysz = (size_t)W * H, csz = ysz / 4;
-    uint8_t* nY = d->ref[1 - d->ref_idx].as<uint8_t>(); uint8_t* nU = nY + ysz; uint8_t* nV = nU + csz;
-    uint8_t* rY = d->ref[d->ref_idx].as<uint8_t>(); uint8_t* rU = rY + ysz; uint8_t* rV = rU + csz;
-    uint8_t* pY = d->pred.as<uint8_t>(); uint8_t* pU = pY + ysz; uint8_t* pV = pU + csz;
-    hipStream_t st = d->st;
-    int16_t* hco = d->h_coeffs.as<int16_t>();
-    // host staging for this frame's uploads; kept alive until the stream is synchronised
-    std::vector<Mv4> mv;
-    std::vector<uint8_t> skip, coded, mask;
-    uint16_t yt[256], ct[64];
+    f.ts = get_u64(head);
+    f.ft = head[8];
+    if (f.ft != 1 && f.ft != 2) { set_error("Bad frame_type"); return RIV_E_INVALID_FRAME; }
+    const int nb = d->nb, bw = d->bw, bh = d->bh;
     try {
-        if (ft == 1) {
+        if (f.ft == 1) {
             uint8_t qb[5];
             if (!d->read_exact(qb, 5)) return RIV_E_END_OF_STREAM;
-            const uint8_t q = qb[0];
-            std::vector<uint8_t> payload(get_u32(qb + 1));
-            if (!d->read_exact(payload.data(), payload.size())) return RIV_E_END_OF_STREAM;
-            // ResidualDecoder::decode_intra (reitero_residual/src/residual.rs:815-887)
-            std::fill(hco, hco + (size_t)nb * 384, 0);
-            coded.assign(nb, 1);
-            rivh::decode_residuals(payload.data(), payload.size(), coded.data(), nb, true, hco);
-            const float qs = quant_step(q);
-            intra_tables(qs, yt, ct);
-            RIV_HIP(hipMemcpyAsync(d->yt.p, yt, sizeof(yt), hipMemcpyHostToDevice, st));
-            RIV_HIP(hipMemcpyAsync(d->ct.p, ct, sizeof(ct), hipMemcpyHostToDevice, st));
-            RIV_HIP(hipMemcpyAsync(d->coeffs.p, hco, 768 * (size_t)nb, hipMemcpyHostToDevice, st));
-            rivk::intra_recon(st, W, H, d->yt.as<uint16_t>(), d->ct.as<uint16_t>(), d->coeffs.as<int16_t>(), nY, nU, nV);
-            if (!d->skip_residuals) rivk::deblock(st, nY, nU, nV, W, H, deblock_level(qs), true, nullptr);
+            f.q = qb[0];
+            f.payload.resize(get_u32(qb + 1));
+            if (!d->read_exact(f.payload.data(), f.payload.size())) return RIV_E_END_OF_STREAM;
+            f.coded.assign(nb, 1);
+            intra_tables(quant_step(f.q), f.yt, f.ct);
             d->have_prev_mvs = false;
             d->mvc.reset();
+            d->parsed_intra = true;
         } else {
             uint8_t hb[8];
             if (!d->read_exact(hb, 8)) return RIV_E_END_OF_STREAM;
-            const uint8_t q = hb[0];
+            f.q = hb[0];
             const int bias_dx = (int8_t)hb[1], bias_dy = (int8_t)hb[2];
             std::vector<uint8_t> mvp(get_u32(hb + 4));
             if (!d->read_exact(mvp.data(), mvp.size())) return RIV_E_END_OF_STREAM;
             uint8_t rs[4];
             if (!d->read_exact(rs, 4)) return RIV_E_END_OF_STREAM;
-            std::vector<uint8_t> res(get_u32(rs));
-            if (!d->read_exact(res.data(), res.size())) return RIV_E_END_OF_STREAM;
-            if (!d->have_ref) { set_error("Inter frame before first intra frame"); return RIV_E_INVALID_FRAME; }
+            f.payload.resize(get_u32(rs));
+            if (!d->read_exact(f.payload.data(), f.payload.size())) return RIV_E_END_OF_STREAM;
+            if (!d->parsed_intra) { set_error("Inter frame before first intra frame"); return RIV_E_INVALID_FRAME; }
             std::vector<rivh::MvBlock> blocks = d->mvc.decode(mvp.data(), mvp.size(), bw, bh);
             // MV reconstruction (decoder.rs:316-382): base from the predictors of the growing list
-            mv.assign(nb, Mv4{});
-            skip.assign(nb, 0); coded.assign(nb, 0); mask.assign(nb, 0);
+            f.mv.assign(nb, Mv4{});
+            f.skip.assign(nb, 0); f.coded.assign(nb, 0); f.mask.assign(nb, 0);
             const Mv4* prev = d->have_prev_mvs ? d->prev_mvs.data() : nullptr;
             for (int i = 0; i < nb; i++) {
                 const rivh::MvBlock& b = blocks[i];
-                Neighbours n = neighbours(mv.data(), i, prev, bw, bh, i % bw, i / bw);
+                Neighbours n = neighbours(f.mv.data(), i, prev, bw, bh, i % bw, i / bw);
                 Mv4 base{};
                 switch (b.mode) {
                     case riv::M_ZERO: break;
@@ -179,31 +198,71 @@ static int decode_next(riv_decoder* d, uint8_t* rgb_out, riv_decoded_info* info)
                 m.v[1] = (int8_t)std::max(-128, std::min(127, (int)base.v[1] + ddy));
                 m.v[2] = b.mode == riv::M_NEW ? b.sx : base.v[2];
                 m.v[3] = b.mode == riv::M_NEW ? b.sy : base.v[3];
-                mv[i] = m;
-                skip[i] = b.skip;
-                coded[i] = !b.skip;
-                mask[i] = !b.skip || m.v[0] || m.v[1] || m.v[2] || m.v[3];   // decoder.rs:412-423
+                f.mv[i] = m;
+                f.skip[i] = b.skip;
+                f.coded[i] = !b.skip;
+                f.mask[i] = !b.skip || m.v[0] || m.v[1] || m.v[2] || m.v[3];   // decoder.rs:412-423
             }
-            RIV_HIP(hipMemcpyAsync(d->mvs.p, mv.data(), 4 * (size_t)nb, hipMemcpyHostToDevice, st));
-            rivk::mc(st, rY, rU, rV, W, H, d->mvs.as<int8_t>(), 4, pY, pU, pV, d->var.as<uint32_t>(), d->lnv.as<float>());
-            if (d->skip_residuals) {
-                RIV_HIP(hipMemcpyAsync(nY, pY, ysz + 2 * csz, hipMemcpyDeviceToDevice, st));
-            } else {
-                std::fill(hco, hco + (size_t)nb * 384, 0);
-                rivh::decode_residuals(res.data(), res.size(), coded.data(), nb, false, hco);
-                RIV_HIP(hipMemcpyAsync(d->coeffs.p, hco, 768 * (size_t)nb, hipMemcpyHostToDevice, st));
-                RIV_HIP(hipMemcpyAsync(d->skip.p, skip.data(), nb, hipMemcpyHostToDevice, st));
-                RIV_HIP(hipMemcpyAsync(d->mask.p, mask.data(), nb, hipMemcpyHostToDevice, st));
-                rivk::aq(st, nb, quant_step(q), d->lnv.as<float>(), d->steps.as<float>());
-                rivk::inter_recon(st, pY, pU, pV, W, H, d->skip.as<uint8_t>(), d->steps.as<float>(), d->coeffs.as<int16_t>(), nY, nU, nV);
-                rivk::deblock(st, nY, nU, nV, W, H, deblock_level(quant_step(q)), false, d->mask.as<uint8_t>());
-            }
-            d->prev_mvs = mv;
+            d->prev_mvs = f.mv;
             d->have_prev_mvs = true;
         }
     } catch (const rivh::CorruptStream& ex) {
         set_error(std::string("corrupt entropy data: ") + ex.what());
         return RIV_E_DECODING_FAILED;
+    } catch (const std::exception& ex) {   // the stage may run on a look-ahead thread: no exception leaves it
+        set_error(std::string("decoding failed: ") + ex.what());
+        return RIV_E_DECODING_FAILED;
+    }
+    return RIV_OK;
+}
+
+// Host stage, part 2 (any order, any thread): the residuals (ResidualDecoder::decode_intra /
+// decode_inter, residual.rs:815-966) into the frame's coefficient buffer.
+static int decode_frame_residuals(riv_decoder* d, HostFrame& f) {
+    int16_t* hco = f.coeffs.as<int16_t>();
+    std::fill(hco, hco + (size_t)d->nb * 384, 0);
+    try {
+        rivh::decode_residuals(f.payload.data(), f.payload.size(), f.coded.data(), d->nb, f.ft == 1, hco);
+    } catch (const rivh::CorruptStream& ex) {
+        set_error(std::string("corrupt entropy data: ") + ex.what());
+        return RIV_E_DECODING_FAILED;
+    } catch (const std::exception& ex) {
+        set_error(std::string("decoding failed: ") + ex.what());
+        return RIV_E_DECODING_FAILED;
+    }
+    return RIV_OK;
+}
+
+// Device stage (frame order): reconstruction, deblock and RGB out.
+static int reconstruct(riv_decoder* d, HostFrame& f, uint8_t* rgb_out, riv_decoded_info* info) {
+    RIV_HIP(hipSetDevice(d->dev));
+    const int W = d->W, H = d->H, nb = d->nb;
+    const size_t ysz = (size_t)W * H, csz = ysz / 4;
+    uint8_t* nY = d->ref[1 - d->ref_idx].as<uint8_t>(); uint8_t* nU = nY + ysz; uint8_t* nV = nU + csz;
+    uint8_t* rY = d->ref[d->ref_idx].as<uint8_t>(); uint8_t* rU = rY + ysz; uint8_t* rV = rU + csz;
+    uint8_t* pY = d->pred.as<uint8_t>(); uint8_t* pU = pY + ysz; uint8_t* pV = pU + csz;
+    hipStream_t st = d->st;
+    const float qs = quant_step(f.q);
+    if (f.ft == 1) {
+        RIV_HIP(hipMemcpyAsync(d->yt.p, f.yt, sizeof(f.yt), hipMemcpyHostToDevice, st));
+        RIV_HIP(hipMemcpyAsync(d->ct.p, f.ct, sizeof(f.ct), hipMemcpyHostToDevice, st));
+        RIV_HIP(hipMemcpyAsync(d->coeffs.p, f.coeffs.p, 768 * (size_t)nb, hipMemcpyHostToDevice, st));
+        rivk::intra_recon(st, W, H, d->yt.as<uint16_t>(), d->ct.as<uint16_t>(), d->coeffs.as<int16_t>(), nY, nU, nV);
+        if (!d->skip_residuals) rivk::deblock(st, nY, nU, nV, W, H, deblock_level(qs), true, nullptr);
+    } else {
+        if (!d->have_ref) { set_error("Inter frame before first intra frame"); return RIV_E_INVALID_FRAME; }
+        RIV_HIP(hipMemcpyAsync(d->mvs.p, f.mv.data(), 4 * (size_t)nb, hipMemcpyHostToDevice, st));
+        rivk::mc(st, rY, rU, rV, W, H, d->mvs.as<int8_t>(), 4, pY, pU, pV, d->var.as<uint32_t>(), d->lnv.as<float>());
+        if (d->skip_residuals) {
+            RIV_HIP(hipMemcpyAsync(nY, pY, ysz + 2 * csz, hipMemcpyDeviceToDevice, st));
+        } else {
+            RIV_HIP(hipMemcpyAsync(d->coeffs.p, f.coeffs.p, 768 * (size_t)nb, hipMemcpyHostToDevice, st));
+            RIV_HIP(hipMemcpyAsync(d->skip.p, f.skip.data(), nb, hipMemcpyHostToDevice, st));
+            RIV_HIP(hipMemcpyAsync(d->mask.p, f.mask.data(), nb, hipMemcpyHostToDevice, st));
+            rivk::aq(st, nb, qs, d->lnv.as<float>(), d->steps.as<float>());
+            rivk::inter_recon(st, pY, pU, pV, W, H, d->skip.as<uint8_t>(), d->steps.as<float>(), d->coeffs.as<int16_t>(), nY, nU, nV);
+            rivk::deblock(st, nY, nU, nV, W, H, deblock_level(qs), false, d->mask.as<uint8_t>());
+        }
     }
     d->ref_idx = 1 - d->ref_idx;
     d->have_ref = true;
@@ -213,8 +272,109 @@ static int decode_next(riv_decoder* d, uint8_t* rgb_out, riv_decoded_info* info)
         RIV_HIP(hipMemcpyAsync(rgb_out, d->rgb.p, (size_t)dw * dh * 3, hipMemcpyDeviceToHost, st));
     }
     RIV_HIP(hipGetLastError());
-    RIV_HIP(hipStreamSynchronize(st));
-    if (info) { info->width = (uint32_t)dw; info->height = (uint32_t)dh; info->timestamp_ms = ts; info->frame_index = d->current; info->frame_type = ft; }
+    RIV_HIP(hipStreamSynchronize(st));   // the frame's host buffers are free again after this
+    if (info) { info->width = (uint32_t)dw; info->height = (uint32_t)dh; info->timestamp_ms = f.ts; info->frame_index = d->current; info->frame_type = f.ft; }
+    return RIV_OK;
+}
+
+static void pipe_producer(riv_decoder* d, uint64_t first) {
+    DecodePipe& P = *d->pipe;
+    for (uint64_t i = first; i < d->hdr.frame_count; i++) {
+        const int k = (int)(i % DecodePipe::kSlots);
+        {
+            std::unique_lock<std::mutex> lk(P.mu);
+            P.cv.wait(lk, [&] { return P.stop || P.state[k] == 0; });
+            if (P.stop) return;
+            P.state[k] = 1;
+        }
+        HostFrame& f = P.slots[k];
+        f.status = parse_frame(d, f);
+        if (f.status != RIV_OK) {   // the frame carries the error; nothing after it is read
+            f.err = last_error();
+            std::lock_guard<std::mutex> lk(P.mu);
+            P.state[k] = 2;
+            P.cv.notify_all();
+            return;
+        }
+        std::lock_guard<std::mutex> lk(P.mu);
+        P.jobs.push_back(k);
+        P.cv.notify_all();
+    }
+}
+
+static void pipe_worker(riv_decoder* d) {
+    DecodePipe& P = *d->pipe;
+    for (;;) {
+        int k;
+        {
+            std::unique_lock<std::mutex> lk(P.mu);
+            P.cv.wait(lk, [&] { return P.stop || !P.jobs.empty(); });
+            if (P.stop) return;
+            k = P.jobs.front();
+            P.jobs.pop_front();
+        }
+        HostFrame& f = P.slots[k];
+        f.status = decode_frame_residuals(d, f);
+        if (f.status != RIV_OK) f.err = last_error();
+        std::lock_guard<std::mutex> lk(P.mu);
+        P.state[k] = 2;
+        P.cv.notify_all();
+    }
+}
+
+static void stop_pipe(riv_decoder* d) {
+    if (!d->pipe) return;
+    {
+        std::lock_guard<std::mutex> lk(d->pipe->mu);
+        d->pipe->stop = true;
+    }
+    d->pipe->cv.notify_all();
+    if (d->pipe->producer.joinable()) d->pipe->producer.join();
+    for (auto& t : d->pipe->workers) t.join();
+    delete d->pipe;
+    d->pipe = nullptr;
+}
+
+static int start_pipe(riv_decoder* d) {
+    DecodePipe* P = new DecodePipe();
+    for (HostFrame& f : P->slots) {
+        if (f.coeffs.alloc(768 * (size_t)d->nb) != hipSuccess) { delete P; set_error("decoder: pinned buffers"); return RIV_E_GPU; }
+    }
+    d->pipe = P;
+    const int cores = (int)std::max(2u, std::thread::hardware_concurrency());
+    const int nw = std::max(1, std::min(DecodePipe::kSlots - 1, cores - 2));
+    P->producer = std::thread(pipe_producer, d, d->current);
+    for (int i = 0; i < nw; i++) P->workers.emplace_back(pipe_worker, d);
+    return RIV_OK;
+}
+
+// The next frame: from the look-ahead (in-memory streams) or decoded in place (reader streams).
+static int decode_next(riv_decoder* d, uint8_t* rgb_out, riv_decoded_info* info) {
+    if (!d->mem) {
+        HostFrame& f = d->inline_frame;
+        if (!f.coeffs.p) RIV_HIP(f.coeffs.alloc(768 * (size_t)d->nb));
+        int r = parse_frame(d, f);
+        if (!r) r = decode_frame_residuals(d, f);
+        if (r) return r;
+        return reconstruct(d, f, rgb_out, info);
+    }
+    if (!d->pipe) {
+        int r = start_pipe(d);
+        if (r) return r;
+    }
+    DecodePipe& P = *d->pipe;
+    const int k = (int)(d->current % DecodePipe::kSlots);
+    {
+        std::unique_lock<std::mutex> lk(P.mu);
+        P.cv.wait(lk, [&] { return P.state[k] == 2; });
+    }
+    HostFrame& f = P.slots[k];
+    if (f.status != RIV_OK) { set_error(f.err); return f.status; }   // kept: the frame stays failed
+    const int r = reconstruct(d, f, rgb_out, info);
+    if (r) return r;
+    std::lock_guard<std::mutex> lk(P.mu);
+    P.state[k] = 0;
+    P.cv.notify_all();
     return RIV_OK;
 }
 
@@ -259,6 +419,7 @@ int riv_decoder_debug_yuv(riv_decoder* d, uint8_t* y, uint8_t* u, uint8_t* v) {
 }
 void riv_decoder_destroy(riv_decoder* d) {
     if (!d) return;
+    stop_pipe(d);
     (void)hipSetDevice(d->dev);
     if (d->st) { (void)hipStreamSynchronize(d->st); (void)hipStreamDestroy(d->st); }
     delete d;
