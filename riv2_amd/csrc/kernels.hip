@@ -1572,16 +1572,20 @@ __global__ void k_deblock(uint8_t* Y, uint8_t* U, uint8_t* V, int W, int H, int 
     else return;
     int px[8];
     if (!horizontal) {
-        int h = grid == 16 ? H : H >> 1;
-        int e = (int)(i / h) + 1, y = (int)(i % h);
+        // consecutive threads take consecutive edges of one row (bw - 1 per row in every plane),
+        // so a wave's segments share a few cache lines; edges 16 (8) pixels apart never overlap
+        int e = (int)(i % (bw - 1)) + 1, y = (int)(i / (bw - 1));
         int x = e * grid, by = y / grid;
         if (mask && !(mask[by * bw + e - 1] || mask[by * bw + e])) return;
-        uint8_t* p = P + (size_t)y * w + x - 4;
+        uint32_t* p = (uint32_t*)(P + (size_t)y * w + x - 4);   // x - 4 is a multiple of 4
+        const uint32_t a = p[0], b = p[1];
 #pragma unroll
-        for (int k = 0; k < 8; k++) px[k] = p[k];
+        for (int k = 0; k < 4; k++) { px[k] = (int)((a >> (8 * k)) & 0xffu); px[4 + k] = (int)((b >> (8 * k)) & 0xffu); }
         mb_filter_line(px, E, I, HEV);
+        uint32_t na = 0u, nb2 = 0u;
 #pragma unroll
-        for (int k = 0; k < 8; k++) p[k] = (uint8_t)px[k];
+        for (int k = 0; k < 4; k++) { na |= (uint32_t)(uint8_t)px[k] << (8 * k); nb2 |= (uint32_t)(uint8_t)px[4 + k] << (8 * k); }
+        p[0] = na; p[1] = nb2;
     } else {
         int e = (int)(i / w) + 1, x = (int)(i % w);
         int y = e * grid, bxx = x / grid;
