@@ -438,7 +438,7 @@ def main():
     print("[bench] oracle prefix check", file=sys.stderr, flush=True)
     bit_exact = check_prefix_bit_exact(frames, stream, 3)
     print("[bench] decode", file=sys.stderr, flush=True)
-    decode = run_decode(stream, nframes, args.cpu_decode_frames)
+    decode = run_decode(stream, nframes, max(1, min(args.cpu_decode_frames, nframes - 1)))
 
     cpu = None
     if not args.no_cpu_baseline:
