@@ -523,12 +523,6 @@ void adam(hipStream_t st, float* w, float* m, float* v, const float* g, const ui
 }
 hipError_t infer(hipStream_t st, const Source* srcs, const Source& s0, const int* samples, int n, const Weights& w, float conf_logit, int sr,
                  uint4* prop, float* yout) {
-    static bool attr = false;
-    if (!attr) {
-        const hipError_t e = hipFuncSetAttribute((const void*)k_net_infer, hipFuncAttributeMaxDynamicSharedMemorySize, kInferLds);
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
     if (n > 0) k_net_infer<<<(n + 15) / 16, 64 * kInferWaves, kInferLds, st>>>(srcs, s0, samples, n, w, conf_logit, sr, prop, yout);
     return hipGetLastError();
 }
