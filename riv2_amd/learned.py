@@ -15,7 +15,6 @@ through the C ABI (include/riv_gpu.h, riv_mvnet_*):
 
 The bitstream of the learned form is an ordinary RIV v5 stream: the plain decoder decodes it.
 """
-import ctypes
 import os
 from ctypes import byref, c_float, c_void_p
 from dataclasses import dataclass, field
@@ -25,7 +24,7 @@ import numpy as np
 import torch   # before the backend initialises the HIP runtime, so that torch finds the device
 
 from . import _lib as L
-from .api import EncodeError, Encoder, EncoderConfig, Frame, KeyframeInterval, _check
+from .api import Encoder, EncoderConfig, Frame, KeyframeInterval, _check
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS_PATH = os.path.join(HERE, "mvnet_weights.npz")
