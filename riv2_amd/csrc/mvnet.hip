@@ -272,8 +272,12 @@ __global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __re
 // waves per CU. Each pixel is converted to bf16 once; a conv fragment is one 8-byte LDS read.
 namespace {
 constexpr int kInferWaves = 8;
-constexpr int kWaveLds = 4096 + 4096 + 1024;   // uint16 entries: input, conv1 out, conv2 out
+// Row pitch of a wave's input tile in uint16 entries: 32 pixels x 4 channels and one pixel of
+// padding, so that the eight rows one patch store covers start in different LDS banks.
+constexpr int kX0Pitch = 32 * 4 + 4;
+constexpr int kWaveLds = 32 * kX0Pitch + 4096 + 1024;   // uint16 entries: input, conv1 out, conv2 out
 constexpr int kInferLds = kInferWaves * kWaveLds * 2 + 16 * rivnet::FC_IN * 2 + 16 * rivnet::FC_H * 2 + 16 * 16 * 4;
+static_assert(kInferLds <= 160 * 1024, "the inference workgroup's LDS exceeds a CU's 160 KiB");
 
 __device__ __forceinline__ bool patch_interior(const rivnet::Source& S, int bx, int by) {
     const int x0 = bx * 16 - 8, y0 = by * 16 - 8;
@@ -301,7 +305,7 @@ __device__ __forceinline__ void patch_store(const uint32_t (&d)[8], uint16_t* x,
         const int q = lane + 64 * u, row = q >> 3, c4 = (q & 7) * 4;
 #pragma unroll
         for (int i = 0; i < 4; i++)
-            *(uint2*)(x + (row * 32 + c4 + i) * 4) = px4((int)((d[u] >> (8 * i)) & 0xffu), (int)((d[u + 4] >> (8 * i)) & 0xffu));
+            *(uint2*)(x + row * kX0Pitch + (c4 + i) * 4) = px4((int)((d[u] >> (8 * i)) & 0xffu), (int)((d[u + 4] >> (8 * i)) & 0xffu));
     }
 }
 __device__ __forceinline__ void patch_edge(const rivnet::Source& S, int bx, int by, uint16_t* x, int lane) {
@@ -309,13 +313,21 @@ __device__ __forceinline__ void patch_edge(const rivnet::Source& S, int bx, int 
     for (int p = lane; p < 1024; p += 64) {
         const int yy = min(max(y0 + (p >> 5), 0), S.H - 1), xx = min(max(x0 + (p & 31), 0), S.W - 1);
         const size_t o = (size_t)yy * S.W + xx;
-        *(uint2*)(x + p * 4) = px4(S.cur[o], S.ref[o]);
+        *(uint2*)(x + (p >> 5) * kX0Pitch + (p & 31) * 4) = px4(S.cur[o], S.ref[o]);
     }
 }
 
-// A fragment of a conv layer from its NHWC bf16 input x (C = 4: tap 4 ks + (lane >> 4), all 4
-// channels; C = 16: tap ks, channels 4 (lane >> 4) .. +3), output row m: one 8-byte LDS read,
-// zero past tap 8 and outside the input.
+// The 16-channel activations (conv1's 16x16 output, conv2's 8x8 output) keep each pixel's 16
+// channels together, with channel group g (channels 4g .. 4g+3) of pixel (x, y) in slot
+// g ^ act_swz(x, y): the 16 pixels one conv fragment reads (stride 2 in x, over two or four rows,
+// 64 bytes apart unswizzled) then fall in different LDS banks.
+template <int HI>
+__device__ __forceinline__ int act_swz(int x, int y) {
+    return HI == 16 ? (((x >> 3) & 1) | (((y >> 1) & 1) << 1)) : ((y >> 1) & 3);
+}
+// A fragment of a conv layer from its NHWC bf16 input x (C = 4: the input tile, row pitch
+// kX0Pitch, tap 4 ks + (lane >> 4), all 4 channels; C = 16: swizzled activations, tap ks, channels
+// 4 (lane >> 4) .. +3), output row m: one 8-byte LDS read, zero past tap 8 and outside the input.
 template <int C, int HI>
 __device__ __forceinline__ v4s conv_a(const uint16_t* x, int m, int ks, int lane) {
     constexpr int HO = HI / 2;
@@ -323,12 +335,14 @@ __device__ __forceinline__ v4s conv_a(const uint16_t* x, int m, int ks, int lane
     const int tap = C == 4 ? 4 * ks + (lane >> 4) : ks;
     const int c0 = C == 4 ? 0 : 4 * (lane >> 4);
     const int iy = 2 * oy + tap / 3 - 1, ix = 2 * ox + tap % 3 - 1;
-    v4s a = {0, 0, 0, 0};
-    if (tap < 9 && iy >= 0 && iy < HI && ix >= 0 && ix < HI) {
-        const uint2 v = *(const uint2*)(x + (iy * HI + ix) * C + c0);
-        a[0] = (short)(v.x & 0xffff); a[1] = (short)(v.x >> 16); a[2] = (short)(v.y & 0xffff); a[3] = (short)(v.y >> 16);
-    }
-    return a;
+    // the read is unconditional (from entry 0 when outside), so that a tile's reads issue together
+    // instead of each waiting behind a branch; the select then zeroes what lies outside
+    const bool ok = tap < 9 && (unsigned)iy < (unsigned)HI && (unsigned)ix < (unsigned)HI;
+    const int off = C == 4 ? iy * kX0Pitch + ix * 4 : (iy * HI + ix) * 16 + 4 * ((c0 >> 2) ^ act_swz<HI>(ix, iy));
+    uint2 v = *(const uint2*)(x + (ok ? off : 0));
+    v.x = ok ? v.x : 0u;
+    v.y = ok ? v.y : 0u;
+    return __builtin_bit_cast(v4s, v);
 }
 // B fragment: W[k][n] for k = 16 ks + 4 (lane >> 4) + j, n = n0 + (lane & 15).
 __device__ __forceinline__ v4s w_frag(const uint16_t* w, int N, int ks, int n0, int lane) {
@@ -337,6 +351,18 @@ __device__ __forceinline__ v4s w_frag(const uint16_t* w, int N, int ks, int n0, 
 #pragma unroll
     for (int j = 0; j < 4; j++) b[j] = (short)w[(k + j) * N + n];
     return b;
+}
+// D rows 4 (lane >> 4) + j (pixels m0 + row of an HO x HO map), column lane & 15 (channel):
+// bf16(ReLU(sum + bias)) to the swizzled 16-channel activations out
+template <int HO>
+__device__ __forceinline__ void store_act(v4f acc, const float* bias, int m0, uint16_t* out, int lane) {
+    const int n = lane & 15;
+    const float bn = bias[n];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int m = m0 + 4 * (lane >> 4) + j;
+        out[m * 16 + 4 * ((n >> 2) ^ act_swz<HO>(m % HO, m / HO)) + (n & 3)] = f2bf(fmaxf(acc[j] + bn, 0.0f));
+    }
 }
 // D rows 4 (lane >> 4) + j, column lane & 15: bf16(ReLU(sum + bias)) to out[(m0 + row) * ldo + col]
 __device__ __forceinline__ void store_relu(v4f acc, const float* bias, int n0, int m0, uint16_t* out, int ldo, int lane) {
@@ -347,39 +373,68 @@ __device__ __forceinline__ void store_relu(v4f acc, const float* bias, int n0, i
 }
 }  // namespace
 
+// One workgroup of 64 per fragment.
+__global__ __launch_bounds__(64) void k_pack_frags(rivnet::Weights w, uint2* __restrict__ pk) {
+    const int f = blockIdx.x, lane = threadIdx.x;
+    const uint16_t* src = w.w1;
+    int N = 16, ks = f, n0 = 0;
+    if (f >= rivnet::FRAG_W5) { src = w.w5; N = rivnet::OUT; ks = f - rivnet::FRAG_W5; }
+    else if (f >= rivnet::FRAG_W4) { src = w.w4; N = rivnet::FC_H; ks = (f - rivnet::FRAG_W4) % 17; n0 = 16 * ((f - rivnet::FRAG_W4) / 17); }
+    else if (f >= rivnet::FRAG_W3) { src = w.w3; ks = f - rivnet::FRAG_W3; }
+    else if (f >= rivnet::FRAG_W2) { src = w.w2; ks = f - rivnet::FRAG_W2; }
+    pk[f * 64 + lane] = __builtin_bit_cast(uint2, w_frag(src, N, ks, n0, lane));
+}
+
 __global__ __launch_bounds__(512) void k_net_infer(const rivnet::Source* __restrict__ srcs, rivnet::Source s0, const int* __restrict__ samples, int n,
                                                    rivnet::Weights wt, float conf_logit, int sr, uint4* __restrict__ prop,
                                                    float* __restrict__ yout) {
     extern __shared__ __align__(16) uint16_t lds16[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     uint16_t* x0 = lds16 + wv * kWaveLds;
-    uint16_t* a1 = x0 + 4096;
+    uint16_t* a1 = x0 + 32 * kX0Pitch;
     uint16_t* a2 = a1 + 4096;
     uint16_t* a3 = lds16 + kInferWaves * kWaveLds;  // [16][FC_IN]
     uint16_t* a4 = a3 + 16 * rivnet::FC_IN;         // [16][FC_H]
     float* yl = (float*)(a4 + 16 * rivnet::FC_H);   // [16][16]
 
     // conv weights as B fragments, loaded once per wave
+    auto frag = [&](int f) { return __builtin_bit_cast(v4s, wt.pk[f * 64 + lane]); };
     v4s w1[3], w2[9], w3[9];
 #pragma unroll
-    for (int ks = 0; ks < 3; ks++) w1[ks] = w_frag(wt.w1, 16, ks, 0, lane);
+    for (int ks = 0; ks < 3; ks++) w1[ks] = frag(rivnet::FRAG_W1 + ks);
 #pragma unroll
-    for (int ks = 0; ks < 9; ks++) { w2[ks] = w_frag(wt.w2, 16, ks, 0, lane); w3[ks] = w_frag(wt.w3, 16, ks, 0, lane); }
+    for (int ks = 0; ks < 9; ks++) { w2[ks] = frag(rivnet::FRAG_W2 + ks); w3[ks] = frag(rivnet::FRAG_W3 + ks); }
 
-    // the next block's interior patch, fetched while the current one is computed
-    uint32_t pf[8];
-    bool pf_ok = false;
-    auto fetch = [&](int q) {
+    // fc1's and the head's weight fragments, and both blocks' sources, interior patches and
+    // predictor inputs, are loaded up front: the computation below waits on no global memory
+    v4s w4[rivnet::FC_IN / 16], w5[rivnet::FC_H / 16];
+#pragma unroll
+    for (int ks = 0; ks < rivnet::FC_IN / 16; ks++) w4[ks] = frag(rivnet::FRAG_W4 + 17 * wv + ks);
+#pragma unroll
+    for (int ks = 0; ks < rivnet::FC_H / 16; ks++) w5[ks] = wv == 0 ? frag(rivnet::FRAG_W5 + ks) : v4s{0, 0, 0, 0};
+    rivnet::Source sq[2];
+    int bxq[2], byq[2];
+    uint32_t pf[2][8];
+    bool inq[2];
+    float pin[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
         const int i = blockIdx.x * 16 + 2 * wv + q;
-        pf_ok = false;
-        if (q >= 2 || i >= n) return;
+        sq[q] = s0;
+        bxq[q] = byq[q] = 0;
+        inq[q] = false;
+        pin[q] = 0.0f;
+        if (i >= n) continue;
         int si, blk;
         sample_of(samples, i, &si, &blk);
-        const rivnet::Source S = srcs ? srcs[si] : s0;
-        const int bx = blk % S.bw, by = blk / S.bw;
-        if (patch_interior(S, bx, by)) { patch_fetch(S, bx, by, pf, lane); pf_ok = true; }
-    };
-    fetch(0);
+        if (srcs) sq[q] = srcs[si];
+        bxq[q] = blk % sq[q].bw;
+        byq[q] = blk / sq[q].bw;
+        inq[q] = patch_interior(sq[q], bxq[q], byq[q]);
+        if (inq[q]) patch_fetch(sq[q], bxq[q], byq[q], pf[q], lane);
+        if (lane < 16) pin[q] = pred_input(sq[q], bxq[q], byq[q], lane);
+    }
+#pragma unroll
     for (int q = 0; q < 2; q++) {
         const int slot = 2 * wv + q, i = blockIdx.x * 16 + slot;
         uint16_t* row3 = a3 + slot * rivnet::FC_IN;
@@ -387,15 +442,10 @@ __global__ __launch_bounds__(512) void k_net_infer(const rivnet::Source* __restr
             for (int k = lane; k < rivnet::FC_IN; k += 64) row3[k] = 0;
             continue;
         }
-        int si, blk;
-        sample_of(samples, i, &si, &blk);
-        const rivnet::Source S = srcs ? srcs[si] : s0;
-        const int bx = blk % S.bw, by = blk / S.bw;
-        if (pf_ok) patch_store(pf, x0, lane);
-        else patch_edge(S, bx, by, x0, lane);
-        fetch(q + 1);
-        if (lane < 16) row3[256 + lane] = f2bf(pred_input(S, bx, by, lane));
-        lds_wait();   // this wave's LDS writes done before its reads; the prefetch stays in flight
+        if (inq[q]) patch_store(pf[q], x0, lane);
+        else patch_edge(sq[q], bxq[q], byq[q], x0, lane);
+        if (lane < 16) row3[256 + lane] = f2bf(pin[q]);
+        lds_wait();   // this wave's LDS writes done before its reads
         __builtin_amdgcn_wave_barrier();
         // conv1: 256 output pixels (16 tiles), K 48
 #pragma unroll 4
@@ -403,7 +453,7 @@ __global__ __launch_bounds__(512) void k_net_infer(const rivnet::Source* __restr
             v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int ks = 0; ks < 3; ks++) acc = mfma16(conv_a<4, 32>(x0, tm * 16 + (lane & 15), ks, lane), w1[ks], acc);
-            store_relu(acc, wt.b1, 0, tm * 16, a1, 16, lane);
+            store_act<16>(acc, wt.b1, tm * 16, a1, lane);
         }
         lds_wait();
         __builtin_amdgcn_wave_barrier();
@@ -413,7 +463,7 @@ __global__ __launch_bounds__(512) void k_net_infer(const rivnet::Source* __restr
             v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int ks = 0; ks < 9; ks++) acc = mfma16(conv_a<16, 16>(a1, tm * 16 + (lane & 15), ks, lane), w2[ks], acc);
-            store_relu(acc, wt.b2, 0, tm * 16, a2, 16, lane);
+            store_act<8>(acc, wt.b2, tm * 16, a2, lane);
         }
         lds_wait();
         __builtin_amdgcn_wave_barrier();
@@ -437,7 +487,7 @@ __global__ __launch_bounds__(512) void k_net_infer(const rivnet::Source* __restr
             const uint2 v = *(const uint2*)(a3 + (lane & 15) * rivnet::FC_IN + 16 * ks + 4 * (lane >> 4));
             v4s a;
             a[0] = (short)(v.x & 0xffff); a[1] = (short)(v.x >> 16); a[2] = (short)(v.y & 0xffff); a[3] = (short)(v.y >> 16);
-            acc = mfma16(a, w_frag(wt.w4, rivnet::FC_H, ks, n0, lane), acc);
+            acc = mfma16(a, w4[ks], acc);
         }
         store_relu(acc, wt.b4, n0, 0, a4, rivnet::FC_H, lane);
     }
@@ -450,7 +500,7 @@ __global__ __launch_bounds__(512) void k_net_infer(const rivnet::Source* __restr
             const uint2 v = *(const uint2*)(a4 + (lane & 15) * rivnet::FC_H + 16 * ks + 4 * (lane >> 4));
             v4s a;
             a[0] = (short)(v.x & 0xffff); a[1] = (short)(v.x >> 16); a[2] = (short)(v.y & 0xffff); a[3] = (short)(v.y >> 16);
-            acc = mfma16(a, w_frag(wt.w5, rivnet::OUT, ks, 0, lane), acc);
+            acc = mfma16(a, w5[ks], acc);
         }
         const int col = lane & 15;
 #pragma unroll
@@ -521,8 +571,14 @@ void adam(hipStream_t st, float* w, float* m, float* v, const float* g, const ui
           float eps, float c1, float c2, uint16_t* wb) {
     if (n) k_adam<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(w, m, v, g, mask, n, lr, b1, b2, eps, c1, c2, wb);
 }
+hipError_t pack_fragments(hipStream_t st, const Weights& w, uint2* pk) {
+    k_pack_frags<<<NFRAG, 64, 0, st>>>(w, pk);
+    return hipGetLastError();
+}
+
 hipError_t infer(hipStream_t st, const Source* srcs, const Source& s0, const int* samples, int n, const Weights& w, float conf_logit, int sr,
                  uint4* prop, float* yout) {
+    if (n > 0 && !w.pk) return hipErrorInvalidValue;   // no fragment-order weights
     if (n > 0) k_net_infer<<<(n + 15) / 16, 64 * kInferWaves, kInferLds, st>>>(srcs, s0, samples, n, w, conf_logit, sr, prop, yout);
     return hipGetLastError();
 }

@@ -34,10 +34,18 @@ struct Source {
     int W, H, bw, bh;
 };
 
-// Device pointers of the weights (bf16, [K][N]) and biases (f32).
+// The bf16 weights in MFMA B-fragment order for the fused inference kernel: fragment f holds,
+// for lane l, W[16 ks + 4 (l >> 4) + j][n0 + (l & 15)], j = 0..3, as one 8-byte word at
+// pk[64 f + l]. Fragments: conv1 ks 0..2, conv2 ks 0..8, conv3 ks 0..8, fc1 (n-tile t, ks)
+// at FRAG_W4 + 17 t + ks, head ks 0..7.
+constexpr int FRAG_W1 = 0, FRAG_W2 = 3, FRAG_W3 = 12, FRAG_W4 = 21, FRAG_W5 = 21 + 8 * 17, NFRAG = FRAG_W5 + 8;
+
+// Device pointers of the weights (bf16, [K][N]), biases (f32) and, for inference, the weights in
+// fragment order (pk, NFRAG x 64 words, made by pack_fragments from w1 .. w5).
 struct Weights {
     const uint16_t *w1, *w2, *w3, *w4, *w5;
     const float *b1, *b2, *b3, *b4, *b5;
+    const uint2* pk;
 };
 
 struct Gemm {
@@ -67,6 +75,8 @@ void adam(hipStream_t st, float* w, float* m, float* v, const float* g, const ui
 // 0..n-1 of s0, srcs then null too); prop[i]: the four candidates rounded to whole pixels and clamped to +-sr (x, y bytes,
 // two per word), the confident flag (logit >= conf_logit) and the logit's bits; yout: the raw
 // outputs [n][OUT], or null.
+// Writes the fragment-order copy of w.w1 .. w.w5 to pk (NFRAG x 64 words).
+hipError_t pack_fragments(hipStream_t st, const Weights& w, uint2* pk);
 hipError_t infer(hipStream_t st, const Source* srcs, const Source& s0, const int* samples, int n, const Weights& w, float conf_logit, int sr,
                  uint4* prop, float* yout);
 

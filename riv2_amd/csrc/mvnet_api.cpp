@@ -2,6 +2,7 @@
 // backward and Adam through the layer-wise kernels) and proposals for the encoder's motion search.
 #include <cmath>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 #include "host_common.hpp"
 #include "mvnet.hpp"
@@ -59,11 +60,27 @@ struct riv_mvnet {
     HostBuf stage;
     hipEvent_t staged = nullptr;
 
+    // the fragment-order copy of p16 for the fused inference kernel, remade (on st, waited for)
+    // the first time the weights are asked for after p16 changed
+    mutable DevBuf pk;
+    mutable std::mutex pk_mu;
+    mutable bool pk_stale = true;
+
+    // pk is null if the copy could not be made; infer then fails
     rivnet::Weights weights() const {
         const uint16_t* w = p16.as<uint16_t>();
         const float* b = p32.as<float>();
-        return {w + rivnet::OFF_W1, w + rivnet::OFF_W2, w + rivnet::OFF_W3, w + rivnet::OFF_W4, w + rivnet::OFF_W5,
-                b + rivnet::OFF_B1, b + rivnet::OFF_B2, b + rivnet::OFF_B3, b + rivnet::OFF_B4, b + rivnet::OFF_B5};
+        rivnet::Weights wt{w + rivnet::OFF_W1, w + rivnet::OFF_W2, w + rivnet::OFF_W3, w + rivnet::OFF_W4, w + rivnet::OFF_W5,
+                           b + rivnet::OFF_B1, b + rivnet::OFF_B2, b + rivnet::OFF_B3, b + rivnet::OFF_B4, b + rivnet::OFF_B5, nullptr};
+        std::lock_guard<std::mutex> lk(pk_mu);
+        if (pk_stale && rivnet::pack_fragments(st, wt, pk.as<uint2>()) == hipSuccess && hipStreamSynchronize(st) == hipSuccess)
+            pk_stale = false;
+        if (!pk_stale) wt.pk = pk.as<uint2>();
+        return wt;
+    }
+    void weights_changed() {
+        std::lock_guard<std::mutex> lk(pk_mu);
+        pk_stale = true;
     }
 };
 
@@ -84,6 +101,7 @@ int upload_params(riv_mvnet* n, const float* host) {
     RIV_HIP(hipMemset(n->adam_m.p, 0, NPARAM * 4));
     RIV_HIP(hipMemset(n->adam_v.p, 0, NPARAM * 4));
     n->step = 0;
+    n->weights_changed();
     return RIV_OK;
 }
 
@@ -251,7 +269,7 @@ int riv_mvnet_create(int32_t device_id, uint64_t seed, riv_mvnet** out) {
     auto fail = [&](int code) { riv_mvnet_destroy(n); return code; };
     if (hipStreamCreateWithFlags(&n->st, hipStreamNonBlocking) != hipSuccess) { set_error("mvnet: stream"); return fail(RIV_E_GPU); }
     if (n->p32.alloc(NPARAM * 4) || n->p16.alloc(NPARAM * 2) || n->adam_m.alloc(NPARAM * 4) || n->adam_v.alloc(NPARAM * 4) ||
-        n->grad.alloc(NPARAM * 4) || n->mask.alloc(NPARAM)) {
+        n->grad.alloc(NPARAM * 4) || n->mask.alloc(NPARAM) || n->pk.alloc((size_t)rivnet::NFRAG * 64 * sizeof(uint2))) {
         set_error("mvnet: out of device memory");
         return fail(RIV_E_GPU);
     }
@@ -336,6 +354,7 @@ int riv_mvnet_train_step(riv_mvnet* n, const int32_t* samples, const int8_t* tar
     const float b1 = 0.9f, b2 = 0.999f;
     rivnet::adam(n->st, n->p32.as<float>(), n->adam_m.as<float>(), n->adam_v.as<float>(), n->grad.as<float>(), n->mask.as<uint8_t>(), NPARAM, lr,
                  b1, b2, 1e-8f, 1.0f - std::pow(b1, (float)n->step), 1.0f - std::pow(b2, (float)n->step), n->p16.as<uint16_t>());
+    n->weights_changed();
     RIV_HIP(hipGetLastError());
     return read_loss(n, count, loss_out);
 }
