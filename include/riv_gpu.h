@@ -139,6 +139,8 @@ int riv_encoder_finish(riv_encoder* enc);
 /* Encoder::frame_count / header (encoder.rs:969-981) */
 uint64_t riv_encoder_frame_count(const riv_encoder* enc);
 int riv_encoder_header(const riv_encoder* enc, riv_video_header* out);
+/* Encoder::config (encoder.rs:974-976): the configuration the encoder was created with. */
+int riv_encoder_get_config(const riv_encoder* enc, riv_encoder_config* out);
 /* VecWriter::into_inner (writer.rs:33): bytes written so far (mem writer only). */
 int riv_encoder_output(const riv_encoder* enc, const uint8_t** data, size_t* len);
 void riv_encoder_destroy(riv_encoder* enc);
@@ -154,6 +156,11 @@ int riv_encoder_debug_yuv(riv_encoder* enc, int which, uint8_t* y, uint8_t* u, u
 int riv_encoder_debug_coeffs(riv_encoder* enc, int16_t* coeffs);
 /* Last P-frame's adaptive quant steps (residual.rs:116-167), nblocks floats. */
 int riv_encoder_debug_aq(riv_encoder* enc, float* steps);
+/* Test seam of the motion search's failure path (no reference counterpart): caps the relaxation
+ * passes a search may run (0 restores the proven bound 2*(bh-1) + bw + 2). A search that has not
+ * settled within the cap fails its frame: the frame returns (synchronous mode) or the next call
+ * and riv_encoder_flush return (pipelined mode) RIV_E_GPU, and no byte of that frame is written. */
+int riv_encoder_debug_pass_limit(riv_encoder* enc, int32_t max_passes);
 
 /* Decoder::new (reitero_decode/src/decoder.rs:112-171): reads and validates the header. */
 int riv_decoder_create(const riv_reader_vtbl* reader, void* user, int32_t device_id, riv_decoder** out);

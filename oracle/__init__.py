@@ -60,6 +60,7 @@ def lib():
             "ro_decoder_header": (None, [P, P]),
             "ro_decoder_last_yuv": (None, [P, P, P, P]),
             "ro_decoder_destroy": (None, [P]),
+            "ro_decoder_set_skip_residuals": (None, [P, c_int]),
             "ro_dct8_aq_plane": (None, [P, c_int, c_int, P, c_float, P, P]),
             "ro_dct16_aq_plane": (None, [P, c_int, c_int, P, c_float, P, P]),
             "ro_basis": (None, [P, P]),
@@ -173,6 +174,9 @@ class OracleDecoder:
         if rc:
             raise RuntimeError(f"oracle decode failed ({rc})")
         return out, info
+
+    def set_skip_residuals(self, skip: bool):
+        lib().ro_decoder_set_skip_residuals(self._h, 1 if skip else 0)
 
     def last_yuv(self):
         y = np.zeros((self.sh, self.sw), np.uint8); u = np.zeros((self.sh // 2, self.sw // 2), np.uint8); v = np.zeros_like(u)

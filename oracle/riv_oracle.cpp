@@ -1692,6 +1692,7 @@ struct ro_decoder {
     std::vector<MV> prev_mvs; bool have_prev_mvs = false;
     MvCtx mvctx;
     Yuv last;
+    bool skip_residuals = false;   // Decoder::set_skip_residuals (decoder.rs:492-494)
 };
 typedef struct { uint64_t timestamp, frame_index; int32_t frame_type; uint32_t width, height; } ro_decoded_info;
 
@@ -1738,7 +1739,7 @@ static int decode_next(ro_decoder* x, uint8_t* rgb_out, ro_decoded_info* info) {
                 for (int j = 0; j < 8; j++) for (int i = 0; i < 8; i++) dst[(size_t)(by * 8 + j) * cw + bx * 8 + i] = (uint8_t)clampv(rc[j * 8 + i] + 128, 0, 255);
             }
         }
-        deblock_yuv420(cur, deblock_level_from_qs(qs), true, nullptr);
+        if (!x->skip_residuals) deblock_yuv420(cur, deblock_level_from_qs(qs), true, nullptr);   // decoder.rs:269-274
         x->prev = cur; x->have_prev = true;
         x->have_prev_mvs = false; x->prev_mvs.clear();
         x->mvctx.reset();
@@ -1776,6 +1777,16 @@ static int decode_next(ro_decoder* x, uint8_t* rgb_out, ro_decoded_info* info) {
             mvsv.push_back(m); skip[i] = b.skip;
         }
         Yuv pred; build_predicted(x->prev, mvsv, pred);
+        if (x->skip_residuals) {   // the prediction alone, unfiltered (residual.rs:918-921, decoder.rs:407)
+            x->last = pred;
+            x->prev = pred;
+            x->prev_mvs = mvsv; x->have_prev_mvs = true;
+            std::vector<uint8_t> rgb((size_t)SW * SH * 3);
+            yuv420_to_rgb(pred, rgb.data());
+            crop_rgb24(rgb.data(), SW, (int)x->hdr.dw, (int)x->hdr.dh, rgb_out);
+            info->timestamp = ts; info->frame_index = x->current; info->frame_type = ft; info->width = x->hdr.dw; info->height = x->hdr.dh;
+            return 0;
+        }
         // ResidualDecoder::decode_inter (residual.rs:889-1079)
         DctRansDec dec(false);
         if (rs) dec.consume(rp, rs);
@@ -1812,6 +1823,7 @@ void ro_decoder_header(ro_decoder* x, uint32_t* out6) { out6[0] = x->hdr.dw; out
 void ro_decoder_last_yuv(ro_decoder* x, uint8_t* y, uint8_t* u, uint8_t* v) {
     memcpy(y, x->last.y.data(), x->last.y.size()); memcpy(u, x->last.u.data(), x->last.u.size()); memcpy(v, x->last.v.data(), x->last.v.size());
 }
+void ro_decoder_set_skip_residuals(ro_decoder* x, int skip) { x->skip_residuals = skip != 0; }
 void ro_decoder_destroy(ro_decoder* x) { delete x; }
 
 // ---------------------------------------------------------------------------

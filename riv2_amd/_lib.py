@@ -99,12 +99,14 @@ PROTOTYPES = [
     ("riv_encoder_finish", c_int, [c_void_p]),
     ("riv_encoder_frame_count", c_uint64, [c_void_p]),
     ("riv_encoder_header", c_int, [c_void_p, POINTER(VideoHeaderC)]),
+    ("riv_encoder_get_config", c_int, [c_void_p, POINTER(EncoderConfigC)]),
     ("riv_encoder_output", c_int, [c_void_p, POINTER(POINTER(c_uint8)), POINTER(c_size_t)]),
     ("riv_encoder_destroy", None, [c_void_p]),
     ("riv_encoder_debug_blocks", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("riv_encoder_debug_yuv", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     ("riv_encoder_debug_coeffs", c_int, [c_void_p, c_void_p]),
     ("riv_encoder_debug_aq", c_int, [c_void_p, c_void_p]),
+    ("riv_encoder_debug_pass_limit", c_int, [c_void_p, c_int32]),
     ("riv_decoder_create", c_int, [POINTER(ReaderVtblC), c_void_p, c_int32, POINTER(c_void_p)]),
     ("riv_decoder_create_mem", c_int, [c_void_p, c_size_t, c_int32, POINTER(c_void_p)]),
     ("riv_decode_frame", c_int, [c_void_p, c_void_p, c_size_t, POINTER(DecodedInfoC)]),
@@ -150,6 +152,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"RIV GPU backend not built: {LIB_PATH} is missing (run __graft_entry__.build())")
+        # PyTorch ships its own HIP runtime under the same soname (libamdhip64.so.7). The first one
+        # loaded serves the whole process, and PyTorch cannot use the GPU through another's, so
+        # where PyTorch is installed it is loaded first and this library shares its runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, res, args in PROTOTYPES:
             fn = getattr(L, name)

@@ -300,6 +300,12 @@ class Encoder:
         _check(L.lib().riv_encoder_header(self._h, byref(h)), EncodeError)
         return h
 
+    def get_config(self) -> EncoderConfig:
+        """Encoder::config (reitero_encode/src/encoder.rs:974-976): the configuration in use."""
+        c = EncoderConfig(1, 1, 1)
+        _check(L.lib().riv_encoder_get_config(self._h, byref(c._c)), EncodeError)
+        return c
+
     def output(self) -> bytes:
         p = POINTER(c_uint8)()
         n = c_size_t()
@@ -366,6 +372,11 @@ class Encoder:
         _check(L.lib().riv_encoder_debug_coeffs(self._h, c.ctypes.data))
         return c
 
+    def debug_pass_limit(self, max_passes: int):
+        """Caps the motion search's relaxation passes (0: the proven bound); a search that does not
+        settle within the cap fails its frame (no bytes are written for it)."""
+        _check(L.lib().riv_encoder_debug_pass_limit(self._h, int(max_passes)), EncodeError)
+
     def debug_aq(self):
         a = np.zeros(self._bw * self._bh, np.float32)
         _check(L.lib().riv_encoder_debug_aq(self._h, a.ctypes.data))
@@ -384,12 +395,36 @@ class DecodedFrame:
 
 
 class Decoder:
-    """Decoder<R: VideoReader> (reitero_decode/src/decoder.rs:89-499) over an in-memory stream."""
+    """Decoder<R: VideoReader> (reitero_decode/src/decoder.rs:89-499) over an in-memory stream, or,
+    with `reader`, over any object with read(n) -> bytes (empty at the end), position() and
+    seek(pos), driven through the C reader vtable (reitero_decode/src/reader.rs:4-13)."""
 
-    def __init__(self, data: bytes, device: int = 0):
+    def __init__(self, data: bytes = None, device: int = 0, reader=None):
         self._h = c_void_p()
-        self._data = bytes(data)
-        _check(L.lib().riv_decoder_create_mem(self._data, len(self._data), device, byref(self._h)), DecodeError)
+        if reader is None:
+            self._data = bytes(data)
+            _check(L.lib().riv_decoder_create_mem(self._data, len(self._data), device, byref(self._h)), DecodeError)
+        else:
+            self._reader = reader
+
+            def rd(_, buf, cap):
+                try:
+                    chunk = reader.read(cap)
+                except Exception:
+                    return 0
+                ctypes.memmove(buf, chunk, len(chunk))
+                return len(chunk)
+
+            def pos(_):
+                return reader.position()
+
+            def sk(_, p):
+                try:
+                    reader.seek(p); return 0
+                except Exception:
+                    return 1
+            self._vt = L.ReaderVtblC(L.READ_FN(rd), L.POS_FN(pos), L.SEEK_FN(sk))
+            _check(L.lib().riv_decoder_create(byref(self._vt), None, device, byref(self._h)), DecodeError)
         h = self.header()
         self._w, self._hh = h.display_width, h.display_height
         self._sw, self._sh = h.storage_width, h.storage_height

@@ -113,6 +113,21 @@ HostStage* g_hs = nullptr;
 int g_hs_refs = 0;
 }  // namespace
 
+// Persistent motion-search launches on one device run one at a time. Every workgroup of a launch
+// must be resident at once (its passes end at a grid barrier), so two launches from different
+// streams sharing the CUs could each hold part of the device and wait on each other until their
+// barriers time out. A launch on another stream than the device's previous one first waits, on
+// the device, for an event recorded on that stream; a stream that launches alone pays nothing.
+struct SearchGate {
+    std::mutex mu;
+    hipEvent_t ev = nullptr;
+    hipStream_t last = nullptr;   // stream of the device's latest search launch (null: none in flight)
+};
+SearchGate& search_gate(int device) {
+    static SearchGate gates[64];
+    return gates[(unsigned)device % 64u];
+}
+
 // Online autotuner of the batched motion search's launch shape, per (blocks, streams, search
 // range): the first steps of a key each launch one candidate shape (every shape reaches the same
 // fixed point, so the bitstreams do not change), timed by two events around the launch that are
@@ -223,6 +238,7 @@ struct riv_encoder {
     float net_logit = 0.0f;
     DevBuf prop, lstat;
 
+    int pass_cap = 0;              // test seam: relaxation passes allowed per search (0: the proven bound)
     int profiling = 0;             // 0 off, 1 motion-search stage only, 2 every stage
     double stage_ms[RIV_NUM_STAGES] = {};
     uint32_t stage_n[RIV_NUM_STAGES] = {};
@@ -250,6 +266,30 @@ struct riv_encoder {
 
 static HostStage* acquire_host_stage();
 static void release_host_stage();
+
+// Launches the motion search on st once the device's previous search (from any stream) is done.
+static int launch_search(int device, hipStream_t st, const riv::SearchParams& p, int groups, int waves) {
+    SearchGate& g = search_gate(device);
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (!g.ev) RIV_HIP(hipEventCreateWithFlags(&g.ev, hipEventDisableTiming | hipEventDisableSystemFence));
+    if (g.last && g.last != st) {   // everything queued on the other stream so far, its search included
+        RIV_HIP(hipEventRecord(g.ev, g.last));
+        RIV_HIP(hipStreamWaitEvent(st, g.ev, 0));
+    }
+    RIV_HIP(rivk::me_search(st, p, groups, waves));
+    g.last = st;
+    return RIV_OK;
+}
+
+// Error word of a finished inter frame's search (mapped host memory the kernel writes only on
+// failure), as a message, or null.
+static const char* search_error(const Slot& s) {
+    if (s.intra) return nullptr;
+    const int code = s.h_passes.as<int>()[1];
+    if (!code) return nullptr;
+    return code == 2 ? "mode decision: reconstructed MV differs from search MV"
+         : code == 3 ? "motion search did not settle within its pass limit" : "motion-search barrier timed out";
+}
 
 extern "C" {
 
@@ -390,7 +430,12 @@ static void destroy_encoder(riv_encoder* e) {
         release_host_stage();
         e->hs = nullptr;
     }
-    if (e->st) (void)hipStreamSynchronize(e->st);
+    if (e->st) {
+        (void)hipStreamSynchronize(e->st);
+        SearchGate& g = search_gate(e->cfg.device_id);
+        std::lock_guard<std::mutex> lk(g.mu);
+        if (g.last == e->st) g.last = nullptr;   // its searches are done; the stream goes away
+    }
     for (Slot& s : e->slots) {
         if (s.done) (void)hipEventDestroy(s.done);
         if (s.t0) (void)hipEventDestroy(s.t0);
@@ -665,6 +710,12 @@ static void waiter_loop(HostStage* hs) {
             run_tails(e, si);
             continue;
         }
+        if (const char* msg = search_error(e->slots[si])) {   // no bytes for a frame whose search failed
+            e->fail(RIV_E_GPU, msg);
+            e->slots[si].failed = true;
+            run_tails(e, si);
+            continue;
+        }
         collect_gpu_times(e->slots[si]);
         hs->pool.submit([e, si] { code_frame(e, si); });
     }
@@ -743,6 +794,7 @@ static int frame_begin(riv_encoder* e, const void* rgb, bool on_device, size_t l
         s.index = e->frame_count; s.ts = ts; s.intra = intra; s.busy = true; s.coded = false; s.modeled = false; s.packed = false;
         s.failed = false;
     }
+    s.h_passes.as<int>()[1] = 0;   // the search's error word: the kernel writes it only on failure
     s.n_marks = 0;
     if (e->profiling && !s.marks[0])
         for (auto& m : s.marks) RIV_HIP(hipEventCreateWithFlags(&m, hipEventDisableSystemFence));
@@ -838,12 +890,8 @@ static int frame_end(riv_encoder* e, const FrameCtx& f, riv_frame_stats* stats) 
         return RIV_OK;
     }
     if (hipEventSynchronize(s.done) != hipSuccess) { e->fail(RIV_E_GPU, "GPU frame failed"); s.failed = true; run_tails(e, f.si); }
+    else if (const char* msg = search_error(s)) { e->fail(RIV_E_GPU, msg); s.failed = true; run_tails(e, f.si); }
     else { collect_gpu_times(s); code_frame(e, f.si); }
-    if (!f.intra) {
-        int werr = 0;
-        RIV_HIP(hipMemcpy(&werr, e->err.p, 4, hipMemcpyDeviceToHost));
-        if (werr) { set_error(werr == 2 ? "mode decision: reconstructed MV differs from search MV" : werr == 3 ? "motion search did not settle within its pass limit" : "motion-search barrier timed out"); return RIV_E_GPU; }
-    }
     std::lock_guard<std::mutex> lk(e->mu);
     if (e->err_code) { set_error(e->err_msg); return e->err_code; }
     if (stats && !e->done_stats.empty()) *stats = e->done_stats.back();
@@ -869,7 +917,9 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
             p.lstat = e->lstat.as<unsigned long long>();
         }
         e->s_bank ^= 1;
-        RIV_HIP(rivk::me_search(f.st, p, e->shape.groups, e->shape.waves));
+        p.plimit = e->pass_cap > 0 ? std::min(p.maxp, e->pass_cap) : p.maxp;
+        int lr = launch_search(e->cfg.device_id, f.st, p, e->shape.groups, e->shape.waves);
+        if (lr) return lr;
         r = frame_after_search(e, f);
         if (r) return r;
     }
@@ -1004,7 +1054,9 @@ static int batch_step(riv_encoder* const* encs, size_t n, const void* const* rgb
         e0->b_bank ^= 1;
         hipEvent_t ta = ci >= 0 ? T.event() : nullptr, tb = ci >= 0 ? T.event() : nullptr;
         if (ta && tb) RIV_HIP(hipEventRecord(ta, st));
-        RIV_HIP(rivk::me_search(st, p, sh.groups, sh.waves));
+        p.plimit = e0->pass_cap > 0 ? std::min(p.maxp, e0->pass_cap) : p.maxp;
+        int lr = launch_search(c0.device_id, st, p, sh.groups, sh.waves);
+        if (lr) return lr;
         if (ta && tb) {
             RIV_HIP(hipEventRecord(tb, st));
             T.pending.push_back({ci, ta, tb});
@@ -1099,6 +1151,11 @@ int riv_encoder_header(const riv_encoder* e, riv_video_header* out) {
     *out = e->hdr;
     return RIV_OK;
 }
+int riv_encoder_get_config(const riv_encoder* e, riv_encoder_config* out) {
+    if (!e || !out) return RIV_E_ARG;
+    *out = e->cfg;
+    return RIV_OK;
+}
 int riv_encoder_output(const riv_encoder* e, const uint8_t** data, size_t* len) {
     if (!e || !e->mem || !data || !len) return RIV_E_ARG;
     *data = e->membuf.data(); *len = e->membuf.size();
@@ -1127,6 +1184,12 @@ int riv_encoder_learned_stats(riv_encoder* e, uint64_t* counts) {
         for (size_t w = 0; w + 2 < h.size(); w += 3)
             for (int i = 0; i < 3; i++) counts[i] += h[w + i];
     }
+    return RIV_OK;
+}
+
+int riv_encoder_debug_pass_limit(riv_encoder* e, int32_t max_passes) {
+    if (!e || max_passes < 0) return RIV_E_ARG;
+    e->pass_cap = max_passes;
     return RIV_OK;
 }
 

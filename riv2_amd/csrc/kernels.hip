@@ -294,10 +294,10 @@ __device__ __forceinline__ int mv_class(int mag) {   // mv_class_from_magnitude 
 // the matched candidate, which equals the search MV; for NEW it is base + clamp(search - base),
 // and with every MV inside [-sr-1, sr] (sr <= 32) the difference never reaches the clamp. So
 // the whole frame's decisions follow from the search field in parallel. The identity is still
-// checked for every NEW block: a violation sets *err = 2.
+// checked for every NEW block: a violation sets *err = 2 (and the stream's *serr, when given).
 __device__ __forceinline__ uint4 mode_record(uint32_t wS, int best_sad, bool has_l, bool has_t, bool has_tr, bool has_tl, bool has_tp,
                                              uint32_t wL, uint32_t wT, uint32_t wTR, uint32_t wTL, uint32_t wP,
-                                             const uint32_t* rdo_bits, int rdo_max, int* err) {
+                                             const uint32_t* rdo_bits, int rdo_max, int* err, int* serr) {
     // deduplicated (dx,dy,sx,sy) of left, top, top-right, top-left, temporal; the first two are
     // Nearest / Near, padded with zero. Every MV word is canonical (sx, sy in {0,1}), so equal
     // half-pel positions are equal words.
@@ -339,7 +339,10 @@ __device__ __forceinline__ uint4 mode_record(uint32_t wS, int best_sad, bool has
         }
         rdx = dclamp(S0 - bbx, -128, 127);
         rdy = dclamp(S1 - bby, -128, 127);
-        if (dclamp(bbx + rdx, -128, 127) != S0 || dclamp(bby + rdy, -128, 127) != S1) atomicExch(err, 2);
+        if (dclamp(bbx + rdx, -128, 127) != S0 || dclamp(bby + rdy, -128, 127) != S1) {
+            atomicExch(err, 2);
+            if (serr) *serr = 2;   // the stream's own error word (mapped host memory: a plain store)
+        }
     }
     // RdoContext::decide via the host-built table (rdo.rs:140-178)
     uint32_t rdo = 0;
@@ -771,7 +774,7 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
     bool mytook = false;   // the block's last evaluation took the network's proposal
     int outcome = 0;     // 1 converged, -1 barrier timeout, 0 pass limit
     int pass = 0;
-    for (; pass < p.maxp; pass++) {
+    for (; pass < p.plimit; pass++) {
         const uint32_t* src = p.field + (size_t)(pass & 1) * nb;
         uint32_t* dst = p.field + (size_t)((pass + 1) & 1) * nb;
         uint32_t nL = 0u, nT = 0u, nTR = 0u, nTL = 0u;
@@ -852,7 +855,11 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
         if (res == 0) { outcome = 1; break; }
     }
     if (outcome != 1) {
-        if (threadIdx.x == 0) __hip_atomic_store(p.err, outcome < 0 ? 1 : 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int code = outcome < 0 ? 1 : 3;
+        if (threadIdx.x == 0) __hip_atomic_store(p.err, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // every stream of the launch failed: its error word tells the host not to code the frame
+        for (int i = threadIdx.x; i < NS; i += blockDim.x)
+            if (sd[i].passes) sd[i].passes[1] = code;
         return;
     }
     // the mode decision, on the final field (every block stored its word in the last pass, which
@@ -866,7 +873,7 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
         const uint32_t wTR = has_tr ? field_load(fin + myb - mbw + 1) & 0x7fffffffu : 0u;
         const uint32_t wTL = has_tl ? field_load(fin + myb - mbw - 1) & 0x7fffffffu : 0u;
         const uint4 r = mode_record(myw, mysad, has_l, has_t, has_tr, has_tl, S.prev_mv != nullptr, wL, wT, wTR, wTL, mypT,
-                                    S.rdo_bits, S.rdo_max, p.err);
+                                    S.rdo_bits, S.rdo_max, p.err, S.passes ? S.passes + 1 : nullptr);
         *(uint4*)&S.rec[mloc] = r;
         *(uint4*)&S.host_rec[mloc] = r;
         // the next launch's start field; where bank 0 is the final field it already holds myw

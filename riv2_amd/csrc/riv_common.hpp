@@ -48,7 +48,9 @@ struct SearchStream {
     BlockRec* rec;               // per block out: the mode decision's record
     BlockRec* host_rec;          // the same records, written to mapped host memory
     const uint32_t* rdo_bits;    // bit s set -> RDO chooses skip at SAD s (integer-aligned MV)
-    int* passes;                 // out: passes the launch ran (may be nullptr)
+    int* passes;                 // out: [0] passes the launch ran, [1] left as the host cleared it (0) unless the
+                                 // stream's search failed: 1 barrier timeout, 3 pass limit, 2 mode-decision
+                                 // identity violated (may be nullptr)
     int W, H, bw, bh;
     int blk0;                    // number of the stream's first block in the launch
     int prev_stride;             // bytes between consecutive MVs of prev_mv
@@ -85,8 +87,9 @@ struct SearchParams {
                                  // (dx, dy, sx, sy), bit 31 set when the pass changed (dx, dy)
     unsigned* arrive;            // 2 banks of barrier_words(maxp) pass-barrier counters
     int bank;                    // bank of this launch (zero at launch); the launch clears the other
-    int maxp;                    // pass limit: 2*(bh-1) + bw + 2 passes of the largest stream always
-                                 // reach the fixed point
+    int maxp;                    // pass bound: 2*(bh-1) + bw + 2 passes of the largest stream always
+                                 // reach the fixed point (it also lays out the barrier counters)
+    int plimit;                  // passes this launch may run (<= maxp); not settling within them fails it
     int nslot, nres;             // LDS window slots per wave: lane k < nres keeps its block's window in
                                  // slot k; when nslot = nres + 1 the other lanes share slot nres
     const double* bits;          // 256 entries: 1.0 for 0, else 2*log2(a)+2 (motion/scalar.rs:495-525)

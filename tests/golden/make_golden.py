@@ -1,20 +1,33 @@
 #!/usr/bin/env python3
-"""Writes tests/golden/cif_config1.json: golden hashes of the CIF reference case, produced by the
-oracle (the C++ restatement of the reference encoder/decoder) in this repository.
+"""Writes the golden fixtures under tests/golden/, produced by the oracle (the C++ restatement of
+the reference encoder/decoder in oracle/) in this repository:
 
-Workload: 352x288 RGB24, 30 frames (1 I + 29 P) from the seeded synthetic generator (seed 1),
-intra/inter quality 85/80, search range 12, skip threshold 3, keyframe interval 1000. For frames
-0..5 the fixture holds SHA-256 digests of the per-MB records (search and reconstructed MVs, mode,
-NEW base and delta), best SADs, RDO and optimised skip flags, AQ steps, quantised coefficients,
-the reconstruction and the frame's bytes; for the whole sequence the stream digest and the digest
-of the decoded RGB frames; and a few MBs of frame 1 in full.
+* cif_config1.json -- the CIF reference case: 352x288 RGB24, 30 frames (1 I + 29 P), seed 1,
+  q85/80, search range 12, skip threshold 3, keyframe interval 1000. For frames 0..5, SHA-256
+  digests of the per-MB records (search and reconstructed MVs, mode, NEW base and delta), best
+  SADs, RDO and optimised skip flags, AQ steps, quantised coefficients, the reconstruction and the
+  frame's bytes; for the whole sequence the stream digest and the digest of the decoded RGB
+  frames; a few MBs of frame 1 in full.
+* hd_config2.json -- the 1080p workload: 1920x1080 RGB24, 50 frames (1 I + 49 P) of the 1080p
+  generator (riv2_amd.synth.hd_params, seed 1), q85/80, sr 12, skip 3, keyframe interval 1000,
+  50 fps, timestamps 20 * index. Per frame: size and digest of the frame record, digest of the
+  reconstruction and of the decoded RGB; the whole stream's size and digest. Any prefix of the
+  sequence encodes to the same frame records, so a run over its first k frames checks against
+  the first k entries.
+* batch_config3.json -- the batched workload: 64 streams, 16 each of 352x288, 640x360, 1280x720
+  and 1920x1080, each 1 I + 20 P of the CIF generator's motion (riv2_amd.synth.custom_params:
+  global motion 3 px, 4 rectangles of 32-96 px at 6 px/frame) with seed 1000 + 100 * size index
+  + stream index, q85/80, sr 12, skip 3, keyframe interval 1000, timestamps 33 * index. Per
+  stream: size and digest of the stream and digest of its decoded RGB frames.
 
-Run: python tests/golden/make_golden.py  (needs the oracle and libriv_synth.so built)
+Run: python tests/golden/make_golden.py [cif|hd|batch|all]  (needs the oracle and libriv_synth.so
+built; the batch fixture uses a process per core)
 """
 import hashlib
 import json
 import os
 import sys
+from multiprocessing import Pool
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
@@ -24,8 +37,13 @@ import numpy as np  # noqa: E402
 import oracle  # noqa: E402
 from riv2_amd import synth  # noqa: E402
 
+HERE = os.path.dirname(os.path.abspath(__file__))
 W, H, N, SEED = 352, 288, 30, 1
 DETAIL_FRAMES = 6
+
+HD_W, HD_H, HD_N, HD_SEED = 1920, 1080, 50, 1
+BATCH_SIZES = [(352, 288), (640, 360), (1280, 720), (1920, 1080)]
+BATCH_PER_SIZE, BATCH_FRAMES = 16, 21
 
 
 def sha(*arrays):
@@ -75,15 +93,88 @@ def compute():
     return out
 
 
-PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cif_config1.json")
+PATH = os.path.join(HERE, "cif_config1.json")
+HD_PATH = os.path.join(HERE, "hd_config2.json")
+BATCH_PATH = os.path.join(HERE, "batch_config3.json")
+
+
+def compute_hd():
+    frames = synth.Sequence(synth.hd_params(seed=HD_SEED, frames=HD_N)).frames(0, HD_N)
+    enc = oracle.OracleEncoder(oracle.config(HD_W, HD_H, fps=50))
+    per_frame = []
+    for i in range(HD_N):
+        enc.encode(frames[i], i * 20)
+        fb = enc.last_frame_bytes()
+        per_frame.append({"index": i, "type": "I" if i == 0 else "P", "frame_bytes": len(fb), "frame_sha256": sha(fb),
+                          "recon_sha256": sha(*enc.last_yuv(2))})
+    stream = enc.finish()
+    dec = oracle.OracleDecoder(stream)
+    for i in range(HD_N):
+        rgb, _info = dec.decode()
+        per_frame[i]["decoded_rgb_sha256"] = sha(rgb)
+    return {
+        "workload": {"width": HD_W, "height": HD_H, "frames": HD_N, "seed": HD_SEED, "generator": "riv2_amd.synth.hd_params",
+                     "fps": 50, "intra_quality": 85, "inter_quality": 80, "search_range": 12, "skip_threshold": 3,
+                     "keyframe_interval": 1000, "timestamps_ms": "20 * index"},
+        "frames": per_frame,
+        "stream_bytes": len(stream),
+        "stream_sha256": sha(stream),
+    }
+
+
+def batch_streams():
+    """(width, height, seed) of every stream of the batched workload, in batch order."""
+    return [(w, h, 1000 + 100 * k + j) for k, (w, h) in enumerate(BATCH_SIZES) for j in range(BATCH_PER_SIZE)]
+
+
+def batch_params(w, h, seed):
+    return synth.custom_params(w, h, BATCH_FRAMES, seed=seed, global_motion=3.0, n_rects=4, rect_min=32, rect_max=96,
+                               rect_velocity=6.0)
+
+
+def _batch_one(job):
+    w, h, seed = job
+    frames = synth.Sequence(batch_params(w, h, seed)).frames(0, BATCH_FRAMES, threads=1)
+    enc = oracle.OracleEncoder(oracle.config(w, h))
+    for i in range(BATCH_FRAMES):
+        enc.encode(frames[i], i * 33)
+    stream = enc.finish()
+    dec = oracle.OracleDecoder(stream)
+    hh = hashlib.sha256()
+    for _ in range(BATCH_FRAMES):
+        rgb, _info = dec.decode()
+        hh.update(np.ascontiguousarray(rgb).tobytes())
+    return {"width": w, "height": h, "seed": seed, "stream_bytes": len(stream), "stream_sha256": sha(stream),
+            "decoded_rgb_sha256": hh.hexdigest()}
+
+
+def compute_batch():
+    with Pool(min(8, os.cpu_count() or 1)) as pool:
+        streams = pool.map(_batch_one, batch_streams(), chunksize=1)
+    return {
+        "workload": {"streams": len(streams), "sizes": BATCH_SIZES, "per_size": BATCH_PER_SIZE, "frames": BATCH_FRAMES,
+                     "seeds": "1000 + 100 * size index + stream index",
+                     "generator": "riv2_amd.synth.custom_params(global_motion=3, n_rects=4, rect 32-96 px, velocity 6)",
+                     "intra_quality": 85, "inter_quality": 80, "search_range": 12, "skip_threshold": 3,
+                     "keyframe_interval": 1000, "timestamps_ms": "33 * index"},
+        "streams": streams,
+    }
+
+
+def _write(path, out):
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(f"wrote {path}")
 
 
 def main():
-    out = compute()
-    path = PATH
-    with open(path, "w") as f:
-        json.dump(out, f, indent=1)
-    print(f"wrote {path}: {out['stream_bytes']} bytes, {out['stream_sha256'][:16]}")
+    which = sys.argv[1] if len(sys.argv) > 1 else "all"
+    if which in ("cif", "all"):
+        _write(PATH, compute())
+    if which in ("hd", "all"):
+        _write(HD_PATH, compute_hd())
+    if which in ("batch", "all"):
+        _write(BATCH_PATH, compute_batch())
 
 
 if __name__ == "__main__":
