@@ -200,6 +200,18 @@ int riv_gpu_dct16_aq_plane(int32_t device_id, const int16_t* plane, uint32_t w, 
                            int16_t* coeffs, int16_t* recon);
 /* compute_aq_quant_steps (reitero_residual/src/residual.rs:116-167) on a luma plane */
 int riv_gpu_aq_steps(int32_t device_id, const uint8_t* y, uint32_t w, uint32_t h, float base_step, float* out);
+/* No reference counterpart: compute_aq_quant_steps (reitero_residual/src/residual.rs:116-167) on
+ * a luma plane already in device memory, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same steps as riv_gpu_aq_steps.
+ * It allocates nothing: its scratch (the per-MB logs and their mean) is a static device array of
+ * RIV_AQ_NOALLOC_MAX_BLOCKS + 64 floats that each device holds until the process ends, and larger
+ * planes are rejected with RIV_E_ARG. y: w x h luma plane, steps: (w/16)*(h/16) floats out, both
+ * device memory of device_id, 4-byte aligned; w, h multiples of 16. stream: a hipStream_t of that
+ * device (NULL: the null stream); the call returns once the work is queued. Every call on a
+ * device shares that device's scratch, so calls that overlap on one device (other streams,
+ * other threads) race; calls queued in order on one stream do not. */
+#define RIV_AQ_NOALLOC_MAX_BLOCKS 32768
+int riv_gpu_aq_steps_noalloc(int32_t device_id, const uint8_t* y, uint32_t w, uint32_t h, float base_step, float* steps, void* stream);
 /* The adaptive quantiser's mean of n per-MB logs (the sequential f32 sum of
  * reitero_residual/src/residual.rs:150-155, divided by n) and the steps base_step derives from it;
  * every log must be positive, as the logs of floored variances are. steps may be NULL. */

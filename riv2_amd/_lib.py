@@ -121,6 +121,7 @@ PROTOTYPES = [
     ("riv_gpu_dct8_aq_plane", c_int, [c_int32, c_void_p, c_uint32, c_uint32, c_void_p, c_float, c_void_p, c_void_p]),
     ("riv_gpu_dct16_aq_plane", c_int, [c_int32, c_void_p, c_uint32, c_uint32, c_void_p, c_float, c_void_p, c_void_p]),
     ("riv_gpu_aq_steps", c_int, [c_int32, c_void_p, c_uint32, c_uint32, c_float, c_void_p]),
+    ("riv_gpu_aq_steps_noalloc", c_int, [c_int32, c_void_p, c_uint32, c_uint32, c_float, c_void_p, c_void_p]),
     ("riv_gpu_aq_mean", c_int, [c_int32, c_void_p, c_size_t, c_float, c_void_p, c_void_p]),
     ("riv_gpu_me_tables", c_int, [c_int32, c_void_p, c_void_p, c_uint32, c_uint32, c_int32, c_void_p, c_void_p]),
     ("riv_gpu_logf_exp2f", c_int, [c_int32, c_void_p, c_size_t, c_int32, c_void_p]),

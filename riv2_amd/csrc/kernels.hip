@@ -1166,6 +1166,24 @@ __global__ __launch_bounds__(1024) void k_aq(float* __restrict__ lnv, int nb, fl
     }
 }
 
+// The adaptive quantiser's per-MB input straight from a luma plane (the plane taken as its own
+// prediction): the log variance of each 16x16 block, one wave per block, lane l summing the four
+// pixels 4(l&3) .. 4(l&3)+3 of row l>>2 (one aligned dword; W is a multiple of 16).
+__global__ __launch_bounds__(256) void k_aq_logs(const uint8_t* __restrict__ Y, int W, int nb, float* __restrict__ lnv) {
+    const int b = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (b >= nb) return;   // whole waves
+    const int bw = W >> 4, x0 = (b % bw) * 16, y0 = (b / bw) * 16;
+    const uint32_t px = *(const uint32_t*)(Y + (size_t)(y0 + (lane >> 2)) * W + x0 + 4 * (lane & 3));
+    uint32_t t1 = 0u, t2 = 0u;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const uint32_t v = (px >> (8 * i)) & 0xffu;
+        t1 += v; t2 += v * v;
+    }
+    const uint32_t s1 = wave_sum(t1), s2 = wave_sum(t2);
+    if (lane == 0) lnv[b] = aq_log_var(s1, s2);
+}
+
 // elementwise logf/exp2f for parity tests of the transcendental restatements
 __global__ void k_logf_exp2f(const float* in, size_t n, int which, float* out) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1705,6 +1723,10 @@ void mc(hipStream_t st, const uint8_t* rY, const uint8_t* rU, const uint8_t* rV,
 }
 void aq(hipStream_t st, int nb, float base, float* lnv, float* steps) {
     hipLaunchKernelGGL(k_aq, dim3(1), dim3(1024), aq_lds_bytes(), st, lnv, nb, base, steps);
+}
+void aq_logs(hipStream_t st, const uint8_t* Y, int W, int H, float* lnv) {
+    const int nb = (W / 16) * (H / 16);
+    hipLaunchKernelGGL(k_aq_logs, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, st, Y, W, nb, lnv);
 }
 void logf_exp2f(hipStream_t st, const float* in, size_t n, int which, float* out) {
     hipLaunchKernelGGL(k_logf_exp2f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, n, which, out);

@@ -30,6 +30,8 @@ void mc(hipStream_t st, const uint8_t* rY, const uint8_t* rU, const uint8_t* rV,
         uint8_t* pY, uint8_t* pU, uint8_t* pV, uint32_t* var_sums, float* lnv);
 // lnv: nb logs written by mc, plus one float for their mean; one launch computes the mean and the steps
 void aq(hipStream_t st, int nb, float base, float* lnv, float* steps);
+// lnv: (W/16)*(H/16) logs of the luma plane Y's block variances (Y 4-byte aligned), aq's input
+void aq_logs(hipStream_t st, const uint8_t* Y, int W, int H, float* lnv);
 void logf_exp2f(hipStream_t st, const float* in, size_t n, int which, float* out);
 void inter_residual(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, const uint8_t* rY, const uint8_t* rU,
                     const uint8_t* rV, uint8_t* pY, uint8_t* pU, uint8_t* pV, int W, int H, const riv::BlockRec* rec, const float* steps, float dz,

@@ -2,11 +2,37 @@
 // each stage against the reference functions it replaces.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 #include "host_common.hpp"
 #include "riv_kernels.hpp"
 
 using namespace rivhost;
+
+// Scratch of the allocation-free AQ entry point: the per-MB logs of one call and their mean
+// (kAqNoAllocMaxBlocks + 1 floats used). Every device the library runs on holds its own copy
+// until the process ends; host code reaches it only through the address hipGetSymbolAddress
+// returns for the current device (the array's name is a host-side stand-in).
+constexpr int kAqNoAllocMaxBlocks = RIV_AQ_NOALLOC_MAX_BLOCKS;
+static __device__ __align__(256) float scratch[kAqNoAllocMaxBlocks + 64];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch[kMaxDevices] = {};
+
+// The scratch array's address on `device`, looked up once per device with that device current.
+int scratch_on(int device, float** out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        RIV_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[device] = (float*)p;
+    }
+    *out = g_scratch[device];
+    return RIV_OK;
+}
+}  // namespace
 
 extern "C" {
 
@@ -114,6 +140,30 @@ int riv_gpu_aq_steps(int32_t dev, const uint8_t* y, uint32_t w, uint32_t h, floa
     rivk::aq(0, (int)nb, base, lnv.as<float>(), steps.as<float>());
     RIV_HIP(hipGetLastError());
     RIV_HIP(hipMemcpy(out, steps.p, 4 * nb, hipMemcpyDeviceToHost));
+    return RIV_OK;
+}
+
+int riv_gpu_aq_steps_noalloc(int32_t dev, const uint8_t* y, uint32_t w, uint32_t h, float base, float* steps, void* stream) {
+    if (!y || !steps || !w || !h || w % 16 || h % 16) { set_error("luma plane and steps are required, dimensions multiples of 16"); return RIV_E_ARG; }
+    if (((uintptr_t)y & 3u) || ((uintptr_t)steps & 3u)) { set_error("luma plane and steps must be 4-byte aligned"); return RIV_E_ARG; }
+    const size_t nb = (size_t)(w / 16) * (h / 16);
+    if (nb > (size_t)kAqNoAllocMaxBlocks) {
+        set_error("riv_gpu_aq_steps_noalloc: " + std::to_string(nb) + " macroblocks exceed its scratch of " +
+                  std::to_string(kAqNoAllocMaxBlocks) + " (RIV_AQ_NOALLOC_MAX_BLOCKS)");
+        return RIV_E_ARG;
+    }
+    // device selection only: no table upload, no allocation, nothing synchronous
+    int n = 0;
+    RIV_HIP(hipGetDeviceCount(&n));
+    if (dev < 0 || dev >= n || dev >= kMaxDevices) { set_error("device_id out of range"); return RIV_E_ARG; }
+    RIV_HIP(hipSetDevice(dev));
+    float* lnv = nullptr;
+    int r = scratch_on(dev, &lnv);
+    if (r) return r;
+    hipStream_t st = (hipStream_t)stream;
+    rivk::aq_logs(st, y, (int)w, (int)h, lnv);   // writes every log the mean then reads: nothing to clear
+    rivk::aq(st, (int)nb, base, lnv, steps);
+    RIV_HIP(hipGetLastError());
     return RIV_OK;
 }
 
