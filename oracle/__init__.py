@@ -91,8 +91,9 @@ def lib():
 
 
 def config(width, height, fps=30, intra_quality=85, inter_quality=80, search_range=12, skip_threshold=3,
-           keyframe_interval=1000, me_threads=0, zero_thr=0, pred_thr=0, lambda_mult=0.49, dead_zone=0.75):
-    return ROConfig(width, height, fps, 0, keyframe_interval, intra_quality, inter_quality, search_range, skip_threshold,
+           keyframe_interval=1000, me_threads=0, zero_thr=0, pred_thr=0, lambda_mult=0.49, dead_zone=0.75, keyframe_mode=0):
+    """keyframe_mode: 0 Fixed(keyframe_interval), 1 AllIntra, 2 Automatic (config.rs:5-19)."""
+    return ROConfig(width, height, fps, keyframe_mode, keyframe_interval, intra_quality, inter_quality, search_range, skip_threshold,
                     zero_thr, pred_thr, lambda_mult, dead_zone, me_threads)
 
 
