@@ -104,6 +104,9 @@ typedef struct {
     double host_entropy_ms;                   /* host residual rANS time of this frame (no reference counterpart) */
     double host_tail_ms;                      /* host ordered tail: MV rANS, packing, write (no reference counterpart) */
     uint32_t me_passes;                       /* relaxation passes of the motion search, 0 for intra (no reference counterpart) */
+    /* Host timeline of the frame, ms since the encoder was created (no reference counterpart):
+     * submitted (its encode call), its GPU work seen finished, its residuals coded, written. */
+    double t_submit_ms, t_gpu_done_ms, t_coded_ms, t_written_ms;
 } riv_frame_stats;
 
 typedef struct riv_encoder riv_encoder;

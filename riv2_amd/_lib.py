@@ -56,6 +56,7 @@ class FrameStatsC(Structure):
         ("rdo_forced_threshold", c_uint32), ("rdo_forced_disabled", c_uint32),
         ("gpu_ms", c_double), ("host_entropy_ms", c_double), ("host_tail_ms", c_double),
         ("me_passes", c_uint32),
+        ("t_submit_ms", c_double), ("t_gpu_done_ms", c_double), ("t_coded_ms", c_double), ("t_written_ms", c_double),
     ]
 
 

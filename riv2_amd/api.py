@@ -139,13 +139,16 @@ class EncodeFrameStats:
     host_entropy_ms: float
     host_tail_ms: float = 0.0
     me_passes: int = 0
+    # host timeline, ms since the encoder was created: submitted, GPU work seen done, coded, written
+    timeline: tuple = ()
 
     @staticmethod
     def from_c(s):
         return EncodeFrameStats(s.frame_index, s.timestamp_ms, s.frame_type, s.total_bytes, s.mv_bytes, s.residual_bytes,
                                 s.storage_width, s.storage_height, s.blocks_total, s.blocks_skipped, list(s.mv_mode_counts),
                                 s.mv_new_blocks, s.mv_bias_dx, s.mv_bias_dy, s.rdo_lambda, s.rdo_blocks_evaluated,
-                                s.rdo_blocks_selected_skip, s.gpu_ms, s.host_entropy_ms, s.host_tail_ms, s.me_passes)
+                                s.rdo_blocks_selected_skip, s.gpu_ms, s.host_entropy_ms, s.host_tail_ms, s.me_passes,
+                                (s.t_submit_ms, s.t_gpu_done_ms, s.t_coded_ms, s.t_written_ms))
 
 
 def encode_batch_step(encoders, frames, timestamps=None, on_device=False):

@@ -8,7 +8,10 @@
 #include <chrono>
 #include <cmath>
 #include <algorithm>
+#include <atomic>
 #include <condition_variable>
+#include <exception>
+#include <memory>
 #include <deque>
 #include <map>
 #include <tuple>
@@ -41,10 +44,12 @@ struct Slot {
     bool timed = false;      // t0 / t1 were recorded (stage profiling): each event costs the stream microseconds
     bool busy = false;       // holds a frame that is not yet written
     // host-stage products of the frame, passed from its coding job to the ordered tail
+    bool gpu_ready = false;  // the GPU work finished; waiting for the ordered MV modelling
     bool coded = false;      // the residual coding finished (products below are valid unless failed)
     bool modeled = false;    // the ordered MV modelling finished
     bool packed = false;     // the frame's bytes are assembled (or it failed)
-    bool failed = false;
+    bool failed = false;     // the GPU part failed (set before the host parts start)
+    bool res_failed = false, mv_failed = false;   // each written only by its own part
     std::vector<uint8_t> mvb, bytes;                 // MV payload, whole frame record
     std::unique_ptr<rivh::RansWriter> mvw;           // the MV symbols between modelling and packing
     riv_frame_stats st{};
@@ -54,6 +59,7 @@ struct Slot {
     double smark[RIV_NUM_STAGES] = {};
     uint32_t nmark[RIV_NUM_STAGES] = {};
     double gpu_ms = 0;
+    double t_submit = 0, t_done = 0;   // host timeline (ms since the encoder was created)
 };
 
 class Pool {
@@ -66,6 +72,50 @@ public:
         { std::lock_guard<std::mutex> lk(mu_); q_.push_back(std::move(f)); }
         cv_.notify_one();
     }
+    // Workers waiting for a job right now (a hint: it may change at once).
+    int idle() const { return idle_.load(std::memory_order_relaxed); }
+    // Runs fn(0) .. fn(n - 1) and returns when all have run: the calling thread takes tasks from
+    // a shared counter, and so do helpers queued at the front of the queue for the idle workers.
+    // A helper that starts after every task is taken returns without touching fn, so the call
+    // never waits on a task no thread holds, from a worker or from any other thread.
+    void parallel(int n, const std::function<void(int)>& fn) {
+        struct Job {
+            std::atomic<int> next{0}, done{0};
+            std::mutex mu;
+            std::condition_variable cv;
+            std::exception_ptr err;   // the first task's exception, rethrown by the caller
+        };
+        auto job = std::make_shared<Job>();
+        const std::function<void(int)>* f = &fn;
+        auto work = [job, n, f] {
+            for (;;) {
+                const int i = job->next.fetch_add(1);
+                if (i >= n) return;
+                try {
+                    (*f)(i);
+                } catch (...) {
+                    std::lock_guard<std::mutex> lk(job->mu);
+                    if (!job->err) job->err = std::current_exception();
+                }
+                if (job->done.fetch_add(1) + 1 == n) {
+                    std::lock_guard<std::mutex> lk(job->mu);
+                    job->cv.notify_all();
+                }
+            }
+        };
+        const int helpers = std::min(n - 1, idle());
+        if (helpers > 0) {
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                for (int h = 0; h < helpers; h++) q_.push_front(work);
+            }
+            if (helpers == 1) cv_.notify_one(); else cv_.notify_all();
+        }
+        work();
+        std::unique_lock<std::mutex> lk(job->mu);
+        job->cv.wait(lk, [&] { return job->done.load() == n; });
+        if (job->err) std::rethrow_exception(job->err);
+    }
     ~Pool() {
         { std::lock_guard<std::mutex> lk(mu_); stop_ = true; }
         cv_.notify_all();
@@ -77,7 +127,9 @@ private:
             std::function<void()> f;
             {
                 std::unique_lock<std::mutex> lk(mu_);
+                idle_.fetch_add(1, std::memory_order_relaxed);
                 cv_.wait(lk, [&] { return stop_ || !q_.empty(); });
+                idle_.fetch_sub(1, std::memory_order_relaxed);
                 if (stop_ && q_.empty()) return;
                 f = std::move(q_.front());
                 q_.pop_front();
@@ -90,6 +142,7 @@ private:
     std::mutex mu_;
     std::condition_variable cv_;
     bool stop_ = false;
+    std::atomic<int> idle_{0};
 };
 
 }  // namespace
@@ -202,6 +255,7 @@ struct riv_encoder {
     std::vector<Slot> slots;
     int depth = 0;                 // configured pipeline depth (0 = synchronous)
     HostStage* hs = nullptr;       // the process's host stage, while pipelined
+    HostStage* par = nullptr;      // the same, held by every encoder: its idle workers help code a frame
     hipEvent_t sync_ev = nullptr;  // orders this encoder's stream against a batch step's stream
     // batch-step scratch (held by the first encoder of a batch): field and barrier counters over all
     // streams of the batch, the stream descriptors and their pinned staging copies
@@ -239,6 +293,8 @@ struct riv_encoder {
     DevBuf prop, lstat;
 
     int pass_cap = 0;              // test seam: relaxation passes allowed per search (0: the proven bound)
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double now_ms() const { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
     int profiling = 0;             // 0 off, 1 motion-search stage only, 2 every stage
     double stage_ms[RIV_NUM_STAGES] = {};
     uint32_t stage_n[RIV_NUM_STAGES] = {};
@@ -368,7 +424,8 @@ static int encoder_init(riv_encoder* e) {
         RIV_HIP(hipEventCreateWithFlags(&s.t1, hipEventDisableSystemFence));
     }
     RIV_HIP(hipEventCreateWithFlags(&e->sync_ev, hipEventDisableTiming));
-    if (e->depth > 0) e->hs = acquire_host_stage();
+    e->par = acquire_host_stage();
+    if (e->depth > 0) e->hs = e->par;
     // MV-cost bit table (motion/scalar.rs:495-525) and lambda-derived costs
     rivhost::RdoModel rdo(c.inter_quality, c.rdo_lambda_mult, c.skip_threshold);
     e->lambda = rdo.lambda;
@@ -427,8 +484,11 @@ static void destroy_encoder(riv_encoder* e) {
         std::unique_lock<std::mutex> lk(e->mu);
         e->cv.wait(lk, [&] { return e->completed == e->frame_count; });
         lk.unlock();
-        release_host_stage();
         e->hs = nullptr;
+    }
+    if (e->par) {
+        release_host_stage();
+        e->par = nullptr;
     }
     if (e->st) {
         (void)hipStreamSynchronize(e->st);
@@ -497,7 +557,6 @@ static void compute_bias(const BlockRec* rec, int nb, int& bx_out, int& by_out) 
 
 // Host stage of one frame, part 1 (any order, any thread): residual rANS. The frame's GPU work
 // has finished. The products stay in the slot for the ordered tail.
-static void run_tails(riv_encoder* e, int si);
 // Device times of a finished frame, read once its events have completed (by the thread that
 // waited for them, so that the host-stage workers make no runtime calls).
 static void collect_gpu_times(Slot& s) {
@@ -510,21 +569,60 @@ static void collect_gpu_times(Slot& s) {
     }
 }
 
-static void code_frame(riv_encoder* e, int si) {
+// Host stage of one frame, once its GPU work has finished: two parts that run concurrently, then
+// the packing and the ordered write.
+//  * the residual coding (any order, any thread; its contexts are fresh every frame);
+//  * the MV modelling (frame order, one thread at a time: the MV contexts and candidate history
+//    carry from frame to frame), which needs only the block records, so it starts as soon as the
+//    GPU is done instead of after the residuals;
+// whichever of the two finishes second hands the frame to the packing (any thread), and the
+// writes run in frame order.
+static void frame_part_done(riv_encoder* e, int si, bool mv_part);
+
+static void code_residuals(riv_encoder* e, int si) {
     Slot& s = e->slots[si];
     const int nb = e->nb;
     auto t0 = std::chrono::steady_clock::now();
     riv_frame_stats& st = s.st;
-    st = riv_frame_stats{};
-    st.frame_index = s.index; st.timestamp_ms = s.ts;
-    st.storage_width = (uint32_t)e->W; st.storage_height = (uint32_t)e->H; st.blocks_total = (uint32_t)nb;
-    st.gpu_ms = s.gpu_ms;
     const uint8_t* pack = s.h_pack.as<uint8_t>();
-    s.res.clear(); s.blocks.clear(); s.bdx = s.bdy = 0;
+    s.res.clear();
     try {
-        if (s.intra) {
+        if (s.failed) {
+            // nothing to code: the frame is dropped at the write
+        } else if (s.intra) {
             std::vector<uint8_t> coded(nb, 1);
             s.res = rivh::encode_residuals_packed(pack, riv::PACK_STRIDE, coded.data(), nb, true);
+        } else {
+            const uint8_t* os = s.h_optskip.as<uint8_t>();
+            std::vector<uint8_t> coded(nb);
+            for (int i = 0; i < nb; i++) coded[i] = !os[i];
+            // with idle host workers the frame's coding is split across them (the same bytes; the
+            // split costs more work in all, so a pool busy with other frames codes it serially)
+            if (e->par && e->par->pool.idle() >= 2)
+                s.res = rivh::encode_residuals_split_par(s.h_main.as<uint8_t>(), pack, coded.data(), nb,
+                                                         [e](int n, const std::function<void(int)>& f) { e->par->pool.parallel(n, f); });
+            else
+                s.res = rivh::encode_residuals_split(s.h_main.as<uint8_t>(), pack, coded.data(), nb);
+        }
+    } catch (const std::exception& ex) {
+        s.res_failed = true;
+        e->fail(RIV_E_ENCODING_FAILED, std::string("entropy coding failed: ") + ex.what());
+    }
+    st.host_entropy_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    st.t_coded_ms = e->now_ms();
+    frame_part_done(e, si, false);
+}
+
+// The MV part of frame s: the block list from the records (NEW deltas after the frame bias,
+// encoder.rs:775-818, bias by compute_delta_bias_for_new, :984-1048), then the MV symbols and
+// history update (mv_rans/encoder.rs; contexts and history restart at every I-frame,
+// encoder.rs:335-337).
+static int model_frame(riv_encoder* e, Slot& s) {
+    riv_frame_stats& st = s.st;
+    const int nb = e->nb;
+    try {
+        if (s.intra) {
+            e->mvc.reset();
             st.frame_type = 1;
         } else {
             const BlockRec* hr = s.h_rec.as<BlockRec>();
@@ -532,48 +630,28 @@ static void code_frame(riv_encoder* e, int si) {
             int bdx = 0, bdy = 0;
             compute_bias(hr, nb, bdx, bdy);
             s.blocks.resize(nb);
-            std::vector<uint8_t> coded(nb);
             for (int i = 0; i < nb; i++) {
                 rivh::MvBlock& b = s.blocks[i];
                 b.mode = hr[i].mode; b.new_base = hr[i].new_base;
                 b.dx = b.dy = 0; b.sx = b.sy = 0;
-                if (b.mode == riv::M_NEW) {   // delta after the frame bias (encoder.rs:775-818)
+                if (b.mode == riv::M_NEW) {
                     b.dx = (int8_t)std::max(-128, std::min(127, (int)hr[i].raw_dx - bdx));
                     b.dy = (int8_t)std::max(-128, std::min(127, (int)hr[i].raw_dy - bdy));
                     b.sx = hr[i].s[2]; b.sy = hr[i].s[3];
                     st.mv_new_blocks++;
                 }
                 b.skip = os[i] != 0;
-                coded[i] = !os[i];
                 st.blocks_skipped += b.skip;
                 st.mv_mode_counts[b.mode]++;
                 st.rdo_blocks_selected_skip += hr[i].rdo_skip;
                 if (hr[i].s[2] || hr[i].s[3]) st.rdo_forced_fractional++;
             }
-            s.res = rivh::encode_residuals_split(s.h_main.as<uint8_t>(), pack, coded.data(), nb);
             s.bdx = bdx; s.bdy = bdy;
             st.frame_type = 2;
             st.mv_bias_dx = bdx; st.mv_bias_dy = bdy;
             st.rdo_lambda = e->lambda; st.rdo_blocks_evaluated = (uint32_t)nb;
             if (e->cfg.skip_threshold == 0) st.rdo_forced_disabled = (uint32_t)nb;
             st.me_passes = (uint32_t)*s.h_passes.as<int>();
-        }
-        s.failed = false;
-    } catch (const std::exception& ex) {
-        s.failed = true;
-        e->fail(RIV_E_ENCODING_FAILED, std::string("entropy coding failed: ") + ex.what());
-    }
-    st.host_entropy_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    run_tails(e, si);
-}
-
-// Host stage, part 2 (frame order, one thread at a time): the MV symbols and history update
-// (mv_rans/encoder.rs; contexts and history carry across frames, and restart at I-frames).
-static int model_frame(riv_encoder* e, Slot& s) {
-    try {
-        if (s.intra) {
-            e->mvc.reset();   // MV contexts restart at every I-frame (encoder.rs:335-337)
-        } else {
             s.mvb.clear();
             s.mvw.reset(new rivh::RansWriter(s.mvb));
             e->mvc.model(s.blocks, e->bw, e->bh, *s.mvw);
@@ -615,43 +693,74 @@ static void run_writes(riv_encoder* e, int si);
 static void pack_then_write(riv_encoder* e, int si) {
     Slot& s = e->slots[si];
     const auto t0 = std::chrono::steady_clock::now();
-    if (!s.failed) pack_frame(e, s);
+    bool ok;
+    {
+        std::lock_guard<std::mutex> lk(e->mu);
+        ok = !s.failed && !s.res_failed && !s.mv_failed && !e->err_code;
+    }
+    if (ok) pack_frame(e, s);
     s.st.host_tail_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     run_writes(e, si);
 }
 
-// Marks slot si coded, then, unless another thread is already doing so, models every ready frame
-// in frame order; each modelled frame goes on to packing. The modelling runs outside the lock.
-static void run_tails(riv_encoder* e, int si) {
+// One of the frame's two parts finished: the second one sends the frame to the packing.
+static void frame_part_done(riv_encoder* e, int si, bool mv_part) {
+    Slot& s = e->slots[si];
+    bool both;
+    {
+        std::lock_guard<std::mutex> lk(e->mu);
+        (mv_part ? s.modeled : s.coded) = true;
+        both = s.coded && s.modeled;
+    }
+    if (!both) return;
+    if (e->hs) e->hs->pool.submit([e, si] { pack_then_write(e, si); });
+    else pack_then_write(e, si);
+}
+
+// Marks slot si's GPU work done, then, unless another thread is already doing so, models the MV
+// part of every such frame in frame order. The modelling runs outside the lock.
+static void run_mv(riv_encoder* e, int si) {
     std::unique_lock<std::mutex> lk(e->mu);
-    e->slots[si].coded = true;
+    e->slots[si].gpu_ready = true;
     if (e->tail_active) return;
     e->tail_active = true;
     for (;;) {
         const int ti = (int)(e->next_tail % e->slots.size());
         Slot& t = e->slots[ti];
-        if (!t.busy || !t.coded || t.index != e->next_tail) break;
-        t.coded = false;
+        if (!t.busy || !t.gpu_ready || t.index != e->next_tail) break;
+        t.gpu_ready = false;
         const bool skip = t.failed || e->err_code;
         lk.unlock();
         const auto t0 = std::chrono::steady_clock::now();
-        if (!skip && model_frame(e, t) != RIV_OK) t.failed = true;
-        if (skip) t.failed = true;
-        t.st.host_tail_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (skip || model_frame(e, t) != RIV_OK) t.mv_failed = true;
+        t.st.host_tail_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         lk.lock();
-        t.modeled = true;
         e->next_tail++;
-        if (e->hs) {
-            lk.unlock();
-            e->hs->pool.submit([e, ti] { pack_then_write(e, ti); });
-            lk.lock();
-        } else {
-            lk.unlock();
-            pack_then_write(e, ti);
-            lk.lock();
-        }
+        lk.unlock();
+        frame_part_done(e, ti, true);
+        lk.lock();
     }
     e->tail_active = false;
+}
+
+// A frame whose GPU work has finished (or failed, when `failed`): its stats start afresh and both
+// parts are started, on the pool when pipelined, else on the calling thread.
+static void start_host_parts(riv_encoder* e, int si, bool failed) {
+    Slot& s = e->slots[si];
+    riv_frame_stats& st = s.st;
+    st = riv_frame_stats{};
+    st.frame_index = s.index; st.timestamp_ms = s.ts;
+    st.storage_width = (uint32_t)e->W; st.storage_height = (uint32_t)e->H; st.blocks_total = (uint32_t)e->nb;
+    st.gpu_ms = s.gpu_ms;
+    st.t_submit_ms = s.t_submit; st.t_gpu_done_ms = s.t_done;
+    if (failed) s.failed = true;
+    if (e->hs) {
+        e->hs->pool.submit([e, si] { run_mv(e, si); });
+        e->hs->pool.submit([e, si] { code_residuals(e, si); });
+    } else {
+        run_mv(e, si);
+        code_residuals(e, si);
+    }
 }
 
 // Marks slot si packed, then, unless another thread is already doing so, writes every packed
@@ -666,7 +775,7 @@ static void run_writes(riv_encoder* e, int si) {
         Slot& t = e->slots[ti];
         if (!t.busy || !t.packed || t.index != e->next_write) break;
         t.packed = false;
-        const bool ok = !t.failed && !e->err_code;
+        const bool ok = !t.failed && !t.res_failed && !t.mv_failed && !e->err_code;
         lk.unlock();
         int r = RIV_OK;
         if (ok) {
@@ -675,6 +784,7 @@ static void run_writes(riv_encoder* e, int si) {
         }
         lk.lock();
         if (ok && r == RIV_OK) {
+            t.st.t_written_ms = e->now_ms();
             if (e->profiling)
                 for (int k = 0; k < RIV_NUM_STAGES; k++) { e->stage_ms[k] += t.smark[k]; e->stage_n[k] += t.nmark[k]; }
             e->done_stats.push_back(t.st);
@@ -706,18 +816,17 @@ static void waiter_loop(HostStage* hs) {
         (void)hipSetDevice(e->cfg.device_id);
         if (hipEventSynchronize(e->slots[si].done) != hipSuccess) {
             e->fail(RIV_E_GPU, "GPU frame failed");
-            e->slots[si].failed = true;
-            run_tails(e, si);
+            start_host_parts(e, si, true);
             continue;
         }
+        e->slots[si].t_done = e->now_ms();
         if (const char* msg = search_error(e->slots[si])) {   // no bytes for a frame whose search failed
             e->fail(RIV_E_GPU, msg);
-            e->slots[si].failed = true;
-            run_tails(e, si);
+            start_host_parts(e, si, true);
             continue;
         }
         collect_gpu_times(e->slots[si]);
-        hs->pool.submit([e, si] { code_frame(e, si); });
+        start_host_parts(e, si, false);
     }
 }
 
@@ -792,9 +901,11 @@ static int frame_begin(riv_encoder* e, const void* rgb, bool on_device, size_t l
         e->cv.wait(lk, [&] { return !s.busy || e->err_code; });
         if (e->err_code) { set_error(e->err_msg); return e->err_code; }
         s.index = e->frame_count; s.ts = ts; s.intra = intra; s.busy = true; s.coded = false; s.modeled = false; s.packed = false;
-        s.failed = false;
+        s.gpu_ready = false;
+        s.failed = false; s.res_failed = false; s.mv_failed = false;
     }
     s.h_passes.as<int>()[1] = 0;   // the search's error word: the kernel writes it only on failure
+    s.t_submit = e->now_ms();
     s.n_marks = 0;
     if (e->profiling && !s.marks[0])
         for (auto& m : s.marks) RIV_HIP(hipEventCreateWithFlags(&m, hipEventDisableSystemFence));
@@ -889,9 +1000,9 @@ static int frame_end(riv_encoder* e, const FrameCtx& f, riv_frame_stats* stats) 
         if (stats) { std::memset(stats, 0, sizeof(*stats)); stats->frame_index = s.index; stats->timestamp_ms = s.ts; stats->frame_type = f.intra ? 1 : 2; }
         return RIV_OK;
     }
-    if (hipEventSynchronize(s.done) != hipSuccess) { e->fail(RIV_E_GPU, "GPU frame failed"); s.failed = true; run_tails(e, f.si); }
-    else if (const char* msg = search_error(s)) { e->fail(RIV_E_GPU, msg); s.failed = true; run_tails(e, f.si); }
-    else { collect_gpu_times(s); code_frame(e, f.si); }
+    if (hipEventSynchronize(s.done) != hipSuccess) { e->fail(RIV_E_GPU, "GPU frame failed"); start_host_parts(e, f.si, true); }
+    else if (const char* msg = search_error(s)) { e->fail(RIV_E_GPU, msg); start_host_parts(e, f.si, true); }
+    else { s.t_done = e->now_ms(); collect_gpu_times(s); start_host_parts(e, f.si, false); }
     std::lock_guard<std::mutex> lk(e->mu);
     if (e->err_code) { set_error(e->err_msg); return e->err_code; }
     if (stats && !e->done_stats.empty()) *stats = e->done_stats.back();

@@ -6,6 +6,9 @@
 #include <cstring>
 #include <cstdlib>
 #include <immintrin.h>
+#include <memory>
+#include <mutex>
+#include <string>
 #include <type_traits>
 
 namespace rivh {
@@ -239,6 +242,15 @@ void RansWriter::flush() {
     } while (b < n_);
     n_ = 0;
     emitted_ = false;
+}
+
+void RansWriter::code_symbols(const uint16_t* sym, size_t n) {
+    if (n_) throw std::logic_error("code_symbols on a writer with buffered symbols");
+    // the blocks of flush(), read from the caller's array (flush only reads sym_)
+    sym_ = const_cast<uint16_t*>(sym);
+    n_ = n; cap_ = n;
+    flush();
+    sym_ = buf_.data(); cap_ = buf_.size();
 }
 
 RansReader::RansReader(const uint8_t* data, size_t len) : d_(data), len_(len) { a_ = state(); b_ = state(); }
@@ -529,6 +541,180 @@ inline const int16_t* split_record(const uint8_t* main, const uint8_t* over, int
 
 std::vector<uint8_t> encode_residuals_split(const uint8_t* main, const uint8_t* over, const uint8_t* coded, int nb) {
     return encode_inter_records(coded, nb, [&](int b, uint64_t* m) { return split_record(main, over, b, m); });
+}
+
+namespace {
+// One position range of the parallel coder: a plane kind (luma, or chroma U then V), positions
+// [lo, hi), and whether it codes the plane's EOB class value first.
+struct CoefRange { bool luma; int lo, hi; bool eob; };
+constexpr int kRanges = 5;
+constexpr CoefRange kCoefRanges[kRanges] = {{true, 0, 16, true}, {true, 16, 64, false}, {true, 64, 256, false},
+                                            {false, 0, 8, true}, {false, 8, 64, false}};
+// Segments per coded MB in the merged order: Y ranges 0, 1, 2; U ranges 3, 4; V ranges 3, 4.
+constexpr int kSegs = 7;
+constexpr int kSegTask[kSegs] = {0, 1, 2, 3, 4, 3, 4};
+constexpr int kSegPart[kSegs] = {0, 0, 0, 0, 0, 1, 1};   // chroma tasks write U (0) then V (1) per MB
+
+// bits lo .. lo + 64 * nout - 1 of an nw-word mask, as nout words
+inline void shift_mask(const uint64_t* m, int nw, int lo, uint64_t* out, int nout) {
+    for (int k = 0; k < nout; k++) {
+        const int bit = lo + 64 * k, w = bit >> 6, sh = bit & 63;
+        const uint64_t a = w < nw ? m[w] >> sh : 0u;
+        const uint64_t b = (sh && w + 1 < nw) ? m[w + 1] << (64 - sh) : 0u;
+        out[k] = a | b;
+    }
+}
+
+// Positions lo .. min(last, hi - 1) of one plane (EOB first when r.eob), in put_plane's order;
+// vals: the plane's values from its first non-zero at or after lo.
+void put_range(uint16_t*& o, ResidualCtx& c, const CoefRange& r, const uint64_t* mask, int nw, int last, const int16_t* vals) {
+    if (r.eob) emit_class_value(o, last, r.luma ? c.eob_y : c.eob_c, r.luma ? 9 : 7);
+    if (last < r.lo) return;
+    const int end = std::min(last, r.hi - 1) - r.lo;   // last position of the range, relative to lo
+    uint64_t sm[4];
+    shift_mask(mask, nw, r.lo, sm, (end >> 6) + 1);
+    uint16_t sym[256 + 16];
+    Ctx* sig = (r.luma ? c.sig_y : c.sig_c) + r.lo;
+    if (g_avx2) sig_symbols_avx2(sig, sm, end, sym);
+    else sig_symbols_scalar(sig, sm, end, sym);
+    int prev = 0;
+    for (int k = 0; 64 * k <= end; k++) {
+        uint64_t m = sm[k];
+        if (64 * k + 63 > end) m &= end - 64 * k == 63 ? ~0ull : ((2ull << (end - 64 * k)) - 1u);
+        for (; m; m &= m - 1) {
+            const int i = 64 * k + __builtin_ctzll(m);
+            std::memcpy(o, sym + prev, sizeof(uint16_t) * (size_t)(i + 1 - prev));
+            o += i + 1 - prev;
+            prev = i + 1;
+            const int v = *vals++;
+            const int pos = i + r.lo;
+            emit_bypass(o, v < 0);
+            emit_class_value(o, std::abs(v) - 1, r.luma ? c.mag_y[band_y(pos)] : c.mag_c[band_c(pos)], 13);
+        }
+    }
+    std::memcpy(o, sym + prev, sizeof(uint16_t) * (size_t)(end + 1 - prev));
+    o += end + 1 - prev;
+}
+
+inline int plane_last(const uint64_t* m, int nw) {
+    for (int k = nw - 1; k >= 0; k--) if (m[k]) return 64 * k + 63 - __builtin_clzll(m[k]);
+    return 0;
+}
+inline int popcount_below(const uint64_t* m, int lo) {
+    int n = 0;
+    for (int k = 0; k < (lo >> 6); k++) n += __builtin_popcountll(m[k]);
+    if (lo & 63) n += __builtin_popcountll(m[lo >> 6] & ((1ull << (lo & 63)) - 1u));
+    return n;
+}
+
+// Buffers of one parallel coding, reused from call to call.
+struct ParScratch {
+    std::vector<int> mbs;                       // coded MBs in raster order
+    std::vector<uint16_t> sym[kRanges];         // each range task's symbols
+    std::vector<uint32_t> seg[kRanges];         // each task's segment ends: one per MB (luma), two (chroma)
+    std::vector<uint32_t> dst;                  // merged start of each coded MB
+    std::vector<uint16_t> merged;
+    std::vector<std::vector<uint8_t>> chunk;    // each rANS group's bytes
+};
+std::mutex g_par_mu;
+std::vector<std::unique_ptr<ParScratch>> g_par_free;
+}  // namespace
+
+std::vector<uint8_t> encode_residuals_split_par(const uint8_t* main, const uint8_t* over, const uint8_t* coded, int nb,
+                                                const ParallelFor& pfor) {
+    std::unique_ptr<ParScratch> S;
+    {
+        std::lock_guard<std::mutex> lk(g_par_mu);
+        if (!g_par_free.empty()) { S = std::move(g_par_free.back()); g_par_free.pop_back(); }
+    }
+    if (!S) S.reset(new ParScratch());
+    S->mbs.clear();
+    for (int b = 0; b < nb; b++) if (coded[b]) S->mbs.push_back(b);
+    std::vector<uint8_t> out;
+    const int n = (int)S->mbs.size();
+    if (n == 0) {   // no coder instantiated when every block is skipped (residual.rs:670-674, :792-798)
+        std::lock_guard<std::mutex> lk(g_par_mu);
+        g_par_free.push_back(std::move(S));
+        return out;
+    }
+    // 1. each range over every coded MB, with its own contexts
+    std::vector<std::string> errs(kRanges);
+    pfor(kRanges, [&](int t) {
+        try {
+            const CoefRange& r = kCoefRanges[t];
+            ResidualCtx c;
+            std::vector<uint16_t>& buf = S->sym[t];
+            std::vector<uint32_t>& seg = S->seg[t];
+            seg.resize((size_t)n * (r.luma ? 1 : 2));
+            const size_t bound = (size_t)(r.hi - r.lo) * 27 + 32;   // symbols of one plane's range at most
+            if (buf.size() < 4 * bound) buf.resize(4 * bound);
+            size_t used = 0;
+            for (int k = 0; k < n; k++) {
+                uint64_t m[6];
+                const int16_t* vals = split_record(main, over, S->mbs[k], m);
+                const int ny = __builtin_popcountll(m[0]) + __builtin_popcountll(m[1]) + __builtin_popcountll(m[2]) +
+                               __builtin_popcountll(m[3]);
+                for (int part = 0; part < (r.luma ? 1 : 2); part++) {
+                    if (buf.size() - used < bound) buf.resize(std::max(2 * buf.size(), used + bound));
+                    uint16_t* o = buf.data() + used;
+                    const uint64_t* pm = r.luma ? m : m + 4 + part;
+                    const int nw = r.luma ? 4 : 1;
+                    const int16_t* pv = r.luma ? vals : vals + ny + (part ? __builtin_popcountll(m[4]) : 0);
+                    put_range(o, c, r, pm, nw, plane_last(pm, nw), pv + popcount_below(pm, r.lo));
+                    used = (size_t)(o - buf.data());
+                    seg[(size_t)k * (r.luma ? 1 : 2) + part] = (uint32_t)used;
+                }
+            }
+        } catch (const std::exception& ex) {
+            errs[t] = ex.what();
+        }
+    });
+    for (const std::string& e : errs) if (!e.empty()) throw CorruptStream(e);
+    // 2. the merged order: per MB the seven segments; starts by prefix sum, copies in parallel
+    auto seg_bounds = [&](int k, int s, uint32_t& a, uint32_t& b) {
+        const int t = kSegTask[s], per = kCoefRanges[t].luma ? 1 : 2, i = k * per + kSegPart[s];
+        const std::vector<uint32_t>& e = S->seg[t];
+        a = i ? e[(size_t)i - 1] : 0u;
+        b = e[(size_t)i];
+    };
+    S->dst.resize((size_t)n + 1);
+    uint32_t total = 0;
+    for (int k = 0; k < n; k++) {
+        S->dst[k] = total;
+        for (int s = 0; s < kSegs; s++) { uint32_t a, b; seg_bounds(k, s, a, b); total += b - a; }
+    }
+    S->dst[n] = total;
+    if (S->merged.size() < total) S->merged.resize(total);
+    const int nparts = std::min(8, n);
+    pfor(nparts, [&](int p) {
+        for (int k = (int)((int64_t)n * p / nparts); k < (int)((int64_t)n * (p + 1) / nparts); k++) {
+            uint16_t* d = S->merged.data() + S->dst[k];
+            for (int s = 0; s < kSegs; s++) {
+                uint32_t a, b;
+                seg_bounds(k, s, a, b);
+                std::memcpy(d, S->sym[kSegTask[s]].data() + a, sizeof(uint16_t) * (b - a));
+                d += b - a;
+            }
+        }
+    });
+    // 3. rANS: groups of eight blocks (the streams' blocks are independent), then the bytes in order
+    constexpr size_t kGroup = 8 * 16386;
+    const int ng = (int)((total + kGroup - 1) / kGroup);
+    if ((int)S->chunk.size() < ng) S->chunk.resize(ng);
+    pfor(ng, [&](int g) {
+        std::vector<uint8_t>& o = S->chunk[g];
+        o.clear();
+        RansWriter w(o);
+        const size_t b = (size_t)g * kGroup;
+        w.code_symbols(S->merged.data() + b, std::min(kGroup, (size_t)total - b));
+    });
+    size_t bytes = 0;
+    for (int g = 0; g < ng; g++) bytes += S->chunk[g].size();
+    out.reserve(bytes);
+    for (int g = 0; g < ng; g++) out.insert(out.end(), S->chunk[g].begin(), S->chunk[g].end());
+    std::lock_guard<std::mutex> lk(g_par_mu);
+    g_par_free.push_back(std::move(S));
+    return out;
 }
 
 void unpack_split(const uint8_t* main, const uint8_t* over, int b, int16_t* q) {

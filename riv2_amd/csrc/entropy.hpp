@@ -4,6 +4,7 @@
 #pragma once
 #include <cstdint>
 #include <cstddef>
+#include <functional>
 #include <vector>
 #include <array>
 #include <stdexcept>
@@ -67,6 +68,9 @@ public:
     }
     void commit(const uint16_t* end) { n_ = (size_t)(end - sym_); }
     void flush();
+    // Codes n symbols held by the caller (start << 8 | freq each) as a whole stream: the blocks
+    // flush() would code, appended to the output; nothing may be buffered.
+    void code_symbols(const uint16_t* sym, size_t n);
     // Fills the calling thread's buffer pool (when empty) with buffers for `symbols` symbols, their
     // pages touched, so that the thread's first frames do not pay for growing them.
     static void prewarm(size_t symbols);
@@ -122,6 +126,18 @@ std::vector<uint8_t> encode_residuals_packed(const uint8_t* pack, size_t stride,
 // The inter form from split records: MB b's masks at main + b * PACK_MAIN, its values after them
 // when there are at most PACK_MAIN_VALUES, else at over + b * PACK_STRIDE + 48 (coded MBs only).
 std::vector<uint8_t> encode_residuals_split(const uint8_t* main, const uint8_t* over, const uint8_t* coded, int nb);
+// Runs fn(0) .. fn(n - 1), possibly on several threads at once, and returns when all have run.
+using ParallelFor = std::function<void(int n, const std::function<void(int)>& fn)>;
+// encode_residuals_split, modelled and coded by parallel tasks, byte for byte the same stream.
+// The luma and chroma contexts are disjoint, and within a plane so are the significance and
+// magnitude contexts of the position ranges [0,16), [16,64), [64,256) (luma) and [0,8), [8,64)
+// (chroma), whose magnitude bands do not straddle them; so each range is modelled over every
+// coded MB by its own task (the EOB contexts go with the first range). The tasks' symbols are
+// interleaved back into the coder's order (per MB: Y, U, V, each plane's ranges in position
+// order), and the stream's 16386-symbol rANS blocks, each coded from fresh states, are coded in
+// parallel groups.
+std::vector<uint8_t> encode_residuals_split_par(const uint8_t* main, const uint8_t* over, const uint8_t* coded, int nb,
+                                                const ParallelFor& pfor);
 // One MB of the split form as 384 coefficients.
 void unpack_split(const uint8_t* main, const uint8_t* over, int b, int16_t* q);
 void unpack_coeffs(const uint8_t* slot, int16_t* q384);
