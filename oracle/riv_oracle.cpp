@@ -1456,6 +1456,103 @@ struct ro_encoder {
     std::vector<uint8_t> last_frame_bytes;
 };
 
+
+// Mode decision over the reconstructed prefix, in raster order (encoder.rs:447-733): mode by
+// exact half-pel equality in the order Zero, Nearest, Near, TopRight, TopLeft, Temporal, else NEW;
+// the NEW base by estimated bits (encoder.rs:128-159, :567-661); the reconstructed MV keeps the
+// predictor's representation (:667-718).
+static void decide_modes(const std::vector<MV>& smv, const std::vector<MV>* pm, int bw, int bh, std::vector<uint8_t>& modes,
+                         std::vector<uint8_t>& new_base, std::vector<int8_t>& raw_dx, std::vector<int8_t>& raw_dy, std::vector<MV>& coded) {
+    const int nb = bw * bh;
+    coded.clear(); coded.reserve(nb);
+    modes.assign(nb, 0); new_base.assign(nb, 0); raw_dx.assign(nb, 0); raw_dy.assign(nb, 0);
+    for (int i = 0; i < nb; i++) {
+        int bx = i % bw, by = i / bw;
+        const MV& mv = smv[i];
+        Preds p = derive_predictors(coded, pm, bw, bh, bx, by);
+        Neigh n = gather_neighbors(coded, pm, bw, bh, bx, by);
+        bool zero_el = mv.dx == 0 && mv.dy == 0 && mv.sx == 0 && mv.sy == 0;
+        uint8_t mode;
+        auto hp = [](const Q4& q, int& x, int& y) { x = q.dx * 2 + q.sx; y = q.dy * 2 + q.sy; };
+        if (zero_el) mode = ZERO;
+        else {
+            int mx = mv.dx_hp(), my = mv.dy_hp(), ax, ay;
+            hp(p.nearest, ax, ay);
+            if (mx == ax && my == ay) mode = NEAREST;
+            else {
+                hp(p.near, ax, ay);
+                if (mx == ax && my == ay) mode = NEAR;
+                else {
+                    bool trm = false, tlm = false;
+                    if (n.htr) { hp(n.tr, ax, ay); trm = mx == ax && my == ay; }
+                    if (trm) mode = TOPRIGHT;
+                    else {
+                        if (n.htl) { hp(n.tl, ax, ay); tlm = mx == ax && my == ay; }
+                        if (tlm) mode = TOPLEFT;
+                        else { hp(p.temporal, ax, ay); mode = (mx == ax && my == ay) ? TEMPORAL : NEWMV; }
+                    }
+                }
+            }
+        }
+        uint8_t chosen = 0; int8_t rdx = 0, rdy = 0;
+        if (mode == NEWMV) {
+            int adx = mv.dx, ady = mv.dy;
+            bool have[5] = {true, true, n.htr, n.htl, true};
+            int bxs[5] = {p.nearest.dx, p.near.dx, n.tr.dx, n.tl.dx, p.temporal.dx};
+            int bys[5] = {p.nearest.dy, p.near.dy, n.tr.dy, n.tl.dy, p.temporal.dy};
+            static const uint32_t selbits[5] = {1, 2, 3, 4, 4};
+            uint32_t best = UINT32_MAX; int bi = 0; int bbx = p.nearest.dx, bby = p.nearest.dy;
+            for (int k = 0; k < 5; k++) {
+                if (!have[k]) continue;
+                int8_t ddx = (int8_t)clampv(adx - bxs[k], -128, 127), ddy = (int8_t)clampv(ady - bys[k], -128, 127);
+                auto cbits = [](int8_t d) { int cl = mv_class_from_magnitude(std::abs((int)d)); return cl == 0 ? 4u : 4u + (uint32_t)cl; };
+                uint32_t cost = selbits[k] + cbits(ddx) + cbits(ddy);
+                if (cost < best) { best = cost; bi = k; bbx = bxs[k]; bby = bys[k]; }
+            }
+            chosen = (uint8_t)bi;
+            rdx = (int8_t)clampv(adx - bbx, -128, 127); rdy = (int8_t)clampv(ady - bby, -128, 127);
+        }
+        modes[i] = mode; new_base[i] = chosen; raw_dx[i] = rdx; raw_dy[i] = rdy;
+        MV rec = mv;
+        auto from = [&](const Q4& q) { return MV::make(q.dx, q.dy, q.sx, q.sy, mv.skip); };
+        if (mode == NEAREST) rec = from(p.nearest);
+        else if (mode == NEAR) rec = from(p.near);
+        else if (mode == TOPRIGHT) { if (n.htr) rec = from(n.tr); }
+        else if (mode == TOPLEFT) { if (n.htl) rec = from(n.tl); }
+        else if (mode == TEMPORAL) rec = from(p.temporal);
+        else if (mode == NEWMV) {
+            Q4 base = chosen == 0 ? p.nearest : chosen == 1 ? p.near : chosen == 2 ? (n.htr ? n.tr : p.nearest)
+                      : chosen == 3 ? (n.htl ? n.tl : p.nearest) : p.temporal;
+            rec.dx = (int8_t)clampv((int)base.dx + rdx, -128, 127);
+            rec.dy = (int8_t)clampv((int)base.dy + rdy, -128, 127);
+        }
+        coded.push_back(rec);
+    }
+}
+
+// compute_delta_bias_for_new / compute_bias_inner (encoder.rs:984-1048)
+static void new_delta_bias(const std::vector<uint8_t>& modes, const std::vector<int8_t>& rdx, const std::vector<int8_t>& rdy,
+                           int& bias_dx, int& bias_dy) {
+    bias_dx = 0; bias_dy = 0;
+    std::vector<std::pair<int8_t, int8_t>> f;
+    for (size_t i = 0; i < modes.size(); i++) if (modes[i] == NEWMV) f.push_back({rdx[i], rdy[i]});
+    if (f.empty()) return;
+    int64_t bzx = -1, bzy = -1, bcx = INT64_MAX, bcy = INT64_MAX;
+    for (int b = -24; b <= 24; b++) {
+        int64_t zx = 0, cx = 0, zy = 0, cy = 0;
+        for (auto& d : f) {
+            int8_t ax = (int8_t)clampv((int)d.first - b, -128, 127);
+            if (ax == 0) zx++;
+            cx += std::abs((int)ax);
+            int8_t ay = (int8_t)clampv((int)d.second - b, -128, 127);
+            if (ay == 0) zy++;
+            cy += std::abs((int)ay);
+        }
+        if (zx > bzx || (zx == bzx && cx < bcx)) { bzx = zx; bcx = cx; bias_dx = b; }
+        if (zy > bzy || (zy == bzy && cy < bcy)) { bzy = zy; bcy = cy; bias_dy = b; }
+    }
+}
+
 static int me_frame_encode(ro_encoder* e, const uint8_t* rgb, uint64_t ts, ro_frame_stats* st) {
     const ro_config& c = e->cfg;
     int SW = (int)e->hdr.sw, SH = (int)e->hdr.sh, bw = SW / 16, bh = SH / 16, nb = bw * bh;
@@ -1491,77 +1588,18 @@ static int me_frame_encode(ro_encoder* e, const uint8_t* rgb, uint64_t ts, ro_fr
                    lam.lambda, c.me_threads, e->search_mvs, e->sads);
         build_predicted(prev, e->search_mvs, e->pred);
         // mode decision  (encoder.rs:447-733)
-        std::vector<MV> coded; coded.reserve(nb);
-        e->modes.assign(nb, 0); e->new_base.assign(nb, 0); e->rdo_skip.assign(nb, 0); e->raw_dx.assign(nb, 0); e->raw_dy.assign(nb, 0);
+        std::vector<MV> coded;
+        e->rdo_skip.assign(nb, 0);
         std::vector<uint8_t> flags(nb, 0);
         uint32_t mode_counts[7] = {0};
+        decide_modes(e->search_mvs, pm, bw, bh, e->modes, e->new_base, e->raw_dx, e->raw_dy, coded);
         for (int i = 0; i < nb; i++) {
-            int bx = i % bw, by = i / bw;
             const MV& mv = e->search_mvs[i];
             bool ia = mv.sx == 0 && mv.sy == 0;
             bool skipc = rdo_decide(lam, c.skip_threshold, e->sads[i], ia);
             e->rdo_skip[i] = skipc;
-            Preds p = derive_predictors(coded, pm, bw, bh, bx, by);
-            Neigh n = gather_neighbors(coded, pm, bw, bh, bx, by);
-            bool zero_el = mv.dx == 0 && mv.dy == 0 && mv.sx == 0 && mv.sy == 0;
-            uint8_t mode;
-            auto hp = [](const Q4& q, int& x, int& y) { x = q.dx * 2 + q.sx; y = q.dy * 2 + q.sy; };
-            if (zero_el) mode = ZERO;
-            else {
-                int mx = mv.dx_hp(), my = mv.dy_hp(), ax, ay;
-                hp(p.nearest, ax, ay);
-                if (mx == ax && my == ay) mode = NEAREST;
-                else {
-                    hp(p.near, ax, ay);
-                    if (mx == ax && my == ay) mode = NEAR;
-                    else {
-                        bool trm = false, tlm = false;
-                        if (n.htr) { hp(n.tr, ax, ay); trm = mx == ax && my == ay; }
-                        if (trm) mode = TOPRIGHT;
-                        else {
-                            if (n.htl) { hp(n.tl, ax, ay); tlm = mx == ax && my == ay; }
-                            if (tlm) mode = TOPLEFT;
-                            else { hp(p.temporal, ax, ay); mode = (mx == ax && my == ay) ? TEMPORAL : NEWMV; }
-                        }
-                    }
-                }
-            }
-            uint8_t chosen = 0; int8_t rdx = 0, rdy = 0;
-            if (mode == NEWMV) {   // NEW base by estimated bits (encoder.rs:567-661, :128-159)
-                int adx = mv.dx, ady = mv.dy;
-                bool have[5] = {true, true, n.htr, n.htl, true};
-                int bxs[5] = {p.nearest.dx, p.near.dx, n.tr.dx, n.tl.dx, p.temporal.dx};
-                int bys[5] = {p.nearest.dy, p.near.dy, n.tr.dy, n.tl.dy, p.temporal.dy};
-                static const uint32_t selbits[5] = {1, 2, 3, 4, 4};
-                uint32_t best = UINT32_MAX; int bi = 0; int bbx = p.nearest.dx, bby = p.nearest.dy;
-                for (int k = 0; k < 5; k++) {
-                    if (!have[k]) continue;
-                    int8_t ddx = (int8_t)clampv(adx - bxs[k], -128, 127), ddy = (int8_t)clampv(ady - bys[k], -128, 127);
-                    auto cbits = [](int8_t d) { int cl = mv_class_from_magnitude(std::abs((int)d)); return cl == 0 ? 4u : 4u + (uint32_t)cl; };
-                    uint32_t cost = selbits[k] + cbits(ddx) + cbits(ddy);
-                    if (cost < best) { best = cost; bi = k; bbx = bxs[k]; bby = bys[k]; }
-                }
-                chosen = (uint8_t)bi;
-                rdx = (int8_t)clampv(adx - bbx, -128, 127); rdy = (int8_t)clampv(ady - bby, -128, 127);
-            }
-            mode_counts[mode]++;
-            e->modes[i] = mode; e->new_base[i] = chosen; e->raw_dx[i] = rdx; e->raw_dy[i] = rdy;
-            // reconstructed MV (encoder.rs:667-718)
-            MV rec = mv;
-            auto from = [&](const Q4& q) { return MV::make(q.dx, q.dy, q.sx, q.sy, mv.skip); };
-            if (mode == NEAREST) rec = from(p.nearest);
-            else if (mode == NEAR) rec = from(p.near);
-            else if (mode == TOPRIGHT) { if (n.htr) rec = from(n.tr); }
-            else if (mode == TOPLEFT) { if (n.htl) rec = from(n.tl); }
-            else if (mode == TEMPORAL) rec = from(p.temporal);
-            else if (mode == NEWMV) {
-                Q4 base = chosen == 0 ? p.nearest : chosen == 1 ? p.near : chosen == 2 ? (n.htr ? n.tr : p.nearest)
-                          : chosen == 3 ? (n.htl ? n.tl : p.nearest) : p.temporal;
-                rec.dx = (int8_t)clampv((int)base.dx + rdx, -128, 127);
-                rec.dy = (int8_t)clampv((int)base.dy + rdy, -128, 127);
-            }
-            coded.push_back(rec);
-            uint8_t f = mode == NEWMV ? (uint8_t)(mv.to_flags() & 0x0f) : 0;
+            mode_counts[e->modes[i]]++;
+            uint8_t f = e->modes[i] == NEWMV ? (uint8_t)(mv.to_flags() & 0x0f) : 0;
             if (skipc) f |= 0x40;
             flags[i] = f;
         }
@@ -1569,26 +1607,7 @@ static int me_frame_encode(ro_encoder* e, const uint8_t* rgb, uint64_t ts, ro_fr
         std::vector<uint8_t> res = encode_inter(e->cur, e->pred, e->rdo_skip, c.inter_quality, c.inter_dead_zone, recon, e->opt_skip, e->coeffs, e->aq);
         // bias  (encoder.rs:965-1048)
         int bias_dx = 0, bias_dy = 0;
-        {
-            std::vector<std::pair<int8_t, int8_t>> f;
-            for (int i = 0; i < nb; i++) if (e->modes[i] == NEWMV) f.push_back({e->raw_dx[i], e->raw_dy[i]});
-            if (!f.empty()) {
-                int64_t bzx = -1, bzy = -1, bcx = INT64_MAX, bcy = INT64_MAX;
-                for (int b = -24; b <= 24; b++) {
-                    int64_t zx = 0, cx = 0, zy = 0, cy = 0;
-                    for (auto& d : f) {
-                        int8_t ax = (int8_t)clampv((int)d.first - b, -128, 127);
-                        if (ax == 0) zx++;
-                        cx += std::abs((int)ax);
-                        int8_t ay = (int8_t)clampv((int)d.second - b, -128, 127);
-                        if (ay == 0) zy++;
-                        cy += std::abs((int)ay);
-                    }
-                    if (zx > bzx || (zx == bzx && cx < bcx)) { bzx = zx; bcx = cx; bias_dx = b; }
-                    if (zy > bzy || (zy == bzy && cy < bcy)) { bzy = zy; bcy = cy; bias_dy = b; }
-                }
-            }
-        }
+        new_delta_bias(e->modes, e->raw_dx, e->raw_dy, bias_dx, bias_dy);
         std::vector<CodedBlock> blocks(nb);
         uint32_t nnew = 0, nskip = 0, nrdo = 0;
         for (int i = 0; i < nb; i++) {
@@ -1919,4 +1938,65 @@ int ro_rans_decode(const uint8_t* data, size_t len, const int8_t* ctx_ids, size_
     return 0;
 }
 
+
+// ---- seams for independent pins (tests only) ----
+// Mode decision and NEW-delta bias of one frame: smv4 = nb search MVs (dx, dy, sx, sy), prev4 =
+// nb previous reconstructed MVs or null; out12 per block as ro_encoder_last_blocks' records
+// (search MV, reconstructed MV, mode, NEW base, raw NEW delta), bias2 = (bias_dx, bias_dy).
+void ro_mode_decision(const int8_t* smv4, const int8_t* prev4, int bw, int bh, int8_t* out12, int32_t* bias2) {
+    const int nb = bw * bh;
+    std::vector<MV> smv(nb), prev;
+    for (int i = 0; i < nb; i++) smv[i] = MV::make(smv4[4 * i], smv4[4 * i + 1], smv4[4 * i + 2], smv4[4 * i + 3], false);
+    if (prev4) { prev.resize(nb); for (int i = 0; i < nb; i++) prev[i] = MV::make(prev4[4 * i], prev4[4 * i + 1], prev4[4 * i + 2], prev4[4 * i + 3], false); }
+    std::vector<uint8_t> modes, nbase; std::vector<int8_t> rdx, rdy; std::vector<MV> coded;
+    decide_modes(smv, prev4 ? &prev : nullptr, bw, bh, modes, nbase, rdx, rdy, coded);
+    for (int i = 0; i < nb; i++) {
+        int8_t* r = out12 + 12 * i;
+        r[0] = smv[i].dx; r[1] = smv[i].dy; r[2] = smv[i].sx; r[3] = smv[i].sy;
+        r[4] = coded[i].dx; r[5] = coded[i].dy; r[6] = coded[i].sx; r[7] = coded[i].sy;
+        r[8] = (int8_t)modes[i]; r[9] = (int8_t)nbase[i]; r[10] = rdx[i]; r[11] = rdy[i];
+    }
+    int bx = 0, by = 0;
+    new_delta_bias(modes, rdx, rdy, bx, by);
+    bias2[0] = bx; bias2[1] = by;
+}
+// The residual coder (DctRansEncoder) over nb blocks of 384 coefficients (Y 256, U 64, V 64),
+// the ones with coded[b] != 0, with or without intra DC prediction; bytes into out (cap), size returned.
+size_t ro_encode_residual_blocks(const int16_t* coeffs, const uint8_t* coded, int nb, int dc_pred, uint8_t* out, size_t cap) {
+    init_zero_prob();
+    bool any = false;
+    for (int b = 0; b < nb; b++) any |= coded[b] != 0;
+    if (!any) return 0;
+    DctRansEnc enc(dc_pred != 0);
+    for (int b = 0; b < nb; b++) if (coded[b]) { const int16_t* q = coeffs + (size_t)b * 384; enc.encode_block(q, q + 256, q + 320); }
+    std::vector<uint8_t> bytes = enc.finish();
+    if (bytes.size() <= cap) memcpy(out, bytes.data(), bytes.size());
+    return bytes.size();
+}
+// The MV coder over nframes consecutive P-frames of one stream (contexts and candidate history
+// carried from frame to frame): blocks7 = nframes * nb blocks of (mode, new_base, delta_x,
+// delta_y, sub_x, sub_y, skip); each frame's bytes appended to out, sizes[f] its length.
+size_t ro_mv_encode_sequence(const int8_t* blocks7, int nframes, int bw, int bh, uint8_t* out, size_t cap, uint32_t* sizes) {
+    init_zero_prob();
+    const int nb = bw * bh;
+    MvCtx ctx;
+    size_t total = 0;
+    for (int f = 0; f < nframes; f++) {
+        std::vector<CodedBlock> blocks(nb);
+        for (int i = 0; i < nb; i++) {
+            const int8_t* q = blocks7 + ((size_t)f * nb + i) * 7;
+            CodedBlock& b = blocks[i];
+            b.mode = (uint8_t)q[0]; b.new_base = (uint8_t)q[1]; b.delta_x = q[2]; b.delta_y = q[3]; b.sub_x = q[4]; b.sub_y = q[5]; b.skip = q[6] != 0;
+        }
+        std::vector<uint8_t> bytes = mv_encode_frame(ctx, blocks, bw, bh);
+        if (total + bytes.size() <= cap) memcpy(out + total, bytes.data(), bytes.size());
+        total += bytes.size();
+        sizes[f] = (uint32_t)bytes.size();
+    }
+    return total;
+}
+// IDCT16 of one dequantised block (decode_plane_16x16_aq after its dequantisation, dct16.rs:345-412)
+void ro_idct16(const int32_t* d256, int16_t* out256) { init_basis(); idct16_2d(d256, out256, 16); }
+// IDCT8 of one block with its quantisation table (idct_block_8x8_residual_simd, fast_dct8.rs:123-174)
+void ro_idct8(const int16_t* coef64, const uint16_t* q64, int16_t* out64) { idct8_aan(coef64, q64, out64, 8); }
 }  // extern "C"
