@@ -85,6 +85,8 @@ struct HostBuf {
         if (p) { (void)hipHostFree(p); p = nullptr; d = nullptr; }
         hipError_t r = hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocMapped);
         if (r != hipSuccess) return r;
+        // the host's first touch of a page costs about 1.5 us; pay it here, not in the frame loop
+        std::memset(p, 0, bytes ? bytes : 1);
         return hipHostGetDevicePointer(&d, p, 0);
     }
     template <typename T> T* as() const { return (T*)p; }

@@ -731,7 +731,7 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int NW = gridDim.x * (blockDim.x >> 6), gw = blockIdx.x * (blockDim.x >> 6) + wv;
     const int nb = p.nb, sr = p.sr, NS = p.nstreams;
-    const int NR = 2 * sr + 18, ND = ((2 * sr + 20) >> 2) + 1, SLOT = riv::search_slot_dwords(sr);
+    const int NR = 2 * sr + 18, ND = riv::search_pitch(sr), SLOT = riv::search_slot_dwords(sr);
     const int TD = 2 * sr + 4;
     int* ctab = (int*)lds32;                           // TD x TD NEW-vector cost terms
     int* wflag = ctab + ((TD * TD + 3) & ~3);          // [0] a wave of this workgroup changed a block, [1] pass outcome

@@ -62,9 +62,12 @@ static_assert(sizeof(SearchStream) == 96, "SearchStream layout");
 
 // LDS layout of the motion search: the MV-cost table (TD x TD ints, TD = 2*sr + 4), 4 flag words,
 // the launch's stream descriptors, then per wave its window slots, each an NR x ND dword window
-// (NR = 2*sr + 18 rows, ND = ((2*sr + 20) >> 2) + 1 dwords) and the block's 16 rows (64 dwords).
+// (NR = 2*sr + 18 rows of ND dwords) and the block's 16 rows (64 dwords).
+// The row pitch ND is ((2*sr + 20) >> 2) + 1 dwords rounded up to odd: the 16 lanes of a row
+// group read 16 consecutive window rows, which an odd pitch spreads over distinct LDS banks.
+__host__ __device__ inline int search_pitch(int sr) { return (((2 * sr + 20) >> 2) + 1) | 1; }
 __host__ __device__ inline int search_window_dwords(int sr) {
-    return (((2 * sr + 18) * (((2 * sr + 20) >> 2) + 1)) + 3) & ~3;
+    return (((2 * sr + 18) * search_pitch(sr)) + 3) & ~3;
 }
 __host__ __device__ inline int search_slot_dwords(int sr) { return search_window_dwords(sr) + 64; }
 __host__ __device__ inline size_t search_slots_offset(int sr, int nstreams) {
