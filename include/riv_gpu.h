@@ -233,7 +233,9 @@ int riv_gpu_logf_exp2f(int32_t device_id, const float* in, size_t n, int32_t whi
  * no reference counterpart). Stages: 0 RGB->YUV, 1 motion search (relaxation passes, mode
  * decision and the prediction variances), 2 motion compensation (on the encoder's P-frame path
  * formed inside stage 4, so not recorded), 3 adaptive quantiser, 4 residual prediction/transform/
- * quantise/reconstruct, 5 deblock, 6 intra transform, 7 unused. `on`: 0 off, 1 the motion-search
+ * quantise/reconstruct, 5 deblock, 6 intra transform, 7 the whole motion-search launch of batch
+ * steps this encoder leads (all streams; the stage-1 figures of encoders in a batch are not the
+ * search alone). `on`: 0 off, 1 the motion-search
  * stage only (two events per frame), 2 every stage and each frame's gpu_ms. Every event costs the
  * stream a few microseconds. */
 #define RIV_NUM_STAGES 8

@@ -315,7 +315,7 @@ class Encoder:
         _check(L.lib().riv_encoder_output(self._h, byref(p), byref(n)), EncodeError)
         return ctypes.string_at(p, n.value)
 
-    STAGES = ["rgb_to_yuv", "motion_search", "mc", "aq", "residual", "deblock", "intra", "unused"]
+    STAGES = ["rgb_to_yuv", "motion_search", "mc", "aq", "residual", "deblock", "intra", "batch_motion_search"]
 
     def set_profiling(self, on=True):
         """0/False off, 1 the motion-search stage only, 2/True every stage."""

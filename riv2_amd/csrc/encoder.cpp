@@ -269,6 +269,10 @@ struct riv_encoder {
     size_t b_prop_n = 0;
     std::map<std::tuple<int, int, int>, ShapeTuner> b_tuners;   // by (blocks, streams, search range)
     ShapeTuner* b_last_tuner = nullptr;
+    // batch steps led by this encoder with profiling on: events around each search launch, read
+    // once complete into stage 7
+    std::deque<std::pair<hipEvent_t, hipEvent_t>> b_prof;
+    std::vector<hipEvent_t> b_prof_spare;
 
     // ordered tail + completion state
     std::mutex mu;
@@ -322,6 +326,28 @@ struct riv_encoder {
 
 static HostStage* acquire_host_stage();
 static void release_host_stage();
+
+// Completed batch-search timings of the encoder leading batch steps, into stage 7.
+static void collect_batch_search_times(riv_encoder* e) {
+    while (!e->b_prof.empty() && hipEventQuery(e->b_prof.front().second) == hipSuccess) {
+        const auto q = e->b_prof.front();
+        e->b_prof.pop_front();
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, q.first, q.second) == hipSuccess) {
+            std::lock_guard<std::mutex> lk(e->mu);
+            e->stage_ms[7] += ms;
+            e->stage_n[7]++;
+        }
+        e->b_prof_spare.push_back(q.first);
+        e->b_prof_spare.push_back(q.second);
+    }
+}
+static hipEvent_t batch_prof_event(riv_encoder* e) {
+    hipEvent_t ev = nullptr;
+    if (!e->b_prof_spare.empty()) { ev = e->b_prof_spare.back(); e->b_prof_spare.pop_back(); return ev; }
+    if (hipEventCreateWithFlags(&ev, hipEventDisableSystemFence) != hipSuccess) return nullptr;
+    return ev;
+}
 
 // Launches the motion search on st once the device's previous search (from any stream) is done.
 static int launch_search(int device, hipStream_t st, const riv::SearchParams& p, int groups, int waves) {
@@ -503,6 +529,8 @@ static void destroy_encoder(riv_encoder* e) {
         for (auto& m : s.marks) if (m) (void)hipEventDestroy(m);
     }
     if (e->sync_ev) (void)hipEventDestroy(e->sync_ev);
+    for (auto& q : e->b_prof) { (void)hipEventDestroy(q.first); (void)hipEventDestroy(q.second); }
+    for (hipEvent_t ev : e->b_prof_spare) (void)hipEventDestroy(ev);
     for (auto& ev : e->b_stage_ev) if (ev) (void)hipEventDestroy(ev);
     if (e->st) (void)hipStreamDestroy(e->st);
     delete e;
@@ -1165,9 +1193,22 @@ static int batch_step(riv_encoder* const* encs, size_t n, const void* const* rgb
         e0->b_bank ^= 1;
         hipEvent_t ta = ci >= 0 ? T.event() : nullptr, tb = ci >= 0 ? T.event() : nullptr;
         if (ta && tb) RIV_HIP(hipEventRecord(ta, st));
+        collect_batch_search_times(e0);
+        hipEvent_t pa = nullptr, pb = nullptr;   // profiling: the whole launch, timed into stage 7
+        if (e0->profiling && ci < 0) {
+            pa = batch_prof_event(e0); pb = batch_prof_event(e0);
+            if (pa && pb) RIV_HIP(hipEventRecord(pa, st));
+        }
         p.plimit = e0->pass_cap > 0 ? std::min(p.maxp, e0->pass_cap) : p.maxp;
         int lr = launch_search(c0.device_id, st, p, sh.groups, sh.waves);
         if (lr) return lr;
+        if (pa && pb) {
+            RIV_HIP(hipEventRecord(pb, st));
+            e0->b_prof.push_back({pa, pb});
+        } else {
+            if (pa) e0->b_prof_spare.push_back(pa);
+            if (pb) e0->b_prof_spare.push_back(pb);
+        }
         if (ta && tb) {
             RIV_HIP(hipEventRecord(tb, st));
             T.pending.push_back({ci, ta, tb});
@@ -1313,6 +1354,10 @@ int riv_encoder_set_profiling(riv_encoder* e, int on) {
 }
 int riv_encoder_stage_times(riv_encoder* e, double* ms, uint32_t* counts) {
     if (!e || !ms || !counts) return RIV_E_ARG;
+    if (!e->b_prof.empty()) {
+        RIV_HIP(hipSetDevice(e->cfg.device_id));
+        collect_batch_search_times(e);
+    }
     std::lock_guard<std::mutex> lk(e->mu);
     for (int i = 0; i < RIV_NUM_STAGES; i++) { ms[i] = e->stage_ms[i]; counts[i] = e->stage_n[i]; }
     return RIV_OK;
