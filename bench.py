@@ -57,6 +57,9 @@ def make_config(riv2_amd, depth):
         .with_skip_threshold(3).with_keyframe_interval(riv2_amd.KeyframeInterval.fixed(1000)).with_pipeline_depth(depth)
 
 
+LEARNED_PASSES = 4   # the learned form's relaxation passes per motion search (profiles/learned_mv_eval.json)
+
+
 def run_gpu(frames, steps, warmup, depth, net=None, d_frames=None, confidence=0.5):
     """Times `steps` P-frames (after one I-frame and `warmup` P-frames) through the pipelined
     encoder, inputs resident in HBM; the motion-search stage is bracketed by HIP events on the
@@ -74,6 +77,7 @@ def run_gpu(frames, steps, warmup, depth, net=None, d_frames=None, confidence=0.
     enc = riv2_amd.Encoder(make_config(riv2_amd, depth))
     if net is not None:
         enc.set_mvnet(net, confidence)
+        enc.set_learned_passes(LEARNED_PASSES)
     ptr = lambda i: d_frames[i].data_ptr()
     enc.encode_frame_device(ptr(0), fbytes, W, H, 0)
     for i in range(1, 1 + warmup):
@@ -156,7 +160,8 @@ def run_learned(frames, d_frames, steps, warmup, depth, plain_stream, plain_fps)
             "fallback_rate": round(1.0 - counts[2] / max(1, counts[0]), 4),
             "confident_rate": round(counts[1] / max(1, counts[0]), 4),
             "note": "same frames and settings as the plain line; bytes and PSNR-Y over the whole 50-frame stream at q80 "
-                    "(rate at equal PSNR over several qualities: profiles/learned_mv_1080p_eval.json)"}
+                    "(rate at equal PSNR over several qualities: profiles/learned_mv_eval.json)",
+            "relaxation_passes": LEARNED_PASSES}
 
 
 def stage_breakdown(d_frames, n, depth):
@@ -410,6 +415,7 @@ def run_stress(per_size, p_frames, depth, with_cpu):
         if with_net:
             for e in encs:
                 e.set_mvnet(net, 0.5)
+                e.set_learned_passes(LEARNED_PASSES)
         steps, _, elapsed, search_ms = _timed_batch(encs, frames, p_frames)
         counts = encs[0].learned_stats()
         outs = []

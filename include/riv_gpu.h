@@ -301,6 +301,13 @@ int riv_mvnet_propose(riv_mvnet* net, const int32_t* samples, size_t n, float co
  * block takes the plain search). The network must outlive its use and keep its parameters while
  * frames are in flight. In a batch step each stream follows its own encoder's setting. */
 int riv_encoder_set_mvnet(riv_encoder* enc, riv_mvnet* net, float confidence);
+/* Learned form only: the motion search runs `passes` relaxation passes and keeps the field it
+ * reached instead of relaxing to the reference's raster-order fixed point (0, the default: relax to
+ * it). The field's vectors are searched as the learned form searches them, and the mode decision
+ * and the bitstream follow the field, so any decoder of the plain form decodes the stream; a plain
+ * encoder always relaxes to the fixed point. In a batch step a fixed pass count applies only when
+ * every inter stream of the step is learned with the same count. */
+int riv_encoder_set_learned_passes(riv_encoder* enc, int32_t passes);
 /* Learned-form counts since the encoder was created: counts[0] P-frame blocks searched, [1] those
  * with a confident proposal, [2] those that took the proposal (the rest fell back). For the first
  * encoder of batch steps with learned streams, the counts also cover every stream of those steps. */

@@ -141,6 +141,7 @@ PROTOTYPES = [
     ("riv_mvnet_forward", c_int, [c_void_p, c_void_p, c_size_t, c_int32, c_void_p]),
     ("riv_mvnet_propose", c_int, [c_void_p, c_void_p, c_size_t, c_float, c_int32, c_void_p]),
     ("riv_encoder_set_mvnet", c_int, [c_void_p, c_void_p, c_float]),
+    ("riv_encoder_set_learned_passes", c_int, [c_void_p, c_int32]),
     ("riv_encoder_learned_stats", c_int, [c_void_p, c_void_p]),
     ("riv_last_error", c_char_p, []),
 ]

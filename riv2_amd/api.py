@@ -346,6 +346,10 @@ class Encoder:
         self._mvnet = net   # the network must outlive its use
         _check(L.lib().riv_encoder_set_mvnet(self._h, net._h if net is not None else None, float(confidence)), EncodeError)
 
+    def set_learned_passes(self, passes: int):
+        """Learned form: relaxation passes per motion search (0: to the raster fixed point)."""
+        _check(L.lib().riv_encoder_set_learned_passes(self._h, int(passes)), EncodeError)
+
     def learned_stats(self):
         """(P-frame blocks searched, blocks with a confident proposal, blocks that took it)."""
         c = np.zeros(3, np.uint64)

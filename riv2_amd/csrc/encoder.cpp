@@ -294,6 +294,7 @@ struct riv_encoder {
     // proposals of the frame being searched and the running counts
     riv_mvnet* net = nullptr;
     float net_logit = 0.0f;
+    int learned_passes = 0;        // learned form: relaxation passes per search (0: to the fixed point)
     DevBuf prop, lstat;
 
     int pass_cap = 0;              // test seam: relaxation passes allowed per search (0: the proven bound)
@@ -1057,6 +1058,10 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
         }
         e->s_bank ^= 1;
         p.plimit = e->pass_cap > 0 ? std::min(p.maxp, e->pass_cap) : p.maxp;
+        if (e->net && e->learned_passes > 0) {   // learned form: a fixed number of passes
+            p.plimit = std::min(p.plimit, e->learned_passes);
+            p.truncate = 1;
+        }
         int lr = launch_search(e->cfg.device_id, f.st, p, e->shape.groups, e->shape.waves);
         if (lr) return lr;
         r = frame_after_search(e, f);
@@ -1200,6 +1205,15 @@ static int batch_step(riv_encoder* const* encs, size_t n, const void* const* rgb
             if (pa && pb) RIV_HIP(hipEventRecord(pa, st));
         }
         p.plimit = e0->pass_cap > 0 ? std::min(p.maxp, e0->pass_cap) : p.maxp;
+        {   // a fixed number of passes only when every inter stream of the launch is learned with one
+            int lp = -1;
+            for (size_t i = 0; i < n && lp != 0; i++) {
+                if (f[i].intra) continue;
+                const int k = encs[i]->net ? encs[i]->learned_passes : 0;
+                lp = lp < 0 ? k : (lp == k ? k : 0);
+            }
+            if (lp > 0) { p.plimit = std::min(p.plimit, lp); p.truncate = 1; }
+        }
         int lr = launch_search(c0.device_id, st, p, sh.groups, sh.waves);
         if (lr) return lr;
         if (pa && pb) {
@@ -1321,6 +1335,12 @@ int riv_encoder_set_mvnet(riv_encoder* e, riv_mvnet* net, float confidence) {
     if (net && !(confidence > 0.0f && confidence <= 1.0f)) { set_error("confidence must be in (0, 1]"); return RIV_E_ARG; }
     e->net = net;
     e->net_logit = net ? rivnet::conf_logit(confidence) : 0.0f;
+    return RIV_OK;
+}
+
+int riv_encoder_set_learned_passes(riv_encoder* e, int32_t passes) {
+    if (!e || passes < 0) return RIV_E_ARG;
+    e->learned_passes = passes;
     return RIV_OK;
 }
 

@@ -854,6 +854,10 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
         if (res < 0) { outcome = -1; break; }
         if (res == 0) { outcome = 1; break; }
     }
+    if (outcome == 0 && p.truncate) {   // keep the field of the last pass run
+        outcome = 1;
+        pass = p.plimit - 1;
+    }
     if (outcome != 1) {
         const int code = outcome < 0 ? 1 : 3;
         if (threadIdx.x == 0) __hip_atomic_store(p.err, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -92,7 +92,10 @@ struct SearchParams {
     int bank;                    // bank of this launch (zero at launch); the launch clears the other
     int maxp;                    // pass bound: 2*(bh-1) + bw + 2 passes of the largest stream always
                                  // reach the fixed point (it also lays out the barrier counters)
-    int plimit;                  // passes this launch may run (<= maxp); not settling within them fails it
+    int plimit;                  // passes this launch may run (<= maxp); not settling within them fails it,
+                                 // unless `truncate`
+    int truncate;                // learned form only: after plimit passes keep the field reached (a valid
+                                 // field: the mode decision follows it) instead of failing
     int nslot, nres;             // LDS window slots per wave: lane k < nres keeps its block's window in
                                  // slot k; when nslot = nres + 1 the other lanes share slot nres
     const double* bits;          // 256 entries: 1.0 for 0, else 2*log2(a)+2 (motion/scalar.rs:495-525)

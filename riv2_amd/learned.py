@@ -138,17 +138,20 @@ class PairSet:
     targets: np.ndarray = None
 
 
-def collect_pairs(seeds, pframes: int = 4, device: int = 0, log=None) -> PairSet:
-    """Encodes 1 I + `pframes` P-frames of each seed's 1080p synthetic sequence with the plain
-    GPU encoder; every P-frame block gives one pair, with the frame pair resident on the device."""
+def collect_pairs(seeds, pframes: int = 4, device: int = 0, log=None, params=None) -> PairSet:
+    """Encodes 1 I + `pframes` P-frames of each seed's synthetic sequence (params(seed, frames);
+    default the 1080p workload's generator) with the plain GPU encoder; every P-frame block gives
+    one pair, with the frame pair resident on the device."""
     from . import synth
+    if params is None:
+        params = lambda seed, n: synth.hd_params(seed=seed, frames=n)
     dev = torch.device("cuda", device)
     ps = PairSet()
     samples, targets = [], []
     for seed in seeds:
-        frames = synth.Sequence(synth.hd_params(seed=seed, frames=pframes + 1)).frames(0, pframes + 1)
-        enc = Encoder(hd_config())
+        frames = synth.Sequence(params(seed, pframes + 1)).frames(0, pframes + 1)
         h, w = frames.shape[1:3]
+        enc = Encoder(hd_config(width=w, height=h))
         enc.encode_frame(Frame(frames[0], w, h, 0))
         ref = torch.from_numpy(enc.debug_yuv(2)[0]).to(dev)
         prev = None
@@ -164,8 +167,23 @@ def collect_pairs(seeds, pframes: int = 4, device: int = 0, log=None) -> PairSet
             prev = torch.from_numpy(np.ascontiguousarray(rec[:, 4:6])).to(dev)
             ref = torch.from_numpy(enc.debug_yuv(2)[0]).to(dev)
         enc.close()
-        if log:
+        if log and (seed % 50 == 0 or len(seeds) <= 100):
             log(f"seed {seed}: {pframes} P-frames")
+    ps.samples = np.concatenate(samples)
+    ps.targets = np.concatenate(targets)
+    return ps
+
+
+def merge_pairs(parts) -> PairSet:
+    """One pair set from several, their source indices renumbered."""
+    ps = PairSet()
+    samples, targets = [], []
+    for p in parts:
+        s = p.samples.copy()
+        s[:, 0] += len(ps.sources)
+        ps.sources.extend(p.sources)
+        samples.append(s)
+        targets.append(p.targets)
     ps.samples = np.concatenate(samples)
     ps.targets = np.concatenate(targets)
     return ps
@@ -225,13 +243,15 @@ def bd_rate(r1, p1, r2, p2) -> float:
     return float((np.exp(a2 - a1) - 1.0) * 100.0)
 
 
-def encode_sequence(frames: np.ndarray, quality: int, net: Optional[MvNet] = None, confidence: float = 0.5):
-    """(bitstream, mean PSNR-Y of the decoded P-frames, learned counts, per-frame block SADs)."""
+def encode_sequence(frames: np.ndarray, quality: int, net: Optional[MvNet] = None, confidence: float = 0.5, passes: int = 0):
+    """(bitstream, mean PSNR-Y of the decoded P-frames, learned counts, per-frame block SADs).
+    With a network, `passes` > 0 stops each motion search after that many relaxation passes."""
     from .api import Decoder
     h, w = frames.shape[1:3]
     enc = Encoder(hd_config(quality, width=w, height=h))
     if net is not None:
         enc.set_mvnet(net, confidence)
+        enc.set_learned_passes(passes)
     sads = []
     for k in range(len(frames)):
         enc.encode_frame(Frame(frames[k], w, h, 33 * k))
@@ -250,18 +270,22 @@ def encode_sequence(frames: np.ndarray, quality: int, net: Optional[MvNet] = Non
     return data, float(np.mean(ps)), counts, sads
 
 
-def compare(net: MvNet, seeds, frames: int = 10, qualities=(70, 75, 80, 85), confidence: float = 0.5, log=None):
+def compare(net: MvNet, seeds, frames: int = 10, qualities=(70, 75, 80, 85), confidence: float = 0.5, log=None, passes: int = 0,
+            params=None):
     """Plain against learned on held-out sequences: totals per quality, BD-rate, fallback rate and
-    the share of first-P-frame blocks (same reference in both forms) with learned SAD <= plain."""
+    the share of first-P-frame blocks (same reference in both forms) with learned SAD <= plain.
+    params(seed, frames) gives a sequence's generator parameters (default: the 1080p workload's)."""
     from . import synth
+    if params is None:
+        params = lambda seed, n: synth.hd_params(seed=seed, frames=n)
     rows = []
     le_num = le_den = 0
     blocks = confident = taken = 0
     for seed in seeds:
-        fr = synth.Sequence(synth.hd_params(seed=seed, frames=frames)).frames(0, frames)
+        fr = synth.Sequence(params(seed, frames)).frames(0, frames)
         for q in qualities:
             pb, pp, _, psad = encode_sequence(fr, q)
-            lb, lp, cnt, lsad = encode_sequence(fr, q, net, confidence)
+            lb, lp, cnt, lsad = encode_sequence(fr, q, net, confidence, passes)
             rows.append((seed, q, len(pb), pp, len(lb), lp))
             le_num += int((lsad[0] <= psad[0]).sum()); le_den += len(psad[0])
             blocks += cnt[0]; confident += cnt[1]; taken += cnt[2]

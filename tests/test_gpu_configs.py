@@ -142,7 +142,8 @@ def _oracle_decode_digests(stream, nframes):
 
 
 @pytest.mark.timeout(600)
-def test_learned_1080p_trained_network_decodes_bit_exact():
+@pytest.mark.parametrize("passes", [0, 4])
+def test_learned_1080p_trained_network_decodes_bit_exact(passes):
     if not os.path.exists(learned.WEIGHTS_PATH):
         pytest.skip("no trained weights committed")
     n = 12
@@ -151,10 +152,13 @@ def test_learned_1080p_trained_network_decodes_bit_exact():
     net = learned.MvNet.load(learned.WEIGHTS_PATH)
     enc = riv2_amd.Encoder(_cfg(w, h, 50, depth=0))
     enc.set_mvnet(net, 0.5)
+    enc.set_learned_passes(passes)
     recon = []
     for i in range(n):
-        print(f"[config learned] frame {i}", flush=True)
-        enc.encode_frame(riv2_amd.Frame(frames[i], w, h, i * 20))
+        print(f"[config learned passes {passes}] frame {i}", flush=True)
+        st = enc.encode_frame_with_stats(riv2_amd.Frame(frames[i], w, h, i * 20))
+        if passes and i > 0:
+            assert st.me_passes == passes
         recon.append(sha(*enc.debug_yuv(2)))
     counts = enc.learned_stats()
     enc.finish()
@@ -171,7 +175,8 @@ def test_learned_1080p_trained_network_decodes_bit_exact():
 
 
 @pytest.mark.timeout(1200)
-def test_learned_batched_stress_64_streams_decode_bit_exact():
+@pytest.mark.parametrize("passes", [0, 4])
+def test_learned_batched_stress_64_streams_decode_bit_exact(passes):
     if not os.path.exists(learned.WEIGHTS_PATH):
         pytest.skip("no trained weights committed")
     import torch
@@ -185,9 +190,10 @@ def test_learned_batched_stress_64_streams_decode_bit_exact():
     encs = [riv2_amd.Encoder(_cfg(w, h, depth=2)) for w, h, _ in jobs]
     for e in encs:
         e.set_mvnet(net, 0.5)
+        e.set_learned_passes(passes)
     recon = [[] for _ in jobs]
     for i in range(n):
-        print(f"[config stress] step {i}", flush=True)
+        print(f"[config stress passes {passes}] step {i}", flush=True)
         riv2_amd.encode_batch_step(encs, [d[i].data_ptr() for d in dev], [i * 33] * len(encs), on_device=True)
         for k, e in enumerate(encs):
             recon[k].append(sha(*e.debug_yuv(2)))

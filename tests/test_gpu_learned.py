@@ -303,3 +303,52 @@ def test_learned_streams_in_a_batch_equal_their_solo_encodes():
             for i, f in enumerate(seq):
                 ref.encode(f, 33 * i)
             assert got == ref.finish(), f"{w}x{h}: plain stream in a learned batch differs from the oracle"
+
+
+def test_fixed_passes_apply_only_to_learned_launches():
+    """learned_passes > 0 stops a learned encoder's search after that many passes (the pass count
+    reported equals it); a plain stream in the same batch keeps the whole launch relaxing to the
+    fixed point, so it still equals the oracle, and a plain encoder ignores the setting."""
+    print("[learned] fixed passes", flush=True)
+    net = make_net(29)
+    w, h = 200, 120
+    seq = synth.Sequence(synth.custom_params(w, h, 5, seed=44, global_motion=6.0, n_rects=3, rect_min=16, rect_max=48,
+                                             rect_velocity=9.0)).frames()
+    ref = oracle.OracleEncoder(oracle.config(w, h))
+    for i, f in enumerate(seq):
+        ref.encode(f, 33 * i)
+    want_plain = ref.finish()
+    # solo learned encoder: every P-frame runs exactly two passes; the stream decodes on the oracle
+    e = riv2_amd.Encoder(learned.hd_config(80, width=w, height=h))
+    e.set_mvnet(net, 1e-6)
+    e.set_learned_passes(2)
+    for i, f in enumerate(seq):
+        st = e.encode_frame_with_stats(riv2_amd.Frame(f, w, h, 33 * i))
+        if i > 0:
+            assert st.me_passes == 2
+    e.finish()
+    data = e.output()
+    e.close()
+    dec, rdec = riv2_amd.Decoder(data), oracle.OracleDecoder(data)
+    for k in range(len(seq)):
+        assert np.array_equal(dec.decode_frame().data, rdec.decode()[0]), f"frame {k}"
+    # a plain encoder ignores the setting
+    p = riv2_amd.Encoder(learned.hd_config(80, width=w, height=h))
+    p.set_learned_passes(2)
+    for i, f in enumerate(seq):
+        p.encode_frame(riv2_amd.Frame(f, w, h, 33 * i))
+    p.finish()
+    assert p.output() == want_plain
+    p.close()
+    # mixed batch: the learned stream's setting does not truncate the plain stream's search
+    encs = [riv2_amd.Encoder(learned.hd_config(80, width=w, height=h)) for _ in range(2)]
+    encs[0].set_mvnet(net, 1e-6)
+    encs[0].set_learned_passes(2)
+    for i in range(len(seq)):
+        riv2_amd.encode_batch_step(encs, [seq[i], seq[i]], [33 * i, 33 * i])
+    outs = []
+    for x in encs:
+        x.finish()
+        outs.append(x.output())
+        x.close()
+    assert outs[1] == want_plain
