@@ -231,10 +231,20 @@ def check_prefix_bit_exact(frames, gpu_stream, nframes):
     return gpu_stream[36:36 + body] == ref_stream[36:]
 
 
-def run_drop_in(frames, steps, warmup, depth):
-    """The drop-in boundary: riv_encode_frame with host RGB24 (pageable numpy frames), timed like
-    the resident-input line, so the H2D copy of every frame is inside the timed region."""
+def run_drop_in(frames, steps, warmup, depth, pinned=False):
+    """The drop-in boundary: riv_encode_frame with host RGB24 (pageable numpy frames, or frames in
+    pinned host memory when `pinned`), timed like the resident-input line, so the H2D copy of every
+    frame is inside the timed region."""
     import riv2_amd
+    if pinned:
+        import torch
+        held = []
+        for f in frames[:1 + warmup + steps]:
+            t = torch.empty(f.size, dtype=torch.uint8, pin_memory=True)
+            a = t.numpy().reshape(f.shape)
+            a[...] = f
+            held.append((t, a))
+        frames = [a for _, a in held]
     enc = riv2_amd.Encoder(make_config(riv2_amd, depth))
     for i in range(1 + warmup):
         enc.encode_frame(riv2_amd.Frame(frames[i], W, H, i * 20))
@@ -565,6 +575,8 @@ def main():
     print("[bench] drop-in boundary: host RGB in", file=sys.stderr, flush=True)
     drop_fps, drop_stream = run_drop_in(frames, args.steps, args.warmup, args.pipeline_depth)
     drop_same = drop_stream == stream
+    pin_fps, pin_stream = run_drop_in(frames, args.steps, args.warmup, args.pipeline_depth, pinned=True)
+    drop_same = drop_same and pin_stream == stream
     print("[bench] decode", file=sys.stderr, flush=True)
     decode = run_decode(stream, nframes, max(1, min(args.cpu_decode_frames, nframes - 1)))
 
@@ -607,8 +619,10 @@ def main():
         "bytes_per_pframe": round(sum(s.total_bytes for s in stats) / len(stats), 1),
         "learned": learned_leg,
         "drop_in": {"value": round(drop_fps, 3), "unit": "fps", "same_stream": bool(drop_same),
+                    "pinned_value": round(pin_fps, 3),
                     "note": "riv_encode_frame with host RGB24 in pageable memory (the reference API's input), same frames "
-                            "and timing; the H2D copy of each frame is inside the timed region"},
+                            "and timing; the H2D copy of each frame is inside the timed region. pinned_value: the same "
+                            "with the frames in pinned host memory (the copy is then asynchronous for the caller)"},
         "decode": decode,
     }
     if cpu:

@@ -117,7 +117,8 @@ int riv_encoder_create(const riv_encoder_config* cfg, const riv_writer_vtbl* wri
 /* Encoder::new with VecWriter (writer.rs:21-77): output kept in memory, see riv_encoder_output. */
 int riv_encoder_create_mem(const riv_encoder_config* cfg, riv_encoder** out);
 /* Encoder::encode_frame / encode_frame_with_stats (encoder.rs:229-256). rgb: host RGB24,
- * width*height*3 bytes. stats may be NULL. */
+ * width*height*3 bytes, pageable or pinned; the frame has been copied to the GPU when the call
+ * returns, so the caller may reuse its memory at once. stats may be NULL. */
 int riv_encode_frame(riv_encoder* enc, const uint8_t* rgb, size_t len, uint32_t width, uint32_t height,
                      uint64_t timestamp_ms, riv_frame_stats* stats);
 /* Same, with the RGB24 frame already resident in device memory of the encoder's device. */
@@ -129,8 +130,9 @@ int riv_encode_frame_device(riv_encoder* enc, const void* d_rgb, size_t len, uin
  * launch over all their blocks. Every stream's output is byte-identical to encoding it alone.
  * The encoders must share device, search range, inter quality / rdo_lambda_mult and the ME
  * thresholds, and appear once each; rgb[i] is a host (on_device = 0) or device (on_device = 1)
- * RGB24 frame of len[i] bytes at encs[i]'s display size. timestamp_ms and stats (n entries) may be
- * NULL. With pipeline_depth > 0 the call returns once the work is queued. */
+ * RGB24 frame of len[i] bytes at encs[i]'s display size; host frames have been copied when the call
+ * returns. timestamp_ms and stats (n entries) may be NULL. With pipeline_depth > 0 the call returns
+ * once the work is queued. */
 int riv_encode_batch_step(riv_encoder* const* encs, size_t n, const void* const* rgb, const size_t* len, const uint64_t* timestamp_ms,
                           int on_device, riv_frame_stats* stats);
 /* Blocks until every submitted frame has been written (pipelined mode; no-op otherwise). */
@@ -164,6 +166,10 @@ int riv_encoder_debug_aq(riv_encoder* enc, float* steps);
  * settled within the cap fails its frame: the frame returns (synchronous mode) or the next call
  * and riv_encoder_flush return (pipelined mode) RIV_E_GPU, and no byte of that frame is written. */
 int riv_encoder_debug_pass_limit(riv_encoder* enc, int32_t max_passes);
+/* Test seam of the host stage (no reference counterpart): how P-frame residuals are coded. 0 (the
+ * default): split across the host workers when at least two are idle, else serially; 1: always
+ * serially; 2: always split. Every mode writes the same bytes. */
+int riv_encoder_debug_split_coding(riv_encoder* enc, int32_t mode);
 
 /* Decoder::new (reitero_decode/src/decoder.rs:112-171): reads and validates the header. */
 int riv_decoder_create(const riv_reader_vtbl* reader, void* user, int32_t device_id, riv_decoder** out);

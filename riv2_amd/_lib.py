@@ -108,6 +108,7 @@ PROTOTYPES = [
     ("riv_encoder_debug_coeffs", c_int, [c_void_p, c_void_p]),
     ("riv_encoder_debug_aq", c_int, [c_void_p, c_void_p]),
     ("riv_encoder_debug_pass_limit", c_int, [c_void_p, c_int32]),
+    ("riv_encoder_debug_split_coding", c_int, [c_void_p, c_int32]),
     ("riv_decoder_create", c_int, [POINTER(ReaderVtblC), c_void_p, c_int32, POINTER(c_void_p)]),
     ("riv_decoder_create_mem", c_int, [c_void_p, c_size_t, c_int32, POINTER(c_void_p)]),
     ("riv_decode_frame", c_int, [c_void_p, c_void_p, c_size_t, POINTER(DecodedInfoC)]),

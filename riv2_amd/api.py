@@ -384,6 +384,11 @@ class Encoder:
         settle within the cap fails its frame (no bytes are written for it)."""
         _check(L.lib().riv_encoder_debug_pass_limit(self._h, int(max_passes)), EncodeError)
 
+    def debug_split_coding(self, mode: int):
+        """How P-frame residuals are coded on the host: 0 split across idle workers when two are
+        idle, else serially (the default); 1 always serially; 2 always split. Same bytes."""
+        _check(L.lib().riv_encoder_debug_split_coding(self._h, int(mode)), EncodeError)
+
     def debug_aq(self):
         a = np.zeros(self._bw * self._bh, np.float32)
         _check(L.lib().riv_encoder_debug_aq(self._h, a.ctypes.data))

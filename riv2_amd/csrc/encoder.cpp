@@ -257,6 +257,14 @@ struct riv_encoder {
     HostStage* hs = nullptr;       // the process's host stage, while pipelined
     HostStage* par = nullptr;      // the same, held by every encoder: its idle workers help code a frame
     hipEvent_t sync_ev = nullptr;  // orders this encoder's stream against a batch step's stream
+    // frames handed over in host memory (riv_encode_frame): each is copied on a stream of its own
+    // into one of two device buffers (the first two thirds of `rgb`), so that the copy of frame k,
+    // which holds the submitting thread when the source is pageable, overlaps the GPU work of
+    // earlier frames; a batch step copies into the last third on the batch's stream
+    hipStream_t up = nullptr;
+    hipEvent_t up_ready[2] = {}, up_free[2] = {};
+    bool up_used[2] = {};
+    int up_idx = 0;
     // batch-step scratch (held by the first encoder of a batch): field and barrier counters over all
     // streams of the batch, the stream descriptors and their pinned staging copies
     DevBuf b_field, b_arrive, b_desc;
@@ -264,6 +272,7 @@ struct riv_encoder {
     int b_bank = 0, b_stage = 0, b_maxp = 0;
     HostBuf b_stage_buf[2];
     hipEvent_t b_stage_ev[2] = {nullptr, nullptr};
+    hipEvent_t b_copy_ev = nullptr;            // after a batch step's copies of host frames
     bool b_autotune = true;
     DevBuf b_prop, b_lstat;   // learned form in batch steps: proposals of every block, per-wave counters
     size_t b_prop_n = 0;
@@ -298,6 +307,7 @@ struct riv_encoder {
     DevBuf prop, lstat;
 
     int pass_cap = 0;              // test seam: relaxation passes allowed per search (0: the proven bound)
+    std::atomic<int> split_coding{0};   // test seam: P-frame residual coding 0 by idle workers, 1 serial, 2 split
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     double now_ms() const { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
     int profiling = 0;             // 0 off, 1 motion-search stage only, 2 every stage
@@ -422,7 +432,7 @@ static int encoder_init(riv_encoder* e) {
     }
     const size_t ysz = (size_t)e->W * e->H, csz = ysz / 4, fsz = ysz + 2 * csz;
     RIV_HIP(hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking));
-    RIV_HIP(e->rgb.alloc((size_t)c.display_width * c.display_height * 3));
+    RIV_HIP(e->rgb.alloc((size_t)c.display_width * c.display_height * 3 * 3));
     RIV_HIP(e->cur.alloc(fsz)); RIV_HIP(e->pred.alloc(fsz));
     RIV_HIP(e->ref[0].alloc(fsz)); RIV_HIP(e->ref[1].alloc(fsz));
     RIV_HIP(e->var.alloc(8 * (size_t)e->nb)); RIV_HIP(e->lnv.alloc(4 * ((size_t)e->nb + 1))); RIV_HIP(e->steps.alloc(4 * (size_t)e->nb));
@@ -530,9 +540,15 @@ static void destroy_encoder(riv_encoder* e) {
         for (auto& m : s.marks) if (m) (void)hipEventDestroy(m);
     }
     if (e->sync_ev) (void)hipEventDestroy(e->sync_ev);
+    if (e->up) {
+        (void)hipStreamSynchronize(e->up);
+        for (int b = 0; b < 2; b++) { (void)hipEventDestroy(e->up_ready[b]); (void)hipEventDestroy(e->up_free[b]); }
+        (void)hipStreamDestroy(e->up);
+    }
     for (auto& q : e->b_prof) { (void)hipEventDestroy(q.first); (void)hipEventDestroy(q.second); }
     for (hipEvent_t ev : e->b_prof_spare) (void)hipEventDestroy(ev);
     for (auto& ev : e->b_stage_ev) if (ev) (void)hipEventDestroy(ev);
+    if (e->b_copy_ev) (void)hipEventDestroy(e->b_copy_ev);
     if (e->st) (void)hipStreamDestroy(e->st);
     delete e;
 }
@@ -627,7 +643,9 @@ static void code_residuals(riv_encoder* e, int si) {
             for (int i = 0; i < nb; i++) coded[i] = !os[i];
             // with idle host workers the frame's coding is split across them (the same bytes; the
             // split costs more work in all, so a pool busy with other frames codes it serially)
-            if (e->par && e->par->pool.idle() >= 2)
+            const int mode = e->split_coding.load();
+            const bool split = e->par && (mode == 2 || (mode == 0 && e->par->pool.idle() >= 2));
+            if (split)
                 s.res = rivh::encode_residuals_split_par(s.h_main.as<uint8_t>(), pack, coded.data(), nb,
                                                          [e](int n, const std::function<void(int)>& f) { e->par->pool.parallel(n, f); });
             else
@@ -950,12 +968,37 @@ static int frame_begin(riv_encoder* e, const void* rgb, bool on_device, size_t l
     s.timed = e->profiling >= 2;
     if (s.timed) RIV_HIP(hipEventRecord(s.t0, st));
     const uint8_t* d_rgb = (const uint8_t*)rgb;
-    if (!on_device) {
-        RIV_HIP(hipMemcpyAsync(e->rgb.p, rgb, len, hipMemcpyHostToDevice, st));
-        d_rgb = e->rgb.as<uint8_t>();
+    int ub = -1;   // the upload buffer this frame reads, if any
+    if (!on_device && st == e->st) {
+        if (!e->up) {
+            RIV_HIP(hipStreamCreateWithFlags(&e->up, hipStreamNonBlocking));
+            for (int b = 0; b < 2; b++) {
+                RIV_HIP(hipEventCreateWithFlags(&e->up_ready[b], hipEventDisableTiming | hipEventDisableSystemFence));
+                RIV_HIP(hipEventCreateWithFlags(&e->up_free[b], hipEventDisableTiming | hipEventDisableSystemFence));
+            }
+        }
+        ub = e->up_idx;
+        e->up_idx ^= 1;
+        uint8_t* buf = e->rgb.as<uint8_t>() + (size_t)ub * len;
+        if (e->up_used[ub]) RIV_HIP(hipStreamWaitEvent(e->up, e->up_free[ub], 0));   // read by frame k-2
+        RIV_HIP(hipMemcpyAsync(buf, rgb, len, hipMemcpyHostToDevice, e->up));
+        RIV_HIP(hipEventRecord(e->up_ready[ub], e->up));
+        RIV_HIP(hipStreamWaitEvent(st, e->up_ready[ub], 0));
+        // the caller may reuse its frame once this returns: wait for the copy (from pinned memory
+        // it is asynchronous; it waits for nothing but frame k-2's conversion)
+        RIV_HIP(hipEventSynchronize(e->up_ready[ub]));
+        d_rgb = buf;
+    } else if (!on_device) {
+        uint8_t* buf = e->rgb.as<uint8_t>() + 2 * len;
+        RIV_HIP(hipMemcpyAsync(buf, rgb, len, hipMemcpyHostToDevice, st));
+        d_rgb = buf;
     }
     mark(e, s, st, -1);
     rivk::rgb_to_yuv420(st, d_rgb, (int)w, (int)h, e->W, e->H, f.cY, f.cU, f.cV);
+    if (ub >= 0) {
+        RIV_HIP(hipEventRecord(e->up_free[ub], st));
+        e->up_used[ub] = true;
+    }
     mark(e, s, st, 0);
     if (intra) {
         rivk::intra(st, f.cY, f.cU, f.cV, e->W, e->H, e->yt.as<uint16_t>(), e->ct.as<uint16_t>(), f.nY, f.nU, f.nV, s.h_pack.dev<uint8_t>());
@@ -1053,6 +1096,7 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
             const rivnet::Source src{f.cY, f.rY, f.prev_mv, (int)sizeof(BlockRec), e->W, e->H, e->bw, e->bh};
             RIV_HIP(rivnet::infer(f.st, nullptr, src, nullptr, e->nb, rivnet::net_weights(e->net), e->net_logit, e->cfg.search_range,
                                   e->prop.as<uint4>(), nullptr));
+            RIV_HIP(rivnet::note_use(e->net, f.st));
             p.prop = e->prop.as<uint4>();
             p.lstat = e->lstat.as<unsigned long long>();
         }
@@ -1107,6 +1151,11 @@ static int batch_step(riv_encoder* const* encs, size_t n, const void* const* rgb
             total += e->nb;
             maxp = std::max(maxp, e->s_maxp);
         }
+    }
+    if (!on_device) {   // the callers' host frames may be reused once the step returns
+        if (!e0->b_copy_ev) RIV_HIP(hipEventCreateWithFlags(&e0->b_copy_ev, hipEventDisableTiming));
+        RIV_HIP(hipEventRecord(e0->b_copy_ev, st));
+        RIV_HIP(hipEventSynchronize(e0->b_copy_ev));
     }
     if (!desc.empty()) {
         riv::SearchParams p = search_params(e0);
@@ -1180,6 +1229,7 @@ static int batch_step(riv_encoder* const* encs, size_t n, const void* const* rgb
                 RIV_HIP(e0->b_lstat.alloc(lb)); RIV_HIP(hipMemset(e0->b_lstat.p, 0, lb));
             }
             uint4* bp = e0->b_prop.as<uint4>();
+            std::vector<const riv_mvnet*> used;   // networks read on the batch stream, each noted once
             for (size_t i = 0, d = 0; i < n; i++) {
                 if (f[i].intra) continue;
                 riv_encoder* e = encs[i];
@@ -1187,10 +1237,12 @@ static int batch_step(riv_encoder* const* encs, size_t n, const void* const* rgb
                 if (e->net) {
                     const rivnet::Source src{S.cur, S.ref, f[i].prev_mv, (int)sizeof(BlockRec), e->W, e->H, e->bw, e->bh};
                     RIV_HIP(rivnet::infer(st, nullptr, src, nullptr, e->nb, rivnet::net_weights(e->net), e->net_logit, p.sr, bp + S.blk0, nullptr));
+                    if (std::find(used.begin(), used.end(), e->net) == used.end()) used.push_back(e->net);
                 } else {
                     RIV_HIP(hipMemsetAsync(bp + S.blk0, 0, 16 * (size_t)e->nb, st));
                 }
             }
+            for (const riv_mvnet* net : used) RIV_HIP(rivnet::note_use(net, st));
             p.prop = bp;
             // counters only while the launch fits the counter array (every MI355X shape does)
             p.lstat = sh.groups * sh.waves <= riv::kMaxSearchWaves ? e0->b_lstat.as<unsigned long long>() : nullptr;
@@ -1362,6 +1414,12 @@ int riv_encoder_learned_stats(riv_encoder* e, uint64_t* counts) {
 int riv_encoder_debug_pass_limit(riv_encoder* e, int32_t max_passes) {
     if (!e || max_passes < 0) return RIV_E_ARG;
     e->pass_cap = max_passes;
+    return RIV_OK;
+}
+
+int riv_encoder_debug_split_coding(riv_encoder* e, int32_t mode) {
+    if (!e || mode < 0 || mode > 2) return RIV_E_ARG;
+    e->split_coding = mode;
     return RIV_OK;
 }
 

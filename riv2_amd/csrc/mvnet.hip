@@ -268,8 +268,8 @@ __global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __re
 // Wave w runs the convolutions of blocks 2w and 2w+1 of its group, one at a time, in its own LDS
 // (the 32x32x4 input in bf16, conv1 out 16x16x16, conv2 out 8x8x16); conv3 writes the block's
 // row of the group's fc1 input. Then the eight waves split fc1's 128 outputs (one 16-column tile
-// each) and wave 0 runs the head and writes the proposals. 157.5 KB of LDS: one workgroup of 8
-// waves per CU. Each pixel is converted to bf16 once; a conv fragment is one 8-byte LDS read.
+// each) and wave 0 runs the head and writes the proposals. 163,328 B (159.5 KiB) of LDS, 512 B
+// under a CU's 160 KiB: one workgroup of 8 waves per CU, and no room for a wider tile or pitch. Each pixel is converted to bf16 once; a conv fragment is one 8-byte LDS read.
 namespace {
 constexpr int kInferWaves = 8;
 // Row pitch of a wave's input tile in uint16 entries: 32 pixels x 4 channels and one pixel of

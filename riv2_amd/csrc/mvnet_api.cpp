@@ -66,6 +66,32 @@ struct riv_mvnet {
     mutable std::mutex pk_mu;
     mutable bool pk_stale = true;
 
+    // inferences queued on other streams (the encoders'): the last event recorded after one on each
+    // such stream. Writes of p16, p32 or pk are queued on st behind all of them.
+    mutable std::mutex use_mu;
+    mutable std::vector<std::pair<hipStream_t, hipEvent_t>> uses;
+
+    hipError_t note_use(hipStream_t s) const {
+        std::lock_guard<std::mutex> lk(use_mu);
+        hipEvent_t ev = nullptr;
+        for (auto& u : uses) if (u.first == s) ev = u.second;
+        if (!ev) {
+            hipError_t r = hipEventCreateWithFlags(&ev, hipEventDisableTiming | hipEventDisableSystemFence);
+            if (r != hipSuccess) return r;
+            uses.push_back({s, ev});
+        }
+        return hipEventRecord(ev, s);
+    }
+    // st waits for every noted inference (an event whose stream has gone away has completed)
+    hipError_t after_uses() const {
+        std::lock_guard<std::mutex> lk(use_mu);
+        for (auto& u : uses) {
+            hipError_t r = hipStreamWaitEvent(st, u.second, 0);
+            if (r != hipSuccess) return r;
+        }
+        return hipSuccess;
+    }
+
     // pk is null if the copy could not be made; infer then fails
     rivnet::Weights weights() const {
         const uint16_t* w = p16.as<uint16_t>();
@@ -73,7 +99,8 @@ struct riv_mvnet {
         rivnet::Weights wt{w + rivnet::OFF_W1, w + rivnet::OFF_W2, w + rivnet::OFF_W3, w + rivnet::OFF_W4, w + rivnet::OFF_W5,
                            b + rivnet::OFF_B1, b + rivnet::OFF_B2, b + rivnet::OFF_B3, b + rivnet::OFF_B4, b + rivnet::OFF_B5, nullptr};
         std::lock_guard<std::mutex> lk(pk_mu);
-        if (pk_stale && rivnet::pack_fragments(st, wt, pk.as<uint2>()) == hipSuccess && hipStreamSynchronize(st) == hipSuccess)
+        if (pk_stale && after_uses() == hipSuccess && rivnet::pack_fragments(st, wt, pk.as<uint2>()) == hipSuccess &&
+            hipStreamSynchronize(st) == hipSuccess)
             pk_stale = false;
         if (!pk_stale) wt.pk = pk.as<uint2>();
         return wt;
@@ -86,6 +113,7 @@ struct riv_mvnet {
 
 namespace rivnet {
 Weights net_weights(const riv_mvnet* n) { return n->weights(); }
+hipError_t note_use(const riv_mvnet* n, hipStream_t st) { return n->note_use(st); }
 int net_device(const riv_mvnet* n) { return n->dev; }
 }  // namespace rivnet
 
@@ -96,6 +124,9 @@ constexpr int kSplitRows = 4096;   // rows of the reduction per split of a weigh
 int upload_params(riv_mvnet* n, const float* host) {
     std::vector<uint16_t> h16(NPARAM);
     for (size_t i = 0; i < NPARAM; i++) h16[i] = host_bf16(host[i]);
+    // inferences still queued on encoder streams read the old weights to the end
+    RIV_HIP(n->after_uses());
+    RIV_HIP(hipStreamSynchronize(n->st));
     RIV_HIP(hipMemcpy(n->p32.p, host, NPARAM * 4, hipMemcpyHostToDevice));
     RIV_HIP(hipMemcpy(n->p16.p, h16.data(), NPARAM * 2, hipMemcpyHostToDevice));
     RIV_HIP(hipMemset(n->adam_m.p, 0, NPARAM * 4));
@@ -303,6 +334,10 @@ void riv_mvnet_destroy(riv_mvnet* n) {
     (void)hipSetDevice(n->dev);
     if (n->st) { (void)hipStreamSynchronize(n->st); (void)hipStreamDestroy(n->st); }
     if (n->staged) (void)hipEventDestroy(n->staged);
+    for (auto& u : n->uses) {
+        (void)hipEventSynchronize(u.second);
+        (void)hipEventDestroy(u.second);
+    }
     delete n;
 }
 
@@ -351,6 +386,7 @@ int riv_mvnet_train_step(riv_mvnet* n, const int32_t* samples, const int8_t* tar
     forward(n, (int)count);
     backward(n, (int)count);
     n->step++;
+    RIV_HIP(n->after_uses());   // the update waits for inferences queued on encoder streams
     const float b1 = 0.9f, b2 = 0.999f;
     rivnet::adam(n->st, n->p32.as<float>(), n->adam_m.as<float>(), n->adam_v.as<float>(), n->grad.as<float>(), n->mask.as<uint8_t>(), NPARAM, lr,
                  b1, b2, 1e-8f, 1.0f - std::pow(b1, (float)n->step), 1.0f - std::pow(b2, (float)n->step), n->p16.as<uint16_t>());

@@ -53,3 +53,34 @@ def test_config_defaults_and_validation_on_cpu():
     bad = riv2_amd.EncoderConfig(64, 64, 30).with_keyframe_interval(riv2_amd.KeyframeInterval.fixed(0))
     with pytest.raises(riv2_amd.EncodeError):
         bad.validate()
+
+
+def test_ctypes_structs_match_the_c_header(tmp_path):
+    """Every ctypes mirror of a header struct has the C compiler's size and field offsets
+    (gcc on a probe that includes include/riv_gpu.h)."""
+    import subprocess
+    import shutil
+    import pytest
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    structs = {"riv_encoder_config": _lib.EncoderConfigC, "riv_video_header": _lib.VideoHeaderC,
+               "riv_frame_stats": _lib.FrameStatsC, "riv_decoded_info": _lib.DecodedInfoC,
+               "riv_mv_source": _lib.MvSourceC, "riv_writer_vtbl": _lib.WriterVtblC, "riv_reader_vtbl": _lib.ReaderVtblC}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "riv_gpu.h"', 'int main(void) {']
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f in py._fields_:
+            lines.append(f'printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines.append("return 0; }")
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        s, f, v = line.split()
+        got[(s, f)] = int(v)
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f in py._fields_:
+            assert got[(cname, f[0])] == getattr(py, f[0]).offset, f"{cname}.{f[0]}"

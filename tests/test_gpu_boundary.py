@@ -148,3 +148,44 @@ def test_two_encoders_on_two_threads(depth):
         assert not errs, errs
         for k, (w, h, _) in enumerate(jobs):
             assert outs[k] == _oracle_stream(seqs[k], w, h), f"thread {k} ({w}x{h}) round {rep}"
+
+
+@pytest.mark.parametrize("pinned", [False, True])
+@pytest.mark.parametrize("depth", [0, 4])
+def test_host_frame_memory_is_free_when_the_call_returns(pinned, depth):
+    """riv_encode_frame copies the host frame before it returns (the copy runs on a stream of its
+    own, double-buffered on the GPU): a caller that refills ONE buffer, pageable or pinned, for
+    every frame gets the oracle's stream. Also through a batch step with host frames."""
+    import torch
+    frames = _frames(seed=9, n=8)
+    want = _oracle_stream(frames)
+    if pinned:
+        t = torch.empty(frames[0].size, dtype=torch.uint8, pin_memory=True)
+        buf = t.numpy().reshape(frames[0].shape)
+    else:
+        buf = np.empty_like(frames[0])
+    enc = riv2_amd.Encoder(_cfg(depth))
+    for i, f in enumerate(frames):
+        print(f"[boundary host memory pinned={pinned} depth {depth}] frame {i}", flush=True)
+        buf[...] = f
+        enc.encode_frame(riv2_amd.Frame(buf, W, H, i * 33))
+        buf[...] = 0   # the next frame's data, or garbage, must not reach this frame
+    enc.finish()
+    assert enc.output() == want
+    enc.close()
+    # batch steps of two streams from two reused buffers
+    bufs = [np.empty_like(frames[0]) for _ in range(2)]
+    if pinned:
+        ts = [torch.empty(frames[0].size, dtype=torch.uint8, pin_memory=True) for _ in range(2)]
+        bufs = [x.numpy().reshape(frames[0].shape) for x in ts]
+    encs = [riv2_amd.Encoder(_cfg(depth)) for _ in range(2)]
+    for i, f in enumerate(frames):
+        for b in bufs:
+            b[...] = f
+        riv2_amd.encode_batch_step(encs, bufs, [i * 33] * 2)
+        for b in bufs:
+            b[...] = 0
+    for e in encs:
+        e.finish()
+        assert e.output() == want
+        e.close()

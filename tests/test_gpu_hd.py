@@ -45,3 +45,29 @@ def test_hd_bitstream_and_round_trip():
             assert np.array_equal(p, q), f"frame {i}: decoder output differs from the encoder reconstruction"
         b, _ = rdec.decode()
         assert np.array_equal(a.data, b), f"frame {i}: decoded RGB differs from the oracle decoder"
+
+
+@pytest.fixture(scope="module")
+def hd_oracle_stream():
+    frames = synth.Sequence(synth.hd_params(seed=5, frames=N)).frames(0, N)
+    ref = oracle.OracleEncoder(oracle.config(W, H, fps=50))
+    for i in range(N):
+        ref.encode(frames[i], i * 20)
+    return frames, ref.finish()
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_hd_serial_and_split_residual_coding_write_the_oracle_stream(hd_oracle_stream, mode):
+    """The host codes a P-frame's residuals serially or split across its workers (position-range
+    modelling tasks, then rANS block groups side by side); both forms write the oracle's bytes."""
+    frames, want = hd_oracle_stream
+    cfg = riv2_amd.EncoderConfig(W, H, 50).with_intra_quality(85).with_inter_quality(80).with_search_range(12) \
+        .with_skip_threshold(3).with_keyframe_interval(riv2_amd.KeyframeInterval.fixed(1000))
+    for depth in (0, 8):
+        print(f"[hd split] mode {mode} depth {depth}", flush=True)
+        enc = riv2_amd.Encoder(cfg.with_pipeline_depth(depth))
+        enc.debug_split_coding(mode)
+        for i in range(N):
+            enc.encode_frame(riv2_amd.Frame(frames[i], W, H, i * 20))
+        enc.finish()
+        assert enc.output() == want, f"mode {mode}, pipeline depth {depth}: stream differs from the oracle"

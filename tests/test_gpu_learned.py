@@ -137,6 +137,61 @@ def test_fused_inference_equals_layerwise():
     np.testing.assert_allclose(b[:, :9], a[:, :9], rtol=0, atol=1e-3 * max(1.0, np.abs(a[:, :9]).max()))
 
 
+def test_fused_inference_follows_new_weights():
+    """The fused kernel reads a fragment-order copy of the weights, remade after they change: after
+    new parameters and after a training step it still equals the layer-wise kernels, and the
+    proposals follow the fused forward."""
+    print("[learned] fused after weight changes", flush=True)
+    net = make_net(31)
+    cur, ref, prev = planes(6, 128, 96)
+    _src = with_source(net, cur, ref, prev)
+    nb = (128 // 16) * (96 // 16)
+    samples = np.stack([np.zeros(nb), np.arange(nb)], 1).astype(np.int32)
+
+    def check(label):
+        a = net.forward(samples, fused=False)
+        b = net.forward(samples, fused=True)
+        np.testing.assert_allclose(b[:, :9], a[:, :9], rtol=0, atol=1e-3 * max(1.0, np.abs(a[:, :9]).max()), err_msg=label)
+        c, conf, logit = learned.unpack_proposals(net.propose(samples, 0.5, 12))
+        np.testing.assert_array_equal(logit, b[:, 8], err_msg=label)
+        np.testing.assert_array_equal(c, np.clip(np.rint(8.0 * b[:, :8].reshape(-1, 4, 2)), -12, 12).astype(int), err_msg=label)
+        return b
+
+    y0 = check("fresh network")
+    other = make_net(37)
+    net.set_params(other.get_params())
+    other.close()
+    y1 = check("after set_params")
+    assert not np.array_equal(y0[:, :9], y1[:, :9]), "new parameters did not change the fused output"
+    targets = np.tile(np.array([[3, -2]], np.int8), (nb, 1))
+    net.train_step(samples, targets, 1e-2)
+    y2 = check("after a training step")
+    assert not np.array_equal(y1[:, :9], y2[:, :9]), "a training step did not change the fused output"
+
+
+def test_new_parameters_leave_frames_in_flight_alone():
+    """Pipelined encoder: frames submitted before set_params run on the old parameters (their
+    inference is queued on the encoder's stream; the upload waits for it), so the stream equals an
+    encode that never saw the new parameters."""
+    print("[learned] set_params with frames in flight", flush=True)
+    fr = cif_frames(27, 10)
+    a, b = make_net(41), make_net(43)
+    want, _, _ = encode(fr, a, 1e-6, 8)
+    net = make_net(41)
+    h, w = fr.shape[1:3]
+    enc = riv2_amd.Encoder(learned.hd_config(80, width=w, height=h, depth=8))
+    enc.set_mvnet(net, 1e-6)
+    for k in range(len(fr)):
+        enc.encode_frame(riv2_amd.Frame(fr[k], w, h, 33 * k))
+    net.set_params(b.get_params())   # every frame is submitted; some are still on the GPU
+    enc.finish()
+    out = enc.output()
+    enc.close()
+    assert out == want, "frames already submitted saw the new parameters"
+    for n in (a, b, net):
+        n.close()
+
+
 def test_gradients_match_autograd():
     print("[learned] gradients vs autograd", flush=True)
     net = make_net(7)
