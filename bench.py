@@ -165,9 +165,11 @@ def run_learned(frames, d_frames, steps, warmup, depth, plain_stream, plain_fps)
 
 
 def stage_breakdown(d_frames, n, depth):
-    """Per-stage device times from a separate, untimed pass with events around every stage."""
+    """Per-stage device times from a separate, untimed pass with events around every stage. The
+    pass is synchronous (each frame completes before the next is submitted), so a stage's time is
+    its own and not a wait for another stream's work."""
     import riv2_amd
-    enc = riv2_amd.Encoder(make_config(riv2_amd, depth))
+    enc = riv2_amd.Encoder(make_config(riv2_amd, 0))
     enc.encode_frame_device(d_frames[0].data_ptr(), W * H * 3, W, H, 0)
     enc.flush()
     enc.set_profiling(2)

@@ -107,6 +107,9 @@ typedef struct {
     /* Host timeline of the frame, ms since the encoder was created (no reference counterpart):
      * submitted (its encode call), its GPU work seen finished, its residuals coded, written. */
     double t_submit_ms, t_gpu_done_ms, t_coded_ms, t_written_ms;
+    /* 1 when the frame's residual symbols were modelled on the GPU and only rANS-coded on the host,
+     * 0 when the host modelled them too (no reference counterpart) */
+    uint32_t residual_symbols_on_gpu;
 } riv_frame_stats;
 
 typedef struct riv_encoder riv_encoder;
@@ -166,10 +169,16 @@ int riv_encoder_debug_aq(riv_encoder* enc, float* steps);
  * settled within the cap fails its frame: the frame returns (synchronous mode) or the next call
  * and riv_encoder_flush return (pipelined mode) RIV_E_GPU, and no byte of that frame is written. */
 int riv_encoder_debug_pass_limit(riv_encoder* enc, int32_t max_passes);
-/* Test seam of the host stage (no reference counterpart): how P-frame residuals are coded. 0 (the
- * default): split across the host workers when at least two are idle, else serially; 1: always
- * serially; 2: always split. Every mode writes the same bytes. */
+/* Test seam (no reference counterpart): how P-frame residuals are coded. 0 (the default): their
+ * symbols are modelled on the GPU and the host codes them with rANS32 (batch steps, and frames
+ * with more symbols than the GPU buffers hold, take mode 3); 1: modelled and coded on the host,
+ * serially; 2: on the host, split across its workers; 3: on the host, split when at least two
+ * workers are idle. Every mode writes the same bytes. */
 int riv_encoder_debug_split_coding(riv_encoder* enc, int32_t mode);
+/* Test seam (no reference counterpart): the symbols a P-frame's GPU residual modelling may hold,
+ * at most the encoder's own limit (256 per MB; 0 restores it). A frame with more is coded on the
+ * host, byte for byte the same. */
+int riv_encoder_debug_model_cap(riv_encoder* enc, uint32_t cap);
 
 /* Decoder::new (reitero_decode/src/decoder.rs:112-171): reads and validates the header. */
 int riv_decoder_create(const riv_reader_vtbl* reader, void* user, int32_t device_id, riv_decoder** out);
@@ -241,10 +250,11 @@ int riv_gpu_logf_exp2f(int32_t device_id, const float* in, size_t n, int32_t whi
  * formed inside stage 4, so not recorded), 3 adaptive quantiser, 4 residual prediction/transform/
  * quantise/reconstruct, 5 deblock, 6 intra transform, 7 the whole motion-search launch of batch
  * steps this encoder leads (all streams; the stage-1 figures of encoders in a batch are not the
- * search alone). `on`: 0 off, 1 the motion-search
+ * search alone), 8 the GPU context modelling of a P-frame's residual symbols and their copy to
+ * host memory. `on`: 0 off, 1 the motion-search
  * stage only (two events per frame), 2 every stage and each frame's gpu_ms. Every event costs the
  * stream a few microseconds. */
-#define RIV_NUM_STAGES 8
+#define RIV_NUM_STAGES 9
 int riv_encoder_set_profiling(riv_encoder* enc, int on);
 /* Accumulated milliseconds and launch counts per stage since profiling was switched on. */
 int riv_encoder_stage_times(riv_encoder* enc, double* ms, uint32_t* counts);

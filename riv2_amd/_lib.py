@@ -57,6 +57,7 @@ class FrameStatsC(Structure):
         ("gpu_ms", c_double), ("host_entropy_ms", c_double), ("host_tail_ms", c_double),
         ("me_passes", c_uint32),
         ("t_submit_ms", c_double), ("t_gpu_done_ms", c_double), ("t_coded_ms", c_double), ("t_written_ms", c_double),
+        ("residual_symbols_on_gpu", c_uint32),
     ]
 
 
@@ -109,6 +110,7 @@ PROTOTYPES = [
     ("riv_encoder_debug_aq", c_int, [c_void_p, c_void_p]),
     ("riv_encoder_debug_pass_limit", c_int, [c_void_p, c_int32]),
     ("riv_encoder_debug_split_coding", c_int, [c_void_p, c_int32]),
+    ("riv_encoder_debug_model_cap", c_int, [c_void_p, c_uint32]),
     ("riv_decoder_create", c_int, [POINTER(ReaderVtblC), c_void_p, c_int32, POINTER(c_void_p)]),
     ("riv_decoder_create_mem", c_int, [c_void_p, c_size_t, c_int32, POINTER(c_void_p)]),
     ("riv_decode_frame", c_int, [c_void_p, c_void_p, c_size_t, POINTER(DecodedInfoC)]),

@@ -141,6 +141,7 @@ class EncodeFrameStats:
     me_passes: int = 0
     # host timeline, ms since the encoder was created: submitted, GPU work seen done, coded, written
     timeline: tuple = ()
+    residual_symbols_on_gpu: bool = False
 
     @staticmethod
     def from_c(s):
@@ -148,7 +149,7 @@ class EncodeFrameStats:
                                 s.storage_width, s.storage_height, s.blocks_total, s.blocks_skipped, list(s.mv_mode_counts),
                                 s.mv_new_blocks, s.mv_bias_dx, s.mv_bias_dy, s.rdo_lambda, s.rdo_blocks_evaluated,
                                 s.rdo_blocks_selected_skip, s.gpu_ms, s.host_entropy_ms, s.host_tail_ms, s.me_passes,
-                                (s.t_submit_ms, s.t_gpu_done_ms, s.t_coded_ms, s.t_written_ms))
+                                (s.t_submit_ms, s.t_gpu_done_ms, s.t_coded_ms, s.t_written_ms), bool(s.residual_symbols_on_gpu))
 
 
 def encode_batch_step(encoders, frames, timestamps=None, on_device=False):
@@ -315,7 +316,7 @@ class Encoder:
         _check(L.lib().riv_encoder_output(self._h, byref(p), byref(n)), EncodeError)
         return ctypes.string_at(p, n.value)
 
-    STAGES = ["rgb_to_yuv", "motion_search", "mc", "aq", "residual", "deblock", "intra", "batch_motion_search"]
+    STAGES = ["rgb_to_yuv", "motion_search", "mc", "aq", "residual", "deblock", "intra", "batch_motion_search", "residual_model"]
 
     def set_profiling(self, on=True):
         """0/False off, 1 the motion-search stage only, 2/True every stage."""
@@ -323,8 +324,8 @@ class Encoder:
         _check(L.lib().riv_encoder_set_profiling(self._h, level))
 
     def stage_times(self):
-        ms = np.zeros(8, np.float64)
-        n = np.zeros(8, np.uint32)
+        ms = np.zeros(len(self.STAGES), np.float64)   # RIV_NUM_STAGES entries
+        n = np.zeros(len(self.STAGES), np.uint32)
         _check(L.lib().riv_encoder_stage_times(self._h, ms.ctypes.data, n.ctypes.data))
         return {name: (float(ms[i]), int(n[i])) for i, name in enumerate(self.STAGES)}
 
@@ -384,9 +385,15 @@ class Encoder:
         settle within the cap fails its frame (no bytes are written for it)."""
         _check(L.lib().riv_encoder_debug_pass_limit(self._h, int(max_passes)), EncodeError)
 
+    def debug_model_cap(self, cap: int):
+        """Symbols a P-frame's GPU residual modelling may hold (0: the encoder's limit); a frame
+        with more is coded on the host, with the same bytes."""
+        _check(L.lib().riv_encoder_debug_model_cap(self._h, int(cap)), EncodeError)
+
     def debug_split_coding(self, mode: int):
-        """How P-frame residuals are coded on the host: 0 split across idle workers when two are
-        idle, else serially (the default); 1 always serially; 2 always split. Same bytes."""
+        """How P-frame residuals are coded: 0 modelled on the GPU, rANS32-coded on the host (the
+        default); on the host: 1 serially, 2 split across its workers, 3 split when two workers are
+        idle. Same bytes."""
         _check(L.lib().riv_encoder_debug_split_coding(self._h, int(mode)), EncodeError)
 
     def debug_aq(self):

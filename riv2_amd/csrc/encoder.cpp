@@ -34,6 +34,8 @@ struct Slot {
     HostBuf h_rec, h_pack, h_optskip;    // mapped host memory the kernels write for the host entropy stage
     HostBuf h_main;                      // inter frames: each coded MB's masks (and few values), PACK_MAIN bytes per MB
     HostBuf h_passes;                    // mapped: the motion search's pass count
+    HostBuf h_sym, h_rmh;                // mapped: a P-frame's residual symbols modelled on the GPU, and their count and status
+    bool gpu_model = false;              // this P-frame's residual symbols were modelled on the GPU
     hipEvent_t done = nullptr;           // the frame's kernels (and so its host-side outputs) finished
     hipEvent_t marks[RIV_NUM_STAGES + 1] = {};
     int mark_stage[RIV_NUM_STAGES + 1] = {};
@@ -245,6 +247,21 @@ struct riv_encoder {
     int s_maxp = 0, s_bank = 0;                // pass limit, counter bank
     hipStream_t st = nullptr;    // kernels, frame after frame
     DevBuf rgb, cur, pred, ref[2], var, lnv, steps, mask, field, arrive, bits, rdo, err, yt, ct;
+    // GPU residual modelling of P-frames: the residual kernel's device records and the modelling's
+    // scratch (sized for nb MBs), and the symbols a frame may have
+    DevBuf rm_pack[2], rm_coded[2];   // the residual kernel's records, by frame parity
+    DevBuf rm_cnt, rm_last, rm_ke, rm_bcnt, rm_wst, rm_sigw, rm_eobw, rm_bad, rm_off, rm_boff, rm_tt, rm_tp, rm_ev, rm_sym, rm_hdr;
+    rivk::ResModelBufs rm{};
+    // The modelling runs on a stream of its own, so that it overlaps the next frame's motion
+    // search: frame k's residual kernel writes record buffer k % 2 and records rm_ready; its
+    // modelling waits for that, reads the buffer and records rm_free, which frame k + 2's residual
+    // kernel waits for. Made at the first GPU-modelled frame, with every slot's mapped symbol buffer.
+    bool rm_host = false;
+    hipStream_t ms = nullptr;
+    hipEvent_t rm_ready[2] = {}, rm_free[2] = {};
+    bool rm_used[2] = {};
+    int rm_q = 0;
+    uint32_t rm_cap_max = 0;       // the symbol buffers' capacity
     int ref_idx = 0;
     bool have_ref = false;
     int prev_inter_slot = -1;      // slot holding the previous P-frame's records, -1 after an intra frame
@@ -307,7 +324,7 @@ struct riv_encoder {
     DevBuf prop, lstat;
 
     int pass_cap = 0;              // test seam: relaxation passes allowed per search (0: the proven bound)
-    std::atomic<int> split_coding{0};   // test seam: P-frame residual coding 0 by idle workers, 1 serial, 2 split
+    std::atomic<int> split_coding{0};   // test seam: P-frame residual coding 0 GPU modelling, 1 host serial, 2 host split, 3 host by idle workers
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     double now_ms() const { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
     int profiling = 0;             // 0 off, 1 motion-search stage only, 2 every stage
@@ -448,6 +465,25 @@ static int encoder_init(riv_encoder* e) {
         const size_t lb = 3 * sizeof(unsigned long long) * (size_t)e->shape.groups * e->shape.waves;
         RIV_HIP(e->lstat.alloc(lb)); RIV_HIP(hipMemset(e->lstat.p, 0, lb));
     }
+    {   // GPU residual modelling: up to 256 symbols per MB on average (a heavier frame is coded on the host)
+        const size_t nb = (size_t)e->nb;
+        e->rm.nb = e->nb;
+        e->rm.cap = e->rm_cap_max = (uint32_t)std::min<size_t>(256 * nb, (1u << 28) - 16);
+        for (int q = 0; q < 2; q++) { RIV_HIP(e->rm_pack[q].alloc(riv::PACK_STRIDE * nb)); RIV_HIP(e->rm_coded[q].alloc(nb)); }
+        RIV_HIP(e->rm_cnt.alloc(3 * 2 * nb)); RIV_HIP(e->rm_last.alloc(3 * nb)); RIV_HIP(e->rm_ke.alloc(3 * nb));
+        RIV_HIP(e->rm_bcnt.alloc(7 * 2 * nb)); RIV_HIP(e->rm_wst.alloc(6 * 2 * nb)); RIV_HIP(e->rm_bad.alloc(nb));
+        RIV_HIP(e->rm_sigw.alloc(384 * 4 * nb)); RIV_HIP(e->rm_eobw.alloc(20 * 4 * nb));
+        RIV_HIP(e->rm_off.alloc(4 * nb)); RIV_HIP(e->rm_boff.alloc(7 * 4 * nb)); RIV_HIP(e->rm_ev.alloc(384 * 4 * nb));
+        const size_t ntiles = (nb + 1023) / 1024;
+        RIV_HIP(e->rm_tt.alloc(9 * 4 * ntiles)); RIV_HIP(e->rm_tp.alloc(8 * 4 * ntiles));
+        RIV_HIP(e->rm_sym.alloc(2 * ((size_t)e->rm_cap_max + 8))); RIV_HIP(e->rm_hdr.alloc(64));
+        e->rm.cnt = e->rm_cnt.as<uint16_t>(); e->rm.last = e->rm_last.as<uint8_t>(); e->rm.ke = e->rm_ke.as<uint8_t>();
+        e->rm.bcnt = e->rm_bcnt.as<uint16_t>(); e->rm.wst = e->rm_wst.as<uint16_t>(); e->rm.bad = e->rm_bad.as<uint8_t>();
+        e->rm.sigw = e->rm_sigw.as<uint32_t>(); e->rm.eobw = e->rm_eobw.as<uint32_t>();
+        e->rm.off = e->rm_off.as<uint32_t>(); e->rm.boff = e->rm_boff.as<uint32_t>(); e->rm.ev = e->rm_ev.as<uint32_t>();
+        e->rm.tt = e->rm_tt.as<uint32_t>(); e->rm.tp = e->rm_tp.as<uint32_t>();
+        e->rm.sym = e->rm_sym.as<uint16_t>(); e->rm.hdr = e->rm_hdr.as<uint32_t>();
+    }
     e->depth = c.pipeline_depth;
     e->slots.resize(std::max(2, e->depth));
     for (Slot& s : e->slots) {
@@ -540,6 +576,11 @@ static void destroy_encoder(riv_encoder* e) {
         for (auto& m : s.marks) if (m) (void)hipEventDestroy(m);
     }
     if (e->sync_ev) (void)hipEventDestroy(e->sync_ev);
+    if (e->ms) {
+        (void)hipStreamSynchronize(e->ms);
+        for (int k = 0; k < 2; k++) { (void)hipEventDestroy(e->rm_ready[k]); (void)hipEventDestroy(e->rm_free[k]); }
+        (void)hipStreamDestroy(e->ms);
+    }
     if (e->up) {
         (void)hipStreamSynchronize(e->up);
         for (int b = 0; b < 2; b++) { (void)hipEventDestroy(e->up_ready[b]); (void)hipEventDestroy(e->up_free[b]); }
@@ -637,6 +678,15 @@ static void code_residuals(riv_encoder* e, int si) {
         } else if (s.intra) {
             std::vector<uint8_t> coded(nb, 1);
             s.res = rivh::encode_residuals_packed(pack, riv::PACK_STRIDE, coded.data(), nb, true);
+        } else if (s.gpu_model && s.h_rmh.as<uint32_t>()[1] == 0) {
+            // the GPU modelled the symbols: only their rANS32 coding is left, in groups of sixteen
+            // 16,386-symbol blocks side by side, across idle workers when there are any
+            const uint32_t n = s.h_rmh.as<uint32_t>()[0];
+            st.residual_symbols_on_gpu = 1;
+            const bool par = e->par && e->par->pool.idle() >= 2;
+            s.res = rivh::code_symbol_stream(s.h_sym.as<uint16_t>(), n, par ? rivh::ParallelFor([e](int k, const std::function<void(int)>& f) {
+                e->par->pool.parallel(k, f);
+            }) : rivh::ParallelFor());
         } else {
             const uint8_t* os = s.h_optskip.as<uint8_t>();
             std::vector<uint8_t> coded(nb);
@@ -644,7 +694,7 @@ static void code_residuals(riv_encoder* e, int si) {
             // with idle host workers the frame's coding is split across them (the same bytes; the
             // split costs more work in all, so a pool busy with other frames codes it serially)
             const int mode = e->split_coding.load();
-            const bool split = e->par && (mode == 2 || (mode == 0 && e->par->pool.idle() >= 2));
+            const bool split = e->par && (mode == 2 || ((mode == 0 || mode == 3) && e->par->pool.idle() >= 2));
             if (split)
                 s.res = rivh::encode_residuals_split_par(s.h_main.as<uint8_t>(), pack, coded.data(), nb,
                                                          [e](int n, const std::function<void(int)>& f) { e->par->pool.parallel(n, f); });
@@ -906,6 +956,7 @@ static void release_host_stage() {
 struct FrameCtx {
     int si = 0;
     bool intra = false;
+    bool batch = false;            // part of a batch step (its residuals are coded on the host)
     hipStream_t st = nullptr;
     uint8_t *cY, *cU, *cV, *rY, *rU, *rV, *nY, *nU, *nV, *pY, *pU, *pV;
     const int8_t* prev_mv = nullptr;
@@ -950,6 +1001,7 @@ static int frame_begin(riv_encoder* e, const void* rgb, bool on_device, size_t l
         s.index = e->frame_count; s.ts = ts; s.intra = intra; s.busy = true; s.coded = false; s.modeled = false; s.packed = false;
         s.gpu_ready = false;
         s.failed = false; s.res_failed = false; s.mv_failed = false;
+        s.gpu_model = false;
     }
     s.h_passes.as<int>()[1] = 0;   // the search's error word: the kernel writes it only on failure
     s.t_submit = e->now_ms();
@@ -1044,12 +1096,42 @@ static int frame_after_search(riv_encoder* e, const FrameCtx& f) {
     // the prediction's variances came from the search (lnv); the residual kernel forms the prediction
     rivk::aq(st, e->nb, e->inter_qs, e->lnv.as<float>(), e->steps.as<float>());
     mark(e, s, st, 3);
+    // the residual symbols are modelled on the GPU for single-frame calls (the host codes them for
+    // batch steps, and when the test seam asks it to)
+    s.gpu_model = !f.batch && e->split_coding.load() == 0;
+    int q = -1;   // the record buffer
+    if (s.gpu_model) {
+        if (!e->rm_host) {
+            for (Slot& t : e->slots) { RIV_HIP(t.h_sym.alloc(2 * ((size_t)e->rm_cap_max + 8))); RIV_HIP(t.h_rmh.alloc(64)); }
+            RIV_HIP(hipStreamCreateWithFlags(&e->ms, hipStreamNonBlocking));
+            for (int k = 0; k < 2; k++) {
+                RIV_HIP(hipEventCreateWithFlags(&e->rm_ready[k], hipEventDisableTiming | hipEventDisableSystemFence));
+                RIV_HIP(hipEventCreateWithFlags(&e->rm_free[k], hipEventDisableTiming | hipEventDisableSystemFence));
+            }
+            e->rm_host = true;
+        }
+        q = e->rm_q;
+        e->rm_q ^= 1;
+        if (e->rm_used[q]) RIV_HIP(hipStreamWaitEvent(st, e->rm_free[q], 0));   // frame k - 2's modelling has read it
+    }
     rivk::inter_residual(st, f.cY, f.cU, f.cV, f.rY, f.rU, f.rV, f.pY, f.pU, f.pV, e->W, e->H, rec, e->steps.as<float>(), c.inter_dead_zone,
                          f.nY, f.nU, f.nV,
-                         s.h_main.dev<uint8_t>(), s.h_pack.dev<uint8_t>(), e->mask.as<uint8_t>(), s.h_optskip.dev<uint8_t>());
+                         s.h_main.dev<uint8_t>(), s.h_pack.dev<uint8_t>(), e->mask.as<uint8_t>(), s.h_optskip.dev<uint8_t>(),
+                         q >= 0 ? e->rm_pack[q].as<uint8_t>() : nullptr, q >= 0 ? e->rm_coded[q].as<uint8_t>() : nullptr);
     mark(e, s, st, 4);
     rivk::deblock(st, f.nY, f.nU, f.nV, e->W, e->H, deblock_level(e->inter_qs), false, e->mask.as<uint8_t>());
     mark(e, s, st, 5);
+    if (s.gpu_model) {   // the frame ends on the modelling stream, which waits for all of its work here
+        RIV_HIP(hipEventRecord(e->rm_ready[q], st));
+        RIV_HIP(hipStreamWaitEvent(e->ms, e->rm_ready[q], 0));
+        rivk::ResModelBufs B = e->rm;
+        B.pack = e->rm_pack[q].as<uint8_t>();
+        B.coded = e->rm_coded[q].as<uint8_t>();
+        RIV_HIP(rivk::res_model(e->ms, B, s.h_sym.dev<uint16_t>(), s.h_rmh.dev<uint32_t>()));
+        RIV_HIP(hipEventRecord(e->rm_free[q], e->ms));
+        e->rm_used[q] = true;
+        mark(e, s, e->ms, 8);
+    }
     RIV_HIP(hipGetLastError());
     return RIV_OK;
 }
@@ -1059,8 +1141,9 @@ static int frame_end(riv_encoder* e, const FrameCtx& f, riv_frame_stats* stats) 
     Slot& s = e->slots[f.si];
     // the kernels wrote the records, skip flags and packed coefficients straight to the slot's
     // mapped host memory: the frame is ready for the host stage once the stream reaches here
-    if (s.timed) RIV_HIP(hipEventRecord(s.t1, f.st));
-    RIV_HIP(hipEventRecord(s.done, f.st));
+    const hipStream_t last = !f.intra && s.gpu_model ? e->ms : f.st;   // the frame's last work
+    if (s.timed) RIV_HIP(hipEventRecord(s.t1, last));
+    RIV_HIP(hipEventRecord(s.done, last));
     e->prev_inter_slot = f.intra ? -1 : f.si;
     e->ref_idx = 1 - e->ref_idx;
     e->have_ref = true;
@@ -1144,6 +1227,7 @@ static int batch_step(riv_encoder* const* encs, size_t n, const void* const* rgb
     int total = 0, maxp = 0;
     for (size_t i = 0; i < n; i++) {
         riv_encoder* e = encs[i];
+        f[i].batch = true;
         int r = frame_begin(e, rgb[i], on_device, len[i], e->cfg.display_width, e->cfg.display_height, ts ? ts[i] : 0, st, f[i]);
         if (r) return r;
         if (!f[i].intra) {
@@ -1417,8 +1501,15 @@ int riv_encoder_debug_pass_limit(riv_encoder* e, int32_t max_passes) {
     return RIV_OK;
 }
 
+int riv_encoder_debug_model_cap(riv_encoder* e, uint32_t cap) {
+    if (!e) return RIV_E_ARG;
+    std::lock_guard<std::mutex> lk(e->mu);
+    e->rm.cap = cap == 0 ? e->rm_cap_max : std::min(cap, e->rm_cap_max);
+    return RIV_OK;
+}
+
 int riv_encoder_debug_split_coding(riv_encoder* e, int32_t mode) {
-    if (!e || mode < 0 || mode > 2) return RIV_E_ARG;
+    if (!e || mode < 0 || mode > 3) return RIV_E_ARG;
     e->split_coding = mode;
     return RIV_OK;
 }

@@ -670,7 +670,9 @@ std::vector<uint8_t> encode_residuals_split_par(const uint8_t* main, const uint8
         }
     });
     for (const std::string& e : errs) if (!e.empty()) throw CorruptStream(e);
-    // 2. the merged order: per MB the seven segments; starts by prefix sum, copies in parallel
+    // 2. the merged order (per MB the seven segments; starts by prefix sum), built and rANS-coded
+    // in groups of sixteen blocks of 16,386 symbols (the stream's blocks are independent): each
+    // group task copies the part of the merged order it covers, then codes it
     auto seg_bounds = [&](int k, int s, uint32_t& a, uint32_t& b) {
         const int t = kSegTask[s], per = kCoefRanges[t].luma ? 1 : 2, i = k * per + kSegPart[s];
         const std::vector<uint32_t>& e = S->seg[t];
@@ -685,28 +687,27 @@ std::vector<uint8_t> encode_residuals_split_par(const uint8_t* main, const uint8
     }
     S->dst[n] = total;
     if (S->merged.size() < total) S->merged.resize(total);
-    const int nparts = std::min(8, n);
-    pfor(nparts, [&](int p) {
-        for (int k = (int)((int64_t)n * p / nparts); k < (int)((int64_t)n * (p + 1) / nparts); k++) {
-            uint16_t* d = S->merged.data() + S->dst[k];
-            for (int s = 0; s < kSegs; s++) {
-                uint32_t a, b;
-                seg_bounds(k, s, a, b);
-                std::memcpy(d, S->sym[kSegTask[s]].data() + a, sizeof(uint16_t) * (b - a));
-                d += b - a;
-            }
-        }
-    });
-    // 3. rANS: groups of eight blocks (the streams' blocks are independent), then the bytes in order
-    constexpr size_t kGroup = 8 * 16386;
+    constexpr size_t kGroup = 16 * 16386;
     const int ng = (int)((total + kGroup - 1) / kGroup);
     if ((int)S->chunk.size() < ng) S->chunk.resize(ng);
     pfor(ng, [&](int g) {
+        const uint32_t lo = (uint32_t)((size_t)g * kGroup), hi = (uint32_t)std::min((size_t)total, (size_t)(g + 1) * kGroup);
+        // the MBs overlapping [lo, hi): from the last one starting at or before lo
+        int k = (int)(std::upper_bound(S->dst.begin(), S->dst.begin() + n, lo) - S->dst.begin()) - 1;
+        for (; k < n && S->dst[k] < hi; k++) {
+            uint32_t at = S->dst[k];
+            for (int s = 0; s < kSegs; s++) {
+                uint32_t a, b;
+                seg_bounds(k, s, a, b);
+                const uint32_t c0 = std::max(at, lo), c1 = std::min(at + (b - a), hi);   // this segment's part in range
+                if (c0 < c1) std::memcpy(S->merged.data() + c0, S->sym[kSegTask[s]].data() + a + (c0 - at), sizeof(uint16_t) * (c1 - c0));
+                at += b - a;
+            }
+        }
         std::vector<uint8_t>& o = S->chunk[g];
         o.clear();
         RansWriter w(o);
-        const size_t b = (size_t)g * kGroup;
-        w.code_symbols(S->merged.data() + b, std::min(kGroup, (size_t)total - b));
+        w.code_symbols(S->merged.data() + lo, hi - lo);
     });
     size_t bytes = 0;
     for (int g = 0; g < ng; g++) bytes += S->chunk[g].size();
@@ -714,6 +715,29 @@ std::vector<uint8_t> encode_residuals_split_par(const uint8_t* main, const uint8
     for (int g = 0; g < ng; g++) out.insert(out.end(), S->chunk[g].begin(), S->chunk[g].end());
     std::lock_guard<std::mutex> lk(g_par_mu);
     g_par_free.push_back(std::move(S));
+    return out;
+}
+
+std::vector<uint8_t> code_symbol_stream(const uint16_t* sym, size_t n, const ParallelFor& pfor) {
+    std::vector<uint8_t> out;
+    if (n == 0) return out;
+    constexpr size_t kGroup = 16 * 16386;
+    const int ng = (int)((n + kGroup - 1) / kGroup);
+    if (ng == 1 || !pfor) {
+        RansWriter w(out);
+        w.code_symbols(sym, n);
+        return out;
+    }
+    std::vector<std::vector<uint8_t>> chunk(ng);
+    pfor(ng, [&](int g) {
+        RansWriter w(chunk[g]);
+        const size_t b = (size_t)g * kGroup;
+        w.code_symbols(sym + b, std::min(kGroup, n - b));
+    });
+    size_t bytes = 0;
+    for (const auto& c : chunk) bytes += c.size();
+    out.reserve(bytes);
+    for (const auto& c : chunk) out.insert(out.end(), c.begin(), c.end());
     return out;
 }
 

@@ -138,6 +138,11 @@ using ParallelFor = std::function<void(int n, const std::function<void(int)>& fn
 // parallel groups.
 std::vector<uint8_t> encode_residuals_split_par(const uint8_t* main, const uint8_t* over, const uint8_t* coded, int nb,
                                                 const ParallelFor& pfor);
+// rANS32 coding of a stream's whole symbol sequence (n symbols, each start << 8 | freq, as the
+// residual coder buffers them): its 16386-symbol blocks, each coded from fresh states, in groups
+// of sixteen, by parallel tasks when pfor is set. An empty sequence gives an empty stream (no
+// coder is instantiated when every block is skipped, residual.rs:670-674).
+std::vector<uint8_t> code_symbol_stream(const uint16_t* sym, size_t n, const ParallelFor& pfor);
 // One MB of the split form as 384 coefficients.
 void unpack_split(const uint8_t* main, const uint8_t* over, int b, int16_t* q);
 void unpack_coeffs(const uint8_t* slot, int16_t* q384);

@@ -1399,7 +1399,7 @@ __global__ __launch_bounds__(256) void k_inter_residual(const uint8_t* cY, const
                                                         uint8_t* pY, uint8_t* pU, uint8_t* pV,
                                                         int W, int H, const BlockRec* rec, const float* steps, float dz,
                                                         uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* main, uint8_t* over,
-                                                        uint8_t* mask, uint8_t* host_opt_skip) {
+                                                        uint8_t* mask, uint8_t* host_opt_skip, uint8_t* dpack, uint8_t* dcoded) {
     __shared__ ResidLds L;
     const int b = blockIdx.x, bw = W >> 4, bx = b % bw, by = b / bw, t = threadIdx.x, cw = W >> 1, ch = H >> 1;
     const int x0 = bx * 16, y0 = by * 16;
@@ -1431,7 +1431,7 @@ __global__ __launch_bounds__(256) void k_inter_residual(const uint8_t* cY, const
     if (rec[b].rdo_skip) {   // RDO skip: prediction only
         oY[o] = (uint8_t)predY;
         if (t < 128) (pl ? oV : oU)[oc] = (uint8_t)predC;
-        if (t == 0) { mask[b] = *(const uint32_t*)rec[b].r != 0u; host_opt_skip[b] = 1; }
+        if (t == 0) { mask[b] = *(const uint32_t*)rec[b].r != 0u; host_opt_skip[b] = 1; if (dcoded) dcoded[b] = 0; }
         return;
     }
     L.res[t] = (int16_t)((int)cY[o] - predY);
@@ -1445,11 +1445,12 @@ __global__ __launch_bounds__(256) void k_inter_residual(const uint8_t* cY, const
     if (!nz) {   // optimised skip (residual.rs:618-674)
         oY[o] = (uint8_t)predY;
         if (t < 128) (pl ? oV : oU)[oc] = (uint8_t)predC;
-        if (t == 0) { mask[b] = *(const uint32_t*)rec[b].r != 0u; host_opt_skip[b] = 1; }
+        if (t == 0) { mask[b] = *(const uint32_t*)rec[b].r != 0u; host_opt_skip[b] = 1; if (dcoded) dcoded[b] = 0; }
         return;
     }
-    if (t == 0) { mask[b] = 1; host_opt_skip[b] = 0; }
+    if (t == 0) { mask[b] = 1; host_opt_skip[b] = 0; if (dcoded) dcoded[b] = 1; }
     write_packed_split(L.coef, main + (size_t)b * riv::PACK_MAIN, over + (size_t)b * riv::PACK_STRIDE, t);
+    if (dpack) write_packed(L.coef, dpack + (size_t)b * riv::PACK_STRIDE, t);
     recon_mb(L, t, qs, false, nullptr, nullptr, pY, pU, pV, oY, oU, oV, W, x0, y0, predY, predC);
 }
 
@@ -1737,9 +1738,10 @@ void logf_exp2f(hipStream_t st, const float* in, size_t n, int which, float* out
 }
 void inter_residual(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, const uint8_t* rY, const uint8_t* rU,
                     const uint8_t* rV, uint8_t* pY, uint8_t* pU, uint8_t* pV, int W, int H, const BlockRec* rec, const float* steps, float dz,
-                    uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* main, uint8_t* over, uint8_t* mask, uint8_t* host_opt_skip) {
+                    uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* main, uint8_t* over, uint8_t* mask, uint8_t* host_opt_skip,
+                    uint8_t* dpack, uint8_t* dcoded) {
     hipLaunchKernelGGL(k_inter_residual, dim3((W / 16) * (H / 16)), dim3(256), 0, st, cY, cU, cV, rY, rU, rV, pY, pU, pV, W, H, rec, steps, dz,
-                       oY, oU, oV, main, over, mask, host_opt_skip);
+                       oY, oU, oV, main, over, mask, host_opt_skip, dpack, dcoded);
 }
 void intra(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, int W, int H, const uint16_t* yt, const uint16_t* ct,
            uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* pack) {

@@ -84,3 +84,9 @@ def test_ctypes_structs_match_the_c_header(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(py), cname
         for f in py._fields_:
             assert got[(cname, f[0])] == getattr(py, f[0]).offset, f"{cname}.{f[0]}"
+
+
+def test_stage_names_cover_the_header_stages():
+    text = open(os.path.join(ROOT, "include", "riv_gpu.h")).read()
+    n = int(re.search(r"#define RIV_NUM_STAGES (\d+)", text).group(1))
+    assert len(riv2_amd.Encoder.STAGES) == n

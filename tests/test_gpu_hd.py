@@ -56,10 +56,11 @@ def hd_oracle_stream():
     return frames, ref.finish()
 
 
-@pytest.mark.parametrize("mode", [1, 2])
-def test_hd_serial_and_split_residual_coding_write_the_oracle_stream(hd_oracle_stream, mode):
-    """The host codes a P-frame's residuals serially or split across its workers (position-range
-    modelling tasks, then rANS block groups side by side); both forms write the oracle's bytes."""
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_hd_residual_coding_modes_write_the_oracle_stream(hd_oracle_stream, mode):
+    """A P-frame's residual symbols modelled on the GPU (mode 0, the default: the host only
+    rANS-codes them), or modelled and coded on the host serially (1), split across its workers (2)
+    or split when workers are idle (3): every form writes the oracle's bytes."""
     frames, want = hd_oracle_stream
     cfg = riv2_amd.EncoderConfig(W, H, 50).with_intra_quality(85).with_inter_quality(80).with_search_range(12) \
         .with_skip_threshold(3).with_keyframe_interval(riv2_amd.KeyframeInterval.fixed(1000))
@@ -67,7 +68,36 @@ def test_hd_serial_and_split_residual_coding_write_the_oracle_stream(hd_oracle_s
         print(f"[hd split] mode {mode} depth {depth}", flush=True)
         enc = riv2_amd.Encoder(cfg.with_pipeline_depth(depth))
         enc.debug_split_coding(mode)
+        on_gpu = []
         for i in range(N):
-            enc.encode_frame(riv2_amd.Frame(frames[i], W, H, i * 20))
+            st = enc.encode_frame_with_stats(riv2_amd.Frame(frames[i], W, H, i * 20))
+            on_gpu.append(st.residual_symbols_on_gpu)
+        if depth:
+            enc.flush()
+            on_gpu = [st.residual_symbols_on_gpu for st in enc.pop_stats()]
         enc.finish()
         assert enc.output() == want, f"mode {mode}, pipeline depth {depth}: stream differs from the oracle"
+        assert on_gpu == [False] + [mode == 0] * (N - 1), on_gpu
+
+
+@pytest.mark.parametrize("cap", [1, 600_000])
+def test_hd_frames_beyond_the_gpu_symbol_buffer_are_coded_on_the_host(hd_oracle_stream, cap):
+    """A P-frame with more residual symbols than the GPU modelling may hold is coded on the host
+    from the same coefficients: with a 1-symbol buffer every P-frame is, with 600,000 symbols the
+    heavier frames are; the stream is the oracle's either way."""
+    frames, want = hd_oracle_stream
+    cfg = riv2_amd.EncoderConfig(W, H, 50).with_intra_quality(85).with_inter_quality(80).with_search_range(12) \
+        .with_skip_threshold(3).with_keyframe_interval(riv2_amd.KeyframeInterval.fixed(1000))
+    for depth in (0, 8):
+        print(f"[hd model cap] cap {cap} depth {depth}", flush=True)
+        enc = riv2_amd.Encoder(cfg.with_pipeline_depth(depth))
+        enc.debug_model_cap(cap)
+        for i in range(N):
+            enc.encode_frame(riv2_amd.Frame(frames[i], W, H, i * 20))
+        enc.flush()
+        on_gpu = [st.residual_symbols_on_gpu for st in enc.pop_stats()] if depth else None
+        enc.finish()
+        assert enc.output() == want, f"cap {cap}, pipeline depth {depth}: stream differs from the oracle"
+        if depth:
+            print("on GPU", on_gpu, flush=True)
+            assert not on_gpu[0] and (cap > 1 or not any(on_gpu)), on_gpu
