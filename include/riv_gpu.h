@@ -33,7 +33,7 @@ typedef enum {
     RIV_E_INVALID_FRAME = 13,     /* DecodeError::InvalidFrame */
     RIV_E_END_OF_STREAM = 14,     /* DecodeError::EndOfStream */
     RIV_E_GPU = 20,               /* HIP runtime / kernel failure (no reference counterpart) */
-    RIV_E_UNSUPPORTED = 21,       /* valid reference config outside this backend's limits (e.g. search_range > 64) */
+    RIV_E_UNSUPPORTED = 21,       /* valid reference config outside this backend's limits (search_range > 63) */
     RIV_E_ARG = 22                /* NULL handle / bad argument */
 } riv_status;
 

@@ -432,7 +432,9 @@ int riv_encoder_config_validate(const riv_encoder_config* c) {
 static int encoder_init(riv_encoder* e) {
     const riv_encoder_config& c = e->cfg;
     if (c.storage_width % 16 || c.storage_height % 16) { set_error("storage dims must be multiples of 16"); return RIV_E_INVALID_CONFIG; }
-    if (c.search_range > 32) { set_error("this backend supports search_range <= 32"); return RIV_E_UNSUPPORTED; }
+    // MV words hold whole-pixel components in 8 bits, and a NEW block's reconstructed vector equals
+    // its search vector while |search - base| fits the delta (every vector within [-sr-1, sr])
+    if (c.search_range > 63) { set_error("this backend supports search_range <= 63"); return RIV_E_UNSUPPORTED; }
     int r = ensure_device(c.device_id);
     if (r) return r;
     e->W = (int)c.storage_width; e->H = (int)c.storage_height;

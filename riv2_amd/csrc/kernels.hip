@@ -292,7 +292,7 @@ __device__ __forceinline__ int mv_class(int mag) {   // mv_class_from_magnitude 
 // top-right and top-left MBs and of the co-located MB of the previous P-frame (derive_mv_predictors_with_stats / gather_mv_candidates,
 // mv_predictor.rs:135-272). A reconstructed MV equals the search MV: for a matched mode it is
 // the matched candidate, which equals the search MV; for NEW it is base + clamp(search - base),
-// and with every MV inside [-sr-1, sr] (sr <= 32) the difference never reaches the clamp. So
+// and with every MV inside [-sr-1, sr] (sr <= 63) the difference never reaches the clamp. So
 // the whole frame's decisions follow from the search field in parallel. The identity is still
 // checked for every NEW block: a violation sets *err = 2 (and the stream's *serr, when given).
 __device__ __forceinline__ uint4 mode_record(uint32_t wS, int best_sad, bool has_l, bool has_t, bool has_tr, bool has_tl, bool has_tp,

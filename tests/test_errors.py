@@ -67,7 +67,7 @@ def test_encoder_frame_and_config_errors():
     enc.close()
     # a valid reference config outside this backend's limits is refused, not mis-encoded
     with pytest.raises(riv2_amd.EncodeError) as ei:
-        riv2_amd.Encoder(riv2_amd.EncoderConfig(64, 48, 30).with_search_range(40))
+        riv2_amd.Encoder(riv2_amd.EncoderConfig(64, 48, 30).with_search_range(64))
     assert ei.value.kind == "Unsupported"
 
 

@@ -14,7 +14,7 @@ def _cfg(w, h, sr=12, q_intra=85, q_inter=80, skip=3):
         .with_search_range(sr).with_skip_threshold(skip).with_keyframe_interval(riv2_amd.KeyframeInterval.fixed(1000))
 
 
-def _run_pair(frames, w, h, sr=12, q_intra=85, q_inter=80, skip=3, label=""):
+def _run_pair(frames, w, h, sr=12, q_intra=85, q_inter=80, skip=3, label="", mv_extent=None):
     enc = riv2_amd.Encoder(_cfg(w, h, sr, q_intra, q_inter, skip))
     ref = oracle.OracleEncoder(oracle.config(w, h, intra_quality=q_intra, inter_quality=q_inter, search_range=sr, skip_threshold=skip))
     for i, f in enumerate(frames):
@@ -28,6 +28,8 @@ def _run_pair(frames, w, h, sr=12, q_intra=85, q_inter=80, skip=3, label=""):
         if i > 0:
             grec, gsad, grdo, gopt = enc.debug_blocks()
             orec, osad, ordo, oopt = ref.last_blocks()
+            if mv_extent is not None:
+                mv_extent.append(int(np.abs(orec[:, :2].astype(np.int8).astype(int)).max()))
             bad = np.argwhere((grec[:, :4] != orec[:, :4]).any(1))
             assert bad.size == 0, f"[{label}] frame {i}: search MV differs at blocks {bad[:5].ravel().tolist()} gpu={grec[bad[:3].ravel(), :4].tolist()} ref={orec[bad[:3].ravel(), :4].tolist()}"
             assert np.array_equal(gsad, osad), f"[{label}] frame {i}: best SAD differs"
@@ -76,6 +78,18 @@ def test_search_range_and_quality_variants():
     for sr, qi, qp, sk in ((4, 60, 40, 3), (24, 95, 90, 0), (16, 30, 10, 12)):
         data = _run_pair(frames, 320, 176, sr=sr, q_intra=qi, q_inter=qp, skip=sk, label=f"sr{sr}")
         _decode_pair(data, len(frames), label=f"sr{sr}")
+
+
+def test_search_ranges_beyond_32():
+    """Search ranges 33-63 (the reference allows up to 127; this backend up to 63): content moving up
+    to 40 px per frame, so the long vectors occur; every stage equals the oracle's."""
+    frames = synth.Sequence(synth.custom_params(320, 176, 4, seed=8, global_motion=24.0, n_rects=4, rect_min=24, rect_max=64,
+                                                rect_velocity=40.0)).frames()
+    for sr in (40, 63):
+        ext = []
+        data = _run_pair(frames, 320, 176, sr=sr, label=f"sr{sr}", mv_extent=ext)
+        _decode_pair(data, len(frames), label=f"sr{sr}")
+        assert max(ext) > 32, f"sr{sr}: no vector beyond 32 px ({ext})"
 
 
 def test_pipelined_encoder_same_bitstream():
