@@ -195,10 +195,31 @@ typedef struct {
 int riv_decode_frame(riv_decoder* dec, uint8_t* rgb_out, size_t cap, riv_decoded_info* info);
 /* Decoder::decode_frame_null (decoder.rs:463-470) */
 int riv_decoder_decode_frame_null(riv_decoder* dec);
-/* Decoder::has_more_frames / header / set_skip_residuals (decoder.rs:472-499) */
+/* Decoder::has_more_frames / header / set_skip_residuals / skip_residuals / current_frame
+ * (decoder.rs:470-499) */
 int riv_decoder_has_more_frames(const riv_decoder* dec);
 int riv_decoder_header(const riv_decoder* dec, riv_video_header* out);
 int riv_decoder_set_skip_residuals(riv_decoder* dec, int skip);
+int riv_decoder_skip_residuals(const riv_decoder* dec);
+uint64_t riv_decoder_current_frame(const riv_decoder* dec);
+/* DecodePhaseTimings (decoder.rs:73-87), nanoseconds summed since the last drain. Host phases, on
+ * whichever thread ran them: read_bits (reading frame records), parse_frame (record headers and
+ * vector reconstruction), mv_decode (MV rANS), residual_rans (residual rANS). Device phases, from
+ * HIP events on the decoder's stream, only while riv_decoder_set_profiling is on (else 0):
+ * build_pred (motion compensation), residual_apply (dequantisation, inverse transforms and the
+ * add: one fused kernel, so residual_dct_y and residual_dct_uv stay 0; with the deblocking
+ * filter), yuv_to_rgb (conversion and the copy out). residual_deinterleave is 0: the residual
+ * decoder writes coefficients in place. residual = the residual counters' sum, as the reference
+ * defines it. */
+typedef struct {
+    uint64_t read_bits_ns, parse_frame_ns, mv_decode_ns, build_pred_ns, residual_ns, residual_rans_ns, residual_deinterleave_ns,
+        residual_dct_y_ns, residual_dct_uv_ns, residual_apply_ns, yuv_to_rgb_ns;
+} riv_decode_timings;
+/* Decoder::drain_timings (decoder.rs:485-489): the timings, then zero. */
+int riv_decoder_drain_timings(riv_decoder* dec, riv_decode_timings* out);
+/* Device-phase timing of riv_decoder_drain_timings (no reference counterpart): 1 on, 0 off (the
+ * default; each timed phase costs the stream two HIP events). */
+int riv_decoder_set_profiling(riv_decoder* dec, int on);
 /* Last decoded frame's storage-size YUV planes (before RGB conversion). */
 int riv_decoder_debug_yuv(riv_decoder* dec, uint8_t* y, uint8_t* u, uint8_t* v);
 void riv_decoder_destroy(riv_decoder* dec);

@@ -78,6 +78,12 @@ class MvSourceC(Structure):
                 ("width", c_uint32), ("height", c_uint32)]
 
 
+class DecodeTimingsC(Structure):
+    _fields_ = [(n, c_uint64) for n in ("read_bits_ns", "parse_frame_ns", "mv_decode_ns", "build_pred_ns", "residual_ns", "residual_rans_ns",
+                                        "residual_deinterleave_ns", "residual_dct_y_ns", "residual_dct_uv_ns", "residual_apply_ns",
+                                        "yuv_to_rgb_ns")]
+
+
 class WriterVtblC(Structure):
     _fields_ = [("write", WRITE_FN), ("position", POS_FN), ("seek", SEEK_FN), ("flush", FLUSH_FN)]
 
@@ -118,6 +124,10 @@ PROTOTYPES = [
     ("riv_decoder_has_more_frames", c_int, [c_void_p]),
     ("riv_decoder_header", c_int, [c_void_p, POINTER(VideoHeaderC)]),
     ("riv_decoder_set_skip_residuals", c_int, [c_void_p, c_int]),
+    ("riv_decoder_skip_residuals", c_int, [c_void_p]),
+    ("riv_decoder_current_frame", c_uint64, [c_void_p]),
+    ("riv_decoder_drain_timings", c_int, [c_void_p, POINTER(DecodeTimingsC)]),
+    ("riv_decoder_set_profiling", c_int, [c_void_p, c_int]),
     ("riv_decoder_debug_yuv", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     ("riv_decoder_destroy", None, [c_void_p]),
     ("riv_gpu_build_predicted", c_int, [c_int32, c_void_p, c_void_p, c_void_p, c_uint32, c_uint32, c_void_p, c_void_p, c_void_p, c_void_p]),

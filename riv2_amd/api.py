@@ -479,6 +479,23 @@ class Decoder:
     def set_skip_residuals(self, skip: bool):
         _check(L.lib().riv_decoder_set_skip_residuals(self._h, int(bool(skip))), DecodeError)
 
+    def skip_residuals(self) -> bool:
+        return bool(L.lib().riv_decoder_skip_residuals(self._h))
+
+    def current_frame(self) -> int:
+        """Index of the next frame decode_frame returns (decoder.rs:474-477)."""
+        return int(L.lib().riv_decoder_current_frame(self._h))
+
+    def set_profiling(self, on: bool = True):
+        """Time the device phases for drain_timings (HIP events on the decoder's stream)."""
+        _check(L.lib().riv_decoder_set_profiling(self._h, int(bool(on))), DecodeError)
+
+    def drain_timings(self) -> dict:
+        """DecodePhaseTimings (decoder.rs:73-87, :485-489): nanoseconds per phase since the last drain."""
+        t = L.DecodeTimingsC()
+        _check(L.lib().riv_decoder_drain_timings(self._h, byref(t)), DecodeError)
+        return {name: int(getattr(t, name)) for name, _ in L.DecodeTimingsC._fields_}
+
     def debug_yuv(self):
         y = np.zeros((self._sh, self._sw), np.uint8)
         u = np.zeros((self._sh // 2, self._sw // 2), np.uint8)

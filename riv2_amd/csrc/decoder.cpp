@@ -1,6 +1,8 @@
 // Decoder driver of the C ABI: Decoder::decode_frame of reitero_decode
 // (reitero_decode/src/decoder.rs). Entropy decoding and MV reconstruction run on the host;
 // motion compensation, dequantisation, IDCT, reconstruction, deblock and YUV -> RGB on the GPU.
+#include <atomic>
+#include <chrono>
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <condition_variable>
@@ -66,9 +68,19 @@ struct riv_decoder {
     bool parsed_intra = false;        // the host stage has seen the first (intra) frame
     DecodePipe* pipe = nullptr;       // in-memory streams, once decoding starts
     HostFrame inline_frame;           // reader streams: the one frame in flight
+    // phase timings (ns since the last drain); host phases may run on the look-ahead threads
+    std::atomic<uint64_t> t_read{0}, t_parse{0}, t_mv{0}, t_rans{0}, t_pred{0}, t_apply{0}, t_rgb{0};
+    bool profiling = false;
+    hipEvent_t ev[4] = {};            // device phases: before MC, after MC, after the reconstruction, after RGB out
 
     // read_exact (decoder.rs:502-511): a short read is EndOfStream
     bool read_exact(uint8_t* dst, size_t n) {
+        const auto t0 = std::chrono::steady_clock::now();
+        const bool ok = read_exact_(dst, n);
+        t_read += (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count();
+        return ok;
+    }
+    bool read_exact_(uint8_t* dst, size_t n) {
         if (mem) {
             if (mempos + n > membuf.size()) return false;
             std::memcpy(dst, membuf.data() + mempos, n);
@@ -137,7 +149,22 @@ int riv_decoder_create_mem(const uint8_t* data, size_t len, int32_t device_id, r
 // decode_next (decoder.rs:195-459), in two stages.
 // Host stage, part 1 (frame order): reads the next record, decodes and reconstructs its vectors
 // (decoder.rs:195-382). The residual payload is kept for part 2.
+static uint64_t ns_since(std::chrono::steady_clock::time_point t0) {
+    return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count();
+}
+
+static int parse_frame_(riv_decoder* d, HostFrame& f);
+// The record and its vectors; parse_frame time excludes the reads and the MV rANS.
 static int parse_frame(riv_decoder* d, HostFrame& f) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint64_t r0 = d->t_read.load(), m0 = d->t_mv.load();
+    const int r = parse_frame_(d, f);
+    const uint64_t all = ns_since(t0), inner = (d->t_read.load() - r0) + (d->t_mv.load() - m0);
+    d->t_parse += all > inner ? all - inner : 0;
+    return r;
+}
+
+static int parse_frame_(riv_decoder* d, HostFrame& f) {
     uint8_t head[9];
     if (!d->read_exact(head, 9)) { set_error("end of stream"); return RIV_E_END_OF_STREAM; }
     f.ts = get_u64(head);
@@ -168,7 +195,9 @@ static int parse_frame(riv_decoder* d, HostFrame& f) {
             f.payload.resize(get_u32(rs));
             if (!d->read_exact(f.payload.data(), f.payload.size())) return RIV_E_END_OF_STREAM;
             if (!d->parsed_intra) { set_error("Inter frame before first intra frame"); return RIV_E_INVALID_FRAME; }
+            const auto tm0 = std::chrono::steady_clock::now();
             std::vector<rivh::MvBlock> blocks = d->mvc.decode(mvp.data(), mvp.size(), bw, bh);
+            d->t_mv += ns_since(tm0);
             // MV reconstruction (decoder.rs:316-382): base from the predictors of the growing list
             f.mv.assign(nb, Mv4{});
             f.skip.assign(nb, 0); f.coded.assign(nb, 0); f.mask.assign(nb, 0);
@@ -220,7 +249,9 @@ static int parse_frame(riv_decoder* d, HostFrame& f) {
 // decode_inter, residual.rs:815-966) into the frame's coefficient buffer.
 static int decode_frame_residuals(riv_decoder* d, HostFrame& f) {
     int16_t* hco = f.coeffs.as<int16_t>();
+    const auto t0 = std::chrono::steady_clock::now();
     std::fill(hco, hco + (size_t)d->nb * 384, 0);
+    struct Timed { riv_decoder* d; std::chrono::steady_clock::time_point t0; ~Timed() { d->t_rans += ns_since(t0); } } timed{d, t0};
     try {
         rivh::decode_residuals(f.payload.data(), f.payload.size(), f.coded.data(), d->nb, f.ft == 1, hco);
     } catch (const rivh::CorruptStream& ex) {
@@ -243,16 +274,22 @@ static int reconstruct(riv_decoder* d, HostFrame& f, uint8_t* rgb_out, riv_decod
     uint8_t* pY = d->pred.as<uint8_t>(); uint8_t* pU = pY + ysz; uint8_t* pV = pU + csz;
     hipStream_t st = d->st;
     const float qs = quant_step(f.q);
+    const bool prof = d->profiling;
+    if (prof && !d->ev[0])
+        for (auto& e : d->ev) RIV_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+    if (prof) RIV_HIP(hipEventRecord(d->ev[0], st));
     if (f.ft == 1) {
         RIV_HIP(hipMemcpyAsync(d->yt.p, f.yt, sizeof(f.yt), hipMemcpyHostToDevice, st));
         RIV_HIP(hipMemcpyAsync(d->ct.p, f.ct, sizeof(f.ct), hipMemcpyHostToDevice, st));
         RIV_HIP(hipMemcpyAsync(d->coeffs.p, f.coeffs.p, 768 * (size_t)nb, hipMemcpyHostToDevice, st));
+        if (prof) RIV_HIP(hipEventRecord(d->ev[1], st));   // an intra frame has no prediction
         rivk::intra_recon(st, W, H, d->yt.as<uint16_t>(), d->ct.as<uint16_t>(), d->coeffs.as<int16_t>(), nY, nU, nV);
         if (!d->skip_residuals) rivk::deblock(st, nY, nU, nV, W, H, deblock_level(qs), true, nullptr);
     } else {
         if (!d->have_ref) { set_error("Inter frame before first intra frame"); return RIV_E_INVALID_FRAME; }
         RIV_HIP(hipMemcpyAsync(d->mvs.p, f.mv.data(), 4 * (size_t)nb, hipMemcpyHostToDevice, st));
         rivk::mc(st, rY, rU, rV, W, H, d->mvs.as<int8_t>(), 4, pY, pU, pV, d->var.as<uint32_t>(), d->lnv.as<float>());
+        if (prof) RIV_HIP(hipEventRecord(d->ev[1], st));
         if (d->skip_residuals) {
             RIV_HIP(hipMemcpyAsync(nY, pY, ysz + 2 * csz, hipMemcpyDeviceToDevice, st));
         } else {
@@ -266,13 +303,24 @@ static int reconstruct(riv_decoder* d, HostFrame& f, uint8_t* rgb_out, riv_decod
     }
     d->ref_idx = 1 - d->ref_idx;
     d->have_ref = true;
+    if (prof) RIV_HIP(hipEventRecord(d->ev[2], st));
     const int dw = (int)d->hdr.display_width, dh = (int)d->hdr.display_height;
     if (rgb_out) {
         rivk::yuv420_to_rgb(st, nY, nU, nV, W, dw, dh, d->rgb.as<uint8_t>());
         RIV_HIP(hipMemcpyAsync(rgb_out, d->rgb.p, (size_t)dw * dh * 3, hipMemcpyDeviceToHost, st));
     }
+    if (prof) RIV_HIP(hipEventRecord(d->ev[3], st));
     RIV_HIP(hipGetLastError());
     RIV_HIP(hipStreamSynchronize(st));   // the frame's host buffers are free again after this
+    if (prof) {
+        float a = 0, b = 0, c = 0;
+        RIV_HIP(hipEventElapsedTime(&a, d->ev[0], d->ev[1]));
+        RIV_HIP(hipEventElapsedTime(&b, d->ev[1], d->ev[2]));
+        RIV_HIP(hipEventElapsedTime(&c, d->ev[2], d->ev[3]));
+        d->t_pred += (uint64_t)(a * 1e6);
+        d->t_apply += (uint64_t)(b * 1e6);
+        d->t_rgb += (uint64_t)(c * 1e6);
+    }
     if (info) { info->width = (uint32_t)dw; info->height = (uint32_t)dh; info->timestamp_ms = f.ts; info->frame_index = d->current; info->frame_type = f.ft; }
     return RIV_OK;
 }
@@ -397,6 +445,27 @@ int riv_decoder_decode_frame_null(riv_decoder* d) {
     return RIV_OK;
 }
 int riv_decoder_has_more_frames(const riv_decoder* d) { return d && d->current < d->hdr.frame_count; }
+int riv_decoder_skip_residuals(const riv_decoder* d) { return d && d->skip_residuals ? 1 : 0; }
+uint64_t riv_decoder_current_frame(const riv_decoder* d) { return d ? d->current : 0; }
+int riv_decoder_drain_timings(riv_decoder* d, riv_decode_timings* out) {
+    if (!d || !out) return RIV_E_ARG;
+    riv_decode_timings t{};
+    t.read_bits_ns = d->t_read.exchange(0);
+    t.parse_frame_ns = d->t_parse.exchange(0);
+    t.mv_decode_ns = d->t_mv.exchange(0);
+    t.residual_rans_ns = d->t_rans.exchange(0);
+    t.build_pred_ns = d->t_pred.exchange(0);
+    t.residual_apply_ns = d->t_apply.exchange(0);
+    t.yuv_to_rgb_ns = d->t_rgb.exchange(0);
+    t.residual_ns = t.residual_rans_ns + t.residual_deinterleave_ns + t.residual_dct_y_ns + t.residual_dct_uv_ns + t.residual_apply_ns;
+    *out = t;
+    return RIV_OK;
+}
+int riv_decoder_set_profiling(riv_decoder* d, int on) {
+    if (!d) return RIV_E_ARG;
+    d->profiling = on != 0;
+    return RIV_OK;
+}
 int riv_decoder_header(const riv_decoder* d, riv_video_header* out) {
     if (!d || !out) return RIV_E_ARG;
     *out = d->hdr;
@@ -422,6 +491,7 @@ void riv_decoder_destroy(riv_decoder* d) {
     stop_pipe(d);
     (void)hipSetDevice(d->dev);
     if (d->st) { (void)hipStreamSynchronize(d->st); (void)hipStreamDestroy(d->st); }
+    for (auto& e : d->ev) if (e) (void)hipEventDestroy(e);
     delete d;
 }
 

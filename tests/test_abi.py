@@ -65,7 +65,8 @@ def test_ctypes_structs_match_the_c_header(tmp_path):
         pytest.skip("no C compiler")
     structs = {"riv_encoder_config": _lib.EncoderConfigC, "riv_video_header": _lib.VideoHeaderC,
                "riv_frame_stats": _lib.FrameStatsC, "riv_decoded_info": _lib.DecodedInfoC,
-               "riv_mv_source": _lib.MvSourceC, "riv_writer_vtbl": _lib.WriterVtblC, "riv_reader_vtbl": _lib.ReaderVtblC}
+               "riv_mv_source": _lib.MvSourceC, "riv_writer_vtbl": _lib.WriterVtblC, "riv_reader_vtbl": _lib.ReaderVtblC,
+               "riv_decode_timings": _lib.DecodeTimingsC}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "riv_gpu.h"', 'int main(void) {']
     for cname, py in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')

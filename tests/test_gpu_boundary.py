@@ -189,3 +189,29 @@ def test_host_frame_memory_is_free_when_the_call_returns(pinned, depth):
         e.finish()
         assert e.output() == want
         e.close()
+
+
+@pytest.mark.parametrize("use_reader", [False, True])
+def test_decoder_current_frame_and_phase_timings(use_reader):
+    """current_frame counts the frames returned; drain_timings returns the phase sums and resets
+    them (decoder.rs:474-489); the device phases are measured only with profiling on."""
+    data = _oracle_stream(_frames())
+    dec = riv2_amd.Decoder(reader=BytesReader(data)) if use_reader else riv2_amd.Decoder(data)
+    assert dec.current_frame() == 0 and not dec.skip_residuals()
+    dec.decode_frame()
+    dec.decode_frame()
+    assert dec.current_frame() == 2
+    t = dec.drain_timings()
+    assert t["read_bits_ns"] > 0 and t["parse_frame_ns"] > 0 and t["mv_decode_ns"] > 0 and t["residual_rans_ns"] > 0
+    assert t["build_pred_ns"] == t["residual_apply_ns"] == t["yuv_to_rgb_ns"] == 0
+    assert t["residual_ns"] == t["residual_rans_ns"]
+    dec.set_profiling(True)
+    dec.decode_frame()
+    t2 = dec.drain_timings()
+    assert t2["build_pred_ns"] > 0 and t2["residual_apply_ns"] > 0 and t2["yuv_to_rgb_ns"] > 0
+    assert t2["residual_ns"] == t2["residual_rans_ns"] + t2["residual_apply_ns"]
+    if use_reader:   # frames are decoded in the calls (an in-memory stream's look-ahead runs on)
+        assert all(v == 0 for v in dec.drain_timings().values())
+    dec.set_skip_residuals(True)
+    assert dec.skip_residuals()
+    dec.close()
