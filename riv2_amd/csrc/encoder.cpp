@@ -36,6 +36,7 @@ struct Slot {
     HostBuf h_passes;                    // mapped: the motion search's pass count
     HostBuf h_sym, h_rmh;                // mapped: a P-frame's residual symbols modelled on the GPU, and their count and status
     bool gpu_model = false;              // this P-frame's residual symbols were modelled on the GPU
+    int rm_q = 0;                        // then: its record buffer on the device
     hipEvent_t done = nullptr;           // the frame's kernels (and so its host-side outputs) finished
     hipEvent_t marks[RIV_NUM_STAGES + 1] = {};
     int mark_stage[RIV_NUM_STAGES + 1] = {};
@@ -689,6 +690,13 @@ static void code_residuals(riv_encoder* e, int si) {
             s.res = rivh::code_symbol_stream(s.h_sym.as<uint16_t>(), n, par ? rivh::ParallelFor([e](int k, const std::function<void(int)>& f) {
                 e->par->pool.parallel(k, f);
             }) : rivh::ParallelFor());
+        } else if (s.gpu_model) {
+            // the GPU could not model this frame (too many symbols, or a magnitude beyond the class
+            // table): it sent the full records instead, and the host codes them (or rejects them)
+            const uint8_t* os = s.h_optskip.as<uint8_t>();
+            std::vector<uint8_t> coded(nb);
+            for (int i = 0; i < nb; i++) coded[i] = !os[i];
+            s.res = rivh::encode_residuals_packed(pack, riv::PACK_STRIDE, coded.data(), nb, false);
         } else {
             const uint8_t* os = s.h_optskip.as<uint8_t>();
             std::vector<uint8_t> coded(nb);
@@ -1116,10 +1124,13 @@ static int frame_after_search(riv_encoder* e, const FrameCtx& f) {
         e->rm_q ^= 1;
         if (e->rm_used[q]) RIV_HIP(hipStreamWaitEvent(st, e->rm_free[q], 0));   // frame k - 2's modelling has read it
     }
+    // GPU-modelled frames keep their coefficient records on the device (the host gets them only if
+    // it has to code the frame itself); host-coded frames get them in mapped host memory
+    s.rm_q = q;
     rivk::inter_residual(st, f.cY, f.cU, f.cV, f.rY, f.rU, f.rV, f.pY, f.pU, f.pV, e->W, e->H, rec, e->steps.as<float>(), c.inter_dead_zone,
                          f.nY, f.nU, f.nV,
-                         s.h_main.dev<uint8_t>(), s.h_pack.dev<uint8_t>(), e->mask.as<uint8_t>(), s.h_optskip.dev<uint8_t>(),
-                         q >= 0 ? e->rm_pack[q].as<uint8_t>() : nullptr, q >= 0 ? e->rm_coded[q].as<uint8_t>() : nullptr);
+                         q >= 0 ? nullptr : s.h_main.dev<uint8_t>(), q >= 0 ? nullptr : s.h_pack.dev<uint8_t>(), e->mask.as<uint8_t>(),
+                         s.h_optskip.dev<uint8_t>(), q >= 0 ? e->rm_pack[q].as<uint8_t>() : nullptr, q >= 0 ? e->rm_coded[q].as<uint8_t>() : nullptr);
     mark(e, s, st, 4);
     rivk::deblock(st, f.nY, f.nU, f.nV, e->W, e->H, deblock_level(e->inter_qs), false, e->mask.as<uint8_t>());
     mark(e, s, st, 5);
@@ -1129,7 +1140,7 @@ static int frame_after_search(riv_encoder* e, const FrameCtx& f) {
         rivk::ResModelBufs B = e->rm;
         B.pack = e->rm_pack[q].as<uint8_t>();
         B.coded = e->rm_coded[q].as<uint8_t>();
-        RIV_HIP(rivk::res_model(e->ms, B, s.h_sym.dev<uint16_t>(), s.h_rmh.dev<uint32_t>()));
+        RIV_HIP(rivk::res_model(e->ms, B, s.h_sym.dev<uint16_t>(), s.h_rmh.dev<uint32_t>(), s.h_pack.dev<uint8_t>()));
         RIV_HIP(hipEventRecord(e->rm_free[q], e->ms));
         e->rm_used[q] = true;
         mark(e, s, e->ms, 8);
@@ -1564,9 +1575,17 @@ int riv_encoder_debug_coeffs(riv_encoder* e, int16_t* out) {
     if (!e || e->last_slot < 0) return RIV_E_ARG;
     const Slot& s = e->slots[e->last_slot];
     const uint8_t* os = s.h_optskip.as<uint8_t>();
+    std::vector<uint8_t> dev;   // a GPU-modelled frame's records, from the device (the last frame's are intact)
+    if (!s.intra && s.gpu_model) {
+        RIV_HIP(hipSetDevice(e->cfg.device_id));
+        RIV_HIP(hipStreamSynchronize(e->st));
+        dev.resize((size_t)riv::PACK_STRIDE * e->nb);
+        RIV_HIP(hipMemcpy(dev.data(), e->rm_pack[s.rm_q].p, dev.size(), hipMemcpyDeviceToHost));
+    }
     for (int b = 0; b < e->nb; b++) {
         int16_t* q = out + (size_t)b * 384;
         if (!s.intra && os[b]) std::memset(q, 0, 768);
+        else if (!dev.empty()) rivh::unpack_coeffs(dev.data() + (size_t)b * riv::PACK_STRIDE, q);
         else if (!s.intra) rivh::unpack_split(s.h_main.as<uint8_t>(), s.h_pack.as<uint8_t>(), b, q);
         else rivh::unpack_coeffs(s.h_pack.as<uint8_t>() + (size_t)b * riv::PACK_STRIDE, q);
     }

@@ -1449,7 +1449,7 @@ __global__ __launch_bounds__(256) void k_inter_residual(const uint8_t* cY, const
         return;
     }
     if (t == 0) { mask[b] = 1; host_opt_skip[b] = 0; if (dcoded) dcoded[b] = 1; }
-    write_packed_split(L.coef, main + (size_t)b * riv::PACK_MAIN, over + (size_t)b * riv::PACK_STRIDE, t);
+    if (main) write_packed_split(L.coef, main + (size_t)b * riv::PACK_MAIN, over + (size_t)b * riv::PACK_STRIDE, t);
     if (dpack) write_packed(L.coef, dpack + (size_t)b * riv::PACK_STRIDE, t);
     recon_mb(L, t, qs, false, nullptr, nullptr, pY, pU, pV, oY, oU, oV, W, x0, y0, predY, predC);
 }

@@ -42,7 +42,8 @@ void inter_residual(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const 
 // PACK_MAIN_VALUES) at main + b * PACK_MAIN, else its values at over + b * PACK_STRIDE + 48
 // mask: per MB the deblock filter flag (not optimised-skip, or a non-zero reconstructed MV; encoder.rs:831-845)
 // dpack / dcoded (optional, device memory): each coded MB's full record (PACK_STRIDE bytes: the six
-// masks, then the values) and per MB 1 when its residual is coded, for the GPU residual modelling
+// masks, then the values) and per MB 1 when its residual is coded, for the GPU residual modelling;
+// main and over may then be null (no host copy of the records)
 // pack: riv::PACK_STRIDE bytes per MB (mapped host memory); the layout is in riv_common.hpp
 void intra(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, int W, int H, const uint16_t* yt, const uint16_t* ct,
            uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* pack);
@@ -77,6 +78,7 @@ struct ResModelBufs {
     uint32_t* hdr = nullptr;        // 16 words
 };
 // host_hdr[0]: the frame's symbols, host_hdr[1]: status (0: host_sym holds them; 1: a magnitude
-// beyond the class table; 2: more than cap symbols); both in mapped host memory.
-hipError_t res_model(hipStream_t st, const ResModelBufs& b, uint16_t* host_sym, uint32_t* host_hdr);
+// beyond the class table; 2: more than cap symbols: host_pack then holds the MBs' records, nb *
+// PACK_STRIDE bytes); all in mapped host memory.
+hipError_t res_model(hipStream_t st, const ResModelBufs& b, uint16_t* host_sym, uint32_t* host_hdr, uint8_t* host_pack);
 }  // namespace rivk
