@@ -410,19 +410,20 @@ __global__ __launch_bounds__(384) void k_rm_walk(const uint8_t* __restrict__ pac
 // significance (256..319), luma EOB (320..327), chroma EOB (328..333), magnitude per band and
 // context (334..417). A chain's uses go 64 per round (row chains: MB l of each 64, or for U then V
 // rows MB l / 2 of each 32, plane l & 1; magnitude chains: event l of each 64), in batches of 64
-// rounds, each in three phases:
-//   A. the waves load the batch's words (eight rounds each, loads issued together) and keep each
-//      round's use and bit masks;
-//   B. wave 0 walks the rounds in order and records each one's entry state: a round in which no
-//      counter can reach 255 (z + zeros <= 255 and o + ones <= 255) only adds its counts;
-//   C. the waves form their rounds' symbols from the entry states, and store them.
-constexpr int kChainWaves = 8, kChainBatch = 64;
+// rounds, through three phases:
+//   A. the words are loaded (rounds spread over the waves, loads issued together) and each
+//      round's use and bit masks kept;
+//   B. one wave finds each round's entry state (see below);
+//   C. the rounds' symbols are formed from their entry states and stored.
+// Only B is serial, so the phases are pipelined over three LDS buffers: while wave 0 runs B on
+// batch k, waves 1..7 run C on batch k - 1 and A on batch k + 1, one barrier per step.
+constexpr int kChainWaves = 8, kChainBatch = 64, kChainBufs = 3;
 __global__ __launch_bounds__(64 * kChainWaves) void k_rm_chains(int nb, const uint32_t* __restrict__ sigw, const uint32_t* __restrict__ eobw,
                                                                 const uint32_t* __restrict__ ev, const uint32_t* __restrict__ hdr,
                                                                 uint16_t* __restrict__ sym) {
-    __shared__ uint32_t words[kChainBatch * 64];
-    __shared__ unsigned long long am[kChainBatch][2];
-    __shared__ int entry[kChainBatch][2];
+    __shared__ uint32_t words[kChainBufs][kChainBatch * 64];
+    __shared__ unsigned long long am[kChainBufs][kChainBatch][2];
+    __shared__ int entry[kChainBufs][kChainBatch][2];
     __shared__ uint32_t rcp[512];
     if (hdr[1]) return;
     const int c = blockIdx.x, t = threadIdx.x, lane = t & 63;
@@ -458,88 +459,101 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_rm_chains(int nb, const ui
         if (b >= nb) return 0u;
         return (r1 && (lane & 1)) ? r1[b] : r0[b];
     };
-    int z = 1, o = 1;   // wave 0: the context's counts (a fresh context: one zero, one one)
-    for (int rb = 0; rb < rounds; rb += kChainBatch) {
-        const int nr = min(kChainBatch, rounds - rb);
-        {   // A
-            constexpr int per = kChainBatch / kChainWaves;
-            uint32_t w[per];
+    // A for batch k by waves w0, w0 + nw, ...: rounds w - w0, w - w0 + nw, ... of the batch
+    auto phase_a = [&](int k, int w0, int nw) {
+        const int bb = k % kChainBufs, rb = k * kChainBatch, nr = min(kChainBatch, rounds - rb);
+        constexpr int kMax = (kChainBatch + kChainWaves - 2) / (kChainWaves - 1);   // rounds per wave with 7 waves
+        uint32_t w[kMax];
 #pragma unroll
-            for (int j = 0; j < per; j++) {
-                const int rr = wv + kChainWaves * j;
-                w[j] = rr < nr ? word(rb + rr) : 0u;
-            }
-#pragma unroll
-            for (int j = 0; j < per; j++) {
-                const int rr = wv + kChainWaves * j;
-                words[rr * 64 + lane] = w[j];
-                const unsigned long long A = __ballot((w[j] & kUsed) != 0u), Om = __ballot((w[j] & (kUsed | 1u)) == (kUsed | 1u));
-                if (lane == 0) { am[rr][0] = A; am[rr][1] = Om; }
-            }
+        for (int j = 0; j < kMax; j++) {
+            const int rr = wv - w0 + nw * j;
+            w[j] = rr < nr ? word(rb + rr) : 0u;
         }
-        __syncthreads();
-        if (wv == 0) {
-            // B: lane r holds round r's masks and counts. From a known state before round `from`,
-            // the entry states of the following rounds are that state plus the counts in between
-            // (one prefix sum per batch), up to the first round in which a counter reaches 255;
-            // that round's exit comes from round_exit, and the walk resumes after it. From a state
-            // that a run of one bit leaves as it is ((255, 1) or (1, 255)), every round up to the
-            // first with the other bit enters with it.
-            const unsigned long long Al = lane < nr ? am[lane][0] : 0ull, Ol = lane < nr ? am[lane][1] : 0ull;
-            const int nzl = __popcll(Al & ~Ol), nol = __popcll(Ol);
-            int xz = nzl, xo = nol;
 #pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const int yz = __shfl_up(xz, d, 64), yo = __shfl_up(xo, d, 64);
-                if (lane >= d) { xz += yz; xo += yo; }
-            }
-            xz -= nzl;   // counts of the rounds before this lane's round
-            xo -= nol;
-            const int alo = (int)(uint32_t)Al, ahi = (int)(uint32_t)(Al >> 32), olo = (int)(uint32_t)Ol, ohi = (int)(uint32_t)(Ol >> 32);
-            auto exit_of = [&](int f) {   // round f's exit from (z, o), its entry state
-                const unsigned long long A = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane(ahi, f) << 32) |
-                                             (uint32_t)__builtin_amdgcn_readlane(alo, f);
-                const unsigned long long Om = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane(ohi, f) << 32) |
-                                              (uint32_t)__builtin_amdgcn_readlane(olo, f);
-                round_exit(z, o, A, Om, lane);
-            };
-            int ez = 0, eo = 0;
-            int from = 0;
-            while (from < nr) {
-                const bool open = lane >= from && lane < nr;
-                if ((z == 255 && o == 1) || (z == 1 && o == 255)) {
-                    const unsigned long long brk = __ballot(open && (z == 255 ? nol : nzl) > 0);
+        for (int j = 0; j < kMax; j++) {
+            const int rr = wv - w0 + nw * j;
+            if (rr >= nr) break;
+            words[bb][rr * 64 + lane] = w[j];
+            const unsigned long long A = __ballot((w[j] & kUsed) != 0u), Om = __ballot((w[j] & (kUsed | 1u)) == (kUsed | 1u));
+            if (lane == 0) { am[bb][rr][0] = A; am[bb][rr][1] = Om; }
+        }
+    };
+    const int nbat = (rounds + kChainBatch - 1) / kChainBatch;
+    if (nbat > 0) phase_a(0, 0, kChainWaves);
+    __syncthreads();
+    int z = 1, o = 1;   // wave 0: the context's counts (a fresh context: one zero, one one)
+    for (int k = 0; k <= nbat; k++) {
+        if (wv == 0) {
+            if (k < nbat) {
+                const int bb = k % kChainBufs, nr = min(kChainBatch, rounds - k * kChainBatch);
+                {
+                // B: lane r holds round r's masks and counts. From a known state before round `from`,
+                // the entry states of the following rounds are that state plus the counts in between
+                // (one prefix sum per batch), up to the first round in which a counter reaches 255;
+                // that round's exit comes from round_exit, and the walk resumes after it. From a state
+                // that a run of one bit leaves as it is ((255, 1) or (1, 255)), every round up to the
+                // first with the other bit enters with it.
+                const unsigned long long Al = lane < nr ? am[bb][lane][0] : 0ull, Ol = lane < nr ? am[bb][lane][1] : 0ull;
+                const int nzl = __popcll(Al & ~Ol), nol = __popcll(Ol);
+                int xz = nzl, xo = nol;
+    #pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const int yz = __shfl_up(xz, d, 64), yo = __shfl_up(xo, d, 64);
+                    if (lane >= d) { xz += yz; xo += yo; }
+                }
+                xz -= nzl;   // counts of the rounds before this lane's round
+                xo -= nol;
+                const int alo = (int)(uint32_t)Al, ahi = (int)(uint32_t)(Al >> 32), olo = (int)(uint32_t)Ol, ohi = (int)(uint32_t)(Ol >> 32);
+                auto exit_of = [&](int f) {   // round f's exit from (z, o), its entry state
+                    const unsigned long long A = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane(ahi, f) << 32) |
+                                                 (uint32_t)__builtin_amdgcn_readlane(alo, f);
+                    const unsigned long long Om = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane(ohi, f) << 32) |
+                                                  (uint32_t)__builtin_amdgcn_readlane(olo, f);
+                    round_exit(z, o, A, Om, lane);
+                };
+                int ez = 0, eo = 0;
+                int from = 0;
+                while (from < nr) {
+                    const bool open = lane >= from && lane < nr;
+                    if ((z == 255 && o == 1) || (z == 1 && o == 255)) {
+                        const unsigned long long brk = __ballot(open && (z == 255 ? nol : nzl) > 0);
+                        const int f = brk ? __ffsll(brk) - 1 : nr;
+                        if (open && lane <= f) { ez = z; eo = o; }
+                        if (f >= nr) break;
+                        exit_of(f);
+                        from = f + 1;
+                        continue;
+                    }
+                    const int oz = __builtin_amdgcn_readlane(xz, from), oo = __builtin_amdgcn_readlane(xo, from);
+                    const int zr = z + xz - oz, orr = o + xo - oo;
+                    const unsigned long long brk = __ballot(open && (zr + nzl > 255 || orr + nol > 255));
                     const int f = brk ? __ffsll(brk) - 1 : nr;
-                    if (open && lane <= f) { ez = z; eo = o; }
-                    if (f >= nr) break;
+                    if (open && lane <= f) { ez = zr; eo = orr; }
+                    if (f >= nr) {
+                        z = __builtin_amdgcn_readlane(zr + nzl, nr - 1);
+                        o = __builtin_amdgcn_readlane(orr + nol, nr - 1);
+                        break;
+                    }
+                    z = __builtin_amdgcn_readlane(zr, f);
+                    o = __builtin_amdgcn_readlane(orr, f);
                     exit_of(f);
                     from = f + 1;
-                    continue;
                 }
-                const int oz = __builtin_amdgcn_readlane(xz, from), oo = __builtin_amdgcn_readlane(xo, from);
-                const int zr = z + xz - oz, orr = o + xo - oo;
-                const unsigned long long brk = __ballot(open && (zr + nzl > 255 || orr + nol > 255));
-                const int f = brk ? __ffsll(brk) - 1 : nr;
-                if (open && lane <= f) { ez = zr; eo = orr; }
-                if (f >= nr) {
-                    z = __builtin_amdgcn_readlane(zr + nzl, nr - 1);
-                    o = __builtin_amdgcn_readlane(orr + nol, nr - 1);
-                    break;
-                }
-                z = __builtin_amdgcn_readlane(zr, f);
-                o = __builtin_amdgcn_readlane(orr, f);
-                exit_of(f);
-                from = f + 1;
+                if (lane < nr) { entry[bb][lane][0] = ez; entry[bb][lane][1] = eo; }
             }
-            if (lane < nr) { entry[lane][0] = ez; entry[lane][1] = eo; }
-        }
-        __syncthreads();
-        for (int rr = wv; rr < nr; rr += kChainWaves) {   // C
-            const uint32_t x = words[rr * 64 + lane];
-            int zz = entry[rr][0], oo = entry[rr][1];
-            uint16_t out = 0;
-            chain_round(zz, oo, (x & kUsed) != 0u, (int)(x & 1u), out, lane, rcp);
-            if (x & kUsed) sym[(x >> 1) & 0x3fffffffu] = out;
+            }
+        } else {
+            if (k >= 1) {   // C for batch k - 1
+                const int bb = (k - 1) % kChainBufs, nr = min(kChainBatch, rounds - (k - 1) * kChainBatch);
+                for (int rr = wv - 1; rr < nr; rr += kChainWaves - 1) {
+                    const uint32_t x = words[bb][rr * 64 + lane];
+                    int zz = entry[bb][rr][0], oo = entry[bb][rr][1];
+                    uint16_t out = 0;
+                    chain_round(zz, oo, (x & kUsed) != 0u, (int)(x & 1u), out, lane, rcp);
+                    if (x & kUsed) sym[(x >> 1) & 0x3fffffffu] = out;
+                }
+            }
+            if (k + 1 < nbat) phase_a(k + 1, 1, kChainWaves - 1);
         }
         __syncthreads();
     }
