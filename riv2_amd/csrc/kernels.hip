@@ -1669,11 +1669,11 @@ void hp_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int
 size_t search_lds_bytes(int sr, int waves, int nstreams, int nslot) {
     return riv::search_slots_offset(sr, nstreams) + (size_t)waves * nslot * riv::search_slot_dwords(sr) * sizeof(uint32_t);
 }
-bool search_shape(int sr, int nb, int nstreams, int cus, size_t budget, SearchShape* out) {
-    // one resident workgroup per CU at most, as many waves per workgroup (up to 16) as hold one
-    // slot each, at least one wave per 64 blocks; then as many slots per wave as fit, up to the
+bool search_shape(int sr, int nb, int nstreams, int cus, size_t budget, SearchShape* out, int max_waves) {
+    // one resident workgroup per CU at most, as many waves per workgroup (up to max_waves) as hold
+    // one slot each, at least one wave per 64 blocks; then as many slots per wave as fit, up to the
     // blocks a wave owns
-    int waves = 16;
+    int waves = std::max(1, std::min(16, max_waves));
     while (waves > 1 && search_lds_bytes(sr, waves, nstreams, 1) > budget) waves--;
     if (search_lds_bytes(sr, waves, nstreams, 1) > budget) return false;
     const int groups = std::max(1, std::min(cus, (nb + waves - 1) / waves));

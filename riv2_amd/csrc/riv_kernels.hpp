@@ -19,9 +19,10 @@ void hp_table(hipStream_t st, const uint8_t* cur, const uint8_t* ref, int W, int
 size_t search_lds_bytes(int sr, int waves, int nstreams, int nslot);
 // Launch shape of the motion search for nb blocks over nstreams streams: workgroups (at most one
 // per CU), waves per workgroup, window slots per wave (nres of them resident) and LDS bytes.
-// False when the blocks do not fit in resident waves or the LDS budget.
+// False when the blocks do not fit in resident waves or the LDS budget. Workgroups have at most
+// max_waves waves (16, the most a workgroup can have).
 struct SearchShape { int waves, groups, nslot, nres; size_t lds; };
-bool search_shape(int sr, int nb, int nstreams, int cus, size_t budget, SearchShape* out);
+bool search_shape(int sr, int nb, int nstreams, int cus, size_t budget, SearchShape* out, int max_waves = 16);
 // The launch shapes an autotuner may try for the same search (search_shape's first); every one
 // reaches the same fixed point, so the choice changes only the time.
 std::vector<SearchShape> search_shapes(int sr, int nb, int nstreams, int cus, size_t budget);
