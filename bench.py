@@ -57,13 +57,14 @@ def make_config(riv2_amd, depth):
         .with_skip_threshold(3).with_keyframe_interval(riv2_amd.KeyframeInterval.fixed(1000)).with_pipeline_depth(depth)
 
 
+PROFILE_EVERY = 4   # timed frames per motion-search launch timed by events
 LEARNED_PASSES = 4   # the learned form's relaxation passes per motion search (profiles/learned_mv_eval.json)
 
 
 def run_gpu(frames, steps, warmup, depth, net=None, d_frames=None, confidence=0.5):
     """Times `steps` P-frames (after one I-frame and `warmup` P-frames) through the pipelined
-    encoder, inputs resident in HBM; the motion-search stage is bracketed by HIP events on the
-    encoder's stream. With `net`, the learned form (the network's proposals seed the search).
+    encoder, inputs resident in HBM; the motion-search launch of every PROFILE_EVERY-th timed frame
+    records HIP events at its start and end on the encoder's stream. With `net`, the learned form (the network's proposals seed the search).
     Returns elapsed seconds, per-frame stats, stage times, the stream, the device frames and the
     learned-form counts."""
     import torch
@@ -84,7 +85,10 @@ def run_gpu(frames, steps, warmup, depth, net=None, d_frames=None, confidence=0.
         enc.encode_frame_device(ptr(i), fbytes, W, H, i * 20)
     enc.flush()
     enc.pop_stats(4096)
-    enc.set_profiling(1)   # two events per frame, around the motion search
+    # the motion search's two events on every PROFILE_EVERY-th frame (the sample of its launch
+    # duration): each event holds the stream ~4.6 us, a few percent of a frame
+    enc.set_profiling(1)
+    enc.set_profiling_sample(PROFILE_EVERY)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(1 + warmup, 1 + warmup + steps):

@@ -271,12 +271,15 @@ int riv_gpu_logf_exp2f(int32_t device_id, const float* in, size_t n, int32_t whi
  * formed inside stage 4, so not recorded), 3 adaptive quantiser, 4 residual prediction/transform/
  * quantise/reconstruct, 5 deblock, 6 intra transform, 7 the whole motion-search launch of batch
  * steps this encoder leads (all streams; the stage-1 figures of encoders in a batch are not the
- * search alone), 8 the GPU context modelling of a P-frame's residual symbols and their copy to
- * host memory. `on`: 0 off, 1 the motion-search
+ * search alone), 8 the GPU context modelling of a P-frame's residual symbols (their copy to host
+ * memory is the host stage's). `on`: 0 off, 1 the motion-search
  * stage only (two events per frame), 2 every stage and each frame's gpu_ms. Every event costs the
  * stream a few microseconds. */
 #define RIV_NUM_STAGES 9
 int riv_encoder_set_profiling(riv_encoder* enc, int on);
+/* Profile only the frames whose index is a multiple of `every` (1, the default: every frame), a
+ * sample of the stage times that costs the stream fewer events (no reference counterpart). */
+int riv_encoder_set_profiling_sample(riv_encoder* enc, uint32_t every);
 /* Accumulated milliseconds and launch counts per stage since profiling was switched on. */
 int riv_encoder_stage_times(riv_encoder* enc, double* ms, uint32_t* counts);
 

@@ -140,6 +140,7 @@ PROTOTYPES = [
     ("riv_gpu_me_tables", c_int, [c_int32, c_void_p, c_void_p, c_uint32, c_uint32, c_int32, c_void_p, c_void_p]),
     ("riv_gpu_logf_exp2f", c_int, [c_int32, c_void_p, c_size_t, c_int32, c_void_p]),
     ("riv_encoder_set_profiling", c_int, [c_void_p, c_int]),
+    ("riv_encoder_set_profiling_sample", c_int, [c_void_p, c_uint32]),
     ("riv_encoder_stage_times", c_int, [c_void_p, c_void_p, c_void_p]),
     ("riv_encoder_set_batch_autotune", c_int, [c_void_p, c_int32]),
     ("riv_encoder_batch_tuning", c_int, [c_void_p, c_void_p]),

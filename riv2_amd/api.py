@@ -323,6 +323,10 @@ class Encoder:
         level = 2 if on is True else 0 if on is False else int(on)
         _check(L.lib().riv_encoder_set_profiling(self._h, level))
 
+    def set_profiling_sample(self, every: int = 1):
+        """Profile only the frames whose index is a multiple of `every`."""
+        _check(L.lib().riv_encoder_set_profiling_sample(self._h, int(every)))
+
     def stage_times(self):
         ms = np.zeros(len(self.STAGES), np.float64)   # RIV_NUM_STAGES entries
         n = np.zeros(len(self.STAGES), np.uint32)

@@ -34,10 +34,13 @@ struct Slot {
     HostBuf h_rec, h_pack, h_optskip;    // mapped host memory the kernels write for the host entropy stage
     HostBuf h_main;                      // inter frames: each coded MB's masks (and few values), PACK_MAIN bytes per MB
     HostBuf h_passes;                    // mapped: the motion search's pass count
-    HostBuf h_sym, h_rmh;                // mapped: a P-frame's residual symbols modelled on the GPU, and their count and status
+    HostBuf h_sym, h_rmh;                // pinned: a P-frame's residual symbols modelled on the GPU; mapped: their count and status
+    DevBuf d_sym;                        // the slot's symbols as the modelling writes them (copied to h_sym by DMA)
+    hipEvent_t sym_ev = nullptr;         // that copy's completion
     bool gpu_model = false;              // this P-frame's residual symbols were modelled on the GPU
     int rm_q = 0;                        // then: its record buffer on the device
     hipEvent_t done = nullptr;           // the frame's kernels (and so its host-side outputs) finished
+    int prof = 0;                        // this frame's profiling level
     hipEvent_t marks[RIV_NUM_STAGES + 1] = {};
     int mark_stage[RIV_NUM_STAGES + 1] = {};
     int n_marks = 0;
@@ -251,7 +254,7 @@ struct riv_encoder {
     // GPU residual modelling of P-frames: the residual kernel's device records and the modelling's
     // scratch (sized for nb MBs), and the symbols a frame may have
     DevBuf rm_pack[2], rm_coded[2];   // the residual kernel's records, by frame parity
-    DevBuf rm_cnt, rm_last, rm_ke, rm_bcnt, rm_wst, rm_sigw, rm_eobw, rm_bad, rm_off, rm_boff, rm_tt, rm_tp, rm_ev, rm_sym, rm_hdr;
+    DevBuf rm_cnt, rm_last, rm_ke, rm_bcnt, rm_wst, rm_sigw, rm_eobw, rm_bad, rm_off, rm_boff, rm_tt, rm_tp, rm_ev, rm_hdr;
     rivk::ResModelBufs rm{};
     // The modelling runs on a stream of its own, so that it overlaps the next frame's motion
     // search: frame k's residual kernel writes record buffer k % 2 and records rm_ready; its
@@ -259,6 +262,7 @@ struct riv_encoder {
     // kernel waits for. Made at the first GPU-modelled frame, with every slot's mapped symbol buffer.
     bool rm_host = false;
     hipStream_t ms = nullptr;
+    hipStream_t dl = nullptr;      // device-to-host copies of modelled symbols (the copy engine, no CUs)
     hipEvent_t rm_ready[2] = {}, rm_free[2] = {};
     bool rm_used[2] = {};
     int rm_q = 0;
@@ -329,6 +333,7 @@ struct riv_encoder {
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     double now_ms() const { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
     int profiling = 0;             // 0 off, 1 motion-search stage only, 2 every stage
+    uint64_t prof_every = 1;       // frames whose index is a multiple of it are profiled
     double stage_ms[RIV_NUM_STAGES] = {};
     uint32_t stage_n[RIV_NUM_STAGES] = {};
 
@@ -379,7 +384,8 @@ static hipEvent_t batch_prof_event(riv_encoder* e) {
 }
 
 // Launches the motion search on st once the device's previous search (from any stream) is done.
-static int launch_search(int device, hipStream_t st, const riv::SearchParams& p, int groups, int waves) {
+static int launch_search(int device, hipStream_t st, const riv::SearchParams& p, int groups, int waves, hipEvent_t ev0 = nullptr,
+                         hipEvent_t ev1 = nullptr) {
     SearchGate& g = search_gate(device);
     std::lock_guard<std::mutex> lk(g.mu);
     if (!g.ev) RIV_HIP(hipEventCreateWithFlags(&g.ev, hipEventDisableTiming | hipEventDisableSystemFence));
@@ -387,7 +393,7 @@ static int launch_search(int device, hipStream_t st, const riv::SearchParams& p,
         RIV_HIP(hipEventRecord(g.ev, g.last));
         RIV_HIP(hipStreamWaitEvent(st, g.ev, 0));
     }
-    RIV_HIP(rivk::me_search(st, p, groups, waves));
+    RIV_HIP(rivk::me_search(st, p, groups, waves, ev0, ev1));
     g.last = st;
     return RIV_OK;
 }
@@ -485,13 +491,13 @@ static int encoder_init(riv_encoder* e) {
         RIV_HIP(e->rm_off.alloc(4 * nb)); RIV_HIP(e->rm_boff.alloc(7 * 4 * nb)); RIV_HIP(e->rm_ev.alloc(384 * 4 * nb));
         const size_t ntiles = (nb + 1023) / 1024;
         RIV_HIP(e->rm_tt.alloc(9 * 4 * ntiles)); RIV_HIP(e->rm_tp.alloc(8 * 4 * ntiles));
-        RIV_HIP(e->rm_sym.alloc(2 * ((size_t)e->rm_cap_max + 8))); RIV_HIP(e->rm_hdr.alloc(64));
+        RIV_HIP(e->rm_hdr.alloc(64));
         e->rm.cnt = e->rm_cnt.as<uint16_t>(); e->rm.last = e->rm_last.as<uint8_t>(); e->rm.ke = e->rm_ke.as<uint8_t>();
         e->rm.bcnt = e->rm_bcnt.as<uint16_t>(); e->rm.wst = e->rm_wst.as<uint16_t>(); e->rm.bad = e->rm_bad.as<uint8_t>();
         e->rm.sigw = e->rm_sigw.as<uint32_t>(); e->rm.eobw = e->rm_eobw.as<uint32_t>();
         e->rm.off = e->rm_off.as<uint32_t>(); e->rm.boff = e->rm_boff.as<uint32_t>(); e->rm.ev = e->rm_ev.as<uint32_t>();
         e->rm.tt = e->rm_tt.as<uint32_t>(); e->rm.tp = e->rm_tp.as<uint32_t>();
-        e->rm.sym = e->rm_sym.as<uint16_t>(); e->rm.hdr = e->rm_hdr.as<uint32_t>();
+        e->rm.hdr = e->rm_hdr.as<uint32_t>();   // the symbol buffer is the frame's slot's
     }
     e->depth = c.pipeline_depth;
     e->slots.resize(std::max(2, e->depth));
@@ -578,10 +584,15 @@ static void destroy_encoder(riv_encoder* e) {
         std::lock_guard<std::mutex> lk(g.mu);
         if (g.last == e->st) g.last = nullptr;   // its searches are done; the stream goes away
     }
+    if (e->dl) {
+        (void)hipStreamSynchronize(e->dl);
+        (void)hipStreamDestroy(e->dl);
+    }
     for (Slot& s : e->slots) {
         if (s.done) (void)hipEventDestroy(s.done);
         if (s.t0) (void)hipEventDestroy(s.t0);
         if (s.t1) (void)hipEventDestroy(s.t1);
+        if (s.sym_ev) (void)hipEventDestroy(s.sym_ev);
         for (auto& m : s.marks) if (m) (void)hipEventDestroy(m);
     }
     if (e->sync_ev) (void)hipEventDestroy(e->sync_ev);
@@ -674,6 +685,18 @@ static void collect_gpu_times(Slot& s) {
 // writes run in frame order.
 static void frame_part_done(riv_encoder* e, int si, bool mv_part);
 
+// A GPU-modelled frame's n symbols from the slot's device buffer to its pinned host buffer, by the
+// copy engine (a kernel storing to host memory would hold CUs for the whole PCIe transfer, and keep
+// the next motion search from starting on them).
+static void fetch_symbols(riv_encoder* e, Slot& s, uint32_t n) {
+    if (n == 0) return;
+    hipError_t r = hipSetDevice(e->cfg.device_id);
+    if (r == hipSuccess) r = hipMemcpyAsync(s.h_sym.p, s.d_sym.p, (size_t)n * 2, hipMemcpyDeviceToHost, e->dl);
+    if (r == hipSuccess) r = hipEventRecord(s.sym_ev, e->dl);
+    if (r == hipSuccess) r = hipEventSynchronize(s.sym_ev);
+    if (r != hipSuccess) throw std::runtime_error(std::string("copying the modelled symbols: ") + hipGetErrorString(r));
+}
+
 static void code_residuals(riv_encoder* e, int si) {
     Slot& s = e->slots[si];
     const int nb = e->nb;
@@ -691,6 +714,7 @@ static void code_residuals(riv_encoder* e, int si) {
             // the GPU modelled the symbols: only their rANS32 coding is left, in groups of sixteen
             // 16,386-symbol blocks side by side, across idle workers when there are any
             const uint32_t n = s.h_rmh.as<uint32_t>()[0];
+            fetch_symbols(e, s, n);
             st.residual_symbols_on_gpu = 1;
             const bool par = e->par && e->par->pool.idle() >= 2;
             s.res = rivh::code_symbol_stream(s.h_sym.as<uint16_t>(), n, par ? rivh::ParallelFor([e](int k, const std::function<void(int)>& f) {
@@ -979,10 +1003,23 @@ struct FrameCtx {
 };
 
 static void mark(riv_encoder* e, Slot& s, hipStream_t st, int stage) {
-    if (!e->profiling || s.n_marks > RIV_NUM_STAGES) return;
-    if (e->profiling == 1 && stage != 0 && stage != 1) return;   // two events per frame: around the search
+    if (!s.prof || s.n_marks > RIV_NUM_STAGES) return;
+    if (s.prof == 1 && stage != 0 && stage != 1) return;   // two events per frame: around the search
     (void)hipEventRecord(s.marks[s.n_marks], st);
     s.mark_stage[s.n_marks++] = stage;
+}
+// The frame's next profiling event for `stage`, taken but not recorded: a kernel launch records it
+// at the kernel's start or end, which costs the stream nothing (a recorded event holds it for
+// microseconds). Null when profiling is off.
+static hipEvent_t mark_by_launch(riv_encoder* e, Slot& s, int stage) {
+    if (!s.prof || s.n_marks > RIV_NUM_STAGES) return nullptr;
+    s.mark_stage[s.n_marks] = stage;
+    return s.marks[s.n_marks++];
+}
+// Single-frame plain P-frames under motion-search profiling: the search launch records the two
+// events around it.
+static bool search_marks_by_launch(const riv_encoder* e, const FrameCtx& f) {
+    return e->slots[f.si].prof == 1 && !f.intra && !f.batch && !e->net;
 }
 
 static int check_frame(riv_encoder* e, size_t len, uint32_t w, uint32_t h) {
@@ -1022,7 +1059,9 @@ static int frame_begin(riv_encoder* e, const void* rgb, bool on_device, size_t l
     s.h_passes.as<int>()[1] = 0;   // the search's error word: the kernel writes it only on failure
     s.t_submit = e->now_ms();
     s.n_marks = 0;
-    if (e->profiling && !s.marks[0])
+    // profiling events on every prof_every-th frame (a sample: each event holds the stream)
+    s.prof = e->frame_count % e->prof_every == 0 ? e->profiling : 0;
+    if (s.prof && !s.marks[0])
         for (auto& m : s.marks) RIV_HIP(hipEventCreateWithFlags(&m, hipEventDisableSystemFence));
 
     f.si = si; f.intra = intra; f.st = st;
@@ -1033,7 +1072,7 @@ static int frame_begin(riv_encoder* e, const void* rgb, bool on_device, size_t l
     f.pY = e->pred.as<uint8_t>(); f.pU = f.pY + ysz; f.pV = f.pU + csz;
     f.prev_mv = e->prev_inter_slot >= 0 ? (const int8_t*)e->slots[e->prev_inter_slot].rec.p + 4 : nullptr;
 
-    s.timed = e->profiling >= 2;
+    s.timed = s.prof >= 2;
     if (s.timed) RIV_HIP(hipEventRecord(s.t0, st));
     const uint8_t* d_rgb = (const uint8_t*)rgb;
     int ub = -1;   // the upload buffer this frame reads, if any
@@ -1067,7 +1106,7 @@ static int frame_begin(riv_encoder* e, const void* rgb, bool on_device, size_t l
         RIV_HIP(hipEventRecord(e->up_free[ub], st));
         e->up_used[ub] = true;
     }
-    mark(e, s, st, 0);
+    if (!search_marks_by_launch(e, f)) mark(e, s, st, 0);
     if (intra) {
         rivk::intra(st, f.cY, f.cU, f.cV, e->W, e->H, e->yt.as<uint16_t>(), e->ct.as<uint16_t>(), f.nY, f.nU, f.nV, s.h_pack.dev<uint8_t>());
         mark(e, s, st, 6);
@@ -1108,7 +1147,7 @@ static int frame_after_search(riv_encoder* e, const FrameCtx& f) {
     Slot& s = e->slots[f.si];
     hipStream_t st = f.st;
     BlockRec* rec = s.rec.as<BlockRec>();   // written by the search's mode-decision epilogue
-    mark(e, s, st, 1);
+    if (!search_marks_by_launch(e, f)) mark(e, s, st, 1);
     // the prediction's variances came from the search (lnv); the residual kernel forms the prediction
     rivk::aq(st, e->nb, e->inter_qs, e->lnv.as<float>(), e->steps.as<float>());
     mark(e, s, st, 3);
@@ -1118,8 +1157,13 @@ static int frame_after_search(riv_encoder* e, const FrameCtx& f) {
     int q = -1;   // the record buffer
     if (s.gpu_model) {
         if (!e->rm_host) {
-            for (Slot& t : e->slots) { RIV_HIP(t.h_sym.alloc(2 * ((size_t)e->rm_cap_max + 8))); RIV_HIP(t.h_rmh.alloc(64)); }
+            for (Slot& t : e->slots) {
+                RIV_HIP(t.h_sym.alloc(2 * ((size_t)e->rm_cap_max + 8))); RIV_HIP(t.h_rmh.alloc(64));
+                RIV_HIP(t.d_sym.alloc(2 * ((size_t)e->rm_cap_max + 8)));
+                RIV_HIP(hipEventCreateWithFlags(&t.sym_ev, hipEventDisableTiming | hipEventBlockingSync));
+            }
             RIV_HIP(hipStreamCreateWithFlags(&e->ms, hipStreamNonBlocking));
+            RIV_HIP(hipStreamCreateWithFlags(&e->dl, hipStreamNonBlocking));
             for (int k = 0; k < 2; k++) {
                 RIV_HIP(hipEventCreateWithFlags(&e->rm_ready[k], hipEventDisableTiming | hipEventDisableSystemFence));
                 RIV_HIP(hipEventCreateWithFlags(&e->rm_free[k], hipEventDisableTiming | hipEventDisableSystemFence));
@@ -1128,7 +1172,9 @@ static int frame_after_search(riv_encoder* e, const FrameCtx& f) {
         }
         q = e->rm_q;
         e->rm_q ^= 1;
-        if (e->rm_used[q]) RIV_HIP(hipStreamWaitEvent(st, e->rm_free[q], 0));   // frame k - 2's modelling has read it
+        // frame k - 2's modelling has read it (a wait packet holds the stream for microseconds even
+        // when its event has completed, so it is queued only while the event is pending)
+        if (e->rm_used[q] && hipEventQuery(e->rm_free[q]) != hipSuccess) RIV_HIP(hipStreamWaitEvent(st, e->rm_free[q], 0));
     }
     // GPU-modelled frames keep their coefficient records on the device (the host gets them only if
     // it has to code the frame itself); host-coded frames get them in mapped host memory
@@ -1146,7 +1192,8 @@ static int frame_after_search(riv_encoder* e, const FrameCtx& f) {
         rivk::ResModelBufs B = e->rm;
         B.pack = e->rm_pack[q].as<uint8_t>();
         B.coded = e->rm_coded[q].as<uint8_t>();
-        RIV_HIP(rivk::res_model(e->ms, B, s.h_sym.dev<uint16_t>(), s.h_rmh.dev<uint32_t>(), s.h_pack.dev<uint8_t>()));
+        B.sym = s.d_sym.as<uint16_t>();   // the slot's own: it is read by a copy the host stage starts
+        RIV_HIP(rivk::res_model(e->ms, B, s.h_rmh.dev<uint32_t>(), s.h_pack.dev<uint8_t>()));
         RIV_HIP(hipEventRecord(e->rm_free[q], e->ms));
         e->rm_used[q] = true;
         mark(e, s, e->ms, 8);
@@ -1208,7 +1255,13 @@ static int encode_impl(riv_encoder* e, const void* rgb, bool on_device, size_t l
             p.plimit = std::min(p.plimit, e->learned_passes);
             p.truncate = 1;
         }
-        int lr = launch_search(e->cfg.device_id, f.st, p, e->shape.groups, e->shape.waves);
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
+        if (search_marks_by_launch(e, f)) {
+            Slot& s = e->slots[f.si];
+            ev0 = mark_by_launch(e, s, 0);
+            ev1 = mark_by_launch(e, s, 1);
+        }
+        int lr = launch_search(e->cfg.device_id, f.st, p, e->shape.groups, e->shape.waves, ev0, ev1);
         if (lr) return lr;
         r = frame_after_search(e, f);
         if (r) return r;
@@ -1538,6 +1591,12 @@ int riv_encoder_set_profiling(riv_encoder* e, int on) {
     std::lock_guard<std::mutex> lk(e->mu);
     e->profiling = on < 0 ? 0 : on > 2 ? 2 : on;
     for (int i = 0; i < RIV_NUM_STAGES; i++) { e->stage_ms[i] = 0; e->stage_n[i] = 0; }
+    return RIV_OK;
+}
+int riv_encoder_set_profiling_sample(riv_encoder* e, uint32_t every) {
+    if (!e || every == 0) return RIV_E_ARG;
+    std::lock_guard<std::mutex> lk(e->mu);
+    e->prof_every = every;
     return RIV_OK;
 }
 int riv_encoder_stage_times(riv_encoder* e, double* ms, uint32_t* counts) {

@@ -2,6 +2,7 @@
 // mode decision, motion compensation, adaptive quantisation, residual transforms, deblock.
 // Each kernel states the reference function it reproduces bit-exactly (paths relative
 // to the reference tree, under reitero_video/).
+#include <hip/hip_ext.h>
 #include "riv_dev.hpp"
 #include "riv_kernels.hpp"
 
@@ -1718,8 +1719,12 @@ std::vector<SearchShape> search_shapes(int sr, int nb, int nstreams, int cus, si
     }
     return out;
 }
-hipError_t me_search(hipStream_t st, const SearchParams& p, int groups, int waves) {
-    hipLaunchKernelGGL(k_me_search, dim3(groups), dim3(64 * waves), search_lds_bytes(p.sr, waves, p.nstreams, p.nslot), st, p);
+hipError_t me_search(hipStream_t st, const SearchParams& p, int groups, int waves, hipEvent_t ev0, hipEvent_t ev1) {
+    const size_t lds = search_lds_bytes(p.sr, waves, p.nstreams, p.nslot);
+    if (ev0 || ev1)   // the launch itself records the events at the kernel's start and end (no packets of their own)
+        hipExtLaunchKernelGGL(k_me_search, dim3(groups), dim3(64 * waves), lds, st, ev0, ev1, 0, p);
+    else
+        hipLaunchKernelGGL(k_me_search, dim3(groups), dim3(64 * waves), lds, st, p);
     return hipGetLastError();
 }
 void mc(hipStream_t st, const uint8_t* rY, const uint8_t* rU, const uint8_t* rV, int W, int H, const int8_t* mvs, int mv_stride,

@@ -559,17 +559,17 @@ __global__ __launch_bounds__(64 * kChainWaves) void k_rm_chains(int nb, const ui
     }
 }
 
-// The frame's symbols and its header words to mapped host memory, in 16-byte stores (both symbol
-// buffers hold 8 symbols past their capacity); when the status is set, the MBs' records instead
-// (nb * PACK_STRIDE bytes), which the host then codes itself.
-__global__ __launch_bounds__(256) void k_rm_push(const uint16_t* __restrict__ sym, const uint32_t* __restrict__ hdr,
-                                                 const uint8_t* __restrict__ pack, int nb, uint16_t* __restrict__ hsym,
+// The frame's header words (symbol count and status) to mapped host memory; when the status is
+// set, the MBs' records too (nb * PACK_STRIDE bytes, in 16-byte stores), which the host then codes
+// itself. The symbols stay on the device: the host stage copies them with the copy engine.
+__global__ __launch_bounds__(256) void k_rm_push(const uint32_t* __restrict__ hdr, const uint8_t* __restrict__ pack, int nb,
                                                  uint32_t* __restrict__ hhdr, uint8_t* __restrict__ hpack) {
     const uint32_t n = hdr[0], status = hdr[1];
     if (blockIdx.x == 0 && threadIdx.x == 0) { hhdr[0] = n; hhdr[1] = status; }
-    const size_t n16 = status ? (size_t)nb * PACK_STRIDE / 16 : ((size_t)n * 2 + 15) / 16;
-    const uint4* s = status ? (const uint4*)pack : (const uint4*)sym;
-    uint4* d = status ? (uint4*)hpack : (uint4*)hsym;
+    if (!status) return;
+    const size_t n16 = (size_t)nb * PACK_STRIDE / 16;
+    const uint4* s = (const uint4*)pack;
+    uint4* d = (uint4*)hpack;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x) d[i] = s[i];
 }
 }  // namespace
@@ -577,7 +577,7 @@ __global__ __launch_bounds__(256) void k_rm_push(const uint16_t* __restrict__ sy
 namespace rivk {
 constexpr int kChains = 256 + 64 + 8 + 6 + 7 * 12;
 
-hipError_t res_model(hipStream_t st, const ResModelBufs& B, uint16_t* host_sym, uint32_t* host_hdr, uint8_t* host_pack) {
+hipError_t res_model(hipStream_t st, const ResModelBufs& B, uint32_t* host_hdr, uint8_t* host_pack) {
     const int nb = B.nb;
     if (nb <= 0 || B.cap >= (1u << 28)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_rm_count, dim3((nb + 3) / 4), dim3(256), 0, st, B.pack, B.coded, nb, B.cnt, B.last, B.ke, B.bcnt, B.wst, B.bad);
@@ -587,7 +587,7 @@ hipError_t res_model(hipStream_t st, const ResModelBufs& B, uint16_t* host_sym, 
     hipLaunchKernelGGL(k_rm_walk, dim3((nb + 63) / 64), dim3(384), 0, st, B.pack, B.coded, nb, B.cnt, B.last, B.ke, B.wst, B.off, B.boff,
                        B.tp, B.hdr, B.sigw, B.eobw, B.ev, B.sym);
     hipLaunchKernelGGL(k_rm_chains, dim3(kChains), dim3(64 * kChainWaves), 0, st, nb, B.sigw, B.eobw, B.ev, B.hdr, B.sym);
-    hipLaunchKernelGGL(k_rm_push, dim3(256), dim3(256), 0, st, B.sym, B.hdr, B.pack, nb, host_sym, host_hdr, host_pack);
+    hipLaunchKernelGGL(k_rm_push, dim3(256), dim3(256), 0, st, B.hdr, B.pack, nb, host_hdr, host_pack);
     return hipGetLastError();
 }
 }  // namespace rivk

@@ -26,7 +26,8 @@ bool search_shape(int sr, int nb, int nstreams, int cus, size_t budget, SearchSh
 // The launch shapes an autotuner may try for the same search (search_shape's first); every one
 // reaches the same fixed point, so the choice changes only the time.
 std::vector<SearchShape> search_shapes(int sr, int nb, int nstreams, int cus, size_t budget);
-hipError_t me_search(hipStream_t st, const riv::SearchParams& p, int groups, int waves);
+// With ev0 / ev1, the launch records them when the kernel starts / ends (profiling).
+hipError_t me_search(hipStream_t st, const riv::SearchParams& p, int groups, int waves, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void mc(hipStream_t st, const uint8_t* rY, const uint8_t* rU, const uint8_t* rV, int W, int H, const int8_t* mvs, int mv_stride,
         uint8_t* pY, uint8_t* pU, uint8_t* pV, uint32_t* var_sums, float* lnv);
 // lnv: nb logs written by mc, plus one float for their mean; one launch computes the mean and the steps
@@ -75,11 +76,11 @@ struct ResModelBufs {
     uint32_t* tt = nullptr;         // 9 per tile: its totals (symbols, seven bands) and a bad flag
     uint32_t* tp = nullptr;         // 8 per tile: the totals of the tiles before it
     uint32_t* ev = nullptr;         // 384 * nb: band event lists
-    uint16_t* sym = nullptr;        // cap + 8
+    uint16_t* sym = nullptr;        // cap + 8: the frame's symbols (device memory the caller copies out)
     uint32_t* hdr = nullptr;        // 16 words
 };
-// host_hdr[0]: the frame's symbols, host_hdr[1]: status (0: host_sym holds them; 1: a magnitude
+// host_hdr[0]: the frame's symbols, host_hdr[1]: status (0: b.sym holds them; 1: a magnitude
 // beyond the class table; 2: more than cap symbols: host_pack then holds the MBs' records, nb *
-// PACK_STRIDE bytes); all in mapped host memory.
-hipError_t res_model(hipStream_t st, const ResModelBufs& b, uint16_t* host_sym, uint32_t* host_hdr, uint8_t* host_pack);
+// PACK_STRIDE bytes); both in mapped host memory.
+hipError_t res_model(hipStream_t st, const ResModelBufs& b, uint32_t* host_hdr, uint8_t* host_pack);
 }  // namespace rivk
