@@ -102,13 +102,12 @@ __device__ __forceinline__ uint32_t lds_read4(const uint32_t* base, int off) {
     return __builtin_amdgcn_alignbyte(hi, lo, (uint32_t)(off & 3));
 }
 // per-byte floor((a + b) / 2)
-__device__ __forceinline__ uint32_t avg2f(uint32_t a, uint32_t b) { return (a & b) + (((a ^ b) >> 1) & 0x7F7F7F7Fu); }
-// per-byte floor((a + b + c + d) / 4)
+__device__ __forceinline__ uint32_t avg2f(uint32_t a, uint32_t b) { return __builtin_amdgcn_lerp(a, b, 0u); }
+// per-byte floor((a + b + c + d) / 4): with h1 = floor((a+b)/2), h2 = floor((c+d)/2), it is
+// floor((h1 + h2 + carry) / 2), carry = 1 where both a+b and c+d are odd (v_lerp_u8 adds bit 8i of
+// its third operand to byte i before halving)
 __device__ __forceinline__ uint32_t avg4f(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
-    const uint32_t m = 0x00FF00FFu;
-    uint32_t e = (a & m) + (b & m) + (c & m) + (d & m);
-    uint32_t o = ((a >> 8) & m) + ((b >> 8) & m) + ((c >> 8) & m) + ((d >> 8) & m);
-    return ((e >> 2) & m) | (((o >> 2) & m) << 8);
+    return __builtin_amdgcn_lerp(__builtin_amdgcn_lerp(a, b, 0u), __builtin_amdgcn_lerp(c, d, 0u), (a ^ b) & (c ^ d) & 0x01010101u);
 }
 __device__ __forceinline__ uint32_t sad4(uint32_t a, uint32_t b, uint32_t acc) { return __builtin_amdgcn_sad_u8(a, b, acc); }
 
