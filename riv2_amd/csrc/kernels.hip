@@ -540,7 +540,7 @@ __device__ __forceinline__ void jw_load(const SearchStream& S, int sr, uint32_t*
 template <bool INTERIOR>
 __device__ uint32_t jsearch(const SearchParams& p, const SearchStream& S, const int* ctab, uint32_t* wp, bool load, int ND, int NR, int bx,
                             int by, uint32_t lS, uint32_t tS, uint32_t trS, uint32_t tlS, uint32_t pT, uint4 pr, int lane, int* sad_out,
-                            bool* took) {
+                            bool* took, uint32_t prev_int, uint32_t prev_word, int prev_sad, uint32_t* int_out) {
     const int sr = p.sr, x0 = bx * 16, y0 = by * 16;
     if (load) jw_load(S, sr, wp, ND, NR, x0, y0, lane);
     JWin w;
@@ -648,6 +648,14 @@ __device__ uint32_t jsearch(const SearchParams& p, const SearchStream& S, const 
         const int key = v ? ((sv + cc.cost(dx, dy)) << 3) | lane : 0x7fffffff;
         const int m = lane_min8(key);
         if ((m >> 3) < scost) { sdx = __builtin_amdgcn_readlane(dx, m & 7); sdy = __builtin_amdgcn_readlane(dy, m & 7); }
+    }
+    // The half-pel refine and the canonical vector depend only on the integer best and the window:
+    // a re-evaluation that lands on the integer vector of the block's previous one (prev_int, with
+    // bit 16 set once there is one) has their result already.
+    {
+        const uint32_t ib = ((uint32_t)(uint8_t)sdx | ((uint32_t)(uint8_t)sdy << 8)) | 0x10000u;
+        *int_out = ib;
+        if (ib == prev_int) { *sad_out = prev_sad; return prev_word; }
     }
     // refine_halfpel (:759-839): the base re-scored with the half-pel kernel, then the 8
     // neighbours in (sy outer, sx inner) order at penalty c_pred (every neighbour of an even base
@@ -772,6 +780,7 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
     // learned form: the block's proposals, read once (an evaluation takes them by lane read)
     const uint4 myprop = (valid && p.prop) ? p.prop[myb] : make_uint4(0u, 0u, 0u, 0u);
     int mysad = 0;
+    uint32_t myint = 0u;   // the integer vector the block's last evaluation found (bit 16: there was one)
     bool mytook = false;   // the block's last evaluation took the network's proposal
     int outcome = 0;     // 1 converged, -1 barrier timeout, 0 pass limit
     int pass = 0;
@@ -815,13 +824,18 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
                                         sgpr((uint32_t)__builtin_amdgcn_readlane((int)myprop.y, k)),
                                         sgpr((uint32_t)__builtin_amdgcn_readlane((int)myprop.z, k)), 0u);
             bool took;
-            const uint32_t wS = mb_interior(bx * 16, by * 16, S.W, S.H, sr)
-                                    ? jsearch<true>(p, S, ctab, wp, load, ND, NR, bx, by, lS, tS, trS, tlS, pT, pr, lane, &sad, &took)
-                                    : jsearch<false>(p, S, ctab, wp, load, ND, NR, bx, by, lS, tS, trS, tlS, pT, pr, lane, &sad, &took);
             const uint32_t old = sgpr((uint32_t)__builtin_amdgcn_readlane((int)myw, k));
+            const uint32_t pint = sgpr((uint32_t)__builtin_amdgcn_readlane((int)myint, k));
+            const int psad = __builtin_amdgcn_readlane(mysad, k);
+            uint32_t nint;
+            const uint32_t wS = mb_interior(bx * 16, by * 16, S.W, S.H, sr)
+                                    ? jsearch<true>(p, S, ctab, wp, load, ND, NR, bx, by, lS, tS, trS, tlS, pT, pr, lane, &sad, &took,
+                                                    pint, old, psad, &nint)
+                                    : jsearch<false>(p, S, ctab, wp, load, ND, NR, bx, by, lS, tS, trS, tlS, pT, pr, lane, &sad, &took,
+                                                     pint, old, psad, &nint);
             const bool c = ((wS ^ old) & 0xffffu) != 0u;
             chg |= c;
-            if (lane == k) { myw = wS; mysad = sad; mytook = took; }
+            if (lane == k) { myw = wS; mysad = sad; mytook = took; myint = nint; }
             if (lane == 0) field_store(dst + b, wS | (c ? 0x80000000u : 0u));
         }
         // ---- pass barrier: a two-level tree of arrival counters (arrivals in bits 0-15, arrivals
