@@ -738,7 +738,7 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
     extern __shared__ __align__(16) uint32_t lds32[];
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int NW = gridDim.x * (blockDim.x >> 6), gw = blockIdx.x * (blockDim.x >> 6) + wv;
+    const int gw = blockIdx.x * (blockDim.x >> 6) + wv;
     const int nb = p.nb, sr = p.sr, NS = p.nstreams;
     const int NR = 2 * sr + 18, ND = riv::search_pitch(sr), SLOT = riv::search_slot_dwords(sr);
     const int TD = 2 * sr + 4;
@@ -762,9 +762,23 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
     if (threadIdx.x == 0) wflag[0] = 0;
     __syncthreads();
 
-    // lane k owns block gw + k*NW (the host keeps nb <= 64*NW), of stream ms: the last with blk0 <= it
-    const int myb = gw + lane * NW;
-    const bool valid = myb < nb;
+    // Each workgroup owns a contiguous range of blocks, nb / groups of them rounded either way, so
+    // the CUs' shares differ by at most one block. The ranges go to the workgroups in XCD-band
+    // order: workgroup i runs on XCD i % 8, and XCD x's workgroups take the x-th eighth of the
+    // blocks, so blocks next to each other, and their overlapping reference windows, share an L2.
+    // Lane k of wave w owns block w + k * waves of the range (the host keeps a range within 64
+    // blocks per wave), so a pass's few changed blocks sit on different waves. The block is of
+    // stream ms: the last with blk0 <= it.
+    int myb;
+    bool valid;
+    {
+        const int G = (int)gridDim.x, x = (int)(blockIdx.x & 7);
+        int ord = (int)(blockIdx.x >> 3);
+        for (int y = 0; y < x; y++) ord += (G - y + 7) >> 3;   // workgroups on XCDs before x
+        const int b0 = (int)(((long long)ord * nb) / G), b1 = (int)(((long long)(ord + 1) * nb) / G);
+        myb = b0 + wv + lane * (int)(blockDim.x >> 6);
+        valid = myb < b1;
+    }
     int ms = 0;
     for (int lo = 0, hi = NS - 1; lo < hi;) {
         const int mid = (lo + hi + 1) >> 1;
