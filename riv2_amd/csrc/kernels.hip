@@ -798,26 +798,34 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
     bool mytook = false;   // the block's last evaluation took the network's proposal
     int outcome = 0;     // 1 converged, -1 barrier timeout, 0 pass limit
     int pass = 0;
+    // The field is double-buffered: pass p reads bank p & 1 (the words as pass p - 1 left them)
+    // and writes bank (p + 1) & 1. A block writes a bank only when the bank does not hold its
+    // current word (bit b of `fresh`: bank b does), so after two passes without a change it writes
+    // nothing. A block is searched again when the integer part of a neighbour word differs from
+    // the one its last search used (uL .. uTL).
+    uint32_t uL = 0u, uT = 0u, uTR = 0u, uTL = 0u, fresh = 0u;
     for (; pass < p.plimit; pass++) {
         const uint32_t* src = p.field + (size_t)(pass & 1) * nb;
         uint32_t* dst = p.field + (size_t)((pass + 1) & 1) * nb;
+        const uint32_t dbit = 1u << ((pass + 1) & 1);
         uint32_t nL = 0u, nT = 0u, nTR = 0u, nTL = 0u;
         bool dirty = valid;
         if (valid) {
-            if (mbx > 0) nL = field_load(src + myb - 1);
+            if (mbx > 0) nL = field_load(src + myb - 1) & 0xffffu;
             if (mby > 0) {
-                nT = field_load(src + myb - mbw);
-                if (mbx + 1 < mbw) nTR = field_load(src + myb - mbw + 1);
-                if (mbx > 0) nTL = field_load(src + myb - mbw - 1);
+                nT = field_load(src + myb - mbw) & 0xffffu;
+                if (mbx + 1 < mbw) nTR = field_load(src + myb - mbw + 1) & 0xffffu;
+                if (mbx > 0) nTL = field_load(src + myb - mbw - 1) & 0xffffu;
             }
             if (pass == 0) {   // every block searches once, from its neighbours' guesses
                 nL = seed_word(nL, sr); nT = seed_word(nT, sr); nTR = seed_word(nTR, sr); nTL = seed_word(nTL, sr);
             } else {
-                dirty = ((nL | nT | nTR | nTL) >> 31) != 0u;
+                dirty = ((nL ^ uL) | (nT ^ uT) | (nTR ^ uTR) | (nTL ^ uTL)) != 0u;
             }
+            uL = nL; uT = nT; uTR = nTR; uTL = nTL;
         }
-        // blocks whose inputs did not change keep their word (flag cleared) in this pass's field
-        if (valid && !dirty) field_store(dst + myb, myw);
+        // blocks whose inputs did not change keep their word, written where the bank lacks it
+        if (valid && !dirty && !(fresh & dbit)) { field_store(dst + myb, myw); fresh |= dbit; }
         bool chg = false;
         for (unsigned long long dm = __ballot(dirty); dm; dm &= dm - 1) {
             const int k = __builtin_ctzll(dm);
@@ -849,8 +857,9 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
                                                      pint, old, psad, &nint);
             const bool c = ((wS ^ old) & 0xffffu) != 0u;
             chg |= c;
-            if (lane == k) { myw = wS; mysad = sad; mytook = took; myint = nint; }
-            if (lane == 0) field_store(dst + b, wS | (c ? 0x80000000u : 0u));
+            const bool wr = wS != old || !(sgpr((uint32_t)__builtin_amdgcn_readlane((int)fresh, k)) & dbit);
+            if (lane == k) { myw = wS; mysad = sad; mytook = took; myint = nint; fresh = wS != old ? dbit : fresh | dbit; }
+            if (lane == 0 && wr) field_store(dst + b, wS);
         }
         // ---- pass barrier: a two-level tree of arrival counters (arrivals in bits 0-15, arrivals
         //      that changed a block in bits 16-31); the last arrival at the root publishes the pass
@@ -895,22 +904,22 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
             if (sd[i].passes) sd[i].passes[1] = code;
         return;
     }
-    // the mode decision, on the final field (every block stored its word in the last pass, which
-    // completed before the last barrier)
+    // the mode decision, on the final field (the last pass's bank holds every block's word: the
+    // writes completed before the last barrier)
     if (valid) {
         const SearchStream& S = sd[ms];
         const uint32_t* fin = p.field + (size_t)((pass + 1) & 1) * nb;
         const bool has_l = mbx > 0, has_t = mby > 0, has_tr = has_t && mbx + 1 < mbw, has_tl = has_t && mbx > 0;
-        const uint32_t wL = has_l ? field_load(fin + myb - 1) & 0x7fffffffu : 0u;
-        const uint32_t wT = has_t ? field_load(fin + myb - mbw) & 0x7fffffffu : 0u;
-        const uint32_t wTR = has_tr ? field_load(fin + myb - mbw + 1) & 0x7fffffffu : 0u;
-        const uint32_t wTL = has_tl ? field_load(fin + myb - mbw - 1) & 0x7fffffffu : 0u;
+        const uint32_t wL = has_l ? field_load(fin + myb - 1) : 0u;
+        const uint32_t wT = has_t ? field_load(fin + myb - mbw) : 0u;
+        const uint32_t wTR = has_tr ? field_load(fin + myb - mbw + 1) : 0u;
+        const uint32_t wTL = has_tl ? field_load(fin + myb - mbw - 1) : 0u;
         const uint4 r = mode_record(myw, mysad, has_l, has_t, has_tr, has_tl, S.prev_mv != nullptr, wL, wT, wTR, wTL, mypT,
                                     S.rdo_bits, S.rdo_max, p.err, S.passes ? S.passes + 1 : nullptr);
         *(uint4*)&S.rec[mloc] = r;
         *(uint4*)&S.host_rec[mloc] = r;
-        // the next launch's start field; where bank 0 is the final field it already holds myw
-        field_store(p.field + myb, myw);
+        // the next launch's start field: bank 0, where it does not hold myw already
+        if (!(fresh & 1u)) field_store(p.field + myb, myw);
     }
     if (p.lstat) {   // this wave's own three counters (same-address atomics from every wave would serialise)
         const unsigned long long vb = __ballot(valid), cb = __ballot(valid && (myprop.z & 1u)), tb = __ballot(valid && mytook);

@@ -87,7 +87,7 @@ struct SearchParams {
     int nb;                      // blocks of all streams
     int sr;
     uint32_t* field;             // 2 * nb MV words: the field after even / odd passes; bytes
-                                 // (dx, dy, sx, sy), bit 31 set when the pass changed (dx, dy)
+                                 // (dx, dy, sx, sy)
     unsigned* arrive;            // 2 banks of barrier_words(maxp) pass-barrier counters
     int bank;                    // bank of this launch (zero at launch); the launch clears the other
     int maxp;                    // pass bound: 2*(bh-1) + bw + 2 passes of the largest stream always
