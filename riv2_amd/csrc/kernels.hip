@@ -362,16 +362,17 @@ __device__ __forceinline__ uint4 mode_record(uint32_t wS, int best_sad, bool has
 // x = F(x). Passes of F, each computed from the previous pass's field, reach it exactly from any
 // start field: a block on anti-diagonal t = bx + 2*by is final after pass t + 1, so 2*(bh-1) + bw
 // passes always suffice, and a pass that changes no integer MV proves that the fixed point has
-// been reached. A block is re-evaluated only when one of its neighbours changed in the previous
-// pass. On camera-like frames the field settles in 10-20 passes, against the 2*(bh-1) + bw
-// dependent steps of a wavefront.
+// been reached. A block is re-evaluated only when the integer part of a neighbour's word differs
+// from the one its last evaluation used. On camera-like frames the field settles in 10-20 passes,
+// against the 2*(bh-1) + bw dependent steps of a wavefront.
 //
 // One persistent launch, every workgroup resident at once (the host sizes the grid to one
-// workgroup per CU). Wave w owns blocks w + k * nwaves, lane k holding block k's current word,
-// so that a pass's few changed blocks land on different waves. Passes end at a grid barrier on a
-// per-pass arrival counter that also counts the workgroups whose blocks changed. The field words
-// and the counters are read and written with relaxed agent-scope atomics (coherent across the
-// chip without cache maintenance); each wave waits for its own stores before arriving.
+// workgroup per CU). Each workgroup owns a contiguous range of blocks (in XCD-band order); wave w
+// of it owns blocks w + k * waves of the range, lane k holding block k's current word, so that a
+// pass's few changed blocks land on different waves. Passes end at a grid barrier on a per-pass
+// arrival counter that also counts the workgroups whose blocks changed. The field words and the
+// counters are read and written with relaxed agent-scope atomics (coherent across the chip
+// without cache maintenance); each wave waits for its own stores before arriving.
 //
 // Each wave keeps its blocks' replicate-padded reference windows, and the blocks' own 16 rows, in
 // LDS slots for the whole launch (the reference frame does not change during the search), so a
