@@ -1149,7 +1149,7 @@ static int frame_after_search(riv_encoder* e, const FrameCtx& f) {
     BlockRec* rec = s.rec.as<BlockRec>();   // written by the search's mode-decision epilogue
     if (!search_marks_by_launch(e, f)) mark(e, s, st, 1);
     // the prediction's variances came from the search (lnv); the residual kernel forms the prediction
-    rivk::aq(st, e->nb, e->inter_qs, e->lnv.as<float>(), e->steps.as<float>());
+    rivk::aq(st, e->nb, e->inter_qs, e->lnv.as<float>(), nullptr);   // the mean; the residual kernel forms the steps
     mark(e, s, st, 3);
     // the residual symbols are modelled on the GPU for single-frame calls (the host codes them for
     // batch steps, and when the test seam asks it to)
@@ -1179,7 +1179,8 @@ static int frame_after_search(riv_encoder* e, const FrameCtx& f) {
     // GPU-modelled frames keep their coefficient records on the device (the host gets them only if
     // it has to code the frame itself); host-coded frames get them in mapped host memory
     s.rm_q = q;
-    rivk::inter_residual(st, f.cY, f.cU, f.cV, f.rY, f.rU, f.rV, f.pY, f.pU, f.pV, e->W, e->H, rec, e->steps.as<float>(), c.inter_dead_zone,
+    rivk::inter_residual(st, f.cY, f.cU, f.cV, f.rY, f.rU, f.rV, f.pY, f.pU, f.pV, e->W, e->H, rec, e->lnv.as<float>(), e->inter_qs,
+                         e->steps.as<float>(), c.inter_dead_zone,
                          f.nY, f.nU, f.nV,
                          q >= 0 ? nullptr : s.h_main.dev<uint8_t>(), q >= 0 ? nullptr : s.h_pack.dev<uint8_t>(), e->mask.as<uint8_t>(),
                          s.h_optskip.dev<uint8_t>(), q >= 0 ? e->rm_pack[q].as<uint8_t>() : nullptr, q >= 0 ? e->rm_coded[q].as<uint8_t>() : nullptr);

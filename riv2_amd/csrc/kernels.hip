@@ -1066,6 +1066,12 @@ __device__ __forceinline__ int aq_pick(uint32_t b, int4 k, int2 eb) {
     return e == eb.x ? (odd ? k.y : k.x) : (e == eb.y ? (odd ? k.w : k.z) : kAqNoK);
 }
 
+// An MB's quantiser step: base * exp2(0.8 * (ln v - mean) / 6) in f32 (residual.rs:150-166).
+__device__ __forceinline__ float aq_step(float lnv, float mean, float base) {
+    const float off = __fmul_rn(0.8f, __fsub_rn(lnv, mean));
+    return __fmul_rn(base, glibc_exp2f(__fdiv_rn(off, 6.0f)));
+}
+
 size_t aq_lds_bytes() {
     const int nsub = kAqStage / kAqSub, nch = nsub / kAqGroup;
     return (size_t)kAqStage * 4 + (size_t)nsub * (8 + 16) + (size_t)nch * (8 + 16 + 8) + 16;
@@ -1101,40 +1107,48 @@ __global__ __launch_bounds__(1024) void k_aq(float* __restrict__ lnv, int nb, fl
             for (int u = 0; u < 8; u++) if (j0 + u * nt < nb) xs[aq_pos(j0 + u * nt)] = v[u];
         }
         __syncthreads();
-        for (int c = t; c < nsub; c += nt) {   // sub-chunk sums (f64: only an estimate, any order will do)
-            const int lo = c * kAqSub, hi = min(lo + kAqSub, nb);
+        // sub-chunk t (nsub <= 1024 threads): its values stay in registers from its sum to its K
+        const bool own = t < nsub;
+        const int lo = t * kAqSub, nv = own ? min(kAqSub, nb - lo) : 0;
+        float v[kAqSub];
+#pragma unroll
+        for (int i = 0; i < kAqSub; i++) v[i] = i < nv ? xs[aq_pos(lo + i)] : 0.0f;
+        {   // sub-chunk sums (f64: only an estimate, any order will do)
             double cs = 0.0;
-            for (int i = lo; i < hi; i++) cs += (double)xs[aq_pos(i)];
-            pre[c] = cs;
+#pragma unroll
+            for (int i = 0; i < kAqSub; i++) cs += (double)v[i];
+            if (own) pre[t] = cs;
         }
         __syncthreads();
-        if (t < 64) {   // exclusive prefix of the sub-chunk sums, 64 at a time
-            double carry = 0.0;
-            for (int g = 0; g < nsub; g += 64) {
-                const double v = g + lane < nsub ? pre[g + lane] : 0.0;
-                double inc = v;
+        {   // exclusive prefix of the sub-chunk sums (an estimate as well): each wave scans 64, the
+            // waves' totals are added after (cpre holds them until it is written)
+            const int w = t >> 6;
+            const int c = t;
+            const double x = c < nsub ? pre[c] : 0.0;
+            double inc = x;
 #pragma unroll
-                for (int d = 1; d < 64; d <<= 1) {
-                    const double o = __shfl_up(inc, d);
-                    if (lane >= d) inc += o;
-                }
-                if (g + lane < nsub) pre[g + lane] = carry + inc - v;
-                carry += __shfl(inc, 63);
+            for (int d = 1; d < 64; d <<= 1) {
+                const double o = __shfl_up(inc, d);
+                if (lane >= d) inc += o;
             }
+            if (lane == 63) cpre[w] = inc;
+            __syncthreads();
+            double carry = 0.0;
+            for (int g = 0; g < w; g++) carry += cpre[g];
+            __syncthreads();
+            if (c < nsub) pre[c] = carry + inc - x;
         }
         __syncthreads();
         for (int k = t; k < nch; k += nt) cpre[k] = pre[k * kAqGroup];
-        __syncthreads();
-        for (int c = t; c < nsub; c += nt) {   // sub-chunk K: the run in f32 from 2^e (+u)
-            const int lo = c * kAqSub, hi = min(lo + kAqSub, nb);
-            const int2 eb = aq_binades(pre[c]);   // read before sbin[c] overwrites it
+        if (own) {   // sub-chunk K: the run in f32 from 2^e (+u)
+            const int2 eb = aq_binades(pre[t]);
             const uint32_t b0 = (uint32_t)eb.x << 23, b2 = (uint32_t)eb.y << 23;
             float s0 = __uint_as_float(b0), s1 = __uint_as_float(b0 | 1u), s2 = __uint_as_float(b2), s3 = __uint_as_float(b2 | 1u);
             bool pos = true;
-            for (int i = lo; i < hi; i++) {
-                const float x = xs[aq_pos(i)];
-                pos &= x > 0.0f;
-                s0 = __fadd_rn(s0, x); s1 = __fadd_rn(s1, x); s2 = __fadd_rn(s2, x); s3 = __fadd_rn(s3, x);
+#pragma unroll
+            for (int i = 0; i < kAqSub; i++) {   // values past nb are +0, which leaves a positive sum as it is
+                pos &= i >= nv || v[i] > 0.0f;
+                s0 = __fadd_rn(s0, v[i]); s1 = __fadd_rn(s1, v[i]); s2 = __fadd_rn(s2, v[i]); s3 = __fadd_rn(s3, v[i]);
             }
             const bool ve = pos && eb.x >= 1 && eb.x <= 253, va = pos && eb.y >= 1 && eb.y <= 253;
             int4 k;
@@ -1142,8 +1156,8 @@ __global__ __launch_bounds__(1024) void k_aq(float* __restrict__ lnv, int nb, fl
             k.y = ve && (fbits(s1) >> 23) == (uint32_t)eb.x ? (int)(fbits(s1) - (b0 | 1u)) : kAqNoK;
             k.z = va && (fbits(s2) >> 23) == (uint32_t)eb.y ? (int)(fbits(s2) - b2) : kAqNoK;
             k.w = va && (fbits(s3) >> 23) == (uint32_t)eb.y ? (int)(fbits(s3) - (b2 | 1u)) : kAqNoK;
-            sk[c] = k;
-            sbin[c] = eb;
+            sk[t] = k;
+            sbin[t] = eb;
         }
         __syncthreads();
         for (int q = t; q < 4 * nch; q += nt) {   // chunk K, one (chunk, binade, parity) per thread
@@ -1151,27 +1165,42 @@ __global__ __launch_bounds__(1024) void k_aq(float* __restrict__ lnv, int nb, fl
             const int2 eb = aq_binades(cpre[k]);
             const int e = which < 2 ? eb.x : eb.y;
             const uint32_t b0 = ((uint32_t)e << 23) | (uint32_t)(which & 1);
+            const int c0 = k * kAqGroup, nsc = min(kAqGroup, nsub - c0);
+            int4 kv[kAqGroup];
+            int2 ev[kAqGroup];
+#pragma unroll
+            for (int j = 0; j < kAqGroup; j++)
+                if (j < nsc) { kv[j] = sk[c0 + j]; ev[j] = sbin[c0 + j]; }
             uint32_t b = b0;
             bool ok = e >= 1 && e <= 253;
-            for (int c = k * kAqGroup; ok && c < min((k + 1) * kAqGroup, nsub); c++) {
-                b += (uint32_t)aq_pick(b, sk[c], sbin[c]);
-                ok = (b >> 23) == (uint32_t)e;
-            }
+#pragma unroll
+            for (int j = 0; j < kAqGroup; j++)
+                if (ok && j < nsc) {
+                    b += (uint32_t)aq_pick(b, kv[j], ev[j]);
+                    ok = (b >> 23) == (uint32_t)e;
+                }
             ((int*)&ck[k])[which] = ok ? (int)(b - b0) : kAqNoK;
             if (which == 0) cbin[k] = eb;
         }
         __syncthreads();
         if (t < 64) {
             // one wave, every lane holding the same sum: lanes fetch together what the chain reads
-            // in turn (a chunk's 8 sub-chunk K, a sub-chunk's 32 values, one LDS instruction each)
-            // and the chain takes them by lane reads; the next chunk's K is read ahead
+            // in turn (the chunks' K, a chunk's 8 sub-chunk K, a sub-chunk's 32 values, one LDS
+            // instruction each) and the chain takes them by lane reads
             uint32_t b = 0u;   // the sum's bits
-            int4 nk = ck[0];
-            int2 ne = cbin[0];
+            // every chunk's K and binades in lanes (chunk k in lane k & 63 of set k >> 6), taken by
+            // lane reads: the chain waits for no memory
+            const int4 z4 = make_int4(0, 0, 0, 0);
+            const int2 z2 = make_int2(-9, -9);
+            const int4 ka = lane < nch ? ck[lane] : z4, kb = lane + 64 < nch ? ck[lane + 64] : z4;
+            const int2 ea = lane < nch ? cbin[lane] : z2, eb2 = lane + 64 < nch ? cbin[lane + 64] : z2;
             for (int k = 0; k < nch; k++) {
-                const int4 kk = nk;
-                const int2 ee = ne;
-                if (k + 1 < nch) { nk = ck[k + 1]; ne = cbin[k + 1]; }
+                const int kl = k & 63;
+                const int4 ks = k < 64 ? ka : kb;
+                const int2 es = k < 64 ? ea : eb2;
+                const int4 kk = make_int4(__builtin_amdgcn_readlane(ks.x, kl), __builtin_amdgcn_readlane(ks.y, kl),
+                                          __builtin_amdgcn_readlane(ks.z, kl), __builtin_amdgcn_readlane(ks.w, kl));
+                const int2 ee = make_int2(__builtin_amdgcn_readlane(es.x, kl), __builtin_amdgcn_readlane(es.y, kl));
                 const uint32_t nx = b + (uint32_t)aq_pick(b, kk, ee);
                 if ((nx >> 23) == (b >> 23)) { b = nx; continue; }
                 const int c0 = k * kAqGroup, nsc = min(kAqGroup, nsub - c0);
@@ -1199,14 +1228,12 @@ __global__ __launch_bounds__(1024) void k_aq(float* __restrict__ lnv, int nb, fl
         }
         __syncthreads();
     }
-    // per-MB steps: base * exp2(0.8 * (ln v - mean) / 6) in f32 (residual.rs:150-166), the logs
-    // from LDS where they are staged
+    // per-MB steps (when asked for: the encoder's residual kernel forms each MB's step itself), the
+    // logs from LDS where they are staged
+    if (!steps) return;
     const float mean = *s_mean;
     const bool staged = nb <= kAqStage;
-    for (int b = t; b < nb; b += nt) {
-        const float off = __fmul_rn(0.8f, __fsub_rn(staged ? xs[aq_pos(b)] : lnv[b], mean));
-        steps[b] = __fmul_rn(base, glibc_exp2f(__fdiv_rn(off, 6.0f)));
-    }
+    for (int b = t; b < nb; b += nt) steps[b] = aq_step(staged ? xs[aq_pos(b)] : lnv[b], mean, base);
 }
 
 // The adaptive quantiser's per-MB input straight from a luma plane (the plane taken as its own
@@ -1436,12 +1463,17 @@ __device__ __forceinline__ void write_packed_split(const int16_t* coef, uint8_t*
 __global__ __launch_bounds__(256) void k_inter_residual(const uint8_t* cY, const uint8_t* cU, const uint8_t* cV,
                                                         const uint8_t* rY, const uint8_t* rU, const uint8_t* rV,
                                                         uint8_t* pY, uint8_t* pU, uint8_t* pV,
-                                                        int W, int H, const BlockRec* rec, const float* steps, float dz,
+                                                        int W, int H, const BlockRec* rec, const float* lnv, float base,
+                                                        float* steps, float dz,
                                                         uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* main, uint8_t* over,
                                                         uint8_t* mask, uint8_t* host_opt_skip, uint8_t* dpack, uint8_t* dcoded) {
     __shared__ ResidLds L;
     const int b = blockIdx.x, bw = W >> 4, bx = b % bw, by = b / bw, t = threadIdx.x, cw = W >> 1, ch = H >> 1;
     const int x0 = bx * 16, y0 = by * 16;
+    // the MB's adaptive quantiser step is formed where it is used, once, from its log variance and
+    // the frame's mean (lnv[nb]); kept in steps (skipped MBs too) for the host's inspection
+    __shared__ float s_qs;
+    if (t == 0) { const float q = aq_step(lnv[b], lnv[gridDim.x], base); s_qs = q; steps[b] = q; }
     // the prediction (motion compensation with the search MV, which the mode decision keeps;
     // fast_motion.rs:103-352): thread t forms luma sample t and, below 128, chroma sample t & 63 of
     // plane t >> 6, the samples it later uses; they also go to pY / pU / pV
@@ -1477,7 +1509,7 @@ __global__ __launch_bounds__(256) void k_inter_residual(const uint8_t* cY, const
     if (t < 128) L.res[256 + t] = (int16_t)((int)(pl ? cV : cU)[oc] - predC);
     __syncthreads();
     fdct_mb(L, t);
-    const float qs = steps[b];
+    const float qs = s_qs;   // written before the barriers above
     int nz = 0;
     for (int i = t; i < 384; i += 256) { int16_t q = quant_aq(L.coef32[i], qs, dz); L.coef[i] = q; nz |= q != 0; }
     nz = __syncthreads_or(nz);
@@ -1780,11 +1812,11 @@ void logf_exp2f(hipStream_t st, const float* in, size_t n, int which, float* out
     hipLaunchKernelGGL(k_logf_exp2f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, n, which, out);
 }
 void inter_residual(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, const uint8_t* rY, const uint8_t* rU,
-                    const uint8_t* rV, uint8_t* pY, uint8_t* pU, uint8_t* pV, int W, int H, const BlockRec* rec, const float* steps, float dz,
-                    uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* main, uint8_t* over, uint8_t* mask, uint8_t* host_opt_skip,
-                    uint8_t* dpack, uint8_t* dcoded) {
-    hipLaunchKernelGGL(k_inter_residual, dim3((W / 16) * (H / 16)), dim3(256), 0, st, cY, cU, cV, rY, rU, rV, pY, pU, pV, W, H, rec, steps, dz,
-                       oY, oU, oV, main, over, mask, host_opt_skip, dpack, dcoded);
+                    const uint8_t* rV, uint8_t* pY, uint8_t* pU, uint8_t* pV, int W, int H, const BlockRec* rec, const float* lnv, float base,
+                    float* steps, float dz, uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* main, uint8_t* over, uint8_t* mask,
+                    uint8_t* host_opt_skip, uint8_t* dpack, uint8_t* dcoded) {
+    hipLaunchKernelGGL(k_inter_residual, dim3((W / 16) * (H / 16)), dim3(256), 0, st, cY, cU, cV, rY, rU, rV, pY, pU, pV, W, H, rec, lnv, base,
+                       steps, dz, oY, oU, oV, main, over, mask, host_opt_skip, dpack, dcoded);
 }
 void intra(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, int W, int H, const uint16_t* yt, const uint16_t* ct,
            uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* pack) {

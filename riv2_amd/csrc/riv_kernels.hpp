@@ -30,15 +30,18 @@ std::vector<SearchShape> search_shapes(int sr, int nb, int nstreams, int cus, si
 hipError_t me_search(hipStream_t st, const riv::SearchParams& p, int groups, int waves, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void mc(hipStream_t st, const uint8_t* rY, const uint8_t* rU, const uint8_t* rV, int W, int H, const int8_t* mvs, int mv_stride,
         uint8_t* pY, uint8_t* pU, uint8_t* pV, uint32_t* var_sums, float* lnv);
-// lnv: nb logs written by mc, plus one float for their mean; one launch computes the mean and the steps
+// lnv: nb logs written by mc, plus one float for their mean; one launch computes the mean and, when
+// steps is not null, the steps
 void aq(hipStream_t st, int nb, float base, float* lnv, float* steps);
 // lnv: (W/16)*(H/16) logs of the luma plane Y's block variances (Y 4-byte aligned), aq's input
 void aq_logs(hipStream_t st, const uint8_t* Y, int W, int H, float* lnv);
 void logf_exp2f(hipStream_t st, const float* in, size_t n, int which, float* out);
+// Each MB's quantiser step from lnv (its log variance; the frame's mean at lnv[nb], as aq leaves
+// it) and base, also written to steps.
 void inter_residual(hipStream_t st, const uint8_t* cY, const uint8_t* cU, const uint8_t* cV, const uint8_t* rY, const uint8_t* rU,
-                    const uint8_t* rV, uint8_t* pY, uint8_t* pU, uint8_t* pV, int W, int H, const riv::BlockRec* rec, const float* steps, float dz,
-                    uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* main, uint8_t* over, uint8_t* mask, uint8_t* host_opt_skip,
-                    uint8_t* dpack = nullptr, uint8_t* dcoded = nullptr);
+                    const uint8_t* rV, uint8_t* pY, uint8_t* pU, uint8_t* pV, int W, int H, const riv::BlockRec* rec, const float* lnv,
+                    float base, float* steps, float dz, uint8_t* oY, uint8_t* oU, uint8_t* oV, uint8_t* main, uint8_t* over, uint8_t* mask,
+                    uint8_t* host_opt_skip, uint8_t* dpack = nullptr, uint8_t* dcoded = nullptr);
 // r*: the reference planes; p*: out, the motion-compensated prediction (with the search MV)
 // main/over: coded MBs' packed coefficients, MB b's masks (and its values if there are at most
 // PACK_MAIN_VALUES) at main + b * PACK_MAIN, else its values at over + b * PACK_STRIDE + 48
