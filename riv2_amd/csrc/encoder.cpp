@@ -61,6 +61,9 @@ struct Slot {
     riv_frame_stats st{};
     std::vector<uint8_t> res;
     std::vector<rivh::MvBlock> blocks;
+    std::vector<uint16_t> mv_evs;                    // the frame's MV decisions as (context, bit) events
+    size_t mv_nev = 0;
+    bool events_ready = false;                       // its MV events are formed (the contexts may run)
     int bdx = 0, bdy = 0;
     double smark[RIV_NUM_STAGES] = {};
     uint32_t nmark[RIV_NUM_STAGES] = {};
@@ -310,10 +313,12 @@ struct riv_encoder {
     std::condition_variable cv;
     // ordered stages of the host tail: MV modelling (its contexts carry from frame to frame), then,
     // after the packing that any thread may do, the write
-    uint64_t next_tail = 0;        // next frame for MV modelling
+    uint64_t next_tail = 0;        // next frame for MV events (and the candidate history)
+    uint64_t next_ctx = 0;         // next frame for the MV contexts
     uint64_t next_write = 0;       // next frame to write
     uint64_t completed = 0;        // frames fully written
-    bool tail_active = false;      // a thread is modelling ready frames in order
+    bool tail_active = false;      // a thread is forming ready frames' MV events in order
+    bool ctx_active = false;       // a thread is running ready frames' MV contexts in order
     bool write_active = false;     // a thread is writing packed frames in order
     int err_code = 0;
     std::string err_msg;
@@ -759,7 +764,7 @@ static int model_frame(riv_encoder* e, Slot& s) {
     const int nb = e->nb;
     try {
         if (s.intra) {
-            e->mvc.reset();
+            e->mvc.reset_history();
             st.frame_type = 1;
         } else {
             const BlockRec* hr = s.h_rec.as<BlockRec>();
@@ -789,9 +794,25 @@ static int model_frame(riv_encoder* e, Slot& s) {
             st.rdo_lambda = e->lambda; st.rdo_blocks_evaluated = (uint32_t)nb;
             if (e->cfg.skip_threshold == 0) st.rdo_forced_disabled = (uint32_t)nb;
             st.me_passes = (uint32_t)*s.h_passes.as<int>();
+            s.mv_nev = e->mvc.events(s.blocks, e->bw, s.mv_evs);
+        }
+    } catch (const std::exception& ex) {
+        e->fail(RIV_E_ENCODING_FAILED, std::string("MV coding failed: ") + ex.what());
+        return RIV_E_ENCODING_FAILED;
+    }
+    return RIV_OK;
+}
+
+// The MV contexts (frame order; the contexts carry from frame to frame) over the frame's events,
+// its symbols into the frame's MV writer. Runs beside the next frame's model_frame.
+static int model_contexts(riv_encoder* e, Slot& s) {
+    try {
+        if (s.intra) {
+            e->mvc.reset_contexts();
+        } else {
             s.mvb.clear();
             s.mvw.reset(new rivh::RansWriter(s.mvb));
-            e->mvc.model(s.blocks, e->bw, e->bh, *s.mvw);
+            e->mvc.contexts(s.mv_evs.data(), s.mv_nev, *s.mvw);
         }
     } catch (const std::exception& ex) {
         e->fail(RIV_E_ENCODING_FAILED, std::string("MV coding failed: ") + ex.what());
@@ -854,8 +875,35 @@ static void frame_part_done(riv_encoder* e, int si, bool mv_part) {
     else pack_then_write(e, si);
 }
 
-// Marks slot si's GPU work done, then, unless another thread is already doing so, models the MV
-// part of every such frame in frame order. The modelling runs outside the lock.
+// The MV part of a frame runs in two ordered stages on the host's workers: its events (and the
+// candidate history for the next frame), then its contexts, so that frame k's contexts run beside
+// frame k + 1's events. Each stage, unless another thread is already doing so, takes every ready
+// frame in frame order; it works outside the lock.
+static void run_ctx(riv_encoder* e, int si) {
+    std::unique_lock<std::mutex> lk(e->mu);
+    e->slots[si].events_ready = true;
+    if (e->ctx_active) return;
+    e->ctx_active = true;
+    for (;;) {
+        const int ti = (int)(e->next_ctx % e->slots.size());
+        Slot& t = e->slots[ti];
+        if (!t.busy || !t.events_ready || t.index != e->next_ctx) break;
+        t.events_ready = false;
+        const bool skip = t.mv_failed || t.failed || e->err_code;
+        lk.unlock();
+        const auto t0 = std::chrono::steady_clock::now();
+        if (skip || model_contexts(e, t) != RIV_OK) t.mv_failed = true;
+        t.st.host_tail_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        lk.lock();
+        e->next_ctx++;
+        lk.unlock();
+        frame_part_done(e, ti, true);
+        lk.lock();
+    }
+    e->ctx_active = false;
+}
+
+// Marks slot si's GPU work done, then forms the MV events of every ready frame in order.
 static void run_mv(riv_encoder* e, int si) {
     std::unique_lock<std::mutex> lk(e->mu);
     e->slots[si].gpu_ready = true;
@@ -874,7 +922,8 @@ static void run_mv(riv_encoder* e, int si) {
         lk.lock();
         e->next_tail++;
         lk.unlock();
-        frame_part_done(e, ti, true);
+        if (e->hs) e->hs->pool.submit([e, ti] { run_ctx(e, ti); });
+        else run_ctx(e, ti);
         lk.lock();
     }
     e->tail_active = false;

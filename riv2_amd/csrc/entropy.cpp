@@ -820,6 +820,11 @@ inline int8_t subpel_of_rank(int r) { return r == 0 ? 0 : r == 1 ? 1 : -1; }
 }  // namespace
 
 void MvCoder::reset() { *this = MvCoder(); }
+void MvCoder::reset_history() {
+    cand_.clear(); live_.clear(); index_.clear(); total_ = 0;
+    for (uint32_t& k : cand_key_) k = ~0u;
+}
+void MvCoder::reset_contexts() { for (Ctx& c : ctx_) c = Ctx(); }
 
 namespace {
 inline uint32_t mv_key(int dx, int dy, int rx, int ry) { return ((uint32_t)(dx + 128) << 24) | ((uint32_t)(dy + 128) << 16) | ((uint32_t)rx << 8) | (uint32_t)ry; }
@@ -903,10 +908,15 @@ std::vector<uint8_t> MvCoder::encode(const std::vector<MvBlock>& blocks, int bw,
 
 void MvCoder::model(const std::vector<MvBlock>& blocks, int bw, int bh, RansWriter& w) {
     (void)bh;
-    const int nb = (int)blocks.size();
-    // 1. the frame's decisions in coding order, as (context, bit) events; every tree writes its
-    //    full depth and advances by the decisions it takes, so the loop has few branches
     thread_local std::vector<uint16_t> evs;
+    const size_t n = events(blocks, bw, evs);
+    contexts(evs.data(), n, w);
+}
+
+size_t MvCoder::events(const std::vector<MvBlock>& blocks, int bw, std::vector<uint16_t>& evs) {
+    const int nb = (int)blocks.size();
+    // the frame's decisions in coding order, as (context, bit) events; every tree writes its full
+    // depth and advances by the decisions it takes, so the loop has few branches
     thread_local std::vector<uint8_t> hitv;
     if (evs.size() < (size_t)nb * kMvMaxEvents + 16) evs.resize((size_t)nb * kMvMaxEvents + 16);
     hitv.assign((size_t)nb, 0);
@@ -955,10 +965,13 @@ void MvCoder::model(const std::vector<MvBlock>& blocks, int bw, int bh, RansWrit
         }
         hitv[i] = hit;
     }
-    // 2. the adaptive contexts over the events, in order, each giving its rANS symbol
-    const size_t n = (size_t)(e - evs.data());
+    refresh_history(blocks, false);
+    return (size_t)(e - evs.data());
+}
+
+void MvCoder::contexts(const uint16_t* ev, size_t n, RansWriter& w) {
+    // the adaptive contexts over the events, in order, each giving its rANS symbol
     uint16_t* o = w.reserve(n);
-    const uint16_t* ev = evs.data();
     uint16_t* const cs = &ctx_[0].c;
     static_assert(sizeof(Ctx) == sizeof(uint16_t), "Ctx is its two counters");
     for (size_t j = 0; j < n; j++) {
@@ -973,7 +986,6 @@ void MvCoder::model(const std::vector<MvBlock>& blocks, int bw, int bh, RansWrit
         o[j] = (uint16_t)(pz ^ ((pz ^ ((pz << 8) | (256u - pz))) & (0u - bit)));
     }
     w.commit(o + n);
-    refresh_history(blocks, false);
 }
 
 std::vector<MvBlock> MvCoder::decode(const uint8_t* data, size_t len, int bw, int bh) {

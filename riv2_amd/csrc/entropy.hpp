@@ -162,8 +162,18 @@ public:
     void reset();
     std::vector<uint8_t> encode(const std::vector<MvBlock>& blocks, int bw, int bh);
     // The frame's symbols into w (not flushed) and the history update: the part of encode that
-    // depends on the previous frames; w.flush() then yields the bytes encode returns.
+    // depends on the previous frames; w.flush() then yields the bytes encode returns. It is
+    // events() then contexts(), which touch disjoint state: frame k's contexts may run beside
+    // frame k + 1's events (each in frame order).
     void model(const std::vector<MvBlock>& blocks, int bw, int bh, RansWriter& w);
+    // The frame's decisions in coding order as (context, bit) events into evs (grown as needed;
+    // returns how many), the candidate list of the vectors so far deciding the hit events, then
+    // the history update for the next frame.
+    size_t events(const std::vector<MvBlock>& blocks, int bw, std::vector<uint16_t>& evs);
+    // The adaptive contexts over a frame's n events, in order, each giving its symbol into w.
+    void contexts(const uint16_t* ev, size_t n, RansWriter& w);
+    void reset_history();    // reset() is both of these
+    void reset_contexts();
     std::vector<MvBlock> decode(const uint8_t* data, size_t len, int bw, int bh);
 private:
     void refresh_history(const std::vector<MvBlock>& blocks, bool from_decoder);
