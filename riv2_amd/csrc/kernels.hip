@@ -295,10 +295,10 @@ __device__ __forceinline__ int mv_class(int mag) {   // mv_class_from_magnitude 
 // the matched candidate, which equals the search MV; for NEW it is base + clamp(search - base),
 // and with every MV inside [-sr-1, sr] (sr <= 63) the difference never reaches the clamp. So
 // the whole frame's decisions follow from the search field in parallel. The identity is still
-// checked for every NEW block: a violation sets *err = 2 (and the stream's *serr, when given).
+// checked for every NEW block: a violation sets the stream's error word *serr to 2.
 __device__ __forceinline__ uint4 mode_record(uint32_t wS, int best_sad, bool has_l, bool has_t, bool has_tr, bool has_tl, bool has_tp,
                                              uint32_t wL, uint32_t wT, uint32_t wTR, uint32_t wTL, uint32_t wP,
-                                             const uint32_t* rdo_bits, int rdo_max, int* err, int* serr) {
+                                             const uint32_t* rdo_bits, int rdo_max, int* serr) {
     // deduplicated (dx,dy,sx,sy) of left, top, top-right, top-left, temporal; the first two are
     // Nearest / Near, padded with zero. Every MV word is canonical (sx, sy in {0,1}), so equal
     // half-pel positions are equal words.
@@ -340,10 +340,8 @@ __device__ __forceinline__ uint4 mode_record(uint32_t wS, int best_sad, bool has
         }
         rdx = dclamp(S0 - bbx, -128, 127);
         rdy = dclamp(S1 - bby, -128, 127);
-        if (dclamp(bbx + rdx, -128, 127) != S0 || dclamp(bby + rdy, -128, 127) != S1) {
-            atomicExch(err, 2);
-            if (serr) *serr = 2;   // the stream's own error word (mapped host memory: a plain store)
-        }
+        if (dclamp(bbx + rdx, -128, 127) != S0 || dclamp(bby + rdy, -128, 127) != S1)
+            *serr = 2;   // the stream's error word (mapped host memory: a plain store)
     }
     // RdoContext::decide via the host-built table (rdo.rs:140-178)
     uint32_t rdo = 0;
@@ -899,10 +897,8 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
     }
     if (outcome != 1) {
         const int code = outcome < 0 ? 1 : 3;
-        if (threadIdx.x == 0) __hip_atomic_store(p.err, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         // every stream of the launch failed: its error word tells the host not to code the frame
-        for (int i = threadIdx.x; i < NS; i += blockDim.x)
-            if (sd[i].passes) sd[i].passes[1] = code;
+        for (int i = threadIdx.x; i < NS; i += blockDim.x) sd[i].passes[1] = code;
         return;
     }
     // the mode decision, on the final field (the last pass's bank holds every block's word: the
@@ -916,7 +912,7 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
         const uint32_t wTR = has_tr ? field_load(fin + myb - mbw + 1) : 0u;
         const uint32_t wTL = has_tl ? field_load(fin + myb - mbw - 1) : 0u;
         const uint4 r = mode_record(myw, mysad, has_l, has_t, has_tr, has_tl, S.prev_mv != nullptr, wL, wT, wTR, wTL, mypT,
-                                    S.rdo_bits, S.rdo_max, p.err, S.passes ? S.passes + 1 : nullptr);
+                                    S.rdo_bits, S.rdo_max, S.passes + 1);
         *(uint4*)&S.rec[mloc] = r;
         *(uint4*)&S.host_rec[mloc] = r;
         // the next launch's start field: bank 0, where it does not hold myw already
@@ -949,7 +945,7 @@ __global__ __launch_bounds__(1024) void k_me_search(SearchParams p) {
     }
     if (blockIdx.x == 0)
         for (int i = threadIdx.x; i < NS; i += blockDim.x)
-            if (sd[i].passes) *sd[i].passes = pass + 1;
+            *sd[i].passes = pass + 1;
 }
 
 

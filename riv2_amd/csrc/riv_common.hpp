@@ -50,7 +50,7 @@ struct SearchStream {
     const uint32_t* rdo_bits;    // bit s set -> RDO chooses skip at SAD s (integer-aligned MV)
     int* passes;                 // out: [0] passes the launch ran, [1] left as the host cleared it (0) unless the
                                  // stream's search failed: 1 barrier timeout, 3 pass limit, 2 mode-decision
-                                 // identity violated (may be nullptr)
+                                 // identity violated (the host reads it after the frame)
     int W, H, bw, bh;
     int blk0;                    // number of the stream's first block in the launch
     int prev_stride;             // bytes between consecutive MVs of prev_mv
@@ -103,8 +103,6 @@ struct SearchParams {
     int c_zero, c_pred;          // (lam2*0.5) as i64, (lam2*1.5) as i64; c_pred is also the half-pel penalty
     int lam_zero;                // lambda == 0.0 -> every MV cost is 0
     long long zthr, pthr;        // me_zero_mv_threshold, me_predictor_threshold
-    int* err;                    // 1 on a barrier timeout, 3 if the pass limit is hit, 2 if a
-                                 // reconstructed MV would differ from its search MV
     // learned form (single-stream launches): per block the network's proposals (candidates 0|1,
     // 2|3 as (x, y) bytes, confident flag, logit bits), or null for the plain search; wave w of
     // the launch accumulates (blocks, confident, proposal taken) at lstat[3w .. 3w+2]
