@@ -57,7 +57,7 @@ struct Slot {
     bool failed = false;     // the GPU part failed (set before the host parts start)
     bool res_failed = false, mv_failed = false;   // each written only by its own part
     std::vector<uint8_t> mvb, bytes;                 // MV payload, whole frame record
-    std::unique_ptr<rivh::RansWriter> mvw;           // the MV symbols between modelling and packing
+    std::unique_ptr<rivh::RansWriter> mvw;           // the MV symbols between their contexts and their rANS coding
     riv_frame_stats st{};
     std::vector<uint8_t> res;
     std::vector<rivh::MvBlock> blocks;
@@ -282,6 +282,7 @@ struct riv_encoder {
     HostStage* hs = nullptr;       // the process's host stage, while pipelined
     HostStage* par = nullptr;      // the same, held by every encoder: its idle workers help code a frame
     hipEvent_t sync_ev = nullptr;  // orders this encoder's stream against a batch step's stream
+    hipEvent_t fin_ev[4] = {};     // flush: the end of each of the encoder's streams (blocking-sync events)
     // frames handed over in host memory (riv_encode_frame): each is copied on a stream of its own
     // into one of two device buffers (the first two thirds of `rgb`), so that the copy of frame k,
     // which holds the submitting thread when the source is pageable, overlaps the GPU work of
@@ -600,6 +601,7 @@ static void destroy_encoder(riv_encoder* e) {
         for (auto& m : s.marks) if (m) (void)hipEventDestroy(m);
     }
     if (e->sync_ev) (void)hipEventDestroy(e->sync_ev);
+    for (hipEvent_t& f : e->fin_ev) if (f) (void)hipEventDestroy(f);
     if (e->ms) {
         (void)hipStreamSynchronize(e->ms);
         for (int k = 0; k < 2; k++) { (void)hipEventDestroy(e->rm_ready[k]); (void)hipEventDestroy(e->rm_free[k]); }
@@ -684,7 +686,7 @@ static void collect_gpu_times(Slot& s) {
 //  * the residual coding (any order, any thread; its contexts are fresh every frame);
 //  * the MV modelling (frame order, one thread at a time: the MV contexts and candidate history
 //    carry from frame to frame), which needs only the block records, so it starts as soon as the
-//    GPU is done instead of after the residuals;
+//    GPU is done instead of after the residuals, then the MV rANS coding (any order);
 // whichever of the two finishes second hands the frame to the packing (any thread), and the
 // writes run in frame order.
 static void frame_part_done(riv_encoder* e, int si, bool mv_part);
@@ -720,10 +722,14 @@ static void code_residuals(riv_encoder* e, int si) {
             const uint32_t n = s.h_rmh.as<uint32_t>()[0];
             fetch_symbols(e, s, n);
             st.residual_symbols_on_gpu = 1;
-            const bool par = e->par && e->par->pool.idle() >= 2;
-            s.res = rivh::code_symbol_stream(s.h_sym.as<uint16_t>(), n, par ? rivh::ParallelFor([e](int k, const std::function<void(int)>& f) {
+            // groups of eight blocks when the idle workers (and this thread) can take them all at
+            // once: twice the tasks of the sixteen-block groups, each done in about half the time
+            const int idle = e->par ? e->par->pool.idle() : 0;
+            const int blocks = (int)((n + 16385) / 16386);
+            const int gb = idle + 1 >= (blocks + 7) / 8 ? 8 : 16;
+            s.res = rivh::code_symbol_stream(s.h_sym.as<uint16_t>(), n, idle >= 2 ? rivh::ParallelFor([e](int k, const std::function<void(int)>& f) {
                 e->par->pool.parallel(k, f);
-            }) : rivh::ParallelFor());
+            }) : rivh::ParallelFor(), gb);
         } else if (s.gpu_model) {
             // the GPU could not model this frame (too many symbols, or a magnitude beyond the class
             // table): it sent the full records instead, and the host codes them (or rejects them)
@@ -754,16 +760,14 @@ static void code_residuals(riv_encoder* e, int si) {
     frame_part_done(e, si, false);
 }
 
-// The MV part of frame s: the block list from the records (NEW deltas after the frame bias,
-// encoder.rs:775-818, bias by compute_delta_bias_for_new, :984-1048), then the MV symbols and
-// history update (mv_rans/encoder.rs; contexts and history restart at every I-frame,
-// encoder.rs:335-337).
-static int model_frame(riv_encoder* e, Slot& s) {
+// The block list of P-frame s from its records (NEW deltas after the frame bias,
+// encoder.rs:775-818, bias by compute_delta_bias_for_new, :984-1048) and its stats: any order,
+// any thread (nothing carries from frame to frame).
+static int prepare_blocks(riv_encoder* e, Slot& s) {
     riv_frame_stats& st = s.st;
     const int nb = e->nb;
     try {
         if (s.intra) {
-            e->mvc.reset_history();
             st.frame_type = 1;
         } else {
             const BlockRec* hr = s.h_rec.as<BlockRec>();
@@ -793,8 +797,20 @@ static int model_frame(riv_encoder* e, Slot& s) {
             st.rdo_lambda = e->lambda; st.rdo_blocks_evaluated = (uint32_t)nb;
             if (e->cfg.skip_threshold == 0) st.rdo_forced_disabled = (uint32_t)nb;
             st.me_passes = (uint32_t)*s.h_passes.as<int>();
-            s.mv_nev = e->mvc.events(s.blocks, e->bw, s.mv_evs);
         }
+    } catch (const std::exception& ex) {
+        e->fail(RIV_E_ENCODING_FAILED, std::string("MV coding failed: ") + ex.what());
+        return RIV_E_ENCODING_FAILED;
+    }
+    return RIV_OK;
+}
+
+// The MV events of frame s and the history update (mv_rans/encoder.rs; contexts and history
+// restart at every I-frame, encoder.rs:335-337): frame order.
+static int model_frame(riv_encoder* e, Slot& s) {
+    try {
+        if (s.intra) e->mvc.reset_history();
+        else s.mv_nev = e->mvc.events(s.blocks, e->bw, s.mv_evs);
     } catch (const std::exception& ex) {
         e->fail(RIV_E_ENCODING_FAILED, std::string("MV coding failed: ") + ex.what());
         return RIV_E_ENCODING_FAILED;
@@ -820,7 +836,7 @@ static int model_contexts(riv_encoder* e, Slot& s) {
     return RIV_OK;
 }
 
-// Part 3 (any order, any thread): the MV rANS bytes and the frame record (format.rs:171-205).
+// Part 3 (any order, any thread): the frame record (format.rs:171-205).
 static void pack_frame(riv_encoder* e, Slot& s) {
     const riv_encoder_config& c = e->cfg;
     std::vector<uint8_t>& bytes = s.bytes;
@@ -832,8 +848,6 @@ static void pack_frame(riv_encoder* e, Slot& s) {
         bytes.insert(bytes.end(), s.res.begin(), s.res.end());
         s.st.residual_bytes = (uint32_t)s.res.size();
     } else {
-        s.mvw->flush();
-        s.mvw.reset();
         bytes.reserve(21 + s.mvb.size() + s.res.size());
         put_u64(bytes, s.ts); bytes.push_back(2); bytes.push_back(c.inter_quality);
         bytes.push_back((uint8_t)(int8_t)s.bdx); bytes.push_back((uint8_t)(int8_t)s.bdy); bytes.push_back(0);
@@ -874,6 +888,30 @@ static void frame_part_done(riv_encoder* e, int si, bool mv_part) {
     else pack_then_write(e, si);
 }
 
+// The frame's MV rANS coding (any order, any thread), as soon as its contexts are done, so that it
+// runs beside the frame's residual coding rather than after it, its blocks across idle workers
+// when there are any; then the MV part is done.
+static void drain_mv(riv_encoder* e, int si) {
+    Slot& s = e->slots[si];
+    const auto t0 = std::chrono::steady_clock::now();
+    if (s.mvw) {
+        try {
+            if (!s.mv_failed) {
+                if (e->par && e->par->pool.idle() >= 2)
+                    s.mvw->flush_parallel([e](int k, const std::function<void(int)>& f) { e->par->pool.parallel(k, f); });
+                else
+                    s.mvw->flush();
+            }
+        } catch (const std::exception& ex) {
+            s.mv_failed = true;
+            e->fail(RIV_E_ENCODING_FAILED, std::string("MV coding failed: ") + ex.what());
+        }
+        s.mvw.reset();
+    }
+    s.st.host_tail_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    frame_part_done(e, si, true);
+}
+
 // The MV part of a frame runs in two ordered stages on the host's workers: its events (and the
 // candidate history for the next frame), then its contexts, so that frame k's contexts run beside
 // frame k + 1's events. Each stage, unless another thread is already doing so, takes every ready
@@ -896,14 +934,22 @@ static void run_ctx(riv_encoder* e, int si) {
         lk.lock();
         e->next_ctx++;
         lk.unlock();
-        frame_part_done(e, ti, true);
+        if (e->hs) e->hs->pool.submit([e, ti] { drain_mv(e, ti); });
+        else drain_mv(e, ti);
         lk.lock();
     }
     e->ctx_active = false;
 }
 
-// Marks slot si's GPU work done, then forms the MV events of every ready frame in order.
+// Marks slot si's GPU work done, then forms the MV events of every ready frame in order; the
+// frame's block list first, outside the ordered stage.
 static void run_mv(riv_encoder* e, int si) {
+    {
+        Slot& s = e->slots[si];
+        const auto t0 = std::chrono::steady_clock::now();
+        if (!s.failed && prepare_blocks(e, s) != RIV_OK) s.mv_failed = true;
+        s.st.host_tail_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
     std::unique_lock<std::mutex> lk(e->mu);
     e->slots[si].gpu_ready = true;
     if (e->tail_active) return;
@@ -913,7 +959,7 @@ static void run_mv(riv_encoder* e, int si) {
         Slot& t = e->slots[ti];
         if (!t.busy || !t.gpu_ready || t.index != e->next_tail) break;
         t.gpu_ready = false;
-        const bool skip = t.failed || e->err_code;
+        const bool skip = t.failed || t.mv_failed || e->err_code;
         lk.unlock();
         const auto t0 = std::chrono::steady_clock::now();
         if (skip || model_frame(e, t) != RIV_OK) t.mv_failed = true;
@@ -1538,10 +1584,21 @@ int riv_encoder_flush(riv_encoder* e) {
     // The encoder's streams first: this thread waits for the GPU while the host stage codes the
     // frames already done, so no work of this encoder is left for a later device-wide wait (the
     // first one after a run of frames costs ~0.15 ms, which this wait takes beside the coding).
-    // Each frame's search failures arrive through its own error word, read by the host stage.
+    // The waits sleep (blocking-sync events), leaving the cores to the host stage; then each
+    // stream is synchronised, now quickly, which settles the runtime's own records of its work
+    // (waiting on events alone leaves that to the next device-wide wait). Each frame's search
+    // failures arrive through its own error word, read by the host stage.
     RIV_HIP(hipSetDevice(e->cfg.device_id));
-    for (hipStream_t s : {e->st, e->ms, e->up, e->dl})
-        if (s) RIV_HIP(hipStreamSynchronize(s));
+    const hipStream_t ss[4] = {e->st, e->ms, e->up, e->dl};
+    for (int k = 0; k < 4; k++) {
+        if (!ss[k]) continue;
+        if (!e->fin_ev[k]) RIV_HIP(hipEventCreateWithFlags(&e->fin_ev[k], hipEventDisableTiming | hipEventBlockingSync));
+        RIV_HIP(hipEventRecord(e->fin_ev[k], ss[k]));
+    }
+    for (int k = 0; k < 4; k++)
+        if (ss[k]) RIV_HIP(hipEventSynchronize(e->fin_ev[k]));
+    for (int k = 0; k < 4; k++)
+        if (ss[k]) RIV_HIP(hipStreamSynchronize(ss[k]));
     std::unique_lock<std::mutex> lk(e->mu);
     e->cv.wait(lk, [&] { return e->completed == e->frame_count || e->err_code; });
     if (e->err_code) { set_error(e->err_msg); return e->err_code; }

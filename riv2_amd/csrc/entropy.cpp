@@ -244,6 +244,19 @@ void RansWriter::flush() {
     emitted_ = false;
 }
 
+void RansWriter::flush_parallel(const std::function<void(int, const std::function<void(int)>&)>& pfor) {
+    const size_t nblk = (n_ + kBatch - 1) / kBatch;
+    if (!pfor || nblk < 2 || nblk >= 8 || emitted_) { flush(); return; }
+    std::vector<std::vector<uint8_t>> part(nblk);
+    pfor((int)nblk, [&](int j) {
+        RansWriter w(part[j]);
+        const size_t b = (size_t)j * kBatch;
+        w.code_symbols(sym_ + b, std::min((size_t)kBatch, n_ - b));
+    });
+    for (const auto& p : part) out_.insert(out_.end(), p.begin(), p.end());
+    n_ = 0;
+}
+
 void RansWriter::code_symbols(const uint16_t* sym, size_t n) {
     if (n_) throw std::logic_error("code_symbols on a writer with buffered symbols");
     // the blocks of flush(), read from the caller's array (flush only reads sym_)
@@ -718,10 +731,11 @@ std::vector<uint8_t> encode_residuals_split_par(const uint8_t* main, const uint8
     return out;
 }
 
-std::vector<uint8_t> code_symbol_stream(const uint16_t* sym, size_t n, const ParallelFor& pfor) {
+std::vector<uint8_t> code_symbol_stream(const uint16_t* sym, size_t n, const ParallelFor& pfor, int group_blocks) {
     std::vector<uint8_t> out;
     if (n == 0) return out;
-    constexpr size_t kGroup = 16 * 16386;
+    if (group_blocks != 16 && group_blocks != 8) throw std::invalid_argument("rANS block groups are of 16 or 8 blocks");
+    const size_t kGroup = (size_t)group_blocks * 16386;
     const int ng = (int)((n + kGroup - 1) / kGroup);
     if (ng == 1 || !pfor) {
         RansWriter w(out);

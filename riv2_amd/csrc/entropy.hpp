@@ -68,6 +68,10 @@ public:
     }
     void commit(const uint16_t* end) { n_ = (size_t)(end - sym_); }
     void flush();
+    // flush() with the blocks of a stream of two to seven blocks coded by parallel tasks, one block
+    // each, through pfor(n, fn) (fn(0) .. fn(n - 1)); the bytes are flush()'s, as every block is
+    // coded from fresh states and they are emitted in order. Other streams, or no pfor: flush().
+    void flush_parallel(const std::function<void(int, const std::function<void(int)>&)>& pfor);
     // Codes n symbols held by the caller (start << 8 | freq each) as a whole stream: the blocks
     // flush() would code, appended to the output; nothing may be buffered.
     void code_symbols(const uint16_t* sym, size_t n);
@@ -140,9 +144,10 @@ std::vector<uint8_t> encode_residuals_split_par(const uint8_t* main, const uint8
                                                 const ParallelFor& pfor);
 // rANS32 coding of a stream's whole symbol sequence (n symbols, each start << 8 | freq, as the
 // residual coder buffers them): its 16386-symbol blocks, each coded from fresh states, in groups
-// of sixteen, by parallel tasks when pfor is set. An empty sequence gives an empty stream (no
-// coder is instantiated when every block is skipped, residual.rs:670-674).
-std::vector<uint8_t> code_symbol_stream(const uint16_t* sym, size_t n, const ParallelFor& pfor);
+// of group_blocks (16 or 8), by parallel tasks when pfor is set; the bytes do not depend on the
+// grouping. An empty sequence gives an empty stream (no coder is instantiated when every block
+// is skipped, residual.rs:670-674).
+std::vector<uint8_t> code_symbol_stream(const uint16_t* sym, size_t n, const ParallelFor& pfor, int group_blocks = 16);
 // One MB of the split form as 384 coefficients.
 void unpack_split(const uint8_t* main, const uint8_t* over, int b, int16_t* q);
 void unpack_coeffs(const uint8_t* slot, int16_t* q384);
