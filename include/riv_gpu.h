@@ -138,7 +138,8 @@ int riv_encode_frame_device(riv_encoder* enc, const void* d_rgb, size_t len, uin
  * once the work is queued. */
 int riv_encode_batch_step(riv_encoder* const* encs, size_t n, const void* const* rgb, const size_t* len, const uint64_t* timestamp_ms,
                           int on_device, riv_frame_stats* stats);
-/* Blocks until every submitted frame has been written (pipelined mode; no-op otherwise). */
+/* Blocks until every submitted frame has been written (pipelined mode; otherwise frames are written
+ * by their encode call) and the encoder's own HIP streams are idle; other streams are not waited for. */
 int riv_encoder_flush(riv_encoder* enc);
 /* Completed frames' stats in encode order (pipelined mode); *n receives the number copied. */
 int riv_encoder_pop_stats(riv_encoder* enc, riv_frame_stats* out, size_t cap, size_t* n);
