@@ -460,7 +460,9 @@ static int encoder_init(riv_encoder* e) {
         // far more than it issues, and the quarter of each SIMD's registers (and the LDS) left free
         // lets the residual-modelling kernels of the previous frame run on the same CUs beside it;
         // with 16 waves they wait for the search to end (1080p: the search itself 183 -> 187 us,
-        // the pipelined encode 2,740 -> 2,920 frames/s)
+        // the pipelined encode 2,740 -> 2,920 frames/s); fewer waves give each wave more blocks
+        // to search in the full passes (8 and 10 waves: ~3,240 and ~3,270 against ~3,360 frames/s
+        // over 20 pipelined frames)
         if (!rivk::search_shape(c.search_range, e->nb, 1, prop.multiProcessorCount, prop.sharedMemPerBlock, &e->shape, 12) &&
             !rivk::search_shape(c.search_range, e->nb, 1, prop.multiProcessorCount, prop.sharedMemPerBlock, &e->shape)) {
             set_error("frame has more macroblocks than the motion search's resident waves can hold");
